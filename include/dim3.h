@@ -1,0 +1,164 @@
+/*
+ * libdim3 — MI355X (gfx950) DIM3 join-project and join-aggregate (COUNT).
+ *
+ * C-ABI drop-in for the reference's query entry points.  Plain pointers and
+ * sizes; relations are arrays of interleaved int32 pairs laid out exactly
+ * like the reference's myvector<pair<int,int>> payload (Include/myvector.hpp:9-15,
+ * main.cpp:6-11): R = (x, y) pairs, S = (z, y) pairs, join key in the second
+ * field.  All counts are 64-bit (the reference's uint32 counters overflow
+ * above ~2.1e9 rows, Include/myvector.hpp:9-15).
+ *
+ * Semantics kept from the reference:
+ *   - join-project returns the set of distinct (x, z) with (x,y) in R and
+ *     (z,y) in S, unordered (dim3.hpp:952-973);
+ *   - when |R| < |S| the roles are swapped and rows are (S.first, R.first)
+ *     (dim3.hpp:993-994, 1325-1326); dim3_result.swapped reports it and
+ *     DIM3_OPT_NO_SWAP turns it off;
+ *   - join-aggregate COUNT counts join paths with multiplicity, so duplicate
+ *     tuples count once per copy (dim3.hpp:1293-1300);
+ *   - empty inputs or no matching key give an empty result (dim3.hpp:1027).
+ * Errors are status codes, never exit(): the reference printf+exit()s on
+ * allocation failure (e.g. dim3.hpp:303-307).
+ *
+ * Thread safety: a context serialises its calls on its own HIP stream; use
+ * one context per host thread, or the server API, which is thread safe.
+ */
+#ifndef DIM3_H
+#define DIM3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define DIM3_OK 0
+#define DIM3_ERR_INVALID 1   /* bad argument */
+#define DIM3_ERR_OOM 2       /* device or host allocation failed, or over the context budget */
+#define DIM3_ERR_DEVICE 3    /* HIP runtime error */
+#define DIM3_ERR_CAPACITY 4  /* caller output buffer too small; dim3_result.n holds the size needed */
+#define DIM3_ERR_ADMISSION 5 /* server: request larger than the HBM budget can ever admit */
+#define DIM3_ERR_BUSY 6      /* server: queue full */
+#define DIM3_ERR_NOT_FOUND 7 /* server: unknown ticket or relation handle */
+#define DIM3_ERR_UNSUPPORTED 8
+
+/* ---- output modes ---- */
+#define DIM3_OUT_MATERIALIZE 0 /* rows are written out */
+#define DIM3_OUT_FINGERPRINT 1 /* only count + order-independent fingerprint */
+
+/* ---- flags ---- */
+#define DIM3_OPT_NO_SWAP 1u       /* never swap R and S */
+#define DIM3_OPT_EXCLUDE_EQUAL 2u /* drop rows with x == z (self-join u != w filter) */
+
+/* ---- dense (heavy-key) strategy ---- */
+#define DIM3_DENSE_AUTO (-1)
+#define DIM3_DENSE_OFF 0    /* every key through the sparse CSR expansion */
+#define DIM3_DENSE_BITOR 1  /* heavy keys as z-bitmaps OR-ed into each row (project only) */
+#define DIM3_DENSE_MFMA 2   /* heavy keys as an int8 0/1 matrix product on the MFMA cores */
+
+typedef struct dim3_opts {
+    uint32_t output_mode; /* DIM3_OUT_* */
+    uint32_t flags;       /* DIM3_OPT_* */
+    int32_t dense_mode;   /* DIM3_DENSE_* */
+    int32_t heavy_k;      /* number of heavy join keys for the dense part; 0 = planner */
+    int32_t tile_bits;    /* z-tile width of the per-row dedup bitmap; 0 = planner */
+    int32_t block;        /* expansion workgroup size (256/512/1024); 0 = planner */
+    uint32_t reserved[4];
+} dim3_opts;
+
+/* Order-independent digest of a row set.  key = (u64)(u32)x << 32 | (u32)z,
+ * h = splitmix64-finaliser(key); for COUNT rows h = finaliser(h + count).
+ * count = rows, sum = sum of h mod 2^64, xr = xor of h, csum = sum of counts. */
+typedef struct dim3_fingerprint {
+    uint64_t count;
+    uint64_t sum;
+    uint64_t xr;
+    uint64_t csum;
+} dim3_fingerprint;
+
+typedef struct dim3_stats {
+    uint64_t n_r, n_s;        /* input sizes after the swap rule */
+    uint64_t nx, ny, nz;      /* distinct x, joined y, z */
+    uint64_t out_j;           /* join paths sum_b degR(b)*degS(b) */
+    uint64_t paths_sparse;    /* join paths handled by the sparse expansion */
+    uint64_t heavy_keys;      /* keys routed to the dense part */
+    uint64_t nx_heavy, nz_heavy;
+    uint32_t dense_mode;      /* DIM3_DENSE_* actually used */
+    uint32_t tile_bits;       /* z-tile width actually used */
+    uint32_t n_tiles;
+    uint32_t block;
+    double ms_map, ms_csr, ms_dense, ms_expand, ms_total; /* device time per phase */
+} dim3_stats;
+
+/* Row record of join-aggregate COUNT; same layout as the reference's
+ * TUPLE<int,int,long long> {x, y, v} (dim3.hpp:20-33). */
+typedef struct dim3_count_row {
+    int32_t x;
+    int32_t z;
+    int64_t count;
+} dim3_count_row;
+
+typedef struct dim3_result {
+    uint64_t n;                 /* rows */
+    int32_t* pairs;             /* project: 2*n int32 (x,z) pairs, malloc'd by the library */
+    dim3_count_row* rows;       /* aggregate: n rows, malloc'd by the library */
+    dim3_fingerprint fp;
+    uint32_t swapped;           /* 1: rows are (S.first, R.first) */
+    uint32_t reserved;
+    dim3_stats stats;
+} dim3_result;
+
+typedef struct dim3_ctx dim3_ctx;
+
+/* Context on HIP device `device`.  hbm_budget_bytes caps the device memory
+ * the context may hold (0 = 90% of the device's free memory). */
+int dim3_ctx_create(int device, uint64_t hbm_budget_bytes, dim3_ctx** out);
+int dim3_ctx_destroy(dim3_ctx* ctx);
+/* Text of the last error on this context ("" if none). */
+const char* dim3_last_error(const dim3_ctx* ctx);
+const char* dim3_status_string(int status);
+/* Bytes of device memory the context currently holds. */
+uint64_t dim3_ctx_device_bytes(const dim3_ctx* ctx);
+
+void dim3_opts_default(dim3_opts* o);
+
+/* Replaces DIM3<int,int,int,...>::doJoinProject (dim3.hpp:982-1055) and
+ * DIM3_parallel<...>::doJoinProject_parallel (dim3_parallel.hpp:137-209).
+ * Host inputs, host output (res->pairs, free with dim3_result_free). */
+int dim3_join_project(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
+                      const dim3_opts* opts, dim3_result* res);
+
+/* Replaces DIM3JoinAggregate<int,int,int,_Tv,_Agg,...>::doJoinAggregate
+ * (dim3.hpp:1313-1373) for the COUNT aggregate.  Host in, host out
+ * (res->rows). */
+int dim3_join_aggregate_count(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
+                              const dim3_opts* opts, dim3_result* res);
+
+void dim3_result_free(dim3_result* res);
+
+/* Device-resident variants: d_R / d_S are device pointers, the rows go to a
+ * caller device buffer of cap rows (d_pairs: 2*cap int32; d_rows: cap
+ * records).  If the rows do not fit, DIM3_ERR_CAPACITY is returned with
+ * res->n set to the rows needed and the buffer holding the first cap rows.
+ * stream: hipStream_t to order the work on, or NULL for the context's
+ * stream; the call returns after the work on it has finished. */
+int dim3_join_project_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_r, const int32_t* d_S, uint64_t n_s,
+                          const dim3_opts* opts, int32_t* d_pairs, uint64_t cap, void* stream, dim3_result* res);
+int dim3_join_aggregate_count_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_r, const int32_t* d_S,
+                                  uint64_t n_s, const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap,
+                                  void* stream, dim3_result* res);
+
+/* Exact OUT_J = sum_b degR(b)*degS(b) on the device (the quantity
+ * EstimateJoinCardinality samples, dim3.hpp:190-270). */
+int dim3_count_join_paths(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
+                          uint64_t* out_j);
+
+/* Library version string. */
+const char* dim3_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIM3_H */

@@ -1,0 +1,289 @@
+"""Python binding of libdim3 (MI355X DIM3 join-project / join-aggregate COUNT).
+
+Host-side plumbing over the C-ABI declared in include/dim3.h, used by the
+tests, the benchmark and the server front end.  It mirrors the reference's
+query entry points (dim3.hpp:982-1055 DIM3::doJoinProject, dim3.hpp:1313-1373
+DIM3JoinAggregate::doJoinAggregate): relations are (n, 2) int32 arrays of
+(x|z, y) pairs with the join key in column 1, results are (x, z) pairs or
+(x, z, count) rows, and the |R| < |S| swap rule is kept (result.swapped).
+
+The product path is the HIP library; there is no CPU fallback.  Loading
+fails loudly when libdim3.so is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdim3.so")
+
+OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_ADMISSION, ERR_BUSY, ERR_NOT_FOUND, ERR_UNSUPPORTED = range(9)
+OUT_MATERIALIZE, OUT_FINGERPRINT = 0, 1
+OPT_NO_SWAP, OPT_EXCLUDE_EQUAL = 1, 2
+DENSE_AUTO, DENSE_OFF, DENSE_BITOR, DENSE_MFMA = -1, 0, 1, 2
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("output_mode", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("dense_mode", ctypes.c_int32),
+                ("heavy_k", ctypes.c_int32), ("tile_bits", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+class Fingerprint(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("sum", ctypes.c_uint64), ("xr", ctypes.c_uint64),
+                ("csum", ctypes.c_uint64)]
+
+    def as_tuple(self):
+        return (int(self.count), int(self.sum), int(self.xr), int(self.csum))
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_r", ctypes.c_uint64), ("n_s", ctypes.c_uint64), ("nx", ctypes.c_uint64),
+                ("ny", ctypes.c_uint64), ("nz", ctypes.c_uint64), ("out_j", ctypes.c_uint64),
+                ("paths_sparse", ctypes.c_uint64), ("heavy_keys", ctypes.c_uint64),
+                ("nx_heavy", ctypes.c_uint64), ("nz_heavy", ctypes.c_uint64), ("dense_mode", ctypes.c_uint32),
+                ("tile_bits", ctypes.c_uint32), ("n_tiles", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("ms_map", ctypes.c_double), ("ms_csr", ctypes.c_double), ("ms_dense", ctypes.c_double),
+                ("ms_expand", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CountRow(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("z", ctypes.c_int32), ("count", ctypes.c_int64)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("pairs", ctypes.c_void_p), ("rows", ctypes.c_void_p),
+                ("fp", Fingerprint), ("swapped", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("stats", Stats)]
+
+
+COUNT_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("count", "<i8")])
+
+# every symbol include/dim3.h declares (checked by the ABI test)
+EXPORTS = [
+    "dim3_ctx_create", "dim3_ctx_destroy", "dim3_last_error", "dim3_status_string", "dim3_ctx_device_bytes",
+    "dim3_opts_default", "dim3_join_project", "dim3_join_aggregate_count", "dim3_result_free",
+    "dim3_join_project_dev", "dim3_join_aggregate_count_dev", "dim3_count_join_paths", "dim3_version",
+]
+
+_lib = None
+
+
+class Dim3Error(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("libdim3 status %d (%s): %s" % (status, status_string(status), msg))
+        self.status = status
+
+
+def lib():
+    """Load libdim3.so; raises when the HIP library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libdim3.so is not built (%s); run __graft_entry__.build() or "
+                          "make -C joinproject-dim3_amd" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+    L.dim3_ctx_create.argtypes = [i32, u64, ctypes.POINTER(vp)]
+    L.dim3_ctx_create.restype = i32
+    L.dim3_ctx_destroy.argtypes = [vp]
+    L.dim3_ctx_destroy.restype = i32
+    L.dim3_last_error.argtypes = [vp]
+    L.dim3_last_error.restype = ctypes.c_char_p
+    L.dim3_status_string.argtypes = [i32]
+    L.dim3_status_string.restype = ctypes.c_char_p
+    L.dim3_ctx_device_bytes.argtypes = [vp]
+    L.dim3_ctx_device_bytes.restype = u64
+    L.dim3_opts_default.argtypes = [ctypes.POINTER(Opts)]
+    L.dim3_opts_default.restype = None
+    for name in ("dim3_join_project", "dim3_join_aggregate_count"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
+        f.restype = i32
+    for name in ("dim3_join_project_dev", "dim3_join_aggregate_count_dev"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(Opts), vp, u64, vp, ctypes.POINTER(Result)]
+        f.restype = i32
+    L.dim3_result_free.argtypes = [ctypes.POINTER(Result)]
+    L.dim3_result_free.restype = None
+    L.dim3_count_join_paths.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
+    L.dim3_count_join_paths.restype = i32
+    L.dim3_version.argtypes = []
+    L.dim3_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def status_string(s):
+    try:
+        return lib().dim3_status_string(s).decode()
+    except ImportError:
+        return str(s)
+
+
+def make_opts(output_mode=OUT_MATERIALIZE, no_swap=False, exclude_equal=False, dense_mode=DENSE_AUTO, heavy_k=0,
+              tile_bits=0, block=0):
+    o = Opts()
+    lib().dim3_opts_default(ctypes.byref(o))
+    o.output_mode = output_mode
+    o.flags = (OPT_NO_SWAP if no_swap else 0) | (OPT_EXCLUDE_EQUAL if exclude_equal else 0)
+    o.dense_mode = dense_mode
+    o.heavy_k = heavy_k
+    o.tile_bits = tile_bits
+    o.block = block
+    return o
+
+
+def _pairs(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a.reshape(-1, 2)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class QueryResult:
+    """Host-side copy of a dim3_result."""
+
+    def __init__(self, res, pairs=None, rows=None):
+        self.n = int(res.n)
+        self.pairs = pairs
+        self.rows = rows
+        self.fp = Fingerprint.from_buffer_copy(res.fp)
+        self.swapped = bool(res.swapped)
+        self.stats = Stats.from_buffer_copy(res.stats)
+
+    @property
+    def counts(self):
+        return None if self.rows is None else self.rows["count"]
+
+    @property
+    def xz(self):
+        if self.pairs is not None:
+            return self.pairs
+        if self.rows is not None:
+            return np.stack([self.rows["x"], self.rows["z"]], axis=1)
+        return None
+
+
+class Context:
+    """One libdim3 context (one device, one HIP stream, one workspace)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.dim3_ctx_create(device, hbm_budget_bytes, ctypes.byref(h))
+        if rc != OK:
+            raise Dim3Error(rc, "dim3_ctx_create(device=%d)" % device)
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self._L.dim3_ctx_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def last_error(self):
+        return self._L.dim3_last_error(self._h).decode()
+
+    def device_bytes(self):
+        return int(self._L.dim3_ctx_device_bytes(self._h))
+
+    def _check(self, rc, what):
+        if rc != OK:
+            raise Dim3Error(rc, "%s: %s" % (what, self.last_error()))
+
+    def _host_call(self, fn, R, S, opts):
+        R, S = _pairs(R), _pairs(S)
+        res = Result()
+        rc = fn(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0], ctypes.byref(opts), ctypes.byref(res))
+        return rc, res
+
+    def join_project(self, R, S, **kw):
+        """DIM3::doJoinProject: distinct (x, z) pairs as an (n, 2) int32 array."""
+        opts = make_opts(**kw)
+        rc, res = self._host_call(self._L.dim3_join_project, R, S, opts)
+        self._check(rc, "dim3_join_project")
+        pairs = None
+        if opts.output_mode == OUT_MATERIALIZE:
+            n = int(res.n)
+            if n:
+                buf = (ctypes.c_int32 * (2 * n)).from_address(res.pairs)
+                pairs = np.frombuffer(buf, dtype=np.int32).reshape(n, 2).copy()
+            else:
+                pairs = np.zeros((0, 2), dtype=np.int32)
+        out = QueryResult(res, pairs=pairs)
+        self._L.dim3_result_free(ctypes.byref(res))
+        return out
+
+    def join_aggregate_count(self, R, S, **kw):
+        """DIM3JoinAggregate with COUNT: rows with fields x, z, count."""
+        opts = make_opts(**kw)
+        rc, res = self._host_call(self._L.dim3_join_aggregate_count, R, S, opts)
+        self._check(rc, "dim3_join_aggregate_count")
+        rows = None
+        if opts.output_mode == OUT_MATERIALIZE:
+            n = int(res.n)
+            if n:
+                buf = (ctypes.c_char * (16 * n)).from_address(res.rows)
+                rows = np.frombuffer(buf, dtype=COUNT_ROW_DTYPE).copy()
+            else:
+                rows = np.zeros((0,), dtype=COUNT_ROW_DTYPE)
+        out = QueryResult(res, rows=rows)
+        self._L.dim3_result_free(ctypes.byref(res))
+        return out
+
+    def join_project_dev(self, d_R, n_r, d_S, n_s, d_out, cap, stream=None, **kw):
+        """Device-resident variant: d_R/d_S/d_out are device addresses (ints).
+        Returns (status, QueryResult); status ERR_CAPACITY reports res.n."""
+        opts = make_opts(**kw)
+        res = Result()
+        rc = self._L.dim3_join_project_dev(self._h, ctypes.c_void_p(d_R), n_r, ctypes.c_void_p(d_S), n_s,
+                                           ctypes.byref(opts), ctypes.c_void_p(d_out), cap,
+                                           ctypes.c_void_p(stream or 0), ctypes.byref(res))
+        if rc not in (OK, ERR_CAPACITY):
+            self._check(rc, "dim3_join_project_dev")
+        return rc, QueryResult(res)
+
+    def join_aggregate_count_dev(self, d_R, n_r, d_S, n_s, d_out, cap, stream=None, **kw):
+        opts = make_opts(**kw)
+        res = Result()
+        rc = self._L.dim3_join_aggregate_count_dev(self._h, ctypes.c_void_p(d_R), n_r, ctypes.c_void_p(d_S), n_s,
+                                                   ctypes.byref(opts), ctypes.c_void_p(d_out), cap,
+                                                   ctypes.c_void_p(stream or 0), ctypes.byref(res))
+        if rc not in (OK, ERR_CAPACITY):
+            self._check(rc, "dim3_join_aggregate_count_dev")
+        return rc, QueryResult(res)
+
+    def count_join_paths(self, R, S):
+        R, S = _pairs(R), _pairs(S)
+        out = ctypes.c_uint64()
+        rc = self._L.dim3_count_join_paths(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0], ctypes.byref(out))
+        self._check(rc, "dim3_count_join_paths")
+        return int(out.value)
+
+
+def version():
+    return lib().dim3_version().decode()

@@ -1,0 +1,668 @@
+// libdim3 C-ABI: context, workspace, planner and the query pipeline.
+//
+// Query pipeline (one HIP stream per context, one host sync after mapping
+// to size the CSR arrays and pick the tile):
+//   swap rule                         dim3.hpp:993-994 / 1325-1326
+//   y / x / z mapping + semi-join     dim3.hpp:399-455      (k_mapping.hip)
+//   CSR of R by x, of S by (y,tile)   dim3.hpp:1058-1253    (k_mapping.hip)
+//   sparse expansion + dedup/count    dim3.hpp:952-973, 1267-1311 (k_expand.hip)
+// Device buffers live in per-context grow-only workspace slots, so repeated
+// queries of similar size allocate nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dim3_internal.h"
+
+using dim3h::Scalars;
+
+namespace {
+
+enum SlotId {
+    WS_IN_R,
+    WS_IN_S,
+    WS_KEYS_Y,
+    WS_DEGR_Y,
+    WS_DEGS_Y,
+    WS_EX_Y,
+    WS_SLOT_R,
+    WS_SLOT_S,
+    WS_KEYS_X,
+    WS_DEG_X,
+    WS_EX_X,
+    WS_SLOT_RX,
+    WS_KEYS_Z,
+    WS_DEG_Z,
+    WS_EX_Z,
+    WS_SLOT_SZ,
+    WS_PARTIALS,
+    WS_SCALARS,
+    WS_ID2X,
+    WS_ID2Z,
+    WS_X2Z,
+    WS_ROWPTR_R,
+    WS_CUR_R,
+    WS_COL_R,
+    WS_CNT_S,
+    WS_ROWPTR_S,
+    WS_CUR_S,
+    WS_KEY_S,
+    WS_ZID_S,
+    WS_COL_S,
+    WS_OUT,
+    WS_COUNT
+};
+
+struct Slot {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+inline uint32_t pow2_at_least(uint64_t v) {
+    uint64_t c = 64;
+    while (c < v) c <<= 1;
+    return (uint32_t)c;
+}
+
+inline uint32_t ceil_log2(uint64_t v) {
+    uint32_t s = 0;
+    while ((1ull << s) < v) s++;
+    return s;
+}
+
+}  // namespace
+
+struct dim3_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t held = 0;
+    Slot ws[WS_COUNT];
+    std::string err;
+    int cu_count = 0;
+    size_t max_lds = 0;
+    Scalars* h_sc = nullptr;  // pinned mirror of the device scalars
+    hipEvent_t ev[6] = {};
+    std::recursive_mutex mu;
+
+    // plan of the last prepared query
+    struct Plan {
+        const int2* R = nullptr;
+        const int2* S = nullptr;
+        uint64_t nR = 0, nS = 0;
+        bool swapped = false;
+        bool empty = true;
+        bool count_mode = false;
+        uint32_t flags = 0;
+        uint32_t capY = 0, capX = 0, capZ = 0;
+        uint32_t nx = 0, ny = 0, nz = 0;
+        uint32_t nnzR = 0, nnzS = 0;
+        uint64_t out_j = 0;
+        uint32_t ntiles = 1, tile_shift = 0, tile_words = 0;
+        uint32_t touched_cap = 0, stage_cap = 0;
+        int block = 256;
+        float ms_map = 0, ms_csr = 0;
+    } plan;
+};
+
+namespace {
+
+int fail(dim3_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail((c), DIM3_ERR_DEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK(c, what)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = hipGetLastError();                                                       \
+        if (e_ != hipSuccess) return fail((c), DIM3_ERR_DEVICE, "%s launch: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+// Grow-only device workspace slot.
+int ws_get(dim3_ctx* c, SlotId id, size_t bytes, void** out) {
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~(size_t)255;
+    Slot& s = c->ws[id];
+    if (s.bytes >= bytes) {
+        *out = s.p;
+        return DIM3_OK;
+    }
+    const uint64_t need = c->held - s.bytes + bytes;
+    if (c->budget && need > c->budget)
+        return fail(c, DIM3_ERR_OOM, "workspace slot %d needs %zu bytes: over the %llu-byte budget", (int)id, bytes,
+                    (unsigned long long)c->budget);
+    if (s.p) {
+        hipStreamSynchronize(c->stream);
+        hipFree(s.p);
+        c->held -= s.bytes;
+        s.p = nullptr;
+        s.bytes = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, DIM3_ERR_OOM, "hipMalloc(%zu) for workspace slot %d: %s", bytes, (int)id, hipGetErrorString(e));
+    }
+    s.p = p;
+    s.bytes = bytes;
+    c->held += bytes;
+    *out = p;
+    return DIM3_OK;
+}
+
+template <typename T>
+int ws(dim3_ctx* c, SlotId id, uint64_t n, T** out) {
+    void* p = nullptr;
+    int rc = ws_get(c, id, (size_t)n * sizeof(T), &p);
+    *out = (T*)p;
+    return rc;
+}
+
+#define TRY(x)                    \
+    do {                          \
+        int rc_ = (x);            \
+        if (rc_ != DIM3_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// planner: z-tile width and workgroup size of the expansion
+// ---------------------------------------------------------------------------
+void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
+    auto& P = c->plan;
+    const bool cnt = P.count_mode;
+    P.touched_cap = cnt ? 4096u : 2048u;
+    P.stage_cap = cnt ? 512u : 1024u;
+    const size_t extra = (size_t)P.touched_cap * 4u + (size_t)P.stage_cap * (cnt ? 16u : 8u) + 64u;
+    const size_t lds_cap = c->max_lds;
+    // entries of the tile that fit in LDS (bits for project, u32 counters for COUNT)
+    const uint64_t fit_units = (lds_cap - extra) / 4u * (cnt ? 1u : 32u);
+    uint32_t shift;
+    if (o.tile_bits > 0) {
+        shift = ceil_log2((uint64_t)o.tile_bits);
+    } else if ((uint64_t)P.nz <= fit_units) {
+        shift = ceil_log2(std::max<uint64_t>(P.nz, 32));
+    } else {
+        shift = 0;
+        while ((2ull << shift) <= fit_units) shift++;
+    }
+    if (!cnt && shift < 5) shift = 5;
+    if (shift > 30) shift = 30;
+    const uint64_t tile = 1ull << shift;
+    P.tile_shift = shift;
+    P.ntiles = (uint32_t)((P.nz + tile - 1) / tile);
+    if (P.ntiles == 0) P.ntiles = 1;
+    const uint64_t units = P.ntiles == 1 ? std::max<uint64_t>(P.nz, 1) : tile;
+    P.tile_words = (uint32_t)(cnt ? units : (units + 31) / 32);
+    const size_t lds = cnt ? dim3h::expand_count_lds(P.tile_words, P.touched_cap, P.stage_cap)
+                           : dim3h::expand_project_lds(P.tile_words, P.touched_cap, P.stage_cap);
+    if (o.block == 256 || o.block == 512 || o.block == 1024) {
+        P.block = o.block;
+    } else if (lds > 80 * 1024) {
+        P.block = 1024;
+    } else if (lds > 40 * 1024) {
+        P.block = 512;
+    } else {
+        P.block = 256;
+    }
+}
+
+// Mapping + CSR of one query.  Leaves the plan in c->plan.
+int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
+            const dim3_opts& o, hipStream_t st) {
+    auto& P = c->plan;
+    P = dim3_ctx::Plan();
+    P.count_mode = count_mode;
+    P.flags = o.flags;
+    if (!(o.flags & DIM3_OPT_NO_SWAP) && nR < nS) {
+        std::swap(dR, dS);
+        std::swap(nR, nS);
+        P.swapped = true;
+    }
+    P.R = dR;
+    P.S = dS;
+    P.nR = nR;
+    P.nS = nS;
+    if (nR >= (1ull << 31) || nS >= (1ull << 31))
+        return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^31-1 tuples (got %llu, %llu)",
+                    (unsigned long long)nR, (unsigned long long)nS);
+
+    Scalars* sc;
+    TRY(ws(c, WS_SCALARS, 1, &sc));
+    hipMemsetAsync(sc, 0, sizeof(Scalars), st);
+    if (nR == 0 || nS == 0) {  // dim3.hpp:1027
+        P.empty = true;
+        return DIM3_OK;
+    }
+
+    HIPCHK(c, hipEventRecord(c->ev[0], st));
+    P.capY = pow2_at_least(2 * nR);
+    P.capX = P.capY;
+    P.capZ = pow2_at_least(2 * nS);
+    const uint64_t cy1 = (uint64_t)P.capY + 1, cx1 = (uint64_t)P.capX + 1, cz1 = (uint64_t)P.capZ + 1;
+
+    uint32_t *keysY, *degR, *degS, *slotR, *keysX, *degX, *keysZ, *degZ;
+    int32_t *slotS, *slotRx, *slotSz;
+    uint64_t *exY, *exX, *exZ, *part;
+    TRY(ws(c, WS_KEYS_Y, cy1, &keysY));
+    TRY(ws(c, WS_DEGR_Y, cy1, &degR));
+    TRY(ws(c, WS_DEGS_Y, cy1, &degS));
+    TRY(ws(c, WS_EX_Y, cy1, &exY));
+    TRY(ws(c, WS_SLOT_R, nR, &slotR));
+    TRY(ws(c, WS_SLOT_S, nS, &slotS));
+    TRY(ws(c, WS_KEYS_X, cx1, &keysX));
+    TRY(ws(c, WS_DEG_X, cx1, &degX));
+    TRY(ws(c, WS_EX_X, cx1, &exX));
+    TRY(ws(c, WS_SLOT_RX, nR, &slotRx));
+    TRY(ws(c, WS_KEYS_Z, cz1, &keysZ));
+    TRY(ws(c, WS_DEG_Z, cz1, &degZ));
+    TRY(ws(c, WS_EX_Z, cz1, &exZ));
+    TRY(ws(c, WS_SLOT_SZ, nS, &slotSz));
+    const uint64_t maxscan = std::max<uint64_t>(std::max(cy1, cz1), std::max<uint64_t>(nS, nR) + 1);
+    TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed(maxscan), &part));
+
+    dim3h::ht_init(keysY, P.capY, st);
+    dim3h::ht_init(keysX, P.capX, st);
+    dim3h::ht_init(keysZ, P.capZ, st);
+    dim3h::fill_u32(degR, cy1, 0, st);
+    dim3h::fill_u32(degS, cy1, 0, st);
+    dim3h::fill_u32(degX, cx1, 0, st);
+    dim3h::fill_u32(degZ, cz1, 0, st);
+
+    const uint32_t mY = P.capY - 1, mX = P.capX - 1, mZ = P.capZ - 1;
+    dim3h::build_y(dR, nR, keysY, mY, degR, slotR, st);
+    dim3h::probe_y(dS, nS, keysY, mY, degS, slotS, st);
+    dim3h::scan_y(degR, degS, (uint32_t)cy1, exY, part, &sc->tot_y, &sc->out_j, st);
+    dim3h::build_x(dR, nR, slotR, degS, keysX, mX, degX, slotRx, st);
+    dim3h::build_z(dS, nS, slotS, keysZ, mZ, degZ, slotSz, st);
+    dim3h::scan_deg(degX, (uint32_t)cx1, exX, part, &sc->tot_x, st);
+    dim3h::scan_deg(degZ, (uint32_t)cz1, exZ, part, &sc->tot_z, st);
+    LAUNCHCHK(c, "mapping");
+    HIPCHK(c, hipEventRecord(c->ev[1], st));
+    HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+
+    P.ny = (uint32_t)(c->h_sc->tot_y >> 32);
+    P.nx = (uint32_t)(c->h_sc->tot_x >> 32);
+    P.nnzR = (uint32_t)c->h_sc->tot_x;
+    P.nz = (uint32_t)(c->h_sc->tot_z >> 32);
+    P.nnzS = (uint32_t)c->h_sc->tot_z;
+    P.out_j = c->h_sc->out_j;
+    {
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+        P.ms_map = ms;
+    }
+    if (P.ny == 0 || P.nx == 0 || P.nz == 0) {
+        P.empty = true;
+        return DIM3_OK;
+    }
+    P.empty = false;
+    plan_tiles(c, o);
+    if ((uint64_t)P.ny * P.ntiles >= (1ull << 32))
+        return fail(c, DIM3_ERR_INVALID, "tiled S index does not fit 32 bits (ny=%u, tiles=%u)", P.ny, P.ntiles);
+
+    HIPCHK(c, hipEventRecord(c->ev[2], st));
+    int32_t *id2x, *id2z, *colR, *zidS, *colS, *x2z;
+    uint32_t *rowR, *curR, *cntS, *rowS, *curS, *keyS;
+    const uint64_t nkeyS = (uint64_t)P.ny * P.ntiles;
+    TRY(ws(c, WS_ID2X, P.nx, &id2x));
+    TRY(ws(c, WS_ID2Z, (uint64_t)P.ntiles << P.tile_shift, &id2z));
+    TRY(ws(c, WS_ROWPTR_R, (uint64_t)P.nx + 1, &rowR));
+    TRY(ws(c, WS_CUR_R, P.nx, &curR));
+    TRY(ws(c, WS_COL_R, P.nnzR, &colR));
+    TRY(ws(c, WS_CNT_S, nkeyS + 1, &cntS));
+    TRY(ws(c, WS_ROWPTR_S, nkeyS + 1, &rowS));
+    TRY(ws(c, WS_CUR_S, nkeyS, &curS));
+    TRY(ws(c, WS_KEY_S, nS, &keyS));
+    TRY(ws(c, WS_ZID_S, nS, &zidS));
+    TRY(ws(c, WS_COL_S, P.nnzS, &colS));
+    TRY(ws(c, WS_X2Z, P.nx, &x2z));
+    TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed(std::max<uint64_t>(maxscan, nkeyS + 1)), &part));
+
+    dim3h::compact_ids(keysX, P.capX, degX, exX, id2x, rowR, st);
+    dim3h::finish_rowptr(rowR, P.nx, &sc->tot_x, st);
+    dim3h::compact_ids(keysZ, P.capZ, degZ, exZ, id2z, nullptr, st);
+    HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
+    dim3h::scatter_r(slotR, slotRx, nR, exY, exX, curR, colR, st);
+    dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
+    dim3h::count_s(slotS, slotSz, nS, exY, exZ, P.ntiles, P.tile_shift, cntS, keyS, zidS, st);
+    dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
+    dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
+    HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
+    dim3h::scatter_s(keyS, zidS, nS, curS, colS, st);
+    if (o.flags & DIM3_OPT_EXCLUDE_EQUAL) dim3h::x2z_self(id2x, P.nx, keysZ, mZ, degZ, exZ, x2z, st);
+    LAUNCHCHK(c, "csr");
+    HIPCHK(c, hipEventRecord(c->ev[3], st));
+    return DIM3_OK;
+}
+
+// Expansion of the prepared plan into `out` (cap rows) or the fingerprint.
+int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
+    auto& P = c->plan;
+    Scalars* sc = (Scalars*)c->ws[WS_SCALARS].p;
+    // reset the counters the expansion owns (a re-run after a capacity miss)
+    hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
+    hipMemsetAsync(&sc->task, 0, sizeof(uint32_t), st);
+    if (P.empty) {
+        HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        return DIM3_OK;
+    }
+    dim3h::ExpandArgs a;
+    a.rowptrR = (const uint32_t*)c->ws[WS_ROWPTR_R].p;
+    a.colR = (const int32_t*)c->ws[WS_COL_R].p;
+    a.rowptrS = (const uint32_t*)c->ws[WS_ROWPTR_S].p;
+    a.colS = (const int32_t*)c->ws[WS_COL_S].p;
+    a.id2x = (const int32_t*)c->ws[WS_ID2X].p;
+    a.id2z = (const int32_t*)c->ws[WS_ID2Z].p;
+    a.x2z = (P.flags & DIM3_OPT_EXCLUDE_EQUAL) ? (const int32_t*)c->ws[WS_X2Z].p : nullptr;
+    a.nx = P.nx;
+    a.ntiles = P.ntiles;
+    a.tile_shift = P.tile_shift;
+    a.tile_words = P.tile_words;
+    a.touched_cap = P.touched_cap;
+    a.stage_cap = P.stage_cap;
+    a.fp_mode = fp_mode ? 1u : 0u;
+    a.sc = sc;
+    a.out = out;
+    a.cap = fp_mode ? 0 : cap;
+    HIPCHK(c, hipEventRecord(c->ev[4], st));
+    int e = P.count_mode ? dim3h::expand_count(a, P.block, c->cu_count, st)
+                         : dim3h::expand_project(a, P.block, c->cu_count, st);
+    if (e != 0) return fail(c, DIM3_ERR_DEVICE, "expansion launch failed: %s", hipGetErrorString((hipError_t)e));
+    HIPCHK(c, hipEventRecord(c->ev[5], st));
+    HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    return DIM3_OK;
+}
+
+void fill_result(dim3_ctx* c, dim3_result* res, bool fp_mode) {
+    auto& P = c->plan;
+    const Scalars& s = *c->h_sc;
+    res->n = P.empty ? 0 : s.out_n;
+    res->swapped = P.swapped ? 1u : 0u;
+    res->fp.count = res->n;
+    res->fp.sum = P.empty ? 0 : s.fp_sum;
+    res->fp.xr = P.empty ? 0 : s.fp_xr;
+    res->fp.csum = P.empty ? 0 : s.fp_csum;
+    (void)fp_mode;
+    dim3_stats& st = res->stats;
+    memset(&st, 0, sizeof(st));
+    st.n_r = P.nR;
+    st.n_s = P.nS;
+    st.nx = P.nx;
+    st.ny = P.ny;
+    st.nz = P.nz;
+    st.out_j = P.out_j;
+    st.paths_sparse = P.empty ? 0 : s.paths;
+    st.dense_mode = DIM3_DENSE_OFF;
+    st.tile_bits = P.empty ? 0 : (1u << P.tile_shift);
+    st.n_tiles = P.ntiles;
+    st.block = (uint32_t)P.block;
+    st.ms_map = P.ms_map;
+    if (!P.empty) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+        st.ms_csr = ms;
+        hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+        st.ms_expand = ms;
+        hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
+        st.ms_total = ms;
+    }
+}
+
+int check_opts(dim3_ctx* c, const dim3_opts* opts, dim3_opts* o) {
+    dim3_opts_default(o);
+    if (opts) *o = *opts;
+    if (o->output_mode > DIM3_OUT_FINGERPRINT) return fail(c, DIM3_ERR_INVALID, "bad output_mode %u", o->output_mode);
+    if (o->dense_mode < -1 || o->dense_mode > 2) return fail(c, DIM3_ERR_INVALID, "bad dense_mode %d", o->dense_mode);
+    if (o->tile_bits < 0 || o->tile_bits > (1 << 30)) return fail(c, DIM3_ERR_INVALID, "bad tile_bits %d", o->tile_bits);
+    if (!(o->block == 0 || o->block == 256 || o->block == 512 || o->block == 1024))
+        return fail(c, DIM3_ERR_INVALID, "bad block %d", o->block);
+    return DIM3_OK;
+}
+
+int run_dev(dim3_ctx* c, bool count_mode, const int32_t* d_R, uint64_t nR, const int32_t* d_S, uint64_t nS,
+            const dim3_opts* opts, void* d_out, uint64_t cap, void* stream, dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if ((nR && !d_R) || (nS && !d_S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
+    if (!fp && cap && !d_out) return fail(c, DIM3_ERR_INVALID, "null output buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    TRY(prepare(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st));
+    TRY(expand(c, fp, d_out, cap, st));
+    fill_result(c, res, fp);
+    if (!fp && res->n > cap)
+        return fail(c, DIM3_ERR_CAPACITY, "output needs %llu rows, buffer holds %llu", (unsigned long long)res->n,
+                    (unsigned long long)cap);
+    return DIM3_OK;
+}
+
+int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const int32_t* S, uint64_t nS,
+             const dim3_opts* opts, dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if ((nR && !R) || (nS && !S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    int2 *dR, *dS;
+    TRY(ws(c, WS_IN_R, std::max<uint64_t>(nR, 1), &dR));
+    TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
+    if (nR) HIPCHK(c, hipMemcpyAsync(dR, R, nR * 8, hipMemcpyHostToDevice, st));
+    if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
+    TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
+    const size_t row_bytes = count_mode ? sizeof(dim3_count_row) : 8;
+    void* dout = nullptr;
+    uint64_t cap = 0;
+    if (!fp && !c->plan.empty) {
+        // rows <= min(OUT_J, nx*nz); size for that bound when it is modest,
+        // otherwise guess and re-run the expansion at the exact size
+        uint64_t bound = std::min<uint64_t>(c->plan.out_j, (uint64_t)c->plan.nx * c->plan.nz);
+        const uint64_t room = c->budget > c->held ? (c->budget - c->held) / row_bytes / 2 : 0;
+        cap = std::min<uint64_t>(bound, std::max<uint64_t>(room, 1u << 20));
+        cap = std::max<uint64_t>(cap, 1);
+        TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
+        cap = c->ws[WS_OUT].bytes / row_bytes;
+    }
+    TRY(expand(c, fp, dout, cap, st));
+    if (!fp && !c->plan.empty && c->h_sc->out_n > cap) {
+        cap = c->h_sc->out_n;
+        TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
+        TRY(expand(c, fp, dout, cap, st));
+    }
+    fill_result(c, res, fp);
+    if (!fp && res->n) {
+        void* h = malloc(res->n * row_bytes);
+        if (!h) return fail(c, DIM3_ERR_OOM, "host malloc of %llu rows", (unsigned long long)res->n);
+        hipError_t e = hipMemcpy(h, dout, res->n * row_bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            free(h);
+            return fail(c, DIM3_ERR_DEVICE, "D2H copy: %s", hipGetErrorString(e));
+        }
+        if (count_mode)
+            res->rows = (dim3_count_row*)h;
+        else
+            res->pairs = (int32_t*)h;
+    }
+    return DIM3_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dim3_version(void) { return "dim3-mi355x 0.1 (gfx950)"; }
+
+const char* dim3_status_string(int s) {
+    switch (s) {
+        case DIM3_OK: return "ok";
+        case DIM3_ERR_INVALID: return "invalid argument";
+        case DIM3_ERR_OOM: return "out of memory";
+        case DIM3_ERR_DEVICE: return "device error";
+        case DIM3_ERR_CAPACITY: return "output capacity too small";
+        case DIM3_ERR_ADMISSION: return "request can never be admitted";
+        case DIM3_ERR_BUSY: return "queue full";
+        case DIM3_ERR_NOT_FOUND: return "not found";
+        case DIM3_ERR_UNSUPPORTED: return "unsupported";
+        default: return "unknown status";
+    }
+}
+
+void dim3_opts_default(dim3_opts* o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->output_mode = DIM3_OUT_MATERIALIZE;
+    o->dense_mode = DIM3_DENSE_AUTO;
+}
+
+int dim3_ctx_create(int device, uint64_t hbm_budget_bytes, dim3_ctx** out) {
+    if (!out) return DIM3_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DIM3_ERR_DEVICE;
+    if (device < 0 || device >= ndev) return DIM3_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return DIM3_ERR_DEVICE;
+    dim3_ctx* c = new (std::nothrow) dim3_ctx();
+    if (!c) return DIM3_ERR_OOM;
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return DIM3_ERR_DEVICE;
+    }
+    c->cu_count = prop.multiProcessorCount;
+    c->max_lds = prop.sharedMemPerBlock;
+    if (c->max_lds > 160 * 1024) c->max_lds = 160 * 1024;
+    if (c->max_lds < 64 * 1024) c->max_lds = 64 * 1024;
+    size_t fr = 0, tot = 0;
+    hipMemGetInfo(&fr, &tot);
+    c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(fr * 0.9);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return DIM3_ERR_DEVICE;
+    }
+    memset(c->h_sc, 0, sizeof(Scalars));
+    for (auto& e : c->ev) hipEventCreate(&e);
+    *out = c;
+    return DIM3_OK;
+}
+
+int dim3_ctx_destroy(dim3_ctx* c) {
+    if (!c) return DIM3_ERR_INVALID;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& s : c->ws)
+        if (s.p) hipFree(s.p);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->h_sc) hipHostFree(c->h_sc);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return DIM3_OK;
+}
+
+const char* dim3_last_error(const dim3_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+uint64_t dim3_ctx_device_bytes(const dim3_ctx* c) { return c ? c->held : 0; }
+
+int dim3_join_project(dim3_ctx* c, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
+                      const dim3_opts* opts, dim3_result* res) {
+    return run_host(c, false, R_xy, n_r, S_zy, n_s, opts, res);
+}
+
+int dim3_join_aggregate_count(dim3_ctx* c, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
+                              const dim3_opts* opts, dim3_result* res) {
+    return run_host(c, true, R_xy, n_r, S_zy, n_s, opts, res);
+}
+
+void dim3_result_free(dim3_result* res) {
+    if (!res) return;
+    free(res->pairs);
+    free(res->rows);
+    res->pairs = nullptr;
+    res->rows = nullptr;
+    res->n = 0;
+}
+
+int dim3_join_project_dev(dim3_ctx* c, const int32_t* d_R, uint64_t n_r, const int32_t* d_S, uint64_t n_s,
+                          const dim3_opts* opts, int32_t* d_pairs, uint64_t cap, void* stream, dim3_result* res) {
+    return run_dev(c, false, d_R, n_r, d_S, n_s, opts, d_pairs, cap, stream, res);
+}
+
+int dim3_join_aggregate_count_dev(dim3_ctx* c, const int32_t* d_R, uint64_t n_r, const int32_t* d_S, uint64_t n_s,
+                                  const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap, void* stream,
+                                  dim3_result* res) {
+    return run_dev(c, true, d_R, n_r, d_S, n_s, opts, d_rows, cap, stream, res);
+}
+
+int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,
+                          uint64_t* out_j) {
+    if (!c || !out_j) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    *out_j = 0;
+    if (nR == 0 || nS == 0) return DIM3_OK;
+    if (!R_xy || !S_zy) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    int2 *dR, *dS;
+    TRY(ws(c, WS_IN_R, nR, &dR));
+    TRY(ws(c, WS_IN_S, nS, &dS));
+    HIPCHK(c, hipMemcpyAsync(dR, R_xy, nR * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(dS, S_zy, nS * 8, hipMemcpyHostToDevice, st));
+    const uint32_t cap = pow2_at_least(2 * nR);
+    uint32_t *keys, *degR, *degS, *slotR;
+    int32_t* slotS;
+    Scalars* sc;
+    TRY(ws(c, WS_KEYS_Y, (uint64_t)cap + 1, &keys));
+    TRY(ws(c, WS_DEGR_Y, (uint64_t)cap + 1, &degR));
+    TRY(ws(c, WS_DEGS_Y, (uint64_t)cap + 1, &degS));
+    TRY(ws(c, WS_SLOT_R, nR, &slotR));
+    TRY(ws(c, WS_SLOT_S, nS, &slotS));
+    TRY(ws(c, WS_SCALARS, 1, &sc));
+    hipMemsetAsync(sc, 0, sizeof(Scalars), st);
+    dim3h::ht_init(keys, cap, st);
+    dim3h::fill_u32(degR, (uint64_t)cap + 1, 0, st);
+    dim3h::fill_u32(degS, (uint64_t)cap + 1, 0, st);
+    dim3h::build_y((const int2*)dR, nR, keys, cap - 1, degR, slotR, st);
+    dim3h::probe_y((const int2*)dS, nS, keys, cap - 1, degS, slotS, st);
+    dim3h::outj_only(degR, degS, cap + 1, &sc->out_j, st);
+    LAUNCHCHK(c, "count_join_paths");
+    HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    *out_j = c->h_sc->out_j;
+    return DIM3_OK;
+}
+
+}  // extern "C"

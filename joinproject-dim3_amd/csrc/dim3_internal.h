@@ -1,0 +1,255 @@
+// Internal declarations shared by the libdim3 translation units.
+//
+// Device data layout (all in HBM, one set per context, reused across calls):
+//   relations        int2[n]        (x|z, y) tuples exactly as the caller passes them
+//   hash tables      u32 keys[cap+1] open addressing, linear probing, cap = 2^k >= 2n;
+//                    0xFFFFFFFF marks an empty slot, slot `cap` stands for the key -1
+//   degree arrays    u32 deg[cap+1]  per slot
+//   prefix arrays    u64 ex[cap+1]   exclusive scan of (occupied << 32 | degree):
+//                                    high word = dense ID, low word = CSR row start
+//   CSR of R by x    u32 rowptr[nx+1], i32 col[nnz] (y IDs)
+//   CSR of S by (y, z-tile)  u32 rowptr[ny*ntiles+1], i32 col[nnz] (z IDs)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dim3.h"
+
+#define DIM3_EMPTY_KEY 0xFFFFFFFFu
+
+namespace dim3d {
+
+// ---------------------------------------------------------------------------
+// hashing (murmur3 fmix32 finaliser; the reference uses the identity hash on
+// int plus radix partitioning, Include/structures.hpp:92-95 — the hash only
+// places keys, it never changes a result)
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint32_t hash32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// splitmix64 finaliser, used for the order-independent row fingerprint
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t k) {
+    k ^= k >> 30;
+    k *= 0xbf58476d1ce4e5b9ULL;
+    k ^= k >> 27;
+    k *= 0x94d049bb133111ebULL;
+    k ^= k >> 31;
+    return k;
+}
+
+__device__ __forceinline__ uint64_t row_key(int32_t x, int32_t z) {
+    return ((uint64_t)(uint32_t)x << 32) | (uint64_t)(uint32_t)z;
+}
+
+// Insert k, return its slot.  Keys only ever go EMPTY -> key, so a stale
+// plain read is either EMPTY (then the CAS settles it) or the final key.
+__device__ __forceinline__ uint32_t ht_insert(uint32_t* keys, uint32_t mask, uint32_t k) {
+    if (k == DIM3_EMPTY_KEY) {
+        if (keys[mask + 1] == 0) keys[mask + 1] = 1;  // benign race, every writer stores 1
+        return mask + 1;
+    }
+    uint32_t s = hash32(k) & mask;
+    for (;;) {
+        uint32_t cur = keys[s];
+        if (cur == k) return s;
+        if (cur == DIM3_EMPTY_KEY) {
+            uint32_t old = atomicCAS(&keys[s], DIM3_EMPTY_KEY, k);
+            if (old == DIM3_EMPTY_KEY || old == k) return s;
+        }
+        s = (s + 1) & mask;
+    }
+}
+
+// Slot of k, or -1.
+__device__ __forceinline__ int32_t ht_find(const uint32_t* keys, uint32_t mask, uint32_t k) {
+    if (k == DIM3_EMPTY_KEY) return keys[mask + 1] ? (int32_t)(mask + 1) : -1;
+    uint32_t s = hash32(k) & mask;
+    for (;;) {
+        uint32_t cur = keys[s];
+        if (cur == k) return (int32_t)s;
+        if (cur == DIM3_EMPTY_KEY) return -1;
+        s = (s + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ int32_t ht_key(const uint32_t* keys, uint32_t cap, uint32_t s) {
+    return s < cap ? (int32_t)keys[s] : -1;
+}
+
+// ---------------------------------------------------------------------------
+// wave / block primitives (64-wide wavefronts)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_reduce_sum(T v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_reduce_xor(T v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v ^= __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Exclusive scan over the block; sh needs BLOCK/64+1 entries.  Ends with a
+// barrier so sh can be reused right away.
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T& total, T* sh) {
+    constexpr int NW = BLOCK / 64;
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    T inc = wave_inclusive_scan(v);
+    if (lane == 63) sh[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            T t = sh[w];
+            sh[w] = acc;
+            acc += t;
+        }
+        sh[NW] = acc;
+    }
+    __syncthreads();
+    T res = sh[wid] + inc - v;
+    total = sh[NW];
+    __syncthreads();
+    return res;
+}
+
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_reduce_sum(T v, T* sh) {
+    constexpr int NW = BLOCK / 64;
+    v = wave_reduce_sum(v);
+    if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    T r = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) r += sh[w];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace dim3d
+
+// ---------------------------------------------------------------------------
+// host-side launch wrappers (defined in k_mapping.hip / k_expand.hip)
+// ---------------------------------------------------------------------------
+namespace dim3h {
+
+// Device scalar block, zeroed at the start of every query.
+struct Scalars {
+    uint64_t tot_y;      // scan total of the y table: (ny << 32) | 0
+    uint64_t tot_x;      // (nx << 32) | nnzR
+    uint64_t tot_z;      // (nz << 32) | nnzS
+    uint64_t out_j;      // sum degR*degS
+    uint64_t tot_s;      // total of the tiled S count scan
+    uint64_t out_n;      // rows emitted (may exceed capacity)
+    uint64_t fp_sum;
+    uint64_t fp_xr;
+    uint64_t fp_csum;
+    uint64_t paths;      // sparse join paths expanded
+    uint32_t task;       // expansion work counter
+    uint32_t pad0;
+    uint64_t heavy_n;    // heavy keys selected
+    uint64_t nxh, nzh;
+    uint64_t spare[4];
+};
+
+struct ScanTmp {
+    uint64_t* partials;  // >= scan_partials_needed(n)
+};
+
+size_t scan_partials_needed(uint64_t n);
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
+void ht_init(uint32_t* keys, uint32_t cap, hipStream_t st);
+
+// y mapping: insert R.y, count degR, record slots; probe S.y, count degS
+void build_y(const int2* R, uint64_t nR, uint32_t* keys, uint32_t mask, uint32_t* degR, uint32_t* slotR,
+             hipStream_t st);
+void probe_y(const int2* S, uint64_t nS, const uint32_t* keys, uint32_t mask, uint32_t* degS, int32_t* slotS,
+             hipStream_t st);
+// scan of the y table: ex[s] = (#joined y before s) << 32; total -> *tot; out_j accumulated
+void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* ex, uint64_t* partials,
+            uint64_t* tot, uint64_t* out_j, hipStream_t st);
+// x / z mapping over the tuples whose y joined
+void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, uint32_t* keys,
+             uint32_t mask, uint32_t* degX, int32_t* slotRx, hipStream_t st);
+void build_z(const int2* S, uint64_t nS, const int32_t* slotS, uint32_t* keys, uint32_t mask, uint32_t* degZ,
+             int32_t* slotSz, hipStream_t st);
+// scan of a degree table: ex[s] = (#occupied before s) << 32 | (sum deg before s)
+void scan_deg(const uint32_t* deg, uint32_t cap1, uint64_t* ex, uint64_t* partials, uint64_t* tot, hipStream_t st);
+// id2v / rowptr from a scanned table
+void compact_ids(const uint32_t* keys, uint32_t cap, const uint32_t* deg, const uint64_t* ex, int32_t* id2v,
+                 uint32_t* rowptr, hipStream_t st);
+void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_t st);
+// CSR of R by x: col = y id
+void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
+               const uint64_t* ex_x, uint32_t* cursor, int32_t* col, hipStream_t st);
+// tiled CSR of S by (y, z-tile)
+void count_s(const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y, const uint64_t* ex_z,
+             uint32_t ntiles, uint32_t tile_shift, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st);
+void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
+              hipStream_t st);
+void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, int32_t* col,
+               hipStream_t st);
+void x2z_self(const int32_t* id2x, uint32_t nx, const uint32_t* keysZ, uint32_t maskZ, const uint32_t* degZ,
+              const uint64_t* ex_z, int32_t* x2z, hipStream_t st);
+// exact OUT_J only (no ID assignment)
+void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* out_j, hipStream_t st);
+
+struct ExpandArgs {
+    const uint32_t* rowptrR;
+    const int32_t* colR;
+    const uint32_t* rowptrS;
+    const int32_t* colS;
+    const int32_t* id2x;
+    const int32_t* id2z;
+    const int32_t* x2z;  // nullable: z ID of each x value (exclude_equal)
+    uint32_t nx;
+    uint32_t ntiles;
+    uint32_t tile_shift;
+    uint32_t tile_words;  // u32 words (project) or counters (aggregate) held in LDS
+    uint32_t touched_cap;
+    uint32_t stage_cap;
+    uint32_t fp_mode;
+    Scalars* sc;
+    void* out;  // int2* (project) or dim3_count_row* (aggregate)
+    uint64_t cap;
+};
+
+// returns 0 or a hipError_t
+int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
+int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
+size_t expand_project_lds(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap);
+size_t expand_count_lds(uint32_t tile_entries, uint32_t touched_cap, uint32_t stage_cap);
+
+}  // namespace dim3h

@@ -1,0 +1,361 @@
+// Mapping (subsystem a) and CSR construction on the GPU.
+//
+// Replaces the reference's L+S mapping pipeline (dim3.hpp:399-455
+// mapping_step::mapping, 600-731 partition_*, 733-882 mapping_xz /
+// mapping_y, Include/L+S_mapper.hpp LS_mapper) and the CSR builds of
+// divide_R / divide_S (dim3.hpp:1058-1086, 1186-1253).
+//
+// Instead of radix partitions sized for a CPU L2, every attribute gets one
+// device-wide open-addressing table (u32 keys, CAS insert, linear probing,
+// load <= 1/2).  Dense IDs are the rank of the occupied slot, obtained by an
+// exclusive scan over the slots, so no ID counter is contended.  The same
+// scan of (occupied << 32 | degree) yields the CSR row starts.  Degree
+// histograms come from atomics on the slot counters.  All kernels here are
+// HBM-bound streaming passes over the tuples or the slots.
+#include "dim3_internal.h"
+
+namespace {
+
+using namespace dim3d;
+
+constexpr int kThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kThreads * kScanItems;  // 2048 elements per scan block
+
+inline unsigned grid_for(uint64_t n, int threads = kThreads) {
+    uint64_t g = (n + threads - 1) / threads;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_ht_init(uint32_t* keys, uint32_t cap) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cap) keys[i] = DIM3_EMPTY_KEY;
+    else if (i == cap) keys[i] = 0;
+}
+
+// mapping_y insert side (dim3.hpp:822-828): every R.y gets a slot; degR counts it.
+__global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, uint32_t* keys, uint32_t mask,
+                          uint32_t* degR, uint32_t* __restrict__ slotR) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nR) return;
+    uint32_t s = ht_insert(keys, mask, (uint32_t)R[i].y);
+    atomicAdd(&degR[s], 1u);
+    slotR[i] = s;
+}
+
+// mapping_y probe side (dim3.hpp:830-836): S tuples whose y is not in R are
+// dropped (slot -1); degS counts the survivors.
+__global__ void k_probe_y(const int2* __restrict__ S, uint64_t nS, const uint32_t* __restrict__ keys,
+                          uint32_t mask, uint32_t* degS, int32_t* __restrict__ slotS) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    int32_t s = ht_find(keys, mask, (uint32_t)S[i].y);
+    slotS[i] = s;
+    if (s >= 0) atomicAdd(&degS[s], 1u);
+}
+
+// mapping_xz over R.x (dim3.hpp:749-755), restricted to tuples whose y joined.
+__global__ void k_build_x(const int2* __restrict__ R, uint64_t nR, const uint32_t* __restrict__ slotR,
+                          const uint32_t* __restrict__ degS_y, uint32_t* keys, uint32_t mask, uint32_t* degX,
+                          int32_t* __restrict__ slotRx) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nR) return;
+    int32_t out = -1;
+    if (degS_y[slotR[i]] > 0) {
+        uint32_t s = ht_insert(keys, mask, (uint32_t)R[i].x);
+        atomicAdd(&degX[s], 1u);
+        out = (int32_t)s;
+    }
+    slotRx[i] = out;
+}
+
+// mapping_xz over the surviving S.z.
+__global__ void k_build_z(const int2* __restrict__ S, uint64_t nS, const int32_t* __restrict__ slotS,
+                          uint32_t* keys, uint32_t mask, uint32_t* degZ, int32_t* __restrict__ slotSz) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    int32_t out = -1;
+    if (slotS[i] >= 0) {
+        uint32_t s = ht_insert(keys, mask, (uint32_t)S[i].x);
+        atomicAdd(&degZ[s], 1u);
+        out = (int32_t)s;
+    }
+    slotSz[i] = out;
+}
+
+// ---------------------------------------------------------------------------
+// three-phase exclusive scan (reduce tiles, scan tile sums, rescan tiles)
+// ---------------------------------------------------------------------------
+struct ValJoinedY {  // slot of a y that occurs in both R and S
+    const uint32_t* degS;
+    __device__ uint64_t operator()(uint64_t s) const { return degS[s] ? (1ull << 32) : 0ull; }
+};
+struct ValDeg {  // occupied slot with its degree
+    const uint32_t* deg;
+    __device__ uint64_t operator()(uint64_t s) const {
+        uint32_t d = deg[s];
+        return d ? ((1ull << 32) | d) : 0ull;
+    }
+};
+struct ValU32 {
+    const uint32_t* a;
+    __device__ uint64_t operator()(uint64_t s) const { return a[s]; }
+};
+struct OutU64 {
+    uint64_t* o;
+    __device__ void operator()(uint64_t i, uint64_t v) const { o[i] = v; }
+};
+struct OutU32 {
+    uint32_t* o;
+    __device__ void operator()(uint64_t i, uint64_t v) const { o[i] = (uint32_t)v; }
+};
+
+template <typename F>
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(F f, uint64_t n, uint64_t* partials) {
+    __shared__ uint64_t sh[kThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) s += f(base + k);
+    s = block_reduce_sum<kThreads>(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_partials(uint64_t* partials, uint64_t nb, uint64_t* tot) {
+    __shared__ uint64_t sh[1024 / 64 + 1];
+    constexpr int IT = 8;
+    uint64_t carry = 0;
+    for (uint64_t c0 = 0; c0 < nb; c0 += 1024 * IT) {
+        const uint64_t base = c0 + (uint64_t)threadIdx.x * IT;
+        uint64_t v[IT];
+        uint64_t s = 0;
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            v[k] = base + k < nb ? partials[base + k] : 0;
+            s += v[k];
+        }
+        uint64_t total;
+        uint64_t ex = block_exclusive_scan<1024>(s, total, sh) + carry;
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            if (base + k < nb) partials[base + k] = ex;
+            ex += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0 && tot) *tot = carry;
+}
+
+template <typename F, typename W>
+__global__ __launch_bounds__(kThreads) void k_scan_down(F f, W w, uint64_t n, const uint64_t* partials) {
+    __shared__ uint64_t sh[kThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < n ? f(base + k) : 0;
+        s += v[k];
+    }
+    uint64_t total;
+    uint64_t ex = block_exclusive_scan<kThreads>(s, total, sh) + partials[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) w(base + k, ex);
+        ex += v[k];
+    }
+}
+
+template <typename F, typename W>
+void scan_exclusive(F f, W w, uint64_t n, uint64_t* partials, uint64_t* tot, hipStream_t st) {
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) {
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partials, (uint64_t)0, tot);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_reduce<F>, dim3((unsigned)nb), dim3(kThreads), 0, st, f, n, partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partials, nb, tot);
+    hipLaunchKernelGGL((k_scan_down<F, W>), dim3((unsigned)nb), dim3(kThreads), 0, st, f, w, n,
+                       (const uint64_t*)partials);
+}
+
+// OUT_J = sum over joined y of degR * degS
+__global__ __launch_bounds__(kThreads) void k_outj(const uint32_t* __restrict__ degR,
+                                                   const uint32_t* __restrict__ degS, uint32_t cap1,
+                                                   uint64_t* out_j) {
+    __shared__ uint64_t sh[kThreads / 64 + 1];
+    uint64_t acc = 0;
+    for (uint64_t s = (uint64_t)blockIdx.x * kThreads + threadIdx.x; s < cap1; s += (uint64_t)gridDim.x * kThreads)
+        acc += (uint64_t)degR[s] * (uint64_t)degS[s];
+    acc = block_reduce_sum<kThreads>(acc, sh);
+    if (threadIdx.x == 0 && acc) atomicAdd((unsigned long long*)out_j, (unsigned long long)acc);
+}
+
+__global__ void k_compact_ids(const uint32_t* __restrict__ keys, uint32_t cap, const uint32_t* __restrict__ deg,
+                              const uint64_t* __restrict__ ex, int32_t* __restrict__ id2v,
+                              uint32_t* __restrict__ rowptr) {
+    uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > cap) return;
+    if (deg[s] == 0) return;
+    const uint64_t e = ex[s];
+    const uint32_t id = (uint32_t)(e >> 32);
+    id2v[id] = ht_key(keys, cap, (uint32_t)s);
+    if (rowptr) rowptr[id] = (uint32_t)e;
+}
+
+__global__ void k_finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) rowptr[n] = (uint32_t)(*tot);
+}
+
+// divide_R (dim3.hpp:1058-1086) as an atomic-cursor scatter: rows are
+// contiguous, the order inside a row is not kept (a row is a set).
+__global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
+                            const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_x,
+                            uint32_t* cursor, int32_t* __restrict__ col) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nR) return;
+    const int32_t sx = slotRx[i];
+    if (sx < 0) return;
+    const uint32_t xid = (uint32_t)(ex_x[sx] >> 32);
+    const uint32_t pos = atomicAdd(&cursor[xid], 1u);
+    col[pos] = (int32_t)(ex_y[slotR[i]] >> 32);
+}
+
+// divide_S (dim3.hpp:1186-1253): CSR of S keyed by (y, z-tile) so that the
+// expansion of one z tile reads only its own column range.
+__global__ void k_count_s(const int32_t* __restrict__ slotS, const int32_t* __restrict__ slotSz, uint64_t nS,
+                          const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_z, uint32_t ntiles,
+                          uint32_t tile_shift, uint32_t* cnt, uint32_t* __restrict__ keyS,
+                          int32_t* __restrict__ zidS) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    const int32_t sy = slotS[i];
+    if (sy < 0) {
+        keyS[i] = 0xFFFFFFFFu;
+        return;
+    }
+    const uint32_t yid = (uint32_t)(ex_y[sy] >> 32);
+    const uint32_t zid = (uint32_t)(ex_z[slotSz[i]] >> 32);
+    const uint32_t key = yid * ntiles + (zid >> tile_shift);
+    atomicAdd(&cnt[key], 1u);
+    keyS[i] = key;
+    zidS[i] = (int32_t)zid;
+}
+
+__global__ void k_scatter_s(const uint32_t* __restrict__ keyS, const int32_t* __restrict__ zidS, uint64_t nS,
+                            uint32_t* cursor, int32_t* __restrict__ col) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    const uint32_t key = keyS[i];
+    if (key == 0xFFFFFFFFu) return;
+    const uint32_t pos = atomicAdd(&cursor[key], 1u);
+    col[pos] = zidS[i];
+}
+
+__global__ void k_x2z_self(const int32_t* __restrict__ id2x, uint32_t nx, const uint32_t* __restrict__ keysZ,
+                           uint32_t maskZ, const uint32_t* __restrict__ degZ, const uint64_t* __restrict__ ex_z,
+                           int32_t* __restrict__ x2z) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx) return;
+    const int32_t s = ht_find(keysZ, maskZ, (uint32_t)id2x[i]);
+    x2z[i] = (s >= 0 && degZ[s] > 0) ? (int32_t)(ex_z[s] >> 32) : -1;
+}
+
+}  // namespace
+
+namespace dim3h {
+
+size_t scan_partials_needed(uint64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(kThreads), 0, st, p, n, v);
+}
+void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(kThreads), 0, st, p, n, v);
+}
+void ht_init(uint32_t* keys, uint32_t cap, hipStream_t st) {
+    hipLaunchKernelGGL(k_ht_init, dim3(grid_for((uint64_t)cap + 1)), dim3(kThreads), 0, st, keys, cap);
+}
+
+void build_y(const int2* R, uint64_t nR, uint32_t* keys, uint32_t mask, uint32_t* degR, uint32_t* slotR,
+             hipStream_t st) {
+    if (nR) hipLaunchKernelGGL(k_build_y, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, keys, mask, degR, slotR);
+}
+void probe_y(const int2* S, uint64_t nS, const uint32_t* keys, uint32_t mask, uint32_t* degS, int32_t* slotS,
+             hipStream_t st) {
+    if (nS) hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, keys, mask, degS, slotS);
+}
+void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* ex, uint64_t* partials,
+            uint64_t* tot, uint64_t* out_j, hipStream_t st) {
+    scan_exclusive(ValJoinedY{degS}, OutU64{ex}, cap1, partials, tot, st);
+    outj_only(degR, degS, cap1, out_j, st);
+}
+void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* out_j, hipStream_t st) {
+    unsigned g = grid_for(cap1);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_outj, dim3(g), dim3(kThreads), 0, st, degR, degS, cap1, out_j);
+}
+void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, uint32_t* keys,
+             uint32_t mask, uint32_t* degX, int32_t* slotRx, hipStream_t st) {
+    if (nR)
+        hipLaunchKernelGGL(k_build_x, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, degS_y, keys, mask,
+                           degX, slotRx);
+}
+void build_z(const int2* S, uint64_t nS, const int32_t* slotS, uint32_t* keys, uint32_t mask, uint32_t* degZ,
+             int32_t* slotSz, hipStream_t st) {
+    if (nS)
+        hipLaunchKernelGGL(k_build_z, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, keys, mask, degZ,
+                           slotSz);
+}
+void scan_deg(const uint32_t* deg, uint32_t cap1, uint64_t* ex, uint64_t* partials, uint64_t* tot, hipStream_t st) {
+    scan_exclusive(ValDeg{deg}, OutU64{ex}, cap1, partials, tot, st);
+}
+void compact_ids(const uint32_t* keys, uint32_t cap, const uint32_t* deg, const uint64_t* ex, int32_t* id2v,
+                 uint32_t* rowptr, hipStream_t st) {
+    hipLaunchKernelGGL(k_compact_ids, dim3(grid_for((uint64_t)cap + 1)), dim3(kThreads), 0, st, keys, cap, deg, ex,
+                       id2v, rowptr);
+}
+void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish_rowptr, dim3(1), dim3(64), 0, st, rowptr, n, tot);
+}
+void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y, const uint64_t* ex_x,
+               uint32_t* cursor, int32_t* col, hipStream_t st) {
+    if (nR)
+        hipLaunchKernelGGL(k_scatter_r, dim3(grid_for(nR)), dim3(kThreads), 0, st, slotR, slotRx, nR, ex_y, ex_x,
+                           cursor, col);
+}
+void count_s(const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y, const uint64_t* ex_z,
+             uint32_t ntiles, uint32_t tile_shift, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
+    if (nS)
+        hipLaunchKernelGGL(k_count_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, slotS, slotSz, nS, ex_y, ex_z,
+                           ntiles, tile_shift, cnt, keyS, zidS);
+}
+void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
+              hipStream_t st) {
+    scan_exclusive(ValU32{cnt}, OutU32{out_excl}, n, partials, tot, st);
+}
+void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, int32_t* col,
+               hipStream_t st) {
+    if (nS)
+        hipLaunchKernelGGL(k_scatter_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, keyS, zidS, nS, cursor, col);
+}
+void x2z_self(const int32_t* id2x, uint32_t nx, const uint32_t* keysZ, uint32_t maskZ, const uint32_t* degZ,
+              const uint64_t* ex_z, int32_t* x2z, hipStream_t st) {
+    if (nx)
+        hipLaunchKernelGGL(k_x2z_self, dim3(grid_for(nx)), dim3(kThreads), 0, st, id2x, nx, keysZ, maskZ, degZ,
+                           ex_z, x2z);
+}
+
+}  // namespace dim3h

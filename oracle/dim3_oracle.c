@@ -1,0 +1,486 @@
+/*
+ * CPU oracle for DIM3 join-project and join-aggregate (COUNT).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links or calls
+ * this file; only tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark load it, and only as the checker.
+ *
+ * It restates, in plain C with 64-bit counts, the algorithm of the reference
+ * (schencoding/JoinProject-DIM3):
+ *
+ *   oracle_join_project     DIM3::doJoinProject          dim3.hpp:987-1055
+ *       swap rule                                         dim3.hpp:993-994
+ *       y-ID mapping + semi-join filter of S              dim3.hpp:794-836 (mapping_y)
+ *       x / z ID mapping                                  dim3.hpp:733-761 (mapping_xz)
+ *       early return when no S tuple survives             dim3.hpp:1027
+ *       CSR of R by x                                     dim3.hpp:1058-1086 (divide_R)
+ *       CSR of S by y (all-sparse branch)                 dim3.hpp:1230-1253 (divide_S)
+ *       SPA expansion with last-writer dedup              dim3.hpp:952-973
+ *   oracle_join_aggregate_count
+ *                           DIM3JoinAggregate + SpmmAggregate with a COUNT
+ *                           policy                        dim3.hpp:1267-1373
+ *       first touch -> insert (count = 1), later -> update (+1); rows are
+ *       emitted in first-touch order                      dim3.hpp:1286-1307
+ *   oracle_estimate         EstimateJoinCardinality       dim3.hpp:193-224, 262-269
+ *   oracle_outj             exact sum_b degR(b)*degS(b)   (closed form of the above)
+ *   oracle_gen_rand_data    gen_rand_data (minstd_rand0)  dim3.hpp:1556-1568
+ *
+ * Deliberate differences, none of which is observable in the output set:
+ *   - no radix partitioning (dim3.hpp:600-731 is a cache optimisation),
+ *   - one open-addressing table per attribute instead of LS_mapper + ska
+ *     overflow cache (Include/L+S_mapper.hpp), IDs in first-seen order,
+ *   - no dense/sparse split: the reference's f2 split (dim3.hpp:1118-1126)
+ *     partitions z values and the dense and sparse outputs are disjoint, so
+ *     the all-sparse SPA yields the same set,
+ *   - counts are 64-bit (the reference's uint32 counters overflow above
+ *     ~2.1e9 rows),
+ *   - the estimate fallback (dim3.hpp:215-221) reads min(1000, n) tuples
+ *     instead of reading past the end of small inputs.
+ *
+ * Fingerprint (order independent, shared with the GPU library):
+ *   key = (u64)(u32)x << 32 | (u32)z, h = mix64(key)
+ *   project:   count += 1, sum += h, xr ^= h
+ *   aggregate: h' = mix64(h + (u64)count); count += 1, sum += h', xr ^= h',
+ *              csum += count
+ * mix64 is the splitmix64 finaliser (Steele, Lea, Flood 2014).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_OK 0
+#define ORC_OOM 2
+#define ORC_INVALID 1
+
+typedef struct {
+    uint64_t count;
+    uint64_t sum;
+    uint64_t xr;
+    uint64_t csum;
+} orc_fp;
+
+static inline uint64_t orc_mix64(uint64_t k) {
+    k ^= k >> 30;
+    k *= 0xbf58476d1ce4e5b9ULL;
+    k ^= k >> 27;
+    k *= 0x94d049bb133111ebULL;
+    k ^= k >> 31;
+    return k;
+}
+
+static inline uint64_t orc_key(int32_t x, int32_t z) {
+    return ((uint64_t)(uint32_t)x << 32) | (uint64_t)(uint32_t)z;
+}
+
+uint64_t oracle_mix64(uint64_t k) { return orc_mix64(k); }
+
+/* ------------------------------------------------------------------ */
+/* int32 -> dense ID map, open addressing with linear probing.          */
+/* Every int32 is a legal key (the reference allows -1 too,             */
+/* Include/L+S_mapper.hpp:67-104), so occupancy is a separate byte.     */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int32_t* keys;
+    int32_t* ids;
+    uint8_t* used;
+    uint64_t mask;
+    int32_t n;
+} omap;
+
+static inline uint32_t omap_hash(int32_t k) {
+    uint32_t h = (uint32_t)k;
+    h ^= h >> 16;
+    h *= 0x85ebca6bU;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35U;
+    h ^= h >> 16;
+    return h;
+}
+
+static int omap_init(omap* m, uint64_t expected) {
+    uint64_t cap = 16;
+    while (cap < 2 * expected + 16) cap <<= 1;
+    m->keys = (int32_t*)malloc(cap * sizeof(int32_t));
+    m->ids = (int32_t*)malloc(cap * sizeof(int32_t));
+    m->used = (uint8_t*)calloc(cap, 1);
+    m->mask = cap - 1;
+    m->n = 0;
+    if (!m->keys || !m->ids || !m->used) return ORC_OOM;
+    return ORC_OK;
+}
+
+static void omap_free(omap* m) {
+    free(m->keys);
+    free(m->ids);
+    free(m->used);
+    memset(m, 0, sizeof(*m));
+}
+
+/* Returns the dense ID of key, assigning the next one on first sight
+ * (LS_mapper::insert, Include/L+S_mapper.hpp:58-114). */
+static int32_t omap_insert(omap* m, int32_t key) {
+    uint64_t s = omap_hash(key) & m->mask;
+    for (;;) {
+        if (!m->used[s]) {
+            m->used[s] = 1;
+            m->keys[s] = key;
+            m->ids[s] = m->n;
+            return m->n++;
+        }
+        if (m->keys[s] == key) return m->ids[s];
+        s = (s + 1) & m->mask;
+    }
+}
+
+/* Returns the dense ID of key or -1 (LS_mapper::find, L+S_mapper.hpp:116-160). */
+static int32_t omap_find(const omap* m, int32_t key) {
+    uint64_t s = omap_hash(key) & m->mask;
+    for (;;) {
+        if (!m->used[s]) return -1;
+        if (m->keys[s] == key) return m->ids[s];
+        s = (s + 1) & m->mask;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Output sinks                                                          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int fingerprint_only;
+    int with_counts;
+    int32_t* pairs;  /* 2*cap int32: x, z */
+    int64_t* counts; /* cap */
+    uint64_t n, cap;
+    orc_fp fp;
+    int oom;
+} orc_sink;
+
+static void sink_push(orc_sink* k, int32_t x, int32_t z, int64_t c) {
+    uint64_t h = orc_mix64(orc_key(x, z));
+    if (k->with_counts) {
+        h = orc_mix64(h + (uint64_t)c);
+        k->fp.csum += (uint64_t)c;
+    }
+    k->fp.count += 1;
+    k->fp.sum += h;
+    k->fp.xr ^= h;
+    if (k->fingerprint_only || k->oom) return;
+    if (k->n == k->cap) {
+        uint64_t nc = k->cap ? 2 * k->cap : 1024;
+        int32_t* p = (int32_t*)realloc(k->pairs, nc * 2 * sizeof(int32_t));
+        if (!p) { k->oom = 1; return; }
+        k->pairs = p;
+        if (k->with_counts) {
+            int64_t* q = (int64_t*)realloc(k->counts, nc * sizeof(int64_t));
+            if (!q) { k->oom = 1; return; }
+            k->counts = q;
+        }
+        k->cap = nc;
+    }
+    k->pairs[2 * k->n] = x;
+    k->pairs[2 * k->n + 1] = z;
+    if (k->with_counts) k->counts[k->n] = c;
+    k->n++;
+}
+
+/* ------------------------------------------------------------------ */
+/* Shared mapping + CSR build (dim3.hpp:399-455, 1058-1086, 1230-1253)  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t* id2x; /* nx */
+    int32_t* id2z; /* nz */
+    uint64_t* rjr; /* nx+1 : R CSR by x, columns = y id */
+    int32_t* ric;
+    uint64_t* sjr; /* ny+1 : S CSR by y, columns = z id */
+    int32_t* sic;
+    uint64_t* sidx; /* S CSR slot -> original S tuple index (for the row's values) */
+} orc_plan;
+
+static void plan_free(orc_plan* p) {
+    free(p->id2x); free(p->id2z); free(p->rjr); free(p->ric);
+    free(p->sjr); free(p->sic); free(p->sidx);
+    memset(p, 0, sizeof(*p));
+}
+
+/* R and S are interleaved (first, second) int32 pairs with the join key in
+ * .second (main.cpp:6-11).  Returns ORC_OK; p->nx == 0 means empty output. */
+static int build_plan(const int32_t* R, uint64_t nR, const int32_t* S, uint64_t nS, orc_plan* p) {
+    memset(p, 0, sizeof(*p));
+    omap my, mx, mz;
+    int rc = ORC_OK;
+    int32_t* ry = NULL; int32_t* rx = NULL; int32_t* sy = NULL; int32_t* sz = NULL;
+    uint64_t nSm = 0;
+    if (omap_init(&my, nR) || omap_init(&mx, nR) || omap_init(&mz, nS)) return ORC_OOM;
+
+    ry = (int32_t*)malloc((nR ? nR : 1) * sizeof(int32_t));
+    rx = (int32_t*)malloc((nR ? nR : 1) * sizeof(int32_t));
+    sy = (int32_t*)malloc((nS ? nS : 1) * sizeof(int32_t));
+    sz = (int32_t*)malloc((nS ? nS : 1) * sizeof(int32_t));
+    if (!ry || !rx || !sy || !sz) { rc = ORC_OOM; goto done; }
+
+    /* mapping_y: y IDs from R, S probes them and drops misses (dim3.hpp:822-836) */
+    for (uint64_t i = 0; i < nR; i++) ry[i] = omap_insert(&my, R[2 * i + 1]);
+    for (uint64_t i = 0; i < nS; i++) {
+        int32_t t = omap_find(&my, S[2 * i + 1]);
+        if (t != -1) { sy[nSm] = t; sz[nSm] = S[2 * i]; nSm++; }
+    }
+    /* dim3.hpp:1027: nothing survives -> empty result */
+    if (nR == 0 || nSm == 0) goto done;
+
+    /* mapping_xz over R.x and the surviving S.z (dim3.hpp:733-761) */
+    for (uint64_t i = 0; i < nR; i++) rx[i] = omap_insert(&mx, R[2 * i]);
+    for (uint64_t i = 0; i < nSm; i++) sz[i] = omap_insert(&mz, sz[i]);
+    p->nx = mx.n; p->ny = my.n; p->nz = mz.n;
+    p->id2x = (int32_t*)malloc((size_t)p->nx * sizeof(int32_t));
+    p->id2z = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    if (!p->id2x || !p->id2z) { rc = ORC_OOM; goto done; }
+    for (uint64_t s = 0; s <= mx.mask; s++) if (mx.used[s]) p->id2x[mx.ids[s]] = mx.keys[s];
+    for (uint64_t s = 0; s <= mz.mask; s++) if (mz.used[s]) p->id2z[mz.ids[s]] = mz.keys[s];
+
+    /* divide_R: stable counting sort of R by x (dim3.hpp:1058-1086) */
+    p->rjr = (uint64_t*)calloc((size_t)p->nx + 1, sizeof(uint64_t));
+    p->ric = (int32_t*)malloc((nR ? nR : 1) * sizeof(int32_t));
+    if (!p->rjr || !p->ric) { rc = ORC_OOM; goto done; }
+    for (uint64_t i = 0; i < nR; i++) p->rjr[rx[i] + 1]++;
+    for (int32_t i = 0; i < p->nx; i++) p->rjr[i + 1] += p->rjr[i];
+    {
+        uint64_t* cur = (uint64_t*)malloc(((size_t)p->nx + 1) * sizeof(uint64_t));
+        if (!cur) { rc = ORC_OOM; goto done; }
+        memcpy(cur, p->rjr, ((size_t)p->nx + 1) * sizeof(uint64_t));
+        for (uint64_t i = 0; i < nR; i++) p->ric[cur[rx[i]]++] = ry[i];
+        free(cur);
+    }
+
+    /* divide_S, all-sparse branch: stable counting sort of S by y
+     * (dim3.hpp:1230-1253) */
+    p->sjr = (uint64_t*)calloc((size_t)p->ny + 1, sizeof(uint64_t));
+    p->sic = (int32_t*)malloc(nSm * sizeof(int32_t));
+    if (!p->sjr || !p->sic) { rc = ORC_OOM; goto done; }
+    for (uint64_t i = 0; i < nSm; i++) p->sjr[sy[i] + 1]++;
+    for (int32_t i = 0; i < p->ny; i++) p->sjr[i + 1] += p->sjr[i];
+    {
+        uint64_t* cur = (uint64_t*)malloc(((size_t)p->ny + 1) * sizeof(uint64_t));
+        if (!cur) { rc = ORC_OOM; goto done; }
+        memcpy(cur, p->sjr, ((size_t)p->ny + 1) * sizeof(uint64_t));
+        for (uint64_t i = 0; i < nSm; i++) p->sic[cur[sy[i]]++] = sz[i];
+        free(cur);
+    }
+
+done:
+    free(ry); free(rx); free(sy); free(sz);
+    omap_free(&my); omap_free(&mx); omap_free(&mz);
+    if (rc != ORC_OK) plan_free(p);
+    return rc;
+}
+
+/* Hybrid_solution sparse part (dim3.hpp:952-973): SPAw[z] holds the last
+ * row that emitted z. */
+static int run_project(const orc_plan* p, int exclude_equal, orc_sink* k) {
+    if (p->nx == 0) return ORC_OK;
+    int32_t* spaw = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    if (!spaw) return ORC_OOM;
+    for (int32_t i = 0; i < p->nz; i++) spaw[i] = -1;
+    for (int32_t x = 0; x < p->nx; x++) {
+        int32_t xv = p->id2x[x];
+        for (uint64_t i = p->rjr[x]; i < p->rjr[x + 1]; i++) {
+            int32_t y = p->ric[i];
+            for (uint64_t j = p->sjr[y]; j < p->sjr[y + 1]; j++) {
+                int32_t z = p->sic[j];
+                if (spaw[z] != x) {
+                    spaw[z] = x;
+                    int32_t zv = p->id2z[z];
+                    if (exclude_equal && zv == xv) continue;
+                    sink_push(k, xv, zv, 1);
+                }
+            }
+        }
+    }
+    free(spaw);
+    return k->oom ? ORC_OOM : ORC_OK;
+}
+
+/* SpmmAggregate with a COUNT policy (dim3.hpp:1286-1307): insert sets the
+ * count to 1, update adds 1, rows are emitted in first-touch order.  A
+ * duplicate (x,y) or (z,y) tuple contributes one join path per copy. */
+static int run_count(const orc_plan* p, int exclude_equal, orc_sink* k) {
+    if (p->nx == 0) return ORC_OK;
+    int32_t* spaw = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    int64_t* cnt = (int64_t*)malloc((size_t)p->nz * sizeof(int64_t));
+    int32_t* list = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    if (!spaw || !cnt || !list) { free(spaw); free(cnt); free(list); return ORC_OOM; }
+    for (int32_t i = 0; i < p->nz; i++) spaw[i] = -1;
+    for (int32_t x = 0; x < p->nx; x++) {
+        int32_t xv = p->id2x[x];
+        int32_t nl = 0;
+        for (uint64_t i = p->rjr[x]; i < p->rjr[x + 1]; i++) {
+            int32_t y = p->ric[i];
+            for (uint64_t j = p->sjr[y]; j < p->sjr[y + 1]; j++) {
+                int32_t z = p->sic[j];
+                if (spaw[z] != x) { spaw[z] = x; cnt[z] = 1; list[nl++] = z; }
+                else cnt[z] += 1;
+            }
+        }
+        for (int32_t i = 0; i < nl; i++) {
+            int32_t zv = p->id2z[list[i]];
+            if (exclude_equal && zv == xv) continue;
+            sink_push(k, xv, zv, cnt[list[i]]);
+        }
+    }
+    free(spaw); free(cnt); free(list);
+    return k->oom ? ORC_OOM : ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Public entry points (ctypes)                                          */
+/* ------------------------------------------------------------------ */
+
+/* Join-project.  R_xy / S_zy hold interleaved int32 pairs.  When nR < nS
+ * the roles are swapped (dim3.hpp:993-994) and rows come out as
+ * (S.first, R.first); *swapped reports that.  With fingerprint_only the
+ * rows are not stored.  Output pairs are malloc'd; free with oracle_free. */
+int oracle_join_project(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,
+                        int exclude_equal, int fingerprint_only,
+                        int32_t** out_pairs, uint64_t* n_out, orc_fp* fp, int* swapped) {
+    const int32_t* R = R_xy; const int32_t* S = S_zy;
+    int sw = 0;
+    if (nR < nS) { const int32_t* t = R; R = S; S = t; uint64_t u = nR; nR = nS; nS = u; sw = 1; }
+    if (swapped) *swapped = sw;
+    orc_plan p;
+    int rc = build_plan(R, nR, S, nS, &p);
+    if (rc) return rc;
+    orc_sink k;
+    memset(&k, 0, sizeof(k));
+    k.fingerprint_only = fingerprint_only;
+    rc = run_project(&p, exclude_equal, &k);
+    plan_free(&p);
+    if (fp) *fp = k.fp;
+    if (n_out) *n_out = k.fp.count;
+    if (rc) { free(k.pairs); return rc; }
+    if (out_pairs) *out_pairs = k.pairs; else free(k.pairs);
+    return ORC_OK;
+}
+
+/* Join-aggregate COUNT(*) GROUP BY x,z.  Same swap rule (dim3.hpp:1325-1326). */
+int oracle_join_aggregate_count(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,
+                                int exclude_equal, int fingerprint_only,
+                                int32_t** out_pairs, int64_t** out_counts, uint64_t* n_out,
+                                orc_fp* fp, int* swapped) {
+    const int32_t* R = R_xy; const int32_t* S = S_zy;
+    int sw = 0;
+    if (nR < nS) { const int32_t* t = R; R = S; S = t; uint64_t u = nR; nR = nS; nS = u; sw = 1; }
+    if (swapped) *swapped = sw;
+    orc_plan p;
+    int rc = build_plan(R, nR, S, nS, &p);
+    if (rc) return rc;
+    orc_sink k;
+    memset(&k, 0, sizeof(k));
+    k.fingerprint_only = fingerprint_only;
+    k.with_counts = 1;
+    rc = run_count(&p, exclude_equal, &k);
+    plan_free(&p);
+    if (fp) *fp = k.fp;
+    if (n_out) *n_out = k.fp.count;
+    if (rc) { free(k.pairs); free(k.counts); return rc; }
+    if (out_pairs) *out_pairs = k.pairs; else free(k.pairs);
+    if (out_counts) *out_counts = k.counts; else free(k.counts);
+    return ORC_OK;
+}
+
+/* Exact number of join paths OUT_J = sum_b degR(b) * degS(b). */
+uint64_t oracle_outj(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS) {
+    omap m;
+    if (omap_init(&m, nR)) return 0;
+    uint64_t* deg = (uint64_t*)calloc(nR + 1, sizeof(uint64_t));
+    if (!deg) { omap_free(&m); return 0; }
+    for (uint64_t i = 0; i < nR; i++) deg[omap_insert(&m, R_xy[2 * i + 1])]++;
+    uint64_t out = 0;
+    for (uint64_t i = 0; i < nS; i++) {
+        int32_t t = omap_find(&m, S_zy[2 * i + 1]);
+        if (t >= 0) out += deg[t];
+    }
+    free(deg);
+    omap_free(&m);
+    return out;
+}
+
+/* EstimateJoinCardinality::estimate (dim3.hpp:193-224) with the identity
+ * hash myHasher (Include/structures.hpp:92-95): keep keys whose low
+ * sample_bits are zero, where sample_bits halves |S| until it is at most
+ * 10000 (dim3.hpp:262-269), and scale the sampled path count back up.
+ * Requires nR >= nS like the reference (dim3.hpp:194). */
+int64_t oracle_estimate(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS) {
+    int bits = 0;
+    int64_t ss = (int64_t)nS;
+    while (ss > 10000) { bits++; ss /= 2; }
+    uint32_t mask = (1u << bits) - 1u;
+    omap m;
+    if (omap_init(&m, nS)) return -1;
+    int64_t* cnt = (int64_t*)calloc(nS + 1, sizeof(int64_t));
+    if (!cnt) { omap_free(&m); return -1; }
+    for (uint64_t i = 0; i < nS; i++)
+        if (((uint32_t)S_zy[2 * i + 1] & mask) == 0) cnt[omap_insert(&m, S_zy[2 * i + 1])]++;
+    int64_t res = 0;
+    for (uint64_t i = 0; i < nR; i++) {
+        if (((uint32_t)R_xy[2 * i + 1] & mask) == 0) {
+            int32_t t = omap_find(&m, R_xy[2 * i + 1]);
+            if (t >= 0) res += cnt[t];
+        }
+    }
+    free(cnt);
+    omap_free(&m);
+    if (res == 0) {
+        /* fallback on a 1000x1000 prefix (dim3.hpp:215-221), clamped to the
+         * input sizes */
+        uint64_t a = nR < 1000 ? nR : 1000, b = nS < 1000 ? nS : 1000;
+        omap f;
+        if (omap_init(&f, a)) return -1;
+        int64_t* c2 = (int64_t*)calloc(a + 1, sizeof(int64_t));
+        if (!c2) { omap_free(&f); return -1; }
+        for (uint64_t i = 0; i < a; i++) c2[omap_insert(&f, R_xy[2 * i + 1])]++;
+        int64_t r2 = 0;
+        for (uint64_t i = 0; i < b; i++) {
+            int32_t t = omap_find(&f, S_zy[2 * i + 1]);
+            if (t >= 0) r2 += c2[t];
+        }
+        free(c2);
+        omap_free(&f);
+        return (int64_t)((double)r2 * ((double)nR / 1e3) * ((double)nS / 1e3));
+    }
+    return res << bits;
+}
+
+/* gen_rand_data (dim3.hpp:1556-1568): std::default_random_engine, which
+ * libstdc++ defines as minstd_rand0 (x <- 16807 x mod 2^31-1, seeded with
+ * seed mod m, or 1 when that is 0).  R gets (x, y) draws, then S gets (z, y)
+ * draws; each draw is engine() % MOD. */
+void oracle_gen_rand_data(int32_t* R_xy, uint64_t nR, int32_t* S_zy, uint64_t nS,
+                          uint32_t modx, uint32_t mody, uint32_t modz, uint32_t seed) {
+    const uint64_t m = 2147483647ULL;
+    uint64_t st = seed % m;
+    if (st == 0) st = 1;
+#define ORC_NEXT() (st = (st * 16807ULL) % m)
+    for (uint64_t i = 0; i < nR; i++) {
+        R_xy[2 * i] = (int32_t)(ORC_NEXT() % modx);
+        R_xy[2 * i + 1] = (int32_t)(ORC_NEXT() % mody);
+    }
+    for (uint64_t i = 0; i < nS; i++) {
+        S_zy[2 * i] = (int32_t)(ORC_NEXT() % modz);
+        S_zy[2 * i + 1] = (int32_t)(ORC_NEXT() % mody);
+    }
+#undef ORC_NEXT
+}
+
+/* Fingerprint of a row set given as arrays (counts may be NULL). */
+void oracle_fingerprint(const int32_t* pairs, const int64_t* counts, uint64_t n, orc_fp* fp) {
+    memset(fp, 0, sizeof(*fp));
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t h = orc_mix64(orc_key(pairs[2 * i], pairs[2 * i + 1]));
+        if (counts) { h = orc_mix64(h + (uint64_t)counts[i]); fp->csum += (uint64_t)counts[i]; }
+        fp->count += 1;
+        fp->sum += h;
+        fp->xr ^= h;
+    }
+}
+
+void oracle_free(void* p) { free(p); }

@@ -1,0 +1,96 @@
+"""Inputs of the golden fixtures.  Shared by make_golden.py (which writes
+the reference's outputs) and the tests (which regenerate the medium inputs
+and check their sha256 before comparing)."""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def _wl():
+    import __graft_entry__ as GE
+
+    GE.load_package()
+    from dim3_amd import workloads
+
+    return workloads
+
+
+def _rand_pairs(rng, n, a_lo, a_hi, b_lo, b_hi):
+    return np.stack([rng.integers(a_lo, a_hi, n), rng.integers(b_lo, b_hi, n)], 1).astype(np.int32)
+
+
+def bernoulli_block(nx, ny, p, seed):
+    """(x, y) for every cell of an nx-by-ny grid kept with probability p."""
+    rng = np.random.default_rng(seed)
+    m = rng.random((nx, ny)) < p
+    x, y = np.nonzero(m)
+    R = np.stack([x, y], 1).astype(np.int32)
+    return R[rng.permutation(R.shape[0])]
+
+
+def small_cases():
+    import oracle
+
+    wl = _wl()
+    rng = np.random.default_rng(20240601)
+    cases = {}
+    # gen_rand_data shape (dim3.hpp:1556-1568), scaled down
+    cases["uniform"] = oracle.gen_rand_data(5000, 5000, 1000, 1000, 1000, 492)
+    # Zipf join key
+    cases["zipf"] = (wl.zipf_relation(4000, 300, 400, 1.0, 11), wl.zipf_relation(4000, 300, 400, 1.0, 12))
+    # |R| < |S|: the reference swaps and emits (S.first, R.first)
+    cases["swap"] = (_rand_pairs(rng, 1500, 0, 400, 0, 150), _rand_pairs(rng, 3000, 0, 400, 0, 150))
+    # duplicates, negative values and the key -1
+    cases["dups_neg"] = (_rand_pairs(rng, 3000, -5, 60, -3, 40), _rand_pairs(rng, 2500, -7, 50, -3, 40))
+    # a dense z block the reference's f2 model routes to its bitmap path
+    cases["dense"] = (bernoulli_block(300, 1000, 0.1, 5), bernoulli_block(300, 1000, 0.1, 6))
+    # one shared key: full cross product of the touched x and z
+    cases["cross"] = (np.stack([np.arange(1200), np.full(1200, 7)], 1).astype(np.int32),
+                      np.stack([np.arange(1000) * 3, np.full(1000, 7) + (np.arange(1000) >= 40) * 5], 1)
+                      .astype(np.int32))
+    return cases
+
+
+# medium cases: count + fingerprint of the reference output
+MEDIUM = {
+    "kat": {"gen": "gen_rand_data", "nR": 10_000_000, "nS": 10_000_000, "mod": 1_000_000, "seed": 492,
+            "aggregate": True},
+    "project_scaled": {"gen": "project_small", "n": 100_000, "ac": 10_000, "b": 1_000, "s": 1.0,
+                       "seed_r": 1, "seed_s": 2},
+    "graph_scaled": {"gen": "two_hop_graph", "nv": 100_000, "ne": 400_000, "gamma": 2.1, "dmax": 5_000,
+                     "seed": 7},
+    "dense_block": {"gen": "bernoulli_block", "nx": 2000, "ny": 5000, "p": 0.05, "seed_r": 21, "seed_s": 22},
+    "project_slice": {"gen": "project_small_slice", "n": 1_000_000, "ac": 100_000, "b": 10_000, "s": 1.0,
+                      "seed_r": 1, "seed_s": 2, "a_hi": 2_000},
+    "aggregate_scaled": {"gen": "aggregate_small", "n": 50_000, "ac": 10_000, "b": 5_000, "s": 1.2,
+                         "seed_r": 3, "seed_s": 4, "project": False, "aggregate": True},
+}
+
+
+def make_medium(spec):
+    g = spec["gen"]
+    if g == "gen_rand_data":
+        import oracle
+
+        return oracle.gen_rand_data(spec["nR"], spec["nS"], spec["mod"], spec["mod"], spec["mod"], spec["seed"])
+    wl = _wl()
+    if g == "project_small":
+        return wl.project_small(spec["seed_r"], spec["seed_s"], spec["n"], spec["ac"], spec["b"], spec["s"])
+    if g == "project_small_slice":
+        R, S = wl.project_small(spec["seed_r"], spec["seed_s"], spec["n"], spec["ac"], spec["b"], spec["s"])
+        return R[R[:, 0] < spec["a_hi"]], S
+    if g == "aggregate_small":
+        return wl.aggregate_large(spec["seed_r"], spec["seed_s"], spec["n"], spec["ac"], spec["b"], spec["s"])
+    if g == "two_hop_graph":
+        E = wl.two_hop_graph(spec["nv"], spec["ne"], spec["gamma"], spec["dmax"], spec["seed"])
+        return E, E
+    if g == "bernoulli_block":
+        return (bernoulli_block(spec["nx"], spec["ny"], spec["p"], spec["seed_r"]),
+                bernoulli_block(spec["nx"], spec["ny"], spec["p"], spec["seed_s"]))
+    raise ValueError(g)

@@ -1,0 +1,72 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference
+(oracle/_ref/libdim3ref.so, built by `make -C oracle ref` from the reference
+sources).  Run in the build container:  python tests/golden/make_golden.py
+
+small_*.npz   inputs R, S and the reference's sorted join-project pairs and
+              sorted COUNT rows (DIM3 serial and DIM3JoinAggregate + COUNT).
+medium.json   inputs regenerated from a named seeded generator (with a
+              sha256 of the inputs to detect generator drift), and the count
+              and fingerprint of the reference's output.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from tests.golden import cases  # noqa: E402
+
+
+def sha(R, S):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(R, dtype=np.int32).tobytes())
+    h.update(np.ascontiguousarray(S, dtype=np.int32).tobytes())
+    return h.hexdigest()
+
+
+def small():
+    for name, (R, S) in cases.small_cases().items():
+        p = oracle.ref_join_project(R, S)
+        cp, cc = oracle.ref_join_aggregate_count(R, S)
+        p = oracle.sort_pairs(p)
+        cp, cc = oracle.sort_pairs(cp, cc)
+        np.savez_compressed(os.path.join(HERE, "small_%s.npz" % name), R=R, S=S, project=p, agg_pairs=cp,
+                            agg_counts=cc, swapped=np.array([R.shape[0] < S.shape[0]]))
+        print(name, R.shape[0], S.shape[0], "->", p.shape[0], "pairs,", cp.shape[0], "groups", flush=True)
+
+
+def medium():
+    out = {}
+    for name, spec in cases.MEDIUM.items():
+        R, S = cases.make_medium(spec)
+        rec = {"spec": spec, "sha256": sha(R, S), "n_r": int(R.shape[0]), "n_s": int(S.shape[0])}
+        if spec.get("project", True):
+            p = oracle.ref_join_project(R, S, parallel=True)
+            fp = oracle.fingerprint(p)
+            rec["project"] = {"n": int(p.shape[0]), "fp": [str(v) for v in fp.as_tuple()]}
+            del p
+        if spec.get("aggregate", False):
+            cp, cc = oracle.ref_join_aggregate_count(R, S)
+            fp = oracle.fingerprint(cp, cc)
+            rec["aggregate"] = {"n": int(cp.shape[0]), "fp": [str(v) for v in fp.as_tuple()]}
+            del cp, cc
+        out[name] = rec
+        print(name, json.dumps({k: v for k, v in rec.items() if k != "spec"}), flush=True)
+    with open(os.path.join(HERE, "medium.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    if not oracle.ref_available():
+        sys.exit("reference checker missing: make -C oracle ref")
+    which = sys.argv[1:] or ["small", "medium"]
+    if "small" in which:
+        small()
+    if "medium" in which:
+        medium()

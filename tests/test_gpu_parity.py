@@ -1,0 +1,227 @@
+"""GPU parity: libdim3 (HIP, through the C-ABI) against the reference's
+golden fixtures and the CPU oracle, bit-exact.  Project results are compared
+as sorted (x, z) sets, COUNT results as sorted (x, z, count) rows, large
+results by count + order-independent fingerprint."""
+import glob
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.golden import cases
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_rel(rng, n, amax, bmax, amin=0, bmin=0):
+    return np.stack([rng.integers(amin, amax, n), rng.integers(bmin, bmax, n)], 1).astype(np.int32)
+
+
+def check_project(ctx, R, S, **kw):
+    say("project", R.shape[0], S.shape[0], kw)
+    got = ctx.join_project(R, S, **kw)
+    want, fp, sw = oracle.join_project(R, S, exclude_equal=kw.get("exclude_equal", False))
+    if kw.get("no_swap"):
+        # the oracle always applies the swap rule; with NO_SWAP compare in R,S orientation
+        if R.shape[0] < S.shape[0]:
+            want, fp, sw = oracle.join_project(S, R, exclude_equal=kw.get("exclude_equal", False))
+            want = want[:, ::-1]
+            fp = oracle.fingerprint(want)
+        sw = False
+    assert got.swapped == sw
+    assert got.n == want.shape[0]
+    assert np.array_equal(oracle.sort_pairs(got.pairs), oracle.sort_pairs(want))
+    return got
+
+
+def check_count(ctx, R, S, **kw):
+    say("count", R.shape[0], S.shape[0], kw)
+    got = ctx.join_aggregate_count(R, S, **kw)
+    wp, wc, wfp, sw = oracle.join_aggregate_count(R, S, exclude_equal=kw.get("exclude_equal", False))
+    assert got.swapped == sw
+    gp, gc = oracle.sort_pairs(got.xz, got.counts)
+    ep, ec = oracle.sort_pairs(wp, wc)
+    assert np.array_equal(gp, ep) and np.array_equal(gc, ec)
+    return got
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "small_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_golden_small(ctx, path):
+    d = np.load(path)
+    R, S = d["R"], d["S"]
+    say("golden", os.path.basename(path))
+    got = ctx.join_project(R, S)
+    assert got.swapped == bool(d["swapped"][0])
+    assert np.array_equal(oracle.sort_pairs(got.pairs), d["project"])
+    agg = ctx.join_aggregate_count(R, S)
+    gp, gc = oracle.sort_pairs(agg.xz, agg.counts)
+    assert np.array_equal(gp, d["agg_pairs"]) and np.array_equal(gc, d["agg_counts"])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_small_vs_oracle(ctx, seed):
+    rng = np.random.default_rng(seed)
+    nR, nS = int(rng.integers(1, 5000)), int(rng.integers(1, 5000))
+    amax = int(rng.integers(2, 3000))
+    bmax = int(rng.integers(1, 500))
+    R = rand_rel(rng, nR, amax, bmax, -int(rng.integers(0, 3)), -int(rng.integers(0, 3)))
+    S = rand_rel(rng, nS, amax, bmax, -int(rng.integers(0, 3)), -int(rng.integers(0, 3)))
+    kw = {"exclude_equal": bool(seed % 3 == 0), "block": [0, 256, 512, 1024][seed % 4]}
+    check_project(ctx, R, S, **kw)
+    check_count(ctx, R, S, **kw)
+
+
+@pytest.mark.parametrize("tile_bits", [32, 64, 1024, 4096])
+def test_z_tiling(ctx, tile_bits):
+    rng = np.random.default_rng(tile_bits)
+    R = rand_rel(rng, 20000, 500, 300)
+    S = rand_rel(rng, 20000, 20000, 300)
+    check_project(ctx, R, S, tile_bits=tile_bits)
+    check_count(ctx, R, S, tile_bits=tile_bits)
+    check_project(ctx, R, S, tile_bits=tile_bits, exclude_equal=True)
+
+
+def test_edge_cases(ctx):
+    e = np.zeros((0, 2), np.int32)
+    R = np.array([[1, 2], [3, 4]], np.int32)
+    for a, b in ((e, R), (R, e), (e, e)):
+        assert ctx.join_project(a, b).n == 0
+        assert ctx.join_aggregate_count(a, b).n == 0
+    assert ctx.join_project(R, np.array([[9, 99]], np.int32)).n == 0
+    check_project(ctx, np.array([[5, 1]], np.int32), np.array([[6, 1]], np.int32))
+    # key -1 and extreme values
+    Rm = np.array([[1, -1], [2, -1], [3, 0], [2**31 - 1, -2**31], [-2**31, 2**31 - 1]], np.int32)
+    Sm = np.array([[7, -1], [8, 0], [-1, -2**31], [9, 2**31 - 1]], np.int32)
+    check_project(ctx, Rm, Sm)
+    check_count(ctx, Rm, Sm)
+    # |R| < |S| swaps; NO_SWAP keeps (R.x, S.z)
+    rng = np.random.default_rng(9)
+    Rs, Ss = rand_rel(rng, 100, 50, 10), rand_rel(rng, 400, 50, 10)
+    assert check_project(ctx, Rs, Ss).swapped
+    assert not check_project(ctx, Rs, Ss, no_swap=True).swapped
+    # self pairs dropped for a self-join
+    E = rand_rel(rng, 3000, 400, 400)
+    g = check_project(ctx, E, E, exclude_equal=True)
+    assert not np.any(g.pairs[:, 0] == g.pairs[:, 1])
+
+
+def test_wide_rows_and_overflowing_touch_list(ctx):
+    # rows with far more outputs than the LDS stage holds (several passes)
+    R = np.stack([np.repeat(np.arange(10), 3), np.tile(np.arange(3), 10)], 1).astype(np.int32)
+    S = np.stack([np.arange(15000), np.arange(15000) % 3], 1).astype(np.int32)
+    check_project(ctx, R, S)
+    check_count(ctx, R, S)
+    # a sparse row that touches more words than the first-touch list holds
+    R2 = np.stack([np.zeros(6000), np.arange(6000)], 1).astype(np.int32)
+    S2 = np.stack([(np.arange(6000) * 97) % 400000, np.arange(6000)], 1).astype(np.int32)
+    S2 = np.concatenate([S2, np.stack([np.arange(400000), np.full(400000, 10**6)], 1).astype(np.int32)])
+    R2 = np.concatenate([R2, np.array([[1, 10**6]], np.int32)])
+    check_project(ctx, R2, S2)
+    check_count(ctx, R2, S2)
+
+
+def test_duplicates_count_multiplicity(ctx):
+    rng = np.random.default_rng(4)
+    R = rand_rel(rng, 30000, 50, 40)
+    S = rand_rel(rng, 20000, 60, 40)
+    g = check_count(ctx, R, S)
+    assert int(g.counts.sum()) == oracle.outj(R, S)
+    p = check_project(ctx, R, S)
+    assert p.n == g.n
+
+
+def _medium():
+    with open(os.path.join(GOLDEN, "medium.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("name", ["kat", "project_scaled", "graph_scaled", "dense_block", "project_slice",
+                                  "aggregate_scaled"])
+def test_medium_fingerprints_vs_reference(ctx, name):
+    rec = _medium()[name]
+    say("medium", name)
+    R, S = cases.make_medium(rec["spec"])
+    from tests.golden.make_golden import sha
+
+    assert sha(R, S) == rec["sha256"], "generator drift"
+    if "project" in rec:
+        g = ctx.join_project(R, S, output_mode=1)
+        assert [str(v) for v in g.fp.as_tuple()] == rec["project"]["fp"]
+        if rec["project"]["n"] < 150_000_000:
+            m = ctx.join_project(R, S)
+            assert [str(v) for v in oracle.fingerprint(m.pairs).as_tuple()] == rec["project"]["fp"]
+    if "aggregate" in rec:
+        g = ctx.join_aggregate_count(R, S, output_mode=1)
+        assert [str(v) for v in g.fp.as_tuple()] == rec["aggregate"]["fp"]
+        m = ctx.join_aggregate_count(R, S)
+        assert [str(v) for v in oracle.fingerprint(m.xz, m.counts).as_tuple()] == rec["aggregate"]["fp"]
+
+
+def test_device_api_capacity_and_stream(ctx, dim3):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(11)
+    R = rand_rel(rng, 50000, 5000, 2000)
+    S = rand_rel(rng, 40000, 5000, 2000)
+    want, _, _ = oracle.join_project(R, S)
+    dR = torch.from_numpy(R).cuda()
+    dS = torch.from_numpy(S).cuda()
+    small = torch.empty((1000, 2), dtype=torch.int32, device="cuda")
+    rc, res = ctx.join_project_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], small.data_ptr(), 1000)
+    assert rc == dim3.ERR_CAPACITY and res.n == want.shape[0]
+    out = torch.empty((res.n, 2), dtype=torch.int32, device="cuda")
+    st = torch.cuda.Stream()
+    rc, res2 = ctx.join_project_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], out.data_ptr(), res.n,
+                                    stream=st.cuda_stream)
+    assert rc == dim3.OK and res2.n == res.n
+    assert np.array_equal(oracle.sort_pairs(out.cpu().numpy()), oracle.sort_pairs(want))
+    # COUNT rows on the device
+    wp, wc, _, _ = oracle.join_aggregate_count(R, S)
+    rows = torch.empty((wp.shape[0] * 16,), dtype=torch.uint8, device="cuda")
+    rc, r3 = ctx.join_aggregate_count_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], rows.data_ptr(),
+                                          wp.shape[0])
+    assert rc == dim3.OK
+    arr = np.frombuffer(rows.cpu().numpy().tobytes(), dtype=dim3.COUNT_ROW_DTYPE)
+    gp, gc = oracle.sort_pairs(np.stack([arr["x"], arr["z"]], 1), arr["count"])
+    ep, ec = oracle.sort_pairs(wp, wc)
+    assert np.array_equal(gp, ep) and np.array_equal(gc, ec)
+
+
+def test_fingerprint_mode_equals_materialized(ctx):
+    rng = np.random.default_rng(12)
+    R = rand_rel(rng, 200000, 20000, 3000)
+    S = rand_rel(rng, 150000, 20000, 3000)
+    for ex in (False, True):
+        m = ctx.join_project(R, S, exclude_equal=ex)
+        f = ctx.join_project(R, S, exclude_equal=ex, output_mode=1)
+        assert f.fp.as_tuple() == oracle.fingerprint(m.pairs).as_tuple()
+        ma = ctx.join_aggregate_count(R, S, exclude_equal=ex)
+        fa = ctx.join_aggregate_count(R, S, exclude_equal=ex, output_mode=1)
+        assert fa.fp.as_tuple() == oracle.fingerprint(ma.xz, ma.counts).as_tuple()
+
+
+def test_count_join_paths(ctx):
+    rng = np.random.default_rng(13)
+    R = rand_rel(rng, 100000, 1000, 5000)
+    S = rand_rel(rng, 70000, 1000, 5000)
+    assert ctx.count_join_paths(R, S) == oracle.outj(R, S)
+
+
+def test_repeated_queries_reuse_workspace(ctx):
+    rng = np.random.default_rng(14)
+    R = rand_rel(rng, 100000, 4000, 1000)
+    S = rand_rel(rng, 100000, 4000, 1000)
+    a = ctx.join_project(R, S, output_mode=1).fp.as_tuple()
+    held = ctx.device_bytes()
+    for _ in range(3):
+        assert ctx.join_project(R, S, output_mode=1).fp.as_tuple() == a
+    assert ctx.device_bytes() == held

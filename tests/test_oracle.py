@@ -1,0 +1,207 @@
+"""Pins the CPU oracle: against the reference's golden fixtures, the
+gen_rand_data known answer, brute force on tiny inputs, closed forms and,
+where it is built, the compiled reference itself."""
+import glob
+import json
+import os
+from collections import defaultdict
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.golden import cases
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def brute_project(R, S, exclude_equal=False):
+    byy = defaultdict(list)
+    for z, y in S:
+        byy[int(y)].append(int(z))
+    out = set()
+    for x, y in R:
+        for z in byy.get(int(y), ()):
+            if exclude_equal and int(x) == z:
+                continue
+            out.add((int(x), z))
+    return out
+
+
+def brute_count(R, S):
+    byy = defaultdict(list)
+    for z, y in S:
+        byy[int(y)].append(int(z))
+    out = defaultdict(int)
+    for x, y in R:
+        for z in byy.get(int(y), ()):
+            out[(int(x), z)] += 1
+    return dict(out)
+
+
+def as_set(p):
+    return set(map(tuple, np.asarray(p).tolist()))
+
+
+def rand_rel(rng, n, amax, bmax, amin=0, bmin=0):
+    return np.stack([rng.integers(amin, amax, n), rng.integers(bmin, bmax, n)], 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_project_matches_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    nR, nS = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+    R = rand_rel(rng, nR, 40, 25, -3, -2)
+    S = rand_rel(rng, nS, 40, 25, -3, -2)
+    pairs, fp, sw = oracle.join_project(R, S, exclude_equal=bool(seed % 2))
+    big, small = (R, S) if nR >= nS else (S, R)
+    assert sw == (nR < nS)
+    want = brute_project(big, small, exclude_equal=bool(seed % 2))
+    assert as_set(pairs) == want
+    assert len(pairs) == len(want)  # distinct
+    assert fp.count == len(want)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_count_matches_brute_force(seed):
+    rng = np.random.default_rng(100 + seed)
+    R = rand_rel(rng, int(rng.integers(1, 400)), 30, 20, -2, -1)
+    S = rand_rel(rng, int(rng.integers(1, 400)), 30, 20, -2, -1)
+    p, c, fp, sw = oracle.join_aggregate_count(R, S)
+    big, small = (R, S) if R.shape[0] >= S.shape[0] else (S, R)
+    want = brute_count(big, small)
+    got = {(int(a), int(b)): int(k) for (a, b), k in zip(p.tolist(), c.tolist())}
+    assert got == want
+    # closed forms: sum of counts = OUT_J, groups = |join-project|
+    assert int(c.sum()) == oracle.outj(big, small) == fp.csum
+    assert len(got) == len(oracle.join_project(R, S)[0])
+
+
+def test_oracle_special_cases():
+    e = np.zeros((0, 2), np.int32)
+    R = np.array([[1, 2], [3, 4]], np.int32)
+    assert oracle.join_project(e, R)[0].shape[0] == 0
+    assert oracle.join_project(R, e)[0].shape[0] == 0
+    # no matching key
+    assert oracle.join_project(R, np.array([[9, 99]], np.int32))[0].shape[0] == 0
+    # single shared key: cross product of the touched x and z
+    Rx = np.stack([np.arange(50), np.full(50, 5)], 1).astype(np.int32)
+    Sx = np.stack([np.arange(30) + 100, np.full(30, 5)], 1).astype(np.int32)
+    p = oracle.join_project(Rx, Sx)[0]
+    assert as_set(p) == {(a, b) for a in range(50) for b in range(100, 130)}
+    # the key -1 is an ordinary key (LS_mapper empty marker is internal)
+    Rm = np.array([[1, -1], [2, -1], [3, 0]], np.int32)
+    Sm = np.array([[7, -1], [8, 0]], np.int32)
+    assert as_set(oracle.join_project(Rm, Sm)[0]) == {(1, 7), (2, 7), (3, 8)}
+
+
+def test_fingerprint_is_order_independent_and_sensitive():
+    rng = np.random.default_rng(3)
+    p = rand_rel(rng, 1000, 1 << 30, 1 << 30)
+    a = oracle.fingerprint(p).as_tuple()
+    b = oracle.fingerprint(p[rng.permutation(1000)]).as_tuple()
+    assert a == b
+    q = p.copy()
+    q[17, 1] += 1
+    assert oracle.fingerprint(q).as_tuple()[1:3] != a[1:3]
+    c = np.ones(1000, np.int64)
+    c2 = c.copy()
+    c2[5] = 2
+    assert oracle.fingerprint(p, c).as_tuple() != oracle.fingerprint(p, c2).as_tuple()
+
+
+def test_mix64_known_values():
+    # splitmix64 finaliser (Steele, Lea, Flood 2014): the outputs of the
+    # splitmix64 generator seeded with 0 are mix64(k * 0x9e3779b97f4a7c15), k = 1, 2, 3
+    golden = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    for k, g in enumerate(golden, start=1):
+        assert oracle.mix64((k * 0x9E3779B97F4A7C15) & ((1 << 64) - 1)) == g
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "small_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_oracle_matches_reference_golden(path):
+    d = np.load(path)
+    R, S = d["R"], d["S"]
+    p, fp, sw = oracle.join_project(R, S)
+    assert sw == bool(d["swapped"][0])
+    assert np.array_equal(oracle.sort_pairs(p), d["project"])
+    cp, cc, _, _ = oracle.join_aggregate_count(R, S)
+    gp, gc = oracle.sort_pairs(cp, cc)
+    assert np.array_equal(gp, d["agg_pairs"]) and np.array_equal(gc, d["agg_counts"])
+
+
+def _medium():
+    with open(os.path.join(GOLDEN, "medium.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("name", ["project_scaled", "dense_block", "aggregate_scaled"])
+def test_oracle_matches_reference_medium(name):
+    rec = _medium()[name]
+    R, S = cases.make_medium(rec["spec"])
+    from tests.golden.make_golden import sha
+
+    assert sha(R, S) == rec["sha256"], "generator drift"
+    if "project" in rec:
+        _, fp, _ = oracle.join_project(R, S, fingerprint_only=True)
+        assert [str(v) for v in fp.as_tuple()] == rec["project"]["fp"]
+    if "aggregate" in rec:
+        _, _, fp, _ = oracle.join_aggregate_count(R, S, fingerprint_only=True)
+        assert [str(v) for v in fp.as_tuple()] == rec["aggregate"]["fp"]
+
+
+@pytest.mark.slow
+def test_oracle_kat_gen_rand_data():
+    """|Result| = 98,931,829 for gen_rand_data defaults (main.cpp, dim3.hpp:1556-1568)."""
+    rec = _medium()["kat"]
+    R, S = oracle.gen_rand_data()
+    from tests.golden.make_golden import sha
+
+    assert sha(R, S) == rec["sha256"]
+    _, fp, _ = oracle.join_project(R, S, fingerprint_only=True)
+    assert fp.count == 98_931_829
+    assert [str(v) for v in fp.as_tuple()] == rec["project"]["fp"]
+
+
+def test_gen_rand_data_prefix():
+    # minstd_rand0 from seed 492: x_1 = 492*16807 mod (2^31-1) = 8269044
+    R, S = oracle.gen_rand_data(2, 1, 1_000_000, 1_000_000, 1_000_000, 492)
+    m = 2147483647
+    s = 492
+    seq = []
+    for _ in range(6):
+        s = s * 16807 % m
+        seq.append(s % 1_000_000)
+    assert R.tolist() == [[seq[0], seq[1]], [seq[2], seq[3]]]
+    assert S.tolist() == [[seq[4], seq[5]]]
+
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference checker not built")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(4))
+def test_oracle_matches_compiled_reference(seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(2000, 30000))
+    R = rand_rel(rng, n, 3000, 2000)
+    S = rand_rel(rng, int(rng.integers(2000, 30000)), 3000, 2000)
+    p = oracle.join_project(R, S)[0]
+    q = oracle.ref_join_project(R, S, parallel=bool(seed % 2))
+    assert np.array_equal(oracle.sort_pairs(p), oracle.sort_pairs(q))
+    cp, cc, _, _ = oracle.join_aggregate_count(R, S)
+    rp, rc = oracle.ref_join_aggregate_count(R, S)
+    a, b = oracle.sort_pairs(cp, cc), oracle.sort_pairs(rp, rc)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@needs_ref
+def test_oracle_estimate_and_generator_match_reference():
+    R, S = oracle.gen_rand_data(300_000, 200_000, 50_000, 20_000, 50_000, 77)
+    R2, S2 = oracle.ref_gen_rand_data(300_000, 200_000, 50_000, 20_000, 50_000, 77)
+    assert np.array_equal(R, R2) and np.array_equal(S, S2)
+    assert oracle.estimate(R, S) == oracle.ref_estimate(R, S)
+    # |S| <= 10000: the sample keeps every key and the estimate is exact
+    R3, S3 = oracle.gen_rand_data(20_000, 8_000, 3_000, 500, 3_000, 5)
+    assert oracle.estimate(R3, S3) == oracle.ref_estimate(R3, S3) == oracle.outj(R3, S3)
