@@ -1,0 +1,259 @@
+// dim3.hpp — drop-in C++ front end of libdim3 for callers of the reference.
+//
+// Re-exposes the reference's query entry points with the same names,
+// template parameters and argument meaning, forwarding to the C-ABI in
+// dim3.h (hand-written HIP kernels for gfx950):
+//
+//   DIM3<_Tx,_Ty,_Tz,_Hx,_Hy,_Hz,_Hxz,_Tcounter>::doJoinProject          dim3.hpp:982-1055
+//   DIM3_parallel<...>::doJoinProject_parallel                           dim3_parallel.hpp:137-209
+//   DIM3JoinAggregate<_Tx,_Ty,_Tz,_Tv,_Agg,...>::doJoinAggregate         dim3.hpp:1313-1373
+//   myvector<T,_CountType> (same layout: data, cur_num, max_num)         Include/myvector.hpp:9-86
+//   TUPLE<_Tx,_Ty,_Tv> (same layout: x, y, v)                            dim3.hpp:20-33
+//   myHasher (placeholder: hashing happens on the device)                Include/structures.hpp:92
+//   gen_rand_data (minstd_rand0, same draws)                             dim3.hpp:1556-1591
+//
+// Conventions kept: inputs by value (shallow), result cleared then filled
+// with malloc-compatible storage the caller frees, |R| < |S| swaps the roles
+// and rows come out as (S.first, R.first), duplicates count with
+// multiplicity in COUNT.  Errors print a message and exit(-1), as the
+// reference does on allocation failure (e.g. dim3.hpp:303-307); the
+// library's own C-ABI returns status codes instead.  A result that does not
+// fit _Tcounter is an error here, where the reference's uint32 counters
+// would wrap (Include/myvector.hpp:33-46).
+//
+// Supported instantiations: 4-byte integer _Tx/_Ty/_Tz (int, as every
+// reference driver uses); join-aggregate with the COUNT policy
+// AggregateFunctionCount<_Tv>.  Device: DIM3_DEVICE env var, default 0.
+#pragma once
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <random>
+#include <type_traits>
+#include <utility>
+
+#include "dim3.h"
+
+using std::pair;
+
+template <class T, typename _CountType = uint32_t>
+struct myvector {
+    T* data;
+    _CountType cur_num;
+    _CountType max_num;
+
+    myvector(_CountType ini_size = 8) { init(ini_size); }
+    void init(_CountType ini_size = 8) {
+        max_num = ini_size;
+        data = (T*)malloc(sizeof(T) * (max_num ? max_num : 1));
+        cur_num = 0;
+    }
+    inline bool isEmpty() { return cur_num == 0; }
+    void push_back(T x) {
+        if (cur_num == max_num) {
+            _CountType nm = max_num ? max_num * 2 : 8;
+            T* p = (T*)realloc(data, sizeof(T) * nm);
+            if (!p) {
+                printf("Failed when resize myvector to %llu\n", (unsigned long long)nm);
+                free(data);
+                exit(1);
+            }
+            data = p;
+            max_num = nm;
+        }
+        data[cur_num++] = x;
+    }
+    void fixSize(_CountType size) {
+        if (size == 0) return;
+        reserve(size);
+        cur_num = size;
+    }
+    void reserve(_CountType size) {
+        if (size < max_num) return;
+        T* p = (T*)realloc(data, sizeof(T) * size);
+        if (!p) {
+            printf("Failed when reserve myvector to %llu\n", (unsigned long long)size);
+            free(data);
+            exit(1);
+        }
+        data = p;
+        max_num = size;
+    }
+    inline _CountType size() const { return cur_num; }
+    inline void clear() { cur_num = 0; }
+};
+
+template <typename _Tx, typename _Ty, typename _Tv>
+struct TUPLE {
+    _Tx x;
+    _Ty y;
+    _Tv v;
+    TUPLE(_Tx x_, _Ty y_, _Tv v_) : x(x_), y(y_), v(v_) {}
+};
+
+struct myHasher {};
+
+// COUNT(*) policy for DIM3JoinAggregate (the reference ships only AVG,
+// dim3.hpp:1453-1480; its policy concept is ctor(int)/insert/update/get).
+template <typename _Tv>
+struct AggregateFunctionCount {
+    typedef _Tv value_type;
+};
+
+namespace dim3_shim {
+
+inline dim3_ctx* context() {
+    static dim3_ctx* ctx = nullptr;
+    if (!ctx) {
+        const char* d = getenv("DIM3_DEVICE");
+        int rc = dim3_ctx_create(d ? atoi(d) : 0, 0, &ctx);
+        if (rc != DIM3_OK) {
+            printf("Fail to create the DIM3 device context: %s.\n", dim3_status_string(rc));
+            exit(-1);
+        }
+    }
+    return ctx;
+}
+
+[[noreturn]] inline void die(const char* what, int rc) {
+    printf("%s failed: %s (%s).\n", what, dim3_status_string(rc), dim3_last_error(context()));
+    exit(-1);
+}
+
+template <typename T>
+constexpr bool is_i32() {
+    return std::is_integral<T>::value && sizeof(T) == 4;
+}
+
+template <typename _Tx, typename _Ty, typename _Tz, typename _Tcounter>
+void join_project(const myvector<pair<_Tx, _Ty>, _Tcounter>& R, const myvector<pair<_Tz, _Ty>, _Tcounter>& S,
+                  myvector<pair<_Tx, _Tz>, _Tcounter>& result) {
+    static_assert(is_i32<_Tx>() && is_i32<_Ty>() && is_i32<_Tz>(), "libdim3 joins 4-byte integer attributes");
+    static_assert(sizeof(pair<_Tx, _Ty>) == 8, "pair layout");
+    result.clear();
+    dim3_opts o;
+    dim3_opts_default(&o);
+    dim3_result res;
+    int rc = dim3_join_project(context(), (const int32_t*)R.data, R.size(), (const int32_t*)S.data, S.size(), &o,
+                               &res);
+    if (rc != DIM3_OK) die("DIM3 doJoinProject", rc);
+    if (res.n > (uint64_t)(_Tcounter)~(_Tcounter)0) {
+        dim3_result_free(&res);
+        printf("DIM3 doJoinProject: %llu rows do not fit the result counter type.\n", (unsigned long long)res.n);
+        exit(-1);
+    }
+    if (res.n) {
+        free(result.data);
+        result.data = (pair<_Tx, _Tz>*)res.pairs;  // malloc'd by the library, caller frees
+        res.pairs = nullptr;
+        result.cur_num = result.max_num = (_Tcounter)res.n;
+    }
+    dim3_result_free(&res);
+}
+
+}  // namespace dim3_shim
+
+template <typename _Tx, typename _Ty, typename _Tz, typename _Hx = myHasher, typename _Hy = myHasher,
+          typename _Hz = myHasher, typename _Hxz = myHasher, typename _Tcounter = uint32_t>
+struct DIM3 {
+    static void doJoinProject(const myvector<pair<_Tx, _Ty>, _Tcounter> _R,
+                              const myvector<pair<_Tz, _Ty>, _Tcounter> _S,
+                              myvector<pair<_Tx, _Tz>, _Tcounter>& result) {
+        dim3_shim::join_project(_R, _S, result);
+    }
+};
+
+template <typename _Tx, typename _Ty, typename _Tz, typename _Hx = myHasher, typename _Hy = myHasher,
+          typename _Hz = myHasher, typename _Hxz = myHasher, typename _Tcounter = uint32_t>
+struct DIM3_parallel : public DIM3<_Tx, _Ty, _Tz, _Hx, _Hy, _Hz, _Hxz, _Tcounter> {
+    static void doJoinProject_parallel(const myvector<pair<_Tx, _Ty>, _Tcounter> _R,
+                                       const myvector<pair<_Tz, _Ty>, _Tcounter> _S,
+                                       myvector<pair<_Tx, _Tz>, _Tcounter>& result) {
+        dim3_shim::join_project(_R, _S, result);
+    }
+};
+
+template <typename _Tx, typename _Ty, typename _Tz, typename _Tv, typename _Aggregate_type,
+          typename _Hx = myHasher, typename _Hy = myHasher, typename _Hz = myHasher, typename _Hxz = myHasher,
+          typename _Tcounter = uint32_t>
+struct DIM3JoinAggregate {
+    static void doJoinAggregate(myvector<TUPLE<_Tx, _Ty, _Tv>, _Tcounter>& _R,
+                                myvector<TUPLE<_Tz, _Ty, _Tv>, _Tcounter>& _S,
+                                myvector<TUPLE<_Tx, _Tz, _Tv>, _Tcounter>& result) {
+        static_assert(std::is_same<_Aggregate_type, AggregateFunctionCount<_Tv>>::value,
+                      "libdim3 implements the COUNT aggregate (AggregateFunctionCount<_Tv>)");
+        static_assert(dim3_shim::is_i32<_Tx>() && dim3_shim::is_i32<_Ty>() && dim3_shim::is_i32<_Tz>(),
+                      "libdim3 joins 4-byte integer attributes");
+        result.clear();
+        int32_t* r = (int32_t*)malloc(sizeof(int32_t) * 2 * (_R.size() ? _R.size() : 1));
+        int32_t* s = (int32_t*)malloc(sizeof(int32_t) * 2 * (_S.size() ? _S.size() : 1));
+        if (!r || !s) {
+            printf("Fail to malloc the packed relations in doJoinAggregate.\n");
+            exit(-1);
+        }
+        for (_Tcounter i = 0; i < _R.size(); i++) {
+            r[2 * i] = (int32_t)_R.data[i].x;
+            r[2 * i + 1] = (int32_t)_R.data[i].y;
+        }
+        for (_Tcounter i = 0; i < _S.size(); i++) {
+            s[2 * i] = (int32_t)_S.data[i].x;
+            s[2 * i + 1] = (int32_t)_S.data[i].y;
+        }
+        dim3_opts o;
+        dim3_opts_default(&o);
+        dim3_result res;
+        int rc = dim3_join_aggregate_count(dim3_shim::context(), r, _R.size(), s, _S.size(), &o, &res);
+        free(r);
+        free(s);
+        if (rc != DIM3_OK) dim3_shim::die("DIM3JoinAggregate doJoinAggregate", rc);
+        if (res.n > (uint64_t)(_Tcounter)~(_Tcounter)0) {
+            printf("DIM3JoinAggregate: %llu rows do not fit the result counter type.\n", (unsigned long long)res.n);
+            exit(-1);
+        }
+        result.reserve((_Tcounter)(res.n ? res.n : 1));
+        for (uint64_t i = 0; i < res.n; i++)
+            result.data[i] = TUPLE<_Tx, _Tz, _Tv>((_Tx)res.rows[i].x, (_Tz)res.rows[i].z, (_Tv)res.rows[i].count);
+        result.cur_num = (_Tcounter)res.n;
+        dim3_result_free(&res);
+    }
+};
+
+// gen_rand_data (dim3.hpp:1556-1568): std::minstd_rand0 (the libstdc++
+// default_random_engine), R gets (x, y) draws then S gets (z, y) draws.
+inline void gen_rand_data(myvector<pair<int, int>>& R, myvector<pair<int, int>>& S, int nR = 1e7, int nS = 1e7,
+                          int MODX = 1e6, int MODY = 1e6, int MODZ = 1e6, int seed = 492) {
+    R.clear();
+    S.clear();
+    std::minstd_rand0 e(seed);
+    for (int i = 0; i < nR; i++) {
+        int x = (int)(e() % MODX);
+        int y = (int)(e() % MODY);
+        R.push_back(pair<int, int>(x, y));
+    }
+    for (int i = 0; i < nS; i++) {
+        int z = (int)(e() % MODZ);
+        int y = (int)(e() % MODY);
+        S.push_back(pair<int, int>(z, y));
+    }
+}
+
+inline void gen_rand_data(myvector<TUPLE<int, int, long long>>& R, myvector<TUPLE<int, int, long long>>& S,
+                          int nR = 1e7, int nS = 1e7, int MODX = 1e6, int MODY = 1e6, int MODZ = 1e6,
+                          int seed = 492) {
+    R.clear();
+    S.clear();
+    std::minstd_rand0 e(seed);
+    for (int i = 0; i < nR; i++) {
+        int x = (int)(e() % MODX);
+        int y = (int)(e() % MODY);
+        R.push_back(TUPLE<int, int, long long>(x, y, i));
+    }
+    for (int i = 0; i < nS; i++) {
+        int z = (int)(e() % MODZ);
+        int y = (int)(e() % MODY);
+        S.push_back(TUPLE<int, int, long long>(z, y, i));
+    }
+}

@@ -87,6 +87,14 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError("libdim3.so is not built (%s); run __graft_entry__.build() or "
                           "make -C joinproject-dim3_amd" % LIB_PATH)
+    # PyTorch-ROCm ships its own HIP runtime (file libamdhip64.so, soname
+    # libamdhip64.so.7).  Loaded first, it satisfies libdim3's dependency on
+    # that soname and the process keeps one HIP runtime; loaded second, it
+    # would bring a second HSA runtime that cannot open the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
     L.dim3_ctx_create.argtypes = [i32, u64, ctypes.POINTER(vp)]
