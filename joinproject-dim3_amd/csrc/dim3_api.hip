@@ -71,12 +71,6 @@ inline uint32_t pow2_at_least(uint64_t v) {
     return (uint32_t)c;
 }
 
-inline uint32_t ceil_log2(uint64_t v) {
-    uint32_t s = 0;
-    while ((1ull << s) < v) s++;
-    return s;
-}
-
 }  // namespace
 
 struct dim3_ctx {
@@ -105,7 +99,7 @@ struct dim3_ctx {
         uint32_t nx = 0, ny = 0, nz = 0;
         uint32_t nnzR = 0, nnzS = 0;
         uint64_t out_j = 0;
-        uint32_t ntiles = 1, tile_shift = 0, tile_words = 0;
+        uint32_t ntiles = 1, tile_size = 0, tile_words = 0;
         uint32_t touched_cap = 0, stage_cap = 0;
         int block = 256;
         float ms_map = 0, ms_csr = 0;
@@ -189,40 +183,57 @@ int ws(dim3_ctx* c, SlotId id, uint64_t n, T** out) {
 void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
     auto& P = c->plan;
     const bool cnt = P.count_mode;
+    const uint32_t per_word = cnt ? 1u : 32u;  // z IDs per LDS word
     P.touched_cap = cnt ? 4096u : 2048u;
-    P.stage_cap = cnt ? 512u : 1024u;
-    const size_t extra = (size_t)P.touched_cap * 4u + (size_t)P.stage_cap * (cnt ? 16u : 8u) + 64u;
-    const size_t lds_cap = c->max_lds;
-    // entries of the tile that fit in LDS (bits for project, u32 counters for COUNT)
-    const uint64_t fit_units = (lds_cap - extra) / 4u * (cnt ? 1u : 32u);
-    uint32_t shift;
-    if (o.tile_bits > 0) {
-        shift = ceil_log2((uint64_t)o.tile_bits);
-    } else if ((uint64_t)P.nz <= fit_units) {
-        shift = ceil_log2(std::max<uint64_t>(P.nz, 32));
-    } else {
-        shift = 0;
-        while ((2ull << shift) <= fit_units) shift++;
+    const uint64_t nz_words = ((uint64_t)P.nz + per_word - 1) / per_word;
+    const uint64_t lds_words = c->max_lds / 4u - 256u;  // static shared + slack
+    struct Cand {
+        int block;
+        uint32_t ntiles, tile_words, stage;
+        bool ok;
+    };
+    // candidates: 1, 2 or 4 workgroups per CU (16 wavefronts per CU in all)
+    auto make = [&](int wgs, int block, uint64_t forced_units) {
+        Cand k{block, 1, 0, 0, false};
+        const uint64_t budget = lds_words / (uint64_t)wgs;
+        const uint64_t min_stage = std::max<uint64_t>(1024u, 3u * (uint64_t)block + 1u);
+        if (budget <= P.touched_cap + min_stage + 1) return k;
+        uint64_t max_words = budget - P.touched_cap - min_stage;
+        uint64_t tw;
+        if (forced_units) {
+            tw = (forced_units + per_word - 1) / per_word;
+        } else {
+            const uint64_t nt = (nz_words + max_words - 1) / max_words;
+            tw = (nz_words + nt - 1) / nt;
+        }
+        if (tw == 0) tw = 1;
+        if (tw > max_words) return k;
+        k.tile_words = (uint32_t)tw;
+        k.ntiles = (uint32_t)(((uint64_t)P.nz + tw * per_word - 1) / (tw * per_word));
+        if (k.ntiles == 0) k.ntiles = 1;
+        k.stage = (uint32_t)(budget - P.touched_cap - tw);
+        k.ok = true;
+        return k;
+    };
+    const uint64_t forced = o.tile_bits > 0 ? (uint64_t)o.tile_bits : 0;
+    Cand best{0, 0, 0, 0, false};
+    const int wg_opts[3] = {4, 2, 1};
+    const int blk_opts[3] = {256, 512, 1024};
+    for (int i = 0; i < 3; i++) {
+        const int block = o.block ? o.block : blk_opts[i];
+        Cand k = make(wg_opts[i], block, forced);
+        if (!k.ok) continue;
+        // fewest tiles first; among equal tile counts the most workgroups per CU
+        if (!best.ok || k.ntiles < best.ntiles) best = k;
     }
-    if (!cnt && shift < 5) shift = 5;
-    if (shift > 30) shift = 30;
-    const uint64_t tile = 1ull << shift;
-    P.tile_shift = shift;
-    P.ntiles = (uint32_t)((P.nz + tile - 1) / tile);
-    if (P.ntiles == 0) P.ntiles = 1;
-    const uint64_t units = P.ntiles == 1 ? std::max<uint64_t>(P.nz, 1) : tile;
-    P.tile_words = (uint32_t)(cnt ? units : (units + 31) / 32);
-    const size_t lds = cnt ? dim3h::expand_count_lds(P.tile_words, P.touched_cap, P.stage_cap)
-                           : dim3h::expand_project_lds(P.tile_words, P.touched_cap, P.stage_cap);
-    if (o.block == 256 || o.block == 512 || o.block == 1024) {
-        P.block = o.block;
-    } else if (lds > 80 * 1024) {
-        P.block = 1024;
-    } else if (lds > 40 * 1024) {
-        P.block = 512;
-    } else {
-        P.block = 256;
+    if (!best.ok) {  // forced tile too big for the candidates: one workgroup per CU
+        best = make(1, o.block ? o.block : 1024, 0);
     }
+    P.block = best.block;
+    P.ntiles = best.ntiles;
+    P.tile_words = best.tile_words;
+    P.tile_size = best.tile_words * per_word;
+    P.stage_cap = best.stage;
 }
 
 // Mapping + CSR of one query.  Leaves the plan in c->plan.
@@ -325,7 +336,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     uint32_t *rowR, *curR, *cntS, *rowS, *curS, *keyS;
     const uint64_t nkeyS = (uint64_t)P.ny * P.ntiles;
     TRY(ws(c, WS_ID2X, P.nx, &id2x));
-    TRY(ws(c, WS_ID2Z, (uint64_t)P.ntiles << P.tile_shift, &id2z));
+    TRY(ws(c, WS_ID2Z, (uint64_t)P.ntiles * P.tile_size, &id2z));
     TRY(ws(c, WS_ROWPTR_R, (uint64_t)P.nx + 1, &rowR));
     TRY(ws(c, WS_CUR_R, P.nx, &curR));
     TRY(ws(c, WS_COL_R, P.nnzR, &colR));
@@ -344,7 +355,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
     dim3h::scatter_r(slotR, slotRx, nR, exY, exX, curR, colR, st);
     dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
-    dim3h::count_s(slotS, slotSz, nS, exY, exZ, P.ntiles, P.tile_shift, cntS, keyS, zidS, st);
+    dim3h::count_s(slotS, slotSz, nS, exY, exZ, P.ntiles, P.tile_size, cntS, keyS, zidS, st);
     dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
     dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
     HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
@@ -377,7 +388,7 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
     a.x2z = (P.flags & DIM3_OPT_EXCLUDE_EQUAL) ? (const int32_t*)c->ws[WS_X2Z].p : nullptr;
     a.nx = P.nx;
     a.ntiles = P.ntiles;
-    a.tile_shift = P.tile_shift;
+    a.tile_size = P.tile_size;
     a.tile_words = P.tile_words;
     a.touched_cap = P.touched_cap;
     a.stage_cap = P.stage_cap;
@@ -415,7 +426,7 @@ void fill_result(dim3_ctx* c, dim3_result* res, bool fp_mode) {
     st.out_j = P.out_j;
     st.paths_sparse = P.empty ? 0 : s.paths;
     st.dense_mode = DIM3_DENSE_OFF;
-    st.tile_bits = P.empty ? 0 : (1u << P.tile_shift);
+    st.tile_bits = P.empty ? 0 : P.tile_size;
     st.n_tiles = P.ntiles;
     st.block = (uint32_t)P.block;
     st.ms_map = P.ms_map;

@@ -216,7 +216,7 @@ void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const 
                const uint64_t* ex_x, uint32_t* cursor, int32_t* col, hipStream_t st);
 // tiled CSR of S by (y, z-tile)
 void count_s(const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y, const uint64_t* ex_z,
-             uint32_t ntiles, uint32_t tile_shift, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st);
+             uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st);
 void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, int32_t* col,
@@ -236,8 +236,8 @@ struct ExpandArgs {
     const int32_t* x2z;  // nullable: z ID of each x value (exclude_equal)
     uint32_t nx;
     uint32_t ntiles;
-    uint32_t tile_shift;
-    uint32_t tile_words;  // u32 words (project) or counters (aggregate) held in LDS
+    uint32_t tile_size;   // z IDs per tile (bits for project, counters for COUNT)
+    uint32_t tile_words;  // u32 words of the tile in LDS
     uint32_t touched_cap;
     uint32_t stage_cap;
     uint32_t fp_mode;
@@ -249,7 +249,6 @@ struct ExpandArgs {
 // returns 0 or a hipError_t
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
-size_t expand_project_lds(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap);
-size_t expand_count_lds(uint32_t tile_entries, uint32_t touched_cap, uint32_t stage_cap);
+size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block);
 
 }  // namespace dim3h

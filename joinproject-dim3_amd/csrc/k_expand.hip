@@ -5,22 +5,25 @@
 // (dim3.hpp:952-973), its OpenMP copy (dim3_parallel.hpp:89-117) and
 // SpmmAggregate (dim3.hpp:1267-1311).
 //
-// The reference keeps a "last writer" int per z (SPAw) in DRAM-sized arrays
-// and pushes each first touch into a growable vector.  Here one workgroup owns
-// one (x, z-tile) task at a time, taken from a device work counter:
-//   - the tile of the row is a bitmap (project) or u32 counters (COUNT) in
-//     LDS; each join path (x, y, z) is one ds_or / ds_add with return,
-//   - the first touch of a word (entry) is recorded in an LDS list with one
-//     LDS atomic per wavefront (ballot + mbcnt), so emission visits only the
-//     touched words of sparse rows and falls back to a full sweep when the
-//     list overflows,
-//   - the row's rows are counted, one global atomic reserves their output
-//     range, and they are staged in LDS and written with contiguous,
-//     coalesced stores (or folded into the fingerprint),
-//   - the words are cleared on the way out, so the tile is zero for the next
-//     task without a separate sweep.
-// Work per join path: one 4-byte column read (mostly L2/MALL hits: S rows of
-// heavy keys are re-read by every x that reaches them) and one LDS atomic.
+// The reference keeps a "last writer" int per z (SPAw) and pushes every first
+// touch into a growable vector.  Here a persistent workgroup takes one
+// (x, z-tile) task at a time from a device work counter:
+//   - the z tile of the row lives in LDS as a bitmap (project) or as u32
+//     counters (COUNT); one join path (x, y, z) is one ds_or / ds_add with
+//     return;
+//   - the row's S segments are cut into pieces of 256 paths, the pieces are
+//     dealt to the wavefronts round robin (so a hub segment is spread over
+//     the whole workgroup) and each lane keeps four column loads in flight;
+//   - the first touch of a word / counter is recorded in an LDS list with one
+//     LDS atomic per wavefront (ballot + mbcnt), so sparse rows emit from the
+//     touched words only; when the list overflows the row sweeps its tile;
+//   - emission counts the row, reserves its output range with one global
+//     atomic, stages the row's local z offsets in LDS and writes (x, z) rows
+//     with coalesced stores, looking z values up in the same pass; the tile
+//     is cleared on the way out.
+// Per join path: one 4-byte column read (mostly L2/MALL hits, as heavy S rows
+// are re-read by every x that reaches them) and one LDS atomic.  Per output
+// row: one 4-byte id->value read and one 8-byte (16-byte for COUNT) store.
 #include "dim3_internal.h"
 
 namespace {
@@ -28,7 +31,10 @@ namespace {
 using namespace dim3d;
 using dim3h::ExpandArgs;
 
-__device__ __forceinline__ uint32_t word_at(const uint32_t* touched, bool listed, uint32_t i) {
+constexpr uint32_t kUnroll = 4;
+constexpr uint32_t kPiece = 64 * kUnroll;  // join paths per wavefront work item
+
+__device__ __forceinline__ uint32_t list_at(const uint32_t* touched, bool listed, uint32_t i) {
     return listed ? touched[i] : i;
 }
 
@@ -62,164 +68,48 @@ __device__ __forceinline__ void push_first_touch(bool first, uint32_t item, uint
     }
 }
 
-// ---------------------------------------------------------------------------
-// join-project: bitmap tile
-// LDS: bits[tile_words] | touched[touched_cap] | (pad) | stage[stage_cap] int2
-// ---------------------------------------------------------------------------
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_expand_project(ExpandArgs a) {
-    extern __shared__ __align__(16) uint32_t smem[];
-    constexpr uint32_t NW = BLOCK / 64;
-    uint32_t* bits = smem;
-    uint32_t* touched = smem + a.tile_words;
-    int2* stage = reinterpret_cast<int2*>(smem + ((a.tile_words + a.touched_cap + 1u) & ~1u));
-    __shared__ uint64_t sh[NW + 1];
-    __shared__ uint32_t s_task, s_nt;
-    __shared__ uint64_t s_gbase;
-
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint32_t ntasks = a.nx * a.ntiles;
-    const uint32_t tile_size = 1u << a.tile_shift;
-    int2* out = reinterpret_cast<int2*>(a.out);
-
-    for (uint32_t w = tid; w < a.tile_words; w += BLOCK) bits[w] = 0;
-    uint64_t fsum = 0, fxr = 0, npaths = 0;
-
-    for (;;) {
-        if (tid == 0) {
-            s_task = atomicAdd(&a.sc->task, 1u);
-            s_nt = 0;
-        }
-        __syncthreads();
-        const uint32_t task = s_task;
-        if (task >= ntasks) break;
-        const uint32_t x = task / a.ntiles;
-        const uint32_t t = task - x * a.ntiles;
-        const uint32_t zbase = t << a.tile_shift;
-        const uint32_t rb = a.rowptrR[x], re = a.rowptrR[x + 1];
-
-        // expansion: wave w takes the y's of the row round robin, lanes walk S[y]
-        for (uint32_t i = rb + wid; i < re; i += NW) {
-            const uint32_t key = (uint32_t)a.colR[i] * a.ntiles + t;
-            const uint32_t sb = a.rowptrS[key], se = a.rowptrS[key + 1];
-            npaths += (lane == 0) ? (uint64_t)(se - sb) : 0ull;
-            for (uint32_t j = sb + lane; j < se; j += 64) {
-                const uint32_t zl = (uint32_t)a.colS[j] - zbase;
-                const uint32_t old = atomicOr(&bits[zl >> 5], 1u << (zl & 31u));
-                push_first_touch(old == 0u, zl >> 5, &s_nt, touched, a.touched_cap);
-            }
-        }
-        __syncthreads();
-        if (a.x2z != nullptr && tid == 0) {  // u != w filter of the self-join
-            const int32_t zs = a.x2z[x];
-            if (zs >= 0) {
-                const uint32_t zl = (uint32_t)zs - zbase;
-                if (zl < tile_size && (zl >> 5) < a.tile_words) bits[zl >> 5] &= ~(1u << (zl & 31u));
-            }
-        }
-        __syncthreads();
-
-        // emission: each thread owns a contiguous slice of the word list
-        const uint32_t nt = s_nt;
-        const bool listed = nt <= a.touched_cap;
-        const uint32_t count = listed ? nt : a.tile_words;
-        const uint32_t per = (count + BLOCK - 1) / BLOCK;
-        const uint32_t lo = min(tid * per, count), hi = min(lo + per, count);
-        uint32_t c = 0;
-        for (uint32_t i = lo; i < hi; i++) c += __popc(bits[word_at(touched, listed, i)]);
-        uint64_t T;
-        const uint64_t o = block_exclusive_scan<BLOCK, uint64_t>((uint64_t)c, T, sh);
-        const int32_t xv = a.id2x[x];
-        const int32_t* id2z = a.id2z + zbase;
-
-        if (a.fp_mode) {
-            for (uint32_t i = lo; i < hi; i++) {
-                const uint32_t w = word_at(touched, listed, i);
-                uint32_t v = bits[w];
-                bits[w] = 0;
-                while (v) {
-                    const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                    v &= v - 1u;
-                    const uint64_t h = mix64(row_key(xv, id2z[w * 32u + b]));
-                    fsum += h;
-                    fxr ^= h;
-                }
-            }
-            if (tid == 0 && T) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
-            __syncthreads();
-            continue;
-        }
-
-        if (tid == 0) s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
-        __syncthreads();
-        const uint64_t gbase = s_gbase;
-        uint32_t ci = lo, cw = 0, cv = 0;
-        if (ci < hi) {
-            cw = word_at(touched, listed, ci);
-            cv = bits[cw];
-        }
-        uint64_t r = o;
-        const uint64_t rend = o + c;
-        for (uint64_t q = 0; q < T; q += a.stage_cap) {
-            const uint64_t qe = min(q + (uint64_t)a.stage_cap, T);
-            while (r < qe && r < rend) {
-                while (cv == 0u) {
-                    ++ci;
-                    cw = word_at(touched, listed, ci);
-                    cv = bits[cw];
-                }
-                const uint32_t b = (uint32_t)__ffs(cv) - 1u;
-                cv &= cv - 1u;
-                stage[r - q] = make_int2(xv, id2z[cw * 32u + b]);
-                ++r;
-            }
-            __syncthreads();
-            const uint32_t nq = (uint32_t)(qe - q);
-            for (uint32_t k = tid; k < nq; k += BLOCK) {
-                const uint64_t g = gbase + q + k;
-                if (g < a.cap) out[g] = stage[k];
-            }
-            __syncthreads();
-        }
-        for (uint32_t i = lo; i < hi; i++) bits[word_at(touched, listed, i)] = 0;
-        __syncthreads();
-    }
-
-    npaths = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
-    if (tid == 0 && npaths) atomicAdd((unsigned long long*)&a.sc->paths, (unsigned long long)npaths);
-    if (a.fp_mode) {
-        fsum = block_reduce_sum<BLOCK, uint64_t>(fsum, sh);
-        fxr = block_reduce_xor64<BLOCK>(fxr, sh);
-        if (tid == 0) {
-            atomicAdd((unsigned long long*)&a.sc->fp_sum, (unsigned long long)fsum);
-            atomicXor((unsigned long long*)&a.sc->fp_xr, (unsigned long long)fxr);
-        }
-    }
+// One join path into the tile.  Project: set bit zl; COUNT: add 1 to counter zl.
+template <bool COUNT>
+__device__ __forceinline__ bool apply_path(uint32_t* tile, uint32_t zl) {
+    if (COUNT) return atomicAdd(&tile[zl], 1u) == 0u;
+    return atomicOr(&tile[zl >> 5], 1u << (zl & 31u)) == 0u;
 }
 
-// ---------------------------------------------------------------------------
-// join-aggregate COUNT: u32 counter tile
-// LDS: cnt[tile_words] | touched[touched_cap] | (pad to 16 B) | stage[stage_cap] rows
-// ---------------------------------------------------------------------------
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_expand_count(ExpandArgs a) {
+template <bool COUNT>
+__device__ __forceinline__ uint32_t list_item(uint32_t zl) {
+    return COUNT ? zl : (zl >> 5);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ uint32_t units_of(uint32_t v) {
+    return COUNT ? (v != 0u) : (uint32_t)__popc(v);
+}
+
+// LDS layout (u32 words):
+//   tile[tile_words] | touched[touched_cap] | area[max(stage_cap, 3*BLOCK+1)]
+// `area` holds the segment table of the row during expansion and the staged
+// local z offsets during emission.
+template <int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr uint32_t NW = BLOCK / 64;
-    uint32_t* cnt = smem;
+    uint32_t* tile = smem;
     uint32_t* touched = smem + a.tile_words;
-    dim3_count_row* stage = reinterpret_cast<dim3_count_row*>(smem + ((a.tile_words + a.touched_cap + 3u) & ~3u));
+    uint32_t* area = touched + a.touched_cap;
+    uint32_t* s_sb = area;
+    uint32_t* s_len = area + BLOCK;
+    uint32_t* s_pre = area + 2 * BLOCK;
+    uint32_t* stage = area;
     __shared__ uint64_t sh[NW + 1];
+    __shared__ uint32_t sh32[NW + 1];
     __shared__ uint32_t s_task, s_nt;
     __shared__ uint64_t s_gbase;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t ntasks = a.nx * a.ntiles;
-    const uint32_t tile_size = 1u << a.tile_shift;
-    dim3_count_row* out = reinterpret_cast<dim3_count_row*>(a.out);
 
-    for (uint32_t w = tid; w < a.tile_words; w += BLOCK) cnt[w] = 0;
+    for (uint32_t w = tid; w < a.tile_words; w += BLOCK) tile[w] = 0;
     uint64_t fsum = 0, fxr = 0, fcs = 0, npaths = 0;
 
     for (;;) {
@@ -232,36 +122,72 @@ __global__ __launch_bounds__(BLOCK) void k_expand_count(ExpandArgs a) {
         if (task >= ntasks) break;
         const uint32_t x = task / a.ntiles;
         const uint32_t t = task - x * a.ntiles;
-        const uint32_t zbase = t << a.tile_shift;
+        const uint32_t zbase = t * a.tile_size;
         const uint32_t rb = a.rowptrR[x], re = a.rowptrR[x + 1];
 
-        for (uint32_t i = rb + wid; i < re; i += NW) {
-            const uint32_t key = (uint32_t)a.colR[i] * a.ntiles + t;
-            const uint32_t sb = a.rowptrS[key], se = a.rowptrS[key + 1];
-            npaths += (lane == 0) ? (uint64_t)(se - sb) : 0ull;
-            for (uint32_t j = sb + lane; j < se; j += 64) {
-                const uint32_t zl = (uint32_t)a.colS[j] - zbase;
-                const uint32_t old = atomicAdd(&cnt[zl], 1u);
-                push_first_touch(old == 0u, zl, &s_nt, touched, a.touched_cap);
+        // ---- expansion ----
+        for (uint32_t c0 = rb; c0 < re; c0 += BLOCK) {
+            const uint32_t i = c0 + tid;
+            uint32_t np = 0;
+            if (i < re) {
+                const uint32_t key = (uint32_t)a.colR[i] * a.ntiles + t;
+                const uint32_t sb = a.rowptrS[key], se = a.rowptrS[key + 1];
+                s_sb[tid] = sb;
+                s_len[tid] = se - sb;
+                np = (se - sb + kPiece - 1) / kPiece;
+                npaths += se - sb;
             }
+            uint32_t NP;
+            const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(np, NP, sh32);
+            s_pre[tid] = pre;
+            __syncthreads();
+            const uint32_t nseg = min((uint32_t)BLOCK, re - c0);
+            for (uint32_t k = wid; k < NP; k += NW) {
+                uint32_t lo = 0, hi = nseg;  // last segment whose first piece is <= k
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_pre[mid] <= k) lo = mid;
+                    else hi = mid;
+                }
+                const uint32_t off = (k - s_pre[lo]) * kPiece;
+                const uint32_t base = s_sb[lo] + off;
+                const uint32_t len = min(s_len[lo] - off, kPiece);
+                uint32_t zl[kUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; u++) {
+                    const uint32_t idx = lane + 64u * u;
+                    zl[u] = idx < len ? (uint32_t)a.colS[base + idx] - zbase : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; u++) {
+                    const bool v = zl[u] != 0xFFFFFFFFu;
+                    bool first = false;
+                    if (v) first = apply_path<COUNT>(tile, zl[u]);
+                    push_first_touch(first, list_item<COUNT>(zl[u]), &s_nt, touched, a.touched_cap);
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (a.x2z != nullptr && tid == 0) {
+        if (a.x2z != nullptr && tid == 0) {  // u != w filter of the self-join
             const int32_t zs = a.x2z[x];
             if (zs >= 0) {
                 const uint32_t zl = (uint32_t)zs - zbase;
-                if (zl < tile_size && zl < a.tile_words) cnt[zl] = 0;
+                if (zl < a.tile_size) {
+                    if (COUNT) tile[zl] = 0;
+                    else tile[zl >> 5] &= ~(1u << (zl & 31u));
+                }
             }
         }
         __syncthreads();
 
+        // ---- emission: each thread owns a contiguous slice of the list ----
         const uint32_t nt = s_nt;
         const bool listed = nt <= a.touched_cap;
         const uint32_t count = listed ? nt : a.tile_words;
         const uint32_t per = (count + BLOCK - 1) / BLOCK;
         const uint32_t lo = min(tid * per, count), hi = min(lo + per, count);
         uint32_t c = 0;
-        for (uint32_t i = lo; i < hi; i++) c += cnt[word_at(touched, listed, i)] != 0u;
+        for (uint32_t i = lo; i < hi; i++) c += units_of<COUNT>(tile[list_at(touched, listed, i)]);
         uint64_t T;
         const uint64_t o = block_exclusive_scan<BLOCK, uint64_t>((uint64_t)c, T, sh);
         const int32_t xv = a.id2x[x];
@@ -269,14 +195,24 @@ __global__ __launch_bounds__(BLOCK) void k_expand_count(ExpandArgs a) {
 
         if (a.fp_mode) {
             for (uint32_t i = lo; i < hi; i++) {
-                const uint32_t e = word_at(touched, listed, i);
-                const uint32_t v = cnt[e];
-                cnt[e] = 0;
-                if (v) {
-                    const uint64_t h = mix64(mix64(row_key(xv, id2z[e])) + (uint64_t)v);
-                    fsum += h;
-                    fxr ^= h;
-                    fcs += v;
+                const uint32_t w = list_at(touched, listed, i);
+                uint32_t v = tile[w];
+                tile[w] = 0;
+                if (COUNT) {
+                    if (v) {
+                        const uint64_t h = mix64(mix64(row_key(xv, id2z[w])) + (uint64_t)v);
+                        fsum += h;
+                        fxr ^= h;
+                        fcs += v;
+                    }
+                } else {
+                    while (v) {
+                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                        v &= v - 1u;
+                        const uint64_t h = mix64(row_key(xv, id2z[w * 32u + b]));
+                        fsum += h;
+                        fxr ^= h;
+                    }
                 }
             }
             if (tid == 0 && T) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
@@ -285,38 +221,80 @@ __global__ __launch_bounds__(BLOCK) void k_expand_count(ExpandArgs a) {
         }
 
         if (tid == 0) s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
-        __syncthreads();
-        const uint64_t gbase = s_gbase;
-        uint32_t ci = lo;
+        uint32_t ci = lo, cw = 0, cv = 0;
+        if (ci < hi) {
+            cw = list_at(touched, listed, ci);
+            cv = tile[cw];
+        }
         uint64_t r = o;
         const uint64_t rend = o + c;
         for (uint64_t q = 0; q < T; q += a.stage_cap) {
             const uint64_t qe = min(q + (uint64_t)a.stage_cap, T);
+            // fill: local z offsets of this thread's rows that fall in [q, qe)
             while (r < qe && r < rend) {
-                uint32_t e = word_at(touched, listed, ci);
-                uint32_t v = cnt[e];
-                while (v == 0u) {
-                    ++ci;
-                    e = word_at(touched, listed, ci);
-                    v = cnt[e];
+                if (COUNT) {
+                    while (cv == 0u) {
+                        ++ci;
+                        cw = list_at(touched, listed, ci);
+                        cv = tile[cw];
+                    }
+                    stage[r - q] = cw;
+                    cv = 0;
+                } else {
+                    while (cv == 0u) {
+                        ++ci;
+                        cw = list_at(touched, listed, ci);
+                        cv = tile[cw];
+                    }
+                    const uint32_t b = (uint32_t)__ffs(cv) - 1u;
+                    cv &= cv - 1u;
+                    stage[r - q] = cw * 32u + b;
                 }
-                ++ci;
-                dim3_count_row row;
-                row.x = xv;
-                row.z = id2z[e];
-                row.count = (int64_t)v;
-                stage[r - q] = row;
                 ++r;
             }
             __syncthreads();
+            const uint64_t gb = s_gbase + q;
             const uint32_t nq = (uint32_t)(qe - q);
-            for (uint32_t k = tid; k < nq; k += BLOCK) {
-                const uint64_t g = gbase + q + k;
-                if (g < a.cap) out[g] = stage[k];
+            if (COUNT) {
+                dim3_count_row* out = reinterpret_cast<dim3_count_row*>(a.out);
+                for (uint32_t k = tid; k < nq; k += BLOCK) {
+                    const uint32_t e = stage[k];
+                    const uint64_t g = gb + k;
+                    if (g < a.cap) {
+                        dim3_count_row row;
+                        row.x = xv;
+                        row.z = id2z[e];
+                        row.count = (int64_t)tile[e];
+                        out[g] = row;
+                    }
+                }
+            } else {
+                int2* out = reinterpret_cast<int2*>(a.out);
+                uint32_t k = tid;
+                for (; k + 3u * BLOCK < nq; k += 4u * BLOCK) {
+                    const int32_t z0 = id2z[stage[k]];
+                    const int32_t z1 = id2z[stage[k + BLOCK]];
+                    const int32_t z2 = id2z[stage[k + 2u * BLOCK]];
+                    const int32_t z3 = id2z[stage[k + 3u * BLOCK]];
+                    if (gb + k + 3u * BLOCK < a.cap) {
+                        out[gb + k] = make_int2(xv, z0);
+                        out[gb + k + BLOCK] = make_int2(xv, z1);
+                        out[gb + k + 2u * BLOCK] = make_int2(xv, z2);
+                        out[gb + k + 3u * BLOCK] = make_int2(xv, z3);
+                    } else {
+                        if (gb + k < a.cap) out[gb + k] = make_int2(xv, z0);
+                        if (gb + k + BLOCK < a.cap) out[gb + k + BLOCK] = make_int2(xv, z1);
+                        if (gb + k + 2u * BLOCK < a.cap) out[gb + k + 2u * BLOCK] = make_int2(xv, z2);
+                    }
+                }
+                for (; k < nq; k += BLOCK) {
+                    const uint64_t g = gb + k;
+                    if (g < a.cap) out[g] = make_int2(xv, id2z[stage[k]]);
+                }
             }
             __syncthreads();
         }
-        for (uint32_t i = lo; i < hi; i++) cnt[word_at(touched, listed, i)] = 0;
+        for (uint32_t i = lo; i < hi; i++) tile[list_at(touched, listed, i)] = 0;
         __syncthreads();
     }
 
@@ -328,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand_count(ExpandArgs a) {
         fxr = block_reduce_xor64<BLOCK>(fxr, sh);
         if (tid == 0) {
             atomicAdd((unsigned long long*)&a.sc->fp_sum, (unsigned long long)fsum);
-            atomicAdd((unsigned long long*)&a.sc->fp_csum, (unsigned long long)fcs);
+            if (COUNT) atomicAdd((unsigned long long*)&a.sc->fp_csum, (unsigned long long)fcs);
             atomicXor((unsigned long long*)&a.sc->fp_xr, (unsigned long long)fxr);
         }
     }
@@ -350,36 +328,32 @@ int launch_persistent(K kern, int block, size_t lds, const ExpandArgs& a, int cu
     return (int)hipGetLastError();
 }
 
+template <bool COUNT>
+int launch(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
+    const size_t lds = dim3h::expand_lds_bytes(a.tile_words, a.touched_cap, a.stage_cap, (uint32_t)block);
+    switch (block) {
+        case 256: return launch_persistent(k_expand<256, COUNT>, 256, lds, a, cu_count, st);
+        case 512: return launch_persistent(k_expand<512, COUNT>, 512, lds, a, cu_count, st);
+        case 1024: return launch_persistent(k_expand<1024, COUNT>, 1024, lds, a, cu_count, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
 
 namespace dim3h {
 
-size_t expand_project_lds(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap) {
-    return (size_t)((tile_words + touched_cap + 1u) & ~1u) * 4u + (size_t)stage_cap * 8u;
-}
-
-size_t expand_count_lds(uint32_t tile_entries, uint32_t touched_cap, uint32_t stage_cap) {
-    return (size_t)((tile_entries + touched_cap + 3u) & ~3u) * 4u + (size_t)stage_cap * 16u;
+size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block) {
+    const uint32_t area = stage_cap > 3u * block + 1u ? stage_cap : 3u * block + 1u;
+    return ((size_t)tile_words + touched_cap + area) * 4u;
 }
 
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
-    const size_t lds = expand_project_lds(a.tile_words, a.touched_cap, a.stage_cap);
-    switch (block) {
-        case 256: return launch_persistent(k_expand_project<256>, 256, lds, a, cu_count, st);
-        case 512: return launch_persistent(k_expand_project<512>, 512, lds, a, cu_count, st);
-        case 1024: return launch_persistent(k_expand_project<1024>, 1024, lds, a, cu_count, st);
-        default: return (int)hipErrorInvalidValue;
-    }
+    return launch<false>(a, block, cu_count, st);
 }
 
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
-    const size_t lds = expand_count_lds(a.tile_words, a.touched_cap, a.stage_cap);
-    switch (block) {
-        case 256: return launch_persistent(k_expand_count<256>, 256, lds, a, cu_count, st);
-        case 512: return launch_persistent(k_expand_count<512>, 512, lds, a, cu_count, st);
-        case 1024: return launch_persistent(k_expand_count<1024>, 1024, lds, a, cu_count, st);
-        default: return (int)hipErrorInvalidValue;
-    }
+    return launch<true>(a, block, cu_count, st);
 }
 
 }  // namespace dim3h

@@ -237,7 +237,7 @@ __global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* _
 // expansion of one z tile reads only its own column range.
 __global__ void k_count_s(const int32_t* __restrict__ slotS, const int32_t* __restrict__ slotSz, uint64_t nS,
                           const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_z, uint32_t ntiles,
-                          uint32_t tile_shift, uint32_t* cnt, uint32_t* __restrict__ keyS,
+                          uint32_t tile_size, uint32_t* cnt, uint32_t* __restrict__ keyS,
                           int32_t* __restrict__ zidS) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
@@ -248,7 +248,7 @@ __global__ void k_count_s(const int32_t* __restrict__ slotS, const int32_t* __re
     }
     const uint32_t yid = (uint32_t)(ex_y[sy] >> 32);
     const uint32_t zid = (uint32_t)(ex_z[slotSz[i]] >> 32);
-    const uint32_t key = yid * ntiles + (zid >> tile_shift);
+    const uint32_t key = ntiles == 1 ? yid : yid * ntiles + zid / tile_size;
     atomicAdd(&cnt[key], 1u);
     keyS[i] = key;
     zidS[i] = (int32_t)zid;
@@ -337,10 +337,10 @@ void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const 
                            cursor, col);
 }
 void count_s(const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y, const uint64_t* ex_z,
-             uint32_t ntiles, uint32_t tile_shift, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
+             uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
     if (nS)
         hipLaunchKernelGGL(k_count_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, slotS, slotSz, nS, ex_y, ex_z,
-                           ntiles, tile_shift, cnt, keyS, zidS);
+                           ntiles, tile_size, cnt, keyS, zidS);
 }
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st) {
