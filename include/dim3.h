@@ -50,6 +50,7 @@ extern "C" {
 /* ---- flags ---- */
 #define DIM3_OPT_NO_SWAP 1u       /* never swap R and S */
 #define DIM3_OPT_EXCLUDE_EQUAL 2u /* drop rows with x == z (self-join u != w filter) */
+#define DIM3_OPT_HASH_ONLY 4u     /* hash every attribute (no dense-range mapping) */
 
 /* ---- dense (heavy-key) strategy ---- */
 #define DIM3_DENSE_AUTO (-1)

@@ -100,6 +100,10 @@ struct dim3_ctx {
         uint32_t nnzR = 0, nnzS = 0;
         uint64_t out_j = 0;
         uint32_t ntiles = 1, tile_size = 0, tile_words = 0;
+        bool x_range = false, y_range = false, z_ident = false;
+        int32_t zlo = 0;
+        uint32_t nz_present = 0;
+        dim3h::KeyMap mx{}, my{}, mz{};
         uint32_t touched_cap = 0, stage_cap = 0;
         int block = 256;
         float ms_map = 0, ms_csr = 0;
@@ -258,54 +262,82 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
 
     Scalars* sc;
     TRY(ws(c, WS_SCALARS, 1, &sc));
-    hipMemsetAsync(sc, 0, sizeof(Scalars), st);
+    HIPCHK(c, hipMemsetAsync(sc, 0, sizeof(Scalars), st));
     if (nR == 0 || nS == 0) {  // dim3.hpp:1027
         P.empty = true;
         return DIM3_OK;
     }
 
     HIPCHK(c, hipEventRecord(c->ev[0], st));
-    P.capY = pow2_at_least(2 * nR);
-    P.capX = P.capY;
-    P.capZ = pow2_at_least(2 * nS);
-    const uint64_t cy1 = (uint64_t)P.capY + 1, cx1 = (uint64_t)P.capX + 1, cz1 = (uint64_t)P.capZ + 1;
+    // value ranges decide hash or range mapping per attribute
+    dim3h::minmax(dR, nR, dS, nS, sc->mm, st);
+    HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const int32_t* mm = c->h_sc->mm;
+    const bool hash_only = (o.flags & DIM3_OPT_HASH_ONLY) != 0;
+    auto span = [](int32_t lo, int32_t hi) { return (uint64_t)((int64_t)hi - (int64_t)lo + 1); };
+    const uint64_t xr = span(mm[0], mm[1]), yr = span(mm[2], mm[3]), zr = span(mm[4], mm[5]);
+    P.x_range = !hash_only && xr <= 2 * nR + 64;
+    P.y_range = !hash_only && yr <= 2 * nR + 64;
+    P.z_ident = !hash_only && zr <= 2 * nS + 64;
+    P.zlo = P.z_ident ? mm[4] : 0;
 
-    uint32_t *keysY, *degR, *degS, *slotR, *keysX, *degX, *keysZ, *degZ;
-    int32_t *slotS, *slotRx, *slotSz;
-    uint64_t *exY, *exX, *exZ, *part;
-    TRY(ws(c, WS_KEYS_Y, cy1, &keysY));
-    TRY(ws(c, WS_DEGR_Y, cy1, &degR));
-    TRY(ws(c, WS_DEGS_Y, cy1, &degS));
-    TRY(ws(c, WS_EX_Y, cy1, &exY));
+    uint32_t *keysY = nullptr, *degR, *degS, *slotR, *keysX = nullptr, *degX, *keysZ = nullptr, *degZ = nullptr;
+    int32_t *slotS, *slotRx, *slotSz = nullptr;
+    uint64_t *exY, *exX, *exZ = nullptr, *part;
+    auto make_map = [&](bool range, int32_t lo, uint64_t span_v, uint64_t n, SlotId keys_slot, uint32_t** keys,
+                        dim3h::KeyMap* m) -> int {
+        m->range = range ? 1u : 0u;
+        m->lo = lo;
+        m->keys = nullptr;
+        m->mask = 0;
+        if (range) {
+            m->nslots = (uint32_t)span_v;
+        } else {
+            const uint32_t cap = pow2_at_least(2 * n);
+            TRY(ws(c, keys_slot, (uint64_t)cap + 1, keys));
+            m->keys = *keys;
+            m->mask = cap - 1;
+            m->nslots = cap + 1;
+        }
+        return DIM3_OK;
+    };
+    TRY(make_map(P.y_range, mm[2], yr, nR, WS_KEYS_Y, &keysY, &P.my));
+    TRY(make_map(P.x_range, mm[0], xr, nR, WS_KEYS_X, &keysX, &P.mx));
+    TRY(make_map(P.z_ident, mm[4], zr, nS, WS_KEYS_Z, &keysZ, &P.mz));
+    const uint64_t ny1 = P.my.nslots, nx1 = P.mx.nslots, nz1 = P.mz.nslots;
+    TRY(ws(c, WS_DEGR_Y, ny1, &degR));
+    TRY(ws(c, WS_DEGS_Y, ny1, &degS));
+    TRY(ws(c, WS_EX_Y, ny1, &exY));
     TRY(ws(c, WS_SLOT_R, nR, &slotR));
     TRY(ws(c, WS_SLOT_S, nS, &slotS));
-    TRY(ws(c, WS_KEYS_X, cx1, &keysX));
-    TRY(ws(c, WS_DEG_X, cx1, &degX));
-    TRY(ws(c, WS_EX_X, cx1, &exX));
+    TRY(ws(c, WS_DEG_X, nx1, &degX));
+    TRY(ws(c, WS_EX_X, nx1, &exX));
     TRY(ws(c, WS_SLOT_RX, nR, &slotRx));
-    TRY(ws(c, WS_KEYS_Z, cz1, &keysZ));
-    TRY(ws(c, WS_DEG_Z, cz1, &degZ));
-    TRY(ws(c, WS_EX_Z, cz1, &exZ));
-    TRY(ws(c, WS_SLOT_SZ, nS, &slotSz));
-    const uint64_t maxscan = std::max<uint64_t>(std::max(cy1, cz1), std::max<uint64_t>(nS, nR) + 1);
+    if (!P.z_ident) {
+        TRY(ws(c, WS_DEG_Z, nz1, &degZ));
+        TRY(ws(c, WS_EX_Z, nz1, &exZ));
+        TRY(ws(c, WS_SLOT_SZ, nS, &slotSz));
+    }
+    const uint64_t maxscan = std::max<uint64_t>(std::max(ny1, std::max(nx1, nz1)), std::max<uint64_t>(nS, nR) + 1);
     TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed(maxscan), &part));
 
-    dim3h::ht_init(keysY, P.capY, st);
-    dim3h::ht_init(keysX, P.capX, st);
-    dim3h::ht_init(keysZ, P.capZ, st);
-    dim3h::fill_u32(degR, cy1, 0, st);
-    dim3h::fill_u32(degS, cy1, 0, st);
-    dim3h::fill_u32(degX, cx1, 0, st);
-    dim3h::fill_u32(degZ, cz1, 0, st);
-
-    const uint32_t mY = P.capY - 1, mX = P.capX - 1, mZ = P.capZ - 1;
-    dim3h::build_y(dR, nR, keysY, mY, degR, slotR, st);
-    dim3h::probe_y(dS, nS, keysY, mY, degS, slotS, st);
-    dim3h::scan_y(degR, degS, (uint32_t)cy1, exY, part, &sc->tot_y, &sc->out_j, st);
-    dim3h::build_x(dR, nR, slotR, degS, keysX, mX, degX, slotRx, st);
-    dim3h::build_z(dS, nS, slotS, keysZ, mZ, degZ, slotSz, st);
-    dim3h::scan_deg(degX, (uint32_t)cx1, exX, part, &sc->tot_x, st);
-    dim3h::scan_deg(degZ, (uint32_t)cz1, exZ, part, &sc->tot_z, st);
+    dim3h::map_init(P.my, st);
+    dim3h::map_init(P.mx, st);
+    dim3h::fill_u32(degR, ny1, 0, st);
+    dim3h::fill_u32(degS, ny1, 0, st);
+    dim3h::fill_u32(degX, nx1, 0, st);
+    dim3h::build_y(dR, nR, P.my, degR, slotR, st);
+    dim3h::probe_y(dS, nS, P.my, degR, degS, slotS, &sc->s_surv, st);
+    dim3h::scan_y(degR, degS, (uint32_t)ny1, exY, part, &sc->tot_y, &sc->out_j, st);
+    dim3h::build_x(dR, nR, slotR, degS, P.mx, degX, slotRx, st);
+    dim3h::scan_deg(degX, (uint32_t)nx1, exX, part, &sc->tot_x, st);
+    if (!P.z_ident) {
+        dim3h::map_init(P.mz, st);
+        dim3h::fill_u32(degZ, nz1, 0, st);
+        dim3h::build_z(dS, nS, slotS, P.mz, degZ, slotSz, st);
+        dim3h::scan_deg(degZ, (uint32_t)nz1, exZ, part, &sc->tot_z, st);
+    }
     LAUNCHCHK(c, "mapping");
     HIPCHK(c, hipEventRecord(c->ev[1], st));
     HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -314,15 +346,16 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     P.ny = (uint32_t)(c->h_sc->tot_y >> 32);
     P.nx = (uint32_t)(c->h_sc->tot_x >> 32);
     P.nnzR = (uint32_t)c->h_sc->tot_x;
-    P.nz = (uint32_t)(c->h_sc->tot_z >> 32);
-    P.nnzS = (uint32_t)c->h_sc->tot_z;
+    P.nnzS = (uint32_t)c->h_sc->s_surv;
+    P.nz = P.z_ident ? (uint32_t)zr : (uint32_t)(c->h_sc->tot_z >> 32);
+    P.nz_present = P.z_ident ? 0 : P.nz;
     P.out_j = c->h_sc->out_j;
     {
         float ms = 0;
         hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
         P.ms_map = ms;
     }
-    if (P.ny == 0 || P.nx == 0 || P.nz == 0) {
+    if (P.ny == 0 || P.nx == 0 || P.nz == 0 || P.nnzS == 0) {
         P.empty = true;
         return DIM3_OK;
     }
@@ -332,11 +365,11 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         return fail(c, DIM3_ERR_INVALID, "tiled S index does not fit 32 bits (ny=%u, tiles=%u)", P.ny, P.ntiles);
 
     HIPCHK(c, hipEventRecord(c->ev[2], st));
-    int32_t *id2x, *id2z, *colR, *zidS, *colS, *x2z;
+    int32_t *id2x, *id2z = nullptr, *colR, *zidS, *colS, *x2z;
     uint32_t *rowR, *curR, *cntS, *rowS, *curS, *keyS;
     const uint64_t nkeyS = (uint64_t)P.ny * P.ntiles;
     TRY(ws(c, WS_ID2X, P.nx, &id2x));
-    TRY(ws(c, WS_ID2Z, (uint64_t)P.ntiles * P.tile_size, &id2z));
+    if (!P.z_ident) TRY(ws(c, WS_ID2Z, (uint64_t)P.ntiles * P.tile_size, &id2z));
     TRY(ws(c, WS_ROWPTR_R, (uint64_t)P.nx + 1, &rowR));
     TRY(ws(c, WS_CUR_R, P.nx, &curR));
     TRY(ws(c, WS_COL_R, P.nnzR, &colR));
@@ -349,18 +382,20 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     TRY(ws(c, WS_X2Z, P.nx, &x2z));
     TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed(std::max<uint64_t>(maxscan, nkeyS + 1)), &part));
 
-    dim3h::compact_ids(keysX, P.capX, degX, exX, id2x, rowR, st);
+    dim3h::compact_ids(P.mx, degX, exX, id2x, rowR, st);
     dim3h::finish_rowptr(rowR, P.nx, &sc->tot_x, st);
-    dim3h::compact_ids(keysZ, P.capZ, degZ, exZ, id2z, nullptr, st);
+    if (!P.z_ident) dim3h::compact_ids(P.mz, degZ, exZ, id2z, nullptr, st);
     HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
     dim3h::scatter_r(slotR, slotRx, nR, exY, exX, curR, colR, st);
     dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
-    dim3h::count_s(slotS, slotSz, nS, exY, exZ, P.ntiles, P.tile_size, cntS, keyS, zidS, st);
+    dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, cntS, keyS,
+                   zidS, st);
     dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
     dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
     HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
     dim3h::scatter_s(keyS, zidS, nS, curS, colS, st);
-    if (o.flags & DIM3_OPT_EXCLUDE_EQUAL) dim3h::x2z_self(id2x, P.nx, keysZ, mZ, degZ, exZ, x2z, st);
+    if (o.flags & DIM3_OPT_EXCLUDE_EQUAL)
+        dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
     LAUNCHCHK(c, "csr");
     HIPCHK(c, hipEventRecord(c->ev[3], st));
     return DIM3_OK;
@@ -384,7 +419,8 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
     a.rowptrS = (const uint32_t*)c->ws[WS_ROWPTR_S].p;
     a.colS = (const int32_t*)c->ws[WS_COL_S].p;
     a.id2x = (const int32_t*)c->ws[WS_ID2X].p;
-    a.id2z = (const int32_t*)c->ws[WS_ID2Z].p;
+    a.id2z = P.z_ident ? nullptr : (const int32_t*)c->ws[WS_ID2Z].p;
+    a.zlo = P.zlo;
     a.x2z = (P.flags & DIM3_OPT_EXCLUDE_EQUAL) ? (const int32_t*)c->ws[WS_X2Z].p : nullptr;
     a.nx = P.nx;
     a.ntiles = P.ntiles;
@@ -662,12 +698,13 @@ int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const i
     TRY(ws(c, WS_SLOT_R, nR, &slotR));
     TRY(ws(c, WS_SLOT_S, nS, &slotS));
     TRY(ws(c, WS_SCALARS, 1, &sc));
-    hipMemsetAsync(sc, 0, sizeof(Scalars), st);
-    dim3h::ht_init(keys, cap, st);
+    HIPCHK(c, hipMemsetAsync(sc, 0, sizeof(Scalars), st));
+    dim3h::KeyMap m{keys, cap - 1, 0, cap + 1, 0};
+    dim3h::map_init(m, st);
     dim3h::fill_u32(degR, (uint64_t)cap + 1, 0, st);
     dim3h::fill_u32(degS, (uint64_t)cap + 1, 0, st);
-    dim3h::build_y((const int2*)dR, nR, keys, cap - 1, degR, slotR, st);
-    dim3h::probe_y((const int2*)dS, nS, keys, cap - 1, degS, slotS, st);
+    dim3h::build_y((const int2*)dR, nR, m, degR, slotR, st);
+    dim3h::probe_y((const int2*)dS, nS, m, degR, degS, slotS, &sc->s_surv, st);
     dim3h::outj_only(degR, degS, cap + 1, &sc->out_j, st);
     LAUNCHCHK(c, "count_join_paths");
     HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));

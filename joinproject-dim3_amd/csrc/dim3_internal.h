@@ -179,52 +179,62 @@ struct Scalars {
     uint32_t pad0;
     uint64_t heavy_n;    // heavy keys selected
     uint64_t nxh, nzh;
+    uint64_t s_surv;     // S tuples whose y joined
+    int32_t mm[8];       // min/max of R.x, R.y, S.z, S.y
     uint64_t spare[4];
-};
-
-struct ScanTmp {
-    uint64_t* partials;  // >= scan_partials_needed(n)
 };
 
 size_t scan_partials_needed(uint64_t n);
 
+// Slot space of one attribute: an open-addressing table (hash mode) or the
+// value range itself (range mode, slot = value - lo).
+struct KeyMap {
+    uint32_t* keys;   // hash mode: keys[mask + 2] (last slot stands for the key -1)
+    uint32_t mask;    // hash mode: capacity - 1
+    int32_t lo;       // range mode: smallest value
+    uint32_t nslots;  // slots to scan: mask + 2 (hash) or hi - lo + 1 (range)
+    uint32_t range;   // 1 = range mode
+};
+
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
-void ht_init(uint32_t* keys, uint32_t cap, hipStream_t st);
+void map_init(const KeyMap& m, hipStream_t st);
+// mm[0..3] = min/max of R.x, R.y; mm[4..7] = min/max of S.z, S.y
+void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm, hipStream_t st);
 
-// y mapping: insert R.y, count degR, record slots; probe S.y, count degS
-void build_y(const int2* R, uint64_t nR, uint32_t* keys, uint32_t mask, uint32_t* degR, uint32_t* slotR,
-             hipStream_t st);
-void probe_y(const int2* S, uint64_t nS, const uint32_t* keys, uint32_t mask, uint32_t* degS, int32_t* slotS,
-             hipStream_t st);
-// scan of the y table: ex[s] = (#joined y before s) << 32; total -> *tot; out_j accumulated
-void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* ex, uint64_t* partials,
+// y mapping: insert R.y, count degR, record slots; probe S.y, count degS and survivors
+void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st);
+void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, uint32_t* degS, int32_t* slotS,
+             uint64_t* surv, hipStream_t st);
+// scan of the y slots: ex[s] = (#joined y before s) << 32; out_j accumulated
+void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* ex, uint64_t* partials,
             uint64_t* tot, uint64_t* out_j, hipStream_t st);
+void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* out_j, hipStream_t st);
 // x / z mapping over the tuples whose y joined
-void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, uint32_t* keys,
-             uint32_t mask, uint32_t* degX, int32_t* slotRx, hipStream_t st);
-void build_z(const int2* S, uint64_t nS, const int32_t* slotS, uint32_t* keys, uint32_t mask, uint32_t* degZ,
-             int32_t* slotSz, hipStream_t st);
-// scan of a degree table: ex[s] = (#occupied before s) << 32 | (sum deg before s)
-void scan_deg(const uint32_t* deg, uint32_t cap1, uint64_t* ex, uint64_t* partials, uint64_t* tot, hipStream_t st);
-// id2v / rowptr from a scanned table
-void compact_ids(const uint32_t* keys, uint32_t cap, const uint32_t* deg, const uint64_t* ex, int32_t* id2v,
-                 uint32_t* rowptr, hipStream_t st);
+void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
+             uint32_t* degX, int32_t* slotRx, hipStream_t st);
+void build_z(const int2* S, uint64_t nS, const int32_t* slotS, const KeyMap& m, uint32_t* degZ, int32_t* slotSz,
+             hipStream_t st);
+// scan of a degree array: ex[s] = (#occupied before s) << 32 | (sum deg before s)
+void scan_deg(const uint32_t* deg, uint32_t nslots, uint64_t* ex, uint64_t* partials, uint64_t* tot,
+              hipStream_t st);
+// id -> value and CSR row starts from a scanned slot space
+void compact_ids(const KeyMap& m, const uint32_t* deg, const uint64_t* ex, int32_t* id2v, uint32_t* rowptr,
+                 hipStream_t st);
 void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_t st);
 // CSR of R by x: col = y id
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
                const uint64_t* ex_x, uint32_t* cursor, int32_t* col, hipStream_t st);
-// tiled CSR of S by (y, z-tile)
-void count_s(const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y, const uint64_t* ex_z,
-             uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st);
+// tiled CSR of S by (y, z-tile); ex_z == nullptr means z identity (zid = z - zlo)
+void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
+             const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS,
+             int32_t* zidS, hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st);
 void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, int32_t* col,
                hipStream_t st);
-void x2z_self(const int32_t* id2x, uint32_t nx, const uint32_t* keysZ, uint32_t maskZ, const uint32_t* degZ,
-              const uint64_t* ex_z, int32_t* x2z, hipStream_t st);
-// exact OUT_J only (no ID assignment)
-void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* out_j, hipStream_t st);
+void x2z_self(const int32_t* id2x, uint32_t nx, const KeyMap& mz, const uint32_t* degZ, const uint64_t* ex_z,
+              int32_t* x2z, hipStream_t st);
 
 struct ExpandArgs {
     const uint32_t* rowptrR;
@@ -232,7 +242,8 @@ struct ExpandArgs {
     const uint32_t* rowptrS;
     const int32_t* colS;
     const int32_t* id2x;
-    const int32_t* id2z;
+    const int32_t* id2z;  // nullptr: z identity, value = zlo + z ID
+    int32_t zlo;
     const int32_t* x2z;  // nullable: z ID of each x value (exclude_equal)
     uint32_t nx;
     uint32_t ntiles;

@@ -14,16 +14,19 @@
 //   - the row's S segments are cut into pieces of 256 paths, the pieces are
 //     dealt to the wavefronts round robin (so a hub segment is spread over
 //     the whole workgroup) and each lane keeps four column loads in flight;
-//   - the first touch of a word / counter is recorded in an LDS list with one
-//     LDS atomic per wavefront (ballot + mbcnt), so sparse rows emit from the
-//     touched words only; when the list overflows the row sweeps its tile;
-//   - emission counts the row, reserves its output range with one global
-//     atomic, stages the row's local z offsets in LDS and writes (x, z) rows
-//     with coalesced stores, looking z values up in the same pass; the tile
-//     is cleared on the way out.
+//   - for rows with few paths the first touch of a word / counter is recorded
+//     in an LDS list with one LDS atomic per wavefront (ballot + mbcnt), so
+//     sparse rows emit from the touched words only; rows with many paths (or
+//     an overflowing list) sweep their tile instead;
+//   - emission splits the list over the wavefronts, reserves the row's
+//     output range with one global atomic, and each wavefront stages its
+//     local z offsets in a private LDS slice and writes (x, z) rows with
+//     coalesced stores (no block barrier between staging and writing); z
+//     values are zlo + ID for dense domains, else one id -> value read;
+//     the tile is cleared on the way out.
 // Per join path: one 4-byte column read (mostly L2/MALL hits, as heavy S rows
 // are re-read by every x that reaches them) and one LDS atomic.  Per output
-// row: one 4-byte id->value read and one 8-byte (16-byte for COUNT) store.
+// row: one 8-byte (16-byte for COUNT) store.
 #include "dim3_internal.h"
 
 namespace {
@@ -87,9 +90,10 @@ __device__ __forceinline__ uint32_t units_of(uint32_t v) {
 
 // LDS layout (u32 words):
 //   tile[tile_words] | touched[touched_cap] | area[max(stage_cap, 3*BLOCK+1)]
-// `area` holds the segment table of the row during expansion and the staged
-// local z offsets during emission.
-template <int BLOCK, bool COUNT>
+// `area` holds the segment table of the row during expansion and, during
+// emission, one private staging slice per wavefront.
+// ZID: z IDs are value - zlo (dense domain), so no id -> value lookup.
+template <int BLOCK, bool COUNT, bool ZID>
 __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr uint32_t NW = BLOCK / 64;
@@ -99,18 +103,21 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     uint32_t* s_sb = area;
     uint32_t* s_len = area + BLOCK;
     uint32_t* s_pre = area + 2 * BLOCK;
-    uint32_t* stage = area;
+    const uint32_t wsc = a.stage_cap / NW;  // staging slice per wavefront
     __shared__ uint64_t sh[NW + 1];
     __shared__ uint32_t sh32[NW + 1];
+    __shared__ uint32_t s_wtot[NW];
     __shared__ uint32_t s_task, s_nt;
     __shared__ uint64_t s_gbase;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t ntasks = a.nx * a.ntiles;
+    const uint64_t track_limit = 4ull * a.tile_words;  // above this many paths a sweep is cheaper
 
     for (uint32_t w = tid; w < a.tile_words; w += BLOCK) tile[w] = 0;
     uint64_t fsum = 0, fxr = 0, fcs = 0, npaths = 0;
+    uint32_t* wst = area + wid * wsc;
 
     for (;;) {
         if (tid == 0) {
@@ -126,6 +133,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         const uint32_t rb = a.rowptrR[x], re = a.rowptrR[x + 1];
 
         // ---- expansion ----
+        bool track = true;
         for (uint32_t c0 = rb; c0 < re; c0 += BLOCK) {
             const uint32_t i = c0 + tid;
             uint32_t np = 0;
@@ -140,6 +148,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             uint32_t NP;
             const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(np, NP, sh32);
             s_pre[tid] = pre;
+            if (c0 == rb) track = (uint64_t)NP * kPiece <= track_limit;
             __syncthreads();
             const uint32_t nseg = min((uint32_t)BLOCK, re - c0);
             for (uint32_t k = wid; k < NP; k += NW) {
@@ -158,39 +167,69 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     const uint32_t idx = lane + 64u * u;
                     zl[u] = idx < len ? (uint32_t)a.colS[base + idx] - zbase : 0xFFFFFFFFu;
                 }
+                if (track) {
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; u++) {
-                    const bool v = zl[u] != 0xFFFFFFFFu;
-                    bool first = false;
-                    if (v) first = apply_path<COUNT>(tile, zl[u]);
-                    push_first_touch(first, list_item<COUNT>(zl[u]), &s_nt, touched, a.touched_cap);
+                    for (uint32_t u = 0; u < kUnroll; u++) {
+                        bool first = false;
+                        if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u]);
+                        push_first_touch(first, list_item<COUNT>(zl[u]), &s_nt, touched, a.touched_cap);
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t u = 0; u < kUnroll; u++) {
+                        if (zl[u] != 0xFFFFFFFFu) {
+                            if (COUNT) atomicAdd(&tile[zl[u]], 1u);
+                            else atomicOr(&tile[zl[u] >> 5], 1u << (zl[u] & 31u));
+                        }
+                    }
                 }
             }
             __syncthreads();
         }
-        if (a.x2z != nullptr && tid == 0) {  // u != w filter of the self-join
+
+        // ---- emission ----
+        const uint32_t nt = s_nt;
+        const bool listed = track && nt <= a.touched_cap;
+        const uint32_t count = listed ? nt : a.tile_words;
+        uint32_t self_item = 0xFFFFFFFFu, self_keep = 0xFFFFFFFFu;  // u != w filter of the self-join
+        if (a.x2z != nullptr) {
             const int32_t zs = a.x2z[x];
-            if (zs >= 0) {
-                const uint32_t zl = (uint32_t)zs - zbase;
-                if (zl < a.tile_size) {
-                    if (COUNT) tile[zl] = 0;
-                    else tile[zl >> 5] &= ~(1u << (zl & 31u));
-                }
+            const uint32_t zl = (uint32_t)zs - zbase;
+            if (zs >= 0 && zl < a.tile_size) {
+                self_item = list_item<COUNT>(zl);
+                self_keep = COUNT ? 0u : ~(1u << (zl & 31u));
             }
         }
-        __syncthreads();
-
-        // ---- emission: each thread owns a contiguous slice of the list ----
-        const uint32_t nt = s_nt;
-        const bool listed = nt <= a.touched_cap;
-        const uint32_t count = listed ? nt : a.tile_words;
-        const uint32_t per = (count + BLOCK - 1) / BLOCK;
-        const uint32_t lo = min(tid * per, count), hi = min(lo + per, count);
+        // wavefront slice of the list, then a contiguous lane slice of it
+        const uint32_t wper = (count + NW - 1) / NW;
+        const uint32_t wlo = min(wid * wper, count), whi = min(wlo + wper, count);
+        const uint32_t lper = (whi - wlo + 63u) / 64u;
+        const uint32_t lo = min(wlo + lane * lper, whi), hi = min(lo + lper, whi);
         uint32_t c = 0;
-        for (uint32_t i = lo; i < hi; i++) c += units_of<COUNT>(tile[list_at(touched, listed, i)]);
-        uint64_t T;
-        const uint64_t o = block_exclusive_scan<BLOCK, uint64_t>((uint64_t)c, T, sh);
+        for (uint32_t i = lo; i < hi; i++) {
+            const uint32_t w = list_at(touched, listed, i);
+            uint32_t v = tile[w];
+            if (w == self_item) {
+                v &= self_keep;
+                tile[w] = v;
+            }
+            c += units_of<COUNT>(v);
+        }
+        const uint32_t incl = wave_inclusive_scan(c);
+        const uint32_t wtot = __shfl(incl, 63, 64);
+        const uint32_t loff = incl - c;
+        if (lane == 0) s_wtot[wid] = wtot;
+        __syncthreads();
+        uint32_t woff = 0;
+        uint64_t T = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < NW; w++) {
+            const uint32_t v = s_wtot[w];
+            if (w < wid) woff += v;
+            T += v;
+        }
         const int32_t xv = a.id2x[x];
+        const uint32_t zoff = (uint32_t)a.zlo + zbase;
         const int32_t* id2z = a.id2z + zbase;
 
         if (a.fp_mode) {
@@ -200,7 +239,8 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                 tile[w] = 0;
                 if (COUNT) {
                     if (v) {
-                        const uint64_t h = mix64(mix64(row_key(xv, id2z[w])) + (uint64_t)v);
+                        const int32_t zv = ZID ? (int32_t)(zoff + w) : id2z[w];
+                        const uint64_t h = mix64(mix64(row_key(xv, zv)) + (uint64_t)v);
                         fsum += h;
                         fxr ^= h;
                         fcs += v;
@@ -209,7 +249,9 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     while (v) {
                         const uint32_t b = (uint32_t)__ffs(v) - 1u;
                         v &= v - 1u;
-                        const uint64_t h = mix64(row_key(xv, id2z[w * 32u + b]));
+                        const uint32_t e = w * 32u + b;
+                        const int32_t zv = ZID ? (int32_t)(zoff + e) : id2z[e];
+                        const uint64_t h = mix64(row_key(xv, zv));
                         fsum += h;
                         fxr ^= h;
                     }
@@ -221,78 +263,68 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         }
 
         if (tid == 0) s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
+        __syncthreads();
+        const uint64_t gb = s_gbase + woff;
         uint32_t ci = lo, cw = 0, cv = 0;
         if (ci < hi) {
             cw = list_at(touched, listed, ci);
             cv = tile[cw];
         }
-        uint64_t r = o;
-        const uint64_t rend = o + c;
-        for (uint64_t q = 0; q < T; q += a.stage_cap) {
-            const uint64_t qe = min(q + (uint64_t)a.stage_cap, T);
-            // fill: local z offsets of this thread's rows that fall in [q, qe)
+        uint32_t r = loff;
+        const uint32_t rend = loff + c;
+        for (uint32_t q = 0; q < wtot; q += wsc) {
+            const uint32_t qe = min(q + wsc, wtot);
+            // fill this wavefront's slice with local z offsets of ranks [q, qe)
             while (r < qe && r < rend) {
+                while (cv == 0u) {
+                    ++ci;
+                    cw = list_at(touched, listed, ci);
+                    cv = tile[cw];
+                }
                 if (COUNT) {
-                    while (cv == 0u) {
-                        ++ci;
-                        cw = list_at(touched, listed, ci);
-                        cv = tile[cw];
-                    }
-                    stage[r - q] = cw;
+                    wst[r - q] = cw;
                     cv = 0;
                 } else {
-                    while (cv == 0u) {
-                        ++ci;
-                        cw = list_at(touched, listed, ci);
-                        cv = tile[cw];
-                    }
                     const uint32_t b = (uint32_t)__ffs(cv) - 1u;
                     cv &= cv - 1u;
-                    stage[r - q] = cw * 32u + b;
+                    wst[r - q] = cw * 32u + b;
                 }
                 ++r;
             }
-            __syncthreads();
-            const uint64_t gb = s_gbase + q;
-            const uint32_t nq = (uint32_t)(qe - q);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t nq = qe - q;
+            const uint64_t g0 = gb + q;
             if (COUNT) {
                 dim3_count_row* out = reinterpret_cast<dim3_count_row*>(a.out);
-                for (uint32_t k = tid; k < nq; k += BLOCK) {
-                    const uint32_t e = stage[k];
-                    const uint64_t g = gb + k;
-                    if (g < a.cap) {
+                for (uint32_t k = lane; k < nq; k += 64u) {
+                    const uint32_t e = wst[k];
+                    if (g0 + k < a.cap) {
                         dim3_count_row row;
                         row.x = xv;
-                        row.z = id2z[e];
+                        row.z = ZID ? (int32_t)(zoff + e) : id2z[e];
                         row.count = (int64_t)tile[e];
-                        out[g] = row;
+                        out[g0 + k] = row;
                     }
                 }
             } else {
                 int2* out = reinterpret_cast<int2*>(a.out);
-                uint32_t k = tid;
-                for (; k + 3u * BLOCK < nq; k += 4u * BLOCK) {
-                    const int32_t z0 = id2z[stage[k]];
-                    const int32_t z1 = id2z[stage[k + BLOCK]];
-                    const int32_t z2 = id2z[stage[k + 2u * BLOCK]];
-                    const int32_t z3 = id2z[stage[k + 3u * BLOCK]];
-                    if (gb + k + 3u * BLOCK < a.cap) {
-                        out[gb + k] = make_int2(xv, z0);
-                        out[gb + k + BLOCK] = make_int2(xv, z1);
-                        out[gb + k + 2u * BLOCK] = make_int2(xv, z2);
-                        out[gb + k + 3u * BLOCK] = make_int2(xv, z3);
-                    } else {
-                        if (gb + k < a.cap) out[gb + k] = make_int2(xv, z0);
-                        if (gb + k + BLOCK < a.cap) out[gb + k + BLOCK] = make_int2(xv, z1);
-                        if (gb + k + 2u * BLOCK < a.cap) out[gb + k + 2u * BLOCK] = make_int2(xv, z2);
+                if (g0 + nq <= a.cap) {
+                    for (uint32_t k = lane; k < nq; k += 64u) {
+                        const uint32_t e = wst[k];
+                        out[g0 + k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                    }
+                } else {
+                    for (uint32_t k = lane; k < nq; k += 64u) {
+                        const uint32_t e = wst[k];
+                        if (g0 + k < a.cap) out[g0 + k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                     }
                 }
-                for (; k < nq; k += BLOCK) {
-                    const uint64_t g = gb + k;
-                    if (g < a.cap) out[g] = make_int2(xv, id2z[stage[k]]);
-                }
             }
-            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         for (uint32_t i = lo; i < hi; i++) tile[list_at(touched, listed, i)] = 0;
         __syncthreads();
@@ -328,15 +360,21 @@ int launch_persistent(K kern, int block, size_t lds, const ExpandArgs& a, int cu
     return (int)hipGetLastError();
 }
 
+template <bool COUNT, bool ZID>
+int launch_z(const ExpandArgs& a, int block, size_t lds, int cu_count, hipStream_t st) {
+    switch (block) {
+        case 256: return launch_persistent(k_expand<256, COUNT, ZID>, 256, lds, a, cu_count, st);
+        case 512: return launch_persistent(k_expand<512, COUNT, ZID>, 512, lds, a, cu_count, st);
+        case 1024: return launch_persistent(k_expand<1024, COUNT, ZID>, 1024, lds, a, cu_count, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
 template <bool COUNT>
 int launch(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
     const size_t lds = dim3h::expand_lds_bytes(a.tile_words, a.touched_cap, a.stage_cap, (uint32_t)block);
-    switch (block) {
-        case 256: return launch_persistent(k_expand<256, COUNT>, 256, lds, a, cu_count, st);
-        case 512: return launch_persistent(k_expand<512, COUNT>, 512, lds, a, cu_count, st);
-        case 1024: return launch_persistent(k_expand<1024, COUNT>, 1024, lds, a, cu_count, st);
-        default: return (int)hipErrorInvalidValue;
-    }
+    if (a.id2z == nullptr) return launch_z<COUNT, true>(a, block, lds, cu_count, st);
+    return launch_z<COUNT, false>(a, block, lds, cu_count, st);
 }
 
 }  // namespace

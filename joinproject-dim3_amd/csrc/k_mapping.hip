@@ -6,12 +6,21 @@
 // divide_R / divide_S (dim3.hpp:1058-1086, 1186-1253).
 //
 // Instead of radix partitions sized for a CPU L2, every attribute gets one
-// device-wide open-addressing table (u32 keys, CAS insert, linear probing,
-// load <= 1/2).  Dense IDs are the rank of the occupied slot, obtained by an
-// exclusive scan over the slots, so no ID counter is contended.  The same
-// scan of (occupied << 32 | degree) yields the CSR row starts.  Degree
-// histograms come from atomics on the slot counters.  All kernels here are
-// HBM-bound streaming passes over the tuples or the slots.
+// device-wide slot space (a KeyMap):
+//   - hash mode: an open-addressing table (u32 keys, CAS insert, linear
+//     probing, load <= 1/2), for sparse value domains;
+//   - range mode: slot = value - min, for dense integer domains (the value
+//     range is at most twice the tuple count), with no table at all.
+// Dense IDs are the rank of the occupied slot, obtained by an exclusive scan
+// over the slots, so no ID counter is contended, and the same scan of
+// (occupied << 32 | degree) yields the CSR row starts.  Output z values can
+// also use the range slot itself as their ID (z identity), so emission needs
+// no id -> value lookup.  Degree histograms come from atomics on the slot
+// counters.  All kernels here are HBM-bound streaming passes over the tuples
+// or the slots.
+#include <algorithm>
+#include <climits>
+
 #include "dim3_internal.h"
 
 namespace {
@@ -44,36 +53,94 @@ __global__ void k_ht_init(uint32_t* keys, uint32_t cap) {
     else if (i == cap) keys[i] = 0;
 }
 
+
+__device__ __forceinline__ uint32_t slot_insert(const dim3h::KeyMap& m, int32_t v) {
+    if (m.range) return (uint32_t)v - (uint32_t)m.lo;
+    return ht_insert(m.keys, m.mask, (uint32_t)v);
+}
+
+__device__ __forceinline__ int32_t slot_find(const dim3h::KeyMap& m, int32_t v) {
+    if (m.range) {
+        const uint32_t s = (uint32_t)v - (uint32_t)m.lo;
+        return s < m.nslots ? (int32_t)s : -1;
+    }
+    return ht_find(m.keys, m.mask, (uint32_t)v);
+}
+
+__device__ __forceinline__ int32_t key_of(const dim3h::KeyMap& m, uint32_t s) {
+    if (m.range) return (int32_t)((uint32_t)m.lo + s);
+    return ht_key(m.keys, m.mask + 1u, s);
+}
+
+__global__ void k_minmax_init(int32_t* mm) {
+    if (threadIdx.x < 8) mm[threadIdx.x] = (threadIdx.x & 1) ? INT32_MIN : INT32_MAX;
+}
+
+// min / max of R.x, R.y, S.z, S.y (decides hash or range mode per attribute)
+__global__ __launch_bounds__(kThreads) void k_minmax(const int2* __restrict__ V, uint64_t n, int32_t* mm) {
+    int32_t a0 = INT32_MAX, a1 = INT32_MIN, b0 = INT32_MAX, b1 = INT32_MIN;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads) {
+        const int2 t = V[i];
+        a0 = min(a0, t.x);
+        a1 = max(a1, t.x);
+        b0 = min(b0, t.y);
+        b1 = max(b1, t.y);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        a0 = min(a0, __shfl_xor(a0, d, 64));
+        a1 = max(a1, __shfl_xor(a1, d, 64));
+        b0 = min(b0, __shfl_xor(b0, d, 64));
+        b1 = max(b1, __shfl_xor(b1, d, 64));
+    }
+    if (lane_id() == 0) {
+        atomicMin(&mm[0], a0);
+        atomicMax(&mm[1], a1);
+        atomicMin(&mm[2], b0);
+        atomicMax(&mm[3], b1);
+    }
+}
+
 // mapping_y insert side (dim3.hpp:822-828): every R.y gets a slot; degR counts it.
-__global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, uint32_t* keys, uint32_t mask,
-                          uint32_t* degR, uint32_t* __restrict__ slotR) {
+__global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, dim3h::KeyMap m, uint32_t* degR,
+                          uint32_t* __restrict__ slotR) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
-    uint32_t s = ht_insert(keys, mask, (uint32_t)R[i].y);
+    const uint32_t s = slot_insert(m, R[i].y);
     atomicAdd(&degR[s], 1u);
     slotR[i] = s;
 }
 
 // mapping_y probe side (dim3.hpp:830-836): S tuples whose y is not in R are
-// dropped (slot -1); degS counts the survivors.
-__global__ void k_probe_y(const int2* __restrict__ S, uint64_t nS, const uint32_t* __restrict__ keys,
-                          uint32_t mask, uint32_t* degS, int32_t* __restrict__ slotS) {
+// dropped (slot -1); degS counts the survivors, *surv their number.
+__global__ __launch_bounds__(kThreads) void k_probe_y(const int2* __restrict__ S, uint64_t nS, dim3h::KeyMap m,
+                                                      const uint32_t* __restrict__ degR, uint32_t* degS,
+                                                      int32_t* __restrict__ slotS, uint64_t* surv) {
+    __shared__ uint64_t sh[kThreads / 64 + 1];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nS) return;
-    int32_t s = ht_find(keys, mask, (uint32_t)S[i].y);
-    slotS[i] = s;
-    if (s >= 0) atomicAdd(&degS[s], 1u);
+    uint64_t ok = 0;
+    if (i < nS) {
+        int32_t s = slot_find(m, S[i].y);
+        if (s >= 0 && m.range && degR[s] == 0) s = -1;
+        slotS[i] = s;
+        if (s >= 0) {
+            atomicAdd(&degS[s], 1u);
+            ok = 1;
+        }
+    }
+    ok = block_reduce_sum<kThreads>(ok, sh);
+    if (threadIdx.x == 0 && ok) atomicAdd((unsigned long long*)surv, (unsigned long long)ok);
 }
 
 // mapping_xz over R.x (dim3.hpp:749-755), restricted to tuples whose y joined.
 __global__ void k_build_x(const int2* __restrict__ R, uint64_t nR, const uint32_t* __restrict__ slotR,
-                          const uint32_t* __restrict__ degS_y, uint32_t* keys, uint32_t mask, uint32_t* degX,
+                          const uint32_t* __restrict__ degS_y, dim3h::KeyMap m, uint32_t* degX,
                           int32_t* __restrict__ slotRx) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
     int32_t out = -1;
     if (degS_y[slotR[i]] > 0) {
-        uint32_t s = ht_insert(keys, mask, (uint32_t)R[i].x);
+        const uint32_t s = slot_insert(m, R[i].x);
         atomicAdd(&degX[s], 1u);
         out = (int32_t)s;
     }
@@ -82,12 +149,12 @@ __global__ void k_build_x(const int2* __restrict__ R, uint64_t nR, const uint32_
 
 // mapping_xz over the surviving S.z.
 __global__ void k_build_z(const int2* __restrict__ S, uint64_t nS, const int32_t* __restrict__ slotS,
-                          uint32_t* keys, uint32_t mask, uint32_t* degZ, int32_t* __restrict__ slotSz) {
+                          dim3h::KeyMap m, uint32_t* degZ, int32_t* __restrict__ slotSz) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     int32_t out = -1;
     if (slotS[i] >= 0) {
-        uint32_t s = ht_insert(keys, mask, (uint32_t)S[i].x);
+        const uint32_t s = slot_insert(m, S[i].x);
         atomicAdd(&degZ[s], 1u);
         out = (int32_t)s;
     }
@@ -203,15 +270,15 @@ __global__ __launch_bounds__(kThreads) void k_outj(const uint32_t* __restrict__ 
     if (threadIdx.x == 0 && acc) atomicAdd((unsigned long long*)out_j, (unsigned long long)acc);
 }
 
-__global__ void k_compact_ids(const uint32_t* __restrict__ keys, uint32_t cap, const uint32_t* __restrict__ deg,
-                              const uint64_t* __restrict__ ex, int32_t* __restrict__ id2v,
-                              uint32_t* __restrict__ rowptr) {
+
+__global__ void k_compact_ids(dim3h::KeyMap m, const uint32_t* __restrict__ deg, const uint64_t* __restrict__ ex,
+                              int32_t* __restrict__ id2v, uint32_t* __restrict__ rowptr) {
     uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > cap) return;
+    if (s >= m.nslots) return;
     if (deg[s] == 0) return;
     const uint64_t e = ex[s];
     const uint32_t id = (uint32_t)(e >> 32);
-    id2v[id] = ht_key(keys, cap, (uint32_t)s);
+    id2v[id] = key_of(m, (uint32_t)s);
     if (rowptr) rowptr[id] = (uint32_t)e;
 }
 
@@ -233,12 +300,20 @@ __global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* _
     col[pos] = (int32_t)(ex_y[slotR[i]] >> 32);
 }
 
+// z ID of a surviving S tuple: the range slot itself (z identity) or the
+// rank of its slot.
+__device__ __forceinline__ uint32_t z_id(const int2* S, uint64_t i, const int32_t* slotSz, const uint64_t* ex_z,
+                                         int32_t zlo) {
+    if (ex_z == nullptr) return (uint32_t)S[i].x - (uint32_t)zlo;
+    return (uint32_t)(ex_z[slotSz[i]] >> 32);
+}
+
 // divide_S (dim3.hpp:1186-1253): CSR of S keyed by (y, z-tile) so that the
 // expansion of one z tile reads only its own column range.
-__global__ void k_count_s(const int32_t* __restrict__ slotS, const int32_t* __restrict__ slotSz, uint64_t nS,
-                          const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_z, uint32_t ntiles,
-                          uint32_t tile_size, uint32_t* cnt, uint32_t* __restrict__ keyS,
-                          int32_t* __restrict__ zidS) {
+__global__ void k_count_s(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
+                          const int32_t* __restrict__ slotSz, uint64_t nS, const uint64_t* __restrict__ ex_y,
+                          const uint64_t* __restrict__ ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size,
+                          uint32_t* cnt, uint32_t* __restrict__ keyS, int32_t* __restrict__ zidS) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const int32_t sy = slotS[i];
@@ -247,7 +322,7 @@ __global__ void k_count_s(const int32_t* __restrict__ slotS, const int32_t* __re
         return;
     }
     const uint32_t yid = (uint32_t)(ex_y[sy] >> 32);
-    const uint32_t zid = (uint32_t)(ex_z[slotSz[i]] >> 32);
+    const uint32_t zid = z_id(S, i, slotSz, ex_z, zlo);
     const uint32_t key = ntiles == 1 ? yid : yid * ntiles + zid / tile_size;
     atomicAdd(&cnt[key], 1u);
     keyS[i] = key;
@@ -264,13 +339,19 @@ __global__ void k_scatter_s(const uint32_t* __restrict__ keyS, const int32_t* __
     col[pos] = zidS[i];
 }
 
-__global__ void k_x2z_self(const int32_t* __restrict__ id2x, uint32_t nx, const uint32_t* __restrict__ keysZ,
-                           uint32_t maskZ, const uint32_t* __restrict__ degZ, const uint64_t* __restrict__ ex_z,
+// z ID of each x value, or -1 (self-pair filter of a self-join)
+__global__ void k_x2z_self(const int32_t* __restrict__ id2x, uint32_t nx, dim3h::KeyMap mz,
+                           const uint32_t* __restrict__ degZ, const uint64_t* __restrict__ ex_z,
                            int32_t* __restrict__ x2z) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nx) return;
-    const int32_t s = ht_find(keysZ, maskZ, (uint32_t)id2x[i]);
-    x2z[i] = (s >= 0 && degZ[s] > 0) ? (int32_t)(ex_z[s] >> 32) : -1;
+    const int32_t s = slot_find(mz, id2x[i]);
+    int32_t r = -1;
+    if (s >= 0) {
+        if (ex_z == nullptr) r = s;  // z identity
+        else if (degZ[s] > 0) r = (int32_t)(ex_z[s] >> 32);
+    }
+    x2z[i] = r;
 }
 
 }  // namespace
@@ -285,47 +366,51 @@ void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
 void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(kThreads), 0, st, p, n, v);
 }
-void ht_init(uint32_t* keys, uint32_t cap, hipStream_t st) {
-    hipLaunchKernelGGL(k_ht_init, dim3(grid_for((uint64_t)cap + 1)), dim3(kThreads), 0, st, keys, cap);
+void map_init(const KeyMap& m, hipStream_t st) {
+    if (!m.range) hipLaunchKernelGGL(k_ht_init, dim3(grid_for((uint64_t)m.mask + 2)), dim3(kThreads), 0, st, m.keys,
+                                     m.mask + 1u);
 }
-
-void build_y(const int2* R, uint64_t nR, uint32_t* keys, uint32_t mask, uint32_t* degR, uint32_t* slotR,
-             hipStream_t st) {
-    if (nR) hipLaunchKernelGGL(k_build_y, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, keys, mask, degR, slotR);
+void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm, hipStream_t st) {
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, mm);
+    unsigned gr = std::min<unsigned>(grid_for(nR), 2048u), gs = std::min<unsigned>(grid_for(nS), 2048u);
+    if (nR) hipLaunchKernelGGL(k_minmax, dim3(gr), dim3(kThreads), 0, st, R, nR, mm);
+    if (nS) hipLaunchKernelGGL(k_minmax, dim3(gs), dim3(kThreads), 0, st, S, nS, mm + 4);
 }
-void probe_y(const int2* S, uint64_t nS, const uint32_t* keys, uint32_t mask, uint32_t* degS, int32_t* slotS,
-             hipStream_t st) {
-    if (nS) hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, keys, mask, degS, slotS);
+void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st) {
+    if (nR) hipLaunchKernelGGL(k_build_y, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, m, degR, slotR);
 }
-void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* ex, uint64_t* partials,
-            uint64_t* tot, uint64_t* out_j, hipStream_t st) {
-    scan_exclusive(ValJoinedY{degS}, OutU64{ex}, cap1, partials, tot, st);
-    outj_only(degR, degS, cap1, out_j, st);
-}
-void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t cap1, uint64_t* out_j, hipStream_t st) {
-    unsigned g = grid_for(cap1);
-    if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_outj, dim3(g), dim3(kThreads), 0, st, degR, degS, cap1, out_j);
-}
-void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, uint32_t* keys,
-             uint32_t mask, uint32_t* degX, int32_t* slotRx, hipStream_t st) {
-    if (nR)
-        hipLaunchKernelGGL(k_build_x, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, degS_y, keys, mask,
-                           degX, slotRx);
-}
-void build_z(const int2* S, uint64_t nS, const int32_t* slotS, uint32_t* keys, uint32_t mask, uint32_t* degZ,
-             int32_t* slotSz, hipStream_t st) {
+void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, uint32_t* degS, int32_t* slotS,
+             uint64_t* surv, hipStream_t st) {
     if (nS)
-        hipLaunchKernelGGL(k_build_z, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, keys, mask, degZ,
-                           slotSz);
+        hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, m, degR, degS, slotS, surv);
 }
-void scan_deg(const uint32_t* deg, uint32_t cap1, uint64_t* ex, uint64_t* partials, uint64_t* tot, hipStream_t st) {
-    scan_exclusive(ValDeg{deg}, OutU64{ex}, cap1, partials, tot, st);
+void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* ex, uint64_t* partials,
+            uint64_t* tot, uint64_t* out_j, hipStream_t st) {
+    scan_exclusive(ValJoinedY{degS}, OutU64{ex}, nslots, partials, tot, st);
+    outj_only(degR, degS, nslots, out_j, st);
 }
-void compact_ids(const uint32_t* keys, uint32_t cap, const uint32_t* deg, const uint64_t* ex, int32_t* id2v,
-                 uint32_t* rowptr, hipStream_t st) {
-    hipLaunchKernelGGL(k_compact_ids, dim3(grid_for((uint64_t)cap + 1)), dim3(kThreads), 0, st, keys, cap, deg, ex,
-                       id2v, rowptr);
+void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* out_j, hipStream_t st) {
+    unsigned g = grid_for(nslots);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_outj, dim3(g), dim3(kThreads), 0, st, degR, degS, nslots, out_j);
+}
+void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
+             uint32_t* degX, int32_t* slotRx, hipStream_t st) {
+    if (nR)
+        hipLaunchKernelGGL(k_build_x, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, degS_y, m, degX,
+                           slotRx);
+}
+void build_z(const int2* S, uint64_t nS, const int32_t* slotS, const KeyMap& m, uint32_t* degZ, int32_t* slotSz,
+             hipStream_t st) {
+    if (nS) hipLaunchKernelGGL(k_build_z, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, m, degZ, slotSz);
+}
+void scan_deg(const uint32_t* deg, uint32_t nslots, uint64_t* ex, uint64_t* partials, uint64_t* tot,
+              hipStream_t st) {
+    scan_exclusive(ValDeg{deg}, OutU64{ex}, nslots, partials, tot, st);
+}
+void compact_ids(const KeyMap& m, const uint32_t* deg, const uint64_t* ex, int32_t* id2v, uint32_t* rowptr,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(k_compact_ids, dim3(grid_for(m.nslots)), dim3(kThreads), 0, st, m, deg, ex, id2v, rowptr);
 }
 void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_t st) {
     hipLaunchKernelGGL(k_finish_rowptr, dim3(1), dim3(64), 0, st, rowptr, n, tot);
@@ -336,11 +421,12 @@ void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const 
         hipLaunchKernelGGL(k_scatter_r, dim3(grid_for(nR)), dim3(kThreads), 0, st, slotR, slotRx, nR, ex_y, ex_x,
                            cursor, col);
 }
-void count_s(const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y, const uint64_t* ex_z,
-             uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
+void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
+             const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS,
+             int32_t* zidS, hipStream_t st) {
     if (nS)
-        hipLaunchKernelGGL(k_count_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, slotS, slotSz, nS, ex_y, ex_z,
-                           ntiles, tile_size, cnt, keyS, zidS);
+        hipLaunchKernelGGL(k_count_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, slotS, slotSz, nS, ex_y, ex_z,
+                           zlo, ntiles, tile_size, cnt, keyS, zidS);
 }
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st) {
@@ -351,11 +437,10 @@ void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t*
     if (nS)
         hipLaunchKernelGGL(k_scatter_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, keyS, zidS, nS, cursor, col);
 }
-void x2z_self(const int32_t* id2x, uint32_t nx, const uint32_t* keysZ, uint32_t maskZ, const uint32_t* degZ,
-              const uint64_t* ex_z, int32_t* x2z, hipStream_t st) {
+void x2z_self(const int32_t* id2x, uint32_t nx, const KeyMap& mz, const uint32_t* degZ, const uint64_t* ex_z,
+              int32_t* x2z, hipStream_t st) {
     if (nx)
-        hipLaunchKernelGGL(k_x2z_self, dim3(grid_for(nx)), dim3(kThreads), 0, st, id2x, nx, keysZ, maskZ, degZ,
-                           ex_z, x2z);
+        hipLaunchKernelGGL(k_x2z_self, dim3(grid_for(nx)), dim3(kThreads), 0, st, id2x, nx, mz, degZ, ex_z, x2z);
 }
 
 }  // namespace dim3h

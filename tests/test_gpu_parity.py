@@ -76,19 +76,48 @@ def test_random_small_vs_oracle(ctx, seed):
     bmax = int(rng.integers(1, 500))
     R = rand_rel(rng, nR, amax, bmax, -int(rng.integers(0, 3)), -int(rng.integers(0, 3)))
     S = rand_rel(rng, nS, amax, bmax, -int(rng.integers(0, 3)), -int(rng.integers(0, 3)))
-    kw = {"exclude_equal": bool(seed % 3 == 0), "block": [0, 256, 512, 1024][seed % 4]}
+    kw = {"exclude_equal": bool(seed % 3 == 0), "block": [0, 256, 512, 1024][seed % 4],
+          "hash_only": bool(seed % 2)}
     check_project(ctx, R, S, **kw)
     check_count(ctx, R, S, **kw)
 
 
-@pytest.mark.parametrize("tile_bits", [32, 64, 1024, 4096])
+@pytest.mark.parametrize("seed", range(4))
+def test_sparse_value_domains_use_hashing(ctx, seed):
+    # values drawn from a few thousand random int32s: ranges far wider than the
+    # tuple counts, so every attribute goes through the hash tables
+    rng = np.random.default_rng(50 + seed)
+    dom = rng.integers(-2**31, 2**31 - 1, 3000).astype(np.int32)
+    dom[0], dom[1] = -1, -2**31
+    R = dom[rng.integers(0, 3000, (6000, 2))]
+    S = dom[rng.integers(0, 3000, (5000, 2))]
+    R[:, 1] = dom[rng.integers(0, 200, 6000)]
+    S[:, 1] = dom[rng.integers(0, 200, 5000)]
+    check_project(ctx, R, S, exclude_equal=bool(seed % 2))
+    check_count(ctx, R, S)
+
+
+def test_hash_and_range_mapping_agree(ctx):
+    rng = np.random.default_rng(77)
+    R = rand_rel(rng, 40000, 3000, 800, -100, -5)
+    S = rand_rel(rng, 30000, 3000, 800, -100, -5)
+    a = ctx.join_project(R, S, output_mode=1).fp.as_tuple()
+    b = ctx.join_project(R, S, output_mode=1, hash_only=True).fp.as_tuple()
+    assert a == b == oracle.join_project(R, S, fingerprint_only=True)[1].as_tuple()
+    a = ctx.join_aggregate_count(R, S, output_mode=1).fp.as_tuple()
+    b = ctx.join_aggregate_count(R, S, output_mode=1, hash_only=True).fp.as_tuple()
+    assert a == b == oracle.join_aggregate_count(R, S, fingerprint_only=True)[2].as_tuple()
+
+
+@pytest.mark.parametrize("tile_bits", [32, 64, 1000, 4096])
 def test_z_tiling(ctx, tile_bits):
     rng = np.random.default_rng(tile_bits)
     R = rand_rel(rng, 20000, 500, 300)
     S = rand_rel(rng, 20000, 20000, 300)
-    check_project(ctx, R, S, tile_bits=tile_bits)
-    check_count(ctx, R, S, tile_bits=tile_bits)
-    check_project(ctx, R, S, tile_bits=tile_bits, exclude_equal=True)
+    for h in (False, True):
+        check_project(ctx, R, S, tile_bits=tile_bits, hash_only=h)
+        check_count(ctx, R, S, tile_bits=tile_bits, hash_only=h)
+        check_project(ctx, R, S, tile_bits=tile_bits, exclude_equal=True, hash_only=h)
 
 
 def test_edge_cases(ctx):
