@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--dense", type=int, default=-1, help="dense mode: -1 auto, 0 off, 1 bit-or, 2 mfma")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--hash-only", action="store_true", help="hash every attribute (no dense-range mapping)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-slices", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -117,7 +118,7 @@ def main():
     dE = torch.from_numpy(E).to(dev)
     torch.cuda.synchronize()
     kw = dict(exclude_equal=True, dense_mode=args.dense, block=args.block, tile_bits=args.tile_bits,
-              output_mode=1 if args.mode == "fingerprint" else 0)
+              output_mode=1 if args.mode == "fingerprint" else 0, hash_only=args.hash_only)
 
     # size the output: a capacity-0 call reports the exact row count
     if args.mode == "materialize":

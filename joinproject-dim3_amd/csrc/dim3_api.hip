@@ -285,14 +285,17 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     uint32_t *keysY = nullptr, *degR, *degS, *slotR, *keysX = nullptr, *degX, *keysZ = nullptr, *degZ = nullptr;
     int32_t *slotS, *slotRx, *slotSz = nullptr;
     uint64_t *exY, *exX, *exZ = nullptr, *part;
-    auto make_map = [&](bool range, int32_t lo, uint64_t span_v, uint64_t n, SlotId keys_slot, uint32_t** keys,
-                        dim3h::KeyMap* m) -> int {
+    auto make_map = [&](bool range, bool identity, int32_t lo, uint64_t span_v, uint64_t n, SlotId keys_slot,
+                        uint32_t** keys, dim3h::KeyMap* m) -> int {
         m->range = range ? 1u : 0u;
         m->lo = lo;
         m->keys = nullptr;
         m->mask = 0;
+        m->span = range ? (uint32_t)span_v : 0u;
         if (range) {
-            m->nslots = (uint32_t)span_v;
+            const uint32_t p2 = pow2_at_least(span_v);
+            m->mask = identity ? 0u : p2 - 1u;
+            m->nslots = identity ? (uint32_t)span_v : p2;
         } else {
             const uint32_t cap = pow2_at_least(2 * n);
             TRY(ws(c, keys_slot, (uint64_t)cap + 1, keys));
@@ -302,9 +305,9 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         }
         return DIM3_OK;
     };
-    TRY(make_map(P.y_range, mm[2], yr, nR, WS_KEYS_Y, &keysY, &P.my));
-    TRY(make_map(P.x_range, mm[0], xr, nR, WS_KEYS_X, &keysX, &P.mx));
-    TRY(make_map(P.z_ident, mm[4], zr, nS, WS_KEYS_Z, &keysZ, &P.mz));
+    TRY(make_map(P.y_range, false, mm[2], yr, nR, WS_KEYS_Y, &keysY, &P.my));
+    TRY(make_map(P.x_range, false, mm[0], xr, nR, WS_KEYS_X, &keysX, &P.mx));
+    TRY(make_map(P.z_ident, true, mm[4], zr, nS, WS_KEYS_Z, &keysZ, &P.mz));
     const uint64_t ny1 = P.my.nslots, nx1 = P.mx.nslots, nz1 = P.mz.nslots;
     TRY(ws(c, WS_DEGR_Y, ny1, &degR));
     TRY(ws(c, WS_DEGS_Y, ny1, &degS));
@@ -699,7 +702,7 @@ int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const i
     TRY(ws(c, WS_SLOT_S, nS, &slotS));
     TRY(ws(c, WS_SCALARS, 1, &sc));
     HIPCHK(c, hipMemsetAsync(sc, 0, sizeof(Scalars), st));
-    dim3h::KeyMap m{keys, cap - 1, 0, cap + 1, 0};
+    dim3h::KeyMap m{keys, cap - 1, 0, cap + 1, 0, 0};
     dim3h::map_init(m, st);
     dim3h::fill_u32(degR, (uint64_t)cap + 1, 0, st);
     dim3h::fill_u32(degS, (uint64_t)cap + 1, 0, st);

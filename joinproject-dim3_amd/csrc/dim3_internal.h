@@ -187,13 +187,15 @@ struct Scalars {
 size_t scan_partials_needed(uint64_t n);
 
 // Slot space of one attribute: an open-addressing table (hash mode) or the
-// value range itself (range mode, slot = value - lo).
+// value range itself (range mode: slot = (value - lo) * odd mod 2^k, or
+// value - lo when mask == 0).
 struct KeyMap {
     uint32_t* keys;   // hash mode: keys[mask + 2] (last slot stands for the key -1)
-    uint32_t mask;    // hash mode: capacity - 1
+    uint32_t mask;    // hash mode: capacity - 1; range mode: 2^k - 1 (0 = identity slots)
     int32_t lo;       // range mode: smallest value
-    uint32_t nslots;  // slots to scan: mask + 2 (hash) or hi - lo + 1 (range)
+    uint32_t nslots;  // slots to scan: mask + 2 (hash), 2^k or span (range)
     uint32_t range;   // 1 = range mode
+    uint32_t span;    // range mode: hi - lo + 1
 };
 
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);

@@ -18,12 +18,10 @@
 //     in an LDS list with one LDS atomic per wavefront (ballot + mbcnt), so
 //     sparse rows emit from the touched words only; rows with many paths (or
 //     an overflowing list) sweep their tile instead;
-//   - emission splits the list over the wavefronts, reserves the row's
-//     output range with one global atomic, and each wavefront stages its
-//     local z offsets in a private LDS slice and writes (x, z) rows with
-//     coalesced stores (no block barrier between staging and writing); z
-//     values are zlo + ID for dense domains, else one id -> value read;
-//     the tile is cleared on the way out.
+//   - emission counts the row, reserves its output range with one global
+//     atomic, stages the row's local z offsets in LDS and writes (x, z) rows
+//     with coalesced stores; z values are zlo + ID for dense domains (no
+//     lookup), else one id -> value read; the tile is cleared on the way out.
 // Per join path: one 4-byte column read (mostly L2/MALL hits, as heavy S rows
 // are re-read by every x that reaches them) and one LDS atomic.  Per output
 // row: one 8-byte (16-byte for COUNT) store.
@@ -54,6 +52,40 @@ __device__ __forceinline__ uint64_t block_reduce_xor64(uint64_t v, uint64_t* sh)
     return r;
 }
 
+// Exclusive scan over the block with two barriers; sh needs NW + 1 words and
+// must not be reused before the next barrier.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_scan2_u32(uint32_t v, uint32_t& total, uint32_t* sh) {
+    constexpr uint32_t NW = BLOCK / 64;
+    const uint32_t wid = threadIdx.x >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane_id() == 63) sh[wid] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; w++) {
+        const uint32_t x = sh[w];
+        before += w < wid ? x : 0u;
+        tot += x;
+    }
+    total = tot;
+    return before + inc - v;
+}
+
+// Sum over the block (two barriers, sh reusable afterwards).
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
+    constexpr uint32_t NW = BLOCK / 64;
+    v = wave_reduce_sum(v);
+    if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t r = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; w++) r += sh[w];
+    __syncthreads();
+    return r;
+}
+
 // Records the first touches of one wavefront instruction in the LDS list.
 __device__ __forceinline__ void push_first_touch(bool first, uint32_t item, uint32_t* s_nt, uint32_t* touched,
                                                  uint32_t cap) {
@@ -71,16 +103,32 @@ __device__ __forceinline__ void push_first_touch(bool first, uint32_t item, uint
     }
 }
 
-// One join path into the tile.  Project: set bit zl; COUNT: add 1 to counter zl.
+// Project tiles store local z offset zl = 32 q + r in bit r of word
+// (q + r * C) mod W (W words, C = W / 32): the 32 offsets that share a word
+// in the plain layout land in 32 different words and LDS banks, so hot z
+// values with neighbouring IDs (hubs of a graph numbered by degree) do not
+// serialise on one word.
+__device__ __forceinline__ uint32_t swz_word(uint32_t zl, uint32_t W, uint32_t C) {
+    const uint32_t q = (zl >> 5) + (zl & 31u) * C;
+    return q >= W ? q - W : q;
+}
+
+__device__ __forceinline__ uint32_t swz_offset(uint32_t word, uint32_t r, uint32_t W, uint32_t C) {
+    int32_t q = (int32_t)word - (int32_t)(r * C);
+    if (q < 0) q += (int32_t)W;
+    return (uint32_t)q * 32u + r;
+}
+
+// One join path into the tile.  Project: set the bit of zl; COUNT: add 1 to counter zl.
 template <bool COUNT>
-__device__ __forceinline__ bool apply_path(uint32_t* tile, uint32_t zl) {
+__device__ __forceinline__ bool apply_path(uint32_t* tile, uint32_t zl, uint32_t W, uint32_t C) {
     if (COUNT) return atomicAdd(&tile[zl], 1u) == 0u;
-    return atomicOr(&tile[zl >> 5], 1u << (zl & 31u)) == 0u;
+    return atomicOr(&tile[swz_word(zl, W, C)], 1u << (zl & 31u)) == 0u;
 }
 
 template <bool COUNT>
-__device__ __forceinline__ uint32_t list_item(uint32_t zl) {
-    return COUNT ? zl : (zl >> 5);
+__device__ __forceinline__ uint32_t list_item(uint32_t zl, uint32_t W, uint32_t C) {
+    return COUNT ? zl : swz_word(zl, W, C);
 }
 
 template <bool COUNT>
@@ -90,8 +138,8 @@ __device__ __forceinline__ uint32_t units_of(uint32_t v) {
 
 // LDS layout (u32 words):
 //   tile[tile_words] | touched[touched_cap] | area[max(stage_cap, 3*BLOCK+1)]
-// `area` holds the segment table of the row during expansion and, during
-// emission, one private staging slice per wavefront.
+// `area` holds the segment table of the row during expansion and the staged
+// local z offsets during emission.
 // ZID: z IDs are value - zlo (dense domain), so no id -> value lookup.
 template <int BLOCK, bool COUNT, bool ZID>
 __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
@@ -103,21 +151,22 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     uint32_t* s_sb = area;
     uint32_t* s_len = area + BLOCK;
     uint32_t* s_pre = area + 2 * BLOCK;
-    const uint32_t wsc = a.stage_cap / NW;  // staging slice per wavefront
+    uint32_t* stage = area;
     __shared__ uint64_t sh[NW + 1];
-    __shared__ uint32_t sh32[NW + 1];
-    __shared__ uint32_t s_wtot[NW];
+    __shared__ uint32_t sh32[2 * (NW + 1)];
     __shared__ uint32_t s_task, s_nt;
     __shared__ uint64_t s_gbase;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t ntasks = a.nx * a.ntiles;
-    const uint64_t track_limit = 4ull * a.tile_words;  // above this many paths a sweep is cheaper
+    const uint32_t W = a.tile_words;
+    const uint32_t C = COUNT ? 0u : W / 32u;  // bit swizzle stride of project tiles
+    // rows with more paths than this sweep their tile instead of listing first touches
+    const uint64_t track_limit = min(4ull * a.touched_cap, 4ull * a.tile_words);
 
     for (uint32_t w = tid; w < a.tile_words; w += BLOCK) tile[w] = 0;
     uint64_t fsum = 0, fxr = 0, fcs = 0, npaths = 0;
-    uint32_t* wst = area + wid * wsc;
 
     for (;;) {
         if (tid == 0) {
@@ -125,12 +174,16 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             s_nt = 0;
         }
         __syncthreads();
-        const uint32_t task = s_task;
+        // task, x and the per-row values are wavefront-uniform: scalar loads,
+        // issued before the expansion so their latency is hidden
+        const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
         if (task >= ntasks) break;
         const uint32_t x = task / a.ntiles;
         const uint32_t t = task - x * a.ntiles;
         const uint32_t zbase = t * a.tile_size;
         const uint32_t rb = a.rowptrR[x], re = a.rowptrR[x + 1];
+        const int32_t xv = a.id2x[x];
+        const int32_t zs_self = a.x2z != nullptr ? a.x2z[x] : -1;
 
         // ---- expansion ----
         bool track = true;
@@ -171,15 +224,16 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
 #pragma unroll
                     for (uint32_t u = 0; u < kUnroll; u++) {
                         bool first = false;
-                        if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u]);
-                        push_first_touch(first, list_item<COUNT>(zl[u]), &s_nt, touched, a.touched_cap);
+                        if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u], W, C);
+                        push_first_touch(first, first ? list_item<COUNT>(zl[u], W, C) : 0u, &s_nt, touched,
+                                         a.touched_cap);
                     }
                 } else {
 #pragma unroll
                     for (uint32_t u = 0; u < kUnroll; u++) {
                         if (zl[u] != 0xFFFFFFFFu) {
                             if (COUNT) atomicAdd(&tile[zl[u]], 1u);
-                            else atomicOr(&tile[zl[u] >> 5], 1u << (zl[u] & 31u));
+                            else atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
                         }
                     }
                 }
@@ -187,24 +241,21 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             __syncthreads();
         }
 
-        // ---- emission ----
+        // ---- emission: each thread owns a contiguous slice of the list ----
         const uint32_t nt = s_nt;
         const bool listed = track && nt <= a.touched_cap;
         const uint32_t count = listed ? nt : a.tile_words;
         uint32_t self_item = 0xFFFFFFFFu, self_keep = 0xFFFFFFFFu;  // u != w filter of the self-join
-        if (a.x2z != nullptr) {
-            const int32_t zs = a.x2z[x];
+        {
+            const int32_t zs = zs_self;
             const uint32_t zl = (uint32_t)zs - zbase;
             if (zs >= 0 && zl < a.tile_size) {
-                self_item = list_item<COUNT>(zl);
+                self_item = list_item<COUNT>(zl, W, C);
                 self_keep = COUNT ? 0u : ~(1u << (zl & 31u));
             }
         }
-        // wavefront slice of the list, then a contiguous lane slice of it
-        const uint32_t wper = (count + NW - 1) / NW;
-        const uint32_t wlo = min(wid * wper, count), whi = min(wlo + wper, count);
-        const uint32_t lper = (whi - wlo + 63u) / 64u;
-        const uint32_t lo = min(wlo + lane * lper, whi), hi = min(lo + lper, whi);
+        const uint32_t per = (count + BLOCK - 1) / BLOCK;
+        const uint32_t lo = min(tid * per, count), hi = min(lo + per, count);
         uint32_t c = 0;
         for (uint32_t i = lo; i < hi; i++) {
             const uint32_t w = list_at(touched, listed, i);
@@ -215,20 +266,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             }
             c += units_of<COUNT>(v);
         }
-        const uint32_t incl = wave_inclusive_scan(c);
-        const uint32_t wtot = __shfl(incl, 63, 64);
-        const uint32_t loff = incl - c;
-        if (lane == 0) s_wtot[wid] = wtot;
-        __syncthreads();
-        uint32_t woff = 0;
-        uint64_t T = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < NW; w++) {
-            const uint32_t v = s_wtot[w];
-            if (w < wid) woff += v;
-            T += v;
-        }
-        const int32_t xv = a.id2x[x];
+        const uint64_t T = block_sum_u64<BLOCK>((uint64_t)c, sh);
         const uint32_t zoff = (uint32_t)a.zlo + zbase;
         const int32_t* id2z = a.id2z + zbase;
 
@@ -249,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     while (v) {
                         const uint32_t b = (uint32_t)__ffs(v) - 1u;
                         v &= v - 1u;
-                        const uint32_t e = w * 32u + b;
+                        const uint32_t e = swz_offset(w, b, W, C);
                         const int32_t zv = ZID ? (int32_t)(zoff + e) : id2z[e];
                         const uint64_t h = mix64(row_key(xv, zv));
                         fsum += h;
@@ -262,44 +300,50 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             continue;
         }
 
-        if (tid == 0) s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
-        __syncthreads();
-        const uint64_t gb = s_gbase + woff;
-        uint32_t ci = lo, cw = 0, cv = 0;
+        // The row's output range is reserved once; its result is consumed
+        // only after the first fill, so neither the atomic's latency nor the
+        // wait it implies (vmcnt also counts earlier stores) is in front of it.
+        uint64_t my_base = 0;
+        if (tid == 0 && T) my_base = atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
+        // P passes; every thread spreads its c rows evenly over them, so all
+        // wavefronts fill in every pass and a pass holds at most
+        // T / P + BLOCK <= stage_cap rows.
+        const uint64_t P = T == 0 ? 0 : (T + (a.stage_cap - BLOCK) - 1) / (a.stage_cap - BLOCK);
+        uint32_t ci = lo, cw = 0, cv = 0, done = 0;
         if (ci < hi) {
             cw = list_at(touched, listed, ci);
             cv = tile[cw];
         }
-        uint32_t r = loff;
-        const uint32_t rend = loff + c;
-        for (uint32_t q = 0; q < wtot; q += wsc) {
-            const uint32_t qe = min(q + wsc, wtot);
-            // fill this wavefront's slice with local z offsets of ranks [q, qe)
-            while (r < qe && r < rend) {
+        uint64_t pass_base = 0;
+        for (uint64_t p = 0; p < P; p++) {
+            const uint32_t upto = (uint32_t)(((p + 1) * (uint64_t)c + P - 1) / P);
+            const uint32_t mine = upto - done;
+            uint32_t Tp;
+            const uint32_t op = block_scan2_u32<BLOCK>(mine, Tp, sh32 + (p & 1) * (NW + 1));
+            for (uint32_t j = 0; j < mine; j++) {
                 while (cv == 0u) {
                     ++ci;
                     cw = list_at(touched, listed, ci);
                     cv = tile[cw];
                 }
                 if (COUNT) {
-                    wst[r - q] = cw;
+                    stage[op + j] = cw;
                     cv = 0;
                 } else {
                     const uint32_t b = (uint32_t)__ffs(cv) - 1u;
                     cv &= cv - 1u;
-                    wst[r - q] = cw * 32u + b;
+                    stage[op + j] = swz_offset(cw, b, W, C);
                 }
-                ++r;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const uint32_t nq = qe - q;
-            const uint64_t g0 = gb + q;
+            done = upto;
+            if (tid == 0 && p == 0) s_gbase = my_base;
+            __syncthreads();
+            const uint64_t g0 = s_gbase + pass_base;
+            const uint32_t nq = Tp;
             if (COUNT) {
                 dim3_count_row* out = reinterpret_cast<dim3_count_row*>(a.out);
-                for (uint32_t k = lane; k < nq; k += 64u) {
-                    const uint32_t e = wst[k];
+                for (uint32_t k = tid; k < nq; k += BLOCK) {
+                    const uint32_t e = stage[k];
                     if (g0 + k < a.cap) {
                         dim3_count_row row;
                         row.x = xv;
@@ -309,22 +353,34 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     }
                 }
             } else {
-                int2* out = reinterpret_cast<int2*>(a.out);
+                int2* out = reinterpret_cast<int2*>(a.out) + g0;
                 if (g0 + nq <= a.cap) {
-                    for (uint32_t k = lane; k < nq; k += 64u) {
-                        const uint32_t e = wst[k];
-                        out[g0 + k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                    uint32_t k = tid;
+                    for (; k + 3u * BLOCK < nq; k += 4u * BLOCK) {
+                        const uint32_t e0 = stage[k], e1 = stage[k + BLOCK];
+                        const uint32_t e2 = stage[k + 2u * BLOCK], e3 = stage[k + 3u * BLOCK];
+                        const int32_t z0 = ZID ? (int32_t)(zoff + e0) : id2z[e0];
+                        const int32_t z1 = ZID ? (int32_t)(zoff + e1) : id2z[e1];
+                        const int32_t z2 = ZID ? (int32_t)(zoff + e2) : id2z[e2];
+                        const int32_t z3 = ZID ? (int32_t)(zoff + e3) : id2z[e3];
+                        out[k] = make_int2(xv, z0);
+                        out[k + BLOCK] = make_int2(xv, z1);
+                        out[k + 2u * BLOCK] = make_int2(xv, z2);
+                        out[k + 3u * BLOCK] = make_int2(xv, z3);
+                    }
+                    for (; k < nq; k += BLOCK) {
+                        const uint32_t e = stage[k];
+                        out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                     }
                 } else {
-                    for (uint32_t k = lane; k < nq; k += 64u) {
-                        const uint32_t e = wst[k];
-                        if (g0 + k < a.cap) out[g0 + k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                    for (uint32_t k = tid; k < nq; k += BLOCK) {
+                        const uint32_t e = stage[k];
+                        if (g0 + k < a.cap) out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                     }
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            pass_base += nq;
+            __syncthreads();
         }
         for (uint32_t i = lo; i < hi; i++) tile[list_at(touched, listed, i)] = 0;
         __syncthreads();

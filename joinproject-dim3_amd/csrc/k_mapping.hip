@@ -54,21 +54,33 @@ __global__ void k_ht_init(uint32_t* keys, uint32_t cap) {
 }
 
 
+// Range slots are scattered by an odd multiplier (a bijection modulo the
+// power-of-two slot count), so that skewed keys with neighbouring values do
+// not pile their degree atomics onto the same cache lines.  With
+// m.span == 0 (z identity) the slot is the offset itself.
+constexpr uint32_t kScatterMul = 0x9E3779B1u;
+constexpr uint32_t kScatterInv = 0x0E8B2F51u;
+static_assert(kScatterMul * kScatterInv == 1u, "kScatterInv must invert kScatterMul modulo 2^32");
+
 __device__ __forceinline__ uint32_t slot_insert(const dim3h::KeyMap& m, int32_t v) {
-    if (m.range) return (uint32_t)v - (uint32_t)m.lo;
+    if (m.range) {
+        const uint32_t off = (uint32_t)v - (uint32_t)m.lo;
+        return m.mask ? (off * kScatterMul) & m.mask : off;
+    }
     return ht_insert(m.keys, m.mask, (uint32_t)v);
 }
 
 __device__ __forceinline__ int32_t slot_find(const dim3h::KeyMap& m, int32_t v) {
     if (m.range) {
-        const uint32_t s = (uint32_t)v - (uint32_t)m.lo;
-        return s < m.nslots ? (int32_t)s : -1;
+        const uint32_t off = (uint32_t)v - (uint32_t)m.lo;
+        if (off >= m.span) return -1;
+        return (int32_t)(m.mask ? (off * kScatterMul) & m.mask : off);
     }
     return ht_find(m.keys, m.mask, (uint32_t)v);
 }
 
 __device__ __forceinline__ int32_t key_of(const dim3h::KeyMap& m, uint32_t s) {
-    if (m.range) return (int32_t)((uint32_t)m.lo + s);
+    if (m.range) return (int32_t)((uint32_t)m.lo + (m.mask ? (s * kScatterInv) & m.mask : s));
     return ht_key(m.keys, m.mask + 1u, s);
 }
 
@@ -93,7 +105,22 @@ __global__ __launch_bounds__(kThreads) void k_minmax(const int2* __restrict__ V,
         b0 = min(b0, __shfl_xor(b0, d, 64));
         b1 = max(b1, __shfl_xor(b1, d, 64));
     }
+    __shared__ int32_t sm[4][kThreads / 64];
+    const uint32_t w = threadIdx.x >> 6;
     if (lane_id() == 0) {
+        sm[0][w] = a0;
+        sm[1][w] = a1;
+        sm[2][w] = b0;
+        sm[3][w] = b1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kThreads / 64; k++) {
+            a0 = min(a0, sm[0][k]);
+            a1 = max(a1, sm[1][k]);
+            b0 = min(b0, sm[2][k]);
+            b1 = max(b1, sm[3][k]);
+        }
         atomicMin(&mm[0], a0);
         atomicMax(&mm[1], a1);
         atomicMin(&mm[2], b0);
@@ -372,7 +399,7 @@ void map_init(const KeyMap& m, hipStream_t st) {
 }
 void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm, hipStream_t st) {
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, mm);
-    unsigned gr = std::min<unsigned>(grid_for(nR), 2048u), gs = std::min<unsigned>(grid_for(nS), 2048u);
+    unsigned gr = std::min<unsigned>(grid_for(nR), 512u), gs = std::min<unsigned>(grid_for(nS), 512u);
     if (nR) hipLaunchKernelGGL(k_minmax, dim3(gr), dim3(kThreads), 0, st, R, nR, mm);
     if (nS) hipLaunchKernelGGL(k_minmax, dim3(gs), dim3(kThreads), 0, st, S, nS, mm + 4);
 }
