@@ -195,10 +195,11 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
         int block;
         uint32_t ntiles, tile_words, stage;
         bool ok;
+        int wgs;
     };
     // candidates: 1, 2 or 4 workgroups per CU (16 wavefronts per CU in all)
     auto make = [&](int wgs, int block, uint64_t forced_units) {
-        Cand k{block, 1, 0, 0, false};
+        Cand k{block, 1, 0, 0, false, wgs};
         const uint64_t budget = lds_words / (uint64_t)wgs;
         const uint64_t min_stage = std::max<uint64_t>(1024u, 3u * (uint64_t)block + 1u);
         if (budget <= P.touched_cap + min_stage + 1) return k;
@@ -220,15 +221,20 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
         return k;
     };
     const uint64_t forced = o.tile_bits > 0 ? (uint64_t)o.tile_bits : 0;
-    Cand best{0, 0, 0, 0, false};
+    Cand best{0, 0, 0, 0, false, 1};
     const int wg_opts[3] = {4, 2, 1};
     const int blk_opts[3] = {256, 512, 1024};
     for (int i = 0; i < 3; i++) {
         const int block = o.block ? o.block : blk_opts[i];
         Cand k = make(wg_opts[i], block, forced);
         if (!k.ok) continue;
-        // fewest tiles first; among equal tile counts the most workgroups per CU
-        if (!best.ok || k.ntiles < best.ntiles) best = k;
+        // Two or more workgroups per CU overlap one row's emission with
+        // another's expansion, which pays for up to twice the tiles (each
+        // tile re-reads the row's y list and its S bounds): score ntiles /
+        // min(workgroups per CU, 2), ties to more workgroups.
+        const double score = (double)k.ntiles / (double)std::min(wg_opts[i], 2);
+        const double best_score = best.ok ? (double)best.ntiles / (double)std::min(best.wgs, 2) : 1e300;
+        if (score < best_score) best = k;
     }
     if (!best.ok) {  // forced tile too big for the candidates: one workgroup per CU
         best = make(1, o.block ? o.block : 1024, 0);

@@ -256,7 +256,9 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         }
         const uint32_t per = (count + BLOCK - 1) / BLOCK;
         const uint32_t lo = min(tid * per, count), hi = min(lo + per, count);
+        // nzmask bit i: word lo + i holds rows (kept when the slice has <= 64 words)
         uint32_t c = 0;
+        uint64_t nzmask = 0;
         for (uint32_t i = lo; i < hi; i++) {
             const uint32_t w = list_at(touched, listed, i);
             uint32_t v = tile[w];
@@ -265,7 +267,9 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                 tile[w] = v;
             }
             c += units_of<COUNT>(v);
+            if (v != 0u && i - lo < 64u) nzmask |= 1ull << (i - lo);
         }
+        const bool use_mask = per <= 64u;
         const uint64_t T = block_sum_u64<BLOCK>((uint64_t)c, sh);
         const uint32_t zoff = (uint32_t)a.zlo + zbase;
         const int32_t* id2z = a.id2z + zbase;
@@ -310,7 +314,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         // T / P + BLOCK <= stage_cap rows.
         const uint64_t P = T == 0 ? 0 : (T + (a.stage_cap - BLOCK) - 1) / (a.stage_cap - BLOCK);
         uint32_t ci = lo, cw = 0, cv = 0, done = 0;
-        if (ci < hi) {
+        if (!use_mask && ci < hi) {
             cw = list_at(touched, listed, ci);
             cv = tile[cw];
         }
@@ -321,10 +325,20 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             uint32_t Tp;
             const uint32_t op = block_scan2_u32<BLOCK>(mine, Tp, sh32 + (p & 1) * (NW + 1));
             for (uint32_t j = 0; j < mine; j++) {
-                while (cv == 0u) {
-                    ++ci;
-                    cw = list_at(touched, listed, ci);
-                    cv = tile[cw];
+                if (use_mask) {
+                    // jump straight to the next word that holds rows
+                    if (cv == 0u) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)nzmask) - 1u;
+                        nzmask &= nzmask - 1ull;
+                        cw = list_at(touched, listed, lo + i);
+                        cv = tile[cw];
+                    }
+                } else {
+                    while (cv == 0u) {
+                        ++ci;
+                        cw = list_at(touched, listed, ci);
+                        cv = tile[cw];
+                    }
                 }
                 if (COUNT) {
                     stage[op + j] = cw;
