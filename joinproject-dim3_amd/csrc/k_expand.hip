@@ -204,25 +204,39 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             if (c0 == rb) track = (uint64_t)NP * kPiece <= track_limit;
             __syncthreads();
             const uint32_t nseg = min((uint32_t)BLOCK, re - c0);
-            for (uint32_t k = wid; k < NP; k += NW) {
-                uint32_t lo = 0, hi = nseg;  // last segment whose first piece is <= k
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_pre[mid] <= k) lo = mid;
-                    else hi = mid;
-                }
-                const uint32_t off = (k - s_pre[lo]) * kPiece;
-                const uint32_t base = s_sb[lo] + off;
-                const uint32_t len = min(s_len[lo] - off, kPiece);
-                uint32_t zl[kUnroll];
+            // Pieces come in increasing order, so each wavefront gallops
+            // forward from the segment of its previous piece instead of
+            // searching the whole table.
+            uint32_t seg = 0;
+            for (uint32_t k0 = wid; k0 < NP; k0 += 2 * NW) {
+                uint32_t zl[2 * kUnroll];
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; u++) {
-                    const uint32_t idx = lane + 64u * u;
-                    zl[u] = idx < len ? (uint32_t)a.colS[base + idx] - zbase : 0xFFFFFFFFu;
+                for (uint32_t h = 0; h < 2; h++) {
+                    const uint32_t k = k0 + h * NW;
+                    uint32_t base = 0, len = 0;
+                    if (k < NP) {
+                        uint32_t step = 1;
+                        while (seg + step < nseg && s_pre[seg + step] <= k) {
+                            seg += step;
+                            step <<= 1;
+                        }
+                        while (step > 1) {
+                            step >>= 1;
+                            if (seg + step < nseg && s_pre[seg + step] <= k) seg += step;
+                        }
+                        const uint32_t off = (k - s_pre[seg]) * kPiece;
+                        base = s_sb[seg] + off;
+                        len = min(s_len[seg] - off, kPiece);
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < kUnroll; u++) {
+                        const uint32_t idx = lane + 64u * u;
+                        zl[h * kUnroll + u] = idx < len ? (uint32_t)a.colS[base + idx] - zbase : 0xFFFFFFFFu;
+                    }
                 }
                 if (track) {
 #pragma unroll
-                    for (uint32_t u = 0; u < kUnroll; u++) {
+                    for (uint32_t u = 0; u < 2 * kUnroll; u++) {
                         bool first = false;
                         if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u], W, C);
                         push_first_touch(first, first ? list_item<COUNT>(zl[u], W, C) : 0u, &s_nt, touched,
@@ -230,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     }
                 } else {
 #pragma unroll
-                    for (uint32_t u = 0; u < kUnroll; u++) {
+                    for (uint32_t u = 0; u < 2 * kUnroll; u++) {
                         if (zl[u] != 0xFFFFFFFFu) {
                             if (COUNT) atomicAdd(&tile[zl[u]], 1u);
                             else atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
