@@ -34,6 +34,7 @@ using dim3h::ExpandArgs;
 
 constexpr uint32_t kUnroll = 4;
 constexpr uint32_t kPiece = 64 * kUnroll;  // join paths per wavefront work item
+constexpr uint32_t kPairs = 2;             // pieces in flight per wavefront
 
 __device__ __forceinline__ uint32_t list_at(const uint32_t* touched, bool listed, uint32_t i) {
     return listed ? touched[i] : i;
@@ -208,10 +209,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             // forward from the segment of its previous piece instead of
             // searching the whole table.
             uint32_t seg = 0;
-            for (uint32_t k0 = wid; k0 < NP; k0 += 2 * NW) {
-                uint32_t zl[2 * kUnroll];
+            for (uint32_t k0 = wid; k0 < NP; k0 += kPairs * NW) {
+                uint32_t zl[kPairs * kUnroll];
 #pragma unroll
-                for (uint32_t h = 0; h < 2; h++) {
+                for (uint32_t h = 0; h < kPairs; h++) {
                     const uint32_t k = k0 + h * NW;
                     uint32_t base = 0, len = 0;
                     if (k < NP) {
@@ -228,15 +229,19 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                         base = s_sb[seg] + off;
                         len = min(s_len[seg] - off, kPiece);
                     }
+                    // unconditional loads from a clamped, in-bounds index keep all of
+                    // them in flight (a branch per load makes the compiler wait on each)
 #pragma unroll
                     for (uint32_t u = 0; u < kUnroll; u++) {
                         const uint32_t idx = lane + 64u * u;
-                        zl[h * kUnroll + u] = idx < len ? (uint32_t)a.colS[base + idx] - zbase : 0xFFFFFFFFu;
+                        const bool ok = idx < len;
+                        const uint32_t v = (uint32_t)a.colS[ok ? base + idx : base];
+                        zl[h * kUnroll + u] = ok ? v - zbase : 0xFFFFFFFFu;
                     }
                 }
                 if (track) {
 #pragma unroll
-                    for (uint32_t u = 0; u < 2 * kUnroll; u++) {
+                    for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
                         bool first = false;
                         if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u], W, C);
                         push_first_touch(first, first ? list_item<COUNT>(zl[u], W, C) : 0u, &s_nt, touched,
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     }
                 } else {
 #pragma unroll
-                    for (uint32_t u = 0; u < 2 * kUnroll; u++) {
+                    for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
                         if (zl[u] != 0xFFFFFFFFu) {
                             if (COUNT) atomicAdd(&tile[zl[u]], 1u);
                             else atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
