@@ -103,6 +103,24 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
     return v;
 }
 
+// Inclusive u32 scan over the wavefront with DPP moves (VALU only, no LDS
+// round trip): Hillis-Steele inside each row of 16 lanes, then the row
+// totals are carried with row_bcast:15 (into rows 1, 3) and row_bcast:31
+// (into rows 2, 3).
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_u32(v), 63);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_reduce_sum(T v) {
 #pragma unroll
@@ -119,11 +137,20 @@ __device__ __forceinline__ T wave_reduce_xor(T v) {
 
 // Exclusive scan over the block; sh needs BLOCK/64+1 entries.  Ends with a
 // barrier so sh can be reused right away.
+template <typename T>
+__device__ __forceinline__ T wave_scan_any(T v) {
+    return wave_inclusive_scan(v);
+}
+template <>
+__device__ __forceinline__ uint32_t wave_scan_any<uint32_t>(uint32_t v) {
+    return wave_scan_u32(v);
+}
+
 template <int BLOCK, typename T>
 __device__ __forceinline__ T block_exclusive_scan(T v, T& total, T* sh) {
     constexpr int NW = BLOCK / 64;
     const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    T inc = wave_inclusive_scan(v);
+    T inc = wave_scan_any(v);
     if (lane == 63) sh[wid] = inc;
     __syncthreads();
     if (threadIdx.x == 0) {

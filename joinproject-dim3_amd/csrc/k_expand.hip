@@ -59,7 +59,7 @@ template <int BLOCK>
 __device__ __forceinline__ uint32_t block_scan2_u32(uint32_t v, uint32_t& total, uint32_t* sh) {
     constexpr uint32_t NW = BLOCK / 64;
     const uint32_t wid = threadIdx.x >> 6;
-    const uint32_t inc = wave_inclusive_scan(v);
+    const uint32_t inc = wave_scan_u32(v);
     if (lane_id() == 63) sh[wid] = inc;
     __syncthreads();
     uint32_t before = 0, tot = 0;
@@ -96,7 +96,7 @@ __device__ __forceinline__ void push_first_touch(bool first, uint32_t item, uint
         const uint32_t leader = (uint32_t)__ffsll((unsigned long long)ft) - 1u;
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(s_nt, (uint32_t)__popcll(ft));
-        base = __shfl(base, (int)leader, 64);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
         if (first) {
             const uint32_t p = base + mbcnt64(ft);
             if (p < cap) touched[p] = item;
@@ -155,6 +155,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     uint32_t* stage = area;
     __shared__ uint64_t sh[NW + 1];
     __shared__ uint32_t sh32[2 * (NW + 1)];
+    __shared__ uint32_t s_wtot[NW];
     __shared__ uint32_t s_task, s_nt;
     __shared__ uint64_t s_gbase;
 
@@ -189,11 +190,13 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         // ---- expansion ----
         bool track = true;
         for (uint32_t c0 = rb; c0 < re; c0 += BLOCK) {
+            // clamped, unconditional loads: the three loads of a y stay in flight
             const uint32_t i = c0 + tid;
+            const bool in_row = i < re;
+            const uint32_t key = (uint32_t)a.colR[in_row ? i : re - 1] * a.ntiles + t;
+            const uint32_t sb = a.rowptrS[key], se = a.rowptrS[key + 1];
             uint32_t np = 0;
-            if (i < re) {
-                const uint32_t key = (uint32_t)a.colR[i] * a.ntiles + t;
-                const uint32_t sb = a.rowptrS[key], se = a.rowptrS[key + 1];
+            if (in_row) {
                 s_sb[tid] = sb;
                 s_len[tid] = se - sb;
                 np = (se - sb + kPiece - 1) / kPiece;
@@ -260,7 +263,11 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             __syncthreads();
         }
 
-        // ---- emission: each thread owns a contiguous slice of the list ----
+        // ---- emission ----
+        // The list (touched words, or the whole tile) is cut into one
+        // contiguous slice per wavefront; lane l of a wavefront owns the slice
+        // words wlo + l + 64 g, so a wavefront reads 64 consecutive words per
+        // step (no bank conflicts).
         const uint32_t nt = s_nt;
         const bool listed = track && nt <= a.touched_cap;
         const uint32_t count = listed ? nt : a.tile_words;
@@ -273,28 +280,38 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                 self_keep = COUNT ? 0u : ~(1u << (zl & 31u));
             }
         }
-        const uint32_t per = (count + BLOCK - 1) / BLOCK;
-        const uint32_t lo = min(tid * per, count), hi = min(lo + per, count);
-        // nzmask bit i: word lo + i holds rows (kept when the slice has <= 64 words)
+        const uint32_t wper = (count + NW - 1) / NW;
+        const uint32_t wlo = min(wid * wper, count), whi = min(wlo + wper, count);
         uint32_t c = 0;
-        uint64_t nzmask = 0;
-        for (uint32_t i = lo; i < hi; i++) {
-            const uint32_t w = list_at(touched, listed, i);
-            uint32_t v = tile[w];
-            if (w == self_item) {
-                v &= self_keep;
-                tile[w] = v;
+        uint64_t nzmask = 0;  // bit g: the lane's word of step g holds rows
+        {
+            uint32_t g = 0;
+            for (uint32_t i = wlo + lane; i < whi; i += 64u, g++) {
+                const uint32_t w = list_at(touched, listed, i);
+                uint32_t v = tile[w];
+                if (w == self_item) {
+                    v &= self_keep;
+                    tile[w] = v;
+                }
+                c += units_of<COUNT>(v);
+                if (v != 0u && g < 64u) nzmask |= 1ull << g;
             }
-            c += units_of<COUNT>(v);
-            if (v != 0u && i - lo < 64u) nzmask |= 1ull << (i - lo);
         }
-        const bool use_mask = per <= 64u;
-        const uint64_t T = block_sum_u64<BLOCK>((uint64_t)c, sh);
+        const uint32_t wtot = wave_sum_u32(c);
+        if (lane == 0) s_wtot[wid] = wtot;
+        __syncthreads();
+        uint32_t woff = 0, T = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < NW; w++) {
+            const uint32_t v = s_wtot[w];
+            woff += w < wid ? v : 0u;
+            T += v;
+        }
         const uint32_t zoff = (uint32_t)a.zlo + zbase;
         const int32_t* id2z = a.id2z + zbase;
 
         if (a.fp_mode) {
-            for (uint32_t i = lo; i < hi; i++) {
+            for (uint32_t i = wlo + lane; i < whi; i += 64u) {
                 const uint32_t w = list_at(touched, listed, i);
                 uint32_t v = tile[w];
                 tile[w] = 0;
@@ -323,39 +340,90 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             continue;
         }
 
-        // The row's output range is reserved once; its result is consumed
-        // only after the first fill, so neither the atomic's latency nor the
-        // wait it implies (vmcnt also counts earlier stores) is in front of it.
-        uint64_t my_base = 0;
-        if (tid == 0 && T) my_base = atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
-        // P passes; every thread spreads its c rows evenly over them, so all
+        if (COUNT || T <= 2u * count) {
+            // Direct: at most ~2 rows per word, so the first rows of 64
+            // neighbouring words land on neighbouring addresses and the
+            // stores coalesce without staging.
+            if (tid == 0) s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
+            __syncthreads();
+            uint64_t pos0 = s_gbase + woff;
+            for (uint32_t i0 = wlo; i0 < whi; i0 += 64u) {
+                const uint32_t i = i0 + lane;
+                uint32_t v = 0, w = 0;
+                if (i < whi) {
+                    w = list_at(touched, listed, i);
+                    v = tile[w];
+                }
+                // prefix of the per-word row counts from one ballot per count
+                // level (rows are sparse here: usually one to three levels)
+                const uint32_t cc = units_of<COUNT>(v);
+                uint32_t before = 0, gtot = 0;
+                for (uint32_t lvl = 1;; lvl++) {
+                    const uint64_t m = __ballot(cc >= lvl);
+                    if (m == 0) break;
+                    before += mbcnt64(m);
+                    gtot += (uint32_t)__popcll(m);
+                    if (COUNT) break;
+                }
+                uint64_t pos = pos0 + before;
+                pos0 += gtot;
+                if (COUNT) {
+                    if (v != 0u && pos < a.cap) {
+                        dim3_count_row row;
+                        row.x = xv;
+                        row.z = ZID ? (int32_t)(zoff + w) : id2z[w];
+                        row.count = (int64_t)v;
+                        reinterpret_cast<dim3_count_row*>(a.out)[pos] = row;
+                    }
+                } else {
+                    int2* out = reinterpret_cast<int2*>(a.out);
+                    while (v) {
+                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                        v &= v - 1u;
+                        const uint32_t e = swz_offset(w, b, W, C);
+                        if (pos < a.cap) out[pos] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                        ++pos;
+                    }
+                }
+                if (i < whi) tile[w] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+
+        // Staged (dense rows): the row's output range is reserved once and its
+        // result consumed only after the first fill.  P passes; every thread
+        // spreads its c rows evenly over them (q or q + 1 per pass), so all
         // wavefronts fill in every pass and a pass holds at most
         // T / P + BLOCK <= stage_cap rows.
-        const uint64_t P = T == 0 ? 0 : (T + (a.stage_cap - BLOCK) - 1) / (a.stage_cap - BLOCK);
-        uint32_t ci = lo, cw = 0, cv = 0, done = 0;
-        if (!use_mask && ci < hi) {
-            cw = list_at(touched, listed, ci);
+        uint64_t my_base = 0;
+        if (tid == 0 && T) my_base = atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
+        const uint32_t P = (T + (a.stage_cap - BLOCK) - 1) / (a.stage_cap - BLOCK);
+        const uint32_t cq = c / P, cr = c - cq * P;
+        const bool use_mask = whi - wlo <= 64u * 64u;
+        uint32_t g = 0, cw = 0, cv = 0;
+        if (!use_mask && wlo + lane < whi) {
+            cw = list_at(touched, listed, wlo + lane);
             cv = tile[cw];
         }
         uint64_t pass_base = 0;
-        for (uint64_t p = 0; p < P; p++) {
-            const uint32_t upto = (uint32_t)(((p + 1) * (uint64_t)c + P - 1) / P);
-            const uint32_t mine = upto - done;
+        for (uint32_t p = 0; p < P; p++) {
+            const uint32_t mine = cq + (p < cr ? 1u : 0u);
             uint32_t Tp;
             const uint32_t op = block_scan2_u32<BLOCK>(mine, Tp, sh32 + (p & 1) * (NW + 1));
             for (uint32_t j = 0; j < mine; j++) {
                 if (use_mask) {
                     // jump straight to the next word that holds rows
                     if (cv == 0u) {
-                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)nzmask) - 1u;
+                        const uint32_t gg = (uint32_t)__ffsll((unsigned long long)nzmask) - 1u;
                         nzmask &= nzmask - 1ull;
-                        cw = list_at(touched, listed, lo + i);
+                        cw = list_at(touched, listed, wlo + lane + 64u * gg);
                         cv = tile[cw];
                     }
                 } else {
                     while (cv == 0u) {
-                        ++ci;
-                        cw = list_at(touched, listed, ci);
+                        ++g;
+                        cw = list_at(touched, listed, wlo + lane + 64u * g);
                         cv = tile[cw];
                     }
                 }
@@ -368,54 +436,39 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     stage[op + j] = swz_offset(cw, b, W, C);
                 }
             }
-            done = upto;
             if (tid == 0 && p == 0) s_gbase = my_base;
             __syncthreads();
             const uint64_t g0 = s_gbase + pass_base;
             const uint32_t nq = Tp;
-            if (COUNT) {
-                dim3_count_row* out = reinterpret_cast<dim3_count_row*>(a.out);
-                for (uint32_t k = tid; k < nq; k += BLOCK) {
+            int2* out = reinterpret_cast<int2*>(a.out) + g0;
+            if (g0 + nq <= a.cap) {
+                uint32_t k = tid;
+                for (; k + 3u * BLOCK < nq; k += 4u * BLOCK) {
+                    const uint32_t e0 = stage[k], e1 = stage[k + BLOCK];
+                    const uint32_t e2 = stage[k + 2u * BLOCK], e3 = stage[k + 3u * BLOCK];
+                    const int32_t z0 = ZID ? (int32_t)(zoff + e0) : id2z[e0];
+                    const int32_t z1 = ZID ? (int32_t)(zoff + e1) : id2z[e1];
+                    const int32_t z2 = ZID ? (int32_t)(zoff + e2) : id2z[e2];
+                    const int32_t z3 = ZID ? (int32_t)(zoff + e3) : id2z[e3];
+                    out[k] = make_int2(xv, z0);
+                    out[k + BLOCK] = make_int2(xv, z1);
+                    out[k + 2u * BLOCK] = make_int2(xv, z2);
+                    out[k + 3u * BLOCK] = make_int2(xv, z3);
+                }
+                for (; k < nq; k += BLOCK) {
                     const uint32_t e = stage[k];
-                    if (g0 + k < a.cap) {
-                        dim3_count_row row;
-                        row.x = xv;
-                        row.z = ZID ? (int32_t)(zoff + e) : id2z[e];
-                        row.count = (int64_t)tile[e];
-                        out[g0 + k] = row;
-                    }
+                    out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                 }
             } else {
-                int2* out = reinterpret_cast<int2*>(a.out) + g0;
-                if (g0 + nq <= a.cap) {
-                    uint32_t k = tid;
-                    for (; k + 3u * BLOCK < nq; k += 4u * BLOCK) {
-                        const uint32_t e0 = stage[k], e1 = stage[k + BLOCK];
-                        const uint32_t e2 = stage[k + 2u * BLOCK], e3 = stage[k + 3u * BLOCK];
-                        const int32_t z0 = ZID ? (int32_t)(zoff + e0) : id2z[e0];
-                        const int32_t z1 = ZID ? (int32_t)(zoff + e1) : id2z[e1];
-                        const int32_t z2 = ZID ? (int32_t)(zoff + e2) : id2z[e2];
-                        const int32_t z3 = ZID ? (int32_t)(zoff + e3) : id2z[e3];
-                        out[k] = make_int2(xv, z0);
-                        out[k + BLOCK] = make_int2(xv, z1);
-                        out[k + 2u * BLOCK] = make_int2(xv, z2);
-                        out[k + 3u * BLOCK] = make_int2(xv, z3);
-                    }
-                    for (; k < nq; k += BLOCK) {
-                        const uint32_t e = stage[k];
-                        out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
-                    }
-                } else {
-                    for (uint32_t k = tid; k < nq; k += BLOCK) {
-                        const uint32_t e = stage[k];
-                        if (g0 + k < a.cap) out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
-                    }
+                for (uint32_t k = tid; k < nq; k += BLOCK) {
+                    const uint32_t e = stage[k];
+                    if (g0 + k < a.cap) out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                 }
             }
             pass_base += nq;
             __syncthreads();
         }
-        for (uint32_t i = lo; i < hi; i++) tile[list_at(touched, listed, i)] = 0;
+        for (uint32_t i = wlo + lane; i < whi; i += 64u) tile[list_at(touched, listed, i)] = 0;
         __syncthreads();
     }
 
