@@ -49,6 +49,19 @@ def query_alg_bytes(st, n_out, row_bytes=8):
     return 16 * (st.n_r + st.n_s) + 4 * st.paths_sparse + row_bytes * n_out
 
 
+def measured_traffic(args):
+    """HBM bytes per expansion launch measured by rocprofv3 FETCH_SIZE +
+    WRITE_SIZE (calibrated against a plain copy), recorded in
+    profiles/expand_two_hop_graph.traffic.json for the default workload."""
+    default = (args.vertices, args.edges, args.max_degree, args.seed, args.mode, args.dense, args.block,
+               args.tile_bits, args.hash_only) == (1_000_000, 4_000_000, 50_000, 7, "materialize", -1, 0, 0, False)
+    path = os.path.join(ROOT, "profiles", "expand_two_hop_graph.traffic.json")
+    if not default or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
 def cpu_baseline(E, target_s, slices):
     """Reference DIM3_parallel on vertex slices x = r (mod slices) of R
     against all of S; stops once target_s of CPU time has been spent."""
@@ -96,7 +109,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-slices", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4}))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -185,8 +212,8 @@ def main():
                    else "fingerprint only", "distinct_pairs": int(n_out), "join_paths": int(st.out_j),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_expand_project", "kernel_ms": exp_ms, "alg_bytes": b_kernel},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                     "kernel": "k_expand", "kernel_ms": exp_ms, "alg_bytes": b_kernel},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -87,6 +87,9 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
     return r;
 }
 
+// One output row (plain store: non-temporal stores measured slower here).
+__device__ __forceinline__ void put_pair(int2* p, int32_t x, int32_t z) { *p = make_int2(x, z); }
+
 // Records the first touches of one wavefront instruction in the LDS list.
 __device__ __forceinline__ void push_first_touch(bool first, uint32_t item, uint32_t* s_nt, uint32_t* touched,
                                                  uint32_t cap) {
@@ -381,7 +384,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                         const uint32_t b = (uint32_t)__ffs(v) - 1u;
                         v &= v - 1u;
                         const uint32_t e = swz_offset(w, b, W, C);
-                        if (pos < a.cap) out[pos] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                        if (pos < a.cap) put_pair(out + pos, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                         ++pos;
                     }
                 }
@@ -450,19 +453,19 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     const int32_t z1 = ZID ? (int32_t)(zoff + e1) : id2z[e1];
                     const int32_t z2 = ZID ? (int32_t)(zoff + e2) : id2z[e2];
                     const int32_t z3 = ZID ? (int32_t)(zoff + e3) : id2z[e3];
-                    out[k] = make_int2(xv, z0);
-                    out[k + BLOCK] = make_int2(xv, z1);
-                    out[k + 2u * BLOCK] = make_int2(xv, z2);
-                    out[k + 3u * BLOCK] = make_int2(xv, z3);
+                    put_pair(out + k, xv, z0);
+                    put_pair(out + k + BLOCK, xv, z1);
+                    put_pair(out + k + 2u * BLOCK, xv, z2);
+                    put_pair(out + k + 3u * BLOCK, xv, z3);
                 }
                 for (; k < nq; k += BLOCK) {
                     const uint32_t e = stage[k];
-                    out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                    put_pair(out + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                 }
             } else {
                 for (uint32_t k = tid; k < nq; k += BLOCK) {
                     const uint32_t e = stage[k];
-                    if (g0 + k < a.cap) out[k] = make_int2(xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                    if (g0 + k < a.cap) put_pair(out + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                 }
             }
             pass_base += nq;
