@@ -155,6 +155,16 @@ int dim3_join_aggregate_count_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_
 int dim3_count_join_paths(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
                           uint64_t* out_j);
 
+/* The dense part's product on its own (the boolean product that
+ * Hybrid_solution's DENSE part evaluates one (x, dense z) pair at a time,
+ * dim3.hpp:894-950): C = A * Bt^T over int8 on the MFMA cores with int32
+ * accumulation.  A is M x K, Bt is N x K, both row-major host arrays.
+ * bits (nullable): M rows of ceil(N / 32) uint32 words, bit (n % 32) of word
+ * n / 32 set iff C[m][n] > 0; counts (nullable): M x N int32 row-major.
+ * K <= 131072 keeps every entry exact; M < 8388480. */
+int dim3_dense_product(dim3_ctx* ctx, const int8_t* A, uint64_t M, const int8_t* Bt, uint64_t N, uint64_t K,
+                       uint32_t* bits, int32_t* counts);
+
 /* Library version string. */
 const char* dim3_version(void);
 

@@ -67,7 +67,8 @@ COUNT_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("count", "<i8")])
 EXPORTS = [
     "dim3_ctx_create", "dim3_ctx_destroy", "dim3_last_error", "dim3_status_string", "dim3_ctx_device_bytes",
     "dim3_opts_default", "dim3_join_project", "dim3_join_aggregate_count", "dim3_result_free",
-    "dim3_join_project_dev", "dim3_join_aggregate_count_dev", "dim3_count_join_paths", "dim3_version",
+    "dim3_join_project_dev", "dim3_join_aggregate_count_dev", "dim3_count_join_paths", "dim3_dense_product",
+    "dim3_version",
 ]
 
 _lib = None
@@ -121,6 +122,8 @@ def lib():
     L.dim3_result_free.restype = None
     L.dim3_count_join_paths.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
     L.dim3_count_join_paths.restype = i32
+    L.dim3_dense_product.argtypes = [vp, vp, u64, vp, u64, u64, vp, vp]
+    L.dim3_dense_product.restype = i32
     L.dim3_version.argtypes = []
     L.dim3_version.restype = ctypes.c_char_p
     _lib = L
@@ -292,6 +295,22 @@ class Context:
         rc = self._L.dim3_count_join_paths(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0], ctypes.byref(out))
         self._check(rc, "dim3_count_join_paths")
         return int(out.value)
+
+    def dense_product(self, A, Bt, bits=True, counts=True):
+        """C = A @ Bt.T on the MFMA cores (int8 in, int32 accumulate).
+        Returns (bit rows as uint32 [M, ceil(N/32)] or None, counts int32 [M, N] or None)."""
+        A = np.ascontiguousarray(A, dtype=np.int8)
+        Bt = np.ascontiguousarray(Bt, dtype=np.int8)
+        if A.ndim != 2 or Bt.ndim != 2 or A.shape[1] != Bt.shape[1]:
+            raise ValueError("A (M x K) and Bt (N x K) must share K")
+        M, K = A.shape
+        N = Bt.shape[0]
+        ob = np.zeros((M, (N + 31) // 32), np.uint32) if bits else None
+        oc = np.zeros((M, N), np.int32) if counts else None
+        rc = self._L.dim3_dense_product(self._h, _ptr(A), M, _ptr(Bt), N, K,
+                                        None if ob is None else _ptr(ob), None if oc is None else _ptr(oc))
+        self._check(rc, "dim3_dense_product")
+        return ob, oc
 
 
 def version():

@@ -57,6 +57,20 @@ enum SlotId {
     WS_ZID_S,
     WS_COL_S,
     WS_OUT,
+    WS_XW,
+    WS_ZW,
+    WS_DHIST,
+    WS_HID,
+    WS_HX,
+    WS_HX2X,
+    WS_ZFLAG,
+    WS_HZPOS,
+    WS_HZ2ZID,
+    WS_HZTILE,
+    WS_DA,
+    WS_DB,
+    WS_DBITS,
+    WS_CINT,
     WS_COUNT
 };
 
@@ -83,7 +97,8 @@ struct dim3_ctx {
     int cu_count = 0;
     size_t max_lds = 0;
     Scalars* h_sc = nullptr;  // pinned mirror of the device scalars
-    hipEvent_t ev[6] = {};
+    unsigned long long* h_hist = nullptr;  // pinned mirror of the dense-split histogram
+    hipEvent_t ev[8] = {};
     std::recursive_mutex mu;
 
     // plan of the last prepared query
@@ -107,6 +122,10 @@ struct dim3_ctx {
         uint32_t touched_cap = 0, stage_cap = 0;
         int block = 256;
         float ms_map = 0, ms_csr = 0;
+        // dense split
+        int dense = DIM3_DENSE_OFF;  // mode used
+        uint32_t bmin = 0;           // heaviness bin threshold
+        uint32_t K = 0, Kp = 0, Mh = 0, Nh = 0, dwords = 0;
     } plan;
 };
 
@@ -246,6 +265,111 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
     P.stage_cap = best.stage;
 }
 
+// ---------------------------------------------------------------------------
+// planner: dense split.  From the heaviness histogram (per bin: keys, join
+// paths, sum degR, sum degS, x rows and z columns whose heaviest key falls in
+// it) every threshold bin b gives the exact product shape of "keys with
+// w >= lo(b) are heavy": K keys, Mh rows, Nh columns, P_h paths.  The cost
+// model (device rates) prices the sparse work those paths would take against
+// the dense product and its seeding; AUTO takes the best positive saving.
+// ---------------------------------------------------------------------------
+struct DenseModel {
+    double path_s = 0.9e-12;   // sparse expansion, per join path
+    double seed_s = 0.3e-12;   // seeding, per product bit or count
+    double mfma_ops = 1.5e15;  // int8 ops/s reached by k_gemm_i8
+    double hbm_bps = 5.0e12;   // streaming bytes/s
+    double l2_bps = 10.0e12;   // bit-or re-reads of key bitmaps
+    double launch_s = 60e-6;   // fixed cost of the dense launches
+};
+
+struct DenseShape {
+    uint64_t K = 0, Mh = 0, Nh = 0, paths = 0, sumR = 0, sumS = 0;
+};
+
+// seconds the dense part takes for one shape and mode, <0 if it cannot run
+double dense_cost(const DenseModel& m, const DenseShape& d, int mode, bool count, uint64_t mem_room,
+                  uint64_t* bytes_out) {
+    if (d.K == 0 || d.Mh == 0 || d.Nh == 0) return -1;
+    const uint64_t Kp = (d.K + 63) / 64 * 64;
+    const uint64_t dwords = (d.Nh + 31) / 32;
+    const double cells = (double)d.Mh * (double)d.Nh;
+    const double set_bits = std::min(cells, (double)d.paths);
+    uint64_t bytes;
+    double t;
+    if (mode == DIM3_DENSE_MFMA) {
+        if (d.Mh >= 65535ull * 128ull || d.Nh >= (1ull << 31)) return -1;
+        if (count && Kp > 131072) return -1;  // int32 accumulator bound for multiplicities up to 127
+        bytes = (d.Mh + d.Nh) * Kp + (count ? d.Mh * d.Nh * 4 : d.Mh * dwords * 4);
+        t = 2.0 * cells * (double)Kp / m.mfma_ops + (double)((d.Mh + d.Nh) * Kp) / m.hbm_bps;
+    } else {
+        if (count) return -1;
+        bytes = d.K * dwords * 4 + d.Mh * dwords * 4;
+        t = (double)d.sumR * (double)dwords * 4.0 / m.l2_bps + (double)(d.K * dwords * 4) / m.hbm_bps;
+    }
+    if (bytes > mem_room) return -1;
+    // product written once, read back by the seeding, plus one LDS op per set entry
+    t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps + set_bits * m.seed_s + m.launch_s;
+    if (bytes_out) *bytes_out = bytes;
+    return t;
+}
+
+void plan_dense(dim3_ctx* c, const dim3_opts& o) {
+    auto& P = c->plan;
+    P.dense = DIM3_DENSE_OFF;
+    if (o.dense_mode == DIM3_DENSE_OFF) return;
+    const unsigned long long* h = c->h_hist;
+    const uint32_t NB = dim3h::kDenseBins;
+    const DenseModel m;
+    const uint64_t room = c->budget > c->held ? (c->budget - c->held) / 3 : 0;
+    const bool count = P.count_mode;
+    // suffix sums from the heaviest bin down
+    DenseShape acc;
+    int best_mode = DIM3_DENSE_OFF;
+    uint32_t best_b = 0;
+    double best_gain = -1e300;
+    DenseShape best_shape;
+    for (int b = (int)NB - 1; b >= 1; b--) {
+        acc.K += h[b];
+        acc.paths += h[NB + b];
+        acc.sumR += h[2 * NB + b];
+        acc.sumS += h[3 * NB + b];
+        acc.Mh += h[4 * NB + b];
+        acc.Nh += h[5 * NB + b];
+        if (acc.K == 0) continue;
+        if (o.heavy_k > 0 && acc.K > (uint64_t)o.heavy_k && best_mode != DIM3_DENSE_OFF) break;
+        const int modes[2] = {DIM3_DENSE_MFMA, DIM3_DENSE_BITOR};
+        int bm = DIM3_DENSE_OFF;
+        double bg = -1e300;
+        for (int mode : modes) {
+            if (o.dense_mode > 0 && o.dense_mode != mode) continue;
+            const double t = dense_cost(m, acc, mode, count, room, nullptr);
+            if (t < 0) continue;
+            const double gain = (double)acc.paths * m.path_s - t;
+            if (gain > bg) {
+                bg = gain;
+                bm = mode;
+            }
+        }
+        if (bm == DIM3_DENSE_OFF) continue;
+        // with heavy_k set, the largest key set within it wins
+        if (o.heavy_k > 0 || bg > best_gain) {
+            best_gain = bg;
+            best_mode = bm;
+            best_b = (uint32_t)b;
+            best_shape = acc;
+        }
+    }
+    if (best_mode == DIM3_DENSE_OFF) return;
+    if (o.dense_mode == DIM3_DENSE_AUTO && o.heavy_k == 0 && best_gain <= 0) return;
+    P.dense = best_mode;
+    P.bmin = best_b;
+    P.K = (uint32_t)best_shape.K;
+    P.Kp = (P.K + 63u) / 64u * 64u;
+    P.Mh = (uint32_t)best_shape.Mh;
+    P.Nh = (uint32_t)best_shape.Nh;
+    P.dwords = (P.Nh + 31u) / 32u;
+}
+
 // Mapping + CSR of one query.  Leaves the plan in c->plan.
 int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
             const dim3_opts& o, hipStream_t st) {
@@ -347,6 +471,22 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         dim3h::build_z(dS, nS, slotS, P.mz, degZ, slotSz, st);
         dim3h::scan_deg(degZ, (uint32_t)nz1, exZ, part, &sc->tot_z, st);
     }
+    // heaviness histogram of the dense split
+    const bool want_dense = o.dense_mode != DIM3_DENSE_OFF;
+    uint32_t *xw = nullptr, *zw = nullptr;
+    unsigned long long* dhist = nullptr;
+    if (want_dense) {
+        TRY(ws(c, WS_XW, nx1, &xw));
+        TRY(ws(c, WS_ZW, nz1, &zw));
+        TRY(ws(c, WS_DHIST, 6ull * dim3h::kDenseBins, &dhist));
+        dim3h::fill_u32(xw, nx1, 0, st);
+        dim3h::fill_u32(zw, nz1, 0, st);
+        dim3h::fill_u64((uint64_t*)dhist, 6ull * dim3h::kDenseBins, 0, st);
+        dim3h::dense_weights(dR, nR, slotR, slotRx, dS, nS, slotS, P.z_ident ? nullptr : slotSz, P.zlo, degR, degS, xw,
+                             zw, st);
+        dim3h::dense_hist(degR, degS, (uint32_t)ny1, xw, (uint32_t)nx1, zw, (uint32_t)nz1, dhist, st);
+        HIPCHK(c, hipMemcpyAsync(c->h_hist, dhist, 6ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
+    }
     LAUNCHCHK(c, "mapping");
     HIPCHK(c, hipEventRecord(c->ev[1], st));
     HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -370,6 +510,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     }
     P.empty = false;
     plan_tiles(c, o);
+    plan_dense(c, o);
     if ((uint64_t)P.ny * P.ntiles >= (1ull << 32))
         return fail(c, DIM3_ERR_INVALID, "tiled S index does not fit 32 bits (ny=%u, tiles=%u)", P.ny, P.ntiles);
 
@@ -397,7 +538,24 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
     dim3h::scatter_r(slotR, slotRx, nR, exY, exX, curR, colR, st);
     dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
-    dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, cntS, keyS,
+    int32_t *hid = nullptr, *hx = nullptr;
+    uint32_t *hx2x = nullptr, *zflag = nullptr, *hzpos = nullptr, *hz2zid = nullptr, *hz_tile = nullptr;
+    if (P.dense != DIM3_DENSE_OFF) {
+        // heavy keys, product rows and columns (columns numbered in z-ID order)
+        TRY(ws(c, WS_HID, P.ny, &hid));
+        TRY(ws(c, WS_HX, P.nx, &hx));
+        TRY(ws(c, WS_HX2X, P.Mh, &hx2x));
+        TRY(ws(c, WS_ZFLAG, P.nz, &zflag));
+        TRY(ws(c, WS_HZPOS, (uint64_t)P.nz + 1, &hzpos));
+        TRY(ws(c, WS_HZ2ZID, P.Nh, &hz2zid));
+        TRY(ws(c, WS_HZTILE, (uint64_t)P.ntiles + 1, &hz_tile));
+        dim3h::dense_mark(degR, degS, (uint32_t)ny1, exY, degX, exX, (uint32_t)nx1, xw, P.z_ident ? nullptr : degZ,
+                          P.z_ident ? nullptr : exZ, (uint32_t)nz1, zw, P.bmin, hid, hx, hx2x, zflag,
+                          (uint32_t*)&sc->spare[0], st);
+        dim3h::scan_u32(zflag, P.nz, hzpos, part, &sc->nzh, st);
+        dim3h::dense_columns(zflag, hzpos, P.nz, hz2zid, P.ntiles, P.tile_size, P.Nh, hz_tile, st);
+    }
+    dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, hid, cntS, keyS,
                    zidS, st);
     dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
     dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
@@ -407,6 +565,36 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
     LAUNCHCHK(c, "csr");
     HIPCHK(c, hipEventRecord(c->ev[3], st));
+    if (P.dense != DIM3_DENSE_OFF) {
+        // the product of the heavy keys: Dbits (project) or Cint (COUNT)
+        uint64_t* overflow = &sc->spare[1];
+        uint32_t *dbits = nullptr, *B = nullptr;
+        int32_t* cint = nullptr;
+        if (P.count_mode) TRY(ws(c, WS_CINT, (uint64_t)P.Mh * P.Nh, &cint));
+        else TRY(ws(c, WS_DBITS, (uint64_t)P.Mh * P.dwords, &dbits));
+        if (P.dense == DIM3_DENSE_MFMA) {
+            uint32_t* A;
+            TRY(ws(c, WS_DA, (uint64_t)P.Mh * P.Kp / 4, &A));
+            TRY(ws(c, WS_DB, (uint64_t)P.Nh * P.Kp / 4, &B));
+            dim3h::fill_u32(A, (uint64_t)P.Mh * P.Kp / 4, 0, st);
+            dim3h::fill_u32(B, (uint64_t)P.Nh * P.Kp / 4, 0, st);
+            dim3h::dense_build_a(P.count_mode, slotR, slotRx, nR, exY, exX, hid, hx, P.Kp, A, overflow, st);
+            dim3h::dense_build_b(P.count_mode ? 1 : 0, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY,
+                                 hid, hzpos, P.Kp, P.dwords, B, overflow, st);
+            const int e = dim3h::gemm_i8((const int8_t*)A, (const int8_t*)B, P.Mh, P.Nh, P.Kp, dbits, P.dwords, cint,
+                                         st);
+            if (e != 0) return fail(c, DIM3_ERR_DEVICE, "dense product launch: %s", hipGetErrorString((hipError_t)e));
+        } else {
+            TRY(ws(c, WS_DB, (uint64_t)P.K * P.dwords, &B));
+            dim3h::fill_u32(B, (uint64_t)P.K * P.dwords, 0, st);
+            dim3h::dense_build_b(2, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY, hid, hzpos, 0,
+                                 P.dwords, B, overflow, st);
+            const int e = dim3h::bitor_rows(rowR, colR, hid, hx2x, P.Mh, B, P.dwords, dbits, st);
+            if (e != 0) return fail(c, DIM3_ERR_DEVICE, "dense bit-or launch: %s", hipGetErrorString((hipError_t)e));
+        }
+        LAUNCHCHK(c, "dense");
+    }
+    HIPCHK(c, hipEventRecord(c->ev[6], st));
     return DIM3_OK;
 }
 
@@ -441,6 +629,22 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
     a.sc = sc;
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
+    a.hx = nullptr;
+    a.dbits = nullptr;
+    a.cint = nullptr;
+    a.hz2zid = nullptr;
+    a.hz_tile = nullptr;
+    a.dwords = 0;
+    a.Nh = 0;
+    if (P.dense != DIM3_DENSE_OFF) {
+        a.hx = (const int32_t*)c->ws[WS_HX].p;
+        if (P.count_mode) a.cint = (const int32_t*)c->ws[WS_CINT].p;
+        else a.dbits = (const uint32_t*)c->ws[WS_DBITS].p;
+        a.hz2zid = (const uint32_t*)c->ws[WS_HZ2ZID].p;
+        a.hz_tile = (const uint32_t*)c->ws[WS_HZTILE].p;
+        a.dwords = P.dwords;
+        a.Nh = P.Nh;
+    }
     HIPCHK(c, hipEventRecord(c->ev[4], st));
     int e = P.count_mode ? dim3h::expand_count(a, P.block, c->cu_count, st)
                          : dim3h::expand_project(a, P.block, c->cu_count, st);
@@ -449,6 +653,11 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
     HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     return DIM3_OK;
+}
+
+// COUNT with the int8 product: some (x, y) or (z, y) multiplicity passed 127
+bool dense_overflowed(const dim3_ctx* c) {
+    return c->plan.dense == DIM3_DENSE_MFMA && c->plan.count_mode && !c->plan.empty && c->h_sc->spare[1] != 0;
 }
 
 void fill_result(dim3_ctx* c, dim3_result* res, bool fp_mode) {
@@ -470,7 +679,10 @@ void fill_result(dim3_ctx* c, dim3_result* res, bool fp_mode) {
     st.nz = P.nz;
     st.out_j = P.out_j;
     st.paths_sparse = P.empty ? 0 : s.paths;
-    st.dense_mode = DIM3_DENSE_OFF;
+    st.dense_mode = (uint32_t)P.dense;
+    st.heavy_keys = P.dense ? P.K : 0;
+    st.nx_heavy = P.dense ? P.Mh : 0;
+    st.nz_heavy = P.dense ? P.Nh : 0;
     st.tile_bits = P.empty ? 0 : P.tile_size;
     st.n_tiles = P.ntiles;
     st.block = (uint32_t)P.block;
@@ -479,6 +691,8 @@ void fill_result(dim3_ctx* c, dim3_result* res, bool fp_mode) {
         float ms = 0;
         hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
         st.ms_csr = ms;
+        hipEventElapsedTime(&ms, c->ev[3], c->ev[6]);
+        st.ms_dense = ms;
         hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
         st.ms_expand = ms;
         hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
@@ -510,8 +724,15 @@ int run_dev(dim3_ctx* c, bool count_mode, const int32_t* d_R, uint64_t nR, const
     if (!fp && cap && !d_out) return fail(c, DIM3_ERR_INVALID, "null output buffer");
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
+        return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
     TRY(prepare(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st));
     TRY(expand(c, fp, d_out, cap, st));
+    if (dense_overflowed(c)) {  // a multiplicity above 127: redo without the int8 product
+        o.dense_mode = DIM3_DENSE_OFF;
+        TRY(prepare(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st));
+        TRY(expand(c, fp, d_out, cap, st));
+    }
     fill_result(c, res, fp);
     if (!fp && res->n > cap)
         return fail(c, DIM3_ERR_CAPACITY, "output needs %llu rows, buffer holds %llu", (unsigned long long)res->n,
@@ -536,6 +757,8 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
     TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
     if (nR) HIPCHK(c, hipMemcpyAsync(dR, R, nR * 8, hipMemcpyHostToDevice, st));
     if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
+    if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
+        return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
     TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
     const size_t row_bytes = count_mode ? sizeof(dim3_count_row) : 8;
     void* dout = nullptr;
@@ -551,6 +774,11 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
         cap = c->ws[WS_OUT].bytes / row_bytes;
     }
     TRY(expand(c, fp, dout, cap, st));
+    if (dense_overflowed(c)) {  // a multiplicity above 127: redo without the int8 product
+        o.dense_mode = DIM3_DENSE_OFF;
+        TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
+        TRY(expand(c, fp, dout, cap, st));
+    }
     if (!fp && !c->plan.empty && c->h_sc->out_n > cap) {
         cap = c->h_sc->out_n;
         TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
@@ -624,7 +852,9 @@ int dim3_ctx_create(int device, uint64_t hbm_budget_bytes, dim3_ctx** out) {
     hipMemGetInfo(&fr, &tot);
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(fr * 0.9);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_hist, 6 * dim3h::kDenseBins * sizeof(unsigned long long), hipHostMallocDefault) !=
+            hipSuccess) {
         delete c;
         return DIM3_ERR_DEVICE;
     }
@@ -643,6 +873,7 @@ int dim3_ctx_destroy(dim3_ctx* c) {
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->h_sc) hipHostFree(c->h_sc);
+    if (c->h_hist) hipHostFree(c->h_hist);
     hipStreamDestroy(c->stream);
     delete c;
     return DIM3_OK;
@@ -719,6 +950,41 @@ int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const i
     HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     *out_j = c->h_sc->out_j;
+    return DIM3_OK;
+}
+
+int dim3_dense_product(dim3_ctx* c, const int8_t* A, uint64_t M, const int8_t* Bt, uint64_t N, uint64_t K,
+                       uint32_t* bits, int32_t* counts) {
+    if (!c) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    if (M == 0 || N == 0) return DIM3_OK;
+    if ((K && (!A || !Bt)) || (!bits && !counts)) return fail(c, DIM3_ERR_INVALID, "null argument");
+    if (K > 131072 || M >= 65535ull * 128ull || N >= (1ull << 31) || M * N >= (1ull << 40))
+        return fail(c, DIM3_ERR_INVALID, "dense product shape %llu x %llu x %llu out of range",
+                    (unsigned long long)M, (unsigned long long)N, (unsigned long long)K);
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const uint64_t Kp = std::max<uint64_t>((K + 63) / 64 * 64, 64);
+    const uint64_t dwords = (N + 31) / 32;
+    uint32_t *dA, *dB, *dbits = nullptr;
+    int32_t* dcnt = nullptr;
+    TRY(ws(c, WS_DA, M * Kp / 4, &dA));
+    TRY(ws(c, WS_DB, N * Kp / 4, &dB));
+    if (bits) TRY(ws(c, WS_DBITS, M * dwords, &dbits));
+    if (counts) TRY(ws(c, WS_CINT, M * N, &dcnt));
+    dim3h::fill_u32(dA, M * Kp / 4, 0, st);
+    dim3h::fill_u32(dB, N * Kp / 4, 0, st);
+    if (K) {
+        HIPCHK(c, hipMemcpy2DAsync(dA, Kp, A, K, K, M, hipMemcpyHostToDevice, st));
+        HIPCHK(c, hipMemcpy2DAsync(dB, Kp, Bt, K, K, N, hipMemcpyHostToDevice, st));
+    }
+    const int e = dim3h::gemm_i8((const int8_t*)dA, (const int8_t*)dB, (uint32_t)M, (uint32_t)N, (uint32_t)Kp, dbits,
+                                 (uint32_t)dwords, dcnt, st);
+    if (e != 0) return fail(c, DIM3_ERR_DEVICE, "dense product launch: %s", hipGetErrorString((hipError_t)e));
+    if (bits) HIPCHK(c, hipMemcpyAsync(bits, dbits, M * dwords * 4, hipMemcpyDeviceToHost, st));
+    if (counts) HIPCHK(c, hipMemcpyAsync(counts, dcnt, M * N * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
     return DIM3_OK;
 }
 

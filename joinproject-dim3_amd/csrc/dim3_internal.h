@@ -213,6 +213,22 @@ struct Scalars {
 
 size_t scan_partials_needed(uint64_t n);
 
+// Heaviness bins of the dense split: w < 16 exact, then four bins per octave
+// (w = min(degR(y), degS(y)) for a key; x and z carry the largest w of their
+// tuples).  128 bins cover every u32.
+constexpr uint32_t kDenseBins = 128;
+__host__ __device__ __forceinline__ uint32_t dense_bin(uint32_t w) {
+    if (w < 16u) return w;
+    const uint32_t e = 31u - (uint32_t)__builtin_clz(w);
+    return 16u + (e - 4u) * 4u + ((w >> (e - 2u)) & 3u);
+}
+// smallest w of bin b
+__host__ __device__ __forceinline__ uint64_t dense_bin_lo(uint32_t b) {
+    if (b < 16u) return b;
+    const uint32_t e = (b - 16u) / 4u + 4u, m = (b - 16u) % 4u;
+    return (uint64_t)(4u + m) << (e - 2u);
+}
+
 // Slot space of one attribute: an open-addressing table (hash mode) or the
 // value range itself (range mode: slot = (value - lo) * odd mod 2^k, or
 // value - lo when mask == 0).
@@ -255,9 +271,10 @@ void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
                const uint64_t* ex_x, uint32_t* cursor, int32_t* col, hipStream_t st);
 // tiled CSR of S by (y, z-tile); ex_z == nullptr means z identity (zid = z - zlo)
+// hid (nullable): heavy index per y ID; tuples of heavy keys are left out
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
-             const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS,
-             int32_t* zidS, hipStream_t st);
+             const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
+             uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st);
 void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, int32_t* col,
@@ -284,9 +301,45 @@ struct ExpandArgs {
     Scalars* sc;
     void* out;  // int2* (project) or dim3_count_row* (aggregate)
     uint64_t cap;
+    // dense seed (hx == nullptr: no dense part)
+    const int32_t* hx;        // product row of each x ID, -1 = none
+    const uint32_t* dbits;    // project: Mh x dwords bit rows over hz
+    const int32_t* cint;      // COUNT: Mh x Nh counts
+    const uint32_t* hz2zid;   // z ID of product column hz
+    const uint32_t* hz_tile;  // first column of each z tile, ntiles + 1 entries
+    uint32_t dwords;
+    uint32_t Nh;
 };
 
 // returns 0 or a hipError_t
+// dense part (k_dense.hip)
+int gemm_i8(const int8_t* A, const int8_t* Bt, uint32_t M, uint32_t N, uint32_t Kp, uint32_t* dbits,
+            uint32_t dwords, int32_t* cint, hipStream_t st);
+int bitor_rows(const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid, const uint32_t* hx2x, uint32_t Mh,
+               const uint32_t* bbits, uint32_t dwords, uint32_t* dbits, hipStream_t st);
+// largest key heaviness over the tuples of each x slot / z slot (slotSz == nullptr: z identity)
+void dense_weights(const int2* R, uint64_t nR, const uint32_t* slotR, const int32_t* slotRx, const int2* S,
+                   uint64_t nS, const int32_t* slotS, const int32_t* slotSz, int32_t zlo, const uint32_t* degR,
+                   const uint32_t* degS, uint32_t* xw, uint32_t* zw, hipStream_t st);
+// hist[6][kDenseBins]: keys, paths, sum degR, sum degS, x rows, z columns per heaviness bin
+void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint32_t* xw, uint32_t nx1,
+                const uint32_t* zw, uint32_t nz1, unsigned long long* hist, hipStream_t st);
+// heavy index per y ID, product row per x ID (+ inverse), column flag per z ID;
+// counters[0] = keys, counters[1] = rows
+void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
+                const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
+                const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
+                uint32_t* hx2x, uint32_t* zflag, uint32_t* counters, hipStream_t st);
+void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, uint32_t* hz2zid, uint32_t ntiles,
+                   uint32_t tile_size, uint32_t Nh, uint32_t* hz_tile, hipStream_t st);
+void dense_build_a(bool count, const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
+                   const uint64_t* ex_x, const int32_t* hid, const int32_t* hx, uint32_t Kp, uint32_t* A,
+                   uint64_t* overflow, hipStream_t st);
+// kind 0: int8 0/1 Bt, 1: int8 multiplicity Bt, 2: bitmap per heavy key
+void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
+                   int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* hzpos,
+                   uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow, hipStream_t st);
+
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block);

@@ -1,0 +1,409 @@
+// Dense part of the DIM3 split (subsystems b, c): heavy join keys as one
+// 0/1 (or multiplicity) int8 matrix product on the MFMA cores, or as OR-ed
+// z bitmaps.
+//
+// The reference's dense part (Hybrid_solution DENSE, dim3.hpp:894-950) keeps
+// one bitvector over y per dense z and tests each x's y list against it (AVX
+// testz sweep, or bit probes).  Here the split is by join key instead: keys
+// whose R and S degrees are both at or above a planner threshold are heavy;
+// every join path through a heavy key goes to
+//   C = A * B^T,  A [Mh rows x with a heavy key][Kp] int8 (A[hx][h] = multiplicity of (x, y_h))
+//                 Bt[Nh columns z with a heavy key][Kp] int8 (Bt[hz][h] = multiplicity of (z, y_h))
+// both K-contiguous, K padded to a multiple of 64, int32 accumulation (exact:
+// entries are at most 127 * 127 * Kp < 2^31 for Kp <= 131072).  Epilogues:
+//   project: bit rows Dbits[Mh][ceil(Nh / 32)] of C > 0, packed with ballots;
+//   COUNT:   int32 counts Cint[Mh][Nh].
+// The bit-packed alternative (project only) keeps one z bitmap per heavy key
+// and ORs the bitmaps of x's heavy keys into Dbits row hx.
+// Columns hz are numbered in z-ID order, so a z tile of the expansion is one
+// contiguous hz range; the expansion seeds each row's LDS tile from Dbits /
+// Cint before adding the light paths.  Heavy S tuples leave the sparse CSR.
+//
+// GEMM: 256-thread workgroup, 128 x 128 output tile, K step 64 staged in
+// LDS (row stride 80 bytes so the 16-byte fragment reads spread over banks),
+// each wavefront a 64 x 64 quadrant = 4 x 4 v_mfma_i32_16x16x64_i8.
+// Operand layout of that instruction (checked on gfx950): lane l holds row
+// (l % 16) of A and column (l % 16) of B for the 16 k values of group l / 16;
+// the result lane l holds column l % 16, rows 4 (l / 16) .. 4 (l / 16) + 3.
+#include <algorithm>
+
+#include "dim3_internal.h"
+
+namespace {
+
+using namespace dim3d;
+using dim3h::kDenseBins;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kTM = 128, kTN = 128, kTK = 64;
+constexpr int kLdsStride = 80;  // bytes per staged row (64 data + 16 pad)
+
+__global__ __launch_bounds__(256) void k_gemm_i8(const int8_t* __restrict__ A, const int8_t* __restrict__ Bt,
+                                                 uint32_t M, uint32_t N, uint32_t Kp, uint32_t* __restrict__ dbits,
+                                                 uint32_t dwords, int32_t* __restrict__ cint) {
+    __shared__ __align__(16) int8_t sA[kTM * kLdsStride];
+    __shared__ __align__(16) int8_t sB[kTN * kLdsStride];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    const uint32_t m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+    const uint32_t wm = (wid >> 1) * 64, wn = (wid & 1) * 64;  // wavefront quadrant
+
+    v4i acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = v4i{0, 0, 0, 0};
+
+    for (uint32_t k0 = 0; k0 < Kp; k0 += kTK) {
+        // stage 128 rows x 64 bytes of A and of Bt: 512 16-byte chunks each
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const uint32_t chunk = tid + (uint32_t)c * 256u;  // 0..511
+            const uint32_t r = chunk >> 2, q = chunk & 3u;
+            v4i va = v4i{0, 0, 0, 0}, vb = v4i{0, 0, 0, 0};
+            if (m0 + r < M) va = *reinterpret_cast<const v4i*>(A + (size_t)(m0 + r) * Kp + k0 + q * 16u);
+            if (n0 + r < N) vb = *reinterpret_cast<const v4i*>(Bt + (size_t)(n0 + r) * Kp + k0 + q * 16u);
+            *reinterpret_cast<v4i*>(sA + r * kLdsStride + q * 16u) = va;
+            *reinterpret_cast<v4i*>(sB + r * kLdsStride + q * 16u) = vb;
+        }
+        __syncthreads();
+        v4i fa[4], fb[4];
+        const uint32_t fr = lane & 15u, fk = (lane >> 4) * 16u;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            fa[i] = *reinterpret_cast<const v4i*>(sA + (wm + i * 16 + fr) * kLdsStride + fk);
+            fb[i] = *reinterpret_cast<const v4i*>(sB + (wn + i * 16 + fr) * kLdsStride + fk);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+
+    // result lane l: column (l % 16), rows 4 (l / 16) + e
+    const uint32_t col_in = lane & 15u, rgrp = (lane >> 4) * 4u;
+    if (cint != nullptr) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t col = n0 + wn + j * 16 + col_in;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint32_t row = m0 + wm + i * 16 + rgrp + e;
+                    if (row < M && col < N) cint[(size_t)row * N + col] = acc[i][j][e];
+                }
+            }
+    }
+    if (dbits != nullptr) {
+        // a ballot over one (i, j, e) gives rows 4g + e (g = lane group) x 16
+        // columns; two neighbouring j tiles make a 32-column word
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int jp = 0; jp < 2; jp++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint64_t b0 = __ballot(acc[i][2 * jp][e] > 0);
+                    const uint64_t b1 = __ballot(acc[i][2 * jp + 1][e] > 0);
+                    if (lane < 4u) {
+                        const uint32_t g = lane;
+                        const uint32_t word = (uint32_t)((b0 >> (16 * g)) & 0xFFFFull) |
+                                              ((uint32_t)((b1 >> (16 * g)) & 0xFFFFull) << 16);
+                        const uint32_t row = m0 + wm + i * 16 + 4 * g + e;
+                        const uint32_t wcol = (n0 + wn + jp * 32) >> 5;
+                        if (row < M && wcol < dwords) dbits[(size_t)row * dwords + wcol] = word;
+                    }
+                }
+    }
+}
+
+// Bit-packed product: Dbits row r = OR of the z bitmaps of the heavy keys of
+// x = hx2x[r].  One workgroup per row; each thread owns words tid + 256 j of
+// a 2048-word chunk of the row.
+constexpr uint32_t kOrWords = 8;
+__global__ __launch_bounds__(256) void k_bitor(const uint32_t* __restrict__ rowptrR, const int32_t* __restrict__ colR,
+                                               const int32_t* __restrict__ hid, const uint32_t* __restrict__ hx2x,
+                                               uint32_t Mh, const uint32_t* __restrict__ bbits, uint32_t dwords,
+                                               uint32_t* __restrict__ dbits) {
+    const uint32_t r = blockIdx.x;
+    if (r >= Mh) return;
+    const uint32_t x = hx2x[r];
+    const uint32_t rb = rowptrR[x], re = rowptrR[x + 1];
+    for (uint32_t w0 = 0; w0 < dwords; w0 += 256u * kOrWords) {
+        uint32_t acc[kOrWords];
+#pragma unroll
+        for (uint32_t j = 0; j < kOrWords; j++) acc[j] = 0;
+        for (uint32_t i = rb; i < re; i++) {
+            const int32_t h = hid[colR[i]];
+            if (h < 0) continue;
+            const uint32_t* src = bbits + (size_t)h * dwords;
+#pragma unroll
+            for (uint32_t j = 0; j < kOrWords; j++) {
+                const uint32_t w = w0 + threadIdx.x + 256u * j;
+                if (w < dwords) acc[j] |= src[w];
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kOrWords; j++) {
+            const uint32_t w = w0 + threadIdx.x + 256u * j;
+            if (w < dwords) dbits[(size_t)r * dwords + w] = acc[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// planning: heaviness w(y) = min(degR(y), degS(y)); x and z carry the largest
+// w over their tuples.  Bins: w < 16 exact, then four bins per octave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t key_w(const uint32_t* degR, const uint32_t* degS, uint32_t s) {
+    return min(degR[s], degS[s]);
+}
+
+__global__ void k_wmax_x(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
+                         const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS, uint32_t* xw) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nR) return;
+    const int32_t sx = slotRx[i];
+    if (sx < 0) return;
+    const uint32_t w = key_w(degR, degS, slotR[i]);
+    if (w > xw[sx]) atomicMax(&xw[sx], w);
+}
+
+__global__ void k_wmax_z(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
+                         const int32_t* __restrict__ slotSz, int32_t zlo, uint64_t nS,
+                         const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS, uint32_t* zw) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    const int32_t sy = slotS[i];
+    if (sy < 0) return;
+    const uint32_t zs = slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i];
+    const uint32_t w = key_w(degR, degS, (uint32_t)sy);
+    if (w > zw[zs]) atomicMax(&zw[zs], w);
+}
+
+// hist layout: [6][kDenseBins] u64 = keys, paths, sum degR, sum degS, x rows, z columns
+__global__ __launch_bounds__(256) void k_dense_hist(const uint32_t* __restrict__ degR,
+                                                    const uint32_t* __restrict__ degS, uint32_t ny1,
+                                                    const uint32_t* __restrict__ xw, uint32_t nx1,
+                                                    const uint32_t* __restrict__ zw, uint32_t nz1,
+                                                    unsigned long long* hist) {
+    __shared__ unsigned long long lh[6 * kDenseBins];
+    for (uint32_t i = threadIdx.x; i < 6 * kDenseBins; i += 256) lh[i] = 0;
+    __syncthreads();
+    const uint32_t n = max(ny1, max(nx1, nz1));
+    for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < n; s += gridDim.x * 256) {
+        if (s < ny1) {
+            const uint32_t r = degR[s], q = degS[s];
+            if (r != 0 && q != 0) {
+                const uint32_t b = dim3h::dense_bin(min(r, q));
+                atomicAdd(&lh[b], 1ull);
+                atomicAdd(&lh[kDenseBins + b], (unsigned long long)r * q);
+                atomicAdd(&lh[2 * kDenseBins + b], (unsigned long long)r);
+                atomicAdd(&lh[3 * kDenseBins + b], (unsigned long long)q);
+            }
+        }
+        if (s < nx1 && xw[s] != 0) atomicAdd(&lh[4 * kDenseBins + dim3h::dense_bin(xw[s])], 1ull);
+        if (s < nz1 && zw[s] != 0) atomicAdd(&lh[5 * kDenseBins + dim3h::dense_bin(zw[s])], 1ull);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 6 * kDenseBins; i += 256)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
+// heavy index of every joined y ID (-1 = light)
+__global__ void k_mark_hid(const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS, uint32_t ny1,
+                           const uint64_t* __restrict__ ex_y, uint32_t bmin, int32_t* __restrict__ hid,
+                           uint32_t* kcount) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ny1) return;
+    const uint32_t r = degR[s], q = degS[s];
+    if (r == 0 || q == 0) return;
+    int32_t h = -1;
+    if (dim3h::dense_bin(min(r, q)) >= bmin) h = (int32_t)atomicAdd(kcount, 1u);
+    hid[(uint32_t)(ex_y[s] >> 32)] = h;
+}
+
+// row index hx of every x ID (-1 = no heavy key) and its inverse
+__global__ void k_mark_hx(const uint32_t* __restrict__ degX, const uint64_t* __restrict__ ex_x, uint32_t nx1,
+                          const uint32_t* __restrict__ xw, uint32_t bmin, int32_t* __restrict__ hx,
+                          uint32_t* __restrict__ hx2x, uint32_t* mcount) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nx1 || degX[s] == 0) return;
+    const uint32_t xid = (uint32_t)(ex_x[s] >> 32);
+    int32_t v = -1;
+    if (xw[s] != 0 && dim3h::dense_bin(xw[s]) >= bmin) {
+        v = (int32_t)atomicAdd(mcount, 1u);
+        hx2x[v] = xid;
+    }
+    hx[xid] = v;
+}
+
+// column flag of every z ID (hash mode: occupied slots; identity: every ID)
+__global__ void k_mark_zflag(const uint32_t* __restrict__ degZ, const uint64_t* __restrict__ ex_z, uint32_t nz1,
+                             const uint32_t* __restrict__ zw, uint32_t bmin, uint32_t* __restrict__ zflag) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nz1) return;
+    uint32_t zid = s;
+    if (ex_z != nullptr) {
+        if (degZ[s] == 0) return;
+        zid = (uint32_t)(ex_z[s] >> 32);
+    }
+    zflag[zid] = (zw[s] != 0 && dim3h::dense_bin(zw[s]) >= bmin) ? 1u : 0u;
+}
+
+// hz -> z ID, and the first column of every z tile (ntiles + 1 entries)
+__global__ void k_hz_fill(const uint32_t* __restrict__ zflag, const uint32_t* __restrict__ hzpos, uint32_t nz,
+                          uint32_t* __restrict__ hz2zid, uint32_t ntiles, uint32_t tile_size, uint32_t Nh,
+                          uint32_t* __restrict__ hz_tile) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nz && zflag[i]) hz2zid[hzpos[i]] = i;
+    if (i <= ntiles) {
+        const uint64_t zt = (uint64_t)i * tile_size;
+        hz_tile[i] = zt < nz ? hzpos[zt] : Nh;
+    }
+}
+
+// A[hx][h] for each R tuple with a heavy key (byte lanes of u32 words):
+// project sets the byte to 1, COUNT adds the multiplicity
+template <bool COUNT>
+__global__ void k_build_a(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
+                          const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_x,
+                          const int32_t* __restrict__ hid, const int32_t* __restrict__ hx, uint32_t Kp, uint32_t* A,
+                          uint64_t* overflow) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nR) return;
+    const int32_t sx = slotRx[i];
+    if (sx < 0) return;
+    const int32_t h = hid[(uint32_t)(ex_y[slotR[i]] >> 32)];
+    if (h < 0) return;
+    const uint64_t byte = (uint64_t)hx[(uint32_t)(ex_x[sx] >> 32)] * Kp + (uint32_t)h;
+    const uint32_t sh = (uint32_t)(byte & 3u) * 8u;
+    if (COUNT) {
+        const uint32_t old = atomicAdd(&A[byte >> 2], 1u << sh);
+        if (((old >> sh) & 0xFFu) >= 127u) atomicOr((unsigned long long*)overflow, 1ull);
+    } else {
+        atomicOr(&A[byte >> 2], 1u << sh);
+    }
+}
+
+__device__ __forceinline__ uint32_t z_of(const int2* S, uint64_t i, const int32_t* slotSz, const uint64_t* ex_z,
+                                         int32_t zlo) {
+    return ex_z == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)(ex_z[slotSz[i]] >> 32);
+}
+
+// Bt[hz][h] for each S tuple with a heavy key (MFMA), or bit hz of z bitmap h (bit-or)
+template <bool COUNT, bool BITS>
+__global__ void k_build_b(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
+                          const int32_t* __restrict__ slotSz, const uint64_t* __restrict__ ex_z, int32_t zlo,
+                          uint64_t nS, const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid,
+                          const uint32_t* __restrict__ hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B,
+                          uint64_t* overflow) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    const int32_t sy = slotS[i];
+    if (sy < 0) return;
+    const int32_t h = hid[(uint32_t)(ex_y[sy] >> 32)];
+    if (h < 0) return;
+    const uint32_t hz = hzpos[z_of(S, i, slotSz, ex_z, zlo)];
+    if (BITS) {
+        atomicOr(&B[(size_t)h * dwords + (hz >> 5)], 1u << (hz & 31u));
+        return;
+    }
+    const uint64_t byte = (uint64_t)hz * Kp + (uint32_t)h;
+    const uint32_t sh = (uint32_t)(byte & 3u) * 8u;
+    if (COUNT) {
+        const uint32_t old = atomicAdd(&B[byte >> 2], 1u << sh);
+        if (((old >> sh) & 0xFFu) >= 127u) atomicOr((unsigned long long*)overflow, 1ull);
+    } else {
+        atomicOr(&B[byte >> 2], 1u << sh);
+    }
+}
+
+inline unsigned grid1(uint64_t n) { return (unsigned)(n == 0 ? 1 : (n + 255) / 256); }
+
+}  // namespace
+
+namespace dim3h {
+
+int gemm_i8(const int8_t* A, const int8_t* Bt, uint32_t M, uint32_t N, uint32_t Kp, uint32_t* dbits,
+            uint32_t dwords, int32_t* cint, hipStream_t st) {
+    if (M == 0 || N == 0) return 0;
+    if (Kp % kTK != 0) return (int)hipErrorInvalidValue;
+    dim3 grid((N + kTN - 1) / kTN, (M + kTM - 1) / kTM);
+    if (grid.y > 65535) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gemm_i8, grid, dim3(256), 0, st, A, Bt, M, N, Kp, dbits, dwords, cint);
+    return (int)hipGetLastError();
+}
+
+int bitor_rows(const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid, const uint32_t* hx2x, uint32_t Mh,
+               const uint32_t* bbits, uint32_t dwords, uint32_t* dbits, hipStream_t st) {
+    if (Mh == 0) return 0;
+    hipLaunchKernelGGL(k_bitor, dim3(Mh), dim3(256), 0, st, rowptrR, colR, hid, hx2x, Mh, bbits, dwords, dbits);
+    return (int)hipGetLastError();
+}
+
+void dense_weights(const int2* R, uint64_t nR, const uint32_t* slotR, const int32_t* slotRx, const int2* S,
+                   uint64_t nS, const int32_t* slotS, const int32_t* slotSz, int32_t zlo, const uint32_t* degR,
+                   const uint32_t* degS, uint32_t* xw, uint32_t* zw, hipStream_t st) {
+    (void)R;
+    if (nR) hipLaunchKernelGGL(k_wmax_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, degR, degS, xw);
+    if (nS)
+        hipLaunchKernelGGL(k_wmax_z, dim3(grid1(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, degR, degS, zw);
+}
+
+void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint32_t* xw, uint32_t nx1,
+                const uint32_t* zw, uint32_t nz1, unsigned long long* hist, hipStream_t st) {
+    uint64_t n = std::max(ny1, std::max(nx1, nz1));
+    unsigned g = grid1(n);
+    if (g > 1024) g = 1024;
+    hipLaunchKernelGGL(k_dense_hist, dim3(g), dim3(256), 0, st, degR, degS, ny1, xw, nx1, zw, nz1, hist);
+}
+
+void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
+                const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
+                const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
+                uint32_t* hx2x, uint32_t* zflag, uint32_t* counters, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_hid, dim3(grid1(ny1)), dim3(256), 0, st, degR, degS, ny1, ex_y, bmin, hid, counters);
+    hipLaunchKernelGGL(k_mark_hx, dim3(grid1(nx1)), dim3(256), 0, st, degX, ex_x, nx1, xw, bmin, hx, hx2x,
+                       counters + 1);
+    hipLaunchKernelGGL(k_mark_zflag, dim3(grid1(nz1)), dim3(256), 0, st, degZ, ex_z, nz1, zw, bmin, zflag);
+}
+
+void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, uint32_t* hz2zid, uint32_t ntiles,
+                   uint32_t tile_size, uint32_t Nh, uint32_t* hz_tile, hipStream_t st) {
+    const uint64_t n = std::max<uint64_t>(nz, (uint64_t)ntiles + 1);
+    hipLaunchKernelGGL(k_hz_fill, dim3(grid1(n)), dim3(256), 0, st, zflag, hzpos, nz, hz2zid, ntiles, tile_size, Nh,
+                       hz_tile);
+}
+
+void dense_build_a(bool count, const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
+                   const uint64_t* ex_x, const int32_t* hid, const int32_t* hx, uint32_t Kp, uint32_t* A,
+                   uint64_t* overflow, hipStream_t st) {
+    if (!nR) return;
+    if (count)
+        hipLaunchKernelGGL(k_build_a<true>, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, ex_x, hid, hx,
+                           Kp, A, overflow);
+    else
+        hipLaunchKernelGGL(k_build_a<false>, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, ex_x, hid,
+                           hx, Kp, A, overflow);
+}
+
+void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
+                   int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* hzpos,
+                   uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow, hipStream_t st) {
+    if (!nS) return;
+    const unsigned g = grid1(nS);
+    if (kind == 2)
+        hipLaunchKernelGGL((k_build_b<false, true>), dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y,
+                           hid, hzpos, Kp, dwords, B, overflow);
+    else if (kind == 1)
+        hipLaunchKernelGGL((k_build_b<true, false>), dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y,
+                           hid, hzpos, Kp, dwords, B, overflow);
+    else
+        hipLaunchKernelGGL((k_build_b<false, false>), dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS,
+                           ex_y, hid, hzpos, Kp, dwords, B, overflow);
+}
+
+}  // namespace dim3h

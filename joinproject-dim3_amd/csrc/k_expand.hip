@@ -190,6 +190,36 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         const int32_t xv = a.id2x[x];
         const int32_t zs_self = a.x2z != nullptr ? a.x2z[x] : -1;
 
+        // ---- dense seed: a row with a heavy key starts from its product row ----
+        const int32_t hxr = a.hx != nullptr ? a.hx[x] : -1;
+        if (hxr >= 0) {
+            const uint32_t h0 = a.hz_tile[t], h1 = a.hz_tile[t + 1];
+            if (COUNT) {
+                // distinct columns are distinct counters of a zeroed tile
+                const int32_t* crow = a.cint + (size_t)hxr * a.Nh;
+                for (uint32_t h = h0 + tid; h < h1; h += BLOCK) {
+                    const int32_t v = crow[h];
+                    if (v != 0) tile[a.hz2zid[h] - zbase] = (uint32_t)v;
+                }
+            } else if (h0 < h1) {
+                const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
+                const uint32_t wl = h0 >> 5, wh = (h1 - 1u) >> 5;
+                for (uint32_t wd = wl + tid; wd <= wh; wd += BLOCK) {
+                    uint32_t v = brow[wd];
+                    const uint32_t base = wd * 32u;
+                    if (base < h0) v &= ~0u << (h0 - base);
+                    if (h1 - base < 32u) v &= (1u << (h1 - base)) - 1u;
+                    while (v) {
+                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                        v &= v - 1u;
+                        const uint32_t zl = a.hz2zid[base + b] - zbase;
+                        atomicOr(&tile[swz_word(zl, W, C)], 1u << (zl & 31u));
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
         // ---- expansion ----
         bool track = true;
         for (uint32_t c0 = rb; c0 < re; c0 += BLOCK) {
@@ -208,7 +238,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             uint32_t NP;
             const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(np, NP, sh32);
             s_pre[tid] = pre;
-            if (c0 == rb) track = (uint64_t)NP * kPiece <= track_limit;
+            if (c0 == rb) track = hxr < 0 && (uint64_t)NP * kPiece <= track_limit;
             __syncthreads();
             const uint32_t nseg = min((uint32_t)BLOCK, re - c0);
             // Pieces come in increasing order, so each wavefront gallops

@@ -340,15 +340,16 @@ __device__ __forceinline__ uint32_t z_id(const int2* S, uint64_t i, const int32_
 __global__ void k_count_s(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
                           const int32_t* __restrict__ slotSz, uint64_t nS, const uint64_t* __restrict__ ex_y,
                           const uint64_t* __restrict__ ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size,
-                          uint32_t* cnt, uint32_t* __restrict__ keyS, int32_t* __restrict__ zidS) {
+                          const int32_t* __restrict__ hid, uint32_t* cnt, uint32_t* __restrict__ keyS,
+                          int32_t* __restrict__ zidS) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const int32_t sy = slotS[i];
-    if (sy < 0) {
+    const uint32_t yid = sy < 0 ? 0u : (uint32_t)(ex_y[sy] >> 32);
+    if (sy < 0 || (hid != nullptr && hid[yid] >= 0)) {  // no match, or a heavy key (dense part)
         keyS[i] = 0xFFFFFFFFu;
         return;
     }
-    const uint32_t yid = (uint32_t)(ex_y[sy] >> 32);
     const uint32_t zid = z_id(S, i, slotSz, ex_z, zlo);
     const uint32_t key = ntiles == 1 ? yid : yid * ntiles + zid / tile_size;
     atomicAdd(&cnt[key], 1u);
@@ -449,11 +450,11 @@ void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const 
                            cursor, col);
 }
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
-             const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, uint32_t* cnt, uint32_t* keyS,
-             int32_t* zidS, hipStream_t st) {
+             const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
+             uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
     if (nS)
         hipLaunchKernelGGL(k_count_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, slotS, slotSz, nS, ex_y, ex_z,
-                           zlo, ntiles, tile_size, cnt, keyS, zidS);
+                           zlo, ntiles, tile_size, hid, cnt, keyS, zidS);
 }
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st) {
