@@ -62,9 +62,10 @@ def measured_traffic(args):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(E, target_s, slices):
+def cpu_baseline(E, target_s, slices, S=None, self_filter=True):
     """Reference DIM3_parallel on vertex slices x = r (mod slices) of R
     against all of S; stops once target_s of CPU time has been spent."""
+    S = E if S is None else S
     import oracle
 
     if not oracle.ref_available():
@@ -75,10 +76,10 @@ def cpu_baseline(E, target_s, slices):
     for r in range(slices):
         Rs = E[(E[:, 0] % slices) == r]
         t0 = time.perf_counter()
-        out = oracle.ref_join_project(Rs, E, parallel=True)
+        out = oracle.ref_join_project(Rs, S, parallel=True)
         dt = time.perf_counter() - t0
         # the reference has no self-pair filter; apply it as the harness does
-        n = int(np.count_nonzero(out[:, 0] != out[:, 1]))
+        n = int(np.count_nonzero(out[:, 0] != out[:, 1])) if self_filter else int(out.shape[0])
         del out
         total_pairs += n
         total_t += dt
@@ -88,8 +89,27 @@ def cpu_baseline(E, target_s, slices):
             break
     return {"value": total_pairs / total_t, "unit": "pairs/s", "cores": threads, "kind": "reference",
             "sample": "reference DIM3_parallel (oracle/_ref, %d OpenMP threads) on %d of %d vertex slices "
-                      "(R restricted to src %% %d == r, all of S), %d pairs in %.2f s; u != w applied "
-                      "after the reference" % (threads, len(done), slices, slices, total_pairs, total_t)}
+                      "(R restricted to src %% %d == r, all of S), %d pairs in %.2f s%s"
+                      % (threads, len(done), slices, slices, total_pairs, total_t,
+                         "; u != w applied after the reference" if self_filter else "")}
+
+
+def device_block(torch, block_keys, density, seed, chunk_rows=4096):
+    """(attr, key) cells of a block_keys-square grid, each kept with
+    probability `density`, drawn and shuffled on the device (torch's Philox
+    generator, seeded); returned as a host int32 array."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    parts = []
+    for r0 in range(0, block_keys, chunk_rows):
+        rows = min(chunk_rows, block_keys - r0)
+        m = torch.rand((rows, block_keys), device="cuda", generator=g) < density
+        nz = torch.nonzero(m).to(torch.int32)
+        nz[:, 0] += r0
+        parts.append(nz)
+    B = torch.cat(parts)
+    B = B[torch.randperm(B.shape[0], device="cuda", generator=g)]
+    return B.cpu().numpy()
 
 
 def main():
@@ -101,6 +121,12 @@ def main():
     ap.add_argument("--edges", type=int, default=4_000_000)
     ap.add_argument("--max-degree", type=int, default=50_000)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--workload", choices=["two_hop", "planted_block"], default="two_hop",
+                    help="two_hop: the Chung-Lu self-join (default); planted_block: R and S are each a "
+                         "--block-keys square block of (attr, key) cells at --block-density (the dense block "
+                         "of the served stream's large requests), for the dense-part comparison")
+    ap.add_argument("--block-keys", type=int, default=20_000)
+    ap.add_argument("--block-density", type=float, default=0.3)
     ap.add_argument("--mode", choices=["materialize", "fingerprint"], default="materialize")
     ap.add_argument("--dense", type=int, default=-1, help="dense mode: -1 auto, 0 off, 1 bit-or, 2 mfma")
     ap.add_argument("--block", type=int, default=0)
@@ -137,19 +163,27 @@ def main():
     from dim3_amd import workloads
 
     t0 = time.perf_counter()
-    E = workloads.two_hop_graph(args.vertices, args.edges, 2.1, args.max_degree, args.seed)
-    log("graph: %d directed pairs in %.1fs" % (E.shape[0], time.perf_counter() - t0))
+    planted = args.workload == "planted_block"
+    if planted:
+        E = device_block(torch, args.block_keys, args.block_density, args.seed)
+        E2 = device_block(torch, args.block_keys, args.block_density, args.seed + 1)
+        log("planted block: %d + %d tuples in %.1fs" % (E.shape[0], E2.shape[0], time.perf_counter() - t0))
+    else:
+        E = workloads.two_hop_graph(args.vertices, args.edges, 2.1, args.max_degree, args.seed)
+        E2 = E
+        log("graph: %d directed pairs in %.1fs" % (E.shape[0], time.perf_counter() - t0))
 
     ctx = d.Context(0)
     dev = torch.device("cuda:0")
     dE = torch.from_numpy(E).to(dev)
+    dE2 = dE if E2 is E else torch.from_numpy(E2).to(dev)
     torch.cuda.synchronize()
-    kw = dict(exclude_equal=True, dense_mode=args.dense, block=args.block, tile_bits=args.tile_bits,
+    kw = dict(exclude_equal=not planted, dense_mode=args.dense, block=args.block, tile_bits=args.tile_bits,
               output_mode=1 if args.mode == "fingerprint" else 0, hash_only=args.hash_only)
 
     # size the output: a capacity-0 call reports the exact row count
     if args.mode == "materialize":
-        rc, r0 = ctx.join_project_dev(dE.data_ptr(), E.shape[0], dE.data_ptr(), E.shape[0], 0, 0, **kw)
+        rc, r0 = ctx.join_project_dev(dE.data_ptr(), E.shape[0], dE2.data_ptr(), E2.shape[0], 0, 0, **kw)
         n_out = r0.n
         out = torch.empty((max(n_out, 1), 2), dtype=torch.int32, device=dev)
         out_ptr, cap = out.data_ptr(), n_out
@@ -159,7 +193,7 @@ def main():
     log("output rows: %s" % n_out)
 
     def step():
-        rc, res = ctx.join_project_dev(dE.data_ptr(), E.shape[0], dE.data_ptr(), E.shape[0], out_ptr, cap, **kw)
+        rc, res = ctx.join_project_dev(dE.data_ptr(), E.shape[0], dE2.data_ptr(), E2.shape[0], out_ptr, cap, **kw)
         if rc != d.OK:
             raise RuntimeError("query failed: %s" % ctx.last_error())
         return res
@@ -188,9 +222,30 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(E, args.cpu_seconds, args.cpu_slices)
+            cpu = cpu_baseline(E, args.cpu_seconds, args.cpu_slices, S=E2, self_filter=not planted)
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %r" % (e,))
+
+    output = "materialized int32 pairs in HBM" if args.mode == "materialize" else "fingerprint only"
+    if planted:
+        config = {"workload": "planted dense block join-project", "block_keys": args.block_keys,
+                  "density": args.block_density, "tuples_r": int(E.shape[0]), "tuples_s": int(E2.shape[0]),
+                  "seed": args.seed, "output": output, "distinct_pairs": int(n_out), "join_paths": int(st.out_j),
+                  "parallelism": "single GPU"}
+    else:
+        config = {"workload": "two-hop self-join-project, Chung-Lu graph", "vertices": args.vertices,
+                  "undirected_edges": args.edges, "directed_pairs": int(E.shape[0]), "gamma": 2.1,
+                  "max_degree": args.max_degree, "seed": args.seed, "filter": "u != w", "output": output,
+                  "distinct_pairs": int(n_out), "join_paths": int(st.out_j), "parallelism": "single GPU"}
+    dense = None
+    if st.dense_mode:
+        kp = (int(st.heavy_keys) + 63) // 64 * 64
+        dms = float(np.mean([r.stats.ms_dense for r in ress]))
+        dense = {"mode": {1: "bitor", 2: "mfma"}[int(st.dense_mode)], "heavy_keys": int(st.heavy_keys),
+                 "rows": int(st.nx_heavy), "cols": int(st.nz_heavy), "phase_ms": dms}
+        if st.dense_mode == 2:
+            ops = 2.0 * st.nx_heavy * st.nz_heavy * kp
+            dense.update({"int8_ops": ops, "phase_tops": ops / (dms * 1e-3) / 1e12, "peak_tops": 5000.0})
 
     line = {
         "metric": METRIC,
@@ -205,16 +260,12 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (seeded Chung-Lu graph generated on the host, resident in HBM before timing)",
-        "config": {"workload": "two-hop self-join-project, Chung-Lu graph", "vertices": args.vertices,
-                   "undirected_edges": args.edges, "directed_pairs": int(E.shape[0]), "gamma": 2.1,
-                   "max_degree": args.max_degree, "seed": args.seed, "filter": "u != w",
-                   "output": "materialized int32 (u,w) pairs in HBM" if args.mode == "materialize"
-                   else "fingerprint only", "distinct_pairs": int(n_out), "join_paths": int(st.out_j),
-                   "parallelism": "single GPU"},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
                      "kernel": "k_expand", "kernel_ms": exp_ms, "alg_bytes": b_kernel},
         "cpu_baseline": cpu,
+        "dense": dense,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

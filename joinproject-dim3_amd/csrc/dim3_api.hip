@@ -61,6 +61,7 @@ enum SlotId {
     WS_ZW,
     WS_DHIST,
     WS_HID,
+    WS_H2Y,
     WS_HX,
     WS_HX2X,
     WS_ZFLAG,
@@ -275,7 +276,7 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
 // ---------------------------------------------------------------------------
 struct DenseModel {
     double path_s = 0.9e-12;   // sparse expansion, per join path
-    double seed_s = 0.3e-12;   // seeding, per product bit or count
+    double seed_s = 1.0e-12;   // seeding, per product bit or count (a gathered z ID + one LDS op)
     double mfma_ops = 1.5e15;  // int8 ops/s reached by k_gemm_i8
     double hbm_bps = 5.0e12;   // streaming bytes/s
     double l2_bps = 10.0e12;   // bit-or re-reads of key bitmaps
@@ -532,29 +533,31 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     TRY(ws(c, WS_X2Z, P.nx, &x2z));
     TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed(std::max<uint64_t>(maxscan, nkeyS + 1)), &part));
 
-    dim3h::compact_ids(P.mx, degX, exX, id2x, rowR, st);
-    dim3h::finish_rowptr(rowR, P.nx, &sc->tot_x, st);
-    if (!P.z_ident) dim3h::compact_ids(P.mz, degZ, exZ, id2z, nullptr, st);
-    HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
-    dim3h::scatter_r(slotR, slotRx, nR, exY, exX, curR, colR, st);
-    dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
     int32_t *hid = nullptr, *hx = nullptr;
-    uint32_t *hx2x = nullptr, *zflag = nullptr, *hzpos = nullptr, *hz2zid = nullptr, *hz_tile = nullptr;
+    uint32_t *hx2x = nullptr, *h2y = nullptr, *zflag = nullptr, *hzpos = nullptr, *hz2zid = nullptr,
+             *hz_tile = nullptr;
     if (P.dense != DIM3_DENSE_OFF) {
         // heavy keys, product rows and columns (columns numbered in z-ID order)
         TRY(ws(c, WS_HID, P.ny, &hid));
         TRY(ws(c, WS_HX, P.nx, &hx));
         TRY(ws(c, WS_HX2X, P.Mh, &hx2x));
+        TRY(ws(c, WS_H2Y, P.K, &h2y));
         TRY(ws(c, WS_ZFLAG, P.nz, &zflag));
         TRY(ws(c, WS_HZPOS, (uint64_t)P.nz + 1, &hzpos));
         TRY(ws(c, WS_HZ2ZID, P.Nh, &hz2zid));
         TRY(ws(c, WS_HZTILE, (uint64_t)P.ntiles + 1, &hz_tile));
         dim3h::dense_mark(degR, degS, (uint32_t)ny1, exY, degX, exX, (uint32_t)nx1, xw, P.z_ident ? nullptr : degZ,
-                          P.z_ident ? nullptr : exZ, (uint32_t)nz1, zw, P.bmin, hid, hx, hx2x, zflag,
+                          P.z_ident ? nullptr : exZ, (uint32_t)nz1, zw, P.bmin, hid, hx, hx2x, h2y, zflag,
                           (uint32_t*)&sc->spare[0], st);
         dim3h::scan_u32(zflag, P.nz, hzpos, part, &sc->nzh, st);
         dim3h::dense_columns(zflag, hzpos, P.nz, hz2zid, P.ntiles, P.tile_size, P.Nh, hz_tile, st);
     }
+    dim3h::compact_ids(P.mx, degX, exX, id2x, rowR, st);
+    dim3h::finish_rowptr(rowR, P.nx, &sc->tot_x, st);
+    if (!P.z_ident) dim3h::compact_ids(P.mz, degZ, exZ, id2z, nullptr, st);
+    HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
+    dim3h::scatter_r(slotR, slotRx, nR, exY, exX, hid, curR, colR, st);
+    dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
     dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, hid, cntS, keyS,
                    zidS, st);
     dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
@@ -573,12 +576,13 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         if (P.count_mode) TRY(ws(c, WS_CINT, (uint64_t)P.Mh * P.Nh, &cint));
         else TRY(ws(c, WS_DBITS, (uint64_t)P.Mh * P.dwords, &dbits));
         if (P.dense == DIM3_DENSE_MFMA) {
+            // A [Mh][Kp] from R's CSR rows, Bt [Nh][Kp] from the heavy S tuples
             uint32_t* A;
             TRY(ws(c, WS_DA, (uint64_t)P.Mh * P.Kp / 4, &A));
             TRY(ws(c, WS_DB, (uint64_t)P.Nh * P.Kp / 4, &B));
-            dim3h::fill_u32(A, (uint64_t)P.Mh * P.Kp / 4, 0, st);
+            if (P.Kp > 65536) dim3h::fill_u32(A, (uint64_t)P.Mh * P.Kp / 4, 0, st);
             dim3h::fill_u32(B, (uint64_t)P.Nh * P.Kp / 4, 0, st);
-            dim3h::dense_build_a(P.count_mode, slotR, slotRx, nR, exY, exX, hid, hx, P.Kp, A, overflow, st);
+            dim3h::dense_build_a(P.count_mode, rowR, colR, hid, hx2x, P.Mh, P.Kp, A, overflow, st);
             dim3h::dense_build_b(P.count_mode ? 1 : 0, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY,
                                  hid, hzpos, P.Kp, P.dwords, B, overflow, st);
             const int e = dim3h::gemm_i8((const int8_t*)A, (const int8_t*)B, P.Mh, P.Nh, P.Kp, dbits, P.dwords, cint,

@@ -135,6 +135,46 @@ __device__ __forceinline__ T wave_reduce_xor(T v) {
     return v;
 }
 
+// Wavefront-aggregated counter updates for skewed keys (Zipf join keys, hub
+// vertices): in two rounds, the lanes whose key equals the first pending
+// lane's key share one atomic; the remaining lanes add on their own.  Every
+// lane still running must call these (valid = false lanes take part without
+// counting).
+__device__ __forceinline__ void wave_count_add(uint32_t* ctr, uint32_t key, bool valid) {
+    uint64_t pending = __ballot(valid);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        if (pending == 0) return;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)leader);
+        const uint64_t same = __ballot(valid && key == lk) & pending;
+        if (lane_id() == leader) atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
+        pending &= ~same;
+    }
+    if ((pending >> lane_id()) & 1ull) atomicAdd(&ctr[key], 1u);
+}
+
+// As wave_count_add, returning each valid lane's reserved position (the
+// counter's old value plus the lane's rank among the lanes sharing its atomic).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* ctr, uint32_t key, bool valid) {
+    uint64_t pending = __ballot(valid);
+    uint32_t pos = 0;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        if (pending == 0) break;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)leader);
+        const uint64_t same = __ballot(valid && key == lk) & pending;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
+        if ((same >> lane_id()) & 1ull) pos = base + mbcnt64(same);
+        pending &= ~same;
+    }
+    if ((pending >> lane_id()) & 1ull) pos = atomicAdd(&ctr[key], 1u);
+    return pos;
+}
+
 // Exclusive scan over the block; sh needs BLOCK/64+1 entries.  Ends with a
 // barrier so sh can be reused right away.
 template <typename T>
@@ -267,11 +307,11 @@ void scan_deg(const uint32_t* deg, uint32_t nslots, uint64_t* ex, uint64_t* part
 void compact_ids(const KeyMap& m, const uint32_t* deg, const uint64_t* ex, int32_t* id2v, uint32_t* rowptr,
                  hipStream_t st);
 void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_t st);
-// CSR of R by x: col = y id
+// CSR of R by x: col = y id, with bit 31 set for heavy keys (hid[y] >= 0; hid nullable)
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
-               const uint64_t* ex_x, uint32_t* cursor, int32_t* col, hipStream_t st);
+               const uint64_t* ex_x, const int32_t* hid, uint32_t* cursor, int32_t* col, hipStream_t st);
 // tiled CSR of S by (y, z-tile); ex_z == nullptr means z identity (zid = z - zlo)
-// hid (nullable): heavy index per y ID; tuples of heavy keys are left out
+// hid (nullable): heavy index per y ID; tuples of heavy keys are left out (dense part)
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
              const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
              uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st);
@@ -329,13 +369,14 @@ void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const 
 void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
                 const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
                 const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
-                uint32_t* hx2x, uint32_t* zflag, uint32_t* counters, hipStream_t st);
+                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, uint32_t* counters, hipStream_t st);
 void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, uint32_t* hz2zid, uint32_t ntiles,
                    uint32_t tile_size, uint32_t Nh, uint32_t* hz_tile, hipStream_t st);
-void dense_build_a(bool count, const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
-                   const uint64_t* ex_x, const int32_t* hid, const int32_t* hx, uint32_t Kp, uint32_t* A,
-                   uint64_t* overflow, hipStream_t st);
-// kind 0: int8 0/1 Bt, 1: int8 multiplicity Bt, 2: bitmap per heavy key
+// A rows from the R CSR (heavy entries carry bit 31); Kp bytes per row
+void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid,
+                   const uint32_t* hx2x, uint32_t Mh, uint32_t Kp, uint32_t* A, uint64_t* overflow, hipStream_t st);
+// from the heavy S tuples: kind 0: Bt [Nh][Kp] 0/1 bytes, 1: Bt byte
+// multiplicities, 2: bits of per-key z bitmaps [K][dwords]
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
                    int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* hzpos,
                    uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow, hipStream_t st);

@@ -136,8 +136,9 @@ __global__ __launch_bounds__(256) void k_bitor(const uint32_t* __restrict__ rowp
 #pragma unroll
         for (uint32_t j = 0; j < kOrWords; j++) acc[j] = 0;
         for (uint32_t i = rb; i < re; i++) {
-            const int32_t h = hid[colR[i]];
-            if (h < 0) continue;
+            const int32_t c = colR[i];
+            if (c >= 0) continue;  // light key
+            const int32_t h = hid[(uint32_t)c & 0x7FFFFFFFu];
             const uint32_t* src = bbits + (size_t)h * dwords;
 #pragma unroll
             for (uint32_t j = 0; j < kOrWords; j++) {
@@ -212,17 +213,21 @@ __global__ __launch_bounds__(256) void k_dense_hist(const uint32_t* __restrict__
         if (lh[i]) atomicAdd(&hist[i], lh[i]);
 }
 
-// heavy index of every joined y ID (-1 = light)
+// heavy index of every joined y ID (-1 = light) and its inverse
 __global__ void k_mark_hid(const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS, uint32_t ny1,
                            const uint64_t* __restrict__ ex_y, uint32_t bmin, int32_t* __restrict__ hid,
-                           uint32_t* kcount) {
+                           uint32_t* __restrict__ h2y, uint32_t* kcount) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ny1) return;
     const uint32_t r = degR[s], q = degS[s];
     if (r == 0 || q == 0) return;
+    const uint32_t yid = (uint32_t)(ex_y[s] >> 32);
     int32_t h = -1;
-    if (dim3h::dense_bin(min(r, q)) >= bmin) h = (int32_t)atomicAdd(kcount, 1u);
-    hid[(uint32_t)(ex_y[s] >> 32)] = h;
+    if (dim3h::dense_bin(min(r, q)) >= bmin) {
+        h = (int32_t)atomicAdd(kcount, 1u);
+        h2y[h] = yid;
+    }
+    hid[yid] = h;
 }
 
 // row index hx of every x ID (-1 = no heavy key) and its inverse
@@ -265,27 +270,68 @@ __global__ void k_hz_fill(const uint32_t* __restrict__ zflag, const uint32_t* __
     }
 }
 
-// A[hx][h] for each R tuple with a heavy key (byte lanes of u32 words):
-// project sets the byte to 1, COUNT adds the multiplicity
+// One byte of a u32 word array: project stores 1, COUNT adds the
+// multiplicity (a byte reaching 128 flags the overflow; the query then runs
+// without the int8 product).
 template <bool COUNT>
-__global__ void k_build_a(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
-                          const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_x,
-                          const int32_t* __restrict__ hid, const int32_t* __restrict__ hx, uint32_t Kp, uint32_t* A,
-                          uint64_t* overflow) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nR) return;
-    const int32_t sx = slotRx[i];
-    if (sx < 0) return;
-    const int32_t h = hid[(uint32_t)(ex_y[slotR[i]] >> 32)];
-    if (h < 0) return;
-    const uint64_t byte = (uint64_t)hx[(uint32_t)(ex_x[sx] >> 32)] * Kp + (uint32_t)h;
+__device__ __forceinline__ void put_byte(uint32_t* words, uint64_t byte, uint64_t* overflow) {
     const uint32_t sh = (uint32_t)(byte & 3u) * 8u;
     if (COUNT) {
-        const uint32_t old = atomicAdd(&A[byte >> 2], 1u << sh);
+        const uint32_t old = atomicAdd(&words[byte >> 2], 1u << sh);
         if (((old >> sh) & 0xFFu) >= 127u) atomicOr((unsigned long long*)overflow, 1ull);
     } else {
-        atomicOr(&A[byte >> 2], 1u << sh);
+        atomicOr(&words[byte >> 2], 1u << sh);
     }
+}
+
+// A row hx from x's CSR row: the heavy entries (bit 31) set or count their
+// key's byte.  One workgroup per row, so the atomics stay in one row.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_build_a_rows(const uint32_t* __restrict__ rowptrR,
+                                                      const int32_t* __restrict__ colR, const int32_t* __restrict__ hid,
+                                                      const uint32_t* __restrict__ hx2x, uint32_t Kp, uint32_t* A,
+                                                      uint64_t* overflow) {
+    const uint32_t r = blockIdx.x;
+    const uint32_t x = hx2x[r];
+    const uint32_t rb = rowptrR[x], re = rowptrR[x + 1];
+    for (uint32_t i = rb + threadIdx.x; i < re; i += 256) {
+        const int32_t c = colR[i];
+        if (c >= 0) continue;
+        const int32_t h = hid[(uint32_t)c & 0x7FFFFFFFu];
+        put_byte<COUNT>(A, (uint64_t)r * Kp + (uint32_t)h, overflow);
+    }
+}
+
+// A row built in LDS (Kp <= kLdsRowBytes): byte atomics on LDS, one
+// coalesced row store.
+constexpr uint32_t kLdsRowBytes = 65536;
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_build_a_lds(const uint32_t* __restrict__ rowptrR,
+                                                     const int32_t* __restrict__ colR, const int32_t* __restrict__ hid,
+                                                     const uint32_t* __restrict__ hx2x, uint32_t Kp,
+                                                     uint32_t* __restrict__ A, uint64_t* overflow) {
+    extern __shared__ uint32_t row[];
+    const uint32_t r = blockIdx.x;
+    const uint32_t words = Kp / 4u;
+    for (uint32_t w = threadIdx.x; w < words; w += 256) row[w] = 0;
+    __syncthreads();
+    const uint32_t x = hx2x[r];
+    const uint32_t rb = rowptrR[x], re = rowptrR[x + 1];
+    for (uint32_t i = rb + threadIdx.x; i < re; i += 256) {
+        const int32_t c = colR[i];
+        if (c >= 0) continue;
+        const uint32_t h = (uint32_t)hid[(uint32_t)c & 0x7FFFFFFFu];
+        const uint32_t sh = (h & 3u) * 8u;
+        if (COUNT) {
+            const uint32_t old = atomicAdd(&row[h >> 2], 1u << sh);
+            if (((old >> sh) & 0xFFu) >= 127u) atomicOr((unsigned long long*)overflow, 1ull);
+        } else {
+            atomicOr(&row[h >> 2], 1u << sh);
+        }
+    }
+    __syncthreads();
+    uint32_t* dst = A + (uint64_t)r * words;
+    for (uint32_t w = threadIdx.x; w < words; w += 256) dst[w] = row[w];
 }
 
 __device__ __forceinline__ uint32_t z_of(const int2* S, uint64_t i, const int32_t* slotSz, const uint64_t* ex_z,
@@ -293,8 +339,9 @@ __device__ __forceinline__ uint32_t z_of(const int2* S, uint64_t i, const int32_
     return ex_z == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)(ex_z[slotSz[i]] >> 32);
 }
 
-// Bt[hz][h] for each S tuple with a heavy key (MFMA), or bit hz of z bitmap h (bit-or)
-template <bool COUNT, bool BITS>
+// Bt[hz][h] for each S tuple with a heavy key (MODE 0: 0/1, 1: multiplicity),
+// or bit hz of the z bitmap of key h (MODE 2, bit-or)
+template <int MODE>
 __global__ void k_build_b(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
                           const int32_t* __restrict__ slotSz, const uint64_t* __restrict__ ex_z, int32_t zlo,
                           uint64_t nS, const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid,
@@ -307,18 +354,8 @@ __global__ void k_build_b(const int2* __restrict__ S, const int32_t* __restrict_
     const int32_t h = hid[(uint32_t)(ex_y[sy] >> 32)];
     if (h < 0) return;
     const uint32_t hz = hzpos[z_of(S, i, slotSz, ex_z, zlo)];
-    if (BITS) {
-        atomicOr(&B[(size_t)h * dwords + (hz >> 5)], 1u << (hz & 31u));
-        return;
-    }
-    const uint64_t byte = (uint64_t)hz * Kp + (uint32_t)h;
-    const uint32_t sh = (uint32_t)(byte & 3u) * 8u;
-    if (COUNT) {
-        const uint32_t old = atomicAdd(&B[byte >> 2], 1u << sh);
-        if (((old >> sh) & 0xFFu) >= 127u) atomicOr((unsigned long long*)overflow, 1ull);
-    } else {
-        atomicOr(&B[byte >> 2], 1u << sh);
-    }
+    if (MODE == 2) atomicOr(&B[(uint64_t)h * dwords + (hz >> 5)], 1u << (hz & 31u));
+    else put_byte<MODE == 1>(B, (uint64_t)hz * Kp + (uint32_t)h, overflow);
 }
 
 inline unsigned grid1(uint64_t n) { return (unsigned)(n == 0 ? 1 : (n + 255) / 256); }
@@ -364,8 +401,9 @@ void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const 
 void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
                 const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
                 const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
-                uint32_t* hx2x, uint32_t* zflag, uint32_t* counters, hipStream_t st) {
-    hipLaunchKernelGGL(k_mark_hid, dim3(grid1(ny1)), dim3(256), 0, st, degR, degS, ny1, ex_y, bmin, hid, counters);
+                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, uint32_t* counters, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_hid, dim3(grid1(ny1)), dim3(256), 0, st, degR, degS, ny1, ex_y, bmin, hid, h2y,
+                       counters);
     hipLaunchKernelGGL(k_mark_hx, dim3(grid1(nx1)), dim3(256), 0, st, degX, ex_x, nx1, xw, bmin, hx, hx2x,
                        counters + 1);
     hipLaunchKernelGGL(k_mark_zflag, dim3(grid1(nz1)), dim3(256), 0, st, degZ, ex_z, nz1, zw, bmin, zflag);
@@ -378,16 +416,28 @@ void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, ui
                        hz_tile);
 }
 
-void dense_build_a(bool count, const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
-                   const uint64_t* ex_x, const int32_t* hid, const int32_t* hx, uint32_t Kp, uint32_t* A,
-                   uint64_t* overflow, hipStream_t st) {
-    if (!nR) return;
+void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid,
+                   const uint32_t* hx2x, uint32_t Mh, uint32_t Kp, uint32_t* A, uint64_t* overflow, hipStream_t st) {
+    if (!Mh) return;
+    if (Kp <= kLdsRowBytes) {
+        hipFuncSetAttribute((const void*)k_build_a_lds<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kLdsRowBytes);
+        hipFuncSetAttribute((const void*)k_build_a_lds<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kLdsRowBytes);
+        if (count)
+            hipLaunchKernelGGL(k_build_a_lds<true>, dim3(Mh), dim3(256), Kp, st, rowptrR, colR, hid, hx2x, Kp, A,
+                               overflow);
+        else
+            hipLaunchKernelGGL(k_build_a_lds<false>, dim3(Mh), dim3(256), Kp, st, rowptrR, colR, hid, hx2x, Kp, A,
+                               overflow);
+        return;
+    }
     if (count)
-        hipLaunchKernelGGL(k_build_a<true>, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, ex_x, hid, hx,
-                           Kp, A, overflow);
+        hipLaunchKernelGGL(k_build_a_rows<true>, dim3(Mh), dim3(256), 0, st, rowptrR, colR, hid, hx2x, Kp, A,
+                           overflow);
     else
-        hipLaunchKernelGGL(k_build_a<false>, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, ex_x, hid,
-                           hx, Kp, A, overflow);
+        hipLaunchKernelGGL(k_build_a_rows<false>, dim3(Mh), dim3(256), 0, st, rowptrR, colR, hid, hx2x, Kp, A,
+                           overflow);
 }
 
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
@@ -396,14 +446,14 @@ void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t*
     if (!nS) return;
     const unsigned g = grid1(nS);
     if (kind == 2)
-        hipLaunchKernelGGL((k_build_b<false, true>), dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y,
-                           hid, hzpos, Kp, dwords, B, overflow);
+        hipLaunchKernelGGL(k_build_b<2>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, hzpos,
+                           Kp, dwords, B, overflow);
     else if (kind == 1)
-        hipLaunchKernelGGL((k_build_b<true, false>), dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y,
-                           hid, hzpos, Kp, dwords, B, overflow);
+        hipLaunchKernelGGL(k_build_b<1>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, hzpos,
+                           Kp, dwords, B, overflow);
     else
-        hipLaunchKernelGGL((k_build_b<false, false>), dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS,
-                           ex_y, hid, hzpos, Kp, dwords, B, overflow);
+        hipLaunchKernelGGL(k_build_b<0>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, hzpos,
+                           Kp, dwords, B, overflow);
 }
 
 }  // namespace dim3h

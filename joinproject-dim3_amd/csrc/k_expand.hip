@@ -209,11 +209,20 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                     const uint32_t base = wd * 32u;
                     if (base < h0) v &= ~0u << (h0 - base);
                     if (h1 - base < 32u) v &= (1u << (h1 - base)) - 1u;
+                    // four columns per round: their z-ID loads are in flight together
                     while (v) {
-                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                        v &= v - 1u;
-                        const uint32_t zl = a.hz2zid[base + b] - zbase;
-                        atomicOr(&tile[swz_word(zl, W, C)], 1u << (zl & 31u));
+                        uint32_t zl[4];
+                        const uint32_t b0 = (uint32_t)__ffs(v) - 1u;
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++) {
+                            const uint32_t b = v ? (uint32_t)__ffs(v) - 1u : 32u;
+                            v &= v - 1u;
+                            const uint32_t z = a.hz2zid[base + (b < 32u ? b : b0)];  // clamped, unconditional
+                            zl[u] = b < 32u ? z - zbase : 0xFFFFFFFFu;
+                        }
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++)
+                            if (zl[u] != 0xFFFFFFFFu) atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
                     }
                 }
             }
@@ -226,8 +235,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             // clamped, unconditional loads: the three loads of a y stay in flight
             const uint32_t i = c0 + tid;
             const bool in_row = i < re;
-            const uint32_t key = (uint32_t)a.colR[in_row ? i : re - 1] * a.ntiles + t;
-            const uint32_t sb = a.rowptrS[key], se = a.rowptrS[key + 1];
+            const uint32_t craw = (uint32_t)a.colR[in_row ? i : re - 1];
+            const uint32_t key = (craw & 0x7FFFFFFFu) * a.ntiles + t;
+            const uint32_t sb = a.rowptrS[key], se_row = a.rowptrS[key + 1];
+            const uint32_t se = (craw >> 31) ? sb : se_row;  // heavy key: its paths are in the dense part
             uint32_t np = 0;
             if (in_row) {
                 s_sb[tid] = sb;

@@ -134,7 +134,7 @@ __global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, dim3h::KeyMap
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
     const uint32_t s = slot_insert(m, R[i].y);
-    atomicAdd(&degR[s], 1u);
+    wave_count_add(degR, s, true);
     slotR[i] = s;
 }
 
@@ -146,15 +146,14 @@ __global__ __launch_bounds__(kThreads) void k_probe_y(const int2* __restrict__ S
     __shared__ uint64_t sh[kThreads / 64 + 1];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t ok = 0;
+    int32_t s = -1;
     if (i < nS) {
-        int32_t s = slot_find(m, S[i].y);
+        s = slot_find(m, S[i].y);
         if (s >= 0 && m.range && degR[s] == 0) s = -1;
         slotS[i] = s;
-        if (s >= 0) {
-            atomicAdd(&degS[s], 1u);
-            ok = 1;
-        }
+        ok = s >= 0 ? 1 : 0;
     }
+    wave_count_add(degS, (uint32_t)s, s >= 0);
     ok = block_reduce_sum<kThreads>(ok, sh);
     if (threadIdx.x == 0 && ok) atomicAdd((unsigned long long*)surv, (unsigned long long)ok);
 }
@@ -166,11 +165,8 @@ __global__ void k_build_x(const int2* __restrict__ R, uint64_t nR, const uint32_
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
     int32_t out = -1;
-    if (degS_y[slotR[i]] > 0) {
-        const uint32_t s = slot_insert(m, R[i].x);
-        atomicAdd(&degX[s], 1u);
-        out = (int32_t)s;
-    }
+    if (degS_y[slotR[i]] > 0) out = (int32_t)slot_insert(m, R[i].x);
+    wave_count_add(degX, (uint32_t)out, out >= 0);
     slotRx[i] = out;
 }
 
@@ -180,12 +176,75 @@ __global__ void k_build_z(const int2* __restrict__ S, uint64_t nS, const int32_t
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     int32_t out = -1;
-    if (slotS[i] >= 0) {
-        const uint32_t s = slot_insert(m, S[i].x);
-        atomicAdd(&degZ[s], 1u);
-        out = (int32_t)s;
-    }
+    if (slotS[i] >= 0) out = (int32_t)slot_insert(m, S[i].x);
+    wave_count_add(degZ, (uint32_t)out, out >= 0);
     slotSz[i] = out;
+}
+
+// Degree counting through a workgroup-private LDS histogram, for range-mode
+// slot spaces of at most kLdsSlots slots (small key / attribute domains,
+// where the per-tuple atomics would otherwise pile onto a few hot lines):
+// LDS atomics per tuple, then one global add per nonzero slot and workgroup.
+// KIND 0: build_y, 1: probe_y, 2: build_x, 3: build_z (same outputs as the
+// per-tuple kernels above).
+constexpr uint32_t kLdsSlots = 32768;
+constexpr int kHistThreads = 1024;
+
+template <int KIND>
+__global__ __launch_bounds__(kHistThreads) void k_count_lds(const int2* __restrict__ V, uint64_t n, dim3h::KeyMap m,
+                                                            const uint32_t* __restrict__ aux,
+                                                            const uint32_t* __restrict__ slot_r,
+                                                            const int32_t* __restrict__ slot_s, uint32_t* deg,
+                                                            int32_t* __restrict__ slot_out, uint64_t* surv) {
+    extern __shared__ uint32_t hist[];
+    __shared__ uint64_t sh[kHistThreads / 64 + 1];
+    for (uint32_t s = threadIdx.x; s < m.nslots; s += kHistThreads) hist[s] = 0;
+    __syncthreads();
+    uint64_t ok = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kHistThreads) {
+        int32_t s;
+        if (KIND == 0) {
+            s = (int32_t)slot_insert(m, V[i].y);
+        } else if (KIND == 1) {
+            s = slot_find(m, V[i].y);
+            if (s >= 0 && aux[s] == 0) s = -1;
+        } else if (KIND == 2) {
+            s = aux[slot_r[i]] > 0 ? (int32_t)slot_insert(m, V[i].x) : -1;
+        } else {
+            s = slot_s[i] >= 0 ? (int32_t)slot_insert(m, V[i].x) : -1;
+        }
+        slot_out[i] = s;
+        if (s >= 0) {
+            atomicAdd(&hist[s], 1u);
+            ok++;
+        }
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < m.nslots; s += kHistThreads)
+        if (hist[s]) atomicAdd(&deg[s], hist[s]);
+    if (KIND == 1) {
+        ok = block_reduce_sum<kHistThreads>(ok, sh);
+        if (threadIdx.x == 0 && ok) atomicAdd((unsigned long long*)surv, (unsigned long long)ok);
+    }
+}
+
+// true (and launched) when the LDS histogram applies
+template <int KIND>
+bool count_lds(const int2* V, uint64_t n, const dim3h::KeyMap& m, const uint32_t* aux, const uint32_t* slot_r,
+               const int32_t* slot_s, uint32_t* deg, int32_t* slot_out, uint64_t* surv, hipStream_t st) {
+    if (!m.range || m.nslots > kLdsSlots || n < 8ull * m.nslots || n == 0) return false;
+    const size_t lds = (size_t)m.nslots * 4u;
+    if (hipFuncSetAttribute((const void*)k_count_lds<KIND>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    uint64_t g = (n + 32767) / 32768;
+    if (g > 256) g = 256;
+    hipLaunchKernelGGL(k_count_lds<KIND>, dim3((unsigned)g), dim3(kHistThreads), lds, st, V, n, m, aux, slot_r, slot_s,
+                       deg, slot_out, surv);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -315,16 +374,19 @@ __global__ void k_finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* to
 
 // divide_R (dim3.hpp:1058-1086) as an atomic-cursor scatter: rows are
 // contiguous, the order inside a row is not kept (a row is a set).
+// Entries of heavy keys (hid >= 0, dense part) carry bit 31.
 __global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
                             const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_x,
-                            uint32_t* cursor, int32_t* __restrict__ col) {
+                            const int32_t* __restrict__ hid, uint32_t* cursor, int32_t* __restrict__ col) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
     const int32_t sx = slotRx[i];
+    const uint32_t xid = sx >= 0 ? (uint32_t)(ex_x[sx] >> 32) : 0u;
+    const uint32_t pos = wave_reserve(cursor, xid, sx >= 0);
     if (sx < 0) return;
-    const uint32_t xid = (uint32_t)(ex_x[sx] >> 32);
-    const uint32_t pos = atomicAdd(&cursor[xid], 1u);
-    col[pos] = (int32_t)(ex_y[slotR[i]] >> 32);
+    const uint32_t yid = (uint32_t)(ex_y[slotR[i]] >> 32);
+    const uint32_t flag = (hid != nullptr && hid[yid] >= 0) ? 0x80000000u : 0u;
+    col[pos] = (int32_t)(yid | flag);
 }
 
 // z ID of a surviving S tuple: the range slot itself (z identity) or the
@@ -352,7 +414,7 @@ __global__ void k_count_s(const int2* __restrict__ S, const int32_t* __restrict_
     }
     const uint32_t zid = z_id(S, i, slotSz, ex_z, zlo);
     const uint32_t key = ntiles == 1 ? yid : yid * ntiles + zid / tile_size;
-    atomicAdd(&cnt[key], 1u);
+    wave_count_add(cnt, key, true);
     keyS[i] = key;
     zidS[i] = (int32_t)zid;
 }
@@ -362,8 +424,8 @@ __global__ void k_scatter_s(const uint32_t* __restrict__ keyS, const int32_t* __
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const uint32_t key = keyS[i];
+    const uint32_t pos = wave_reserve(cursor, key, key != 0xFFFFFFFFu);
     if (key == 0xFFFFFFFFu) return;
-    const uint32_t pos = atomicAdd(&cursor[key], 1u);
     col[pos] = zidS[i];
 }
 
@@ -405,10 +467,12 @@ void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm,
     if (nS) hipLaunchKernelGGL(k_minmax, dim3(gs), dim3(kThreads), 0, st, S, nS, mm + 4);
 }
 void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st) {
+    if (count_lds<0>(R, nR, m, nullptr, nullptr, nullptr, degR, (int32_t*)slotR, nullptr, st)) return;
     if (nR) hipLaunchKernelGGL(k_build_y, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, m, degR, slotR);
 }
 void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, uint32_t* degS, int32_t* slotS,
              uint64_t* surv, hipStream_t st) {
+    if (count_lds<1>(S, nS, m, degR, nullptr, nullptr, degS, slotS, surv, st)) return;
     if (nS)
         hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, m, degR, degS, slotS, surv);
 }
@@ -424,12 +488,14 @@ void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint
 }
 void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
              uint32_t* degX, int32_t* slotRx, hipStream_t st) {
+    if (count_lds<2>(R, nR, m, degS_y, slotR, nullptr, degX, slotRx, nullptr, st)) return;
     if (nR)
         hipLaunchKernelGGL(k_build_x, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, degS_y, m, degX,
                            slotRx);
 }
 void build_z(const int2* S, uint64_t nS, const int32_t* slotS, const KeyMap& m, uint32_t* degZ, int32_t* slotSz,
              hipStream_t st) {
+    if (count_lds<3>(S, nS, m, nullptr, nullptr, slotS, degZ, slotSz, nullptr, st)) return;
     if (nS) hipLaunchKernelGGL(k_build_z, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, m, degZ, slotSz);
 }
 void scan_deg(const uint32_t* deg, uint32_t nslots, uint64_t* ex, uint64_t* partials, uint64_t* tot,
@@ -444,10 +510,10 @@ void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_
     hipLaunchKernelGGL(k_finish_rowptr, dim3(1), dim3(64), 0, st, rowptr, n, tot);
 }
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y, const uint64_t* ex_x,
-               uint32_t* cursor, int32_t* col, hipStream_t st) {
+               const int32_t* hid, uint32_t* cursor, int32_t* col, hipStream_t st) {
     if (nR)
         hipLaunchKernelGGL(k_scatter_r, dim3(grid_for(nR)), dim3(kThreads), 0, st, slotR, slotRx, nR, ex_y, ex_x,
-                           cursor, col);
+                           hid, cursor, col);
 }
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
              const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,

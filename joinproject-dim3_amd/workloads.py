@@ -156,6 +156,19 @@ def planted_block_relation(n_background, block_keys, density, s, attr_domain, ke
     return _unpack(rng.permutation(both))
 
 
+def bernoulli_block(block_keys, density, seed, attr_offset=0, chunk_rows=2048):
+    """Every (attr, key) cell of a block_keys x block_keys grid kept with
+    probability `density` (distinct by construction), shuffled; int32."""
+    rng = _rng(seed)
+    parts = []
+    for r0 in range(0, block_keys, chunk_rows):
+        rows = min(chunk_rows, block_keys - r0)
+        xi, yi = np.nonzero(rng.random((rows, block_keys), dtype=np.float32) < density)
+        parts.append(np.stack([(xi + r0 + attr_offset).astype(np.int32), yi.astype(np.int32)], 1))
+    B = np.concatenate(parts)
+    return B[rng.permutation(B.shape[0])]
+
+
 def served_stream(n_requests=200, rate_per_s=50.0, large_frac=0.05, agg_frac=0.2, seed=4242, large_n=None):
     """Poisson arrival times and request descriptors: 80% project / 20%
     COUNT, sizes from batch_instance_params, plus large_frac large requests
