@@ -150,6 +150,28 @@ int dim3_join_aggregate_count_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_
                                   uint64_t n_s, const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap,
                                   void* stream, dim3_result* res);
 
+/* ---- batches: many independent instances in one persistent launch ----
+ * No counterpart in the reference (it runs one query per call); the batch
+ * returns, per instance, exactly what dim3_join_project_dev /
+ * dim3_join_aggregate_count_dev return for that instance alone. */
+typedef struct dim3_instance {
+    const int32_t* R_xy; /* device (x, y) pairs */
+    uint64_t n_r;
+    const int32_t* S_zy; /* device (z, y) pairs */
+    uint64_t n_s;
+    void* out;           /* device rows: 2*cap int32 pairs / cap dim3_count_row; NULL in FINGERPRINT mode */
+    uint64_t cap;
+} dim3_instance;
+
+/* results[i] is instance i's result (rows written to inst[i].out).  Returns
+ * DIM3_ERR_CAPACITY when some instance's rows exceed its cap (its
+ * results[i].n holds the rows needed).  The context keeps every instance's
+ * workspace (one bank per instance) for reuse by the next batch. */
+int dim3_batch_join_project_dev(dim3_ctx* ctx, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
+                                void* stream, dim3_result* results);
+int dim3_batch_join_aggregate_count_dev(dim3_ctx* ctx, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
+                                        void* stream, dim3_result* results);
+
 /* Exact OUT_J = sum_b degR(b)*degS(b) on the device (the quantity
  * EstimateJoinCardinality samples, dim3.hpp:190-270). */
 int dim3_count_join_paths(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,

@@ -61,6 +61,11 @@ class Result(ctypes.Structure):
                 ("stats", Stats)]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("R_xy", ctypes.c_void_p), ("n_r", ctypes.c_uint64), ("S_zy", ctypes.c_void_p),
+                ("n_s", ctypes.c_uint64), ("out", ctypes.c_void_p), ("cap", ctypes.c_uint64)]
+
+
 COUNT_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("count", "<i8")])
 
 # every symbol include/dim3.h declares (checked by the ABI test)
@@ -68,6 +73,7 @@ EXPORTS = [
     "dim3_ctx_create", "dim3_ctx_destroy", "dim3_last_error", "dim3_status_string", "dim3_ctx_device_bytes",
     "dim3_opts_default", "dim3_join_project", "dim3_join_aggregate_count", "dim3_result_free",
     "dim3_join_project_dev", "dim3_join_aggregate_count_dev", "dim3_count_join_paths", "dim3_dense_product",
+    "dim3_batch_join_project_dev", "dim3_batch_join_aggregate_count_dev",
     "dim3_version",
 ]
 
@@ -122,6 +128,10 @@ def lib():
     L.dim3_result_free.restype = None
     L.dim3_count_join_paths.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
     L.dim3_count_join_paths.restype = i32
+    for f in (L.dim3_batch_join_project_dev, L.dim3_batch_join_aggregate_count_dev):
+        f.argtypes = [vp, ctypes.POINTER(Instance), ctypes.c_uint32, ctypes.POINTER(Opts), vp,
+                      ctypes.POINTER(Result)]
+        f.restype = i32
     L.dim3_dense_product.argtypes = [vp, vp, u64, vp, u64, u64, vp, vp]
     L.dim3_dense_product.restype = i32
     L.dim3_version.argtypes = []
@@ -288,6 +298,30 @@ class Context:
         if rc not in (OK, ERR_CAPACITY):
             self._check(rc, "dim3_join_aggregate_count_dev")
         return rc, QueryResult(res)
+
+    def _batch(self, fn, name, instances, stream, kw):
+        n = len(instances)
+        arr = (Instance * max(n, 1))()
+        for i, t in enumerate(instances):
+            d_R, n_r, d_S, n_s = t[:4]
+            d_out, cap = (t[4], t[5]) if len(t) > 4 else (0, 0)
+            arr[i] = Instance(d_R, n_r, d_S, n_s, d_out or None, cap)
+        res = (Result * max(n, 1))()
+        opts = make_opts(**kw)
+        rc = fn(self._h, arr, n, ctypes.byref(opts), ctypes.c_void_p(stream or 0), res)
+        if rc not in (OK, ERR_CAPACITY):
+            self._check(rc, name)
+        return rc, [QueryResult(res[i]) for i in range(n)]
+
+    def batch_join_project_dev(self, instances, stream=None, **kw):
+        """Many instances in one persistent launch.  instances: tuples
+        (d_R, n_r, d_S, n_s[, d_out, cap]) of device addresses.  Returns
+        (status, [QueryResult per instance])."""
+        return self._batch(self._L.dim3_batch_join_project_dev, "dim3_batch_join_project_dev", instances, stream, kw)
+
+    def batch_join_aggregate_count_dev(self, instances, stream=None, **kw):
+        return self._batch(self._L.dim3_batch_join_aggregate_count_dev, "dim3_batch_join_aggregate_count_dev",
+                           instances, stream, kw)
 
     def count_join_paths(self, R, S):
         R, S = _pairs(R), _pairs(S)

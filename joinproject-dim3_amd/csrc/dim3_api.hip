@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -72,6 +73,10 @@ enum SlotId {
     WS_DB,
     WS_DBITS,
     WS_CINT,
+    WS_BATCH_ARGS,
+    WS_BATCH_START,
+    WS_BATCH_SCL,
+    WS_BATCH_SCOUT,
     WS_COUNT
 };
 
@@ -88,46 +93,65 @@ inline uint32_t pow2_at_least(uint64_t v) {
 
 }  // namespace
 
+namespace {
+
+// Plan of one prepared query (the host-side decisions and sizes).
+struct QueryPlan {
+    const int2* R = nullptr;
+    const int2* S = nullptr;
+    uint64_t nR = 0, nS = 0;
+    bool swapped = false;
+    bool empty = true;
+    bool count_mode = false;
+    uint32_t flags = 0;
+    uint32_t nx = 0, ny = 0, nz = 0;
+    uint32_t nnzR = 0, nnzS = 0;
+    uint64_t out_j = 0;
+    uint32_t ntiles = 1, tile_size = 0, tile_words = 0;
+    bool x_range = false, y_range = false, z_ident = false;
+    int32_t zlo = 0;
+    dim3h::KeyMap mx{}, my{}, mz{};
+    uint32_t touched_cap = 0, stage_cap = 0;
+    int block = 256;
+    float ms_map = 0, ms_csr = 0, ms_dense = 0;
+    // dense split
+    int dense = DIM3_DENSE_OFF;  // mode used
+    uint32_t bmin = 0;           // heaviness bin threshold
+    uint32_t K = 0, Kp = 0, Mh = 0, Nh = 0, dwords = 0;
+};
+
+// One query's device workspace and plan.  A context holds one bank per
+// query that must stay prepared at the same time (a batch holds one per
+// instance); slots grow and are reused across calls.
+struct Bank {
+    Slot ws[WS_COUNT];
+    QueryPlan plan;
+};
+
+}  // namespace
+
 struct dim3_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t held = 0;
-    Slot ws[WS_COUNT];
+    std::vector<std::unique_ptr<Bank>> banks;
+    uint32_t cur = 0;  // bank of the query being prepared / expanded
     std::string err;
     int cu_count = 0;
     size_t max_lds = 0;
     Scalars* h_sc = nullptr;  // pinned mirror of the device scalars
     unsigned long long* h_hist = nullptr;  // pinned mirror of the dense-split histogram
+    Scalars* h_batch = nullptr;            // pinned scalars of a batch
+    uint32_t h_batch_cap = 0;
     hipEvent_t ev[8] = {};
     std::recursive_mutex mu;
 
-    // plan of the last prepared query
-    struct Plan {
-        const int2* R = nullptr;
-        const int2* S = nullptr;
-        uint64_t nR = 0, nS = 0;
-        bool swapped = false;
-        bool empty = true;
-        bool count_mode = false;
-        uint32_t flags = 0;
-        uint32_t capY = 0, capX = 0, capZ = 0;
-        uint32_t nx = 0, ny = 0, nz = 0;
-        uint32_t nnzR = 0, nnzS = 0;
-        uint64_t out_j = 0;
-        uint32_t ntiles = 1, tile_size = 0, tile_words = 0;
-        bool x_range = false, y_range = false, z_ident = false;
-        int32_t zlo = 0;
-        uint32_t nz_present = 0;
-        dim3h::KeyMap mx{}, my{}, mz{};
-        uint32_t touched_cap = 0, stage_cap = 0;
-        int block = 256;
-        float ms_map = 0, ms_csr = 0;
-        // dense split
-        int dense = DIM3_DENSE_OFF;  // mode used
-        uint32_t bmin = 0;           // heaviness bin threshold
-        uint32_t K = 0, Kp = 0, Mh = 0, Nh = 0, dwords = 0;
-    } plan;
+    Bank& bank() { return *banks[cur]; }
+    void use_bank(uint32_t b) {
+        while (banks.size() <= b) banks.emplace_back(new Bank());
+        cur = b;
+    }
 };
 
 namespace {
@@ -158,7 +182,7 @@ int fail(dim3_ctx* c, int code, const char* fmt, ...) {
 int ws_get(dim3_ctx* c, SlotId id, size_t bytes, void** out) {
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~(size_t)255;
-    Slot& s = c->ws[id];
+    Slot& s = c->bank().ws[id];
     if (s.bytes >= bytes) {
         *out = s.p;
         return DIM3_OK;
@@ -204,8 +228,10 @@ int ws(dim3_ctx* c, SlotId id, uint64_t n, T** out) {
 // ---------------------------------------------------------------------------
 // planner: z-tile width and workgroup size of the expansion
 // ---------------------------------------------------------------------------
-void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
-    auto& P = c->plan;
+// force_wgs > 0: only the candidate with that many workgroups per CU (a
+// batch shares one workgroup shape and LDS size across its instances).
+void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
+    auto& P = c->bank().plan;
     const bool cnt = P.count_mode;
     const uint32_t per_word = cnt ? 1u : 32u;  // z IDs per LDS word
     P.touched_cap = cnt ? 4096u : 2048u;
@@ -245,6 +271,7 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
     const int wg_opts[3] = {4, 2, 1};
     const int blk_opts[3] = {256, 512, 1024};
     for (int i = 0; i < 3; i++) {
+        if (force_wgs > 0 && wg_opts[i] != force_wgs) continue;
         const int block = o.block ? o.block : blk_opts[i];
         Cand k = make(wg_opts[i], block, forced);
         if (!k.ok) continue;
@@ -255,6 +282,9 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o) {
         const double score = (double)k.ntiles / (double)std::min(wg_opts[i], 2);
         const double best_score = best.ok ? (double)best.ntiles / (double)std::min(best.wgs, 2) : 1e300;
         if (score < best_score) best = k;
+    }
+    if (!best.ok && force_wgs > 0) {  // forced tile too big: the batch shape with the planner's tile
+        best = make(force_wgs, o.block ? o.block : blk_opts[force_wgs == 4 ? 0 : force_wgs == 2 ? 1 : 2], 0);
     }
     if (!best.ok) {  // forced tile too big for the candidates: one workgroup per CU
         best = make(1, o.block ? o.block : 1024, 0);
@@ -315,7 +345,7 @@ double dense_cost(const DenseModel& m, const DenseShape& d, int mode, bool count
 }
 
 void plan_dense(dim3_ctx* c, const dim3_opts& o) {
-    auto& P = c->plan;
+    auto& P = c->bank().plan;
     P.dense = DIM3_DENSE_OFF;
     if (o.dense_mode == DIM3_DENSE_OFF) return;
     const unsigned long long* h = c->h_hist;
@@ -371,11 +401,11 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     P.dwords = (P.Nh + 31u) / 32u;
 }
 
-// Mapping + CSR of one query.  Leaves the plan in c->plan.
+// Mapping + CSR of one query.  Leaves the plan in c->bank().plan.
 int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
-            const dim3_opts& o, hipStream_t st) {
-    auto& P = c->plan;
-    P = dim3_ctx::Plan();
+            const dim3_opts& o, hipStream_t st, int force_wgs = 0) {
+    auto& P = c->bank().plan;
+    P = QueryPlan();
     P.count_mode = count_mode;
     P.flags = o.flags;
     if (!(o.flags & DIM3_OPT_NO_SWAP) && nR < nS) {
@@ -498,7 +528,6 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     P.nnzR = (uint32_t)c->h_sc->tot_x;
     P.nnzS = (uint32_t)c->h_sc->s_surv;
     P.nz = P.z_ident ? (uint32_t)zr : (uint32_t)(c->h_sc->tot_z >> 32);
-    P.nz_present = P.z_ident ? 0 : P.nz;
     P.out_j = c->h_sc->out_j;
     {
         float ms = 0;
@@ -510,7 +539,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         return DIM3_OK;
     }
     P.empty = false;
-    plan_tiles(c, o);
+    plan_tiles(c, o, force_wgs);
     plan_dense(c, o);
     if ((uint64_t)P.ny * P.ntiles >= (1ull << 32))
         return fail(c, DIM3_ERR_INVALID, "tiled S index does not fit 32 bits (ny=%u, tiles=%u)", P.ny, P.ntiles);
@@ -603,9 +632,47 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
 }
 
 // Expansion of the prepared plan into `out` (cap rows) or the fingerprint.
+// Expansion arguments of the current bank's prepared plan.
+dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) {
+    auto& P = c->bank().plan;
+    auto& W = c->bank().ws;
+    dim3h::ExpandArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sc = (Scalars*)W[WS_SCALARS].p;
+    if (P.empty) return a;  // nx = 0: no tasks
+    a.rowptrR = (const uint32_t*)W[WS_ROWPTR_R].p;
+    a.colR = (const int32_t*)W[WS_COL_R].p;
+    a.rowptrS = (const uint32_t*)W[WS_ROWPTR_S].p;
+    a.colS = (const int32_t*)W[WS_COL_S].p;
+    a.id2x = (const int32_t*)W[WS_ID2X].p;
+    a.id2z = P.z_ident ? nullptr : (const int32_t*)W[WS_ID2Z].p;
+    a.zlo = P.zlo;
+    a.x2z = (P.flags & DIM3_OPT_EXCLUDE_EQUAL) ? (const int32_t*)W[WS_X2Z].p : nullptr;
+    a.nx = P.nx;
+    a.ntiles = P.ntiles;
+    a.tile_size = P.tile_size;
+    a.tile_words = P.tile_words;
+    a.touched_cap = P.touched_cap;
+    a.stage_cap = P.stage_cap;
+    a.fp_mode = fp_mode ? 1u : 0u;
+    a.out = out;
+    a.cap = fp_mode ? 0 : cap;
+    if (P.dense != DIM3_DENSE_OFF) {
+        a.hx = (const int32_t*)W[WS_HX].p;
+        if (P.count_mode) a.cint = (const int32_t*)W[WS_CINT].p;
+        else a.dbits = (const uint32_t*)W[WS_DBITS].p;
+        a.hz2zid = (const uint32_t*)W[WS_HZ2ZID].p;
+        a.hz_tile = (const uint32_t*)W[WS_HZTILE].p;
+        a.dwords = P.dwords;
+        a.Nh = P.Nh;
+    }
+    return a;
+}
+
+// Expansion of the prepared plan into `out` (cap rows) or the fingerprint.
 int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
-    auto& P = c->plan;
-    Scalars* sc = (Scalars*)c->ws[WS_SCALARS].p;
+    auto& P = c->bank().plan;
+    Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
     // reset the counters the expansion owns (a re-run after a capacity miss)
     hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
     hipMemsetAsync(&sc->task, 0, sizeof(uint32_t), st);
@@ -614,41 +681,7 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
         HIPCHK(c, hipStreamSynchronize(st));
         return DIM3_OK;
     }
-    dim3h::ExpandArgs a;
-    a.rowptrR = (const uint32_t*)c->ws[WS_ROWPTR_R].p;
-    a.colR = (const int32_t*)c->ws[WS_COL_R].p;
-    a.rowptrS = (const uint32_t*)c->ws[WS_ROWPTR_S].p;
-    a.colS = (const int32_t*)c->ws[WS_COL_S].p;
-    a.id2x = (const int32_t*)c->ws[WS_ID2X].p;
-    a.id2z = P.z_ident ? nullptr : (const int32_t*)c->ws[WS_ID2Z].p;
-    a.zlo = P.zlo;
-    a.x2z = (P.flags & DIM3_OPT_EXCLUDE_EQUAL) ? (const int32_t*)c->ws[WS_X2Z].p : nullptr;
-    a.nx = P.nx;
-    a.ntiles = P.ntiles;
-    a.tile_size = P.tile_size;
-    a.tile_words = P.tile_words;
-    a.touched_cap = P.touched_cap;
-    a.stage_cap = P.stage_cap;
-    a.fp_mode = fp_mode ? 1u : 0u;
-    a.sc = sc;
-    a.out = out;
-    a.cap = fp_mode ? 0 : cap;
-    a.hx = nullptr;
-    a.dbits = nullptr;
-    a.cint = nullptr;
-    a.hz2zid = nullptr;
-    a.hz_tile = nullptr;
-    a.dwords = 0;
-    a.Nh = 0;
-    if (P.dense != DIM3_DENSE_OFF) {
-        a.hx = (const int32_t*)c->ws[WS_HX].p;
-        if (P.count_mode) a.cint = (const int32_t*)c->ws[WS_CINT].p;
-        else a.dbits = (const uint32_t*)c->ws[WS_DBITS].p;
-        a.hz2zid = (const uint32_t*)c->ws[WS_HZ2ZID].p;
-        a.hz_tile = (const uint32_t*)c->ws[WS_HZTILE].p;
-        a.dwords = P.dwords;
-        a.Nh = P.Nh;
-    }
+    const dim3h::ExpandArgs a = make_args(c, fp_mode, out, cap);
     HIPCHK(c, hipEventRecord(c->ev[4], st));
     int e = P.count_mode ? dim3h::expand_count(a, P.block, c->cu_count, st)
                          : dim3h::expand_project(a, P.block, c->cu_count, st);
@@ -660,20 +693,21 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
 }
 
 // COUNT with the int8 product: some (x, y) or (z, y) multiplicity passed 127
-bool dense_overflowed(const dim3_ctx* c) {
-    return c->plan.dense == DIM3_DENSE_MFMA && c->plan.count_mode && !c->plan.empty && c->h_sc->spare[1] != 0;
+bool dense_overflowed(dim3_ctx* c, const Scalars& s) {
+    const auto& P = c->bank().plan;
+    return P.dense == DIM3_DENSE_MFMA && P.count_mode && !P.empty && s.spare[1] != 0;
 }
 
-void fill_result(dim3_ctx* c, dim3_result* res, bool fp_mode) {
-    auto& P = c->plan;
-    const Scalars& s = *c->h_sc;
+// Result of the current bank's query from its scalars; timing = the event
+// times of a single query (a batch fills its own).
+void fill_result(dim3_ctx* c, dim3_result* res, const Scalars& s, bool timing) {
+    auto& P = c->bank().plan;
     res->n = P.empty ? 0 : s.out_n;
     res->swapped = P.swapped ? 1u : 0u;
     res->fp.count = res->n;
     res->fp.sum = P.empty ? 0 : s.fp_sum;
     res->fp.xr = P.empty ? 0 : s.fp_xr;
     res->fp.csum = P.empty ? 0 : s.fp_csum;
-    (void)fp_mode;
     dim3_stats& st = res->stats;
     memset(&st, 0, sizeof(st));
     st.n_r = P.nR;
@@ -691,7 +725,7 @@ void fill_result(dim3_ctx* c, dim3_result* res, bool fp_mode) {
     st.n_tiles = P.ntiles;
     st.block = (uint32_t)P.block;
     st.ms_map = P.ms_map;
-    if (!P.empty) {
+    if (!P.empty && timing) {
         float ms = 0;
         hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
         st.ms_csr = ms;
@@ -732,12 +766,12 @@ int run_dev(dim3_ctx* c, bool count_mode, const int32_t* d_R, uint64_t nR, const
         return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
     TRY(prepare(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st));
     TRY(expand(c, fp, d_out, cap, st));
-    if (dense_overflowed(c)) {  // a multiplicity above 127: redo without the int8 product
+    if (dense_overflowed(c, *c->h_sc)) {  // a multiplicity above 127: redo without the int8 product
         o.dense_mode = DIM3_DENSE_OFF;
         TRY(prepare(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st));
         TRY(expand(c, fp, d_out, cap, st));
     }
-    fill_result(c, res, fp);
+    fill_result(c, res, *c->h_sc, true);
     if (!fp && res->n > cap)
         return fail(c, DIM3_ERR_CAPACITY, "output needs %llu rows, buffer holds %llu", (unsigned long long)res->n,
                     (unsigned long long)cap);
@@ -767,28 +801,28 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
     const size_t row_bytes = count_mode ? sizeof(dim3_count_row) : 8;
     void* dout = nullptr;
     uint64_t cap = 0;
-    if (!fp && !c->plan.empty) {
+    if (!fp && !c->bank().plan.empty) {
         // rows <= min(OUT_J, nx*nz); size for that bound when it is modest,
         // otherwise guess and re-run the expansion at the exact size
-        uint64_t bound = std::min<uint64_t>(c->plan.out_j, (uint64_t)c->plan.nx * c->plan.nz);
+        uint64_t bound = std::min<uint64_t>(c->bank().plan.out_j, (uint64_t)c->bank().plan.nx * c->bank().plan.nz);
         const uint64_t room = c->budget > c->held ? (c->budget - c->held) / row_bytes / 2 : 0;
         cap = std::min<uint64_t>(bound, std::max<uint64_t>(room, 1u << 20));
         cap = std::max<uint64_t>(cap, 1);
         TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
-        cap = c->ws[WS_OUT].bytes / row_bytes;
+        cap = c->bank().ws[WS_OUT].bytes / row_bytes;
     }
     TRY(expand(c, fp, dout, cap, st));
-    if (dense_overflowed(c)) {  // a multiplicity above 127: redo without the int8 product
+    if (dense_overflowed(c, *c->h_sc)) {  // a multiplicity above 127: redo without the int8 product
         o.dense_mode = DIM3_DENSE_OFF;
         TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
         TRY(expand(c, fp, dout, cap, st));
     }
-    if (!fp && !c->plan.empty && c->h_sc->out_n > cap) {
+    if (!fp && !c->bank().plan.empty && c->h_sc->out_n > cap) {
         cap = c->h_sc->out_n;
         TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
         TRY(expand(c, fp, dout, cap, st));
     }
-    fill_result(c, res, fp);
+    fill_result(c, res, *c->h_sc, true);
     if (!fp && res->n) {
         void* h = malloc(res->n * row_bytes);
         if (!h) return fail(c, DIM3_ERR_OOM, "host malloc of %llu rows", (unsigned long long)res->n);
@@ -803,6 +837,126 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
             res->pairs = (int32_t*)h;
     }
     return DIM3_OK;
+}
+
+// Many instances: each is planned (mapping, CSR, dense part) into its own
+// bank, then all of them expand in one persistent launch fed by one device
+// work counter.  The launch (scalar reset, expansion, scalar gather and the
+// copy to the host) is captured as a HIP graph and replayed once.
+int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
+              void* stream, dim3_result* results) {
+    if (!c || (n && (!inst || !results))) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    if (n == 0) return DIM3_OK;
+    memset(results, 0, sizeof(dim3_result) * n);
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
+    if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
+        return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
+    for (uint32_t i = 0; i < n; i++) {
+        if ((inst[i].n_r && !inst[i].R_xy) || (inst[i].n_s && !inst[i].S_zy))
+            return fail(c, DIM3_ERR_INVALID, "instance %u: null relation pointer", i);
+        if (!fp && inst[i].cap && !inst[i].out) return fail(c, DIM3_ERR_INVALID, "instance %u: null output", i);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    struct Reset {  // leave the context on bank 0 whatever happens
+        dim3_ctx* c;
+        ~Reset() { c->use_bank(0); }
+    } reset{c};
+
+    // plan every instance into bank i + 1 with one shared workgroup shape
+    const int block = o.block ? o.block : 512;
+    dim3_opts ob = o;
+    ob.block = block;
+    const int wgs = block == 256 ? 4 : block == 512 ? 2 : 1;
+    std::vector<dim3h::ExpandArgs> args(n);
+    std::vector<uint32_t> start(n + 1, 0);
+    std::vector<Scalars*> scl(n);
+    size_t lds = 0;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        c->use_bank(i + 1);
+        TRY(prepare(c, count_mode, (const int2*)inst[i].R_xy, inst[i].n_r, (const int2*)inst[i].S_zy, inst[i].n_s,
+                    ob, st, wgs));
+        const auto& P = c->bank().plan;
+        args[i] = make_args(c, fp, inst[i].out, inst[i].cap);
+        scl[i] = args[i].sc;
+        start[i] = (uint32_t)total;
+        if (!P.empty) {
+            total += (uint64_t)P.nx * P.ntiles;
+            lds = std::max(lds, dim3h::expand_lds_bytes(P.tile_words, P.touched_cap, P.stage_cap, (uint32_t)block));
+        }
+        if (total >= (1ull << 32)) return fail(c, DIM3_ERR_INVALID, "batch holds more than 2^32 row tasks");
+    }
+    start[n] = (uint32_t)total;
+    c->use_bank(0);
+    dim3h::ExpandArgs* d_args;
+    uint32_t* d_start;
+    Scalars** d_scl;
+    Scalars* d_scout;
+    TRY(ws(c, WS_BATCH_ARGS, n, &d_args));
+    TRY(ws(c, WS_BATCH_START, (uint64_t)n + 2, &d_start));  // + the work counter
+    TRY(ws(c, WS_BATCH_SCL, n, &d_scl));
+    TRY(ws(c, WS_BATCH_SCOUT, n, &d_scout));
+    if (c->h_batch_cap < n) {
+        if (c->h_batch) hipHostFree(c->h_batch);
+        c->h_batch = nullptr;
+        c->h_batch_cap = 0;
+        HIPCHK(c, hipHostMalloc((void**)&c->h_batch, sizeof(Scalars) * n, hipHostMallocDefault));
+        c->h_batch_cap = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(d_args, args.data(), sizeof(dim3h::ExpandArgs) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(d_start, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(d_scl, scl.data(), sizeof(Scalars*) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipStreamSynchronize(st));  // host vectors are pageable
+    if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block);
+    unsigned grid = 1;
+    int e = dim3h::expand_batch_setup(count_mode, block, lds, c->cu_count, std::max<uint64_t>(total, 1), &grid);
+    if (e != 0) return fail(c, DIM3_ERR_DEVICE, "batch setup: %s", hipGetErrorString((hipError_t)e));
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    dim3h::expand_batch_launch(count_mode, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl, d_scout,
+                               st);
+    hipMemcpyAsync(c->h_batch, d_scout, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st);
+    HIPCHK(c, hipStreamEndCapture(st, &graph));
+    hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ge != hipSuccess) {
+        hipGraphDestroy(graph);
+        return fail(c, DIM3_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ge));
+    }
+    HIPCHK(c, hipEventRecord(c->ev[4], st));
+    ge = hipGraphLaunch(exec, st);
+    if (ge == hipSuccess) ge = hipEventRecord(c->ev[5], st);
+    if (ge == hipSuccess) ge = hipStreamSynchronize(st);
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    if (ge != hipSuccess) return fail(c, DIM3_ERR_DEVICE, "batch launch: %s", hipGetErrorString(ge));
+    float ms_expand = 0;
+    hipEventElapsedTime(&ms_expand, c->ev[4], c->ev[5]);
+
+    int rc = DIM3_OK;
+    for (uint32_t i = 0; i < n; i++) {
+        c->use_bank(i + 1);
+        Scalars s = c->h_batch[i];
+        if (dense_overflowed(c, s)) {  // a multiplicity above 127: this instance again, all-sparse
+            dim3_opts oi = ob;
+            oi.dense_mode = DIM3_DENSE_OFF;
+            TRY(prepare(c, count_mode, (const int2*)inst[i].R_xy, inst[i].n_r, (const int2*)inst[i].S_zy,
+                        inst[i].n_s, oi, st, wgs));
+            TRY(expand(c, fp, inst[i].out, inst[i].cap, st));
+            s = *c->h_sc;
+        }
+        fill_result(c, &results[i], s, false);
+        results[i].stats.ms_expand = ms_expand;
+        if (!fp && results[i].n > inst[i].cap) rc = DIM3_ERR_CAPACITY;
+    }
+    if (rc == DIM3_ERR_CAPACITY) fail(c, rc, "at least one instance needs more rows than its buffer holds");
+    return rc;
 }
 
 }  // namespace
@@ -843,6 +997,7 @@ int dim3_ctx_create(int device, uint64_t hbm_budget_bytes, dim3_ctx** out) {
     dim3_ctx* c = new (std::nothrow) dim3_ctx();
     if (!c) return DIM3_ERR_OOM;
     c->device = device;
+    c->use_bank(0);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
         delete c;
@@ -872,8 +1027,10 @@ int dim3_ctx_destroy(dim3_ctx* c) {
     if (!c) return DIM3_ERR_INVALID;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    for (auto& s : c->ws)
-        if (s.p) hipFree(s.p);
+    for (auto& b : c->banks)
+        for (auto& s : b->ws)
+            if (s.p) hipFree(s.p);
+    if (c->h_batch) hipHostFree(c->h_batch);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->h_sc) hipHostFree(c->h_sc);
@@ -915,6 +1072,16 @@ int dim3_join_aggregate_count_dev(dim3_ctx* c, const int32_t* d_R, uint64_t n_r,
                                   const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap, void* stream,
                                   dim3_result* res) {
     return run_dev(c, true, d_R, n_r, d_S, n_s, opts, d_rows, cap, stream, res);
+}
+
+int dim3_batch_join_project_dev(dim3_ctx* c, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
+                                void* stream, dim3_result* results) {
+    return run_batch(c, false, inst, n, opts, stream, results);
+}
+
+int dim3_batch_join_aggregate_count_dev(dim3_ctx* c, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
+                                        void* stream, dim3_result* results) {
+    return run_batch(c, true, inst, n, opts, stream, results);
 }
 
 int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,

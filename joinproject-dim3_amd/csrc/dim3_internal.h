@@ -385,4 +385,14 @@ int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st)
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block);
 
+// Batched expansion: one persistent launch over the tasks of n_inst
+// instances (args[i], tasks task_start[i] .. task_start[i + 1]).
+// setup: sets the LDS attribute and returns the grid (no stream work, so it
+// runs before a graph capture); launch: scalar reset, expansion, gather of
+// every instance's scalars into sc_out[n_inst] (kernels only).
+int expand_batch_setup(bool count, int block, size_t lds, int cu_count, uint64_t total_tasks, unsigned* grid);
+void expand_batch_launch(bool count, int block, size_t lds, unsigned grid, const ExpandArgs* args,
+                         const uint32_t* task_start, uint32_t n_inst, uint32_t* counter, Scalars* const* sc_list,
+                         Scalars* sc_out, hipStream_t st);
+
 }  // namespace dim3h

@@ -145,32 +145,382 @@ __device__ __forceinline__ uint32_t units_of(uint32_t v) {
 // `area` holds the segment table of the row during expansion and the staged
 // local z offsets during emission.
 // ZID: z IDs are value - zlo (dense domain), so no id -> value lookup.
+// One (x, z-tile) task: dense seed, expansion of the light paths into the
+// LDS tile, then emission (rows or fingerprint) and the clear of the tile.
+struct Lds {
+    uint32_t* tile;
+    uint32_t* touched;
+    uint32_t* area;
+    uint64_t* sh;
+    uint32_t* sh32;
+    uint32_t* s_wtot;
+    uint32_t* s_nt;
+    uint64_t* s_gbase;
+};
+
+template <int BLOCK, bool COUNT, bool ZID>
+__device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32_t t, const Lds& L, uint64_t& fsum,
+                                         uint64_t& fxr, uint64_t& fcs, uint64_t& npaths) {
+    constexpr uint32_t NW = BLOCK / 64;
+    uint32_t* tile = L.tile;
+    uint32_t* touched = L.touched;
+    uint32_t* s_sb = L.area;
+    uint32_t* s_len = L.area + BLOCK;
+    uint32_t* s_pre = L.area + 2 * BLOCK;
+    uint32_t* stage = L.area;
+    uint32_t* sh32 = L.sh32;
+    uint32_t* s_wtot = L.s_wtot;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t W = a.tile_words;
+    const uint32_t C = COUNT ? 0u : W / 32u;  // bit swizzle stride of project tiles
+    // rows with more paths than this sweep their tile instead of listing first touches
+    const uint64_t track_limit = min(4ull * a.touched_cap, 4ull * a.tile_words);
+    const uint32_t zbase = t * a.tile_size;
+    const uint32_t rb = a.rowptrR[x], re = a.rowptrR[x + 1];
+    const int32_t xv = a.id2x[x];
+    const int32_t zs_self = a.x2z != nullptr ? a.x2z[x] : -1;
+
+    // ---- dense seed: a row with a heavy key starts from its product row ----
+    const int32_t hxr = a.hx != nullptr ? a.hx[x] : -1;
+    if (hxr >= 0) {
+        const uint32_t h0 = a.hz_tile[t], h1 = a.hz_tile[t + 1];
+        if (COUNT) {
+            // distinct columns are distinct counters of a zeroed tile
+            const int32_t* crow = a.cint + (size_t)hxr * a.Nh;
+            for (uint32_t h = h0 + tid; h < h1; h += BLOCK) {
+                const int32_t v = crow[h];
+                if (v != 0) tile[a.hz2zid[h] - zbase] = (uint32_t)v;
+            }
+        } else if (h0 < h1) {
+            const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
+            const uint32_t wl = h0 >> 5, wh = (h1 - 1u) >> 5;
+            for (uint32_t wd = wl + tid; wd <= wh; wd += BLOCK) {
+                uint32_t v = brow[wd];
+                const uint32_t base = wd * 32u;
+                if (base < h0) v &= ~0u << (h0 - base);
+                if (h1 - base < 32u) v &= (1u << (h1 - base)) - 1u;
+                // four columns per round: their z-ID loads are in flight together
+                while (v) {
+                    uint32_t zl[4];
+                    const uint32_t b0 = (uint32_t)__ffs(v) - 1u;
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; u++) {
+                        const uint32_t b = v ? (uint32_t)__ffs(v) - 1u : 32u;
+                        v &= v - 1u;
+                        const uint32_t z = a.hz2zid[base + (b < 32u ? b : b0)];  // clamped, unconditional
+                        zl[u] = b < 32u ? z - zbase : 0xFFFFFFFFu;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; u++)
+                        if (zl[u] != 0xFFFFFFFFu) atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- expansion ----
+    bool track = true;
+    for (uint32_t c0 = rb; c0 < re; c0 += BLOCK) {
+        // clamped, unconditional loads: the three loads of a y stay in flight
+        const uint32_t i = c0 + tid;
+        const bool in_row = i < re;
+        const uint32_t craw = (uint32_t)a.colR[in_row ? i : re - 1];
+        const uint32_t key = (craw & 0x7FFFFFFFu) * a.ntiles + t;
+        const uint32_t sb = a.rowptrS[key], se_row = a.rowptrS[key + 1];
+        const uint32_t se = (craw >> 31) ? sb : se_row;  // heavy key: its paths are in the dense part
+        uint32_t np = 0;
+        if (in_row) {
+            s_sb[tid] = sb;
+            s_len[tid] = se - sb;
+            np = (se - sb + kPiece - 1) / kPiece;
+            npaths += se - sb;
+        }
+        uint32_t NP;
+        const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(np, NP, sh32);
+        s_pre[tid] = pre;
+        if (c0 == rb) track = hxr < 0 && (uint64_t)NP * kPiece <= track_limit;
+        __syncthreads();
+        const uint32_t nseg = min((uint32_t)BLOCK, re - c0);
+        // Pieces come in increasing order, so each wavefront gallops
+        // forward from the segment of its previous piece instead of
+        // searching the whole table.
+        uint32_t seg = 0;
+        for (uint32_t k0 = wid; k0 < NP; k0 += kPairs * NW) {
+            uint32_t zl[kPairs * kUnroll];
+#pragma unroll
+            for (uint32_t h = 0; h < kPairs; h++) {
+                const uint32_t k = k0 + h * NW;
+                uint32_t base = 0, len = 0;
+                if (k < NP) {
+                    uint32_t step = 1;
+                    while (seg + step < nseg && s_pre[seg + step] <= k) {
+                        seg += step;
+                        step <<= 1;
+                    }
+                    while (step > 1) {
+                        step >>= 1;
+                        if (seg + step < nseg && s_pre[seg + step] <= k) seg += step;
+                    }
+                    const uint32_t off = (k - s_pre[seg]) * kPiece;
+                    base = s_sb[seg] + off;
+                    len = min(s_len[seg] - off, kPiece);
+                }
+                // unconditional loads from a clamped, in-bounds index keep all of
+                // them in flight (a branch per load makes the compiler wait on each)
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; u++) {
+                    const uint32_t idx = lane + 64u * u;
+                    const bool ok = idx < len;
+                    const uint32_t v = (uint32_t)a.colS[ok ? base + idx : base];
+                    zl[h * kUnroll + u] = ok ? v - zbase : 0xFFFFFFFFu;
+                }
+            }
+            if (track) {
+#pragma unroll
+                for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                    bool first = false;
+                    if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u], W, C);
+                    push_first_touch(first, first ? list_item<COUNT>(zl[u], W, C) : 0u, L.s_nt, touched,
+                                     a.touched_cap);
+                }
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                    if (zl[u] != 0xFFFFFFFFu) {
+                        if (COUNT) atomicAdd(&tile[zl[u]], 1u);
+                        else atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- emission ----
+    // The list (touched words, or the whole tile) is cut into one
+    // contiguous slice per wavefront; lane l of a wavefront owns the slice
+    // words wlo + l + 64 g, so a wavefront reads 64 consecutive words per
+    // step (no bank conflicts).
+    const uint32_t nt = *L.s_nt;
+    const bool listed = track && nt <= a.touched_cap;
+    const uint32_t count = listed ? nt : a.tile_words;
+    uint32_t self_item = 0xFFFFFFFFu, self_keep = 0xFFFFFFFFu;  // u != w filter of the self-join
+    {
+        const int32_t zs = zs_self;
+        const uint32_t zl = (uint32_t)zs - zbase;
+        if (zs >= 0 && zl < a.tile_size) {
+            self_item = list_item<COUNT>(zl, W, C);
+            self_keep = COUNT ? 0u : ~(1u << (zl & 31u));
+        }
+    }
+    const uint32_t wper = (count + NW - 1) / NW;
+    const uint32_t wlo = min(wid * wper, count), whi = min(wlo + wper, count);
+    uint32_t c = 0;
+    uint64_t nzmask = 0;  // bit g: the lane's word of step g holds rows
+    {
+        uint32_t g = 0;
+        for (uint32_t i = wlo + lane; i < whi; i += 64u, g++) {
+            const uint32_t w = list_at(touched, listed, i);
+            uint32_t v = tile[w];
+            if (w == self_item) {
+                v &= self_keep;
+                tile[w] = v;
+            }
+            c += units_of<COUNT>(v);
+            if (v != 0u && g < 64u) nzmask |= 1ull << g;
+        }
+    }
+    const uint32_t wtot = wave_sum_u32(c);
+    if (lane == 0) s_wtot[wid] = wtot;
+    __syncthreads();
+    uint32_t woff = 0, T = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; w++) {
+        const uint32_t v = s_wtot[w];
+        woff += w < wid ? v : 0u;
+        T += v;
+    }
+    const uint32_t zoff = (uint32_t)a.zlo + zbase;
+    const int32_t* id2z = a.id2z + zbase;
+
+    if (a.fp_mode) {
+        for (uint32_t i = wlo + lane; i < whi; i += 64u) {
+            const uint32_t w = list_at(touched, listed, i);
+            uint32_t v = tile[w];
+            tile[w] = 0;
+            if (COUNT) {
+                if (v) {
+                    const int32_t zv = ZID ? (int32_t)(zoff + w) : id2z[w];
+                    const uint64_t h = mix64(mix64(row_key(xv, zv)) + (uint64_t)v);
+                    fsum += h;
+                    fxr ^= h;
+                    fcs += v;
+                }
+            } else {
+                while (v) {
+                    const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                    v &= v - 1u;
+                    const uint32_t e = swz_offset(w, b, W, C);
+                    const int32_t zv = ZID ? (int32_t)(zoff + e) : id2z[e];
+                    const uint64_t h = mix64(row_key(xv, zv));
+                    fsum += h;
+                    fxr ^= h;
+                }
+            }
+        }
+        if (tid == 0 && T) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
+        __syncthreads();
+        return;
+    }
+
+    if (COUNT || T <= 2u * count) {
+        // Direct: at most ~2 rows per word, so the first rows of 64
+        // neighbouring words land on neighbouring addresses and the
+        // stores coalesce without staging.
+        if (tid == 0) *L.s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
+        __syncthreads();
+        uint64_t pos0 = *L.s_gbase + woff;
+        for (uint32_t i0 = wlo; i0 < whi; i0 += 64u) {
+            const uint32_t i = i0 + lane;
+            uint32_t v = 0, w = 0;
+            if (i < whi) {
+                w = list_at(touched, listed, i);
+                v = tile[w];
+            }
+            // prefix of the per-word row counts from one ballot per count
+            // level (rows are sparse here: usually one to three levels)
+            const uint32_t cc = units_of<COUNT>(v);
+            uint32_t before = 0, gtot = 0;
+            for (uint32_t lvl = 1;; lvl++) {
+                const uint64_t m = __ballot(cc >= lvl);
+                if (m == 0) break;
+                before += mbcnt64(m);
+                gtot += (uint32_t)__popcll(m);
+                if (COUNT) break;
+            }
+            uint64_t pos = pos0 + before;
+            pos0 += gtot;
+            if (COUNT) {
+                if (v != 0u && pos < a.cap) {
+                    dim3_count_row row;
+                    row.x = xv;
+                    row.z = ZID ? (int32_t)(zoff + w) : id2z[w];
+                    row.count = (int64_t)v;
+                    reinterpret_cast<dim3_count_row*>(a.out)[pos] = row;
+                }
+            } else {
+                int2* out = reinterpret_cast<int2*>(a.out);
+                while (v) {
+                    const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                    v &= v - 1u;
+                    const uint32_t e = swz_offset(w, b, W, C);
+                    if (pos < a.cap) put_pair(out + pos, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                    ++pos;
+                }
+            }
+            if (i < whi) tile[w] = 0;
+        }
+        __syncthreads();
+        return;
+    }
+
+    // Staged (dense rows): the row's output range is reserved once and its
+    // result consumed only after the first fill.  P passes; every thread
+    // spreads its c rows evenly over them (q or q + 1 per pass), so all
+    // wavefronts fill in every pass and a pass holds at most
+    // T / P + BLOCK <= stage_cap rows.
+    uint64_t my_base = 0;
+    if (tid == 0 && T) my_base = atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
+    const uint32_t P = (T + (a.stage_cap - BLOCK) - 1) / (a.stage_cap - BLOCK);
+    const uint32_t cq = c / P, cr = c - cq * P;
+    const bool use_mask = whi - wlo <= 64u * 64u;
+    uint32_t g = 0, cw = 0, cv = 0;
+    if (!use_mask && wlo + lane < whi) {
+        cw = list_at(touched, listed, wlo + lane);
+        cv = tile[cw];
+    }
+    uint64_t pass_base = 0;
+    for (uint32_t p = 0; p < P; p++) {
+        const uint32_t mine = cq + (p < cr ? 1u : 0u);
+        uint32_t Tp;
+        const uint32_t op = block_scan2_u32<BLOCK>(mine, Tp, sh32 + (p & 1) * (NW + 1));
+        for (uint32_t j = 0; j < mine; j++) {
+            if (use_mask) {
+                // jump straight to the next word that holds rows
+                if (cv == 0u) {
+                    const uint32_t gg = (uint32_t)__ffsll((unsigned long long)nzmask) - 1u;
+                    nzmask &= nzmask - 1ull;
+                    cw = list_at(touched, listed, wlo + lane + 64u * gg);
+                    cv = tile[cw];
+                }
+            } else {
+                while (cv == 0u) {
+                    ++g;
+                    cw = list_at(touched, listed, wlo + lane + 64u * g);
+                    cv = tile[cw];
+                }
+            }
+            if (COUNT) {
+                stage[op + j] = cw;
+                cv = 0;
+            } else {
+                const uint32_t b = (uint32_t)__ffs(cv) - 1u;
+                cv &= cv - 1u;
+                stage[op + j] = swz_offset(cw, b, W, C);
+            }
+        }
+        if (tid == 0 && p == 0) *L.s_gbase = my_base;
+        __syncthreads();
+        const uint64_t g0 = *L.s_gbase + pass_base;
+        const uint32_t nq = Tp;
+        int2* out = reinterpret_cast<int2*>(a.out) + g0;
+        if (g0 + nq <= a.cap) {
+            uint32_t k = tid;
+            for (; k + 3u * BLOCK < nq; k += 4u * BLOCK) {
+                const uint32_t e0 = stage[k], e1 = stage[k + BLOCK];
+                const uint32_t e2 = stage[k + 2u * BLOCK], e3 = stage[k + 3u * BLOCK];
+                const int32_t z0 = ZID ? (int32_t)(zoff + e0) : id2z[e0];
+                const int32_t z1 = ZID ? (int32_t)(zoff + e1) : id2z[e1];
+                const int32_t z2 = ZID ? (int32_t)(zoff + e2) : id2z[e2];
+                const int32_t z3 = ZID ? (int32_t)(zoff + e3) : id2z[e3];
+                put_pair(out + k, xv, z0);
+                put_pair(out + k + BLOCK, xv, z1);
+                put_pair(out + k + 2u * BLOCK, xv, z2);
+                put_pair(out + k + 3u * BLOCK, xv, z3);
+            }
+            for (; k < nq; k += BLOCK) {
+                const uint32_t e = stage[k];
+                put_pair(out + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+            }
+        } else {
+            for (uint32_t k = tid; k < nq; k += BLOCK) {
+                const uint32_t e = stage[k];
+                if (g0 + k < a.cap) put_pair(out + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+            }
+        }
+        pass_base += nq;
+        __syncthreads();
+    }
+    for (uint32_t i = wlo + lane; i < whi; i += 64u) tile[list_at(touched, listed, i)] = 0;
+    __syncthreads();
+}
+
 template <int BLOCK, bool COUNT, bool ZID>
 __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr uint32_t NW = BLOCK / 64;
-    uint32_t* tile = smem;
-    uint32_t* touched = smem + a.tile_words;
-    uint32_t* area = touched + a.touched_cap;
-    uint32_t* s_sb = area;
-    uint32_t* s_len = area + BLOCK;
-    uint32_t* s_pre = area + 2 * BLOCK;
-    uint32_t* stage = area;
     __shared__ uint64_t sh[NW + 1];
     __shared__ uint32_t sh32[2 * (NW + 1)];
     __shared__ uint32_t s_wtot[NW];
     __shared__ uint32_t s_task, s_nt;
     __shared__ uint64_t s_gbase;
+    const Lds L{smem, smem + a.tile_words, smem + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
 
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t tid = threadIdx.x;
     const uint32_t ntasks = a.nx * a.ntiles;
-    const uint32_t W = a.tile_words;
-    const uint32_t C = COUNT ? 0u : W / 32u;  // bit swizzle stride of project tiles
-    // rows with more paths than this sweep their tile instead of listing first touches
-    const uint64_t track_limit = min(4ull * a.touched_cap, 4ull * a.tile_words);
-
-    for (uint32_t w = tid; w < a.tile_words; w += BLOCK) tile[w] = 0;
+    for (uint32_t w = tid; w < a.tile_words; w += BLOCK) L.tile[w] = 0;
     uint64_t fsum = 0, fxr = 0, fcs = 0, npaths = 0;
 
     for (;;) {
@@ -179,341 +529,11 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             s_nt = 0;
         }
         __syncthreads();
-        // task, x and the per-row values are wavefront-uniform: scalar loads,
-        // issued before the expansion so their latency is hidden
+        // task and x are wavefront-uniform: scalar loads
         const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
         if (task >= ntasks) break;
         const uint32_t x = task / a.ntiles;
-        const uint32_t t = task - x * a.ntiles;
-        const uint32_t zbase = t * a.tile_size;
-        const uint32_t rb = a.rowptrR[x], re = a.rowptrR[x + 1];
-        const int32_t xv = a.id2x[x];
-        const int32_t zs_self = a.x2z != nullptr ? a.x2z[x] : -1;
-
-        // ---- dense seed: a row with a heavy key starts from its product row ----
-        const int32_t hxr = a.hx != nullptr ? a.hx[x] : -1;
-        if (hxr >= 0) {
-            const uint32_t h0 = a.hz_tile[t], h1 = a.hz_tile[t + 1];
-            if (COUNT) {
-                // distinct columns are distinct counters of a zeroed tile
-                const int32_t* crow = a.cint + (size_t)hxr * a.Nh;
-                for (uint32_t h = h0 + tid; h < h1; h += BLOCK) {
-                    const int32_t v = crow[h];
-                    if (v != 0) tile[a.hz2zid[h] - zbase] = (uint32_t)v;
-                }
-            } else if (h0 < h1) {
-                const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
-                const uint32_t wl = h0 >> 5, wh = (h1 - 1u) >> 5;
-                for (uint32_t wd = wl + tid; wd <= wh; wd += BLOCK) {
-                    uint32_t v = brow[wd];
-                    const uint32_t base = wd * 32u;
-                    if (base < h0) v &= ~0u << (h0 - base);
-                    if (h1 - base < 32u) v &= (1u << (h1 - base)) - 1u;
-                    // four columns per round: their z-ID loads are in flight together
-                    while (v) {
-                        uint32_t zl[4];
-                        const uint32_t b0 = (uint32_t)__ffs(v) - 1u;
-#pragma unroll
-                        for (uint32_t u = 0; u < 4; u++) {
-                            const uint32_t b = v ? (uint32_t)__ffs(v) - 1u : 32u;
-                            v &= v - 1u;
-                            const uint32_t z = a.hz2zid[base + (b < 32u ? b : b0)];  // clamped, unconditional
-                            zl[u] = b < 32u ? z - zbase : 0xFFFFFFFFu;
-                        }
-#pragma unroll
-                        for (uint32_t u = 0; u < 4; u++)
-                            if (zl[u] != 0xFFFFFFFFu) atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
-                    }
-                }
-            }
-            __syncthreads();
-        }
-
-        // ---- expansion ----
-        bool track = true;
-        for (uint32_t c0 = rb; c0 < re; c0 += BLOCK) {
-            // clamped, unconditional loads: the three loads of a y stay in flight
-            const uint32_t i = c0 + tid;
-            const bool in_row = i < re;
-            const uint32_t craw = (uint32_t)a.colR[in_row ? i : re - 1];
-            const uint32_t key = (craw & 0x7FFFFFFFu) * a.ntiles + t;
-            const uint32_t sb = a.rowptrS[key], se_row = a.rowptrS[key + 1];
-            const uint32_t se = (craw >> 31) ? sb : se_row;  // heavy key: its paths are in the dense part
-            uint32_t np = 0;
-            if (in_row) {
-                s_sb[tid] = sb;
-                s_len[tid] = se - sb;
-                np = (se - sb + kPiece - 1) / kPiece;
-                npaths += se - sb;
-            }
-            uint32_t NP;
-            const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(np, NP, sh32);
-            s_pre[tid] = pre;
-            if (c0 == rb) track = hxr < 0 && (uint64_t)NP * kPiece <= track_limit;
-            __syncthreads();
-            const uint32_t nseg = min((uint32_t)BLOCK, re - c0);
-            // Pieces come in increasing order, so each wavefront gallops
-            // forward from the segment of its previous piece instead of
-            // searching the whole table.
-            uint32_t seg = 0;
-            for (uint32_t k0 = wid; k0 < NP; k0 += kPairs * NW) {
-                uint32_t zl[kPairs * kUnroll];
-#pragma unroll
-                for (uint32_t h = 0; h < kPairs; h++) {
-                    const uint32_t k = k0 + h * NW;
-                    uint32_t base = 0, len = 0;
-                    if (k < NP) {
-                        uint32_t step = 1;
-                        while (seg + step < nseg && s_pre[seg + step] <= k) {
-                            seg += step;
-                            step <<= 1;
-                        }
-                        while (step > 1) {
-                            step >>= 1;
-                            if (seg + step < nseg && s_pre[seg + step] <= k) seg += step;
-                        }
-                        const uint32_t off = (k - s_pre[seg]) * kPiece;
-                        base = s_sb[seg] + off;
-                        len = min(s_len[seg] - off, kPiece);
-                    }
-                    // unconditional loads from a clamped, in-bounds index keep all of
-                    // them in flight (a branch per load makes the compiler wait on each)
-#pragma unroll
-                    for (uint32_t u = 0; u < kUnroll; u++) {
-                        const uint32_t idx = lane + 64u * u;
-                        const bool ok = idx < len;
-                        const uint32_t v = (uint32_t)a.colS[ok ? base + idx : base];
-                        zl[h * kUnroll + u] = ok ? v - zbase : 0xFFFFFFFFu;
-                    }
-                }
-                if (track) {
-#pragma unroll
-                    for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                        bool first = false;
-                        if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u], W, C);
-                        push_first_touch(first, first ? list_item<COUNT>(zl[u], W, C) : 0u, &s_nt, touched,
-                                         a.touched_cap);
-                    }
-                } else {
-#pragma unroll
-                    for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                        if (zl[u] != 0xFFFFFFFFu) {
-                            if (COUNT) atomicAdd(&tile[zl[u]], 1u);
-                            else atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-        }
-
-        // ---- emission ----
-        // The list (touched words, or the whole tile) is cut into one
-        // contiguous slice per wavefront; lane l of a wavefront owns the slice
-        // words wlo + l + 64 g, so a wavefront reads 64 consecutive words per
-        // step (no bank conflicts).
-        const uint32_t nt = s_nt;
-        const bool listed = track && nt <= a.touched_cap;
-        const uint32_t count = listed ? nt : a.tile_words;
-        uint32_t self_item = 0xFFFFFFFFu, self_keep = 0xFFFFFFFFu;  // u != w filter of the self-join
-        {
-            const int32_t zs = zs_self;
-            const uint32_t zl = (uint32_t)zs - zbase;
-            if (zs >= 0 && zl < a.tile_size) {
-                self_item = list_item<COUNT>(zl, W, C);
-                self_keep = COUNT ? 0u : ~(1u << (zl & 31u));
-            }
-        }
-        const uint32_t wper = (count + NW - 1) / NW;
-        const uint32_t wlo = min(wid * wper, count), whi = min(wlo + wper, count);
-        uint32_t c = 0;
-        uint64_t nzmask = 0;  // bit g: the lane's word of step g holds rows
-        {
-            uint32_t g = 0;
-            for (uint32_t i = wlo + lane; i < whi; i += 64u, g++) {
-                const uint32_t w = list_at(touched, listed, i);
-                uint32_t v = tile[w];
-                if (w == self_item) {
-                    v &= self_keep;
-                    tile[w] = v;
-                }
-                c += units_of<COUNT>(v);
-                if (v != 0u && g < 64u) nzmask |= 1ull << g;
-            }
-        }
-        const uint32_t wtot = wave_sum_u32(c);
-        if (lane == 0) s_wtot[wid] = wtot;
-        __syncthreads();
-        uint32_t woff = 0, T = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < NW; w++) {
-            const uint32_t v = s_wtot[w];
-            woff += w < wid ? v : 0u;
-            T += v;
-        }
-        const uint32_t zoff = (uint32_t)a.zlo + zbase;
-        const int32_t* id2z = a.id2z + zbase;
-
-        if (a.fp_mode) {
-            for (uint32_t i = wlo + lane; i < whi; i += 64u) {
-                const uint32_t w = list_at(touched, listed, i);
-                uint32_t v = tile[w];
-                tile[w] = 0;
-                if (COUNT) {
-                    if (v) {
-                        const int32_t zv = ZID ? (int32_t)(zoff + w) : id2z[w];
-                        const uint64_t h = mix64(mix64(row_key(xv, zv)) + (uint64_t)v);
-                        fsum += h;
-                        fxr ^= h;
-                        fcs += v;
-                    }
-                } else {
-                    while (v) {
-                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                        v &= v - 1u;
-                        const uint32_t e = swz_offset(w, b, W, C);
-                        const int32_t zv = ZID ? (int32_t)(zoff + e) : id2z[e];
-                        const uint64_t h = mix64(row_key(xv, zv));
-                        fsum += h;
-                        fxr ^= h;
-                    }
-                }
-            }
-            if (tid == 0 && T) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
-            __syncthreads();
-            continue;
-        }
-
-        if (COUNT || T <= 2u * count) {
-            // Direct: at most ~2 rows per word, so the first rows of 64
-            // neighbouring words land on neighbouring addresses and the
-            // stores coalesce without staging.
-            if (tid == 0) s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
-            __syncthreads();
-            uint64_t pos0 = s_gbase + woff;
-            for (uint32_t i0 = wlo; i0 < whi; i0 += 64u) {
-                const uint32_t i = i0 + lane;
-                uint32_t v = 0, w = 0;
-                if (i < whi) {
-                    w = list_at(touched, listed, i);
-                    v = tile[w];
-                }
-                // prefix of the per-word row counts from one ballot per count
-                // level (rows are sparse here: usually one to three levels)
-                const uint32_t cc = units_of<COUNT>(v);
-                uint32_t before = 0, gtot = 0;
-                for (uint32_t lvl = 1;; lvl++) {
-                    const uint64_t m = __ballot(cc >= lvl);
-                    if (m == 0) break;
-                    before += mbcnt64(m);
-                    gtot += (uint32_t)__popcll(m);
-                    if (COUNT) break;
-                }
-                uint64_t pos = pos0 + before;
-                pos0 += gtot;
-                if (COUNT) {
-                    if (v != 0u && pos < a.cap) {
-                        dim3_count_row row;
-                        row.x = xv;
-                        row.z = ZID ? (int32_t)(zoff + w) : id2z[w];
-                        row.count = (int64_t)v;
-                        reinterpret_cast<dim3_count_row*>(a.out)[pos] = row;
-                    }
-                } else {
-                    int2* out = reinterpret_cast<int2*>(a.out);
-                    while (v) {
-                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                        v &= v - 1u;
-                        const uint32_t e = swz_offset(w, b, W, C);
-                        if (pos < a.cap) put_pair(out + pos, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
-                        ++pos;
-                    }
-                }
-                if (i < whi) tile[w] = 0;
-            }
-            __syncthreads();
-            continue;
-        }
-
-        // Staged (dense rows): the row's output range is reserved once and its
-        // result consumed only after the first fill.  P passes; every thread
-        // spreads its c rows evenly over them (q or q + 1 per pass), so all
-        // wavefronts fill in every pass and a pass holds at most
-        // T / P + BLOCK <= stage_cap rows.
-        uint64_t my_base = 0;
-        if (tid == 0 && T) my_base = atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
-        const uint32_t P = (T + (a.stage_cap - BLOCK) - 1) / (a.stage_cap - BLOCK);
-        const uint32_t cq = c / P, cr = c - cq * P;
-        const bool use_mask = whi - wlo <= 64u * 64u;
-        uint32_t g = 0, cw = 0, cv = 0;
-        if (!use_mask && wlo + lane < whi) {
-            cw = list_at(touched, listed, wlo + lane);
-            cv = tile[cw];
-        }
-        uint64_t pass_base = 0;
-        for (uint32_t p = 0; p < P; p++) {
-            const uint32_t mine = cq + (p < cr ? 1u : 0u);
-            uint32_t Tp;
-            const uint32_t op = block_scan2_u32<BLOCK>(mine, Tp, sh32 + (p & 1) * (NW + 1));
-            for (uint32_t j = 0; j < mine; j++) {
-                if (use_mask) {
-                    // jump straight to the next word that holds rows
-                    if (cv == 0u) {
-                        const uint32_t gg = (uint32_t)__ffsll((unsigned long long)nzmask) - 1u;
-                        nzmask &= nzmask - 1ull;
-                        cw = list_at(touched, listed, wlo + lane + 64u * gg);
-                        cv = tile[cw];
-                    }
-                } else {
-                    while (cv == 0u) {
-                        ++g;
-                        cw = list_at(touched, listed, wlo + lane + 64u * g);
-                        cv = tile[cw];
-                    }
-                }
-                if (COUNT) {
-                    stage[op + j] = cw;
-                    cv = 0;
-                } else {
-                    const uint32_t b = (uint32_t)__ffs(cv) - 1u;
-                    cv &= cv - 1u;
-                    stage[op + j] = swz_offset(cw, b, W, C);
-                }
-            }
-            if (tid == 0 && p == 0) s_gbase = my_base;
-            __syncthreads();
-            const uint64_t g0 = s_gbase + pass_base;
-            const uint32_t nq = Tp;
-            int2* out = reinterpret_cast<int2*>(a.out) + g0;
-            if (g0 + nq <= a.cap) {
-                uint32_t k = tid;
-                for (; k + 3u * BLOCK < nq; k += 4u * BLOCK) {
-                    const uint32_t e0 = stage[k], e1 = stage[k + BLOCK];
-                    const uint32_t e2 = stage[k + 2u * BLOCK], e3 = stage[k + 3u * BLOCK];
-                    const int32_t z0 = ZID ? (int32_t)(zoff + e0) : id2z[e0];
-                    const int32_t z1 = ZID ? (int32_t)(zoff + e1) : id2z[e1];
-                    const int32_t z2 = ZID ? (int32_t)(zoff + e2) : id2z[e2];
-                    const int32_t z3 = ZID ? (int32_t)(zoff + e3) : id2z[e3];
-                    put_pair(out + k, xv, z0);
-                    put_pair(out + k + BLOCK, xv, z1);
-                    put_pair(out + k + 2u * BLOCK, xv, z2);
-                    put_pair(out + k + 3u * BLOCK, xv, z3);
-                }
-                for (; k < nq; k += BLOCK) {
-                    const uint32_t e = stage[k];
-                    put_pair(out + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
-                }
-            } else {
-                for (uint32_t k = tid; k < nq; k += BLOCK) {
-                    const uint32_t e = stage[k];
-                    if (g0 + k < a.cap) put_pair(out + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
-                }
-            }
-            pass_base += nq;
-            __syncthreads();
-        }
-        for (uint32_t i = wlo + lane; i < whi; i += 64u) tile[list_at(touched, listed, i)] = 0;
-        __syncthreads();
+        row_task<BLOCK, COUNT, ZID>(a, x, task - x * a.ntiles, L, fsum, fxr, fcs, npaths);
     }
 
     npaths = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
@@ -528,6 +548,103 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
             atomicXor((unsigned long long*)&a.sc->fp_xr, (unsigned long long)fxr);
         }
     }
+}
+
+// Sums of the per-lane row statistics into the instance's scalars (block-uniform call).
+template <int BLOCK, bool COUNT>
+__device__ __forceinline__ void flush_stats(const ExpandArgs& a, uint64_t& fsum, uint64_t& fxr, uint64_t& fcs,
+                                            uint64_t& npaths, uint64_t* sh) {
+    const uint64_t np = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
+    if (threadIdx.x == 0 && np) atomicAdd((unsigned long long*)&a.sc->paths, (unsigned long long)np);
+    if (a.fp_mode) {
+        const uint64_t s1 = block_reduce_sum<BLOCK, uint64_t>(fsum, sh);
+        const uint64_t s2 = block_reduce_sum<BLOCK, uint64_t>(fcs, sh);
+        const uint64_t s3 = block_reduce_xor64<BLOCK>(fxr, sh);
+        if (threadIdx.x == 0) {
+            atomicAdd((unsigned long long*)&a.sc->fp_sum, (unsigned long long)s1);
+            if (COUNT) atomicAdd((unsigned long long*)&a.sc->fp_csum, (unsigned long long)s2);
+            atomicXor((unsigned long long*)&a.sc->fp_xr, (unsigned long long)s3);
+        }
+    }
+    fsum = fxr = fcs = npaths = 0;
+}
+
+// Many join instances in one persistent launch fed by one device work
+// counter: global task g belongs to the instance i with
+// task_start[i] <= g < task_start[i + 1] (tasks of instance i are its
+// (x, z-tile) pairs).  A workgroup keeps one instance's arguments while its
+// consecutive tasks stay in that instance, and flushes the instance's row
+// statistics when it moves on.  LDS is sized for the largest tile of the batch.
+template <int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __restrict__ args,
+                                                        const uint32_t* __restrict__ task_start, uint32_t n_inst,
+                                                        uint32_t* counter) {
+    extern __shared__ __align__(16) uint32_t smem[];
+    constexpr uint32_t NW = BLOCK / 64;
+    __shared__ uint64_t sh[NW + 1];
+    __shared__ uint32_t sh32[2 * (NW + 1)];
+    __shared__ uint32_t s_wtot[NW];
+    __shared__ uint32_t s_task, s_nt;
+    __shared__ uint64_t s_gbase;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t total = task_start[n_inst];
+    uint64_t fsum = 0, fxr = 0, fcs = 0, npaths = 0;
+    int32_t cur = -1;
+    ExpandArgs a;
+    for (;;) {
+        if (tid == 0) {
+            s_task = atomicAdd(counter, 1u);
+            s_nt = 0;
+        }
+        __syncthreads();
+        const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
+        if (task >= total) break;
+        // largest i with task_start[i] <= task (task_start[n_inst] = total > task)
+        uint32_t lo = 0, hi = n_inst;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (task_start[mid] <= task) lo = mid;
+            else hi = mid;
+        }
+        if ((int32_t)lo != cur) {
+            if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fxr, fcs, npaths, sh);
+            cur = (int32_t)lo;
+            a = args[lo];
+            // the previous instance's touched list / staging area may overlap this tile
+            for (uint32_t w = tid; w < a.tile_words; w += BLOCK) smem[w] = 0;
+            __syncthreads();
+        }
+        const uint32_t local = task - task_start[lo];
+        const uint32_t x = local / a.ntiles;
+        const Lds L{smem, smem + a.tile_words, smem + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
+        if (a.id2z == nullptr) row_task<BLOCK, COUNT, true>(a, x, local - x * a.ntiles, L, fsum, fxr, fcs, npaths);
+        else row_task<BLOCK, COUNT, false>(a, x, local - x * a.ntiles, L, fsum, fxr, fcs, npaths);
+    }
+    if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fxr, fcs, npaths, sh);
+}
+
+// Zero the expansion-owned scalars of every instance of a batch (a kernel, so
+// a captured graph holds no memset node).
+__global__ void k_reset_scalars(dim3h::Scalars* const* __restrict__ list, uint32_t n, uint32_t* counter) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *counter = 0;
+    if (i >= n) return;
+    dim3h::Scalars* s = list[i];
+    s->out_n = 0;
+    s->fp_sum = 0;
+    s->fp_xr = 0;
+    s->fp_csum = 0;
+    s->paths = 0;
+    s->task = 0;
+}
+
+__global__ void k_gather_scalars(dim3h::Scalars* const* __restrict__ list, uint32_t n, dim3h::Scalars* out) {
+    const uint32_t i = blockIdx.x;
+    if (i >= n) return;
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(list[i]);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(out + i);
+    for (uint32_t w = threadIdx.x; w < sizeof(dim3h::Scalars) / 8; w += blockDim.x) dst[w] = src[w];
 }
 
 template <typename K>
@@ -578,6 +695,53 @@ int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st)
 
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
     return launch<true>(a, block, cu_count, st);
+}
+
+template <int BLOCK, bool COUNT>
+int batch_prepare_one(size_t lds, int cu_count, uint64_t total_tasks, unsigned* grid) {
+    const void* kern = (const void*)k_expand_batch<BLOCK, COUNT>;
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return (int)e;
+    int occ = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_expand_batch<BLOCK, COUNT>, BLOCK, lds);
+    if (e != hipSuccess) return (int)e;
+    if (occ < 1) occ = 1;
+    uint64_t g = (uint64_t)cu_count * (uint64_t)occ;
+    if (g > total_tasks) g = total_tasks;
+    *grid = (unsigned)(g ? g : 1);
+    return 0;
+}
+
+int expand_batch_setup(bool count, int block, size_t lds, int cu_count, uint64_t total_tasks, unsigned* grid) {
+    switch (block) {
+        case 256: return count ? batch_prepare_one<256, true>(lds, cu_count, total_tasks, grid)
+                               : batch_prepare_one<256, false>(lds, cu_count, total_tasks, grid);
+        case 512: return count ? batch_prepare_one<512, true>(lds, cu_count, total_tasks, grid)
+                               : batch_prepare_one<512, false>(lds, cu_count, total_tasks, grid);
+        case 1024: return count ? batch_prepare_one<1024, true>(lds, cu_count, total_tasks, grid)
+                                : batch_prepare_one<1024, false>(lds, cu_count, total_tasks, grid);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
+void expand_batch_launch(bool count, int block, size_t lds, unsigned grid, const ExpandArgs* args,
+                         const uint32_t* task_start, uint32_t n_inst, uint32_t* counter, Scalars* const* sc_list,
+                         Scalars* sc_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_scalars, dim3((n_inst + 255) / 256), dim3(256), 0, st, sc_list, n_inst, counter);
+#define DIM3_BATCH_LAUNCH(B, CNT) \
+    hipLaunchKernelGGL((k_expand_batch<B, CNT>), dim3(grid), dim3(B), lds, st, args, task_start, n_inst, counter)
+    if (block == 256) {
+        if (count) DIM3_BATCH_LAUNCH(256, true);
+        else DIM3_BATCH_LAUNCH(256, false);
+    } else if (block == 512) {
+        if (count) DIM3_BATCH_LAUNCH(512, true);
+        else DIM3_BATCH_LAUNCH(512, false);
+    } else {
+        if (count) DIM3_BATCH_LAUNCH(1024, true);
+        else DIM3_BATCH_LAUNCH(1024, false);
+    }
+#undef DIM3_BATCH_LAUNCH
+    hipLaunchKernelGGL(k_gather_scalars, dim3(n_inst), dim3(64), 0, st, sc_list, n_inst, sc_out);
 }
 
 }  // namespace dim3h

@@ -142,6 +142,39 @@ def batch_instance(p):
     return R, S
 
 
+def zipf_relation_device(torch, n, attr_domain, key_domain, s, seed, max_rounds=400):
+    """zipf_relation drawn on the GPU (torch's Philox generator, seeded): the
+    same law (uniform attribute, Zipf(s) key, duplicates redrawn), other
+    random streams.  Returns an (n, 2) int32 device tensor."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    w = torch.arange(1, key_domain + 1, dtype=torch.float64, device="cuda") ** (-float(s))
+    cdf = torch.cumsum(w, 0)
+    cdf /= cdf[-1].clone()
+    have = torch.empty(0, dtype=torch.int64, device="cuda")
+    need = n
+    for _ in range(max_rounds):
+        if need <= 0:
+            break
+        m = int(need * 1.1) + 1024
+        a = torch.randint(0, attr_domain, (m,), device="cuda", generator=g, dtype=torch.int64)
+        u = torch.rand(m, device="cuda", generator=g, dtype=torch.float64)
+        b = torch.clamp(torch.searchsorted(cdf, u, right=True), max=key_domain - 1)
+        have = torch.unique(torch.cat([have, (a << 32) | b]))
+        need = n - have.numel()
+    if have.numel() > n:
+        have = have[torch.randperm(have.numel(), generator=g, device="cuda")[:n]]
+    have = have[torch.randperm(have.numel(), generator=g, device="cuda")]
+    return torch.stack([(have >> 32).to(torch.int32), (have & 0xFFFFFFFF).to(torch.int32)], 1).contiguous()
+
+
+def batch_instance_device(torch, p):
+    """batch_instance drawn on the GPU: (R, S) int32 device tensors."""
+    R = zipf_relation_device(torch, p["n"], p["ac_domain"], p["b_domain"], p["s"], p["seed_r"])
+    S = zipf_relation_device(torch, p["n"], p["ac_domain"], p["b_domain"], p["s"], p["seed_s"])
+    return R, S
+
+
 def planted_block_relation(n_background, block_keys, density, s, attr_domain, key_domain, seed):
     """Zipf(s) background plus a planted dense block: the first block_keys
     attribute values x the first block_keys join keys, each cell present with
