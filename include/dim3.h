@@ -68,13 +68,19 @@ typedef struct dim3_opts {
     uint32_t reserved[4];
 } dim3_opts;
 
-/* Order-independent digest of a row set.  key = (u64)(u32)x << 32 | (u32)z,
- * h = splitmix64-finaliser(key); for COUNT rows h = finaliser(h + count).
- * count = rows, sum = sum of h mod 2^64, xr = xor of h, csum = sum of counts. */
+/* Order-independent digest of a row set, with u(v) = (u32)v, mix64 the
+ * splitmix64 finaliser and fmix32 the MurmurHash3 32-bit finaliser:
+ *   k1(x) = mix64(u(x) ^ 0x243F6A8885A308D3) | 1, k2(x) = mix64(u(x) ^ 0x13198A2E03707344) | 1,
+ *   g1(z) = fmix32(u(z) ^ 0x243F6A88), g2(z) = (g1(z) * 0x2545F491 mod 2^32) ^ (g1(z) >> 16);
+ *   per project row (x, z):   count += 1, sum += k1(x) g1(z), sum2 += k2(x) g2(z)   (mod 2^64);
+ *   per COUNT row (x, z, c):  count += 1, sum += k1(x) g1(z), sum2 += k2(x) c g2(z), csum += c.
+ * Every term factors into an x part and a z part, so the device sums the z
+ * parts of a row (or, for a mostly full word, subtracts its missing ones
+ * from the word's total) and multiplies once per row. */
 typedef struct dim3_fingerprint {
     uint64_t count;
     uint64_t sum;
-    uint64_t xr;
+    uint64_t sum2;
     uint64_t csum;
 } dim3_fingerprint;
 

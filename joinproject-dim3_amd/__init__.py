@@ -31,11 +31,11 @@ class Opts(ctypes.Structure):
 
 
 class Fingerprint(ctypes.Structure):
-    _fields_ = [("count", ctypes.c_uint64), ("sum", ctypes.c_uint64), ("xr", ctypes.c_uint64),
+    _fields_ = [("count", ctypes.c_uint64), ("sum", ctypes.c_uint64), ("sum2", ctypes.c_uint64),
                 ("csum", ctypes.c_uint64)]
 
     def as_tuple(self):
-        return (int(self.count), int(self.sum), int(self.xr), int(self.csum))
+        return (int(self.count), int(self.sum), int(self.sum2), int(self.csum))
 
 
 class Stats(ctypes.Structure):

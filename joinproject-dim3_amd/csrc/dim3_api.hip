@@ -73,6 +73,8 @@ enum SlotId {
     WS_DB,
     WS_DBITS,
     WS_CINT,
+    WS_FP_GW,
+    WS_FP_GWM,
     WS_BATCH_ARGS,
     WS_BATCH_START,
     WS_BATCH_SCL,
@@ -118,6 +120,7 @@ struct QueryPlan {
     int dense = DIM3_DENSE_OFF;  // mode used
     uint32_t bmin = 0;           // heaviness bin threshold
     uint32_t K = 0, Kp = 0, Mh = 0, Nh = 0, dwords = 0;
+    bool all_z = false;  // every z ID is a product column
 };
 
 // One query's device workspace and plan.  A context holds one bank per
@@ -315,6 +318,7 @@ struct DenseModel {
 
 struct DenseShape {
     uint64_t K = 0, Mh = 0, Nh = 0, paths = 0, sumR = 0, sumS = 0;
+    uint64_t nz = 0;  // all z IDs: product columns covering nearly all of them seed whole words
 };
 
 // seconds the dense part takes for one shape and mode, <0 if it cannot run
@@ -338,8 +342,11 @@ double dense_cost(const DenseModel& m, const DenseShape& d, int mode, bool count
         t = (double)d.sumR * (double)dwords * 4.0 / m.l2_bps + (double)(d.K * dwords * 4) / m.hbm_bps;
     }
     if (bytes > mem_room) return -1;
-    // product written once, read back by the seeding, plus one LDS op per set entry
-    t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps + set_bits * m.seed_s + m.launch_s;
+    // product written once, read back by the seeding, plus one LDS op per set
+    // entry (per word when the columns are nearly all z IDs: word copies)
+    const bool words = !count && d.Nh * 10 >= d.nz * 9;
+    t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps + (words ? cells / 32.0 : set_bits) * m.seed_s +
+         m.launch_s;
     if (bytes_out) *bytes_out = bytes;
     return t;
 }
@@ -355,6 +362,7 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     const bool count = P.count_mode;
     // suffix sums from the heaviest bin down
     DenseShape acc;
+    acc.nz = P.nz;
     int best_mode = DIM3_DENSE_OFF;
     uint32_t best_b = 0;
     double best_gain = -1e300;
@@ -398,6 +406,10 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     P.Kp = (P.K + 63u) / 64u * 64u;
     P.Mh = (uint32_t)best_shape.Mh;
     P.Nh = (uint32_t)best_shape.Nh;
+    // columns covering nearly every z: take them all, so that each tile's
+    // columns are its consecutive z IDs and the seeding copies words
+    P.all_z = !P.count_mode && (uint64_t)P.Nh * 10 >= (uint64_t)P.nz * 9;
+    if (P.all_z) P.Nh = P.nz;
     P.dwords = (P.Nh + 31u) / 32u;
 }
 
@@ -576,7 +588,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         TRY(ws(c, WS_HZ2ZID, P.Nh, &hz2zid));
         TRY(ws(c, WS_HZTILE, (uint64_t)P.ntiles + 1, &hz_tile));
         dim3h::dense_mark(degR, degS, (uint32_t)ny1, exY, degX, exX, (uint32_t)nx1, xw, P.z_ident ? nullptr : degZ,
-                          P.z_ident ? nullptr : exZ, (uint32_t)nz1, zw, P.bmin, hid, hx, hx2x, h2y, zflag,
+                          P.z_ident ? nullptr : exZ, (uint32_t)nz1, zw, P.bmin, hid, hx, hx2x, h2y, zflag, P.all_z,
                           (uint32_t*)&sc->spare[0], st);
         dim3h::scan_u32(zflag, P.nz, hzpos, part, &sc->nzh, st);
         dim3h::dense_columns(zflag, hzpos, P.nz, hz2zid, P.ntiles, P.tile_size, P.Nh, hz_tile, st);
@@ -628,6 +640,16 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         LAUNCHCHK(c, "dense");
     }
     HIPCHK(c, hipEventRecord(c->ev[6], st));
+    if (o.output_mode == DIM3_OUT_FINGERPRINT && !P.count_mode) {
+        ulonglong2* gw;
+        uint32_t* gwm;
+        const uint64_t nw = (uint64_t)P.ntiles * P.tile_words;
+        TRY(ws(c, WS_FP_GW, nw, &gw));
+        TRY(ws(c, WS_FP_GWM, nw, &gwm));
+        dim3h::fp_tables(P.z_ident ? nullptr : id2z, P.zlo, P.nz, P.ntiles, P.tile_words, P.tile_size,
+                         P.dense == DIM3_DENSE_OFF ? 1u : 0u, gw, gwm, st);
+        LAUNCHCHK(c, "fingerprint word totals");
+    }
     return DIM3_OK;
 }
 
@@ -655,8 +677,13 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.touched_cap = P.touched_cap;
     a.stage_cap = P.stage_cap;
     a.fp_mode = fp_mode ? 1u : 0u;
+    a.swz = P.dense == DIM3_DENSE_OFF ? 1u : 0u;
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
+    if (fp_mode && !P.count_mode) {
+        a.gw = (const ulonglong2*)W[WS_FP_GW].p;
+        a.gwm = (const uint32_t*)W[WS_FP_GWM].p;
+    }
     if (P.dense != DIM3_DENSE_OFF) {
         a.hx = (const int32_t*)W[WS_HX].p;
         if (P.count_mode) a.cint = (const int32_t*)W[WS_CINT].p;
@@ -706,7 +733,7 @@ void fill_result(dim3_ctx* c, dim3_result* res, const Scalars& s, bool timing) {
     res->swapped = P.swapped ? 1u : 0u;
     res->fp.count = res->n;
     res->fp.sum = P.empty ? 0 : s.fp_sum;
-    res->fp.xr = P.empty ? 0 : s.fp_xr;
+    res->fp.sum2 = P.empty ? 0 : s.fp_sum2;
     res->fp.csum = P.empty ? 0 : s.fp_csum;
     dim3_stats& st = res->stats;
     memset(&st, 0, sizeof(st));
@@ -868,7 +895,7 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     } reset{c};
 
     // plan every instance into bank i + 1 with one shared workgroup shape
-    const int block = o.block ? o.block : 512;
+    const int block = o.block ? o.block : 256;
     dim3_opts ob = o;
     ob.block = block;
     const int wgs = block == 256 ? 4 : block == 512 ? 2 : 1;

@@ -239,7 +239,7 @@ struct Scalars {
     uint64_t tot_s;      // total of the tiled S count scan
     uint64_t out_n;      // rows emitted (may exceed capacity)
     uint64_t fp_sum;
-    uint64_t fp_xr;
+    uint64_t fp_sum2;
     uint64_t fp_csum;
     uint64_t paths;      // sparse join paths expanded
     uint32_t task;       // expansion work counter
@@ -338,6 +338,7 @@ struct ExpandArgs {
     uint32_t touched_cap;
     uint32_t stage_cap;
     uint32_t fp_mode;
+    uint32_t swz;  // project tile bit layout: 1 swizzled, 0 plain (dense seeding copies words)
     Scalars* sc;
     void* out;  // int2* (project) or dim3_count_row* (aggregate)
     uint64_t cap;
@@ -349,6 +350,9 @@ struct ExpandArgs {
     const uint32_t* hz_tile;  // first column of each z tile, ntiles + 1 entries
     uint32_t dwords;
     uint32_t Nh;
+    // fingerprint word totals (project fp_mode)
+    const ulonglong2* gw;     // z-factor sums per tile word
+    const uint32_t* gwm;      // valid-bit mask per tile word
 };
 
 // returns 0 or a hipError_t
@@ -369,7 +373,7 @@ void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const 
 void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
                 const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
                 const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
-                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, uint32_t* counters, hipStream_t st);
+                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, bool all_z, uint32_t* counters, hipStream_t st);
 void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, uint32_t* hz2zid, uint32_t ntiles,
                    uint32_t tile_size, uint32_t Nh, uint32_t* hz_tile, hipStream_t st);
 // A rows from the R CSR (heavy entries carry bit 31); Kp bytes per row
@@ -380,6 +384,10 @@ void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, con
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
                    int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* hzpos,
                    uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow, hipStream_t st);
+
+// fingerprint word totals of one project plan: gw / gwm[ntiles * tile_words]
+void fp_tables(const int32_t* id2z, int32_t zlo, uint32_t nz, uint32_t ntiles, uint32_t tile_words,
+               uint32_t tile_size, uint32_t swz, ulonglong2* gw, uint32_t* gwm, hipStream_t st);
 
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);

@@ -246,6 +246,7 @@ __global__ void k_mark_hx(const uint32_t* __restrict__ degX, const uint64_t* __r
 }
 
 // column flag of every z ID (hash mode: occupied slots; identity: every ID)
+// (bmin == 0: every z ID is a column, so the columns of a tile are its z IDs)
 __global__ void k_mark_zflag(const uint32_t* __restrict__ degZ, const uint64_t* __restrict__ ex_z, uint32_t nz1,
                              const uint32_t* __restrict__ zw, uint32_t bmin, uint32_t* __restrict__ zflag) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,7 +256,7 @@ __global__ void k_mark_zflag(const uint32_t* __restrict__ degZ, const uint64_t* 
         if (degZ[s] == 0) return;
         zid = (uint32_t)(ex_z[s] >> 32);
     }
-    zflag[zid] = (zw[s] != 0 && dim3h::dense_bin(zw[s]) >= bmin) ? 1u : 0u;
+    zflag[zid] = (bmin == 0u || (zw[s] != 0 && dim3h::dense_bin(zw[s]) >= bmin)) ? 1u : 0u;
 }
 
 // hz -> z ID, and the first column of every z tile (ntiles + 1 entries)
@@ -401,12 +402,13 @@ void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const 
 void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
                 const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
                 const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
-                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, uint32_t* counters, hipStream_t st) {
+                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, bool all_z, uint32_t* counters, hipStream_t st) {
     hipLaunchKernelGGL(k_mark_hid, dim3(grid1(ny1)), dim3(256), 0, st, degR, degS, ny1, ex_y, bmin, hid, h2y,
                        counters);
     hipLaunchKernelGGL(k_mark_hx, dim3(grid1(nx1)), dim3(256), 0, st, degX, ex_x, nx1, xw, bmin, hx, hx2x,
                        counters + 1);
-    hipLaunchKernelGGL(k_mark_zflag, dim3(grid1(nz1)), dim3(256), 0, st, degZ, ex_z, nz1, zw, bmin, zflag);
+    hipLaunchKernelGGL(k_mark_zflag, dim3(grid1(nz1)), dim3(256), 0, st, degZ, ex_z, nz1, zw, all_z ? 0u : bmin,
+                       zflag);
 }
 
 void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, uint32_t* hz2zid, uint32_t ntiles,

@@ -40,19 +40,6 @@ __device__ __forceinline__ uint32_t list_at(const uint32_t* touched, bool listed
     return listed ? touched[i] : i;
 }
 
-template <int BLOCK>
-__device__ __forceinline__ uint64_t block_reduce_xor64(uint64_t v, uint64_t* sh) {
-    constexpr int NW = BLOCK / 64;
-    v = wave_reduce_xor(v);
-    if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    uint64_t r = 0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) r ^= sh[w];
-    __syncthreads();
-    return r;
-}
-
 // Exclusive scan over the block with two barriers; sh needs NW + 1 words and
 // must not be reused before the next barrier.
 template <int BLOCK>
@@ -117,6 +104,13 @@ __device__ __forceinline__ uint32_t swz_word(uint32_t zl, uint32_t W, uint32_t C
     return q >= W ? q - W : q;
 }
 
+// z factors of the row fingerprint: g1 = fmix32(u(z) ^ 0x243F6A88),
+// g2 = (g1 * 0x2545F491) ^ (g1 >> 16)
+__device__ __forceinline__ void zfactors(int32_t z, uint32_t& g1, uint32_t& g2) {
+    g1 = hash32((uint32_t)z ^ 0x243F6A88u);
+    g2 = (g1 * 0x2545F491u) ^ (g1 >> 16);
+}
+
 __device__ __forceinline__ uint32_t swz_offset(uint32_t word, uint32_t r, uint32_t W, uint32_t C) {
     int32_t q = (int32_t)word - (int32_t)(r * C);
     if (q < 0) q += (int32_t)W;
@@ -160,7 +154,7 @@ struct Lds {
 
 template <int BLOCK, bool COUNT, bool ZID>
 __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32_t t, const Lds& L, uint64_t& fsum,
-                                         uint64_t& fxr, uint64_t& fcs, uint64_t& npaths) {
+                                         uint64_t& fsum2, uint64_t& fcs, uint64_t& npaths) {
     constexpr uint32_t NW = BLOCK / 64;
     uint32_t* tile = L.tile;
     uint32_t* touched = L.touched;
@@ -173,7 +167,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t W = a.tile_words;
-    const uint32_t C = COUNT ? 0u : W / 32u;  // bit swizzle stride of project tiles
+    const uint32_t C = (COUNT || !a.swz) ? 0u : W / 32u;  // bit swizzle stride of project tiles (0: plain)
     // rows with more paths than this sweep their tile instead of listing first touches
     const uint64_t track_limit = min(4ull * a.touched_cap, 4ull * a.tile_words);
     const uint32_t zbase = t * a.tile_size;
@@ -191,6 +185,25 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
             for (uint32_t h = h0 + tid; h < h1; h += BLOCK) {
                 const int32_t v = crow[h];
                 if (v != 0) tile[a.hz2zid[h] - zbase] = (uint32_t)v;
+            }
+        } else if (h0 < h1 && C == 0u && a.hz2zid[h1 - 1u] - a.hz2zid[h0] == h1 - 1u - h0) {
+            // the tile's product columns are consecutive z IDs: shifted word
+            // copies (plain bit layout), tile bit p <- column p - off
+            const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
+            const int64_t off = (int64_t)a.hz2zid[h0] - (int64_t)zbase - (int64_t)h0;
+            const uint32_t ql = (uint32_t)(((int64_t)h0 + off) >> 5), qh = (uint32_t)(((int64_t)h1 - 1 + off) >> 5);
+            for (uint32_t q = ql + tid; q <= qh; q += BLOCK) {
+                const int64_t c0 = (int64_t)q * 32 - off;  // column of tile bit 32 q
+                const int64_t wlo = c0 >= 0 ? c0 >> 5 : -1;
+                const uint32_t sh = (uint32_t)(c0 - wlo * 32);
+                const uint32_t lo = (wlo >= 0 && wlo < (int64_t)a.dwords) ? brow[wlo] : 0u;
+                const uint32_t hi = (sh != 0u && wlo + 1 < (int64_t)a.dwords) ? brow[wlo + 1] : 0u;
+                uint32_t v = sh ? (lo >> sh) | (hi << (32u - sh)) : lo;
+                // keep columns h0 .. h1 - 1 only
+                const int64_t first = (int64_t)h0 - c0, last = (int64_t)h1 - 1 - c0;  // bit range in v
+                if (first > 0) v &= first >= 32 ? 0u : ~0u << first;
+                if (last < 31) v &= last < 0 ? 0u : (~0u >> (31 - last));
+                tile[q] = v;
             }
         } else if (h0 < h1) {
             const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
@@ -346,29 +359,49 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const int32_t* id2z = a.id2z + zbase;
 
     if (a.fp_mode) {
+        // Fingerprint: sum the z factors of the row's rows (a word with more
+        // than 16 bits starts from its precomputed word total and subtracts
+        // the missing ones), then one multiply by the row's x factors.
+        uint64_t S1 = 0, S2 = 0;
+        const uint32_t wbase = t * W;
         for (uint32_t i = wlo + lane; i < whi; i += 64u) {
             const uint32_t w = list_at(touched, listed, i);
             uint32_t v = tile[w];
             tile[w] = 0;
             if (COUNT) {
                 if (v) {
-                    const int32_t zv = ZID ? (int32_t)(zoff + w) : id2z[w];
-                    const uint64_t h = mix64(mix64(row_key(xv, zv)) + (uint64_t)v);
-                    fsum += h;
-                    fxr ^= h;
+                    uint32_t g1, g2;
+                    zfactors(ZID ? (int32_t)(zoff + w) : id2z[w], g1, g2);
+                    S1 += g1;
+                    S2 += (uint64_t)v * g2;
                     fcs += v;
                 }
-            } else {
+            } else if (v) {
+                uint64_t s1 = 0, s2 = 0;
+                const bool full = __popc(v) > 16;
+                if (full) v = ~v & a.gwm[wbase + w];
                 while (v) {
                     const uint32_t b = (uint32_t)__ffs(v) - 1u;
                     v &= v - 1u;
                     const uint32_t e = swz_offset(w, b, W, C);
-                    const int32_t zv = ZID ? (int32_t)(zoff + e) : id2z[e];
-                    const uint64_t h = mix64(row_key(xv, zv));
-                    fsum += h;
-                    fxr ^= h;
+                    uint32_t g1, g2;
+                    zfactors(ZID ? (int32_t)(zoff + e) : id2z[e], g1, g2);
+                    s1 += g1;
+                    s2 += g2;
                 }
+                if (full) {
+                    const ulonglong2 G = a.gw[wbase + w];
+                    s1 = G.x - s1;
+                    s2 = G.y - s2;
+                }
+                S1 += s1;
+                S2 += s2;
             }
+        }
+        if (S1 | S2) {
+            const uint64_t ux = (uint64_t)(uint32_t)xv;
+            fsum += (mix64(ux ^ 0x243F6A8885A308D3ull) | 1ull) * S1;
+            fsum2 += (mix64(ux ^ 0x13198A2E03707344ull) | 1ull) * S2;
         }
         if (tid == 0 && T) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
         __syncthreads();
@@ -507,6 +540,25 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     __syncthreads();
 }
 
+// Sums of the per-lane row statistics into the instance's scalars (block-uniform call).
+template <int BLOCK, bool COUNT>
+__device__ __forceinline__ void flush_stats(const ExpandArgs& a, uint64_t& fsum, uint64_t& fsum2, uint64_t& fcs,
+                                            uint64_t& npaths, uint64_t* sh) {
+    const uint64_t np = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
+    if (threadIdx.x == 0 && np) atomicAdd((unsigned long long*)&a.sc->paths, (unsigned long long)np);
+    if (a.fp_mode) {
+        const uint64_t s1 = block_reduce_sum<BLOCK, uint64_t>(fsum, sh);
+        const uint64_t s2 = block_reduce_sum<BLOCK, uint64_t>(fsum2, sh);
+        const uint64_t s3 = block_reduce_sum<BLOCK, uint64_t>(fcs, sh);
+        if (threadIdx.x == 0) {
+            atomicAdd((unsigned long long*)&a.sc->fp_sum, (unsigned long long)s1);
+            atomicAdd((unsigned long long*)&a.sc->fp_sum2, (unsigned long long)s2);
+            if (COUNT) atomicAdd((unsigned long long*)&a.sc->fp_csum, (unsigned long long)s3);
+        }
+    }
+    fsum = fsum2 = fcs = npaths = 0;
+}
+
 template <int BLOCK, bool COUNT, bool ZID>
 __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     extern __shared__ __align__(16) uint32_t smem[];
@@ -521,7 +573,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     const uint32_t tid = threadIdx.x;
     const uint32_t ntasks = a.nx * a.ntiles;
     for (uint32_t w = tid; w < a.tile_words; w += BLOCK) L.tile[w] = 0;
-    uint64_t fsum = 0, fxr = 0, fcs = 0, npaths = 0;
+    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
 
     for (;;) {
         if (tid == 0) {
@@ -533,40 +585,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
         if (task >= ntasks) break;
         const uint32_t x = task / a.ntiles;
-        row_task<BLOCK, COUNT, ZID>(a, x, task - x * a.ntiles, L, fsum, fxr, fcs, npaths);
+        row_task<BLOCK, COUNT, ZID>(a, x, task - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
     }
 
-    npaths = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
-    if (tid == 0 && npaths) atomicAdd((unsigned long long*)&a.sc->paths, (unsigned long long)npaths);
-    if (a.fp_mode) {
-        fsum = block_reduce_sum<BLOCK, uint64_t>(fsum, sh);
-        fcs = block_reduce_sum<BLOCK, uint64_t>(fcs, sh);
-        fxr = block_reduce_xor64<BLOCK>(fxr, sh);
-        if (tid == 0) {
-            atomicAdd((unsigned long long*)&a.sc->fp_sum, (unsigned long long)fsum);
-            if (COUNT) atomicAdd((unsigned long long*)&a.sc->fp_csum, (unsigned long long)fcs);
-            atomicXor((unsigned long long*)&a.sc->fp_xr, (unsigned long long)fxr);
-        }
-    }
-}
-
-// Sums of the per-lane row statistics into the instance's scalars (block-uniform call).
-template <int BLOCK, bool COUNT>
-__device__ __forceinline__ void flush_stats(const ExpandArgs& a, uint64_t& fsum, uint64_t& fxr, uint64_t& fcs,
-                                            uint64_t& npaths, uint64_t* sh) {
-    const uint64_t np = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
-    if (threadIdx.x == 0 && np) atomicAdd((unsigned long long*)&a.sc->paths, (unsigned long long)np);
-    if (a.fp_mode) {
-        const uint64_t s1 = block_reduce_sum<BLOCK, uint64_t>(fsum, sh);
-        const uint64_t s2 = block_reduce_sum<BLOCK, uint64_t>(fcs, sh);
-        const uint64_t s3 = block_reduce_xor64<BLOCK>(fxr, sh);
-        if (threadIdx.x == 0) {
-            atomicAdd((unsigned long long*)&a.sc->fp_sum, (unsigned long long)s1);
-            if (COUNT) atomicAdd((unsigned long long*)&a.sc->fp_csum, (unsigned long long)s2);
-            atomicXor((unsigned long long*)&a.sc->fp_xr, (unsigned long long)s3);
-        }
-    }
-    fsum = fxr = fcs = npaths = 0;
+    flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);
 }
 
 // Many join instances in one persistent launch fed by one device work
@@ -589,7 +611,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
 
     const uint32_t tid = threadIdx.x;
     const uint32_t total = task_start[n_inst];
-    uint64_t fsum = 0, fxr = 0, fcs = 0, npaths = 0;
+    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
     int32_t cur = -1;
     ExpandArgs a;
     for (;;) {
@@ -608,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
             else hi = mid;
         }
         if ((int32_t)lo != cur) {
-            if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fxr, fcs, npaths, sh);
+            if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);
             cur = (int32_t)lo;
             a = args[lo];
             // the previous instance's touched list / staging area may overlap this tile
@@ -618,10 +640,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
         const uint32_t local = task - task_start[lo];
         const uint32_t x = local / a.ntiles;
         const Lds L{smem, smem + a.tile_words, smem + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
-        if (a.id2z == nullptr) row_task<BLOCK, COUNT, true>(a, x, local - x * a.ntiles, L, fsum, fxr, fcs, npaths);
-        else row_task<BLOCK, COUNT, false>(a, x, local - x * a.ntiles, L, fsum, fxr, fcs, npaths);
+        if (a.id2z == nullptr) row_task<BLOCK, COUNT, true>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
+        else row_task<BLOCK, COUNT, false>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
     }
-    if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fxr, fcs, npaths, sh);
+    if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);
 }
 
 // Zero the expansion-owned scalars of every instance of a batch (a kernel, so
@@ -633,7 +655,7 @@ __global__ void k_reset_scalars(dim3h::Scalars* const* __restrict__ list, uint32
     dim3h::Scalars* s = list[i];
     s->out_n = 0;
     s->fp_sum = 0;
-    s->fp_xr = 0;
+    s->fp_sum2 = 0;
     s->fp_csum = 0;
     s->paths = 0;
     s->task = 0;
@@ -645,6 +667,31 @@ __global__ void k_gather_scalars(dim3h::Scalars* const* __restrict__ list, uint3
     const uint64_t* src = reinterpret_cast<const uint64_t*>(list[i]);
     uint64_t* dst = reinterpret_cast<uint64_t*>(out + i);
     for (uint32_t w = threadIdx.x; w < sizeof(dim3h::Scalars) / 8; w += blockDim.x) dst[w] = src[w];
+}
+
+// Fingerprint word totals: for every word of a project tile (bit layout
+// above) the sums of the z factors of its valid IDs and their bit mask.
+__global__ void k_fp_gw(const int32_t* __restrict__ id2z, int32_t zlo, uint32_t nz, uint32_t ntiles, uint32_t W,
+                        uint32_t tile_size, uint32_t swz, ulonglong2* __restrict__ gw, uint32_t* __restrict__ gwm) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)ntiles * W) return;
+    const uint32_t t = (uint32_t)(idx / W), w = (uint32_t)(idx - (uint64_t)t * W);
+    const uint32_t C = swz ? W / 32u : 0u;
+    uint64_t s1 = 0, s2 = 0;
+    uint32_t m = 0;
+    for (uint32_t r = 0; r < 32; r++) {
+        const uint32_t e = swz_offset(w, r, W, C);
+        const uint64_t zid = (uint64_t)t * tile_size + e;
+        if (e < tile_size && zid < nz) {
+            uint32_t g1, g2;
+            zfactors(id2z != nullptr ? id2z[zid] : (int32_t)((uint32_t)zlo + (uint32_t)zid), g1, g2);
+            s1 += g1;
+            s2 += g2;
+            m |= 1u << r;
+        }
+    }
+    gw[idx] = make_ulonglong2(s1, s2);
+    gwm[idx] = m;
 }
 
 template <typename K>
@@ -687,6 +734,14 @@ namespace dim3h {
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block) {
     const uint32_t area = stage_cap > 3u * block + 1u ? stage_cap : 3u * block + 1u;
     return ((size_t)tile_words + touched_cap + area) * 4u;
+}
+
+void fp_tables(const int32_t* id2z, int32_t zlo, uint32_t nz, uint32_t ntiles, uint32_t tile_words,
+               uint32_t tile_size, uint32_t swz, ulonglong2* gw, uint32_t* gwm, hipStream_t st) {
+    const uint64_t nw = (uint64_t)ntiles * tile_words;
+    if (nw)
+        hipLaunchKernelGGL(k_fp_gw, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, id2z, zlo, nz, ntiles,
+                           tile_words, tile_size, swz, gw, gwm);
 }
 
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {

@@ -32,10 +32,10 @@ REF_SO = os.path.join(_HERE, "_ref", "libdim3ref.so")
 
 class Fingerprint(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("sum", ctypes.c_uint64),
-                ("xr", ctypes.c_uint64), ("csum", ctypes.c_uint64)]
+                ("sum2", ctypes.c_uint64), ("csum", ctypes.c_uint64)]
 
     def as_tuple(self):
-        return (int(self.count), int(self.sum), int(self.xr), int(self.csum))
+        return (int(self.count), int(self.sum), int(self.sum2), int(self.csum))
 
 
 _lib = None

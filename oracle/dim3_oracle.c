@@ -37,12 +37,16 @@
  *   - the estimate fallback (dim3.hpp:215-221) reads min(1000, n) tuples
  *     instead of reading past the end of small inputs.
  *
- * Fingerprint (order independent, shared with the GPU library):
- *   key = (u64)(u32)x << 32 | (u32)z, h = mix64(key)
- *   project:   count += 1, sum += h, xr ^= h
- *   aggregate: h' = mix64(h + (u64)count); count += 1, sum += h', xr ^= h',
- *              csum += count
- * mix64 is the splitmix64 finaliser (Steele, Lea, Flood 2014).
+ * Fingerprint (order independent, shared with the GPU library), all
+ * arithmetic mod 2^64 (x factors) or 2^32 (z factors), u(v) = (u32)v:
+ *   k1(x) = mix64(u(x) ^ 0x243F6A8885A308D3) | 1, k2(x) = mix64(u(x) ^ 0x13198A2E03707344) | 1
+ *   g1(z) = fmix32(u(z) ^ 0x243F6A88), g2(z) = (g1(z) * 0x2545F491) ^ (g1(z) >> 16)
+ *   project row (x, z):          count += 1, sum += k1(x) g1(z), sum2 += k2(x) g2(z)
+ *   aggregate row (x, z, c):     count += 1, sum += k1(x) g1(z), sum2 += k2(x) c g2(z), csum += c
+ * mix64 is the splitmix64 finaliser (Steele, Lea, Flood 2014), fmix32 the
+ * MurmurHash3 32-bit finaliser (a bijection, so one wrong z always changes
+ * sum).  Each term is an x factor times a z factor, so a row's share is k(x)
+ * times the sum of its z factors (the GPU sums z factors per row).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -55,7 +59,7 @@
 typedef struct {
     uint64_t count;
     uint64_t sum;
-    uint64_t xr;
+    uint64_t sum2;
     uint64_t csum;
 } orc_fp;
 
@@ -68,11 +72,32 @@ static inline uint64_t orc_mix64(uint64_t k) {
     return k;
 }
 
-static inline uint64_t orc_key(int32_t x, int32_t z) {
-    return ((uint64_t)(uint32_t)x << 32) | (uint64_t)(uint32_t)z;
+uint64_t oracle_mix64(uint64_t k) { return orc_mix64(k); }
+
+static inline uint32_t orc_fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
 }
 
-uint64_t oracle_mix64(uint64_t k) { return orc_mix64(k); }
+/* one row's terms of the fingerprint (see the header comment) */
+static inline void orc_fp_add(orc_fp* fp, int32_t x, int32_t z, const int64_t* count) {
+    const uint64_t ux = (uint64_t)(uint32_t)x;
+    const uint64_t k1 = orc_mix64(ux ^ 0x243F6A8885A308D3ULL) | 1u, k2 = orc_mix64(ux ^ 0x13198A2E03707344ULL) | 1u;
+    const uint32_t g1 = orc_fmix32((uint32_t)z ^ 0x243F6A88u);
+    const uint32_t g2 = (g1 * 0x2545F491u) ^ (g1 >> 16);
+    fp->count += 1;
+    fp->sum += k1 * (uint64_t)g1;
+    if (count) {
+        fp->sum2 += k2 * ((uint64_t)*count * (uint64_t)g2);
+        fp->csum += (uint64_t)*count;
+    } else {
+        fp->sum2 += k2 * (uint64_t)g2;
+    }
+}
 
 /* ------------------------------------------------------------------ */
 /* int32 -> dense ID map, open addressing with linear probing.          */
@@ -156,14 +181,7 @@ typedef struct {
 } orc_sink;
 
 static void sink_push(orc_sink* k, int32_t x, int32_t z, int64_t c) {
-    uint64_t h = orc_mix64(orc_key(x, z));
-    if (k->with_counts) {
-        h = orc_mix64(h + (uint64_t)c);
-        k->fp.csum += (uint64_t)c;
-    }
-    k->fp.count += 1;
-    k->fp.sum += h;
-    k->fp.xr ^= h;
+    orc_fp_add(&k->fp, x, z, k->with_counts ? &c : NULL);
     if (k->fingerprint_only || k->oom) return;
     if (k->n == k->cap) {
         uint64_t nc = k->cap ? 2 * k->cap : 1024;
@@ -474,13 +492,7 @@ void oracle_gen_rand_data(int32_t* R_xy, uint64_t nR, int32_t* S_zy, uint64_t nS
 /* Fingerprint of a row set given as arrays (counts may be NULL). */
 void oracle_fingerprint(const int32_t* pairs, const int64_t* counts, uint64_t n, orc_fp* fp) {
     memset(fp, 0, sizeof(*fp));
-    for (uint64_t i = 0; i < n; i++) {
-        uint64_t h = orc_mix64(orc_key(pairs[2 * i], pairs[2 * i + 1]));
-        if (counts) { h = orc_mix64(h + (uint64_t)counts[i]); fp->csum += (uint64_t)counts[i]; }
-        fp->count += 1;
-        fp->sum += h;
-        fp->xr ^= h;
-    }
+    for (uint64_t i = 0; i < n; i++) orc_fp_add(fp, pairs[2 * i], pairs[2 * i + 1], counts ? &counts[i] : NULL);
 }
 
 void oracle_free(void* p) { free(p); }

@@ -110,6 +110,29 @@ def test_fingerprint_is_order_independent_and_sensitive():
     assert oracle.fingerprint(p, c).as_tuple() != oracle.fingerprint(p, c2).as_tuple()
 
 
+def test_fingerprint_factorises_over_a_cross_product():
+    """Every term is k(x) g(z): a full cross product X x Z therefore has
+    sum = (sum_x k1(x)) (sum_z g1(z)) mod 2^64, and disjoint sets add."""
+    M = (1 << 64) - 1
+    X = np.array([-7, 0, 3, 1 << 30, 99], np.int64)
+    Z = np.array([5, -1, 2 ** 31 - 1, 8], np.int64)
+    p = np.array([(x, z) for x in X for z in Z], np.int32)
+    fp = oracle.fingerprint(p)
+    u = lambda v: int(v) & 0xFFFFFFFF
+    k1 = sum(oracle.mix64(u(x) ^ 0x243F6A8885A308D3) | 1 for x in X) & M
+    # the z factors of the row set, read off single-row fingerprints with x = 0
+    k0 = oracle.mix64(0x243F6A8885A308D3) | 1
+    inv = pow(k0, -1, 1 << 64)
+    g1 = sum((oracle.fingerprint(np.array([[0, z]], np.int32)).sum * inv) & M for z in Z) & M
+    assert all(0 <= (oracle.fingerprint(np.array([[0, z]], np.int32)).sum * inv) & M < (1 << 32) for z in Z)
+    assert fp.sum == (k1 * g1) & M and fp.count == len(p)
+    a, b = oracle.fingerprint(p[:7]), oracle.fingerprint(p[7:])
+    assert fp.sum == (a.sum + b.sum) & M and fp.sum2 == (a.sum2 + b.sum2) & M
+    # COUNT: a row with count c weighs its second term by c
+    c1, c2 = oracle.fingerprint(p[:1], np.array([1])), oracle.fingerprint(p[:1], np.array([3]))
+    assert c2.sum == c1.sum and c2.sum2 == (3 * c1.sum2) & M and c2.csum == 3
+
+
 def test_mix64_known_values():
     # splitmix64 finaliser (Steele, Lea, Flood 2014): the outputs of the
     # splitmix64 generator seeded with 0 are mix64(k * 0x9e3779b97f4a7c15), k = 1, 2, 3
