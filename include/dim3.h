@@ -193,6 +193,30 @@ int dim3_count_join_paths(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, cons
 int dim3_dense_product(dim3_ctx* ctx, const int8_t* A, uint64_t M, const int8_t* Bt, uint64_t N, uint64_t K,
                        uint32_t* bits, int32_t* counts);
 
+/* ---- cost-model calibration (autotuner) ----
+ * The planner prices the dense part against the sparse expansion with device
+ * rates.  dim3_autotune measures them on the context's device with short
+ * synthetic queries (complete bipartite joins with 1 and 32 paths per output
+ * row, all-sparse, MFMA and bit-or) plus an int8 product and a streaming fill,
+ * and installs them in the context.  It replaces the reference's offline
+ * constants (server_parameters.hpp, produced by server_parameter_estimator.cpp).
+ * cache_path (nullable): a text file; read when it holds a calibration of the
+ * same device model, else written after measuring. */
+typedef struct dim3_tuning {
+    double path_ns;      /* sparse expansion per join path */
+    double seed_bit_ns;  /* dense seeding per product bit */
+    double seed_word_ns; /* dense seeding per 32-bit word copied */
+    double mfma_tops;    /* int8 product, tera-ops/s */
+    double bitor_gbs;    /* bit-or product, GB/s of key-bitmap reads */
+    double hbm_gbs;      /* streaming writes, GB/s */
+    uint32_t source;     /* 0 static defaults, 1 measured, 2 read from cache_path */
+    uint32_t reserved;
+} dim3_tuning;
+
+int dim3_autotune(dim3_ctx* ctx, const char* cache_path, dim3_tuning* out);
+int dim3_get_tuning(const dim3_ctx* ctx, dim3_tuning* out);
+int dim3_set_tuning(dim3_ctx* ctx, const dim3_tuning* in);
+
 /* Library version string. */
 const char* dim3_version(void);
 

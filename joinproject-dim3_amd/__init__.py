@@ -66,6 +66,15 @@ class Instance(ctypes.Structure):
                 ("n_s", ctypes.c_uint64), ("out", ctypes.c_void_p), ("cap", ctypes.c_uint64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("path_ns", ctypes.c_double), ("seed_bit_ns", ctypes.c_double), ("seed_word_ns", ctypes.c_double),
+                ("mfma_tops", ctypes.c_double), ("bitor_gbs", ctypes.c_double), ("hbm_gbs", ctypes.c_double),
+                ("source", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 COUNT_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("count", "<i8")])
 
 # every symbol include/dim3.h declares (checked by the ABI test)
@@ -73,7 +82,8 @@ EXPORTS = [
     "dim3_ctx_create", "dim3_ctx_destroy", "dim3_last_error", "dim3_status_string", "dim3_ctx_device_bytes",
     "dim3_opts_default", "dim3_join_project", "dim3_join_aggregate_count", "dim3_result_free",
     "dim3_join_project_dev", "dim3_join_aggregate_count_dev", "dim3_count_join_paths", "dim3_dense_product",
-    "dim3_batch_join_project_dev", "dim3_batch_join_aggregate_count_dev",
+    "dim3_batch_join_project_dev", "dim3_batch_join_aggregate_count_dev", "dim3_autotune", "dim3_get_tuning",
+    "dim3_set_tuning",
     "dim3_version",
 ]
 
@@ -132,6 +142,12 @@ def lib():
         f.argtypes = [vp, ctypes.POINTER(Instance), ctypes.c_uint32, ctypes.POINTER(Opts), vp,
                       ctypes.POINTER(Result)]
         f.restype = i32
+    L.dim3_autotune.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(Tuning)]
+    L.dim3_autotune.restype = i32
+    L.dim3_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.dim3_get_tuning.restype = i32
+    L.dim3_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.dim3_set_tuning.restype = i32
     L.dim3_dense_product.argtypes = [vp, vp, u64, vp, u64, u64, vp, vp]
     L.dim3_dense_product.restype = i32
     L.dim3_version.argtypes = []
@@ -322,6 +338,24 @@ class Context:
     def batch_join_aggregate_count_dev(self, instances, stream=None, **kw):
         return self._batch(self._L.dim3_batch_join_aggregate_count_dev, "dim3_batch_join_aggregate_count_dev",
                            instances, stream, kw)
+
+    def autotune(self, cache_path=None):
+        """Measure the planner's device rates (dim3_autotune); returns them as a dict."""
+        t = Tuning()
+        rc = self._L.dim3_autotune(self._h, cache_path.encode() if cache_path else None, ctypes.byref(t))
+        self._check(rc, "dim3_autotune")
+        return t.as_dict()
+
+    def tuning(self):
+        t = Tuning()
+        self._check(self._L.dim3_get_tuning(self._h, ctypes.byref(t)), "dim3_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, d):
+        t = Tuning()
+        for k, v in d.items():
+            setattr(t, k, v)
+        self._check(self._L.dim3_set_tuning(self._h, ctypes.byref(t)), "dim3_set_tuning")
 
     def count_join_paths(self, R, S):
         R, S = _pairs(R), _pairs(S)

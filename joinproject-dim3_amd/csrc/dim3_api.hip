@@ -131,6 +131,18 @@ struct Bank {
     QueryPlan plan;
 };
 
+// Device rates of the planner's cost model.  The defaults are static
+// estimates; dim3_autotune replaces them with rates measured on the device.
+struct DenseModel {
+    double path_s = 0.9e-12;       // sparse expansion, per join path
+    double seed_s = 1.0e-12;       // seeding, per product bit or count (a gathered z ID + one LDS op)
+    double seed_word_s = 1.0e-12;  // seeding, per 32-bit word copied
+    double mfma_ops = 1.5e15;      // int8 ops/s reached by k_gemm_i8
+    double hbm_bps = 5.0e12;       // streaming bytes/s
+    double l2_bps = 10.0e12;       // bit-or re-reads of key bitmaps
+    double launch_s = 60e-6;       // fixed cost of the dense launches
+};
+
 }  // namespace
 
 struct dim3_ctx {
@@ -149,6 +161,8 @@ struct dim3_ctx {
     uint32_t h_batch_cap = 0;
     hipEvent_t ev[8] = {};
     std::recursive_mutex mu;
+    DenseModel model;           // cost-model rates (defaults or measured)
+    uint32_t model_source = 0;  // 0 defaults, 1 measured, 2 read from a cache file
 
     Bank& bank() { return *banks[cur]; }
     void use_bank(uint32_t b) {
@@ -307,15 +321,6 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
 // model (device rates) prices the sparse work those paths would take against
 // the dense product and its seeding; AUTO takes the best positive saving.
 // ---------------------------------------------------------------------------
-struct DenseModel {
-    double path_s = 0.9e-12;   // sparse expansion, per join path
-    double seed_s = 1.0e-12;   // seeding, per product bit or count (a gathered z ID + one LDS op)
-    double mfma_ops = 1.5e15;  // int8 ops/s reached by k_gemm_i8
-    double hbm_bps = 5.0e12;   // streaming bytes/s
-    double l2_bps = 10.0e12;   // bit-or re-reads of key bitmaps
-    double launch_s = 60e-6;   // fixed cost of the dense launches
-};
-
 struct DenseShape {
     uint64_t K = 0, Mh = 0, Nh = 0, paths = 0, sumR = 0, sumS = 0;
     uint64_t nz = 0;  // all z IDs: product columns covering nearly all of them seed whole words
@@ -345,8 +350,8 @@ double dense_cost(const DenseModel& m, const DenseShape& d, int mode, bool count
     // product written once, read back by the seeding, plus one LDS op per set
     // entry (per word when the columns are nearly all z IDs: word copies)
     const bool words = !count && d.Nh * 10 >= d.nz * 9;
-    t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps + (words ? cells / 32.0 : set_bits) * m.seed_s +
-         m.launch_s;
+    t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps +
+         (words ? cells / 32.0 * m.seed_word_s : set_bits * m.seed_s) + m.launch_s;
     if (bytes_out) *bytes_out = bytes;
     return t;
 }
@@ -357,7 +362,7 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     if (o.dense_mode == DIM3_DENSE_OFF) return;
     const unsigned long long* h = c->h_hist;
     const uint32_t NB = dim3h::kDenseBins;
-    const DenseModel m;
+    const DenseModel& m = c->model;
     const uint64_t room = c->budget > c->held ? (c->budget - c->held) / 3 : 0;
     const bool count = P.count_mode;
     // suffix sums from the heaviest bin down
@@ -986,9 +991,173 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     return rc;
 }
 
+void model_to_tuning(const DenseModel& m, uint32_t source, dim3_tuning* t) {
+    memset(t, 0, sizeof(*t));
+    t->path_ns = m.path_s * 1e9;
+    t->seed_bit_ns = m.seed_s * 1e9;
+    t->seed_word_ns = m.seed_word_s * 1e9;
+    t->mfma_tops = m.mfma_ops / 1e12;
+    t->bitor_gbs = m.l2_bps / 1e9;
+    t->hbm_gbs = m.hbm_bps / 1e9;
+    t->source = source;
+}
+
+void tuning_to_model(const dim3_tuning& t, DenseModel* m) {
+    m->path_s = t.path_ns * 1e-9;
+    m->seed_s = t.seed_bit_ns * 1e-9;
+    m->seed_word_s = t.seed_word_ns * 1e-9;
+    m->mfma_ops = t.mfma_tops * 1e12;
+    m->l2_bps = t.bitor_gbs * 1e9;
+    m->hbm_bps = t.hbm_gbs * 1e9;
+}
+
+float elapsed(dim3_ctx* c, int a, int b) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
+    return ms;
+}
+
+// One fingerprint query of two complete bipartite relations X x Y and Z x Y
+// (every output row reached by Y paths); returns the plan's timings.
+int tune_query(dim3_ctx* c, uint32_t X, uint32_t Z, uint32_t Y, int dense, float* ms_dense, float* ms_expand,
+               QueryPlan* plan) {
+    hipStream_t st = c->stream;
+    int2 *R, *S;
+    TRY(ws(c, WS_IN_R, (uint64_t)X * Y, &R));
+    TRY(ws(c, WS_IN_S, (uint64_t)Z * Y, &S));
+    dim3h::grid_relation(R, X, Y, st);
+    dim3h::grid_relation(S, Z, Y, st);
+    dim3_opts o;
+    dim3_opts_default(&o);
+    o.output_mode = DIM3_OUT_FINGERPRINT;
+    o.flags = DIM3_OPT_NO_SWAP;
+    o.dense_mode = dense;
+    TRY(prepare(c, false, R, (uint64_t)X * Y, S, (uint64_t)Z * Y, o, st));
+    TRY(expand(c, true, nullptr, 0, st));
+    *ms_dense = elapsed(c, 3, 6);
+    *ms_expand = elapsed(c, 4, 5);
+    *plan = c->bank().plan;
+    return DIM3_OK;
+}
+
+int autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    std::string dev = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+    for (auto& ch : dev)
+        if (ch == ' ') ch = '_';
+    if (cache_path) {
+        FILE* f = fopen(cache_path, "r");
+        if (f) {
+            char name[256] = {0};
+            dim3_tuning t;
+            memset(&t, 0, sizeof(t));
+            const int n = fscanf(f, "dim3-tuning-v1 %255s %lf %lf %lf %lf %lf %lf", name, &t.path_ns, &t.seed_bit_ns,
+                                 &t.seed_word_ns, &t.mfma_tops, &t.bitor_gbs, &t.hbm_gbs);
+            fclose(f);
+            if (n == 7 && dev == name && t.path_ns > 0 && t.mfma_tops > 0 && t.hbm_gbs > 0) {
+                tuning_to_model(t, &c->model);
+                c->model_source = 2;
+                if (out) model_to_tuning(c->model, 2, out);
+                return DIM3_OK;
+            }
+        }
+    }
+    hipStream_t st = c->stream;
+    DenseModel m = c->model;
+    // streaming writes: 256 MiB fill
+    {
+        uint32_t* buf;
+        const uint64_t n = 64ull << 20;
+        TRY(ws(c, WS_OUT, n, &buf));
+        dim3h::fill_u32(buf, n, 0, st);
+        HIPCHK(c, hipEventRecord(c->ev[0], st));
+        for (int r = 0; r < 4; r++) dim3h::fill_u32(buf, n, (uint32_t)r, st);
+        HIPCHK(c, hipEventRecord(c->ev[1], st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        m.hbm_bps = 4.0 * n * 4 / (elapsed(c, 0, 1) * 1e-3);
+    }
+    // int8 product on the MFMA cores: 4096 x 4096 x 4096
+    {
+        const uint32_t M = 4096, K = 4096;
+        uint32_t *A, *B, *D;
+        TRY(ws(c, WS_DA, (uint64_t)M * K / 4, &A));
+        TRY(ws(c, WS_DB, (uint64_t)M * K / 4, &B));
+        TRY(ws(c, WS_DBITS, (uint64_t)M * (M / 32), &D));
+        dim3h::fill_u32(A, (uint64_t)M * K / 4, 0x01010101u, st);
+        dim3h::fill_u32(B, (uint64_t)M * K / 4, 0x01000100u, st);
+        dim3h::gemm_i8((const int8_t*)A, (const int8_t*)B, M, M, K, D, M / 32, nullptr, st);
+        HIPCHK(c, hipEventRecord(c->ev[0], st));
+        for (int r = 0; r < 3; r++) dim3h::gemm_i8((const int8_t*)A, (const int8_t*)B, M, M, K, D, M / 32, nullptr, st);
+        HIPCHK(c, hipEventRecord(c->ev[1], st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        m.mfma_ops = 3.0 * 2.0 * M * M * (double)K / (elapsed(c, 0, 1) * 1e-3);
+    }
+    // sparse paths: the same 16384 x 16384 output reached by 1 and by 32 paths
+    const uint32_t X = 16384, Z = 16384;
+    float d1, e1, d32, e32, dm, em, db, eb;
+    QueryPlan p1, p32, pm, pb;
+    TRY(tune_query(c, X, Z, 1, DIM3_DENSE_OFF, &d1, &e1, &p1));
+    TRY(tune_query(c, X, Z, 32, DIM3_DENSE_OFF, &d32, &e32, &p32));
+    const double dpaths = (double)p32.out_j - (double)p1.out_j;
+    if (e32 > e1 && dpaths > 0) m.path_s = (e32 - e1) * 1e-3 / dpaths;
+    m.seed_s = m.path_s;  // a product bit costs what a path costs: one z ID and one LDS op
+    // dense products of the 32-path query: MFMA, then bit-or
+    TRY(tune_query(c, X, Z, 32, DIM3_DENSE_MFMA, &dm, &em, &pm));
+    TRY(tune_query(c, X, Z, 32, DIM3_DENSE_BITOR, &db, &eb, &pb));
+    const double words = (double)X * ((Z + 31) / 32);
+    // seeding copies every word of the product; the emission is that of the 1-path query
+    if (pm.dense == DIM3_DENSE_MFMA && em > e1) m.seed_word_s = (em - e1) * 1e-3 / words;
+    if (pb.dense == DIM3_DENSE_BITOR && db > 0) {
+        const double reads = (double)X * 32.0 * (double)pb.dwords * 4.0;  // each row ORs 32 key bitmaps
+        m.l2_bps = reads / (db * 1e-3);
+    }
+    c->model = m;
+    c->model_source = 1;
+    if (out) model_to_tuning(m, 1, out);
+    if (cache_path) {
+        FILE* f = fopen(cache_path, "w");
+        if (f) {
+            dim3_tuning t;
+            model_to_tuning(m, 1, &t);
+            fprintf(f, "dim3-tuning-v1 %s %.6g %.6g %.6g %.6g %.6g %.6g\n", dev.c_str(), t.path_ns, t.seed_bit_ns,
+                    t.seed_word_ns, t.mfma_tops, t.bitor_gbs, t.hbm_gbs);
+            fclose(f);
+        }
+    }
+    return DIM3_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int dim3_autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
+    if (!c) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    HIPCHK(c, hipSetDevice(c->device));
+    c->use_bank(0);
+    return autotune(c, cache_path, out);
+}
+
+int dim3_get_tuning(const dim3_ctx* c, dim3_tuning* out) {
+    if (!c || !out) return DIM3_ERR_INVALID;
+    model_to_tuning(c->model, c->model_source, out);
+    return DIM3_OK;
+}
+
+int dim3_set_tuning(dim3_ctx* c, const dim3_tuning* in) {
+    if (!c || !in) return DIM3_ERR_INVALID;
+    if (!(in->path_ns > 0 && in->seed_bit_ns > 0 && in->seed_word_ns > 0 && in->mfma_tops > 0 && in->bitor_gbs > 0 &&
+          in->hbm_gbs > 0))
+        return fail(c, DIM3_ERR_INVALID, "tuning rates must be positive");
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    tuning_to_model(*in, &c->model);
+    c->model_source = in->source;
+    return DIM3_OK;
+}
+
 
 const char* dim3_version(void) { return "dim3-mi355x 0.1 (gfx950)"; }
 

@@ -444,9 +444,22 @@ __global__ void k_x2z_self(const int32_t* __restrict__ id2x, uint32_t nx, dim3h:
     x2z[i] = r;
 }
 
+// Complete bipartite relation: tuple i = (i / Y, i % Y) for i < X * Y
+// (synthetic calibration queries).
+__global__ void k_grid_rel(int2* __restrict__ out, uint32_t X, uint32_t Y) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)X * Y) return;
+    out[i] = make_int2((int32_t)(i / Y), (int32_t)(i % Y));
+}
+
 }  // namespace
 
 namespace dim3h {
+
+void grid_relation(int2* out, uint32_t X, uint32_t Y, hipStream_t st) {
+    const uint64_t n = (uint64_t)X * Y;
+    if (n) hipLaunchKernelGGL(k_grid_rel, dim3(grid_for(n)), dim3(kThreads), 0, st, out, X, Y);
+}
 
 size_t scan_partials_needed(uint64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
 
