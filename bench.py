@@ -94,10 +94,11 @@ def cpu_baseline(E, target_s, slices, S=None, self_filter=True):
                          "; u != w applied after the reference" if self_filter else "")}
 
 
-def device_block(torch, block_keys, density, seed, chunk_rows=4096):
+def device_block(torch, block_keys, density, seed, chunk_rows=4096, as_host=True):
     """(attr, key) cells of a block_keys-square grid, each kept with
     probability `density`, drawn and shuffled on the device (torch's Philox
-    generator, seeded); returned as a host int32 array."""
+    generator, seeded); returned as a host int32 array (or the device
+    tensor)."""
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     parts = []
@@ -109,7 +110,7 @@ def device_block(torch, block_keys, density, seed, chunk_rows=4096):
         parts.append(nz)
     B = torch.cat(parts)
     B = B[torch.randperm(B.shape[0], device="cuda", generator=g)]
-    return B.cpu().numpy()
+    return B.cpu().numpy() if as_host else B
 
 
 def batch_bench(args, d, torch):
@@ -217,6 +218,129 @@ def batch_bench(args, d, torch):
     tctx.close()
 
 
+def large_request_relation(torch, workloads, n_background, seed, block_keys=20_000, density=0.3):
+    """A large served request's relation: Zipf(1.0) background of
+    n_background tuples (attribute domain 10%, key domain 1% of the size)
+    plus a planted block_keys x block_keys block at `density`, distinct."""
+    bg = workloads.zipf_relation_device(torch, n_background, max(1, n_background // 10),
+                                        max(1, n_background // 100), 1.0, seed)
+    blk = device_block(torch, block_keys, density, seed + 7, as_host=False)
+    key = torch.cat([(bg[:, 0].to(torch.int64) << 32) | (bg[:, 1].to(torch.int64) & 0xFFFFFFFF),
+                     (blk[:, 0].to(torch.int64) << 32) | (blk[:, 1].to(torch.int64) & 0xFFFFFFFF)])
+    key = torch.unique(key)
+    return torch.stack([(key >> 32).to(torch.int32), (key & 0xFFFFFFFF).to(torch.int32)], 1).contiguous()
+
+
+def percentile(v, q):
+    return float(np.percentile(np.asarray(v, dtype=np.float64), q))
+
+
+def serve_bench(args, d, torch):
+    """Served stream: Poisson arrivals of join-project (80%) and COUNT (20%)
+    requests with the batch workload's size law plus 5% large requests
+    (Zipf background plus a planted 20k x 20k block at 0.3).  The same
+    stream is replayed against the server (workers, continuous batching)
+    and against one request at a time (one worker, no batching) at an
+    arrival rate of --load times the one-at-a-time capacity."""
+    import threading
+
+    from dim3_amd import workloads
+
+    t0 = time.perf_counter()
+    reqs = workloads.served_stream(args.requests, rate_per_s=1.0, seed=4242, large_n=args.large_n)
+    rels = []
+    for q in reqs:
+        p = q["params"]
+        if p.get("large"):
+            R = large_request_relation(torch, workloads, p["n"], p["seed_r"])
+            S = large_request_relation(torch, workloads, p["n"], p["seed_s"])
+        else:
+            R, S = workloads.batch_instance_device(torch, p)
+        rels.append((R, S))
+    torch.cuda.synchronize()
+    n_tuples = sum(int(R.shape[0]) + int(S.shape[0]) for R, S in rels)
+    n_large = sum(1 for q in reqs if q["params"].get("large"))
+    log("stream: %d requests (%d large), %d tuples in %.1fs" % (len(reqs), n_large, n_tuples, time.perf_counter() - t0))
+
+    def replay(workers, max_batch, arrivals):
+        with d.Server(0, workers=workers, max_batch=max_batch) as srv:
+            hs = [(srv.put_dev(R.data_ptr(), int(R.shape[0])), srv.put_dev(S.data_ptr(), int(S.shape[0])))
+                  for R, S in rels]
+            tickets = [None] * len(reqs)
+
+            def submitter():
+                start = time.perf_counter()
+                for i, q in enumerate(reqs):
+                    delay = arrivals[i] - (time.perf_counter() - start)
+                    if delay > 0:
+                        time.sleep(delay)
+                    rc, t = srv.submit(hs[i][0], hs[i][1], count=q["kind"] == "aggregate", output_mode=1)
+                    if rc != d.OK:
+                        raise RuntimeError("submit %d: status %d" % (i, rc))
+                    tickets[i] = t
+
+            th = threading.Thread(target=submitter)
+            t_begin = time.perf_counter()
+            th.start()
+            out = []
+            for i in range(len(reqs)):
+                while tickets[i] is None:
+                    time.sleep(0.0005)
+                rc, res, qs, tot = srv.wait(tickets[i])
+                if rc != d.OK:
+                    raise RuntimeError("request %d: status %d" % (i, rc))
+                out.append((res, qs, tot))
+            th.join()
+            makespan = time.perf_counter() - t_begin
+            st = srv.stats()
+        return out, makespan, st
+
+    # service capacity of the plain path: the requests back to back, one at a time
+    _, t_plain_all, _ = replay(1, 1, [0.0] * len(reqs))
+    capacity = len(reqs) / t_plain_all
+    lam = args.load * capacity
+    gaps = np.array([reqs[0]["t"]] + [reqs[i]["t"] - reqs[i - 1]["t"] for i in range(1, len(reqs))])
+    arrivals = list(np.cumsum(gaps / lam))  # unit-rate Poisson times rescaled to rate lam
+    log("plain capacity %.1f req/s (%.2fs back to back); arrival rate %.1f req/s" % (capacity, t_plain_all, lam))
+    served, span_s, st_s = replay(args.workers, 32, arrivals)
+    plain, span_p, _ = replay(1, 1, arrivals)
+    for i, ((a, _, _), (b, _, _)) in enumerate(zip(served, plain)):
+        if a.fp.as_tuple() != b.fp.as_tuple():
+            raise RuntimeError("request %d: served and one-at-a-time results differ" % i)
+    pairs = sum(r.n for r, _, _ in served)
+    lat_s = [1000 * t for _, _, t in served]
+    lat_p = [1000 * t for _, _, t in plain]
+    line = {
+        "metric": METRIC,
+        "value": pairs / span_s,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": 1000 * span_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded Zipf / planted-block requests drawn on the device, resident in HBM)",
+        "config": {"workload": "served stream of join-project (80%) and COUNT (20%) requests", "requests": len(reqs),
+                   "large_requests": n_large, "large_background_tuples": args.large_n,
+                   "large_block": "20000 x 20000 keys at 0.3, added to the background", "tuples": n_tuples,
+                   "output": "fingerprint per request", "parallelism": "single GPU"},
+        "serving": {"arrival_rate_per_s": lam, "load_vs_plain_capacity": args.load,
+                    "plain_capacity_req_per_s": capacity,
+                    "p50_ms": percentile(lat_s, 50), "p99_ms": percentile(lat_s, 99),
+                    "plain_p50_ms": percentile(lat_p, 50), "plain_p99_ms": percentile(lat_p, 99),
+                    "served_req_per_s": len(reqs) / span_s, "plain_req_per_s": len(reqs) / span_p,
+                    "plain_pairs_per_s": pairs / span_p, "workers": args.workers, "max_batch": 32,
+                    "batches": st_s["batches"], "batched_requests": st_s["batched_requests"],
+                    "peak_bytes": st_s["peak_bytes"]},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def batch_cpu_baseline(params, dev, target_s):
     """Reference DIM3_parallel (oracle/_ref) on whole instances, smallest
     first, until target_s of CPU time has been spent."""
@@ -253,13 +377,20 @@ def main():
     ap.add_argument("--edges", type=int, default=4_000_000)
     ap.add_argument("--max-degree", type=int, default=50_000)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--workload", choices=["two_hop", "planted_block", "batch"], default="two_hop",
+    ap.add_argument("--workload", choices=["two_hop", "planted_block", "batch", "serve"], default="two_hop",
                     help="two_hop: the Chung-Lu self-join (default); planted_block: R and S are each a "
                          "--block-keys square block of (attr, key) cells at --block-density (the dense block "
                          "of the served stream's large requests), for the dense-part comparison; batch: "
                          "--instances independent Zipf join-projects in one batched launch against the same "
                          "instances one query at a time")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="serve: arrival rate as a fraction of the plain one-at-a-time capacity")
+    ap.add_argument("--large-n", type=int, default=5_000_000,
+                    help="serve: Zipf background tuples per side of a large request (the full served-stream "
+                         "specification says 50,000,000)")
+    ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--block-keys", type=int, default=20_000)
     ap.add_argument("--block-density", type=float, default=0.3)
     ap.add_argument("--mode", choices=["materialize", "fingerprint"], default="materialize")
@@ -299,6 +430,8 @@ def main():
 
     if args.workload == "batch":
         return batch_bench(args, d, torch)
+    if args.workload == "serve":
+        return serve_bench(args, d, torch)
 
     t0 = time.perf_counter()
     planted = args.workload == "planted_block"

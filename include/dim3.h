@@ -65,7 +65,10 @@ typedef struct dim3_opts {
     int32_t heavy_k;      /* number of heavy join keys for the dense part; 0 = planner */
     int32_t tile_bits;    /* z-tile width of the per-row dedup bitmap; 0 = planner */
     int32_t block;        /* expansion workgroup size (256/512/1024); 0 = planner */
-    uint32_t reserved[4];
+    int32_t dense_core;   /* dense rows / columns: -1 planner, 0 every row and column with a heavy
+                             key, 1 only those with many heavy keys (a dense core such as a
+                             planted block; the other heavy paths stay sparse) */
+    uint32_t reserved[3];
 } dim3_opts;
 
 /* Order-independent digest of a row set, with u(v) = (u32)v, mix64 the
@@ -127,6 +130,13 @@ const char* dim3_last_error(const dim3_ctx* ctx);
 const char* dim3_status_string(int status);
 /* Bytes of device memory the context currently holds. */
 uint64_t dim3_ctx_device_bytes(const dim3_ctx* ctx);
+/* Frees every workspace buffer the context holds (they grow back on the next
+ * query); the context stays usable. */
+int dim3_ctx_release(dim3_ctx* ctx);
+/* Fraction (0, 1] of the device's compute units the context's persistent
+ * expansion grid occupies (default 1), leaving room for other streams'
+ * kernels beside a long query. */
+int dim3_ctx_set_cu_share(dim3_ctx* ctx, double share);
 
 void dim3_opts_default(dim3_opts* o);
 
@@ -192,6 +202,65 @@ int dim3_count_join_paths(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, cons
  * K <= 131072 keeps every entry exact; M < 8388480. */
 int dim3_dense_product(dim3_ctx* ctx, const int8_t* A, uint64_t M, const int8_t* Bt, uint64_t N, uint64_t K,
                        uint32_t* bits, int32_t* counts);
+
+/* ---- serving: a thread-safe request loop over resident relations ----
+ * No counterpart in the reference (a library of blocking calls).  Relations
+ * are copied once into server-owned device memory and named by handles.
+ * Requests are admitted by HBM footprint: one whose estimated workspace can
+ * never fit next to the resident relations is refused at submit
+ * (DIM3_ERR_ADMISSION); others wait until memory and a worker are free.
+ * Each worker owns a context (stream and workspace), so admitted requests
+ * run concurrently; a worker that finds several fingerprint requests of the
+ * same kind and options waiting runs them as one batch
+ * (dim3_batch_join_*_dev).  A request's result is the same as the
+ * single-query call's. */
+typedef struct dim3_server dim3_server;
+
+typedef struct dim3_server_opts {
+    int device;
+    uint64_t hbm_budget_bytes; /* 0 = 90% of the device's free memory */
+    uint32_t workers;          /* concurrent requests (0 = 4) */
+    uint32_t max_batch;        /* requests per batch (0 = 32; 1 = no batching) */
+    uint32_t queue_cap;        /* pending requests before DIM3_ERR_BUSY (0 = 4096) */
+    uint32_t large_share_pct;  /* with 2+ workers, worker 0 is the lane of large requests and its
+                                  expansions occupy this % of the compute units (0 = 75) */
+    uint64_t large_tuples;     /* a request with at least this many input tuples is large (0 = 2e7) */
+} dim3_server_opts;
+
+#define DIM3_REQ_PROJECT 0
+#define DIM3_REQ_COUNT 1
+
+typedef struct dim3_request {
+    uint32_t kind;     /* DIM3_REQ_* */
+    uint32_t reserved;
+    uint64_t r, s;     /* relation handles: R (x, y) and S (z, y) */
+    dim3_opts opts;
+} dim3_request;
+
+typedef struct dim3_server_stats {
+    uint64_t submitted, completed, rejected, batches, batched_requests;
+    uint64_t resident_bytes;   /* relations */
+    uint64_t workspace_bytes;  /* held by the workers' contexts */
+    uint64_t peak_bytes;
+} dim3_server_stats;
+
+void dim3_server_opts_default(dim3_server_opts* o);
+int dim3_server_create(const dim3_server_opts* opts, dim3_server** out);
+/* Waits for the running requests, then frees everything. */
+int dim3_server_destroy(dim3_server* srv);
+/* Copies n (x|z, y) pairs (host or device memory) into the server. */
+int dim3_server_put_relation(dim3_server* srv, const int32_t* pairs, uint64_t n, uint64_t* handle);
+int dim3_server_put_relation_dev(dim3_server* srv, const int32_t* d_pairs, uint64_t n, uint64_t* handle);
+/* Frees a relation once no pending request uses it. */
+int dim3_server_drop_relation(dim3_server* srv, uint64_t handle);
+int dim3_submit(dim3_server* srv, const dim3_request* req, uint64_t* ticket);
+/* *done = 1 when the request has finished. */
+int dim3_poll(dim3_server* srv, uint64_t ticket, int* done);
+/* Blocks until the request finishes; returns its status and result (rows in
+ * host memory for MATERIALIZE, freed with dim3_result_free).  The ticket is
+ * released.  timing (nullable): seconds from submit to start and to end. */
+int dim3_wait(dim3_server* srv, uint64_t ticket, dim3_result* res, double* queued_s, double* total_s);
+int dim3_server_get_stats(dim3_server* srv, dim3_server_stats* st);
 
 /* ---- cost-model calibration (autotuner) ----
  * The planner prices the dense part against the sparse expansion with device

@@ -27,7 +27,7 @@ DENSE_AUTO, DENSE_OFF, DENSE_BITOR, DENSE_MFMA = -1, 0, 1, 2
 class Opts(ctypes.Structure):
     _fields_ = [("output_mode", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("dense_mode", ctypes.c_int32),
                 ("heavy_k", ctypes.c_int32), ("tile_bits", ctypes.c_int32), ("block", ctypes.c_int32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("dense_core", ctypes.c_int32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class Fingerprint(ctypes.Structure):
@@ -75,6 +75,27 @@ class Tuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_uint64), ("workers", ctypes.c_uint32),
+                ("max_batch", ctypes.c_uint32), ("queue_cap", ctypes.c_uint32), ("large_share_pct", ctypes.c_uint32),
+                ("large_tuples", ctypes.c_uint64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("r", ctypes.c_uint64),
+                ("s", ctypes.c_uint64), ("opts", Opts)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("batched_requests", ctypes.c_uint64),
+                ("resident_bytes", ctypes.c_uint64), ("workspace_bytes", ctypes.c_uint64),
+                ("peak_bytes", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 COUNT_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("count", "<i8")])
 
 # every symbol include/dim3.h declares (checked by the ABI test)
@@ -83,7 +104,9 @@ EXPORTS = [
     "dim3_opts_default", "dim3_join_project", "dim3_join_aggregate_count", "dim3_result_free",
     "dim3_join_project_dev", "dim3_join_aggregate_count_dev", "dim3_count_join_paths", "dim3_dense_product",
     "dim3_batch_join_project_dev", "dim3_batch_join_aggregate_count_dev", "dim3_autotune", "dim3_get_tuning",
-    "dim3_set_tuning",
+    "dim3_set_tuning", "dim3_ctx_release", "dim3_ctx_set_cu_share", "dim3_server_opts_default", "dim3_server_create", "dim3_server_destroy",
+    "dim3_server_put_relation", "dim3_server_put_relation_dev", "dim3_server_drop_relation", "dim3_submit",
+    "dim3_poll", "dim3_wait", "dim3_server_get_stats",
     "dim3_version",
 ]
 
@@ -148,6 +171,30 @@ def lib():
     L.dim3_get_tuning.restype = i32
     L.dim3_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.dim3_set_tuning.restype = i32
+    L.dim3_ctx_set_cu_share.argtypes = [vp, ctypes.c_double]
+    L.dim3_ctx_set_cu_share.restype = i32
+    L.dim3_ctx_release.argtypes = [vp]
+    L.dim3_ctx_release.restype = i32
+    L.dim3_server_opts_default.argtypes = [ctypes.POINTER(ServerOpts)]
+    L.dim3_server_opts_default.restype = None
+    L.dim3_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]
+    L.dim3_server_create.restype = i32
+    L.dim3_server_destroy.argtypes = [vp]
+    L.dim3_server_destroy.restype = i32
+    for f in (L.dim3_server_put_relation, L.dim3_server_put_relation_dev):
+        f.argtypes = [vp, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
+        f.restype = i32
+    L.dim3_server_drop_relation.argtypes = [vp, u64]
+    L.dim3_server_drop_relation.restype = i32
+    L.dim3_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    L.dim3_submit.restype = i32
+    L.dim3_poll.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int)]
+    L.dim3_poll.restype = i32
+    L.dim3_wait.argtypes = [vp, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double)]
+    L.dim3_wait.restype = i32
+    L.dim3_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.dim3_server_get_stats.restype = i32
     L.dim3_dense_product.argtypes = [vp, vp, u64, vp, u64, u64, vp, vp]
     L.dim3_dense_product.restype = i32
     L.dim3_version.argtypes = []
@@ -164,7 +211,7 @@ def status_string(s):
 
 
 def make_opts(output_mode=OUT_MATERIALIZE, no_swap=False, exclude_equal=False, dense_mode=DENSE_AUTO, heavy_k=0,
-              tile_bits=0, block=0, hash_only=False):
+              tile_bits=0, block=0, hash_only=False, dense_core=-1):
     o = Opts()
     lib().dim3_opts_default(ctypes.byref(o))
     o.output_mode = output_mode
@@ -174,6 +221,7 @@ def make_opts(output_mode=OUT_MATERIALIZE, no_swap=False, exclude_equal=False, d
     o.heavy_k = heavy_k
     o.tile_bits = tile_bits
     o.block = block
+    o.dense_core = dense_core
     return o
 
 
@@ -379,6 +427,102 @@ class Context:
                                         None if ob is None else _ptr(ob), None if oc is None else _ptr(oc))
         self._check(rc, "dim3_dense_product")
         return ob, oc
+
+
+class Server:
+    """The libdim3 request loop (dim3_server_*): resident relations,
+    admission by HBM footprint, concurrent workers, continuous batching."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, workers=4, max_batch=32, queue_cap=4096):
+        L = lib()
+        o = ServerOpts()
+        L.dim3_server_opts_default(ctypes.byref(o))
+        o.device, o.hbm_budget_bytes, o.workers, o.max_batch, o.queue_cap = (device, hbm_budget_bytes, workers,
+                                                                              max_batch, queue_cap)
+        h = ctypes.c_void_p()
+        rc = L.dim3_server_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != OK:
+            raise Dim3Error(rc, "dim3_server_create")
+        self._h, self._L = h, L
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self._L.dim3_server_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def put(self, pairs):
+        """Copy a host (n, 2) int32 relation into the server; returns its handle."""
+        a = _pairs(pairs)
+        h = ctypes.c_uint64()
+        rc = self._L.dim3_server_put_relation(self._h, _ptr(a), a.shape[0], ctypes.byref(h))
+        if rc != OK:
+            raise Dim3Error(rc, "dim3_server_put_relation")
+        return int(h.value)
+
+    def put_dev(self, d_ptr, n):
+        h = ctypes.c_uint64()
+        rc = self._L.dim3_server_put_relation_dev(self._h, ctypes.c_void_p(d_ptr), n, ctypes.byref(h))
+        if rc != OK:
+            raise Dim3Error(rc, "dim3_server_put_relation_dev")
+        return int(h.value)
+
+    def drop(self, handle):
+        rc = self._L.dim3_server_drop_relation(self._h, handle)
+        if rc != OK:
+            raise Dim3Error(rc, "dim3_server_drop_relation")
+
+    def submit(self, r, s, count=False, **kw):
+        """Returns (status, ticket); status OK, ERR_BUSY or ERR_ADMISSION."""
+        req = Request()
+        req.kind = 1 if count else 0
+        req.r, req.s = r, s
+        req.opts = make_opts(**kw)
+        t = ctypes.c_uint64()
+        rc = self._L.dim3_submit(self._h, ctypes.byref(req), ctypes.byref(t))
+        if rc not in (OK, ERR_BUSY, ERR_ADMISSION):
+            raise Dim3Error(rc, "dim3_submit")
+        return rc, int(t.value)
+
+    def poll(self, ticket):
+        d = ctypes.c_int()
+        rc = self._L.dim3_poll(self._h, ticket, ctypes.byref(d))
+        if rc != OK:
+            raise Dim3Error(rc, "dim3_poll")
+        return bool(d.value)
+
+    def wait(self, ticket):
+        """Returns (status, QueryResult, queued seconds, total seconds)."""
+        res = Result()
+        q, t = ctypes.c_double(), ctypes.c_double()
+        rc = self._L.dim3_wait(self._h, ticket, ctypes.byref(res), ctypes.byref(q), ctypes.byref(t))
+        if rc == ERR_NOT_FOUND:
+            raise Dim3Error(rc, "dim3_wait")
+        pairs = rows = None
+        n = int(res.n)
+        if rc == OK and res.pairs:
+            pairs = np.frombuffer((ctypes.c_int32 * (2 * n)).from_address(res.pairs), dtype=np.int32).reshape(n, 2).copy()
+        if rc == OK and res.rows:
+            rows = np.frombuffer((CountRow * n).from_address(res.rows), dtype=COUNT_ROW_DTYPE).copy()
+        out = QueryResult(res, pairs=pairs, rows=rows)
+        self._L.dim3_result_free(ctypes.byref(res))
+        return rc, out, q.value, t.value
+
+    def stats(self):
+        st = ServerStats()
+        self._L.dim3_server_get_stats(self._h, ctypes.byref(st))
+        return st.as_dict()
 
 
 def version():

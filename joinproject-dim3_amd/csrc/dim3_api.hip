@@ -63,6 +63,10 @@ enum SlotId {
     WS_DHIST,
     WS_HID,
     WS_H2Y,
+    WS_HDX,
+    WS_HDZ,
+    WS_CNT_SD,
+    WS_CUR_SD,
     WS_HX,
     WS_HX2X,
     WS_ZFLAG,
@@ -121,6 +125,13 @@ struct QueryPlan {
     uint32_t bmin = 0;           // heaviness bin threshold
     uint32_t K = 0, Kp = 0, Mh = 0, Nh = 0, dwords = 0;
     bool all_z = false;  // every z ID is a product column
+    // dense core: rows / columns with at least thx / thz heavy entries
+    bool core_try = false;     // analyse the core of the keys with bin >= bc
+    uint32_t bc = 0;
+    uint64_t core_K = 0, core_paths = 0;
+    double p1_gain = -1e300;   // saving of the row-and-column-complete choice (phase 1)
+    bool core = false;         // rows outside the core keep heavy entries (split S rows)
+    uint32_t thx = 1, thz = 1;
 };
 
 // One query's device workspace and plan.  A context holds one bank per
@@ -148,12 +159,15 @@ struct DenseModel {
 struct dim3_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t qstream = nullptr;  // stream of the call in progress (workspace allocations are ordered on it)
     uint64_t budget = 0;
     uint64_t held = 0;
     std::vector<std::unique_ptr<Bank>> banks;
     uint32_t cur = 0;  // bank of the query being prepared / expanded
     std::string err;
     int cu_count = 0;
+    double cu_share = 1.0;  // fraction of the compute units the expansion grid may occupy
+    int grid_cus() const { return std::max(1, (int)(cu_count * cu_share)); }
     size_t max_lds = 0;
     Scalars* h_sc = nullptr;  // pinned mirror of the device scalars
     unsigned long long* h_hist = nullptr;  // pinned mirror of the dense-split histogram
@@ -208,15 +222,16 @@ int ws_get(dim3_ctx* c, SlotId id, size_t bytes, void** out) {
     if (c->budget && need > c->budget)
         return fail(c, DIM3_ERR_OOM, "workspace slot %d needs %zu bytes: over the %llu-byte budget", (int)id, bytes,
                     (unsigned long long)c->budget);
+    // stream-ordered: no device-wide synchronisation, so other contexts'
+    // queries keep running while this one grows its workspace
     if (s.p) {
-        hipStreamSynchronize(c->stream);
-        hipFree(s.p);
+        hipFreeAsync(s.p, c->qstream);
         c->held -= s.bytes;
         s.p = nullptr;
         s.bytes = 0;
     }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMallocAsync(&p, bytes, c->qstream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(c, DIM3_ERR_OOM, "hipMalloc(%zu) for workspace slot %d: %s", bytes, (int)id, hipGetErrorString(e));
@@ -403,6 +418,29 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
             best_shape = acc;
         }
     }
+    // dense-core candidate: the keys whose R and S degrees are both >= 64, when
+    // there are many of them (a block of keys rather than a few hubs)
+    // (large inputs only: the analysis costs a host round trip)
+    const bool core_ok = o.dense_core == 1 || (o.dense_core == -1 && P.nR + P.nS >= 8000000ull);
+    if (core_ok && !(count && o.dense_mode == DIM3_DENSE_BITOR)) {
+        uint64_t k = 0, pth = 0;
+        uint32_t bc = 0;
+        for (int b = (int)NB - 1; b >= 1; b--) {
+            if (dim3h::dense_bin_lo((uint32_t)b) < 64 && o.dense_core != 1) break;
+            if (dim3h::dense_bin_lo((uint32_t)b) < 2) break;
+            k += h[b];
+            pth += h[NB + b];
+            if (h[b]) bc = (uint32_t)b;
+            if (o.dense_core != 1 && k >= 256 && dim3h::dense_bin_lo((uint32_t)b) <= 1024) break;
+        }
+        if (bc && (k >= 256 || o.dense_core == 1)) {
+            P.core_try = true;
+            P.bc = bc;
+            P.core_K = k;
+            P.core_paths = pth;
+        }
+    }
+    P.p1_gain = best_mode == DIM3_DENSE_OFF ? -1e300 : best_gain;
     if (best_mode == DIM3_DENSE_OFF) return;
     if (o.dense_mode == DIM3_DENSE_AUTO && o.heavy_k == 0 && best_gain <= 0) return;
     P.dense = best_mode;
@@ -416,6 +454,75 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     P.all_z = !P.count_mode && (uint64_t)P.Nh * 10 >= (uint64_t)P.nz * 9;
     if (P.all_z) P.Nh = P.nz;
     P.dwords = (P.Nh + 31u) / 32u;
+}
+
+// Dense core of the keys with bin >= P.bc, from the heavy-degree histogram
+// in c->h_hist ([4][bins]: rows, their heavy entries, columns, their heavy
+// tuples): rows with >= thx heavy entries times columns with >= thz heavy
+// tuples.  The paths kept inside the core are estimated as the key paths
+// times the fractions of heavy entries and of heavy tuples inside it.
+// Returns true when the core (or, forced, any core) beats the phase-1 plan.
+bool plan_core(dim3_ctx* c, const dim3_opts& o) {
+    auto& P = c->bank().plan;
+    const unsigned long long* h = c->h_hist;
+    const uint32_t NB = dim3h::kDenseBins;
+    const DenseModel& m = c->model;
+    const uint64_t room = c->budget > c->held ? (c->budget - c->held) / 3 : 0;
+    const bool count = P.count_mode;
+    std::vector<uint64_t> xd(NB + 1, 0), sx(NB + 1, 0), zd(NB + 1, 0), sz(NB + 1, 0);
+    for (int b = (int)NB - 1; b >= 0; b--) {
+        xd[b] = xd[b + 1] + h[b];
+        sx[b] = sx[b + 1] + h[NB + b];
+        zd[b] = zd[b + 1] + h[2 * NB + b];
+        sz[b] = sz[b + 1] + h[3 * NB + b];
+    }
+    if (sx[1] == 0 || sz[1] == 0) return false;
+    double best = -1e300;
+    int bm = DIM3_DENSE_OFF;
+    uint32_t bxb = 1, bzb = 1;
+    for (uint32_t bx = 1; bx < NB; bx++) {
+        if (xd[bx] == 0) break;
+        for (uint32_t bz = 1; bz < NB; bz++) {
+            if (zd[bz] == 0) break;
+            if (o.dense_core == 1 && bx == 1) continue;  // forced: a real core of rows
+            DenseShape d;
+            d.K = P.core_K;
+            d.Mh = xd[bx];
+            d.Nh = zd[bz];
+            d.paths = (uint64_t)((double)P.core_paths * ((double)sx[bx] / (double)sx[1]) *
+                                 ((double)sz[bz] / (double)sz[1]));
+            d.sumR = sx[bx];
+            d.nz = P.nz;
+            const int modes[2] = {DIM3_DENSE_MFMA, DIM3_DENSE_BITOR};
+            for (int mode : modes) {
+                if (o.dense_mode > 0 && o.dense_mode != mode) continue;
+                const double t = dense_cost(m, d, mode, count, room, nullptr);
+                if (t < 0) continue;
+                const double gain = (double)d.paths * m.path_s - t;
+                if (gain > best) {
+                    best = gain;
+                    bm = mode;
+                    bxb = bx;
+                    bzb = bz;
+                }
+            }
+        }
+    }
+    if (bm == DIM3_DENSE_OFF) return false;
+    if (o.dense_core != 1 && (best <= P.p1_gain || (o.dense_mode == DIM3_DENSE_AUTO && best <= 0))) return false;
+    P.dense = bm;
+    P.bmin = P.bc;
+    P.K = (uint32_t)P.core_K;
+    P.Kp = (P.K + 63u) / 64u * 64u;
+    P.Mh = (uint32_t)xd[bxb];
+    P.Nh = (uint32_t)zd[bzb];
+    P.thx = (uint32_t)dim3h::dense_bin_lo(bxb);
+    P.thz = (uint32_t)dim3h::dense_bin_lo(bzb);
+    P.core = P.thx > 1;
+    P.all_z = !count && (uint64_t)P.Nh * 10 >= (uint64_t)P.nz * 9;
+    if (P.all_z) P.Nh = P.nz;
+    P.dwords = (P.Nh + 31u) / 32u;
+    return true;
 }
 
 // Mapping + CSR of one query.  Leaves the plan in c->bank().plan.
@@ -558,7 +665,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     P.empty = false;
     plan_tiles(c, o, force_wgs);
     plan_dense(c, o);
-    if ((uint64_t)P.ny * P.ntiles >= (1ull << 32))
+    if ((uint64_t)P.ny * P.ntiles >= (1ull << 31))
         return fail(c, DIM3_ERR_INVALID, "tiled S index does not fit 32 bits (ny=%u, tiles=%u)", P.ny, P.ntiles);
 
     HIPCHK(c, hipEventRecord(c->ev[2], st));
@@ -581,35 +688,69 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
 
     int32_t *hid = nullptr, *hx = nullptr;
     uint32_t *hx2x = nullptr, *h2y = nullptr, *zflag = nullptr, *hzpos = nullptr, *hz2zid = nullptr,
-             *hz_tile = nullptr;
-    if (P.dense != DIM3_DENSE_OFF) {
-        // heavy keys, product rows and columns (columns numbered in z-ID order)
+             *hz_tile = nullptr, *hdx = nullptr, *hdz = nullptr;
+    uint32_t* counters = (uint32_t*)&sc->spare[0];
+    if (P.dense != DIM3_DENSE_OFF || P.core_try) {
+        // heavy keys (of the core candidate when there is one)
         TRY(ws(c, WS_HID, P.ny, &hid));
+        TRY(ws(c, WS_H2Y, std::max<uint64_t>(P.core_try ? P.core_K : 0, P.K), &h2y));
+        dim3h::dense_mark_keys(degR, degS, (uint32_t)ny1, exY, P.core_try ? P.bc : P.bmin, hid, h2y, counters, st);
+        if (P.core_try) {
+            unsigned long long* hh;
+            TRY(ws(c, WS_HDX, nx1, &hdx));
+            TRY(ws(c, WS_HDZ, nz1, &hdz));
+            TRY(ws(c, WS_DHIST, 6ull * dim3h::kDenseBins, &hh));
+            dim3h::fill_u32(hdx, nx1, 0, st);
+            dim3h::fill_u32(hdz, nz1, 0, st);
+            dim3h::fill_u64((uint64_t*)hh, 4ull * dim3h::kDenseBins, 0, st);
+            dim3h::dense_heavy_degrees(slotR, slotRx, nR, dS, slotS, P.z_ident ? nullptr : slotSz, P.zlo, nS, exY,
+                                       hid, hdx, (uint32_t)nx1, hdz, (uint32_t)nz1, hh, st);
+            HIPCHK(c, hipMemcpyAsync(c->h_hist, hh, 4ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipStreamSynchronize(st));
+            const bool core = plan_core(c, o);
+            if (!core && P.dense != DIM3_DENSE_OFF) {  // back to the phase-1 keys
+                HIPCHK(c, hipMemsetAsync(counters, 0, 8, st));
+                dim3h::dense_mark_keys(degR, degS, (uint32_t)ny1, exY, P.bmin, hid, h2y, counters, st);
+            }
+        }
+    }
+    if (P.dense != DIM3_DENSE_OFF) {
+        // product rows and columns (columns numbered in z-ID order)
         TRY(ws(c, WS_HX, P.nx, &hx));
         TRY(ws(c, WS_HX2X, P.Mh, &hx2x));
-        TRY(ws(c, WS_H2Y, P.K, &h2y));
         TRY(ws(c, WS_ZFLAG, P.nz, &zflag));
         TRY(ws(c, WS_HZPOS, (uint64_t)P.nz + 1, &hzpos));
         TRY(ws(c, WS_HZ2ZID, P.Nh, &hz2zid));
         TRY(ws(c, WS_HZTILE, (uint64_t)P.ntiles + 1, &hz_tile));
-        dim3h::dense_mark(degR, degS, (uint32_t)ny1, exY, degX, exX, (uint32_t)nx1, xw, P.z_ident ? nullptr : degZ,
-                          P.z_ident ? nullptr : exZ, (uint32_t)nz1, zw, P.bmin, hid, hx, hx2x, h2y, zflag, P.all_z,
-                          (uint32_t*)&sc->spare[0], st);
+        const bool deg_rule = P.thx > 1 || P.thz > 1;
+        dim3h::dense_mark_rows_cols(degX, exX, (uint32_t)nx1, xw, P.z_ident ? nullptr : degZ, P.z_ident ? nullptr : exZ,
+                                    (uint32_t)nz1, zw, P.bmin, deg_rule ? hdx : nullptr, P.thx,
+                                    deg_rule ? hdz : nullptr, P.thz, P.all_z, hx, hx2x, zflag, counters, st);
         dim3h::scan_u32(zflag, P.nz, hzpos, part, &sc->nzh, st);
         dim3h::dense_columns(zflag, hzpos, P.nz, hz2zid, P.ntiles, P.tile_size, P.Nh, hz_tile, st);
+    } else {
+        hid = nullptr;
     }
     dim3h::compact_ids(P.mx, degX, exX, id2x, rowR, st);
     dim3h::finish_rowptr(rowR, P.nx, &sc->tot_x, st);
     if (!P.z_ident) dim3h::compact_ids(P.mz, degZ, exZ, id2z, nullptr, st);
     HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
-    dim3h::scatter_r(slotR, slotRx, nR, exY, exX, hid, curR, colR, st);
+    dim3h::scatter_r(slotR, slotRx, nR, exY, exX, hid, hx, curR, colR, st);
     dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
-    dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, hid, cntS, keyS,
-                   zidS, st);
+    uint32_t *cntSD = nullptr, *curSD = nullptr;
+    if (P.core) {
+        TRY(ws(c, WS_CNT_SD, nkeyS + 1, &cntSD));
+        TRY(ws(c, WS_CUR_SD, nkeyS, &curSD));
+        dim3h::fill_u32(cntSD, nkeyS + 1, 0, st);
+    }
+    dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, hid, zflag,
+                   cntS, cntSD, keyS, zidS, st);
+
     dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
     dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
-    HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
-    dim3h::scatter_s(keyS, zidS, nS, curS, colS, st);
+    if (P.core) dim3h::split_cursors(rowS, cntSD, (uint32_t)nkeyS, curS, curSD, st);
+    else HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
+    dim3h::scatter_s(keyS, zidS, nS, curS, curSD, colS, st);
     if (o.flags & DIM3_OPT_EXCLUDE_EQUAL)
         dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
     LAUNCHCHK(c, "csr");
@@ -630,15 +771,15 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
             dim3h::fill_u32(B, (uint64_t)P.Nh * P.Kp / 4, 0, st);
             dim3h::dense_build_a(P.count_mode, rowR, colR, hid, hx2x, P.Mh, P.Kp, A, overflow, st);
             dim3h::dense_build_b(P.count_mode ? 1 : 0, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY,
-                                 hid, hzpos, P.Kp, P.dwords, B, overflow, st);
+                                 hid, zflag, hzpos, P.Kp, P.dwords, B, overflow, st);
             const int e = dim3h::gemm_i8((const int8_t*)A, (const int8_t*)B, P.Mh, P.Nh, P.Kp, dbits, P.dwords, cint,
                                          st);
             if (e != 0) return fail(c, DIM3_ERR_DEVICE, "dense product launch: %s", hipGetErrorString((hipError_t)e));
         } else {
             TRY(ws(c, WS_DB, (uint64_t)P.K * P.dwords, &B));
             dim3h::fill_u32(B, (uint64_t)P.K * P.dwords, 0, st);
-            dim3h::dense_build_b(2, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY, hid, hzpos, 0,
-                                 P.dwords, B, overflow, st);
+            dim3h::dense_build_b(2, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY, hid, zflag, hzpos,
+                                 0, P.dwords, B, overflow, st);
             const int e = dim3h::bitor_rows(rowR, colR, hid, hx2x, P.Mh, B, P.dwords, dbits, st);
             if (e != 0) return fail(c, DIM3_ERR_DEVICE, "dense bit-or launch: %s", hipGetErrorString((hipError_t)e));
         }
@@ -689,6 +830,7 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
         a.gw = (const ulonglong2*)W[WS_FP_GW].p;
         a.gwm = (const uint32_t*)W[WS_FP_GWM].p;
     }
+    if (P.core) a.splitS = (const uint32_t*)W[WS_CNT_SD].p;
     if (P.dense != DIM3_DENSE_OFF) {
         a.hx = (const int32_t*)W[WS_HX].p;
         if (P.count_mode) a.cint = (const int32_t*)W[WS_CINT].p;
@@ -715,8 +857,8 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
     }
     const dim3h::ExpandArgs a = make_args(c, fp_mode, out, cap);
     HIPCHK(c, hipEventRecord(c->ev[4], st));
-    int e = P.count_mode ? dim3h::expand_count(a, P.block, c->cu_count, st)
-                         : dim3h::expand_project(a, P.block, c->cu_count, st);
+    int e = P.count_mode ? dim3h::expand_count(a, P.block, c->grid_cus(), st)
+                         : dim3h::expand_project(a, P.block, c->grid_cus(), st);
     if (e != 0) return fail(c, DIM3_ERR_DEVICE, "expansion launch failed: %s", hipGetErrorString((hipError_t)e));
     HIPCHK(c, hipEventRecord(c->ev[5], st));
     HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -794,6 +936,7 @@ int run_dev(dim3_ctx* c, bool count_mode, const int32_t* d_R, uint64_t nR, const
     if (!fp && cap && !d_out) return fail(c, DIM3_ERR_INVALID, "null output buffer");
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    c->qstream = st;
     if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
         return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
     TRY(prepare(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st));
@@ -822,6 +965,7 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
     const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
+    c->qstream = st;
     int2 *dR, *dS;
     TRY(ws(c, WS_IN_R, std::max<uint64_t>(nR, 1), &dR));
     TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
@@ -894,6 +1038,7 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     }
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    c->qstream = st;
     struct Reset {  // leave the context on bank 0 whatever happens
         dim3_ctx* c;
         ~Reset() { c->use_bank(0); }
@@ -937,8 +1082,9 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
         if (c->h_batch) hipHostFree(c->h_batch);
         c->h_batch = nullptr;
         c->h_batch_cap = 0;
-        HIPCHK(c, hipHostMalloc((void**)&c->h_batch, sizeof(Scalars) * n, hipHostMallocDefault));
-        c->h_batch_cap = n;
+        const uint32_t cap = std::max<uint32_t>(n, 512);
+        HIPCHK(c, hipHostMalloc((void**)&c->h_batch, sizeof(Scalars) * cap, hipHostMallocDefault));
+        c->h_batch_cap = cap;
     }
     HIPCHK(c, hipMemcpyAsync(d_args, args.data(), sizeof(dim3h::ExpandArgs) * n, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(d_start, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
@@ -946,7 +1092,7 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     HIPCHK(c, hipStreamSynchronize(st));  // host vectors are pageable
     if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block);
     unsigned grid = 1;
-    int e = dim3h::expand_batch_setup(count_mode, block, lds, c->cu_count, std::max<uint64_t>(total, 1), &grid);
+    int e = dim3h::expand_batch_setup(count_mode, block, lds, c->grid_cus(), std::max<uint64_t>(total, 1), &grid);
     if (e != 0) return fail(c, DIM3_ERR_DEVICE, "batch setup: %s", hipGetErrorString((hipError_t)e));
 
     hipGraph_t graph = nullptr;
@@ -1022,6 +1168,7 @@ float elapsed(dim3_ctx* c, int a, int b) {
 int tune_query(dim3_ctx* c, uint32_t X, uint32_t Z, uint32_t Y, int dense, float* ms_dense, float* ms_expand,
                QueryPlan* plan) {
     hipStream_t st = c->stream;
+    c->qstream = st;
     int2 *R, *S;
     TRY(ws(c, WS_IN_R, (uint64_t)X * Y, &R));
     TRY(ws(c, WS_IN_S, (uint64_t)Z * Y, &S));
@@ -1064,6 +1211,7 @@ int autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
         }
     }
     hipStream_t st = c->stream;
+    c->qstream = st;
     DenseModel m = c->model;
     // streaming writes: 256 MiB fill
     {
@@ -1215,6 +1363,7 @@ int dim3_ctx_create(int device, uint64_t hbm_budget_bytes, dim3_ctx** out) {
     }
     memset(c->h_sc, 0, sizeof(Scalars));
     for (auto& e : c->ev) hipEventCreate(&e);
+    c->qstream = c->stream;
     *out = c;
     return DIM3_OK;
 }
@@ -1239,6 +1388,33 @@ int dim3_ctx_destroy(dim3_ctx* c) {
 const char* dim3_last_error(const dim3_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 uint64_t dim3_ctx_device_bytes(const dim3_ctx* c) { return c ? c->held : 0; }
+
+int dim3_ctx_set_cu_share(dim3_ctx* c, double share) {
+    if (!c || !(share > 0.0) || share > 1.0) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->cu_share = share;
+    return DIM3_OK;
+}
+
+int dim3_ctx_release(dim3_ctx* c) {
+    if (!c) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& b : c->banks)
+        for (auto& s : b->ws)
+            if (s.p) {
+                hipFreeAsync(s.p, c->stream);
+                c->held -= s.bytes;
+                s.p = nullptr;
+                s.bytes = 0;
+            }
+    c->banks.resize(1);
+    c->banks[0]->plan = QueryPlan();
+    c->cur = 0;
+    hipStreamSynchronize(c->stream);
+    return DIM3_OK;
+}
 
 int dim3_join_project(dim3_ctx* c, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
                       const dim3_opts* opts, dim3_result* res) {
@@ -1290,6 +1466,7 @@ int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const i
     if (!R_xy || !S_zy) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
+    c->qstream = st;
     int2 *dR, *dS;
     TRY(ws(c, WS_IN_R, nR, &dR));
     TRY(ws(c, WS_IN_S, nS, &dS));
@@ -1332,6 +1509,7 @@ int dim3_dense_product(dim3_ctx* c, const int8_t* A, uint64_t M, const int8_t* B
                     (unsigned long long)M, (unsigned long long)N, (unsigned long long)K);
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
+    c->qstream = st;
     const uint64_t Kp = std::max<uint64_t>((K + 63) / 64 * 64, 64);
     const uint64_t dwords = (N + 31) / 32;
     uint32_t *dA, *dB, *dbits = nullptr;

@@ -175,6 +175,25 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* ctr, uint32_t key, bo
     return pos;
 }
 
+// As wave_count_add for a running maximum: the lanes sharing the first
+// pending lane's key (two rounds) send one atomicMax of their largest value.
+__device__ __forceinline__ void wave_max_update(uint32_t* arr, uint32_t key, uint32_t v, bool valid) {
+    uint64_t pending = __ballot(valid);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        if (pending == 0) return;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)leader);
+        const bool mine = valid && key == lk && ((pending >> lane_id()) & 1ull);
+        uint32_t m = mine ? v : 0u;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
+        if (lane_id() == leader && m > arr[lk]) atomicMax(&arr[lk], m);
+        pending &= ~__ballot(mine);
+    }
+    if (((pending >> lane_id()) & 1ull) && v > arr[key]) atomicMax(&arr[key], v);
+}
+
 // Exclusive scan over the block; sh needs BLOCK/64+1 entries.  Ends with a
 // barrier so sh can be reused right away.
 template <typename T>
@@ -309,18 +328,24 @@ void scan_deg(const uint32_t* deg, uint32_t nslots, uint64_t* ex, uint64_t* part
 void compact_ids(const KeyMap& m, const uint32_t* deg, const uint64_t* ex, int32_t* id2v, uint32_t* rowptr,
                  hipStream_t st);
 void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_t st);
-// CSR of R by x: col = y id, with bit 31 set for heavy keys (hid[y] >= 0; hid nullable)
+// CSR of R by x: col = y id, with bit 31 set for the heavy keys of product
+// rows (hid[y] >= 0 and hx[x] >= 0; hid nullable)
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
-               const uint64_t* ex_x, const int32_t* hid, uint32_t* cursor, int32_t* col, hipStream_t st);
+               const uint64_t* ex_x, const int32_t* hid, const int32_t* hx, uint32_t* cursor, int32_t* col,
+               hipStream_t st);
 // tiled CSR of S by (y, z-tile); ex_z == nullptr means z identity (zid = z - zlo)
-// hid (nullable): heavy index per y ID; tuples of heavy keys are left out (dense part)
+// hid (nullable): heavy index per y ID; a heavy key's tuple with a product
+// column (zflag) is left out, or, with cntD (dense core: rows outside the
+// core still need it), counted in cntD and placed at the front of its row
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
              const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
-             uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st);
+             const uint32_t* zflag, uint32_t* cnt, uint32_t* cntD, uint32_t* keyS, int32_t* zidS, hipStream_t st);
+void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uint32_t* curL, uint32_t* curD,
+                   hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st);
-void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, int32_t* col,
-               hipStream_t st);
+void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, uint32_t* cursorD,
+               int32_t* col, hipStream_t st);
 void x2z_self(const int32_t* id2x, uint32_t nx, const KeyMap& mz, const uint32_t* degZ, const uint64_t* ex_z,
               int32_t* x2z, hipStream_t st);
 
@@ -346,6 +371,7 @@ struct ExpandArgs {
     uint64_t cap;
     // dense seed (hx == nullptr: no dense part)
     const int32_t* hx;        // product row of each x ID, -1 = none
+    const uint32_t* splitS;   // dense core: product-column tuples at the front of each S row (skipped by core rows)
     const uint32_t* dbits;    // project: Mh x dwords bit rows over hz
     const int32_t* cint;      // COUNT: Mh x Nh counts
     const uint32_t* hz2zid;   // z ID of product column hz
@@ -370,12 +396,22 @@ void dense_weights(const int2* R, uint64_t nR, const uint32_t* slotR, const int3
 // hist[6][kDenseBins]: keys, paths, sum degR, sum degS, x rows, z columns per heaviness bin
 void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint32_t* xw, uint32_t nx1,
                 const uint32_t* zw, uint32_t nz1, unsigned long long* hist, hipStream_t st);
-// heavy index per y ID, product row per x ID (+ inverse), column flag per z ID;
-// counters[0] = keys, counters[1] = rows
-void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
-                const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
-                const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
-                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, bool all_z, uint32_t* counters, hipStream_t st);
+// heavy index per y ID (+ inverse); counters[0] = keys
+void dense_mark_keys(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y, uint32_t bmin,
+                     int32_t* hid, uint32_t* h2y, uint32_t* counters, hipStream_t st);
+// heavy entries per x slot (hdx) and heavy tuples per z slot (hdz), and
+// hist[4][kDenseBins] of rows / entries / columns / tuples by heavy-degree bin
+void dense_heavy_degrees(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const int2* S,
+                         const int32_t* slotS, const int32_t* slotSz, int32_t zlo, uint64_t nS, const uint64_t* ex_y,
+                         const int32_t* hid, uint32_t* hdx, uint32_t nx1, uint32_t* hdz, uint32_t nz1,
+                         unsigned long long* hist, hipStream_t st);
+// product row per x ID (+ inverse) and column flag per z ID: by heaviness
+// bin (hdx / hdz null) or by heavy degree >= thx / thz (the dense core);
+// counters[1] = rows
+void dense_mark_rows_cols(const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw,
+                          const uint32_t* degZ, const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin,
+                          const uint32_t* hdx, uint32_t thx, const uint32_t* hdz, uint32_t thz, bool all_z,
+                          int32_t* hx, uint32_t* hx2x, uint32_t* zflag, uint32_t* counters, hipStream_t st);
 void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, uint32_t* hz2zid, uint32_t ntiles,
                    uint32_t tile_size, uint32_t Nh, uint32_t* hz_tile, hipStream_t st);
 // A rows from the R CSR (heavy entries carry bit 31); Kp bytes per row
@@ -384,8 +420,9 @@ void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, con
 // from the heavy S tuples: kind 0: Bt [Nh][Kp] 0/1 bytes, 1: Bt byte
 // multiplicities, 2: bits of per-key z bitmaps [K][dwords]
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
-                   int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* hzpos,
-                   uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow, hipStream_t st);
+                   int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* zflag,
+                   const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
+                   hipStream_t st);
 
 // fingerprint word totals of one project plan: gw / gwm[ntiles * tile_words]
 void fp_tables(const int32_t* id2z, int32_t zlo, uint32_t nz, uint32_t ntiles, uint32_t tile_words,

@@ -167,9 +167,8 @@ __global__ void k_wmax_x(const uint32_t* __restrict__ slotR, const int32_t* __re
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
     const int32_t sx = slotRx[i];
-    if (sx < 0) return;
-    const uint32_t w = key_w(degR, degS, slotR[i]);
-    if (w > xw[sx]) atomicMax(&xw[sx], w);
+    const uint32_t w = sx >= 0 ? key_w(degR, degS, slotR[i]) : 0u;
+    wave_max_update(xw, (uint32_t)sx, w, sx >= 0);
 }
 
 __global__ void k_wmax_z(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
@@ -178,10 +177,9 @@ __global__ void k_wmax_z(const int2* __restrict__ S, const int32_t* __restrict__
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const int32_t sy = slotS[i];
-    if (sy < 0) return;
-    const uint32_t zs = slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i];
-    const uint32_t w = key_w(degR, degS, (uint32_t)sy);
-    if (w > zw[zs]) atomicMax(&zw[zs], w);
+    const uint32_t zs = sy < 0 ? 0u : (slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i]);
+    const uint32_t w = sy >= 0 ? key_w(degR, degS, (uint32_t)sy) : 0u;
+    wave_max_update(zw, zs, w, sy >= 0);
 }
 
 // hist layout: [6][kDenseBins] u64 = keys, paths, sum degR, sum degS, x rows, z columns
@@ -231,14 +229,16 @@ __global__ void k_mark_hid(const uint32_t* __restrict__ degR, const uint32_t* __
 }
 
 // row index hx of every x ID (-1 = no heavy key) and its inverse
+// (hdx != nullptr: a row needs at least thx heavy entries, the dense core)
 __global__ void k_mark_hx(const uint32_t* __restrict__ degX, const uint64_t* __restrict__ ex_x, uint32_t nx1,
-                          const uint32_t* __restrict__ xw, uint32_t bmin, int32_t* __restrict__ hx,
-                          uint32_t* __restrict__ hx2x, uint32_t* mcount) {
+                          const uint32_t* __restrict__ xw, uint32_t bmin, const uint32_t* __restrict__ hdx,
+                          uint32_t thx, int32_t* __restrict__ hx, uint32_t* __restrict__ hx2x, uint32_t* mcount) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nx1 || degX[s] == 0) return;
     const uint32_t xid = (uint32_t)(ex_x[s] >> 32);
     int32_t v = -1;
-    if (xw[s] != 0 && dim3h::dense_bin(xw[s]) >= bmin) {
+    const bool dense = hdx != nullptr ? hdx[s] >= thx : (xw[s] != 0 && dim3h::dense_bin(xw[s]) >= bmin);
+    if (dense) {
         v = (int32_t)atomicAdd(mcount, 1u);
         hx2x[v] = xid;
     }
@@ -247,8 +247,10 @@ __global__ void k_mark_hx(const uint32_t* __restrict__ degX, const uint64_t* __r
 
 // column flag of every z ID (hash mode: occupied slots; identity: every ID)
 // (bmin == 0: every z ID is a column, so the columns of a tile are its z IDs)
+// (hdz != nullptr: a column needs at least thz heavy tuples, the dense core)
 __global__ void k_mark_zflag(const uint32_t* __restrict__ degZ, const uint64_t* __restrict__ ex_z, uint32_t nz1,
-                             const uint32_t* __restrict__ zw, uint32_t bmin, uint32_t* __restrict__ zflag) {
+                             const uint32_t* __restrict__ zw, uint32_t bmin, const uint32_t* __restrict__ hdz,
+                             uint32_t thz, uint32_t* __restrict__ zflag) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nz1) return;
     uint32_t zid = s;
@@ -256,7 +258,57 @@ __global__ void k_mark_zflag(const uint32_t* __restrict__ degZ, const uint64_t* 
         if (degZ[s] == 0) return;
         zid = (uint32_t)(ex_z[s] >> 32);
     }
-    zflag[zid] = (bmin == 0u || (zw[s] != 0 && dim3h::dense_bin(zw[s]) >= bmin)) ? 1u : 0u;
+    bool dense;
+    if (bmin == 0u) dense = true;
+    else if (hdz != nullptr) dense = hdz[s] >= thz;
+    else dense = zw[s] != 0 && dim3h::dense_bin(zw[s]) >= bmin;
+    zflag[zid] = dense ? 1u : 0u;
+}
+
+// Heavy entries per x slot and heavy tuples per z slot (dense-core selection).
+__global__ void k_hdeg_x(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
+                         const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid, uint32_t* hdx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nR) return;
+    const int32_t sx = slotRx[i];
+    const bool heavy = sx >= 0 && hid[(uint32_t)(ex_y[slotR[i]] >> 32)] >= 0;
+    wave_count_add(hdx, (uint32_t)sx, heavy);
+}
+
+__global__ void k_hdeg_z(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
+                         const int32_t* __restrict__ slotSz, int32_t zlo, uint64_t nS, const uint64_t* __restrict__ ex_y,
+                         const int32_t* __restrict__ hid, uint32_t* hdz) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    const int32_t sy = slotS[i];
+    const bool heavy = sy >= 0 && hid[(uint32_t)(ex_y[sy] >> 32)] >= 0;
+    const uint32_t zs = heavy ? (slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i]) : 0u;
+    wave_count_add(hdz, zs, heavy);
+}
+
+// hist[4][kDenseBins]: x rows, their heavy entries, z columns, their heavy tuples, by heavy-degree bin
+__global__ __launch_bounds__(256) void k_hdeg_hist(const uint32_t* __restrict__ hdx, uint32_t nx1,
+                                                   const uint32_t* __restrict__ hdz, uint32_t nz1,
+                                                   unsigned long long* hist) {
+    __shared__ unsigned long long lh[4 * kDenseBins];
+    for (uint32_t i = threadIdx.x; i < 4 * kDenseBins; i += 256) lh[i] = 0;
+    __syncthreads();
+    const uint32_t n = max(nx1, nz1);
+    for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < n; s += gridDim.x * 256) {
+        if (s < nx1 && hdx[s] != 0) {
+            const uint32_t b = dim3h::dense_bin(hdx[s]);
+            atomicAdd(&lh[b], 1ull);
+            atomicAdd(&lh[kDenseBins + b], (unsigned long long)hdx[s]);
+        }
+        if (s < nz1 && hdz[s] != 0) {
+            const uint32_t b = dim3h::dense_bin(hdz[s]);
+            atomicAdd(&lh[2 * kDenseBins + b], 1ull);
+            atomicAdd(&lh[3 * kDenseBins + b], (unsigned long long)hdz[s]);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 4 * kDenseBins; i += 256)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
 }
 
 // hz -> z ID, and the first column of every z tile (ntiles + 1 entries)
@@ -346,15 +398,17 @@ template <int MODE>
 __global__ void k_build_b(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
                           const int32_t* __restrict__ slotSz, const uint64_t* __restrict__ ex_z, int32_t zlo,
                           uint64_t nS, const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid,
-                          const uint32_t* __restrict__ hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B,
-                          uint64_t* overflow) {
+                          const uint32_t* __restrict__ zflag, const uint32_t* __restrict__ hzpos, uint32_t Kp,
+                          uint32_t dwords, uint32_t* B, uint64_t* overflow) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const int32_t sy = slotS[i];
     if (sy < 0) return;
     const int32_t h = hid[(uint32_t)(ex_y[sy] >> 32)];
     if (h < 0) return;
-    const uint32_t hz = hzpos[z_of(S, i, slotSz, ex_z, zlo)];
+    const uint32_t zid = z_of(S, i, slotSz, ex_z, zlo);
+    if (!zflag[zid]) return;  // a light column: its paths stay sparse
+    const uint32_t hz = hzpos[zid];
     if (MODE == 2) atomicOr(&B[(uint64_t)h * dwords + (hz >> 5)], 1u << (hz & 31u));
     else put_byte<MODE == 1>(B, (uint64_t)hz * Kp + (uint32_t)h, overflow);
 }
@@ -399,16 +453,32 @@ void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const 
     hipLaunchKernelGGL(k_dense_hist, dim3(g), dim3(256), 0, st, degR, degS, ny1, xw, nx1, zw, nz1, hist);
 }
 
-void dense_mark(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y,
-                const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw, const uint32_t* degZ,
-                const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin, int32_t* hid, int32_t* hx,
-                uint32_t* hx2x, uint32_t* h2y, uint32_t* zflag, bool all_z, uint32_t* counters, hipStream_t st) {
+void dense_mark_keys(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint64_t* ex_y, uint32_t bmin,
+                     int32_t* hid, uint32_t* h2y, uint32_t* counters, hipStream_t st) {
     hipLaunchKernelGGL(k_mark_hid, dim3(grid1(ny1)), dim3(256), 0, st, degR, degS, ny1, ex_y, bmin, hid, h2y,
                        counters);
-    hipLaunchKernelGGL(k_mark_hx, dim3(grid1(nx1)), dim3(256), 0, st, degX, ex_x, nx1, xw, bmin, hx, hx2x,
+}
+
+void dense_heavy_degrees(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const int2* S,
+                         const int32_t* slotS, const int32_t* slotSz, int32_t zlo, uint64_t nS, const uint64_t* ex_y,
+                         const int32_t* hid, uint32_t* hdx, uint32_t nx1, uint32_t* hdz, uint32_t nz1,
+                         unsigned long long* hist, hipStream_t st) {
+    if (nR) hipLaunchKernelGGL(k_hdeg_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, hid, hdx);
+    if (nS)
+        hipLaunchKernelGGL(k_hdeg_z, dim3(grid1(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, ex_y, hid, hdz);
+    unsigned g = grid1(std::max(nx1, nz1));
+    if (g > 1024) g = 1024;
+    hipLaunchKernelGGL(k_hdeg_hist, dim3(g), dim3(256), 0, st, hdx, nx1, hdz, nz1, hist);
+}
+
+void dense_mark_rows_cols(const uint32_t* degX, const uint64_t* ex_x, uint32_t nx1, const uint32_t* xw,
+                          const uint32_t* degZ, const uint64_t* ex_z, uint32_t nz1, const uint32_t* zw, uint32_t bmin,
+                          const uint32_t* hdx, uint32_t thx, const uint32_t* hdz, uint32_t thz, bool all_z,
+                          int32_t* hx, uint32_t* hx2x, uint32_t* zflag, uint32_t* counters, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_hx, dim3(grid1(nx1)), dim3(256), 0, st, degX, ex_x, nx1, xw, bmin, hdx, thx, hx, hx2x,
                        counters + 1);
-    hipLaunchKernelGGL(k_mark_zflag, dim3(grid1(nz1)), dim3(256), 0, st, degZ, ex_z, nz1, zw, all_z ? 0u : bmin,
-                       zflag);
+    hipLaunchKernelGGL(k_mark_zflag, dim3(grid1(nz1)), dim3(256), 0, st, degZ, ex_z, nz1, zw, all_z ? 0u : bmin, hdz,
+                       thz, zflag);
 }
 
 void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, uint32_t* hz2zid, uint32_t ntiles,
@@ -443,19 +513,20 @@ void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, con
 }
 
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
-                   int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* hzpos,
-                   uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow, hipStream_t st) {
+                   int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* zflag,
+                   const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
+                   hipStream_t st) {
     if (!nS) return;
     const unsigned g = grid1(nS);
     if (kind == 2)
-        hipLaunchKernelGGL(k_build_b<2>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, hzpos,
-                           Kp, dwords, B, overflow);
+        hipLaunchKernelGGL(k_build_b<2>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, zflag,
+                           hzpos, Kp, dwords, B, overflow);
     else if (kind == 1)
-        hipLaunchKernelGGL(k_build_b<1>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, hzpos,
-                           Kp, dwords, B, overflow);
+        hipLaunchKernelGGL(k_build_b<1>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, zflag,
+                           hzpos, Kp, dwords, B, overflow);
     else
-        hipLaunchKernelGGL(k_build_b<0>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, hzpos,
-                           Kp, dwords, B, overflow);
+        hipLaunchKernelGGL(k_build_b<0>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, zflag,
+                           hzpos, Kp, dwords, B, overflow);
 }
 
 }  // namespace dim3h

@@ -241,8 +241,14 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
         const bool in_row = i < re;
         const uint32_t craw = (uint32_t)a.colR[in_row ? i : re - 1];
         const uint32_t key = (craw & 0x7FFFFFFFu) * a.ntiles + t;
-        const uint32_t sb = a.rowptrS[key], se_row = a.rowptrS[key + 1];
-        const uint32_t se = (craw >> 31) ? sb : se_row;  // heavy key: its paths are in the dense part
+        uint32_t sb = a.rowptrS[key];
+        const uint32_t se = a.rowptrS[key + 1];
+        // a product row's heavy key: its product-column paths are in the dense
+        // part (left out of the S row, or at its front in a dense core)
+        if (a.splitS != nullptr) {
+            const uint32_t sp = a.splitS[key];
+            if (craw >> 31) sb += sp;
+        }
         uint32_t np = 0;
         if (in_row) {
             s_sb[tid] = sb;

@@ -377,7 +377,8 @@ __global__ void k_finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* to
 // Entries of heavy keys (hid >= 0, dense part) carry bit 31.
 __global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
                             const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_x,
-                            const int32_t* __restrict__ hid, uint32_t* cursor, int32_t* __restrict__ col) {
+                            const int32_t* __restrict__ hid, const int32_t* __restrict__ hx, uint32_t* cursor,
+                            int32_t* __restrict__ col) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
     const int32_t sx = slotRx[i];
@@ -385,7 +386,8 @@ __global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* _
     const uint32_t pos = wave_reserve(cursor, xid, sx >= 0);
     if (sx < 0) return;
     const uint32_t yid = (uint32_t)(ex_y[slotR[i]] >> 32);
-    const uint32_t flag = (hid != nullptr && hid[yid] >= 0) ? 0x80000000u : 0u;
+    // a dense-core row's heavy entry: its product-column paths come from the product
+    const uint32_t flag = (hid != nullptr && hid[yid] >= 0 && hx[xid] >= 0) ? 0x80000000u : 0u;
     col[pos] = (int32_t)(yid | flag);
 }
 
@@ -402,30 +404,55 @@ __device__ __forceinline__ uint32_t z_id(const int2* S, uint64_t i, const int32_
 __global__ void k_count_s(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
                           const int32_t* __restrict__ slotSz, uint64_t nS, const uint64_t* __restrict__ ex_y,
                           const uint64_t* __restrict__ ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size,
-                          const int32_t* __restrict__ hid, uint32_t* cnt, uint32_t* __restrict__ keyS,
-                          int32_t* __restrict__ zidS) {
+                          const int32_t* __restrict__ hid, const uint32_t* __restrict__ zflag,
+                          uint32_t* cnt, uint32_t* cntD, uint32_t* __restrict__ keyS, int32_t* __restrict__ zidS) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const int32_t sy = slotS[i];
-    const uint32_t yid = sy < 0 ? 0u : (uint32_t)(ex_y[sy] >> 32);
-    if (sy < 0 || (hid != nullptr && hid[yid] >= 0)) {  // no match, or a heavy key (dense part)
+    if (sy < 0) {  // no match
         keyS[i] = 0xFFFFFFFFu;
         return;
     }
+    const uint32_t yid = (uint32_t)(ex_y[sy] >> 32);
     const uint32_t zid = z_id(S, i, slotSz, ex_z, zlo);
+    // a heavy key's tuple with a product column: only rows outside the dense
+    // core (cntD != nullptr) still need it, at the front of its CSR row
+    const bool dcol = hid != nullptr && hid[yid] >= 0 && zflag[zid] != 0u;
+    if (dcol && cntD == nullptr) {
+        keyS[i] = 0xFFFFFFFFu;
+        return;
+    }
     const uint32_t key = ntiles == 1 ? yid : yid * ntiles + zid / tile_size;
     wave_count_add(cnt, key, true);
-    keyS[i] = key;
+    if (cntD != nullptr) wave_count_add(cntD, key, dcol);
+    keyS[i] = key | (dcol ? 0x80000000u : 0u);
     zidS[i] = (int32_t)zid;
 }
 
+// cursors of the split S rows: product-column tuples from the row start,
+// the others after them
+__global__ void k_split_cursors(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ cntD, uint32_t n,
+                                uint32_t* __restrict__ curL, uint32_t* __restrict__ curD) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    curD[i] = rowptr[i];
+    curL[i] = rowptr[i] + cntD[i];
+}
+
 __global__ void k_scatter_s(const uint32_t* __restrict__ keyS, const int32_t* __restrict__ zidS, uint64_t nS,
-                            uint32_t* cursor, int32_t* __restrict__ col) {
+                            uint32_t* cursor, uint32_t* cursorD, int32_t* __restrict__ col) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
-    const uint32_t key = keyS[i];
-    const uint32_t pos = wave_reserve(cursor, key, key != 0xFFFFFFFFu);
-    if (key == 0xFFFFFFFFu) return;
+    const uint32_t raw = keyS[i];
+    const bool valid = raw != 0xFFFFFFFFu;
+    const bool dcol = valid && (raw >> 31) != 0u;
+    const uint32_t key = raw & 0x7FFFFFFFu;
+    uint32_t pos = wave_reserve(cursor, key, valid && !dcol);
+    if (cursorD != nullptr) {
+        const uint32_t pd = wave_reserve(cursorD, key, dcol);
+        if (dcol) pos = pd;
+    }
+    if (!valid) return;
     col[pos] = zidS[i];
 }
 
@@ -523,26 +550,31 @@ void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_
     hipLaunchKernelGGL(k_finish_rowptr, dim3(1), dim3(64), 0, st, rowptr, n, tot);
 }
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y, const uint64_t* ex_x,
-               const int32_t* hid, uint32_t* cursor, int32_t* col, hipStream_t st) {
+               const int32_t* hid, const int32_t* hx, uint32_t* cursor, int32_t* col, hipStream_t st) {
     if (nR)
         hipLaunchKernelGGL(k_scatter_r, dim3(grid_for(nR)), dim3(kThreads), 0, st, slotR, slotRx, nR, ex_y, ex_x,
-                           hid, cursor, col);
+                           hid, hx, cursor, col);
 }
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
              const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
-             uint32_t* cnt, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
+             const uint32_t* zflag, uint32_t* cnt, uint32_t* cntD, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
     if (nS)
         hipLaunchKernelGGL(k_count_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, slotS, slotSz, nS, ex_y, ex_z,
-                           zlo, ntiles, tile_size, hid, cnt, keyS, zidS);
+                           zlo, ntiles, tile_size, hid, zflag, cnt, cntD, keyS, zidS);
+}
+void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uint32_t* curL, uint32_t* curD,
+                   hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_split_cursors, dim3(grid_for(n)), dim3(kThreads), 0, st, rowptr, cntD, n, curL, curD);
 }
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st) {
     scan_exclusive(ValU32{cnt}, OutU32{out_excl}, n, partials, tot, st);
 }
-void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, int32_t* col,
-               hipStream_t st) {
+void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, uint32_t* cursorD,
+               int32_t* col, hipStream_t st) {
     if (nS)
-        hipLaunchKernelGGL(k_scatter_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, keyS, zidS, nS, cursor, col);
+        hipLaunchKernelGGL(k_scatter_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, keyS, zidS, nS, cursor, cursorD,
+                           col);
 }
 void x2z_self(const int32_t* id2x, uint32_t nx, const KeyMap& mz, const uint32_t* degZ, const uint64_t* ex_z,
               int32_t* x2z, hipStream_t st) {

@@ -158,3 +158,34 @@ def test_bitor_rejected_for_count(ctx, dim3):
     with pytest.raises(dim3.Dim3Error) as e:
         ctx.join_aggregate_count(R, S, dense_mode=BITOR)
     assert e.value.status == dim3.ERR_UNSUPPORTED
+
+
+def zipf_with_block(seed, n_bg, block, density, dom_a, dom_b, s=1.0):
+    """Zipf(s) background whose hottest keys also carry a planted block."""
+    return planted(seed, n_bg, block, density, dom_a, dom_b, s=s)
+
+
+@pytest.mark.parametrize("mode", [MFMA, BITOR])
+@pytest.mark.parametrize("tile_bits", [0, 640])
+def test_dense_core_vs_oracle(ctx, mode, tile_bits):
+    """Forced dense core: only rows / columns with many heavy keys enter the
+    product; the other heavy paths stay in the (split) sparse CSR."""
+    R = zipf_with_block(41, 30000, 250, 0.35, 4000, 1500)
+    S = zipf_with_block(42, 30000, 250, 0.35, 4000, 1500)
+    got = check_project(ctx, R, S, dense_mode=mode, dense_core=1, tile_bits=tile_bits)
+    st = got.stats
+    say("  core", st.dense_mode, "keys", st.heavy_keys, "rows", st.nx_heavy, "of", st.nx, "cols", st.nz_heavy,
+        "of", st.nz, "sparse paths", st.paths_sparse, "of", st.out_j)
+    assert st.dense_mode == mode and st.heavy_keys > 0
+    assert st.nx_heavy < st.nx  # a real core: not every row with a heavy key
+    if mode == MFMA:
+        agg = check_count(ctx, R, S, dense_mode=MFMA, dense_core=1, tile_bits=tile_bits)
+        assert agg.stats.dense_mode == MFMA
+
+
+def test_dense_core_self_join_and_hashing(ctx):
+    R = zipf_with_block(43, 20000, 200, 0.4, 3000, 1000)
+    R = (R.astype(np.int64) * 7919 - 5_000_000).astype(np.int32)
+    got = check_project(ctx, R, R, dense_mode=MFMA, dense_core=1, exclude_equal=True)
+    assert got.stats.dense_mode == MFMA
+    check_count(ctx, R, R, dense_mode=MFMA, dense_core=1, exclude_equal=True, hash_only=True)

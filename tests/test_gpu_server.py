@@ -1,0 +1,77 @@
+"""GPU tests of the serving loop: every request's result in a concurrent,
+batched mix equals the same request run alone (and the CPU oracle's);
+admission by footprint, unknown handles and dropped relations."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.test_gpu_batch import mixed_instances
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_mix_equals_alone(ctx, dim3):
+    inst = mixed_instances(11) + mixed_instances(12)
+    with dim3.Server(0, workers=4, max_batch=8) as srv:
+        hs = [(srv.put(R), srv.put(S)) for R, S in inst]
+        reqs = []
+        for i, (r, s) in enumerate(hs):
+            count = i % 5 == 3
+            mat = i % 7 == 2
+            kw = dict(output_mode=0 if mat else 1, exclude_equal=(i % 4 == 1))
+            rc, t = srv.submit(r, s, count=count, **kw)
+            assert rc == dim3.OK
+            reqs.append((i, count, kw, t))
+        say("submitted", len(reqs), "requests")
+        for i, count, kw, t in reqs:
+            rc, res, q, tot = srv.wait(t)
+            assert rc == dim3.OK, i
+            assert tot >= q >= 0
+            R, S = inst[i]
+            alone = ctx.join_aggregate_count(R, S, **kw) if count else ctx.join_project(R, S, **kw)
+            assert res.fp.as_tuple() == alone.fp.as_tuple(), i
+            assert res.swapped == alone.swapped and res.n == alone.n, i
+            if kw["output_mode"] == 0:
+                if count:
+                    gp, gc = oracle.sort_pairs(res.xz, res.counts)
+                    wp, wc, _, _ = oracle.join_aggregate_count(R, S, exclude_equal=kw["exclude_equal"])
+                    ep, ec = oracle.sort_pairs(wp, wc)
+                    assert np.array_equal(gp, ep) and np.array_equal(gc, ec), i
+                else:
+                    want, _, _ = oracle.join_project(R, S, exclude_equal=kw["exclude_equal"])
+                    assert np.array_equal(oracle.sort_pairs(res.pairs), oracle.sort_pairs(want)), i
+        st = srv.stats()
+        say("server stats", st)
+        assert st["completed"] == len(reqs)
+        assert st["batches"] >= 1 and st["batched_requests"] >= 2
+
+
+def test_admission_and_handles(dim3):
+    rng = np.random.default_rng(5)
+    big = np.stack([rng.integers(0, 1000, 1_000_000), rng.integers(0, 1000, 1_000_000)], 1).astype(np.int32)
+    small = big[:1000].copy()
+    with dim3.Server(0, hbm_budget_bytes=64 << 20, workers=2) as srv:
+        hb = srv.put(big)
+        hs = srv.put(small)
+        rc, _ = srv.submit(hb, hb, output_mode=1)
+        assert rc == dim3.ERR_ADMISSION  # ~200 MB of workspace never fits 64 MiB
+        rc, t = srv.submit(hs, hs, output_mode=1)
+        assert rc == dim3.OK
+        srv.drop(hs)  # freed once the request is done
+        rc, res, _, _ = srv.wait(t)
+        assert rc == dim3.OK
+        _, want, _ = oracle.join_project(small, small, fingerprint_only=True)
+        assert res.fp.as_tuple() == want.as_tuple()
+        with pytest.raises(dim3.Dim3Error):
+            srv.submit(hs, hs, output_mode=1)
+        with pytest.raises(dim3.Dim3Error):
+            srv.wait(123456)
+        st = srv.stats()
+        assert st["rejected"] == 1 and st["completed"] == 1
