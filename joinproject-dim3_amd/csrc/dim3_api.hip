@@ -638,7 +638,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         dim3h::fill_u32(zw, nz1, 0, st);
         dim3h::fill_u64((uint64_t*)dhist, 6ull * dim3h::kDenseBins, 0, st);
         dim3h::dense_weights(dR, nR, slotR, slotRx, dS, nS, slotS, P.z_ident ? nullptr : slotSz, P.zlo, degR, degS, xw,
-                             zw, st);
+                             (uint32_t)nx1, zw, (uint32_t)nz1, st);
         dim3h::dense_hist(degR, degS, (uint32_t)ny1, xw, (uint32_t)nx1, zw, (uint32_t)nz1, dhist, st);
         HIPCHK(c, hipMemcpyAsync(c->h_hist, dhist, 6ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
     }
@@ -1225,9 +1225,9 @@ int autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
         HIPCHK(c, hipStreamSynchronize(st));
         m.hbm_bps = 4.0 * n * 4 / (elapsed(c, 0, 1) * 1e-3);
     }
-    // int8 product on the MFMA cores: 4096 x 4096 x 4096
+    // int8 product on the MFMA cores: 8192 x 8192 x 4096
     {
-        const uint32_t M = 4096, K = 4096;
+        const uint32_t M = 8192, K = 4096;
         uint32_t *A, *B, *D;
         TRY(ws(c, WS_DA, (uint64_t)M * K / 4, &A));
         TRY(ws(c, WS_DB, (uint64_t)M * K / 4, &B));

@@ -392,7 +392,7 @@ int bitor_rows(const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid,
 // largest key heaviness over the tuples of each x slot / z slot (slotSz == nullptr: z identity)
 void dense_weights(const int2* R, uint64_t nR, const uint32_t* slotR, const int32_t* slotRx, const int2* S,
                    uint64_t nS, const int32_t* slotS, const int32_t* slotSz, int32_t zlo, const uint32_t* degR,
-                   const uint32_t* degS, uint32_t* xw, uint32_t* zw, hipStream_t st);
+                   const uint32_t* degS, uint32_t* xw, uint32_t nx1, uint32_t* zw, uint32_t nz1, hipStream_t st);
 // hist[6][kDenseBins]: keys, paths, sum degR, sum degS, x rows, z columns per heaviness bin
 void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint32_t* xw, uint32_t nx1,
                 const uint32_t* zw, uint32_t nz1, unsigned long long* hist, hipStream_t st);

@@ -119,6 +119,129 @@ __global__ __launch_bounds__(256) void k_gemm_i8(const int8_t* __restrict__ A, c
     }
 }
 
+// Large-tile product for M, N >= 256: 256 threads, each wavefront a
+// (16 TI) x (16 TJ) quadrant of TI x TJ MFMA tiles, workgroup tile
+// (32 TI) x (32 TJ); every 16-byte fragment read from LDS feeds TJ (A) or
+// TI (B) MFMAs.  K steps of 64 are double-buffered in LDS with the next
+// step's global loads issued before the current step's MFMAs (one barrier
+// per step).  Tiles are numbered so that each XCD (workgroups are dealt to
+// the 8 XCDs round robin) works through a contiguous run of 8-row tile
+// groups, sharing A and B panels in its own L2.
+template <int TI, int TJ>
+__global__ __launch_bounds__(256, 2) void k_gemm_i8_big(const int8_t* __restrict__ A, const int8_t* __restrict__ Bt,
+                                                     uint32_t M, uint32_t N, uint32_t Kp,
+                                                     uint32_t* __restrict__ dbits, uint32_t dwords,
+                                                     int32_t* __restrict__ cint) {
+    constexpr uint32_t BM = 32 * TI, BN = 32 * TJ;   // workgroup tile
+    constexpr uint32_t CA = BM * 4 / 256, CB = BN * 4 / 256;  // 16-byte chunks per thread per step
+    __shared__ __align__(16) int8_t sA[2][BM * kLdsStride];
+    __shared__ __align__(16) int8_t sB[2][BN * kLdsStride];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    const uint32_t gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN, total = gm * gn;
+    const uint32_t per_xcd = (total + 7u) / 8u;
+    const uint32_t seq = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+    if (seq >= total) return;
+    const uint32_t grp = seq / (8u * gn), r = seq - grp * 8u * gn;
+    const uint32_t gh = min(8u, gm - grp * 8u);
+    const uint32_t m0 = (grp * 8u + r % gh) * BM, n0 = (r / gh) * BN;
+    const uint32_t wm = (wid >> 1) * (16u * TI), wn = (wid & 1u) * (16u * TJ);
+
+    v4i acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++) acc[i][j] = v4i{0, 0, 0, 0};
+
+    v4i pa[CA], pb[CB];
+    auto load = [&](uint32_t k0) {
+#pragma unroll
+        for (uint32_t c = 0; c < CA; c++) {
+            const uint32_t chunk = tid + c * 256u;
+            const uint32_t row = chunk >> 2, q = chunk & 3u;
+            pa[c] = m0 + row < M ? *reinterpret_cast<const v4i*>(A + (size_t)(m0 + row) * Kp + k0 + q * 16u)
+                                 : v4i{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < CB; c++) {
+            const uint32_t chunk = tid + c * 256u;
+            const uint32_t row = chunk >> 2, q = chunk & 3u;
+            pb[c] = n0 + row < N ? *reinterpret_cast<const v4i*>(Bt + (size_t)(n0 + row) * Kp + k0 + q * 16u)
+                                 : v4i{0, 0, 0, 0};
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (uint32_t c = 0; c < CA; c++) {
+            const uint32_t chunk = tid + c * 256u;
+            *reinterpret_cast<v4i*>(&sA[buf][(chunk >> 2) * kLdsStride + (chunk & 3u) * 16u]) = pa[c];
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < CB; c++) {
+            const uint32_t chunk = tid + c * 256u;
+            *reinterpret_cast<v4i*>(&sB[buf][(chunk >> 2) * kLdsStride + (chunk & 3u) * 16u]) = pb[c];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    const uint32_t fr = lane & 15u, fk = (lane >> 4) * 16u;
+    int buf = 0;
+    for (uint32_t k0 = 0; k0 < Kp; k0 += kTK) {
+        const bool more = k0 + kTK < Kp;
+        if (more) load(k0 + kTK);
+        // all fragments of the step first, so the MFMAs issue back to back
+        v4i fb[TJ], fa[TI];
+#pragma unroll
+        for (int j = 0; j < TJ; j++)
+            fb[j] = *reinterpret_cast<const v4i*>(&sB[buf][(wn + j * 16 + fr) * kLdsStride + fk]);
+#pragma unroll
+        for (int i = 0; i < TI; i++)
+            fa[i] = *reinterpret_cast<const v4i*>(&sA[buf][(wm + i * 16 + fr) * kLdsStride + fk]);
+#pragma unroll
+        for (int i = 0; i < TI; i++)
+#pragma unroll
+            for (int j = 0; j < TJ; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    const uint32_t col_in = lane & 15u, rgrp = (lane >> 4) * 4u;
+    if (cint != nullptr) {
+#pragma unroll
+        for (int i = 0; i < TI; i++)
+#pragma unroll
+            for (int j = 0; j < TJ; j++) {
+                const uint32_t col = n0 + wn + j * 16 + col_in;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint32_t row = m0 + wm + i * 16 + rgrp + e;
+                    if (row < M && col < N) cint[(size_t)row * N + col] = acc[i][j][e];
+                }
+            }
+    }
+    if (dbits != nullptr) {
+#pragma unroll
+        for (int i = 0; i < TI; i++)
+#pragma unroll
+            for (int jp = 0; jp < TJ / 2; jp++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint64_t b0 = __ballot(acc[i][2 * jp][e] > 0);
+                    const uint64_t b1 = __ballot(acc[i][2 * jp + 1][e] > 0);
+                    if (lane < 4u) {
+                        const uint32_t g = lane;
+                        const uint32_t word = (uint32_t)((b0 >> (16 * g)) & 0xFFFFull) |
+                                              ((uint32_t)((b1 >> (16 * g)) & 0xFFFFull) << 16);
+                        const uint32_t row = m0 + wm + i * 16 + 4 * g + e;
+                        const uint32_t wcol = (n0 + wn + jp * 32) >> 5;
+                        if (row < M && wcol < dwords) dbits[(size_t)row * dwords + wcol] = word;
+                    }
+                }
+    }
+}
+
 // Bit-packed product: Dbits row r = OR of the z bitmaps of the heavy keys of
 // x = hx2x[r].  One workgroup per row; each thread owns words tid + 256 j of
 // a 2048-word chunk of the row.
@@ -286,6 +409,72 @@ __global__ void k_hdeg_z(const int2* __restrict__ S, const int32_t* __restrict__
     wave_count_add(hdz, zs, heavy);
 }
 
+// The same counts (KIND 0: heavy entries per x slot, 1: heavy tuples per z
+// slot) and the same maxima (KIND 2 / 3: key heaviness per x / z slot)
+// through a workgroup-private LDS array, for slot spaces of at most 32768
+// slots (small attribute domains, where per-tuple atomics would pile onto a
+// few hot lines); one global update per nonzero slot and workgroup.
+constexpr uint32_t kLdsSlotsD = 32768;
+template <int KIND>
+__global__ __launch_bounds__(1024) void k_slot_lds(const int2* __restrict__ S, const uint32_t* __restrict__ slotR,
+                                                   const int32_t* __restrict__ slotRx,
+                                                   const int32_t* __restrict__ slotS,
+                                                   const int32_t* __restrict__ slotSz, int32_t zlo, uint64_t n,
+                                                   const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid,
+                                                   const uint32_t* __restrict__ degR,
+                                                   const uint32_t* __restrict__ degS, uint32_t nslots,
+                                                   uint32_t* out) {
+    extern __shared__ uint32_t lds[];
+    for (uint32_t s = threadIdx.x; s < nslots; s += 1024) lds[s] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 1024) {
+        int32_t slot = -1;
+        uint32_t ys = 0;
+        if (KIND == 0 || KIND == 2) {
+            slot = slotRx[i];
+            if (slot >= 0) ys = slotR[i];
+        } else {
+            const int32_t sy = slotS[i];
+            if (sy >= 0) {
+                ys = (uint32_t)sy;
+                slot = slotSz == nullptr ? (int32_t)((uint32_t)S[i].x - (uint32_t)zlo) : slotSz[i];
+            }
+        }
+        if (slot < 0) continue;
+        if (KIND <= 1) {
+            if (hid[(uint32_t)(ex_y[ys] >> 32)] >= 0) atomicAdd(&lds[slot], 1u);
+        } else {
+            const uint32_t w = key_w(degR, degS, ys);
+            if (w > lds[slot]) atomicMax(&lds[slot], w);
+        }
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < nslots; s += 1024) {
+        const uint32_t v = lds[s];
+        if (v == 0) continue;
+        if (KIND <= 1) atomicAdd(&out[s], v);
+        else if (v > out[s]) atomicMax(&out[s], v);
+    }
+}
+
+template <int KIND>
+bool slot_lds(const int2* S, const uint32_t* slotR, const int32_t* slotRx, const int32_t* slotS, const int32_t* slotSz,
+              int32_t zlo, uint64_t n, const uint64_t* ex_y, const int32_t* hid, const uint32_t* degR,
+              const uint32_t* degS, uint32_t nslots, uint32_t* out, hipStream_t st) {
+    if (nslots > kLdsSlotsD || n < 8ull * nslots || n == 0) return false;
+    const size_t lds = (size_t)nslots * 4u;
+    if (hipFuncSetAttribute((const void*)k_slot_lds<KIND>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    uint64_t g = (n + 32767) / 32768;
+    if (g > 256) g = 256;
+    hipLaunchKernelGGL(k_slot_lds<KIND>, dim3((unsigned)g), dim3(1024), lds, st, S, slotR, slotRx, slotS, slotSz, zlo,
+                       n, ex_y, hid, degR, degS, nslots, out);
+    return true;
+}
+
 // hist[4][kDenseBins]: x rows, their heavy entries, z columns, their heavy tuples, by heavy-degree bin
 __global__ __launch_bounds__(256) void k_hdeg_hist(const uint32_t* __restrict__ hdx, uint32_t nx1,
                                                    const uint32_t* __restrict__ hdz, uint32_t nz1,
@@ -423,6 +612,15 @@ int gemm_i8(const int8_t* A, const int8_t* Bt, uint32_t M, uint32_t N, uint32_t 
             uint32_t dwords, int32_t* cint, hipStream_t st) {
     if (M == 0 || N == 0) return 0;
     if (Kp % kTK != 0) return (int)hipErrorInvalidValue;
+    if (M >= 256u && N >= 128u) {
+        constexpr int TI = 8, TJ = 4;  // 256 x 128 workgroup tiles, 128 x 64 per wavefront
+        const uint64_t tiles = (uint64_t)((M + 32 * TI - 1) / (32 * TI)) * ((N + 32 * TJ - 1) / (32 * TJ));
+        const uint64_t grid = (tiles + 7) / 8 * 8;
+        if (grid > 0x7FFFFFFFull) return (int)hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_gemm_i8_big<TI, TJ>), dim3((unsigned)grid), dim3(256), 0, st, A, Bt, M, N, Kp, dbits,
+                           dwords, cint);
+        return (int)hipGetLastError();
+    }
     dim3 grid((N + kTN - 1) / kTN, (M + kTM - 1) / kTM);
     if (grid.y > 65535) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gemm_i8, grid, dim3(256), 0, st, A, Bt, M, N, Kp, dbits, dwords, cint);
@@ -438,10 +636,11 @@ int bitor_rows(const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid,
 
 void dense_weights(const int2* R, uint64_t nR, const uint32_t* slotR, const int32_t* slotRx, const int2* S,
                    uint64_t nS, const int32_t* slotS, const int32_t* slotSz, int32_t zlo, const uint32_t* degR,
-                   const uint32_t* degS, uint32_t* xw, uint32_t* zw, hipStream_t st) {
+                   const uint32_t* degS, uint32_t* xw, uint32_t nx1, uint32_t* zw, uint32_t nz1, hipStream_t st) {
     (void)R;
-    if (nR) hipLaunchKernelGGL(k_wmax_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, degR, degS, xw);
-    if (nS)
+    if (nR && !slot_lds<2>(S, slotR, slotRx, slotS, slotSz, zlo, nR, nullptr, nullptr, degR, degS, nx1, xw, st))
+        hipLaunchKernelGGL(k_wmax_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, degR, degS, xw);
+    if (nS && !slot_lds<3>(S, slotR, slotRx, slotS, slotSz, zlo, nS, nullptr, nullptr, degR, degS, nz1, zw, st))
         hipLaunchKernelGGL(k_wmax_z, dim3(grid1(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, degR, degS, zw);
 }
 
@@ -463,8 +662,9 @@ void dense_heavy_degrees(const uint32_t* slotR, const int32_t* slotRx, uint64_t 
                          const int32_t* slotS, const int32_t* slotSz, int32_t zlo, uint64_t nS, const uint64_t* ex_y,
                          const int32_t* hid, uint32_t* hdx, uint32_t nx1, uint32_t* hdz, uint32_t nz1,
                          unsigned long long* hist, hipStream_t st) {
-    if (nR) hipLaunchKernelGGL(k_hdeg_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, hid, hdx);
-    if (nS)
+    if (nR && !slot_lds<0>(S, slotR, slotRx, slotS, slotSz, zlo, nR, ex_y, hid, nullptr, nullptr, nx1, hdx, st))
+        hipLaunchKernelGGL(k_hdeg_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, hid, hdx);
+    if (nS && !slot_lds<1>(S, slotR, slotRx, slotS, slotSz, zlo, nS, ex_y, hid, nullptr, nullptr, nz1, hdz, st))
         hipLaunchKernelGGL(k_hdeg_z, dim3(grid1(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, ex_y, hid, hdz);
     unsigned g = grid1(std::max(nx1, nz1));
     if (g > 1024) g = 1024;
