@@ -419,7 +419,7 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
         }
     }
     // dense-core candidate: the keys whose R and S degrees are both >= 64, when
-    // there are many of them (a block of keys rather than a few hubs)
+    // there are many of them (>= 2048: a block of keys rather than hubs)
     // (large inputs only: the analysis costs a host round trip)
     const bool core_ok = o.dense_core == 1 || (o.dense_core == -1 && P.nR + P.nS >= 8000000ull);
     if (core_ok && !(count && o.dense_mode == DIM3_DENSE_BITOR)) {
@@ -431,9 +431,9 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
             k += h[b];
             pth += h[NB + b];
             if (h[b]) bc = (uint32_t)b;
-            if (o.dense_core != 1 && k >= 256 && dim3h::dense_bin_lo((uint32_t)b) <= 1024) break;
+            if (o.dense_core != 1 && k >= 2048 && dim3h::dense_bin_lo((uint32_t)b) <= 1024) break;
         }
-        if (bc && (k >= 256 || o.dense_core == 1)) {
+        if (bc && (k >= 2048 || o.dense_core == 1)) {
             P.core_try = true;
             P.bc = bc;
             P.core_K = k;
@@ -493,6 +493,10 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
                                  ((double)sz[bz] / (double)sz[1]));
             d.sumR = sx[bx];
             d.nz = P.nz;
+            // a core pays off only where its cells are reached by many paths
+            // (a dense block); thinner cores are priced wrongly by the
+            // independence estimate above and are left to the sparse part
+            if (o.dense_core != 1 && (double)d.paths < 256.0 * (double)d.Mh * (double)d.Nh) continue;
             const int modes[2] = {DIM3_DENSE_MFMA, DIM3_DENSE_BITOR};
             for (int mode : modes) {
                 if (o.dense_mode > 0 && o.dense_mode != mode) continue;
@@ -509,7 +513,9 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
         }
     }
     if (bm == DIM3_DENSE_OFF) return false;
-    if (o.dense_core != 1 && (best <= P.p1_gain || (o.dense_mode == DIM3_DENSE_AUTO && best <= 0))) return false;
+    // (a core must save at least a millisecond: small cores are within the
+    // error of the estimate and cost their own passes)
+    if (o.dense_core != 1 && (best <= P.p1_gain || (o.dense_mode == DIM3_DENSE_AUTO && best <= 1e-3))) return false;
     P.dense = bm;
     P.bmin = P.bc;
     P.K = (uint32_t)P.core_K;
@@ -793,7 +799,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         TRY(ws(c, WS_FP_GW, nw, &gw));
         TRY(ws(c, WS_FP_GWM, nw, &gwm));
         dim3h::fp_tables(P.z_ident ? nullptr : id2z, P.zlo, P.nz, P.ntiles, P.tile_words, P.tile_size,
-                         P.dense == DIM3_DENSE_OFF ? 1u : 0u, gw, gwm, st);
+                         (P.dense == DIM3_DENSE_OFF || !P.all_z) ? 1u : 0u, gw, gwm, st);
         LAUNCHCHK(c, "fingerprint word totals");
     }
     return DIM3_OK;
@@ -823,7 +829,9 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.touched_cap = P.touched_cap;
     a.stage_cap = P.stage_cap;
     a.fp_mode = fp_mode ? 1u : 0u;
-    a.swz = P.dense == DIM3_DENSE_OFF ? 1u : 0u;
+    // plain bit layout only where the dense seed copies whole words (every z
+    // ID a product column); elsewhere the swizzle keeps hub z off one word
+    a.swz = (P.dense == DIM3_DENSE_OFF || !P.all_z) ? 1u : 0u;
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
     if (fp_mode && !P.count_mode) {

@@ -222,23 +222,30 @@ __global__ __launch_bounds__(256, 2) void k_gemm_i8_big(const int8_t* __restrict
             }
     }
     if (dbits != nullptr) {
+        // ballots over (i, j, e) are wavefront-uniform: for each 16-row tile
+        // i, lane L < 16 assembles the words of row 16 i + L (g = L / 4,
+        // e = L % 4) and stores them, one instruction per 32-column word
+        const uint32_t g = (lane >> 2) & 3u, ee = lane & 3u;
 #pragma unroll
-        for (int i = 0; i < TI; i++)
+        for (int i = 0; i < TI; i++) {
+            const uint32_t row = m0 + wm + i * 16 + lane;
 #pragma unroll
-            for (int jp = 0; jp < TJ / 2; jp++)
+            for (int jp = 0; jp < TJ / 2; jp++) {
+                uint64_t b0[4], b1[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const uint64_t b0 = __ballot(acc[i][2 * jp][e] > 0);
-                    const uint64_t b1 = __ballot(acc[i][2 * jp + 1][e] > 0);
-                    if (lane < 4u) {
-                        const uint32_t g = lane;
-                        const uint32_t word = (uint32_t)((b0 >> (16 * g)) & 0xFFFFull) |
-                                              ((uint32_t)((b1 >> (16 * g)) & 0xFFFFull) << 16);
-                        const uint32_t row = m0 + wm + i * 16 + 4 * g + e;
-                        const uint32_t wcol = (n0 + wn + jp * 32) >> 5;
-                        if (row < M && wcol < dwords) dbits[(size_t)row * dwords + wcol] = word;
-                    }
+                    b0[e] = __ballot(acc[i][2 * jp][e] > 0);
+                    b1[e] = __ballot(acc[i][2 * jp + 1][e] > 0);
                 }
+                const uint64_t lo = ee == 0 ? b0[0] : ee == 1 ? b0[1] : ee == 2 ? b0[2] : b0[3];
+                const uint64_t hi = ee == 0 ? b1[0] : ee == 1 ? b1[1] : ee == 2 ? b1[2] : b1[3];
+                const uint32_t word = (uint32_t)((lo >> (16 * g)) & 0xFFFFull) |
+                                      ((uint32_t)((hi >> (16 * g)) & 0xFFFFull) << 16);
+                // lane L holds row 4 g + e = L of the tile
+                const uint32_t wcol = (n0 + wn + jp * 32) >> 5;
+                if (lane < 16u && row < M && wcol < dwords) dbits[(size_t)row * dwords + wcol] = word;
+            }
+        }
     }
 }
 
