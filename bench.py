@@ -33,13 +33,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def expand_alg_bytes(st, n_out, row_bytes=8):
+def z_lookup(S):
+    """True when the z values of S are hash-mapped (their span exceeds
+    2 |S| + 64), so that each output row reads its z value from a table;
+    dense-range z IDs are value - min and need no lookup."""
+    if S.shape[0] == 0:
+        return False
+    z = S[:, 0]
+    return int(z.max()) - int(z.min()) + 1 > 2 * S.shape[0] + 64
+
+
+def expand_alg_bytes(st, n_out, row_bytes=8, lookup=False):
     """Algorithmic HBM bytes of one expansion launch: one 4-byte column per
     sparse join path, the row's y list and its S row bounds (4 + 8 bytes per
     R entry), the row bounds and value of every x (12 bytes), and the output
-    rows plus the z value lookup of each (row_bytes + 4)."""
-    nnz_r = st.n_r  # upper bound of surviving R tuples; refined below when known
-    return 4 * st.paths_sparse + 12 * nnz_r + 12 * st.nx + (row_bytes + 4) * n_out
+    rows (row_bytes each, plus a 4-byte z value read when z is hash-mapped)."""
+    nnz_r = st.n_r  # upper bound of surviving R tuples
+    return 4 * st.paths_sparse + 12 * nnz_r + 12 * st.nx + (row_bytes + (4 if lookup else 0)) * n_out
 
 
 def query_alg_bytes(st, n_out, row_bytes=8):
@@ -218,6 +228,136 @@ def batch_bench(args, d, torch):
     tctx.close()
 
 
+def aggregate_cpu_baseline(R, S, target_s, slices):
+    """Reference DIM3JoinAggregate (COUNT policy, single thread) on z slices
+    S restricted to z = r (mod slices) against all of R; stops once target_s
+    of CPU time has been spent.  Returns groups/s."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    total, total_t, done = 0, 0.0, 0
+    for r in range(slices):
+        Ss = S[(S[:, 0] % slices) == r]
+        t0 = time.perf_counter()
+        xz, cnt = oracle.ref_join_aggregate_count(R, Ss)
+        dt = time.perf_counter() - t0
+        total += int(xz.shape[0])
+        del xz, cnt
+        total_t += dt
+        done += 1
+        log("cpu baseline slice %d/%d: %d groups in %.2fs" % (r, slices, total, dt))
+        if total_t >= target_s:
+            break
+    return {"value": total / total_t, "unit": "pairs/s", "cores": 1, "kind": "reference",
+            "sample": "reference DIM3JoinAggregate with a COUNT policy (oracle/_ref, one thread) on %d of %d z "
+                      "slices (S restricted to c %% %d == r, all of R), %d groups in %.2f s"
+                      % (done, slices, slices, total, total_t)}
+
+
+def config_bench(args, d, torch):
+    """The two single-query Zipf workloads.
+    project_small: join-project of R(a,b), S(c,b), 1M distinct tuples each,
+      a, c ~ U[0,1e5), b ~ Zipf(1.0) on [0,1e4); output materialised in HBM.
+    aggregate: SELECT a, c, COUNT(*) over R(a,b), S(c,b), 10M distinct tuples
+      each, a, c ~ U[0,2e6), b ~ Zipf(1.2) on [0,1e6); ~2.4e12 groups, so the
+      result is the order-independent fingerprint (count, sums, count sum),
+      materialising 2.4e12 rows of 16 bytes being out of any memory."""
+    from dim3_amd import workloads
+
+    agg = args.workload == "aggregate"
+    t0 = time.perf_counter()
+    if agg:
+        R, S = workloads.aggregate_large()
+    else:
+        R, S = workloads.project_small()
+    log("%s: %d + %d tuples in %.1fs" % (args.workload, R.shape[0], S.shape[0], time.perf_counter() - t0))
+    ctx = d.Context(0)
+    dev = torch.device("cuda:0")
+    dR = torch.from_numpy(R).to(dev)
+    dS = torch.from_numpy(S).to(dev)
+    torch.cuda.synchronize()
+    mode = "fingerprint" if agg else args.mode
+    kw = dict(dense_mode=args.dense, block=args.block, tile_bits=args.tile_bits,
+              output_mode=1 if mode == "fingerprint" else 0, hash_only=args.hash_only)
+    fn = ctx.join_aggregate_count_dev if agg else ctx.join_project_dev
+    out_ptr, cap, n_out = 0, 0, None
+    if mode == "materialize":
+        rc, r0 = fn(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, **kw)
+        n_out = r0.n
+        out = torch.empty((max(n_out, 1), 4 if agg else 2), dtype=torch.int32, device=dev)
+        out_ptr, cap = out.data_ptr(), n_out
+    log("output rows: %s" % n_out)
+
+    def step():
+        rc, res = fn(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], out_ptr, cap, **kw)
+        if rc != d.OK:
+            raise RuntimeError("query failed: %s" % ctx.last_error())
+        return res
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ress = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    res = ress[-1]
+    st = res.stats
+    n_out = res.n
+    exp_ms = float(np.mean([r.stats.ms_expand for r in ress]))
+    row_bytes = 0 if mode == "fingerprint" else (16 if agg else 8)
+    b_kernel = expand_alg_bytes(st, n_out, row_bytes, z_lookup(S))
+    achieved = b_kernel / (exp_ms * 1e-3) / 1e9
+    log("stats: " + json.dumps({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.as_dict().items()}))
+    log("fingerprint: %s" % (res.fp.as_tuple(),))
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            if agg:
+                cpu = aggregate_cpu_baseline(R, S, args.cpu_seconds, 8192)
+            else:
+                cpu = cpu_baseline(R, args.cpu_seconds, 64, S=S, self_filter=False)
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline failed: %r" % (e,))
+    if agg:
+        config = {"workload": "join-aggregate COUNT, Zipf(1.2) keys", "tuples_per_side": int(R.shape[0]),
+                  "a_c_domain": 2_000_000, "b_domain": 1_000_000, "zipf_s": 1.2, "output": "fingerprint",
+                  "groups": int(n_out), "join_paths": int(st.out_j), "parallelism": "single GPU"}
+    else:
+        config = {"workload": "join-project, Zipf(1.0) keys", "tuples_per_side": int(R.shape[0]),
+                  "a_c_domain": 100_000, "b_domain": 10_000, "zipf_s": 1.0,
+                  "output": "materialized int32 pairs in HBM" if mode == "materialize" else "fingerprint",
+                  "distinct_pairs": int(n_out), "join_paths": int(st.out_j), "parallelism": "single GPU"}
+    line = {
+        "metric": METRIC,
+        "value": n_out * args.steps / elapsed,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded Zipf relations generated on the host, resident in HBM before timing)",
+        "config": config,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_expand", "kernel_ms": exp_ms,
+                     "alg_bytes": b_kernel,
+                     "limiter": ("vector ALU: the z factors of every group's fingerprint (~2.4e12 groups), "
+                                 "not HBM") if mode == "fingerprint" else "HBM: output rows"},
+        "cpu_baseline": cpu,
+        "dense": {"mode": {0: "off", 1: "bitor", 2: "mfma"}[int(st.dense_mode)], "heavy_keys": int(st.heavy_keys),
+                  "rows": int(st.nx_heavy), "cols": int(st.nz_heavy),
+                  "phase_ms": float(np.mean([r.stats.ms_dense for r in ress]))},
+        "fingerprint": [str(v) for v in res.fp.as_tuple()],
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def large_request_relation(torch, workloads, n_background, seed, block_keys=20_000, density=0.3):
     """A large served request's relation: Zipf(1.0) background of
     n_background tuples (attribute domain 10%, key domain 1% of the size)
@@ -377,12 +517,15 @@ def main():
     ap.add_argument("--edges", type=int, default=4_000_000)
     ap.add_argument("--max-degree", type=int, default=50_000)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--workload", choices=["two_hop", "planted_block", "batch", "serve"], default="two_hop",
-                    help="two_hop: the Chung-Lu self-join (default); planted_block: R and S are each a "
+    ap.add_argument("--workload", choices=["two_hop", "project_small", "aggregate", "planted_block", "batch", "serve"],
+                    default="two_hop",
+                    help="two_hop: the Chung-Lu self-join (default); project_small: 1M x 1M Zipf(1.0) "
+                         "join-project; aggregate: 10M x 10M Zipf(1.2) COUNT join-aggregate (fingerprint); "
+                         "planted_block: R and S are each a "
                          "--block-keys square block of (attr, key) cells at --block-density (the dense block "
                          "of the served stream's large requests), for the dense-part comparison; batch: "
                          "--instances independent Zipf join-projects in one batched launch against the same "
-                         "instances one query at a time")
+                         "instances one query at a time; serve: the served request stream")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
     ap.add_argument("--load", type=float, default=0.7,
@@ -432,6 +575,8 @@ def main():
         return batch_bench(args, d, torch)
     if args.workload == "serve":
         return serve_bench(args, d, torch)
+    if args.workload in ("project_small", "aggregate"):
+        return config_bench(args, d, torch)
 
     t0 = time.perf_counter()
     planted = args.workload == "planted_block"
@@ -483,7 +628,7 @@ def main():
     value = n_out * args.steps / elapsed
     exp_ms = float(np.mean([r.stats.ms_expand for r in ress]))
     row_bytes = 8 if args.mode == "materialize" else 0
-    b_kernel = expand_alg_bytes(st, n_out, row_bytes)
+    b_kernel = expand_alg_bytes(st, n_out, row_bytes, z_lookup(E2))
     achieved = b_kernel / (exp_ms * 1e-3) / 1e9
     log("stats: " + json.dumps({k: (round(v, 4) if isinstance(v, float) else v)
                                 for k, v in st.as_dict().items()}))

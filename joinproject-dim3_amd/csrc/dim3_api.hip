@@ -45,6 +45,8 @@ enum SlotId {
     WS_SLOT_SZ,
     WS_PARTIALS,
     WS_SCALARS,
+    WS_AGG_KR,
+    WS_AGG_GS,
     WS_ID2X,
     WS_ID2Z,
     WS_X2Z,
@@ -108,7 +110,8 @@ struct QueryPlan {
     uint64_t nR = 0, nS = 0;
     bool swapped = false;
     bool empty = true;
-    bool count_mode = false;
+    bool count_mode = false;  // expansion with counters (false also for a COUNT fingerprint, see agg_fp)
+    bool agg_fp = false;      // COUNT fingerprint: groups from the project expansion, sum2 / csum from path sums
     uint32_t flags = 0;
     uint32_t nx = 0, ny = 0, nz = 0;
     uint32_t nnzR = 0, nnzS = 0;
@@ -125,6 +128,7 @@ struct QueryPlan {
     uint32_t bmin = 0;           // heaviness bin threshold
     uint32_t K = 0, Kp = 0, Mh = 0, Nh = 0, dwords = 0;
     bool all_z = false;  // every z ID is a product column
+    bool fused = false;  // bit-or product read from the key bitmaps by the seeding (no product rows)
     // dense core: rows / columns with at least thx / thz heavy entries
     bool core_try = false;     // analyse the core of the keys with bin >= bc
     uint32_t bc = 0;
@@ -349,24 +353,30 @@ double dense_cost(const DenseModel& m, const DenseShape& d, int mode, bool count
     const uint64_t dwords = (d.Nh + 31) / 32;
     const double cells = (double)d.Mh * (double)d.Nh;
     const double set_bits = std::min(cells, (double)d.paths);
-    uint64_t bytes;
-    double t;
+    uint64_t bytes = 0;
+    double t = 0;
+    if (mode != DIM3_DENSE_MFMA && mode != DIM3_DENSE_BITOR) return -1;
     if (mode == DIM3_DENSE_MFMA) {
         if (d.Mh >= 65535ull * 128ull || d.Nh >= (1ull << 31)) return -1;
         if (count && Kp > 131072) return -1;  // int32 accumulator bound for multiplicities up to 127
         bytes = (d.Mh + d.Nh) * Kp + (count ? d.Mh * d.Nh * 4 : d.Mh * dwords * 4);
         t = 2.0 * cells * (double)Kp / m.mfma_ops + (double)((d.Mh + d.Nh) * Kp) / m.hbm_bps;
-    } else {
+    }
+    const bool words = !count && d.Nh * 10 >= d.nz * 9;
+    // bit-or over (nearly) all z IDs is fused into the seeding: the product
+    // rows are never written, each row tile ORs its keys' bitmap words
+    const bool fused = mode == DIM3_DENSE_BITOR && words;
+    if (mode == DIM3_DENSE_BITOR) {
         if (count) return -1;
-        bytes = d.K * dwords * 4 + d.Mh * dwords * 4;
+        bytes = d.K * dwords * 4 + (fused ? 0 : d.Mh * dwords * 4);
         t = (double)d.sumR * (double)dwords * 4.0 / m.l2_bps + (double)(d.K * dwords * 4) / m.hbm_bps;
     }
     if (bytes > mem_room) return -1;
-    // product written once, read back by the seeding, plus one LDS op per set
-    // entry (per word when the columns are nearly all z IDs: word copies)
-    const bool words = !count && d.Nh * 10 >= d.nz * 9;
-    t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps +
-         (words ? cells / 32.0 * m.seed_word_s : set_bits * m.seed_s) + m.launch_s;
+    // product written once and read back by the seeding (unless fused), plus
+    // one LDS op per set entry (per word when the columns are nearly all z
+    // IDs: word copies)
+    if (!fused) t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps;
+    t += (words ? cells / 32.0 * m.seed_word_s : set_bits * m.seed_s) + m.launch_s;
     if (bytes_out) *bytes_out = bytes;
     return t;
 }
@@ -536,6 +546,12 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
             const dim3_opts& o, hipStream_t st, int force_wgs = 0) {
     auto& P = c->bank().plan;
     P = QueryPlan();
+    // A COUNT fingerprint needs no counters: its groups are the join-project
+    // rows (n, sum) and its sum2 / csum are sums over the join paths, taken
+    // from the key degrees.  With the u != w filter the paths are counted
+    // per group instead.
+    P.agg_fp = count_mode && o.output_mode == DIM3_OUT_FINGERPRINT && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL);
+    if (P.agg_fp) count_mode = false;
     P.count_mode = count_mode;
     P.flags = o.flags;
     if (!(o.flags & DIM3_OPT_NO_SWAP) && nR < nS) {
@@ -624,6 +640,12 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     dim3h::build_y(dR, nR, P.my, degR, slotR, st);
     dim3h::probe_y(dS, nS, P.my, degR, degS, slotS, &sc->s_surv, st);
     dim3h::scan_y(degR, degS, (uint32_t)ny1, exY, part, &sc->tot_y, &sc->out_j, st);
+    if (P.agg_fp) {
+        uint64_t *kr, *gs;
+        TRY(ws(c, WS_AGG_KR, ny1, &kr));
+        TRY(ws(c, WS_AGG_GS, ny1, &gs));
+        dim3h::path_sums(dR, nR, slotR, dS, nS, slotS, (uint32_t)ny1, kr, gs, &sc->agg_sum2, st);
+    }
     dim3h::build_x(dR, nR, slotR, degS, P.mx, degX, slotRx, st);
     dim3h::scan_deg(degX, (uint32_t)nx1, exX, part, &sc->tot_x, st);
     if (!P.z_ident) {
@@ -671,6 +693,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     P.empty = false;
     plan_tiles(c, o, force_wgs);
     plan_dense(c, o);
+    P.fused = P.dense == DIM3_DENSE_BITOR && P.all_z;
     if ((uint64_t)P.ny * P.ntiles >= (1ull << 31))
         return fail(c, DIM3_ERR_INVALID, "tiled S index does not fit 32 bits (ny=%u, tiles=%u)", P.ny, P.ntiles);
 
@@ -714,6 +737,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
             HIPCHK(c, hipMemcpyAsync(c->h_hist, hh, 4ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
             HIPCHK(c, hipStreamSynchronize(st));
             const bool core = plan_core(c, o);
+            P.fused = P.dense == DIM3_DENSE_BITOR && P.all_z;
             if (!core && P.dense != DIM3_DENSE_OFF) {  // back to the phase-1 keys
                 HIPCHK(c, hipMemsetAsync(counters, 0, 8, st));
                 dim3h::dense_mark_keys(degR, degS, (uint32_t)ny1, exY, P.bmin, hid, h2y, counters, st);
@@ -767,7 +791,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         uint32_t *dbits = nullptr, *B = nullptr;
         int32_t* cint = nullptr;
         if (P.count_mode) TRY(ws(c, WS_CINT, (uint64_t)P.Mh * P.Nh, &cint));
-        else TRY(ws(c, WS_DBITS, (uint64_t)P.Mh * P.dwords, &dbits));
+        else if (!P.fused) TRY(ws(c, WS_DBITS, (uint64_t)P.Mh * P.dwords, &dbits));
         if (P.dense == DIM3_DENSE_MFMA) {
             // A [Mh][Kp] from R's CSR rows, Bt [Nh][Kp] from the heavy S tuples
             uint32_t* A;
@@ -786,8 +810,11 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
             dim3h::fill_u32(B, (uint64_t)P.K * P.dwords, 0, st);
             dim3h::dense_build_b(2, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY, hid, zflag, hzpos,
                                  0, P.dwords, B, overflow, st);
-            const int e = dim3h::bitor_rows(rowR, colR, hid, hx2x, P.Mh, B, P.dwords, dbits, st);
-            if (e != 0) return fail(c, DIM3_ERR_DEVICE, "dense bit-or launch: %s", hipGetErrorString((hipError_t)e));
+            if (!P.fused) {  // fused: the seeding reads B itself
+                const int e = dim3h::bitor_rows(rowR, colR, hid, hx2x, P.Mh, B, P.dwords, dbits, st);
+                if (e != 0)
+                    return fail(c, DIM3_ERR_DEVICE, "dense bit-or launch: %s", hipGetErrorString((hipError_t)e));
+            }
         }
         LAUNCHCHK(c, "dense");
     }
@@ -841,8 +868,14 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     if (P.core) a.splitS = (const uint32_t*)W[WS_CNT_SD].p;
     if (P.dense != DIM3_DENSE_OFF) {
         a.hx = (const int32_t*)W[WS_HX].p;
-        if (P.count_mode) a.cint = (const int32_t*)W[WS_CINT].p;
-        else a.dbits = (const uint32_t*)W[WS_DBITS].p;
+        if (P.count_mode) {
+            a.cint = (const int32_t*)W[WS_CINT].p;
+        } else if (P.fused) {
+            a.bkeys = (const uint32_t*)W[WS_DB].p;
+            a.hid = (const int32_t*)W[WS_HID].p;
+        } else {
+            a.dbits = (const uint32_t*)W[WS_DBITS].p;
+        }
         a.hz2zid = (const uint32_t*)W[WS_HZ2ZID].p;
         a.hz_tile = (const uint32_t*)W[WS_HZTILE].p;
         a.dwords = P.dwords;
@@ -888,8 +921,8 @@ void fill_result(dim3_ctx* c, dim3_result* res, const Scalars& s, bool timing) {
     res->swapped = P.swapped ? 1u : 0u;
     res->fp.count = res->n;
     res->fp.sum = P.empty ? 0 : s.fp_sum;
-    res->fp.sum2 = P.empty ? 0 : s.fp_sum2;
-    res->fp.csum = P.empty ? 0 : s.fp_csum;
+    res->fp.sum2 = P.empty ? 0 : P.agg_fp ? s.agg_sum2 : s.fp_sum2;
+    res->fp.csum = P.empty ? 0 : P.agg_fp ? P.out_j : s.fp_csum;
     dim3_stats& st = res->stats;
     memset(&st, 0, sizeof(st));
     st.n_r = P.nR;
@@ -1100,13 +1133,16 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     HIPCHK(c, hipStreamSynchronize(st));  // host vectors are pageable
     if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block);
     unsigned grid = 1;
-    int e = dim3h::expand_batch_setup(count_mode, block, lds, c->grid_cus(), std::max<uint64_t>(total, 1), &grid);
+    // the expansion kernel of every instance (a COUNT fingerprint without
+    // the u != w filter expands as a join-project)
+    const bool kcount = count_mode && !(fp && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL));
+    int e = dim3h::expand_batch_setup(kcount, block, lds, c->grid_cus(), std::max<uint64_t>(total, 1), &grid);
     if (e != 0) return fail(c, DIM3_ERR_DEVICE, "batch setup: %s", hipGetErrorString((hipError_t)e));
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    dim3h::expand_batch_launch(count_mode, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl, d_scout,
+    dim3h::expand_batch_launch(kcount, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl, d_scout,
                                st);
     hipMemcpyAsync(c->h_batch, d_scout, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st);
     HIPCHK(c, hipStreamEndCapture(st, &graph));
@@ -1264,9 +1300,11 @@ int autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
     const double words = (double)X * ((Z + 31) / 32);
     // seeding copies every word of the product; the emission is that of the 1-path query
     if (pm.dense == DIM3_DENSE_MFMA && em > e1) m.seed_word_s = (em - e1) * 1e-3 / words;
-    if (pb.dense == DIM3_DENSE_BITOR && db > 0) {
+    if (pb.dense == DIM3_DENSE_BITOR) {
         const double reads = (double)X * 32.0 * (double)pb.dwords * 4.0;  // each row ORs 32 key bitmaps
-        m.l2_bps = reads / (db * 1e-3);
+        // fused into the seeding: the expansion's extra time less the word stores
+        const double t = pb.fused ? (eb - e1) * 1e-3 - words * m.seed_word_s : db * 1e-3;
+        if (t > 0) m.l2_bps = reads / t;
     }
     c->model = m;
     c->model_source = 1;

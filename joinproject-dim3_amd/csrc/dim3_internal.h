@@ -44,6 +44,21 @@ __host__ __device__ __forceinline__ uint64_t mix64(uint64_t k) {
     return k;
 }
 
+// Row fingerprint factors.  A row (x, z) with count c adds k1(x) g1(z) to
+// sum and k2(x) c g2(z) to sum2 (mod 2^64):
+//   g1 = fmix32(u(z) ^ 0x243F6A88), g2 = (g1 * 0x2545F491) ^ (g1 >> 16),
+//   k1 = mix64(u(x) ^ 0x243F6A8885A308D3) | 1, k2 = mix64(u(x) ^ 0x13198A2E03707344) | 1.
+__host__ __device__ __forceinline__ void zfactors(int32_t z, uint32_t& g1, uint32_t& g2) {
+    g1 = hash32((uint32_t)z ^ 0x243F6A88u);
+    g2 = (g1 * 0x2545F491u) ^ (g1 >> 16);
+}
+__host__ __device__ __forceinline__ uint64_t xfactor1(int32_t x) {
+    return mix64((uint64_t)(uint32_t)x ^ 0x243F6A8885A308D3ull) | 1ull;
+}
+__host__ __device__ __forceinline__ uint64_t xfactor2(int32_t x) {
+    return mix64((uint64_t)(uint32_t)x ^ 0x13198A2E03707344ull) | 1ull;
+}
+
 __device__ __forceinline__ uint64_t row_key(int32_t x, int32_t z) {
     return ((uint64_t)(uint32_t)x << 32) | (uint64_t)(uint32_t)z;
 }
@@ -194,6 +209,25 @@ __device__ __forceinline__ void wave_max_update(uint32_t* arr, uint32_t key, uin
     if (((pending >> lane_id()) & 1ull) && v > arr[key]) atomicMax(&arr[key], v);
 }
 
+// As wave_count_add for 64-bit sums (mod 2^64): the lanes sharing the first
+// pending lane's key (two rounds) send one atomicAdd of their total.
+__device__ __forceinline__ void wave_add_u64(uint64_t* arr, uint32_t key, uint64_t v, bool valid) {
+    uint64_t pending = __ballot(valid);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        if (pending == 0) return;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)leader);
+        const bool mine = valid && key == lk && ((pending >> lane_id()) & 1ull);
+        uint64_t m = mine ? v : 0ull;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) m += (uint64_t)__shfl_xor((unsigned long long)m, d, 64);
+        if (lane_id() == leader) atomicAdd((unsigned long long*)&arr[lk], (unsigned long long)m);
+        pending &= ~__ballot(mine);
+    }
+    if ((pending >> lane_id()) & 1ull) atomicAdd((unsigned long long*)&arr[key], (unsigned long long)v);
+}
+
 // Exclusive scan over the block; sh needs BLOCK/64+1 entries.  Ends with a
 // barrier so sh can be reused right away.
 template <typename T>
@@ -267,6 +301,7 @@ struct Scalars {
     uint64_t nxh, nzh;
     uint64_t s_surv;     // S tuples whose y joined
     int32_t mm[8];       // min/max of R.x, R.y, S.z, S.y
+    uint64_t agg_sum2;   // COUNT fingerprint from the join paths: sum of k2(x) g2(z) over paths
     uint64_t spare[4];
 };
 
@@ -316,6 +351,10 @@ void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, 
 void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* ex, uint64_t* partials,
             uint64_t* tot, uint64_t* out_j, hipStream_t st);
 void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* out_j, hipStream_t st);
+// *out += sum over y slots of KR(y) GS(y): the path-sum term of the COUNT
+// fingerprint (kr, gs: nslots u64 scratch each)
+void path_sums(const int2* R, uint64_t nR, const uint32_t* slotR, const int2* S, uint64_t nS, const int32_t* slotS,
+               uint32_t nslots, uint64_t* kr, uint64_t* gs, uint64_t* out, hipStream_t st);
 // x / z mapping over the tuples whose y joined
 void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
              uint32_t* degX, int32_t* slotRx, hipStream_t st);
@@ -378,6 +417,11 @@ struct ExpandArgs {
     const uint32_t* hz_tile;  // first column of each z tile, ntiles + 1 entries
     uint32_t dwords;
     uint32_t Nh;
+    // fused bit-or seed (project, every z ID a product column): a product
+    // row's tile is the OR of its heavy keys' bitmaps, read here instead of
+    // a materialised product row
+    const uint32_t* bkeys;    // K x dwords key bitmaps over z IDs (nullptr: seed from dbits / cint)
+    const int32_t* hid;       // heavy index of each y ID
     // fingerprint word totals (project fp_mode)
     const ulonglong2* gw;     // z-factor sums per tile word
     const uint32_t* gwm;      // valid-bit mask per tile word

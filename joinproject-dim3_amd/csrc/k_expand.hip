@@ -104,13 +104,6 @@ __device__ __forceinline__ uint32_t swz_word(uint32_t zl, uint32_t W, uint32_t C
     return q >= W ? q - W : q;
 }
 
-// z factors of the row fingerprint: g1 = fmix32(u(z) ^ 0x243F6A88),
-// g2 = (g1 * 0x2545F491) ^ (g1 >> 16)
-__device__ __forceinline__ void zfactors(int32_t z, uint32_t& g1, uint32_t& g2) {
-    g1 = hash32((uint32_t)z ^ 0x243F6A88u);
-    g2 = (g1 * 0x2545F491u) ^ (g1 >> 16);
-}
-
 __device__ __forceinline__ uint32_t swz_offset(uint32_t word, uint32_t r, uint32_t W, uint32_t C) {
     int32_t q = (int32_t)word - (int32_t)(r * C);
     if (q < 0) q += (int32_t)W;
@@ -179,7 +172,44 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const int32_t hxr = a.hx != nullptr ? a.hx[x] : -1;
     if (hxr >= 0) {
         const uint32_t h0 = a.hz_tile[t], h1 = a.hz_tile[t + 1];
-        if (COUNT) {
+        if (!COUNT && a.bkeys != nullptr) {
+            // fused bit-or: the OR of the row's heavy key bitmaps (plain
+            // layout, every z ID a column: tile word q is bitmap word t W + q).
+            // The heavy keys are listed BLOCK entries at a time.
+            uint32_t* s_hk = L.area;
+            const uint32_t wb = t * W;
+            const uint32_t wn = wb < a.dwords ? min(W, a.dwords - wb) : 0u;
+            for (uint32_t c0 = rb; c0 < re; c0 += BLOCK) {
+                const uint32_t i = c0 + tid;
+                int32_t h = -1;
+                if (i < re) {
+                    const int32_t craw = a.colR[i];
+                    if (craw < 0) h = a.hid[(uint32_t)craw & 0x7FFFFFFFu];
+                }
+                uint32_t nh;
+                const uint32_t pos = block_exclusive_scan<BLOCK, uint32_t>(h >= 0 ? 1u : 0u, nh, sh32);
+                if (h >= 0) s_hk[pos] = (uint32_t)h;
+                __syncthreads();
+                for (uint32_t q0 = tid; q0 < wn; q0 += 4u * BLOCK) {
+                    uint32_t acc[4] = {0u, 0u, 0u, 0u};
+                    for (uint32_t j = 0; j < nh; j++) {
+                        const uint32_t* src = a.bkeys + (size_t)s_hk[j] * a.dwords + wb;
+                        // clamped, unconditional loads: four words in flight
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++) {
+                            const uint32_t q = q0 + u * BLOCK;
+                            acc[u] |= src[q < wn ? q : q0];
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; u++) {
+                        const uint32_t q = q0 + u * BLOCK;
+                        if (q < wn && acc[u] != 0u) tile[q] |= acc[u];
+                    }
+                }
+                __syncthreads();
+            }
+        } else if (COUNT) {
             // distinct columns are distinct counters of a zeroed tile
             const int32_t* crow = a.cint + (size_t)hxr * a.Nh;
             for (uint32_t h = h0 + tid; h < h1; h += BLOCK) {
@@ -405,9 +435,8 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
             }
         }
         if (S1 | S2) {
-            const uint64_t ux = (uint64_t)(uint32_t)xv;
-            fsum += (mix64(ux ^ 0x243F6A8885A308D3ull) | 1ull) * S1;
-            fsum2 += (mix64(ux ^ 0x13198A2E03707344ull) | 1ull) * S2;
+            fsum += xfactor1(xv) * S1;
+            fsum2 += xfactor2(xv) * S2;
         }
         if (tid == 0 && T) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
         __syncthreads();

@@ -479,6 +479,36 @@ __global__ void k_grid_rel(int2* __restrict__ out, uint32_t X, uint32_t Y) {
     out[i] = make_int2((int32_t)(i / Y), (int32_t)(i % Y));
 }
 
+// COUNT fingerprint terms that are sums over join paths: sum2 = sum over
+// paths (x, y, z) of k2(x) g2(z) = sum over y of KR(y) GS(y), with
+// KR(y) = sum of k2 over R(y) and GS(y) = sum of g2 over S(y) (mod 2^64;
+// tuples counted with multiplicity, as the COUNT is).
+__global__ __launch_bounds__(kThreads) void k_path_kr(const int2* __restrict__ R, uint64_t nR,
+                                                      const uint32_t* __restrict__ slotR, uint64_t* kr) {
+    const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    const bool ok = i < nR;
+    const uint32_t s = ok ? slotR[i] : 0u;
+    const uint64_t v = ok ? dim3d::xfactor2(R[i].x) : 0ull;
+    wave_add_u64(kr, s, v, ok);
+}
+__global__ __launch_bounds__(kThreads) void k_path_gs(const int2* __restrict__ S, uint64_t nS,
+                                                      const int32_t* __restrict__ slotS, uint64_t* gs) {
+    const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int32_t s = i < nS ? slotS[i] : -1;
+    uint32_t g1 = 0, g2 = 0;
+    if (s >= 0) dim3d::zfactors(S[i].x, g1, g2);
+    wave_add_u64(gs, s >= 0 ? (uint32_t)s : 0u, (uint64_t)g2, s >= 0);
+}
+__global__ __launch_bounds__(kThreads) void k_path_dot(const uint64_t* __restrict__ kr,
+                                                       const uint64_t* __restrict__ gs, uint32_t n, uint64_t* out) {
+    __shared__ uint64_t sh[kThreads / 64 + 1];
+    uint64_t acc = 0;
+    for (uint64_t s = (uint64_t)blockIdx.x * kThreads + threadIdx.x; s < n; s += (uint64_t)gridDim.x * kThreads)
+        acc += kr[s] * gs[s];
+    acc = block_reduce_sum<kThreads>(acc, sh);
+    if (threadIdx.x == 0 && acc) atomicAdd((unsigned long long*)out, (unsigned long long)acc);
+}
+
 }  // namespace
 
 namespace dim3h {
@@ -525,6 +555,16 @@ void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint
     unsigned g = grid_for(nslots);
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(k_outj, dim3(g), dim3(kThreads), 0, st, degR, degS, nslots, out_j);
+}
+void path_sums(const int2* R, uint64_t nR, const uint32_t* slotR, const int2* S, uint64_t nS, const int32_t* slotS,
+               uint32_t nslots, uint64_t* kr, uint64_t* gs, uint64_t* out, hipStream_t st) {
+    fill_u64(kr, nslots, 0, st);
+    fill_u64(gs, nslots, 0, st);
+    if (nR) hipLaunchKernelGGL(k_path_kr, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, kr);
+    if (nS) hipLaunchKernelGGL(k_path_gs, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, gs);
+    unsigned g = grid_for(nslots);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_path_dot, dim3(g), dim3(kThreads), 0, st, kr, gs, nslots, out);
 }
 void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
              uint32_t* degX, int32_t* slotRx, hipStream_t st) {

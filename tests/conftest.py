@@ -25,3 +25,10 @@ def ctx(dim3):
     c = dim3.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def torch():
+    import torch as _torch
+
+    return _torch

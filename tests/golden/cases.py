@@ -73,6 +73,35 @@ MEDIUM = {
 }
 
 
+# config-scale cases: the full 1M x 1M join-project (fingerprint from the
+# oracle: the reference would materialise 6.9e9 pairs) and a z slice of the
+# 10M x 10M COUNT join-aggregate (reference and oracle)
+LARGE = {
+    "project_full": {"gen": "project_small", "n": 1_000_000, "ac": 100_000, "b": 10_000, "s": 1.0,
+                     "seed_r": 1, "seed_s": 2, "checker": "oracle"},
+    "aggregate_zslice": {"gen": "aggregate_zslice", "n": 10_000_000, "ac": 2_000_000, "b": 1_000_000, "s": 1.2,
+                         "seed_r": 1, "seed_s": 2, "mod": 4096, "r": 0, "checker": "reference",
+                         "project": False, "aggregate": True},
+}
+
+
+def make_large_full(spec):
+    """The unsliced relations of a LARGE case."""
+    wl = _wl()
+    if spec["gen"] == "project_small":
+        return wl.project_small(spec["seed_r"], spec["seed_s"], spec["n"], spec["ac"], spec["b"], spec["s"])
+    if spec["gen"] == "aggregate_zslice":
+        return wl.aggregate_large(spec["seed_r"], spec["seed_s"], spec["n"], spec["ac"], spec["b"], spec["s"])
+    raise ValueError(spec["gen"])
+
+
+def slice_large(spec, R, S):
+    """A LARGE case's inputs from its unsliced relations."""
+    if spec["gen"] == "aggregate_zslice":
+        return R, S[(S[:, 0] % spec["mod"]) == spec["r"]]
+    return R, S
+
+
 def make_medium(spec):
     g = spec["gen"]
     if g == "gen_rand_data":

@@ -7,6 +7,7 @@ small_*.npz   inputs R, S and the reference's sorted join-project pairs and
 medium.json   inputs regenerated from a named seeded generator (with a
               sha256 of the inputs to detect generator drift), and the count
               and fingerprint of the reference's output.
+large.json    the same for the config-scale cases (python make_golden.py large).
 """
 import hashlib
 import json
@@ -62,6 +63,36 @@ def medium():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def large():
+    """Config-scale fingerprints.  checker "oracle": the C restatement in
+    fingerprint mode (no materialised output); "reference": the compiled
+    reference, cross-checked by the oracle."""
+    out = {}
+    for name, spec in cases.LARGE.items():
+        R, S = cases.slice_large(spec, *cases.make_large_full(spec))
+        rec = {"spec": spec, "sha256": sha(R, S), "n_r": int(R.shape[0]), "n_s": int(S.shape[0])}
+        if spec.get("project", True):
+            _, fp, _ = oracle.join_project(R, S, fingerprint_only=True)
+            n = fp.as_tuple()[0]
+            if spec["checker"] == "reference":
+                p = oracle.ref_join_project(R, S, parallel=True)
+                assert oracle.fingerprint(p).as_tuple() == fp.as_tuple(), name
+                del p
+            rec["project"] = {"n": int(n), "fp": [str(v) for v in fp.as_tuple()]}
+        if spec.get("aggregate", False):
+            _, _, fp, _ = oracle.join_aggregate_count(R, S, fingerprint_only=True)
+            n = fp.as_tuple()[0]
+            if spec["checker"] == "reference":
+                cp, cc = oracle.ref_join_aggregate_count(R, S)
+                assert oracle.fingerprint(cp, cc).as_tuple() == fp.as_tuple(), name
+                del cp, cc
+            rec["aggregate"] = {"n": int(n), "fp": [str(v) for v in fp.as_tuple()]}
+        out[name] = rec
+        print(name, json.dumps({k: v for k, v in rec.items() if k != "spec"}), flush=True)
+    with open(os.path.join(HERE, "large.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("reference checker missing: make -C oracle ref")
@@ -70,3 +101,5 @@ if __name__ == "__main__":
         small()
     if "medium" in which:
         medium()
+    if "large" in which:
+        large()
