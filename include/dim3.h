@@ -72,14 +72,17 @@ typedef struct dim3_opts {
 } dim3_opts;
 
 /* Order-independent digest of a row set, with u(v) = (u32)v, mix64 the
- * splitmix64 finaliser and fmix32 the MurmurHash3 32-bit finaliser:
+ * splitmix64 finaliser and fmix32 the MurmurHash3 32-bit finaliser, all
+ * arithmetic mod 2^64:
  *   k1(x) = mix64(u(x) ^ 0x243F6A8885A308D3) | 1, k2(x) = mix64(u(x) ^ 0x13198A2E03707344) | 1,
- *   g1(z) = fmix32(u(z) ^ 0x243F6A88), g2(z) = (g1(z) * 0x2545F491 mod 2^32) ^ (g1(z) >> 16);
- *   per project row (x, z):   count += 1, sum += k1(x) g1(z), sum2 += k2(x) g2(z)   (mod 2^64);
- *   per COUNT row (x, z, c):  count += 1, sum += k1(x) g1(z), sum2 += k2(x) c g2(z), csum += c.
+ *   g(z)  = a(u(z) >> 5) c(u(z) & 31), a(w) = fmix32(w ^ 0x243F6A88) | 1,
+ *           c(b) = mix64(0x9E3779B97F4A7C15 (b + 1)) | 1;
+ *   per project row (x, z):   count += 1, sum += k1(x) g(z), sum2 += k2(x) g(z);
+ *   per COUNT row (x, z, c):  count += 1, sum += k1(x) g(z), sum2 += k2(x) c g(z), csum += c.
  * Every term factors into an x part and a z part, so the device sums the z
- * parts of a row (or, for a mostly full word, subtracts its missing ones
- * from the word's total) and multiplies once per row. */
+ * parts of a row and multiplies once per row; the z parts of one 32-aligned
+ * word of z values share a(w), so a bitmap word sums with four byte-table
+ * lookups.  A COUNT's sum2 and csum are sums over join paths. */
 typedef struct dim3_fingerprint {
     uint64_t count;
     uint64_t sum;

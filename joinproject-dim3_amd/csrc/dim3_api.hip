@@ -79,8 +79,6 @@ enum SlotId {
     WS_DB,
     WS_DBITS,
     WS_CINT,
-    WS_FP_GW,
-    WS_FP_GWM,
     WS_BATCH_ARGS,
     WS_BATCH_START,
     WS_BATCH_SCL,
@@ -272,7 +270,9 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
     const uint32_t per_word = cnt ? 1u : 32u;  // z IDs per LDS word
     P.touched_cap = cnt ? 4096u : 2048u;
     const uint64_t nz_words = ((uint64_t)P.nz + per_word - 1) / per_word;
-    const uint64_t lds_words = c->max_lds / 4u - 256u;  // static shared + slack
+    // static shared + slack, and the fingerprint table of fingerprint mode
+    const uint64_t fp_words = o.output_mode == DIM3_OUT_FINGERPRINT ? 2u * dim3d::kFpTableU64 : 0u;
+    const uint64_t lds_words = c->max_lds / 4u - 256u;
     struct Cand {
         int block;
         uint32_t ntiles, tile_words, stage;
@@ -282,7 +282,7 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
     // candidates: 1, 2 or 4 workgroups per CU (16 wavefronts per CU in all)
     auto make = [&](int wgs, int block, uint64_t forced_units) {
         Cand k{block, 1, 0, 0, false, wgs};
-        const uint64_t budget = lds_words / (uint64_t)wgs;
+        const uint64_t budget = lds_words / (uint64_t)wgs - fp_words;
         const uint64_t min_stage = std::max<uint64_t>(1024u, 3u * (uint64_t)block + 1u);
         if (budget <= P.touched_cap + min_stage + 1) return k;
         uint64_t max_words = budget - P.touched_cap - min_stage;
@@ -583,11 +583,16 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     const int32_t* mm = c->h_sc->mm;
     const bool hash_only = (o.flags & DIM3_OPT_HASH_ONLY) != 0;
     auto span = [](int32_t lo, int32_t hi) { return (uint64_t)((int64_t)hi - (int64_t)lo + 1); };
-    const uint64_t xr = span(mm[0], mm[1]), yr = span(mm[2], mm[3]), zr = span(mm[4], mm[5]);
+    const uint64_t xr = span(mm[0], mm[1]), yr = span(mm[2], mm[3]);
+    uint64_t zr = span(mm[4], mm[5]);
     P.x_range = !hash_only && xr <= 2 * nR + 64;
     P.y_range = !hash_only && yr <= 2 * nR + 64;
     P.z_ident = !hash_only && zr <= 2 * nS + 64;
-    P.zlo = P.z_ident ? mm[4] : 0;
+    // dense z IDs start at a multiple of 32 below min(z), so that a word of
+    // a project tile is one 32-aligned word of z values (the fingerprint
+    // sums such a word at once); the IDs below min(z) stay empty
+    P.zlo = P.z_ident ? (int32_t)((uint32_t)mm[4] & ~31u) : 0;
+    if (P.z_ident) zr += (uint64_t)((int64_t)mm[4] - (int64_t)P.zlo);
 
     uint32_t *keysY = nullptr, *degR, *degS, *slotR, *keysX = nullptr, *degX, *keysZ = nullptr, *degZ = nullptr;
     int32_t *slotS, *slotRx, *slotSz = nullptr;
@@ -614,7 +619,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     };
     TRY(make_map(P.y_range, false, mm[2], yr, nR, WS_KEYS_Y, &keysY, &P.my));
     TRY(make_map(P.x_range, false, mm[0], xr, nR, WS_KEYS_X, &keysX, &P.mx));
-    TRY(make_map(P.z_ident, true, mm[4], zr, nS, WS_KEYS_Z, &keysZ, &P.mz));
+    TRY(make_map(P.z_ident, true, P.z_ident ? P.zlo : mm[4], zr, nS, WS_KEYS_Z, &keysZ, &P.mz));
     const uint64_t ny1 = P.my.nslots, nx1 = P.mx.nslots, nz1 = P.mz.nslots;
     TRY(ws(c, WS_DEGR_Y, ny1, &degR));
     TRY(ws(c, WS_DEGS_Y, ny1, &degS));
@@ -819,16 +824,6 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         LAUNCHCHK(c, "dense");
     }
     HIPCHK(c, hipEventRecord(c->ev[6], st));
-    if (o.output_mode == DIM3_OUT_FINGERPRINT && !P.count_mode) {
-        ulonglong2* gw;
-        uint32_t* gwm;
-        const uint64_t nw = (uint64_t)P.ntiles * P.tile_words;
-        TRY(ws(c, WS_FP_GW, nw, &gw));
-        TRY(ws(c, WS_FP_GWM, nw, &gwm));
-        dim3h::fp_tables(P.z_ident ? nullptr : id2z, P.zlo, P.nz, P.ntiles, P.tile_words, P.tile_size,
-                         (P.dense == DIM3_DENSE_OFF || !P.all_z) ? 1u : 0u, gw, gwm, st);
-        LAUNCHCHK(c, "fingerprint word totals");
-    }
     return DIM3_OK;
 }
 
@@ -840,6 +835,7 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     dim3h::ExpandArgs a;
     memset(&a, 0, sizeof(a));
     a.sc = (Scalars*)W[WS_SCALARS].p;
+    a.fp_mode = fp_mode ? 1u : 0u;
     if (P.empty) return a;  // nx = 0: no tasks
     a.rowptrR = (const uint32_t*)W[WS_ROWPTR_R].p;
     a.colR = (const int32_t*)W[WS_COL_R].p;
@@ -855,16 +851,11 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.tile_words = P.tile_words;
     a.touched_cap = P.touched_cap;
     a.stage_cap = P.stage_cap;
-    a.fp_mode = fp_mode ? 1u : 0u;
     // plain bit layout only where the dense seed copies whole words (every z
     // ID a product column); elsewhere the swizzle keeps hub z off one word
     a.swz = (P.dense == DIM3_DENSE_OFF || !P.all_z) ? 1u : 0u;
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
-    if (fp_mode && !P.count_mode) {
-        a.gw = (const ulonglong2*)W[WS_FP_GW].p;
-        a.gwm = (const uint32_t*)W[WS_FP_GWM].p;
-    }
     if (P.core) a.splitS = (const uint32_t*)W[WS_CNT_SD].p;
     if (P.dense != DIM3_DENSE_OFF) {
         a.hx = (const int32_t*)W[WS_HX].p;
@@ -1105,7 +1096,7 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
         start[i] = (uint32_t)total;
         if (!P.empty) {
             total += (uint64_t)P.nx * P.ntiles;
-            lds = std::max(lds, dim3h::expand_lds_bytes(P.tile_words, P.touched_cap, P.stage_cap, (uint32_t)block));
+            lds = std::max(lds, dim3h::expand_lds_bytes(P.tile_words, P.touched_cap, P.stage_cap, (uint32_t)block, fp));
         }
         if (total >= (1ull << 32)) return fail(c, DIM3_ERR_INVALID, "batch holds more than 2^32 row tasks");
     }
@@ -1131,7 +1122,7 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     HIPCHK(c, hipMemcpyAsync(d_start, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(d_scl, scl.data(), sizeof(Scalars*) * n, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipStreamSynchronize(st));  // host vectors are pageable
-    if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block);
+    if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block, fp);
     unsigned grid = 1;
     // the expansion kernel of every instance (a COUNT fingerprint without
     // the u != w filter expands as a join-project)
@@ -1142,7 +1133,7 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    dim3h::expand_batch_launch(kcount, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl, d_scout,
+    dim3h::expand_batch_launch(kcount, fp, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl, d_scout,
                                st);
     hipMemcpyAsync(c->h_batch, d_scout, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st);
     HIPCHK(c, hipStreamEndCapture(st, &graph));

@@ -44,14 +44,24 @@ __host__ __device__ __forceinline__ uint64_t mix64(uint64_t k) {
     return k;
 }
 
-// Row fingerprint factors.  A row (x, z) with count c adds k1(x) g1(z) to
-// sum and k2(x) c g2(z) to sum2 (mod 2^64):
-//   g1 = fmix32(u(z) ^ 0x243F6A88), g2 = (g1 * 0x2545F491) ^ (g1 >> 16),
-//   k1 = mix64(u(x) ^ 0x243F6A8885A308D3) | 1, k2 = mix64(u(x) ^ 0x13198A2E03707344) | 1.
-__host__ __device__ __forceinline__ void zfactors(int32_t z, uint32_t& g1, uint32_t& g2) {
-    g1 = hash32((uint32_t)z ^ 0x243F6A88u);
-    g2 = (g1 * 0x2545F491u) ^ (g1 >> 16);
+// Row fingerprint factors.  A row (x, z) with count c adds k1(x) g(z) to
+// sum and k2(x) c g(z) to sum2 (mod 2^64, u(v) = (u32)v):
+//   k1 = mix64(u(x) ^ 0x243F6A8885A308D3) | 1, k2 = mix64(u(x) ^ 0x13198A2E03707344) | 1,
+//   g(z) = a(u(z) >> 5) * c(u(z) & 31), a(w) = fmix32(w ^ 0x243F6A88) | 1,
+//   c(b) = mix64(0x9E3779B97F4A7C15 * (b + 1)) | 1.
+// The z factors of one 32-aligned word of z values share a(w), so a word of
+// a dense z range sums as a(w) times the position factors of its set bits.
+__host__ __device__ __forceinline__ uint32_t zword_factor(uint32_t w) { return hash32(w ^ 0x243F6A88u) | 1u; }
+__host__ __device__ __forceinline__ uint64_t zpos_factor(uint32_t b) {
+    return mix64(0x9E3779B97F4A7C15ull * (uint64_t)(b + 1u)) | 1ull;
 }
+__host__ __device__ __forceinline__ uint64_t zfactor(int32_t z) {
+    const uint32_t u = (uint32_t)z;
+    return (uint64_t)zword_factor(u >> 5) * zpos_factor(u & 31u);
+}
+// LDS table of the fingerprint emission: [32] position factors, then
+// [4][256] sums of the position factors of the set bits of byte j of a word
+constexpr uint32_t kFpTableU64 = 32 + 4 * 256;
 __host__ __device__ __forceinline__ uint64_t xfactor1(int32_t x) {
     return mix64((uint64_t)(uint32_t)x ^ 0x243F6A8885A308D3ull) | 1ull;
 }
@@ -301,7 +311,7 @@ struct Scalars {
     uint64_t nxh, nzh;
     uint64_t s_surv;     // S tuples whose y joined
     int32_t mm[8];       // min/max of R.x, R.y, S.z, S.y
-    uint64_t agg_sum2;   // COUNT fingerprint from the join paths: sum of k2(x) g2(z) over paths
+    uint64_t agg_sum2;   // COUNT fingerprint from the join paths: sum of k2(x) g(z) over paths
     uint64_t spare[4];
 };
 
@@ -422,9 +432,6 @@ struct ExpandArgs {
     // a materialised product row
     const uint32_t* bkeys;    // K x dwords key bitmaps over z IDs (nullptr: seed from dbits / cint)
     const int32_t* hid;       // heavy index of each y ID
-    // fingerprint word totals (project fp_mode)
-    const ulonglong2* gw;     // z-factor sums per tile word
-    const uint32_t* gwm;      // valid-bit mask per tile word
 };
 
 // returns 0 or a hipError_t
@@ -468,13 +475,10 @@ void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t*
                    const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
                    hipStream_t st);
 
-// fingerprint word totals of one project plan: gw / gwm[ntiles * tile_words]
-void fp_tables(const int32_t* id2z, int32_t zlo, uint32_t nz, uint32_t ntiles, uint32_t tile_words,
-               uint32_t tile_size, uint32_t swz, ulonglong2* gw, uint32_t* gwm, hipStream_t st);
-
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
-size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block);
+// dynamic LDS of one expansion workgroup (fp: plus the fingerprint table)
+size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp);
 
 // Batched expansion: one persistent launch over the tasks of n_inst
 // instances (args[i], tasks task_start[i] .. task_start[i + 1]).
@@ -482,7 +486,7 @@ size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stag
 // runs before a graph capture); launch: scalar reset, expansion, gather of
 // every instance's scalars into sc_out[n_inst] (kernels only).
 int expand_batch_setup(bool count, int block, size_t lds, int cu_count, uint64_t total_tasks, unsigned* grid);
-void expand_batch_launch(bool count, int block, size_t lds, unsigned grid, const ExpandArgs* args,
+void expand_batch_launch(bool count, bool fp, int block, size_t lds, unsigned grid, const ExpandArgs* args,
                          const uint32_t* task_start, uint32_t n_inst, uint32_t* counter, Scalars* const* sc_list,
                          Scalars* sc_out, hipStream_t st);
 

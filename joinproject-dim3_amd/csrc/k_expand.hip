@@ -135,6 +135,7 @@ __device__ __forceinline__ uint32_t units_of(uint32_t v) {
 // One (x, z-tile) task: dense seed, expansion of the light paths into the
 // LDS tile, then emission (rows or fingerprint) and the clear of the tile.
 struct Lds {
+    const uint64_t* fpt;  // fingerprint table (fp mode): kFpTableU64 entries
     uint32_t* tile;
     uint32_t* touched;
     uint32_t* area;
@@ -395,44 +396,51 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const int32_t* id2z = a.id2z + zbase;
 
     if (a.fp_mode) {
-        // Fingerprint: sum the z factors of the row's rows (a word with more
-        // than 16 bits starts from its precomputed word total and subtracts
-        // the missing ones), then one multiply by the row's x factors.
+        // Fingerprint: sum the z factors of the row's rows, then one multiply
+        // by each x factor.  A plain-layout word of a dense z range is one
+        // 32-aligned word of z values (zlo and the tile base are multiples
+        // of 32): its word factor times the byte-table sums of its position
+        // factors.  Otherwise (swizzled layout, hashed z, counters) each row
+        // takes its own z factor.
+        const uint64_t* fpt = L.fpt;
         uint64_t S1 = 0, S2 = 0;
-        const uint32_t wbase = t * W;
-        for (uint32_t i = wlo + lane; i < whi; i += 64u) {
-            const uint32_t w = list_at(touched, listed, i);
-            uint32_t v = tile[w];
-            tile[w] = 0;
-            if (COUNT) {
+        if (!COUNT && ZID && C == 0u) {
+            const uint32_t wv0 = zoff >> 5;
+            for (uint32_t i = wlo + lane; i < whi; i += 64u) {
+                const uint32_t w = list_at(touched, listed, i);
+                const uint32_t v = tile[w];
+                tile[w] = 0;
                 if (v) {
-                    uint32_t g1, g2;
-                    zfactors(ZID ? (int32_t)(zoff + w) : id2z[w], g1, g2);
-                    S1 += g1;
-                    S2 += (uint64_t)v * g2;
-                    fcs += v;
+                    const uint64_t ps = fpt[32u + (v & 255u)] + fpt[288u + ((v >> 8) & 255u)] +
+                                        fpt[544u + ((v >> 16) & 255u)] + fpt[800u + (v >> 24)];
+                    S1 += (uint64_t)zword_factor(wv0 + w) * ps;
                 }
-            } else if (v) {
-                uint64_t s1 = 0, s2 = 0;
-                const bool full = __popc(v) > 16;
-                if (full) v = ~v & a.gwm[wbase + w];
-                while (v) {
-                    const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                    v &= v - 1u;
-                    const uint32_t e = swz_offset(w, b, W, C);
-                    uint32_t g1, g2;
-                    zfactors(ZID ? (int32_t)(zoff + e) : id2z[e], g1, g2);
-                    s1 += g1;
-                    s2 += g2;
-                }
-                if (full) {
-                    const ulonglong2 G = a.gw[wbase + w];
-                    s1 = G.x - s1;
-                    s2 = G.y - s2;
-                }
-                S1 += s1;
-                S2 += s2;
             }
+            S2 = S1;
+        } else {
+            for (uint32_t i = wlo + lane; i < whi; i += 64u) {
+                const uint32_t w = list_at(touched, listed, i);
+                uint32_t v = tile[w];
+                tile[w] = 0;
+                if (COUNT) {
+                    if (v) {
+                        const uint32_t u = ZID ? zoff + w : (uint32_t)id2z[w];
+                        const uint64_t g = (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
+                        S1 += g;
+                        S2 += (uint64_t)v * g;
+                        fcs += v;
+                    }
+                } else {
+                    while (v) {
+                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                        v &= v - 1u;
+                        const uint32_t e = swz_offset(w, b, W, C);
+                        const uint32_t u = ZID ? zoff + e : (uint32_t)id2z[e];
+                        S1 += (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
+                    }
+                }
+            }
+            if (!COUNT) S2 = S1;
         }
         if (S1 | S2) {
             fsum += xfactor1(xv) * S1;
@@ -594,6 +602,24 @@ __device__ __forceinline__ void flush_stats(const ExpandArgs& a, uint64_t& fsum,
     fsum = fsum2 = fcs = npaths = 0;
 }
 
+// The fingerprint table in LDS: the 32 position factors, then for each byte
+// j of a word and each byte value m the sum of the position factors of m's
+// set bits (positions 8 j .. 8 j + 7).  Ends with a barrier.
+template <int BLOCK>
+__device__ void fp_table_init(uint64_t* T) {
+    for (uint32_t k = threadIdx.x; k < 32u; k += BLOCK) T[k] = zpos_factor(k);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 1024u; k += BLOCK) {
+        const uint32_t j = k >> 8, m = k & 255u;
+        uint64_t sum = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 8u; i++)
+            if ((m >> i) & 1u) sum += T[8u * j + i];
+        T[32u + k] = sum;
+    }
+    __syncthreads();
+}
+
 template <int BLOCK, bool COUNT, bool ZID>
 __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     extern __shared__ __align__(16) uint32_t smem[];
@@ -603,10 +629,13 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     __shared__ uint32_t s_wtot[NW];
     __shared__ uint32_t s_task, s_nt;
     __shared__ uint64_t s_gbase;
-    const Lds L{smem, smem + a.tile_words, smem + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
+    uint32_t* dyn = smem + (a.fp_mode ? 2u * kFpTableU64 : 0u);
+    const Lds L{reinterpret_cast<const uint64_t*>(smem), dyn, dyn + a.tile_words,
+                dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
 
     const uint32_t tid = threadIdx.x;
     const uint32_t ntasks = a.nx * a.ntiles;
+    if (a.fp_mode) fp_table_init<BLOCK>(reinterpret_cast<uint64_t*>(smem));
     for (uint32_t w = tid; w < a.tile_words; w += BLOCK) L.tile[w] = 0;
     uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
 
@@ -635,7 +664,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
 template <int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __restrict__ args,
                                                         const uint32_t* __restrict__ task_start, uint32_t n_inst,
-                                                        uint32_t* counter) {
+                                                        uint32_t* counter, uint32_t fp_mode) {
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr uint32_t NW = BLOCK / 64;
     __shared__ uint64_t sh[NW + 1];
@@ -649,6 +678,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
     uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
     int32_t cur = -1;
     ExpandArgs a;
+    // every instance of a batch has the batch's output mode
+    const bool fp = fp_mode != 0u;
+    uint32_t* dyn = smem + (fp ? 2u * kFpTableU64 : 0u);
+    if (fp) fp_table_init<BLOCK>(reinterpret_cast<uint64_t*>(smem));
     for (;;) {
         if (tid == 0) {
             s_task = atomicAdd(counter, 1u);
@@ -669,12 +702,13 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
             cur = (int32_t)lo;
             a = args[lo];
             // the previous instance's touched list / staging area may overlap this tile
-            for (uint32_t w = tid; w < a.tile_words; w += BLOCK) smem[w] = 0;
+            for (uint32_t w = tid; w < a.tile_words; w += BLOCK) dyn[w] = 0;
             __syncthreads();
         }
         const uint32_t local = task - task_start[lo];
         const uint32_t x = local / a.ntiles;
-        const Lds L{smem, smem + a.tile_words, smem + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
+        const Lds L{reinterpret_cast<const uint64_t*>(smem), dyn, dyn + a.tile_words,
+                    dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
         if (a.id2z == nullptr) row_task<BLOCK, COUNT, true>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
         else row_task<BLOCK, COUNT, false>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
     }
@@ -704,30 +738,6 @@ __global__ void k_gather_scalars(dim3h::Scalars* const* __restrict__ list, uint3
     for (uint32_t w = threadIdx.x; w < sizeof(dim3h::Scalars) / 8; w += blockDim.x) dst[w] = src[w];
 }
 
-// Fingerprint word totals: for every word of a project tile (bit layout
-// above) the sums of the z factors of its valid IDs and their bit mask.
-__global__ void k_fp_gw(const int32_t* __restrict__ id2z, int32_t zlo, uint32_t nz, uint32_t ntiles, uint32_t W,
-                        uint32_t tile_size, uint32_t swz, ulonglong2* __restrict__ gw, uint32_t* __restrict__ gwm) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)ntiles * W) return;
-    const uint32_t t = (uint32_t)(idx / W), w = (uint32_t)(idx - (uint64_t)t * W);
-    const uint32_t C = swz ? W / 32u : 0u;
-    uint64_t s1 = 0, s2 = 0;
-    uint32_t m = 0;
-    for (uint32_t r = 0; r < 32; r++) {
-        const uint32_t e = swz_offset(w, r, W, C);
-        const uint64_t zid = (uint64_t)t * tile_size + e;
-        if (e < tile_size && zid < nz) {
-            uint32_t g1, g2;
-            zfactors(id2z != nullptr ? id2z[zid] : (int32_t)((uint32_t)zlo + (uint32_t)zid), g1, g2);
-            s1 += g1;
-            s2 += g2;
-            m |= 1u << r;
-        }
-    }
-    gw[idx] = make_ulonglong2(s1, s2);
-    gwm[idx] = m;
-}
 
 template <typename K>
 int launch_persistent(K kern, int block, size_t lds, const ExpandArgs& a, int cu_count, hipStream_t st) {
@@ -757,7 +767,8 @@ int launch_z(const ExpandArgs& a, int block, size_t lds, int cu_count, hipStream
 
 template <bool COUNT>
 int launch(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
-    const size_t lds = dim3h::expand_lds_bytes(a.tile_words, a.touched_cap, a.stage_cap, (uint32_t)block);
+    const size_t lds = dim3h::expand_lds_bytes(a.tile_words, a.touched_cap, a.stage_cap, (uint32_t)block,
+                                               a.fp_mode != 0u);
     if (a.id2z == nullptr) return launch_z<COUNT, true>(a, block, lds, cu_count, st);
     return launch_z<COUNT, false>(a, block, lds, cu_count, st);
 }
@@ -766,17 +777,9 @@ int launch(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
 
 namespace dim3h {
 
-size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block) {
+size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp) {
     const uint32_t area = stage_cap > 3u * block + 1u ? stage_cap : 3u * block + 1u;
-    return ((size_t)tile_words + touched_cap + area) * 4u;
-}
-
-void fp_tables(const int32_t* id2z, int32_t zlo, uint32_t nz, uint32_t ntiles, uint32_t tile_words,
-               uint32_t tile_size, uint32_t swz, ulonglong2* gw, uint32_t* gwm, hipStream_t st) {
-    const uint64_t nw = (uint64_t)ntiles * tile_words;
-    if (nw)
-        hipLaunchKernelGGL(k_fp_gw, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, id2z, zlo, nz, ntiles,
-                           tile_words, tile_size, swz, gw, gwm);
+    return ((size_t)tile_words + touched_cap + area) * 4u + (fp ? (size_t)kFpTableU64 * 8u : 0u);
 }
 
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
@@ -814,12 +817,13 @@ int expand_batch_setup(bool count, int block, size_t lds, int cu_count, uint64_t
     }
 }
 
-void expand_batch_launch(bool count, int block, size_t lds, unsigned grid, const ExpandArgs* args,
+void expand_batch_launch(bool count, bool fp, int block, size_t lds, unsigned grid, const ExpandArgs* args,
                          const uint32_t* task_start, uint32_t n_inst, uint32_t* counter, Scalars* const* sc_list,
                          Scalars* sc_out, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_scalars, dim3((n_inst + 255) / 256), dim3(256), 0, st, sc_list, n_inst, counter);
 #define DIM3_BATCH_LAUNCH(B, CNT) \
-    hipLaunchKernelGGL((k_expand_batch<B, CNT>), dim3(grid), dim3(B), lds, st, args, task_start, n_inst, counter)
+    hipLaunchKernelGGL((k_expand_batch<B, CNT>), dim3(grid), dim3(B), lds, st, args, task_start, n_inst, counter, \
+                       fp ? 1u : 0u)
     if (block == 256) {
         if (count) DIM3_BATCH_LAUNCH(256, true);
         else DIM3_BATCH_LAUNCH(256, false);

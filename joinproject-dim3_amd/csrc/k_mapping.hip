@@ -480,8 +480,8 @@ __global__ void k_grid_rel(int2* __restrict__ out, uint32_t X, uint32_t Y) {
 }
 
 // COUNT fingerprint terms that are sums over join paths: sum2 = sum over
-// paths (x, y, z) of k2(x) g2(z) = sum over y of KR(y) GS(y), with
-// KR(y) = sum of k2 over R(y) and GS(y) = sum of g2 over S(y) (mod 2^64;
+// paths (x, y, z) of k2(x) g(z) = sum over y of KR(y) GS(y), with
+// KR(y) = sum of k2 over R(y) and GS(y) = sum of g over S(y) (mod 2^64;
 // tuples counted with multiplicity, as the COUNT is).
 __global__ __launch_bounds__(kThreads) void k_path_kr(const int2* __restrict__ R, uint64_t nR,
                                                       const uint32_t* __restrict__ slotR, uint64_t* kr) {
@@ -495,9 +495,8 @@ __global__ __launch_bounds__(kThreads) void k_path_gs(const int2* __restrict__ S
                                                       const int32_t* __restrict__ slotS, uint64_t* gs) {
     const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     const int32_t s = i < nS ? slotS[i] : -1;
-    uint32_t g1 = 0, g2 = 0;
-    if (s >= 0) dim3d::zfactors(S[i].x, g1, g2);
-    wave_add_u64(gs, s >= 0 ? (uint32_t)s : 0u, (uint64_t)g2, s >= 0);
+    const uint64_t g = s >= 0 ? dim3d::zfactor(S[i].x) : 0ull;
+    wave_add_u64(gs, s >= 0 ? (uint32_t)s : 0u, g, s >= 0);
 }
 __global__ __launch_bounds__(kThreads) void k_path_dot(const uint64_t* __restrict__ kr,
                                                        const uint64_t* __restrict__ gs, uint32_t n, uint64_t* out) {

@@ -68,6 +68,8 @@ def lib():
         L.oracle_fingerprint.restype = None
         L.oracle_mix64.argtypes = [u64]
         L.oracle_mix64.restype = u64
+        L.oracle_zfactor.argtypes = [ctypes.c_int32]
+        L.oracle_zfactor.restype = u64
         L.oracle_free.argtypes = [vp]
         L.oracle_free.restype = None
         _lib = L
@@ -203,6 +205,11 @@ def fingerprint(pairs, counts=None):
 
 def mix64(k):
     return int(lib().oracle_mix64(k & 0xFFFFFFFFFFFFFFFF))
+
+
+def zfactor(z):
+    """The fingerprint's z factor g(z) (mod 2^64)."""
+    return int(lib().oracle_zfactor(int(np.int32(z))))
 
 
 # ---------------- reference checker ----------------

@@ -38,15 +38,20 @@
  *     instead of reading past the end of small inputs.
  *
  * Fingerprint (order independent, shared with the GPU library), all
- * arithmetic mod 2^64 (x factors) or 2^32 (z factors), u(v) = (u32)v:
+ * arithmetic mod 2^64, u(v) = (u32)v:
  *   k1(x) = mix64(u(x) ^ 0x243F6A8885A308D3) | 1, k2(x) = mix64(u(x) ^ 0x13198A2E03707344) | 1
- *   g1(z) = fmix32(u(z) ^ 0x243F6A88), g2(z) = (g1(z) * 0x2545F491) ^ (g1(z) >> 16)
- *   project row (x, z):          count += 1, sum += k1(x) g1(z), sum2 += k2(x) g2(z)
- *   aggregate row (x, z, c):     count += 1, sum += k1(x) g1(z), sum2 += k2(x) c g2(z), csum += c
+ *   g(z)  = a(u(z) >> 5) * c(u(z) & 31), with the word factor
+ *           a(w) = fmix32(w ^ 0x243F6A88) | 1 and the bit-position factor
+ *           c(b) = mix64(0x9E3779B97F4A7C15 * (b + 1)) | 1 (the splitmix64
+ *           outputs for seed 0)
+ *   project row (x, z):          count += 1, sum += k1(x) g(z), sum2 += k2(x) g(z)
+ *   aggregate row (x, z, c):     count += 1, sum += k1(x) g(z), sum2 += k2(x) c g(z), csum += c
  * mix64 is the splitmix64 finaliser (Steele, Lea, Flood 2014), fmix32 the
- * MurmurHash3 32-bit finaliser (a bijection, so one wrong z always changes
- * sum).  Each term is an x factor times a z factor, so a row's share is k(x)
- * times the sum of its z factors (the GPU sums z factors per row).
+ * MurmurHash3 32-bit finaliser.  Each term is an x factor times a z factor,
+ * so a row's share is k(x) times the sum of its z factors, and the z factors
+ * of one 32-aligned word of z values are one word factor times a sum of
+ * position factors (the GPU sums a dense word with four byte-table lookups).
+ * All factors are odd, so z values of one word never share a factor.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -83,19 +88,27 @@ static inline uint32_t orc_fmix32(uint32_t h) {
     return h;
 }
 
+/* z factor of the fingerprint (see the header comment) */
+static inline uint64_t orc_zfactor(int32_t z) {
+    const uint32_t u = (uint32_t)z;
+    const uint64_t a = (uint64_t)(orc_fmix32((u >> 5) ^ 0x243F6A88u) | 1u);
+    return a * (orc_mix64(0x9E3779B97F4A7C15ULL * (uint64_t)((u & 31u) + 1u)) | 1u);
+}
+
+uint64_t oracle_zfactor(int32_t z) { return orc_zfactor(z); }
+
 /* one row's terms of the fingerprint (see the header comment) */
 static inline void orc_fp_add(orc_fp* fp, int32_t x, int32_t z, const int64_t* count) {
     const uint64_t ux = (uint64_t)(uint32_t)x;
     const uint64_t k1 = orc_mix64(ux ^ 0x243F6A8885A308D3ULL) | 1u, k2 = orc_mix64(ux ^ 0x13198A2E03707344ULL) | 1u;
-    const uint32_t g1 = orc_fmix32((uint32_t)z ^ 0x243F6A88u);
-    const uint32_t g2 = (g1 * 0x2545F491u) ^ (g1 >> 16);
+    const uint64_t g = orc_zfactor(z);
     fp->count += 1;
-    fp->sum += k1 * (uint64_t)g1;
+    fp->sum += k1 * g;
     if (count) {
-        fp->sum2 += k2 * ((uint64_t)*count * (uint64_t)g2);
+        fp->sum2 += k2 * ((uint64_t)*count * g);
         fp->csum += (uint64_t)*count;
     } else {
-        fp->sum2 += k2 * (uint64_t)g2;
+        fp->sum2 += k2 * g;
     }
 }
 

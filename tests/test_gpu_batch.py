@@ -81,6 +81,27 @@ def test_batch_fingerprints_equal_single_and_oracle(ctx, count, exclude_equal):
 
 
 @pytest.mark.parametrize("count", [False, True])
+def test_batch_led_by_empty_instances(ctx, count):
+    """A batch whose first instances have no join path (no matching key,
+    empty R): the output mode is the batch's, not read off its first plan."""
+    import torch
+
+    inst = mixed_instances(21)
+    inst = [inst[4], inst[1]] + inst
+    dev = to_dev(torch, inst)
+    fn = ctx.batch_join_aggregate_count_dev if count else ctx.batch_join_project_dev
+    say("batch led by empty instances", "count" if count else "project", len(inst))
+    rc, res = fn([(d[0].data_ptr(), d[1], d[2].data_ptr(), d[3]) for d in dev], output_mode=1)
+    assert rc == 0
+    for i, ((R, S), r) in enumerate(zip(inst, res)):
+        if count:
+            _, _, want, _ = oracle.join_aggregate_count(R, S, fingerprint_only=True)
+        else:
+            _, want, _ = oracle.join_project(R, S, fingerprint_only=True)
+        assert r.fp.as_tuple() == want.as_tuple(), i
+
+
+@pytest.mark.parametrize("count", [False, True])
 def test_batch_materialized_rows_and_capacity(ctx, count):
     import torch
 

@@ -50,6 +50,15 @@ def _mix64(k):
     return k ^ ((k >> 31) & ((1 << 33) - 1))
 
 
+def _mix64_int(k):
+    """splitmix64 finaliser on a Python int (mod 2^64)."""
+    k ^= k >> 30
+    k = (k * 0xBF58476D1CE4E5B9) & M64
+    k ^= k >> 27
+    k = (k * 0x94D049BB133111EB) & M64
+    return k ^ (k >> 31)
+
+
 def _fmix32(h):
     h = h ^ (h >> 16)
     h = (h * 0x85EBCA6B) & 0xFFFFFFFF
@@ -68,24 +77,26 @@ def _sum_mod64(v):
 def device_fingerprint(torch, xz, counts=None, chunk=1 << 26):
     """The row fingerprint (count, sum, sum2, csum) of device rows: xz an
     (n, 2) int32 tensor, counts an int64 tensor or None (join-project).
-    A row adds k1(x) g1(z) to sum and k2(x) c g2(z) to sum2 (mod 2^64)."""
+    A row adds k1(x) g(z) to sum and k2(x) c g(z) to sum2 (mod 2^64), with
+    g(z) = a(z >> 5) c(z & 31) (word factor times bit-position factor)."""
     n = int(xz.shape[0])
     s1 = s2 = cs = 0
+    pos = torch.tensor([_s64(_mix64_int((0x9E3779B97F4A7C15 * (b + 1)) & M64) | 1) for b in range(32)],
+                       dtype=torch.int64, device=xz.device)
     for i in range(0, n, chunk):
         x = xz[i:i + chunk, 0].to(torch.int64) & 0xFFFFFFFF
         z = xz[i:i + chunk, 1].to(torch.int64) & 0xFFFFFFFF
-        g1 = _fmix32(z ^ 0x243F6A88)
-        g2 = ((g1 * 0x2545F491) & 0xFFFFFFFF) ^ (g1 >> 16)
+        g = (_fmix32((z >> 5) ^ 0x243F6A88) | 1) * pos[z & 31]
         k1 = _mix64(x ^ _s64(0x243F6A8885A308D3)) | 1
         k2 = _mix64(x ^ _s64(0x13198A2E03707344)) | 1
-        s1 = (s1 + _sum_mod64(k1 * g1)) & M64
+        s1 = (s1 + _sum_mod64(k1 * g)) & M64
         if counts is None:
-            s2 = (s2 + _sum_mod64(k2 * g2)) & M64
+            s2 = (s2 + _sum_mod64(k2 * g)) & M64
         else:
             c = counts[i:i + chunk]
-            s2 = (s2 + _sum_mod64(k2 * c * g2)) & M64
+            s2 = (s2 + _sum_mod64(k2 * c * g)) & M64
             cs = (cs + _sum_mod64(c)) & M64
-        del x, z, g1, g2, k1, k2
+        del x, z, g, k1, k2
     return (n, s1, s2, cs)
 
 

@@ -112,7 +112,7 @@ def test_fingerprint_is_order_independent_and_sensitive():
 
 def test_fingerprint_factorises_over_a_cross_product():
     """Every term is k(x) g(z): a full cross product X x Z therefore has
-    sum = (sum_x k1(x)) (sum_z g1(z)) mod 2^64, and disjoint sets add."""
+    sum = (sum_x k1(x)) (sum_z g(z)) mod 2^64, and disjoint sets add."""
     M = (1 << 64) - 1
     X = np.array([-7, 0, 3, 1 << 30, 99], np.int64)
     Z = np.array([5, -1, 2 ** 31 - 1, 8], np.int64)
@@ -124,13 +124,34 @@ def test_fingerprint_factorises_over_a_cross_product():
     k0 = oracle.mix64(0x243F6A8885A308D3) | 1
     inv = pow(k0, -1, 1 << 64)
     g1 = sum((oracle.fingerprint(np.array([[0, z]], np.int32)).sum * inv) & M for z in Z) & M
-    assert all(0 <= (oracle.fingerprint(np.array([[0, z]], np.int32)).sum * inv) & M < (1 << 32) for z in Z)
+    assert all((oracle.fingerprint(np.array([[0, z]], np.int32)).sum * inv) & M == oracle.zfactor(z) for z in Z)
     assert fp.sum == (k1 * g1) & M and fp.count == len(p)
     a, b = oracle.fingerprint(p[:7]), oracle.fingerprint(p[7:])
     assert fp.sum == (a.sum + b.sum) & M and fp.sum2 == (a.sum2 + b.sum2) & M
     # COUNT: a row with count c weighs its second term by c
     c1, c2 = oracle.fingerprint(p[:1], np.array([1])), oracle.fingerprint(p[:1], np.array([3]))
     assert c2.sum == c1.sum and c2.sum2 == (3 * c1.sum2) & M and c2.csum == 3
+
+
+def test_zfactor_word_structure():
+    """g(z) = a(u(z) >> 5) c(u(z) & 31): within one 32-aligned word of z
+    values g(z) / c(b) is one odd constant, c(b) being the splitmix64
+    outputs for seed 0 (forced odd), so the z factors of a word sum to
+    a(w) times a sum of position factors; other words have other constants."""
+    M = (1 << 64) - 1
+    c = [oracle.mix64((0x9E3779B97F4A7C15 * (b + 1)) & M) | 1 for b in range(32)]
+    assert c[0] == 0xE220A8397B1DCDAF and c[2] == 0x06C45D188009454F  # published splitmix64 outputs
+    words = {}
+    for base in (0, 32 * 12345, -64, (1 << 31) - 32, -(1 << 31)):
+        a = {(oracle.zfactor(base + b) * pow(c[b], -1, 1 << 64)) & M for b in range(32)}
+        assert len(a) == 1
+        a = a.pop()
+        assert a & 1 and a < (1 << 32)
+        words[base] = a
+        zs = [base + b for b in (0, 3, 17, 31)]
+        total = sum(oracle.zfactor(z) for z in zs) & M
+        assert total == (a * sum(c[b] for b in (0, 3, 17, 31))) & M
+    assert len(set(words.values())) == len(words)
 
 
 def test_mix64_known_values():
