@@ -77,44 +77,20 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
 // One output row (plain store: non-temporal stores measured slower here).
 __device__ __forceinline__ void put_pair(int2* p, int32_t x, int32_t z) { *p = make_int2(x, z); }
 
-// Records the first touches of one wavefront instruction in the LDS list.
-__device__ __forceinline__ void push_first_touch(bool first, uint32_t item, uint32_t* s_nt, uint32_t* touched,
-                                                 uint32_t cap) {
-    const uint64_t ft = __ballot(first);
-    if (ft) {
-        const uint32_t lane = lane_id();
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)ft) - 1u;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(s_nt, (uint32_t)__popcll(ft));
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
-        if (first) {
-            const uint32_t p = base + mbcnt64(ft);
-            if (p < cap) touched[p] = item;
-        }
-    }
-}
-
 // Project tiles store local z offset zl = 32 q + r in bit r of word
 // (q + r * C) mod W (W words, C = W / 32): the 32 offsets that share a word
 // in the plain layout land in 32 different words and LDS banks, so hot z
 // values with neighbouring IDs (hubs of a graph numbered by degree) do not
 // serialise on one word.
 __device__ __forceinline__ uint32_t swz_word(uint32_t zl, uint32_t W, uint32_t C) {
-    const uint32_t q = (zl >> 5) + (zl & 31u) * C;
+    const uint32_t q = (zl >> 5) + __umul24(zl & 31u, C);  // C < 2^24: the full-rate 24-bit multiply
     return q >= W ? q - W : q;
 }
 
 __device__ __forceinline__ uint32_t swz_offset(uint32_t word, uint32_t r, uint32_t W, uint32_t C) {
-    int32_t q = (int32_t)word - (int32_t)(r * C);
+    int32_t q = (int32_t)word - (int32_t)__umul24(r, C);
     if (q < 0) q += (int32_t)W;
     return (uint32_t)q * 32u + r;
-}
-
-// One join path into the tile.  Project: set the bit of zl; COUNT: add 1 to counter zl.
-template <bool COUNT>
-__device__ __forceinline__ bool apply_path(uint32_t* tile, uint32_t zl, uint32_t W, uint32_t C) {
-    if (COUNT) return atomicAdd(&tile[zl], 1u) == 0u;
-    return atomicOr(&tile[swz_word(zl, W, C)], 1u << (zl & 31u)) == 0u;
 }
 
 template <bool COUNT>
@@ -162,6 +138,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t W = a.tile_words;
     const uint32_t C = (COUNT || !a.swz) ? 0u : W / 32u;  // bit swizzle stride of project tiles (0: plain)
+    const uint32_t dummy = min(lane, W - 1u);             // an empty path slot's own word (it adds 0)
     // rows with more paths than this sweep their tile instead of listing first touches
     const uint64_t track_limit = min(4ull * a.touched_cap, 4ull * a.tile_words);
     const uint32_t zbase = t * a.tile_size;
@@ -327,21 +304,45 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                     zl[h * kUnroll + u] = ok ? v - zbase : 0xFFFFFFFFu;
                 }
             }
+            // Every slot applies one LDS atomic (an empty slot ORs / adds 0
+            // into a word of its own), so the atomics issue back to back with
+            // no branch and, tracked, one wait covers all their returns.
+            uint32_t addr[kPairs * kUnroll], val[kPairs * kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                const bool ok = zl[u] != 0xFFFFFFFFu;
+                addr[u] = ok ? list_item<COUNT>(zl[u], W, C) : dummy;
+                val[u] = ok ? (COUNT ? 1u : 1u << (zl[u] & 31u)) : 0u;
+            }
             if (track) {
+                // first touches (the word / counter was zero) of all slots go
+                // to the list with one reservation per wavefront
+                uint64_t fm[kPairs * kUnroll];
+                uint32_t ftot = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                    bool first = false;
-                    if (zl[u] != 0xFFFFFFFFu) first = apply_path<COUNT>(tile, zl[u], W, C);
-                    push_first_touch(first, first ? list_item<COUNT>(zl[u], W, C) : 0u, L.s_nt, touched,
-                                     a.touched_cap);
+                    const uint32_t old = COUNT ? atomicAdd(&tile[addr[u]], val[u]) : atomicOr(&tile[addr[u]], val[u]);
+                    fm[u] = __ballot(val[u] != 0u && old == 0u);
+                    ftot += (uint32_t)__popcll(fm[u]);
+                }
+                if (ftot) {
+                    uint32_t fbase = 0;
+                    if (lane == 0u) fbase = atomicAdd(L.s_nt, ftot);
+                    fbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)fbase);
+#pragma unroll
+                    for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                        if ((fm[u] >> lane) & 1ull) {
+                            const uint32_t p = fbase + mbcnt64(fm[u]);
+                            if (p < a.touched_cap) touched[p] = addr[u];
+                        }
+                        fbase += (uint32_t)__popcll(fm[u]);
+                    }
                 }
             } else {
 #pragma unroll
                 for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                    if (zl[u] != 0xFFFFFFFFu) {
-                        if (COUNT) atomicAdd(&tile[zl[u]], 1u);
-                        else atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
-                    }
+                    if (COUNT) atomicAdd(&tile[addr[u]], val[u]);
+                    else atomicOr(&tile[addr[u]], val[u]);
                 }
             }
         }
