@@ -225,8 +225,9 @@ typedef struct dim3_server_opts {
     uint32_t workers;          /* concurrent requests (0 = 4) */
     uint32_t max_batch;        /* requests per batch (0 = 32; 1 = no batching) */
     uint32_t queue_cap;        /* pending requests before DIM3_ERR_BUSY (0 = 4096) */
-    uint32_t large_share_pct;  /* with 2+ workers, worker 0 is the lane of large requests and its
-                                  expansions occupy this % of the compute units (0 = 75) */
+    uint32_t large_share_pct;  /* with 2+ workers, worker 0 is the lane of large requests: they run
+                                  on this % of the compute units (a CU-masked stream) and the other
+                                  workers on the rest while one runs (0 = 90) */
     uint64_t large_tuples;     /* a request with at least this many input tuples is large (0 = 2e7) */
 } dim3_server_opts;
 

@@ -433,12 +433,17 @@ class Server:
     """The libdim3 request loop (dim3_server_*): resident relations,
     admission by HBM footprint, concurrent workers, continuous batching."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, workers=4, max_batch=32, queue_cap=4096):
+    def __init__(self, device=0, hbm_budget_bytes=0, workers=4, max_batch=32, queue_cap=4096, large_share_pct=None,
+                 large_tuples=None):
         L = lib()
         o = ServerOpts()
         L.dim3_server_opts_default(ctypes.byref(o))
         o.device, o.hbm_budget_bytes, o.workers, o.max_batch, o.queue_cap = (device, hbm_budget_bytes, workers,
                                                                               max_batch, queue_cap)
+        if large_share_pct is not None:
+            o.large_share_pct = large_share_pct
+        if large_tuples is not None:
+            o.large_tuples = large_tuples
         h = ctypes.c_void_p()
         rc = L.dim3_server_create(ctypes.byref(o), ctypes.byref(h))
         if rc != OK:

@@ -10,6 +10,10 @@
 // one query), records the result and wakes the waiters and the other
 // workers.  Workspaces stay with their worker for reuse; a worker whose
 // request does not fit asks the idle workers to release theirs.
+// With 2+ workers, worker 0 serves the large requests (>= large_tuples) on a
+// stream whose CU mask holds large_share_pct of the compute units; while one
+// runs, the other workers launch on streams masked to the remaining CUs, so
+// small requests never share a CU with (or queue behind) a large one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,6 +82,14 @@ struct dim3_server {
     std::vector<char> release;    // per worker: asked to give its idle workspace back
     std::vector<dim3_ctx*> ctxs;
     std::vector<std::thread> threads;
+    // Compute-unit partition (2+ workers): lane 0's large requests run on a
+    // stream masked to the first large_cus CUs; while one runs, the other
+    // workers launch on streams masked to the remaining CUs, so neither kind
+    // shares a CU with the other.  Otherwise every worker uses all CUs.
+    std::vector<hipStream_t> full_stream;  // per worker, all CUs
+    std::vector<hipStream_t> part_stream;  // per worker: lane 0 the large part, others the rest
+    uint32_t cus = 0, large_cus = 0;
+    uint32_t large_running = 0;
     bool stop = false;
     dim3_server_stats st{};
 
@@ -100,6 +112,25 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
                       uint64_t* out_bytes) {
     dim3_ctx* c = ctxs[w];
     const bool count = js[0]->req.kind == DIM3_REQ_COUNT;
+    // the stream of this batch: the CU partition when it applies
+    const bool large = nworkers > 1 && w == 0 && js[0]->tuples >= large_tuples;
+    bool part = false;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (large) large_running++;
+        part = large || (nworkers > 1 && w != 0 && large_running > 0);
+    }
+    hipStream_t st = part ? part_stream[w] : full_stream[w];
+    dim3_ctx_set_cu_share(c, !part ? 1.0 : (double)(large ? large_cus : cus - large_cus) / (double)cus);
+    struct Done {  // the large lane is free again when this batch ends
+        dim3_server* s;
+        bool large;
+        ~Done() {
+            if (!large) return;
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->large_running--;
+        }
+    } done{this, large};
     std::vector<const Rel*> R(js.size()), S(js.size());
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -114,9 +145,9 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
             inst[i] = dim3_instance{R[i]->d, R[i]->n, S[i]->d, S[i]->n, nullptr, 0};
         std::vector<dim3_result> res(js.size());
         const int rc = count ? dim3_batch_join_aggregate_count_dev(c, inst.data(), (uint32_t)js.size(),
-                                                                  &js[0]->req.opts, nullptr, res.data())
+                                                                  &js[0]->req.opts, st, res.data())
                              : dim3_batch_join_project_dev(c, inst.data(), (uint32_t)js.size(), &js[0]->req.opts,
-                                                           nullptr, res.data());
+                                                           st, res.data());
         for (size_t i = 0; i < js.size(); i++) {
             js[i]->status = rc;
             js[i]->res = res[i];
@@ -131,9 +162,9 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
     dim3_result res;
     for (int attempt = 0; attempt < 2; attempt++) {
         int rc = count ? dim3_join_aggregate_count_dev(c, R[0]->d, R[0]->n, S[0]->d, S[0]->n, &o,
-                                                       (dim3_count_row*)*out_buf, cap, nullptr, &res)
+                                                       (dim3_count_row*)*out_buf, cap, st, &res)
                        : dim3_join_project_dev(c, R[0]->d, R[0]->n, S[0]->d, S[0]->n, &o, (int32_t*)*out_buf, cap,
-                                               nullptr, &res);
+                                               st, &res);
         if (rc == DIM3_ERR_CAPACITY && attempt == 0) {  // grow the output buffer to the size reported
             if (*out_buf) hipFree(*out_buf);
             *out_buf = nullptr;
@@ -334,7 +365,7 @@ int dim3_server_create(const dim3_server_opts* opts, dim3_server** out) {
     s->max_batch = o.max_batch;
     s->queue_cap = o.queue_cap;
     s->large_tuples = o.large_tuples ? o.large_tuples : 20000000ull;
-    const double share = (o.large_share_pct ? std::min<uint32_t>(o.large_share_pct, 100) : 75) / 100.0;
+    const double share = (o.large_share_pct ? std::min<uint32_t>(o.large_share_pct, 100) : 90) / 100.0;
     s->held.assign(o.workers, 0);
     s->busy.assign(o.workers, 0);
     s->release.assign(o.workers, 0);
@@ -346,8 +377,35 @@ int dim3_server_create(const dim3_server_opts* opts, dim3_server** out) {
             delete s;
             return rc;
         }
-        if (w == 0 && o.workers > 1) dim3_ctx_set_cu_share(c, share);
         s->ctxs.push_back(c);
+    }
+    // streams: all CUs, and the two sides of the CU partition (masks of
+    // consecutive CU indices; the first large_cus go to the large lane)
+    {
+        hipDeviceProp_t prop;
+        hipGetDeviceProperties(&prop, o.device);
+        s->cus = (uint32_t)std::max(2, prop.multiProcessorCount);
+        s->large_cus = std::min<uint32_t>(s->cus - 1, std::max<uint32_t>(1, (uint32_t)(share * s->cus + 0.5)));
+        std::vector<uint32_t> big((s->cus + 31) / 32, 0), rest((s->cus + 31) / 32, 0);
+        for (uint32_t i = 0; i < s->cus; i++) (i < s->large_cus ? big : rest)[i / 32] |= 1u << (i % 32);
+        bool ok = true;
+        for (uint32_t w = 0; w < o.workers; w++) {
+            hipStream_t f = nullptr, p = nullptr;
+            ok = ok && hipStreamCreateWithFlags(&f, hipStreamNonBlocking) == hipSuccess;
+            const std::vector<uint32_t>& m = w == 0 ? big : rest;
+            ok = ok && hipExtStreamCreateWithCUMask(&p, (uint32_t)m.size(), m.data()) == hipSuccess;
+            s->full_stream.push_back(f);
+            s->part_stream.push_back(p);
+        }
+        if (!ok) {
+            for (auto x : s->full_stream)
+                if (x) hipStreamDestroy(x);
+            for (auto x : s->part_stream)
+                if (x) hipStreamDestroy(x);
+            for (auto* x : s->ctxs) dim3_ctx_destroy(x);
+            delete s;
+            return DIM3_ERR_DEVICE;
+        }
     }
     for (uint32_t w = 0; w < o.workers; w++) s->threads.emplace_back(&dim3_server::worker, s, w);
     *out = s;
@@ -366,6 +424,8 @@ int dim3_server_destroy(dim3_server* s) {
     for (auto& kv : s->rels) hipFree(kv.second.d);
     for (auto& kv : s->jobs) dim3_result_free(&kv.second.res);
     for (auto* c : s->ctxs) dim3_ctx_destroy(c);
+    for (auto x : s->full_stream) hipStreamDestroy(x);
+    for (auto x : s->part_stream) hipStreamDestroy(x);
     delete s;
     return DIM3_OK;
 }
