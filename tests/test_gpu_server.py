@@ -75,3 +75,28 @@ def test_admission_and_handles(dim3):
             srv.wait(123456)
         st = srv.stats()
         assert st["rejected"] == 1 and st["completed"] == 1
+
+
+def test_large_lane_partition(ctx, dim3):
+    """Requests above large_tuples go to worker 0 on a CU-masked stream while
+    the others run on the remaining CUs; every result equals the same
+    request run alone."""
+    inst = mixed_instances(31)
+    with dim3.Server(0, workers=3, max_batch=4, large_share_pct=80, large_tuples=4000) as srv:
+        hs = [(srv.put(R), srv.put(S)) for R, S in inst]
+        reqs = []
+        for i, (r, s) in enumerate(hs):
+            count = i % 3 == 1
+            rc, t = srv.submit(r, s, count=count, output_mode=1)
+            assert rc == dim3.OK
+            reqs.append((i, count, t))
+        n_large = sum(1 for R, S in inst if R.shape[0] + S.shape[0] >= 4000)
+        say("submitted", len(reqs), "requests,", n_large, "large")
+        assert n_large >= 2
+        for i, count, t in reqs:
+            rc, res, _, _ = srv.wait(t)
+            assert rc == dim3.OK, i
+            R, S = inst[i]
+            alone = ctx.join_aggregate_count(R, S, output_mode=1) if count else ctx.join_project(R, S, output_mode=1)
+            assert res.fp.as_tuple() == alone.fp.as_tuple(), i
+        assert srv.stats()["completed"] == len(reqs)
