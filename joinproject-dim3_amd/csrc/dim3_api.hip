@@ -125,6 +125,7 @@ struct QueryPlan {
     int dense = DIM3_DENSE_OFF;  // mode used
     uint32_t bmin = 0;           // heaviness bin threshold
     uint32_t K = 0, Kp = 0, Mh = 0, Nh = 0, dwords = 0;
+    uint64_t z_span = 0;  // dense z IDs (range mapping): the 32-aligned span of z values
     bool all_z = false;  // every z ID is a product column
     bool fused = false;  // bit-or product read from the key bitmaps by the seeding (no product rows)
     // dense core: rows / columns with at least thx / thz heavy entries
@@ -142,6 +143,14 @@ struct QueryPlan {
 struct Bank {
     Slot ws[WS_COUNT];
     QueryPlan plan;
+    // pinned host mirrors of this query's scalars and dense-split histogram
+    // (a batch plans all its instances with one synchronisation per stage)
+    Scalars* hsc = nullptr;
+    unsigned long long* hhist = nullptr;
+    ~Bank() {
+        if (hsc) dim3h::host_free(hsc);
+        if (hhist) dim3h::host_free(hhist);
+    }
 };
 
 // Device rates of the planner's cost model.  The defaults are static
@@ -189,6 +198,15 @@ struct dim3_ctx {
 
 namespace {
 
+}  // namespace
+
+std::shared_mutex& dim3h::capture_mutex() {
+    static std::shared_mutex mu;
+    return mu;
+}
+
+namespace {
+
 int fail(dim3_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -227,13 +245,13 @@ int ws_get(dim3_ctx* c, SlotId id, size_t bytes, void** out) {
     // stream-ordered: no device-wide synchronisation, so other contexts'
     // queries keep running while this one grows its workspace
     if (s.p) {
-        hipFreeAsync(s.p, c->qstream);
+        dim3h::dev_free_async(s.p, c->qstream);
         c->held -= s.bytes;
         s.p = nullptr;
         s.bytes = 0;
     }
     void* p = nullptr;
-    hipError_t e = hipMallocAsync(&p, bytes, c->qstream);
+    hipError_t e = dim3h::dev_malloc_async(&p, bytes, c->qstream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(c, DIM3_ERR_OOM, "hipMalloc(%zu) for workspace slot %d: %s", bytes, (int)id, hipGetErrorString(e));
@@ -385,7 +403,7 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     auto& P = c->bank().plan;
     P.dense = DIM3_DENSE_OFF;
     if (o.dense_mode == DIM3_DENSE_OFF) return;
-    const unsigned long long* h = c->h_hist;
+    const unsigned long long* h = c->bank().hhist;
     const uint32_t NB = dim3h::kDenseBins;
     const DenseModel& m = c->model;
     const uint64_t room = c->budget > c->held ? (c->budget - c->held) / 3 : 0;
@@ -467,14 +485,14 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
 }
 
 // Dense core of the keys with bin >= P.bc, from the heavy-degree histogram
-// in c->h_hist ([4][bins]: rows, their heavy entries, columns, their heavy
+// in the bank's hhist ([4][bins]: rows, their heavy entries, columns, their heavy
 // tuples): rows with >= thx heavy entries times columns with >= thz heavy
 // tuples.  The paths kept inside the core are estimated as the key paths
 // times the fractions of heavy entries and of heavy tuples inside it.
 // Returns true when the core (or, forced, any core) beats the phase-1 plan.
 bool plan_core(dim3_ctx* c, const dim3_opts& o) {
     auto& P = c->bank().plan;
-    const unsigned long long* h = c->h_hist;
+    const unsigned long long* h = c->bank().hhist;
     const uint32_t NB = dim3h::kDenseBins;
     const DenseModel& m = c->model;
     const uint64_t room = c->budget > c->held ? (c->budget - c->held) / 3 : 0;
@@ -542,10 +560,21 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
 }
 
 // Mapping + CSR of one query.  Leaves the plan in c->bank().plan.
-int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
-            const dim3_opts& o, hipStream_t st, int force_wgs = 0) {
-    auto& P = c->bank().plan;
+// A query is prepared in three stages, each ending with a copy of the
+// device scalars to the bank's host mirror: start (swap rule, min/max of
+// the attributes), map (dense IDs, degrees, the dense-split histogram) and
+// plan (tiles, dense split, CSRs, dense product).  Each stage reads what the
+// previous one copied, so the host synchronises between stages; a batch runs
+// each stage over all its instances before one synchronisation.
+int prepare_start(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
+                  const dim3_opts& o, hipStream_t st) {
+    Bank& B = c->bank();
+    auto& P = B.plan;
     P = QueryPlan();
+    if (!B.hsc) {
+        HIPCHK(c, dim3h::host_malloc((void**)&B.hsc, sizeof(Scalars)));
+        HIPCHK(c, dim3h::host_malloc((void**)&B.hhist, 6 * dim3h::kDenseBins * sizeof(unsigned long long)));
+    }
     // A COUNT fingerprint needs no counters: its groups are the join-project
     // rows (n, sum) and its sum2 / csum are sums over the join paths, taken
     // from the key degrees.  With the u != w filter the paths are counted
@@ -574,13 +603,23 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         P.empty = true;
         return DIM3_OK;
     }
+    P.empty = false;  // until the mapping finds no joined key
 
     HIPCHK(c, hipEventRecord(c->ev[0], st));
     // value ranges decide hash or range mapping per attribute
     dim3h::minmax(dR, nR, dS, nS, sc->mm, st);
-    HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    const int32_t* mm = c->h_sc->mm;
+    HIPCHK(c, hipMemcpyAsync(B.hsc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    return DIM3_OK;
+}
+
+int prepare_map(dim3_ctx* c, const dim3_opts& o, hipStream_t st) {
+    Bank& B = c->bank();
+    auto& P = B.plan;
+    if (P.empty) return DIM3_OK;
+    const int2 *dR = P.R, *dS = P.S;
+    const uint64_t nR = P.nR, nS = P.nS;
+    Scalars* sc = (Scalars*)B.ws[WS_SCALARS].p;
+    const int32_t* mm = B.hsc->mm;
     const bool hash_only = (o.flags & DIM3_OPT_HASH_ONLY) != 0;
     auto span = [](int32_t lo, int32_t hi) { return (uint64_t)((int64_t)hi - (int64_t)lo + 1); };
     const uint64_t xr = span(mm[0], mm[1]), yr = span(mm[2], mm[3]);
@@ -593,6 +632,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     // sums such a word at once); the IDs below min(z) stay empty
     P.zlo = P.z_ident ? (int32_t)((uint32_t)mm[4] & ~31u) : 0;
     if (P.z_ident) zr += (uint64_t)((int64_t)mm[4] - (int64_t)P.zlo);
+    P.z_span = zr;
 
     uint32_t *keysY = nullptr, *degR, *degS, *slotR, *keysX = nullptr, *degX, *keysZ = nullptr, *degZ = nullptr;
     int32_t *slotS, *slotRx, *slotSz = nullptr;
@@ -673,24 +713,48 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
         dim3h::dense_weights(dR, nR, slotR, slotRx, dS, nS, slotS, P.z_ident ? nullptr : slotSz, P.zlo, degR, degS, xw,
                              (uint32_t)nx1, zw, (uint32_t)nz1, st);
         dim3h::dense_hist(degR, degS, (uint32_t)ny1, xw, (uint32_t)nx1, zw, (uint32_t)nz1, dhist, st);
-        HIPCHK(c, hipMemcpyAsync(c->h_hist, dhist, 6ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipMemcpyAsync(B.hhist, dhist, 6ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
     }
     LAUNCHCHK(c, "mapping");
     HIPCHK(c, hipEventRecord(c->ev[1], st));
-    HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
+    HIPCHK(c, hipMemcpyAsync(B.hsc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    return DIM3_OK;
+}
 
-    P.ny = (uint32_t)(c->h_sc->tot_y >> 32);
-    P.nx = (uint32_t)(c->h_sc->tot_x >> 32);
-    P.nnzR = (uint32_t)c->h_sc->tot_x;
-    P.nnzS = (uint32_t)c->h_sc->s_surv;
-    P.nz = P.z_ident ? (uint32_t)zr : (uint32_t)(c->h_sc->tot_z >> 32);
-    P.out_j = c->h_sc->out_j;
-    {
-        float ms = 0;
-        hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-        P.ms_map = ms;
-    }
+int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs) {
+    Bank& B = c->bank();
+    auto& P = B.plan;
+    if (P.empty) return DIM3_OK;
+    const int2* dS = P.S;
+    const uint64_t nR = P.nR, nS = P.nS;
+    Slot* W = B.ws;
+    Scalars* sc = (Scalars*)W[WS_SCALARS].p;
+    // the mapping stage's buffers
+    auto slot = [&](SlotId id) { return W[id].p; };
+    uint32_t* degR = (uint32_t*)slot(WS_DEGR_Y);
+    uint32_t* degS = (uint32_t*)slot(WS_DEGS_Y);
+    uint64_t* exY = (uint64_t*)slot(WS_EX_Y);
+    uint32_t* slotR = (uint32_t*)slot(WS_SLOT_R);
+    int32_t* slotS = (int32_t*)slot(WS_SLOT_S);
+    uint32_t* degX = (uint32_t*)slot(WS_DEG_X);
+    uint64_t* exX = (uint64_t*)slot(WS_EX_X);
+    int32_t* slotRx = (int32_t*)slot(WS_SLOT_RX);
+    uint32_t* degZ = P.z_ident ? nullptr : (uint32_t*)slot(WS_DEG_Z);
+    uint64_t* exZ = P.z_ident ? nullptr : (uint64_t*)slot(WS_EX_Z);
+    int32_t* slotSz = P.z_ident ? nullptr : (int32_t*)slot(WS_SLOT_SZ);
+    const bool want_dense = o.dense_mode != DIM3_DENSE_OFF;
+    uint32_t* xw = want_dense ? (uint32_t*)slot(WS_XW) : nullptr;
+    uint32_t* zw = want_dense ? (uint32_t*)slot(WS_ZW) : nullptr;
+    uint64_t* part = (uint64_t*)slot(WS_PARTIALS);
+    const uint64_t ny1 = P.my.nslots, nx1 = P.mx.nslots, nz1 = P.mz.nslots;
+    const uint64_t maxscan = std::max<uint64_t>(std::max(ny1, std::max(nx1, nz1)), std::max<uint64_t>(nS, nR) + 1);
+
+    P.ny = (uint32_t)(B.hsc->tot_y >> 32);
+    P.nx = (uint32_t)(B.hsc->tot_x >> 32);
+    P.nnzR = (uint32_t)B.hsc->tot_x;
+    P.nnzS = (uint32_t)B.hsc->s_surv;
+    P.nz = P.z_ident ? (uint32_t)P.z_span : (uint32_t)(B.hsc->tot_z >> 32);
+    P.out_j = B.hsc->out_j;
     if (P.ny == 0 || P.nx == 0 || P.nz == 0 || P.nnzS == 0) {
         P.empty = true;
         return DIM3_OK;
@@ -739,7 +803,7 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
             dim3h::fill_u64((uint64_t*)hh, 4ull * dim3h::kDenseBins, 0, st);
             dim3h::dense_heavy_degrees(slotR, slotRx, nR, dS, slotS, P.z_ident ? nullptr : slotSz, P.zlo, nS, exY,
                                        hid, hdx, (uint32_t)nx1, hdz, (uint32_t)nz1, hh, st);
-            HIPCHK(c, hipMemcpyAsync(c->h_hist, hh, 4ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipMemcpyAsync(B.hhist, hh, 4ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
             HIPCHK(c, hipStreamSynchronize(st));
             const bool core = plan_core(c, o);
             P.fused = P.dense == DIM3_DENSE_BITOR && P.all_z;
@@ -827,7 +891,21 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     return DIM3_OK;
 }
 
-// Expansion of the prepared plan into `out` (cap rows) or the fingerprint.
+// The three stages of one query, synchronising after each.
+int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
+            const dim3_opts& o, hipStream_t st, int force_wgs = 0) {
+    TRY(prepare_start(c, count_mode, dR, nR, dS, nS, o, st));
+    auto& P = c->bank().plan;
+    if (P.empty) return DIM3_OK;
+    HIPCHK(c, hipStreamSynchronize(st));
+    TRY(prepare_map(c, o, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    P.ms_map = ms;
+    return prepare_plan(c, o, st, force_wgs);
+}
+
 // Expansion arguments of the current bank's prepared plan.
 dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) {
     auto& P = c->bank().plan;
@@ -1086,10 +1164,21 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     std::vector<Scalars*> scl(n);
     size_t lds = 0;
     uint64_t total = 0;
+    // each planning stage over every instance, then one synchronisation
     for (uint32_t i = 0; i < n; i++) {
         c->use_bank(i + 1);
-        TRY(prepare(c, count_mode, (const int2*)inst[i].R_xy, inst[i].n_r, (const int2*)inst[i].S_zy, inst[i].n_s,
-                    ob, st, wgs));
+        TRY(prepare_start(c, count_mode, (const int2*)inst[i].R_xy, inst[i].n_r, (const int2*)inst[i].S_zy,
+                          inst[i].n_s, ob, st));
+    }
+    HIPCHK(c, hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < n; i++) {
+        c->use_bank(i + 1);
+        TRY(prepare_map(c, ob, st));
+    }
+    HIPCHK(c, hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < n; i++) {
+        c->use_bank(i + 1);
+        TRY(prepare_plan(c, ob, st, wgs));
         const auto& P = c->bank().plan;
         args[i] = make_args(c, fp, inst[i].out, inst[i].cap);
         scl[i] = args[i].sc;
@@ -1111,11 +1200,11 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     TRY(ws(c, WS_BATCH_SCL, n, &d_scl));
     TRY(ws(c, WS_BATCH_SCOUT, n, &d_scout));
     if (c->h_batch_cap < n) {
-        if (c->h_batch) hipHostFree(c->h_batch);
+        if (c->h_batch) dim3h::host_free(c->h_batch);
         c->h_batch = nullptr;
         c->h_batch_cap = 0;
         const uint32_t cap = std::max<uint32_t>(n, 512);
-        HIPCHK(c, hipHostMalloc((void**)&c->h_batch, sizeof(Scalars) * cap, hipHostMallocDefault));
+        HIPCHK(c, dim3h::host_malloc((void**)&c->h_batch, sizeof(Scalars) * cap));
         c->h_batch_cap = cap;
     }
     HIPCHK(c, hipMemcpyAsync(d_args, args.data(), sizeof(dim3h::ExpandArgs) * n, hipMemcpyHostToDevice, st));
@@ -1132,11 +1221,14 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // no allocation anywhere in the process while the capture is open
+    std::unique_lock<std::shared_mutex> capture_lock(dim3h::capture_mutex());
     HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     dim3h::expand_batch_launch(kcount, fp, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl, d_scout,
                                st);
     hipMemcpyAsync(c->h_batch, d_scout, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st);
     HIPCHK(c, hipStreamEndCapture(st, &graph));
+    capture_lock.unlock();
     hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (ge != hipSuccess) {
         hipGraphDestroy(graph);
@@ -1392,8 +1484,8 @@ int dim3_ctx_create(int device, uint64_t hbm_budget_bytes, dim3_ctx** out) {
     hipMemGetInfo(&fr, &tot);
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(fr * 0.9);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_hist, 6 * dim3h::kDenseBins * sizeof(unsigned long long), hipHostMallocDefault) !=
+        dim3h::host_malloc((void**)&c->h_sc, sizeof(Scalars)) != hipSuccess ||
+        dim3h::host_malloc((void**)&c->h_hist, 6 * dim3h::kDenseBins * sizeof(unsigned long long)) !=
             hipSuccess) {
         delete c;
         return DIM3_ERR_DEVICE;
@@ -1411,12 +1503,12 @@ int dim3_ctx_destroy(dim3_ctx* c) {
     hipStreamSynchronize(c->stream);
     for (auto& b : c->banks)
         for (auto& s : b->ws)
-            if (s.p) hipFree(s.p);
-    if (c->h_batch) hipHostFree(c->h_batch);
+            if (s.p) dim3h::dev_free(s.p);
+    if (c->h_batch) dim3h::host_free(c->h_batch);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
-    if (c->h_sc) hipHostFree(c->h_sc);
-    if (c->h_hist) hipHostFree(c->h_hist);
+    if (c->h_sc) dim3h::host_free(c->h_sc);
+    if (c->h_hist) dim3h::host_free(c->h_hist);
     hipStreamDestroy(c->stream);
     delete c;
     return DIM3_OK;
@@ -1441,7 +1533,7 @@ int dim3_ctx_release(dim3_ctx* c) {
     for (auto& b : c->banks)
         for (auto& s : b->ws)
             if (s.p) {
-                hipFreeAsync(s.p, c->stream);
+                dim3h::dev_free_async(s.p, c->stream);
                 c->held -= s.bytes;
                 s.p = nullptr;
                 s.bytes = 0;

@@ -14,6 +14,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <shared_mutex>
+
 #include "../../include/dim3.h"
 
 #define DIM3_EMPTY_KEY 0xFFFFFFFFu
@@ -292,6 +295,36 @@ __device__ __forceinline__ T block_reduce_sum(T v, T* sh) {
 // host-side launch wrappers (defined in k_mapping.hip / k_expand.hip)
 // ---------------------------------------------------------------------------
 namespace dim3h {
+
+// HIP refuses memory (de)allocation while any stream of the process is
+// capturing a graph, so batch captures (exclusive) and every allocation or
+// free of the library (shared) exclude each other through one lock.
+std::shared_mutex& capture_mutex();
+inline hipError_t dev_malloc(void** p, size_t n) {
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    return hipMalloc(p, n);
+}
+inline hipError_t dev_free(void* p) {
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    return hipFree(p);
+}
+inline hipError_t dev_malloc_async(void** p, size_t n, hipStream_t st) {
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    return hipMallocAsync(p, n, st);
+}
+inline hipError_t dev_free_async(void* p, hipStream_t st) {
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    return hipFreeAsync(p, st);
+}
+inline hipError_t host_malloc(void** p, size_t n) {
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    return hipHostMalloc(p, n, hipHostMallocDefault);
+}
+inline hipError_t host_free(void* p) {
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    return hipHostFree(p);
+}
+
 
 // Device scalar block, zeroed at the start of every query.
 struct Scalars {

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/dim3.h"
+#include "dim3_internal.h"
 
 namespace {
 
@@ -166,10 +167,10 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
                        : dim3_join_project_dev(c, R[0]->d, R[0]->n, S[0]->d, S[0]->n, &o, (int32_t*)*out_buf, cap,
                                                st, &res);
         if (rc == DIM3_ERR_CAPACITY && attempt == 0) {  // grow the output buffer to the size reported
-            if (*out_buf) hipFree(*out_buf);
+            if (*out_buf) dim3h::dev_free(*out_buf);
             *out_buf = nullptr;
             *out_bytes = 0;
-            if (hipMalloc(out_buf, res.n * row) != hipSuccess) {
+            if (dim3h::dev_malloc(out_buf, res.n * row) != hipSuccess) {
                 (void)hipGetLastError();
                 *out_buf = nullptr;
                 j->status = DIM3_ERR_OOM;
@@ -210,7 +211,7 @@ void dim3_server::worker(uint32_t w) {
                     release[w] = 0;
                     lk.unlock();
                     dim3_ctx_release(ctxs[w]);
-                    if (out_buf) hipFree(out_buf);
+                    if (out_buf) dim3h::dev_free(out_buf);
                     out_buf = nullptr;
                     out_bytes = 0;
                     lk.lock();
@@ -304,7 +305,7 @@ void dim3_server::worker(uint32_t w) {
         uint64_t h = dim3_ctx_device_bytes(ctxs[w]) + out_bytes;
         if (h > budget / 2) {  // give very large workspaces back (others are released on demand)
             dim3_ctx_release(ctxs[w]);
-            if (out_buf) hipFree(out_buf);
+            if (out_buf) dim3h::dev_free(out_buf);
             out_buf = nullptr;
             out_bytes = 0;
             h = 0;
@@ -319,7 +320,7 @@ void dim3_server::worker(uint32_t w) {
                 for (uint64_t hdl : {j->req.r, j->req.s}) {
                     Rel& r = rels.at(hdl);
                     if (--r.users == 0 && r.dropped) {
-                        hipFree(r.d);
+                        dim3h::dev_free(r.d);
                         resident -= r.bytes;
                         rels.erase(hdl);
                     }
@@ -331,7 +332,7 @@ void dim3_server::worker(uint32_t w) {
         cv_done.notify_all();
         cv_work.notify_all();
     }
-    if (out_buf) hipFree(out_buf);
+    if (out_buf) dim3h::dev_free(out_buf);
 }
 
 extern "C" {
@@ -421,7 +422,7 @@ int dim3_server_destroy(dim3_server* s) {
     s->cv_work.notify_all();
     for (auto& t : s->threads) t.join();
     hipSetDevice(s->device);
-    for (auto& kv : s->rels) hipFree(kv.second.d);
+    for (auto& kv : s->rels) dim3h::dev_free(kv.second.d);
     for (auto& kv : s->jobs) dim3_result_free(&kv.second.res);
     for (auto* c : s->ctxs) dim3_ctx_destroy(c);
     for (auto x : s->full_stream) hipStreamDestroy(x);
@@ -441,14 +442,14 @@ static int put_relation(dim3_server* s, const int32_t* pairs, uint64_t n, uint64
     }
     hipSetDevice(s->device);
     int32_t* d = nullptr;
-    if (hipMalloc(&d, bytes) != hipSuccess) {
+    if (dim3h::dev_malloc((void**)&d, bytes) != hipSuccess) {
         (void)hipGetLastError();
         std::lock_guard<std::mutex> lk(s->mu);
         s->resident -= bytes;
         return DIM3_ERR_OOM;
     }
     if (n && hipMemcpy(d, pairs, n * 8, kind) != hipSuccess) {
-        hipFree(d);
+        dim3h::dev_free(d);
         std::lock_guard<std::mutex> lk(s->mu);
         s->resident -= bytes;
         return DIM3_ERR_DEVICE;
@@ -482,7 +483,7 @@ int dim3_server_drop_relation(dim3_server* s, uint64_t handle) {
         return DIM3_OK;
     }
     hipSetDevice(s->device);
-    hipFree(it->second.d);
+    dim3h::dev_free(it->second.d);
     s->resident -= it->second.bytes;
     s->rels.erase(it);
     return DIM3_OK;
