@@ -104,9 +104,11 @@ __device__ __forceinline__ uint32_t units_of(uint32_t v) {
 }
 
 // LDS layout (u32 words):
+//   [fingerprint table, 2 * kFpTableU64 words, fingerprint mode only] |
 //   tile[tile_words] | touched[touched_cap] | area[max(stage_cap, 3*BLOCK+1)]
-// `area` holds the segment table of the row during expansion and the staged
-// local z offsets during emission.
+// `area` holds the segment table of the row during expansion (and the heavy
+// key list of a fused bit-or seed) and the staged local z offsets during
+// emission.
 // ZID: z IDs are value - zlo (dense domain), so no id -> value lookup.
 // One (x, z-tile) task: dense seed, expansion of the light paths into the
 // LDS tile, then emission (rows or fingerprint) and the clear of the tile.
