@@ -42,6 +42,7 @@ extern "C" {
 #define DIM3_ERR_BUSY 6      /* server: queue full */
 #define DIM3_ERR_NOT_FOUND 7 /* server: unknown ticket or relation handle */
 #define DIM3_ERR_UNSUPPORTED 8
+#define DIM3_ERR_CANCELLED 9   /* a streamed query stopped by its row callback */
 
 /* ---- output modes ---- */
 #define DIM3_OUT_MATERIALIZE 0 /* rows are written out */
@@ -168,6 +169,25 @@ int dim3_join_project_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_r, const
 int dim3_join_aggregate_count_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_r, const int32_t* d_S,
                                   uint64_t n_s, const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap,
                                   void* stream, dim3_result* res);
+
+/* ---- streamed output: rows handed to the caller in chunks ----
+ * For outputs larger than any one buffer (the chunked STREAM output mode).
+ * The query runs in chunks of consecutive x rows, each producing at most
+ * max(chunk_rows, the largest row's output) rows (0 = 2^24); after each chunk
+ * `cb` gets n rows in pinned host memory that stays valid during the call:
+ * project 2n int32 (x, z) pairs, COUNT n dim3_count_row.  Rows are distinct
+ * across chunks and in no particular order (as dim3_join_project's); a
+ * nonzero return from `cb` stops the query with DIM3_ERR_CANCELLED.  res
+ * carries n (all rows), swapped and the stats; no rows.  opts->output_mode is
+ * ignored.  No counterpart in the reference (its result is one in-memory
+ * myvector, dim3.hpp:982-989). */
+typedef int (*dim3_rows_cb)(void* user, const void* rows, uint64_t n);
+int dim3_join_project_stream(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
+                             const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb, void* user,
+                             dim3_result* res);
+int dim3_join_aggregate_count_stream(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy,
+                                     uint64_t n_s, const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb,
+                                     void* user, dim3_result* res);
 
 /* ---- batches: many independent instances in one persistent launch ----
  * No counterpart in the reference (it runs one query per call); the batch

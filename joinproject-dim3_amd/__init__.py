@@ -18,7 +18,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdim3.so")
 
-OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_ADMISSION, ERR_BUSY, ERR_NOT_FOUND, ERR_UNSUPPORTED = range(9)
+(OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_ADMISSION, ERR_BUSY, ERR_NOT_FOUND, ERR_UNSUPPORTED,
+ ERR_CANCELLED) = range(10)
 OUT_MATERIALIZE, OUT_FINGERPRINT = 0, 1
 OPT_NO_SWAP, OPT_EXCLUDE_EQUAL, OPT_HASH_ONLY = 1, 2, 4
 DENSE_AUTO, DENSE_OFF, DENSE_BITOR, DENSE_MFMA = -1, 0, 1, 2
@@ -106,11 +107,14 @@ EXPORTS = [
     "dim3_batch_join_project_dev", "dim3_batch_join_aggregate_count_dev", "dim3_autotune", "dim3_get_tuning",
     "dim3_set_tuning", "dim3_ctx_release", "dim3_ctx_set_cu_share", "dim3_server_opts_default", "dim3_server_create", "dim3_server_destroy",
     "dim3_server_put_relation", "dim3_server_put_relation_dev", "dim3_server_drop_relation", "dim3_submit",
-    "dim3_poll", "dim3_wait", "dim3_server_get_stats",
-    "dim3_version",
+    "dim3_poll", "dim3_wait", "dim3_server_get_stats", "dim3_join_project_stream",
+    "dim3_join_aggregate_count_stream", "dim3_version",
 ]
 
 _lib = None
+
+# int (*dim3_rows_cb)(void* user, const void* rows, uint64_t n)
+ROWS_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64)
 
 
 class Dim3Error(RuntimeError):
@@ -156,6 +160,10 @@ def lib():
     for name in ("dim3_join_project_dev", "dim3_join_aggregate_count_dev"):
         f = getattr(L, name)
         f.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(Opts), vp, u64, vp, ctypes.POINTER(Result)]
+        f.restype = i32
+    for name in ("dim3_join_project_stream", "dim3_join_aggregate_count_stream"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(Opts), u64, ROWS_CB, vp, ctypes.POINTER(Result)]
         f.restype = i32
     L.dim3_result_free.argtypes = [ctypes.POINTER(Result)]
     L.dim3_result_free.restype = None
@@ -340,6 +348,43 @@ class Context:
         out = QueryResult(res, rows=rows)
         self._L.dim3_result_free(ctypes.byref(res))
         return out
+
+    def _stream(self, fn, name, R, S, on_rows, chunk_rows, width, dtype, kw):
+        opts = make_opts(**kw)
+        R, S = _pairs(R), _pairs(S)
+        res = Result()
+        err = []
+
+        def cb(_user, rows, n):
+            try:
+                buf = (ctypes.c_char * (width * n)).from_address(rows)
+                return 1 if on_rows(np.frombuffer(buf, dtype=dtype)) is False else 0
+            except BaseException as e:  # stop the query, re-raised below
+                err.append(e)
+                return 1
+
+        c_cb = ROWS_CB(cb)
+        rc = fn(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0], ctypes.byref(opts), chunk_rows, c_cb, None,
+                ctypes.byref(res))
+        if err:
+            raise err[0]
+        if rc != ERR_CANCELLED:
+            self._check(rc, name)
+        return rc, QueryResult(res)
+
+    def join_project_stream(self, R, S, on_rows, chunk_rows=0, **kw):
+        """Streamed join-project: on_rows(pairs) gets each chunk as an (n, 2)
+        int32 view valid during the call (copy what you keep); returning
+        False stops the query (status ERR_CANCELLED).  Returns (status, result)."""
+        pair = np.dtype([("x", np.int32), ("z", np.int32)])
+        return self._stream(self._L.dim3_join_project_stream, "dim3_join_project_stream", R, S,
+                            lambda a: on_rows(a.view(np.int32).reshape(-1, 2)), chunk_rows, 8, pair, kw)
+
+    def join_aggregate_count_stream(self, R, S, on_rows, chunk_rows=0, **kw):
+        """Streamed COUNT join-aggregate: on_rows(rows) gets each chunk as a
+        COUNT_ROW_DTYPE view (fields x, z, count) valid during the call."""
+        return self._stream(self._L.dim3_join_aggregate_count_stream, "dim3_join_aggregate_count_stream", R, S,
+                            on_rows, chunk_rows, 16, COUNT_ROW_DTYPE, kw)
 
     def join_project_dev(self, d_R, n_r, d_S, n_s, d_out, cap, stream=None, **kw):
         """Device-resident variant: d_R/d_S/d_out are device addresses (ints).

@@ -60,6 +60,7 @@ enum SlotId {
     WS_ZID_S,
     WS_COL_S,
     WS_OUT,
+    WS_ROW_BOUND,
     WS_XW,
     WS_ZW,
     WS_DHIST,
@@ -184,6 +185,8 @@ struct dim3_ctx {
     unsigned long long* h_hist = nullptr;  // pinned mirror of the dense-split histogram
     Scalars* h_batch = nullptr;            // pinned scalars of a batch
     uint32_t h_batch_cap = 0;
+    void* h_stream = nullptr;              // pinned chunk buffer of a streamed query
+    size_t h_stream_bytes = 0;
     hipEvent_t ev[8] = {};
     std::recursive_mutex mu;
     DenseModel model;           // cost-model rates (defaults or measured)
@@ -925,6 +928,8 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.x2z = (P.flags & DIM3_OPT_EXCLUDE_EQUAL) ? (const int32_t*)W[WS_X2Z].p : nullptr;
     a.nx = P.nx;
     a.ntiles = P.ntiles;
+    a.task_lo = 0;
+    a.task_hi = P.nx * P.ntiles;
     a.tile_size = P.tile_size;
     a.tile_words = P.tile_words;
     a.touched_cap = P.touched_cap;
@@ -953,19 +958,30 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     return a;
 }
 
+int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st);
+
 // Expansion of the prepared plan into `out` (cap rows) or the fingerprint.
 int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
+    auto& P = c->bank().plan;
+    Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
+    if (P.empty) {
+        // reset the counters the expansion owns (a re-run after a capacity miss)
+        hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
+        HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        return DIM3_OK;
+    }
+    return expand_args(c, make_args(c, fp_mode, out, cap), st);
+}
+
+// One expansion launch with prepared arguments (all rows, or one chunk of
+// rows of a streamed query); the scalars are reset first and read back after.
+int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st) {
     auto& P = c->bank().plan;
     Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
     // reset the counters the expansion owns (a re-run after a capacity miss)
     hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
     hipMemsetAsync(&sc->task, 0, sizeof(uint32_t), st);
-    if (P.empty) {
-        HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-        HIPCHK(c, hipStreamSynchronize(st));
-        return DIM3_OK;
-    }
-    const dim3h::ExpandArgs a = make_args(c, fp_mode, out, cap);
     HIPCHK(c, hipEventRecord(c->ev[4], st));
     int e = P.count_mode ? dim3h::expand_count(a, P.block, c->grid_cus(), st)
                          : dim3h::expand_project(a, P.block, c->grid_cus(), st);
@@ -1122,6 +1138,98 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
         else
             res->pairs = (int32_t*)h;
     }
+    return DIM3_OK;
+}
+
+// Streamed output: the plan's rows are expanded in chunks of consecutive x
+// rows whose output bound (row_bounds) fits a device buffer of
+// max(chunk_rows, largest row bound) rows; each chunk is copied to a pinned
+// host buffer and handed to the callback before the next one runs.
+int run_stream(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const int32_t* S, uint64_t nS,
+               const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb, void* user, dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if (!cb) return fail(c, DIM3_ERR_INVALID, "null row callback");
+    if ((nR && !R) || (nS && !S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    o.output_mode = DIM3_OUT_MATERIALIZE;
+    if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
+        return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    c->qstream = st;
+    int2 *dR, *dS;
+    TRY(ws(c, WS_IN_R, std::max<uint64_t>(nR, 1), &dR));
+    TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
+    if (nR) HIPCHK(c, hipMemcpyAsync(dR, R, nR * 8, hipMemcpyHostToDevice, st));
+    if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
+    TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
+    {
+        // a multiplicity above 127 in the int8 product: plan again all-sparse
+        Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
+        HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if (dense_overflowed(c, *c->h_sc)) {
+            o.dense_mode = DIM3_DENSE_OFF;
+            TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
+        }
+    }
+    const auto& P = c->bank().plan;
+    const size_t row_bytes = P.count_mode ? sizeof(dim3_count_row) : 8;
+    uint64_t total = 0, paths = 0;
+    double ms_expand = 0;
+    if (!P.empty) {
+        uint32_t* bound;
+        TRY(ws(c, WS_ROW_BOUND, P.nx, &bound));
+        dim3h::ExpandArgs a = make_args(c, false, nullptr, 0);
+        dim3h::row_bounds(a, P.nz, bound, st);
+        std::vector<uint32_t> hb(P.nx);
+        HIPCHK(c, hipMemcpyAsync(hb.data(), bound, (size_t)P.nx * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        uint64_t cap = chunk_rows ? chunk_rows : (1ull << 24);
+        for (uint32_t b : hb) cap = std::max<uint64_t>(cap, b);
+        void* dout;
+        TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
+        if (c->h_stream_bytes < cap * row_bytes) {
+            if (c->h_stream) dim3h::host_free(c->h_stream);
+            c->h_stream = nullptr;
+            c->h_stream_bytes = 0;
+            HIPCHK(c, dim3h::host_malloc(&c->h_stream, cap * row_bytes));
+            c->h_stream_bytes = cap * row_bytes;
+        }
+        for (uint32_t x0 = 0; x0 < P.nx;) {
+            uint32_t x1 = x0;
+            uint64_t acc = 0;
+            while (x1 < P.nx && (x1 == x0 || acc + hb[x1] <= cap)) acc += hb[x1++];
+            a = make_args(c, false, dout, cap);
+            a.task_lo = x0 * P.ntiles;
+            a.task_hi = x1 * P.ntiles;
+            TRY(expand_args(c, a, st));
+            const uint64_t n = c->h_sc->out_n;
+            if (n > cap)
+                return fail(c, DIM3_ERR_DEVICE, "rows %u..%u produced %llu rows above their bound %llu", x0, x1,
+                            (unsigned long long)n, (unsigned long long)cap);
+            float ms = 0;
+            hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+            ms_expand += ms;
+            paths += c->h_sc->paths;
+            if (n) {
+                HIPCHK(c, hipMemcpyAsync(c->h_stream, dout, n * row_bytes, hipMemcpyDeviceToHost, st));
+                HIPCHK(c, hipStreamSynchronize(st));
+                if (cb(user, c->h_stream, n) != 0) return fail(c, DIM3_ERR_CANCELLED, "stopped by the row callback");
+            }
+            total += n;
+            x0 = x1;
+        }
+    }
+    fill_result(c, res, *c->h_sc, false);
+    res->n = total;
+    res->fp.count = total;
+    res->stats.paths_sparse = paths;
+    res->stats.ms_expand = ms_expand;
     return DIM3_OK;
 }
 
@@ -1443,6 +1551,7 @@ const char* dim3_status_string(int s) {
         case DIM3_OK: return "ok";
         case DIM3_ERR_INVALID: return "invalid argument";
         case DIM3_ERR_OOM: return "out of memory";
+        case DIM3_ERR_CANCELLED: return "stopped by the row callback";
         case DIM3_ERR_DEVICE: return "device error";
         case DIM3_ERR_CAPACITY: return "output capacity too small";
         case DIM3_ERR_ADMISSION: return "request can never be admitted";
@@ -1505,6 +1614,7 @@ int dim3_ctx_destroy(dim3_ctx* c) {
         for (auto& s : b->ws)
             if (s.p) dim3h::dev_free(s.p);
     if (c->h_batch) dim3h::host_free(c->h_batch);
+    if (c->h_stream) dim3h::host_free(c->h_stream);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->h_sc) dim3h::host_free(c->h_sc);
@@ -1553,6 +1663,18 @@ int dim3_join_project(dim3_ctx* c, const int32_t* R_xy, uint64_t n_r, const int3
 int dim3_join_aggregate_count(dim3_ctx* c, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
                               const dim3_opts* opts, dim3_result* res) {
     return run_host(c, true, R_xy, n_r, S_zy, n_s, opts, res);
+}
+
+int dim3_join_project_stream(dim3_ctx* c, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
+                             const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb, void* user,
+                             dim3_result* res) {
+    return run_stream(c, false, R_xy, n_r, S_zy, n_s, opts, chunk_rows, cb, user, res);
+}
+
+int dim3_join_aggregate_count_stream(dim3_ctx* c, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy,
+                                     uint64_t n_s, const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb,
+                                     void* user, dim3_result* res) {
+    return run_stream(c, true, R_xy, n_r, S_zy, n_s, opts, chunk_rows, cb, user, res);
 }
 
 void dim3_result_free(dim3_result* res) {

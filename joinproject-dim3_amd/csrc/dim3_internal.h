@@ -448,6 +448,9 @@ struct ExpandArgs {
     uint32_t stage_cap;
     uint32_t fp_mode;
     uint32_t swz;  // project tile bit layout: 1 swizzled, 0 plain (dense seeding copies words)
+    // the (x, z-tile) tasks of this launch: x * ntiles + t in [task_lo, task_hi)
+    // (all rows, or one chunk of rows of a streamed query)
+    uint32_t task_lo, task_hi;
     Scalars* sc;
     void* out;  // int2* (project) or dim3_count_row* (aggregate)
     uint64_t cap;
@@ -508,6 +511,9 @@ void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t*
                    const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
                    hipStream_t st);
 
+// bound[x] >= output rows of row x: its S paths (plus the product columns
+// of a product row), at most nz
+void row_bounds(const ExpandArgs& a, uint32_t nz, uint32_t* bound, hipStream_t st);
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 // dynamic LDS of one expansion workgroup (fp: plus the fingerprint table)

@@ -639,7 +639,6 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                 dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t ntasks = a.nx * a.ntiles;
     if (a.fp_mode) fp_table_init<BLOCK>(reinterpret_cast<uint64_t*>(smem));
     for (uint32_t w = tid; w < a.tile_words; w += BLOCK) L.tile[w] = 0;
     uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
@@ -651,8 +650,8 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         }
         __syncthreads();
         // task and x are wavefront-uniform: scalar loads
-        const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
-        if (task >= ntasks) break;
+        const uint32_t task = a.task_lo + __builtin_amdgcn_readfirstlane(s_task);
+        if (task >= a.task_hi) break;
         const uint32_t x = task / a.ntiles;
         row_task<BLOCK, COUNT, ZID>(a, x, task - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
     }
@@ -720,6 +719,19 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
     if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);
 }
 
+// Upper bound of each row's output: its S paths over all tiles (a product
+// row also its product columns), capped at nz.
+__global__ __launch_bounds__(256) void k_row_bound(ExpandArgs a, uint32_t nz, uint32_t* __restrict__ bound) {
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
+    if (x >= a.nx) return;
+    uint64_t b = (a.hx != nullptr && a.hx[x] >= 0) ? a.Nh : 0u;
+    for (uint32_t i = a.rowptrR[x], e = a.rowptrR[x + 1]; i < e && b < nz; i++) {
+        const uint32_t y = (uint32_t)a.colR[i] & 0x7FFFFFFFu;
+        b += a.rowptrS[(y + 1u) * a.ntiles] - a.rowptrS[y * a.ntiles];
+    }
+    bound[x] = (uint32_t)min<uint64_t>(b, nz);
+}
+
 // Zero the expansion-owned scalars of every instance of a batch (a kernel, so
 // a captured graph holds no memset node).
 __global__ void k_reset_scalars(dim3h::Scalars* const* __restrict__ list, uint32_t n, uint32_t* counter) {
@@ -752,7 +764,7 @@ int launch_persistent(K kern, int block, size_t lds, const ExpandArgs& a, int cu
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, lds);
     if (e != hipSuccess) return (int)e;
     if (occ < 1) occ = 1;
-    const uint64_t ntasks = (uint64_t)a.nx * a.ntiles;
+    const uint64_t ntasks = a.task_hi > a.task_lo ? a.task_hi - a.task_lo : 0;
     uint64_t grid = (uint64_t)cu_count * (uint64_t)occ;
     if (grid > ntasks) grid = ntasks;
     if (grid == 0) return 0;
@@ -785,6 +797,10 @@ namespace dim3h {
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp) {
     const uint32_t area = stage_cap > 3u * block + 1u ? stage_cap : 3u * block + 1u;
     return ((size_t)tile_words + touched_cap + area) * 4u + (fp ? (size_t)kFpTableU64 * 8u : 0u);
+}
+
+void row_bounds(const ExpandArgs& a, uint32_t nz, uint32_t* bound, hipStream_t st) {
+    if (a.nx) hipLaunchKernelGGL(k_row_bound, dim3((a.nx + 255u) / 256u), dim3(256), 0, st, a, nz, bound);
 }
 
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
