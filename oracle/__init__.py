@@ -15,6 +15,11 @@ Pinning of each oracle function (what would fail if it were wrong):
   oracle_join_aggregate_count  compiled reference with a COUNT policy,
                                brute force via a dict, closed forms
                                sum(count) = OUT_J and #groups = |project|.
+  oracle_join_aggregate        compiled reference with SUM / MIN / MAX
+                               policies and its AggregateFunctionAvg<long long>,
+                               brute force via a dict on tiny inputs, and
+                               closed forms (SUM with vS = 0 is vR times the
+                               COUNT, AVG of constant values is the constant).
   oracle_outj                  brute force on tiny inputs, sum of counts.
   oracle_estimate              compiled reference estimate on the same inputs.
   oracle_gen_rand_data         compiled reference generator (byte equality)
@@ -57,6 +62,8 @@ def lib():
         L.oracle_join_project.restype = i32
         L.oracle_join_aggregate_count.argtypes = [vp, u64, vp, u64, i32, i32, vp, vp, vp, vp, vp]
         L.oracle_join_aggregate_count.restype = i32
+        L.oracle_join_aggregate.argtypes = [vp, vp, u64, vp, vp, u64, i32, i32, i32, vp, vp, vp, vp, vp]
+        L.oracle_join_aggregate.restype = i32
         L.oracle_outj.argtypes = [vp, u64, vp, u64]
         L.oracle_outj.restype = u64
         L.oracle_estimate.argtypes = [vp, u64, vp, u64]
@@ -91,6 +98,8 @@ def ref():
         L.ref_join_project.restype = ctypes.c_int
         L.ref_join_aggregate_count.argtypes = [vp, u32, vp, u32, vp, vp, vp]
         L.ref_join_aggregate_count.restype = ctypes.c_int
+        L.ref_join_aggregate.argtypes = [vp, vp, u32, vp, vp, u32, ctypes.c_int, vp, vp, vp]
+        L.ref_join_aggregate.restype = ctypes.c_int
         L.ref_estimate.argtypes = [vp, u32, vp, u32]
         L.ref_estimate.restype = ctypes.c_longlong
         L.ref_gen_rand_data.argtypes = [vp, u32, vp, u32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -170,6 +179,39 @@ def join_aggregate_count(R, S, exclude_equal=False, fingerprint_only=False):
             fp, bool(sw.value))
 
 
+AGG = {"sum": 1, "min": 2, "max": 3, "avg": 4}
+
+
+def _vals(v, n):
+    v = np.ascontiguousarray(v, dtype=np.int32).reshape(-1)
+    if v.shape[0] != n:
+        raise ValueError("one value per tuple")
+    return v
+
+
+def join_aggregate(R, vR, S, vS, agg, exclude_equal=False, fingerprint_only=False):
+    """Value aggregate agg ("sum", "min", "max", "avg") of vR + vS over the
+    join paths of each (x, z).  Returns (pairs[n,2] int32, values[n] int64,
+    Fingerprint, swapped)."""
+    L = lib()
+    R, S = _pairs(R), _pairs(S)
+    vR, vS = _vals(vR, R.shape[0]), _vals(vS, S.shape[0])
+    out = ctypes.c_void_p()
+    val = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    fp = Fingerprint()
+    sw = ctypes.c_int()
+    rc = L.oracle_join_aggregate(_p(R), _p(vR), R.shape[0], _p(S), _p(vS), S.shape[0], AGG[agg],
+                                 int(exclude_equal), int(fingerprint_only), ctypes.byref(out),
+                                 ctypes.byref(val), ctypes.byref(n), ctypes.byref(fp), ctypes.byref(sw))
+    if rc != 0:
+        raise MemoryError("oracle_join_aggregate failed rc=%d" % rc)
+    if fingerprint_only:
+        return None, None, fp, bool(sw.value)
+    return (_take_pairs(L, out, n.value, L.oracle_free), _take_counts(val, n.value, L.oracle_free),
+            fp, bool(sw.value))
+
+
 def outj(R, S):
     L = lib()
     R, S = _pairs(R), _pairs(S)
@@ -233,6 +275,20 @@ def ref_join_aggregate_count(R, S):
     L.ref_join_aggregate_count(_p(R), R.shape[0], _p(S), S.shape[0], ctypes.byref(out),
                                ctypes.byref(cnt), ctypes.byref(n))
     return _take_pairs(L, out, n.value, L.ref_free), _take_counts(cnt, n.value, L.ref_free)
+
+
+def ref_join_aggregate(R, vR, S, vS, agg):
+    L = ref()
+    R, S = _pairs(R), _pairs(S)
+    vR, vS = _vals(vR, R.shape[0]), _vals(vS, S.shape[0])
+    out = ctypes.c_void_p()
+    val = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    rc = L.ref_join_aggregate(_p(R), _p(vR), R.shape[0], _p(S), _p(vS), S.shape[0], AGG[agg], ctypes.byref(out),
+                              ctypes.byref(val), ctypes.byref(n))
+    if rc != 0:
+        raise ValueError("ref_join_aggregate rc=%d" % rc)
+    return _take_pairs(L, out, n.value, L.ref_free), _take_counts(val, n.value, L.ref_free)
 
 
 def ref_estimate(R, S):

@@ -21,6 +21,9 @@
  *                           policy                        dim3.hpp:1267-1373
  *       first touch -> insert (count = 1), later -> update (+1); rows are
  *       emitted in first-touch order                      dim3.hpp:1286-1307
+ *   oracle_join_aggregate   SpmmAggregate with a SUM / MIN / MAX / AVG policy
+ *                           over the path value vR + vS   dim3.hpp:1267-1311,
+ *                           (the value AggregateFunctionAvg combines) 1453-1480
  *   oracle_estimate         EstimateJoinCardinality       dim3.hpp:193-224, 262-269
  *   oracle_outj             exact sum_b degR(b)*degS(b)   (closed form of the above)
  *   oracle_gen_rand_data    gen_rand_data (minstd_rand0)  dim3.hpp:1556-1568
@@ -226,21 +229,32 @@ typedef struct {
     uint64_t* sjr; /* ny+1 : S CSR by y, columns = z id */
     int32_t* sic;
     uint64_t* sidx; /* S CSR slot -> original S tuple index (for the row's values) */
+    int32_t* rval;  /* value of each R CSR entry (value aggregates only) */
+    int32_t* sval;  /* value of each S CSR entry */
 } orc_plan;
 
 static void plan_free(orc_plan* p) {
     free(p->id2x); free(p->id2z); free(p->rjr); free(p->ric);
-    free(p->sjr); free(p->sic); free(p->sidx);
+    free(p->sjr); free(p->sic); free(p->sidx); free(p->rval); free(p->sval);
     memset(p, 0, sizeof(*p));
 }
 
 /* R and S are interleaved (first, second) int32 pairs with the join key in
  * .second (main.cpp:6-11).  Returns ORC_OK; p->nx == 0 means empty output. */
+static int build_plan_v(const int32_t* R, const int32_t* vR, uint64_t nR, const int32_t* S, const int32_t* vS,
+                        uint64_t nS, orc_plan* p);
 static int build_plan(const int32_t* R, uint64_t nR, const int32_t* S, uint64_t nS, orc_plan* p) {
+    return build_plan_v(R, NULL, nR, S, NULL, nS, p);
+}
+
+/* As build_plan; with vR / vS given, the CSR entries also carry the value
+ * of their tuple (the VAL arrays of dim3.hpp:1376-1450). */
+static int build_plan_v(const int32_t* R, const int32_t* vR, uint64_t nR, const int32_t* S, const int32_t* vS,
+                        uint64_t nS, orc_plan* p) {
     memset(p, 0, sizeof(*p));
     omap my, mx, mz;
     int rc = ORC_OK;
-    int32_t* ry = NULL; int32_t* rx = NULL; int32_t* sy = NULL; int32_t* sz = NULL;
+    int32_t* ry = NULL; int32_t* rx = NULL; int32_t* sy = NULL; int32_t* sz = NULL; int32_t* sv = NULL;
     uint64_t nSm = 0;
     if (omap_init(&my, nR) || omap_init(&mx, nR) || omap_init(&mz, nS)) return ORC_OOM;
 
@@ -248,13 +262,14 @@ static int build_plan(const int32_t* R, uint64_t nR, const int32_t* S, uint64_t 
     rx = (int32_t*)malloc((nR ? nR : 1) * sizeof(int32_t));
     sy = (int32_t*)malloc((nS ? nS : 1) * sizeof(int32_t));
     sz = (int32_t*)malloc((nS ? nS : 1) * sizeof(int32_t));
-    if (!ry || !rx || !sy || !sz) { rc = ORC_OOM; goto done; }
+    sv = (int32_t*)malloc((nS ? nS : 1) * sizeof(int32_t));
+    if (!ry || !rx || !sy || !sz || !sv) { rc = ORC_OOM; goto done; }
 
     /* mapping_y: y IDs from R, S probes them and drops misses (dim3.hpp:822-836) */
     for (uint64_t i = 0; i < nR; i++) ry[i] = omap_insert(&my, R[2 * i + 1]);
     for (uint64_t i = 0; i < nS; i++) {
         int32_t t = omap_find(&my, S[2 * i + 1]);
-        if (t != -1) { sy[nSm] = t; sz[nSm] = S[2 * i]; nSm++; }
+        if (t != -1) { sy[nSm] = t; sz[nSm] = S[2 * i]; sv[nSm] = vS ? vS[i] : 0; nSm++; }
     }
     /* dim3.hpp:1027: nothing survives -> empty result */
     if (nR == 0 || nSm == 0) goto done;
@@ -279,7 +294,14 @@ static int build_plan(const int32_t* R, uint64_t nR, const int32_t* S, uint64_t 
         uint64_t* cur = (uint64_t*)malloc(((size_t)p->nx + 1) * sizeof(uint64_t));
         if (!cur) { rc = ORC_OOM; goto done; }
         memcpy(cur, p->rjr, ((size_t)p->nx + 1) * sizeof(uint64_t));
-        for (uint64_t i = 0; i < nR; i++) p->ric[cur[rx[i]]++] = ry[i];
+        if (vR) {
+            p->rval = (int32_t*)malloc((nR ? nR : 1) * sizeof(int32_t));
+            if (!p->rval) { free(cur); rc = ORC_OOM; goto done; }
+        }
+        for (uint64_t i = 0; i < nR; i++) {
+            if (vR) p->rval[cur[rx[i]]] = vR[i];
+            p->ric[cur[rx[i]]++] = ry[i];
+        }
         free(cur);
     }
 
@@ -294,12 +316,19 @@ static int build_plan(const int32_t* R, uint64_t nR, const int32_t* S, uint64_t 
         uint64_t* cur = (uint64_t*)malloc(((size_t)p->ny + 1) * sizeof(uint64_t));
         if (!cur) { rc = ORC_OOM; goto done; }
         memcpy(cur, p->sjr, ((size_t)p->ny + 1) * sizeof(uint64_t));
-        for (uint64_t i = 0; i < nSm; i++) p->sic[cur[sy[i]]++] = sz[i];
+        if (vS) {
+            p->sval = (int32_t*)malloc((nSm ? nSm : 1) * sizeof(int32_t));
+            if (!p->sval) { free(cur); rc = ORC_OOM; goto done; }
+        }
+        for (uint64_t i = 0; i < nSm; i++) {
+            if (vS) p->sval[cur[sy[i]]] = sv[i];
+            p->sic[cur[sy[i]]++] = sz[i];
+        }
         free(cur);
     }
 
 done:
-    free(ry); free(rx); free(sy); free(sz);
+    free(ry); free(rx); free(sy); free(sz); free(sv);
     omap_free(&my); omap_free(&mx); omap_free(&mz);
     if (rc != ORC_OK) plan_free(p);
     return rc;
@@ -393,6 +422,75 @@ int oracle_join_project(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, u
 }
 
 /* Join-aggregate COUNT(*) GROUP BY x,z.  Same swap rule (dim3.hpp:1325-1326). */
+/* Value aggregates: SpmmAggregate (dim3.hpp:1267-1311) with a policy over
+ * the path value v = vR + vS (the combination AggregateFunctionAvg uses,
+ * dim3.hpp:1466-1473): 1 SUM of v, 2 MIN, 3 MAX, 4 AVG = SUM / number of
+ * paths, truncated toward zero (its get() on integral values).  All int64. */
+static int run_value(const orc_plan* p, int agg, int exclude_equal, orc_sink* k) {
+    if (p->nx == 0) return ORC_OK;
+    int32_t* spaw = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    int64_t* cnt = (int64_t*)malloc((size_t)p->nz * sizeof(int64_t));
+    int64_t* acc = (int64_t*)malloc((size_t)p->nz * sizeof(int64_t));
+    int32_t* list = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    if (!spaw || !cnt || !acc || !list) { free(spaw); free(cnt); free(acc); free(list); return ORC_OOM; }
+    for (int32_t i = 0; i < p->nz; i++) spaw[i] = -1;
+    for (int32_t x = 0; x < p->nx; x++) {
+        int32_t xv = p->id2x[x];
+        int32_t nl = 0;
+        for (uint64_t i = p->rjr[x]; i < p->rjr[x + 1]; i++) {
+            int32_t y = p->ric[i];
+            for (uint64_t j = p->sjr[y]; j < p->sjr[y + 1]; j++) {
+                int32_t z = p->sic[j];
+                int64_t v = (int64_t)p->rval[i] + (int64_t)p->sval[j];
+                if (spaw[z] != x) { spaw[z] = x; cnt[z] = 1; acc[z] = v; list[nl++] = z; continue; }
+                cnt[z] += 1;
+                if (agg == 2) { if (v < acc[z]) acc[z] = v; }
+                else if (agg == 3) { if (v > acc[z]) acc[z] = v; }
+                else acc[z] += v;
+            }
+        }
+        for (int32_t i = 0; i < nl; i++) {
+            int32_t zv = p->id2z[list[i]];
+            if (exclude_equal && zv == xv) continue;
+            int64_t out = agg == 4 ? acc[list[i]] / cnt[list[i]] : acc[list[i]];
+            sink_push(k, xv, zv, out);
+        }
+    }
+    free(spaw); free(cnt); free(acc); free(list);
+    return k->oom ? ORC_OOM : ORC_OK;
+}
+
+/* The value aggregate (agg 1-4, see run_value) of R(x, y, vR) join S(z, y, vS);
+ * the swap rule moves the values with their relation. */
+int oracle_join_aggregate(const int32_t* R_xy, const int32_t* vR, uint64_t nR, const int32_t* S_zy,
+                          const int32_t* vS, uint64_t nS, int agg, int exclude_equal, int fingerprint_only,
+                          int32_t** out_pairs, int64_t** out_vals, uint64_t* n_out, orc_fp* fp, int* swapped) {
+    if (agg < 1 || agg > 4) return ORC_INVALID;
+    const int32_t* R = R_xy; const int32_t* S = S_zy;
+    int sw = 0;
+    if (nR < nS) {
+        const int32_t* t = R; R = S; S = t;
+        t = vR; vR = vS; vS = t;
+        uint64_t u = nR; nR = nS; nS = u; sw = 1;
+    }
+    if (swapped) *swapped = sw;
+    orc_plan p;
+    int rc = build_plan_v(R, vR, nR, S, vS, nS, &p);
+    if (rc) return rc;
+    orc_sink k;
+    memset(&k, 0, sizeof(k));
+    k.fingerprint_only = fingerprint_only;
+    k.with_counts = 1;
+    rc = p.nx ? run_value(&p, agg, exclude_equal, &k) : ORC_OK;
+    plan_free(&p);
+    if (fp) *fp = k.fp;
+    if (n_out) *n_out = k.fp.count;
+    if (rc) { free(k.pairs); free(k.counts); return rc; }
+    if (out_pairs) *out_pairs = k.pairs; else free(k.pairs);
+    if (out_vals) *out_vals = k.counts; else free(k.counts);
+    return ORC_OK;
+}
+
 int oracle_join_aggregate_count(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,
                                 int exclude_equal, int fingerprint_only,
                                 int32_t** out_pairs, int64_t** out_counts, uint64_t* n_out,

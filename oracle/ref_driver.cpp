@@ -7,6 +7,8 @@
 //   DIM3<int,int,int,myHasher x4,uint32_t>::doJoinProject          dim3.hpp:987
 //   DIM3_parallel<...>::doJoinProject_parallel (OpenMP, 8 threads)  dim3_parallel.hpp:142
 //   DIM3JoinAggregate<int,int,int,long long,AggCount,...>           dim3.hpp:1318
+//   DIM3JoinAggregate<int,int,int,long long,AggSum|AggMin|AggMax,...> and
+//   DIM3JoinAggregate<...,AggregateFunctionAvg<long long>,...>      dim3.hpp:1318, 1453
 //   EstimateJoinCardinality<...>::estimate                          dim3.hpp:193
 //
 // Guards (the reference's own behaviour there is undefined, see the
@@ -32,6 +34,23 @@ struct AggCount {
     inline void insert(const int key, const long long, const long long) { c[key] = 1; }
     inline void update(const int key, const long long, const long long) { c[key] += 1; }
     inline long long get(const int key) const { return c[key]; }
+};
+
+// Value policies over the path value v1 + v2 (the combination the shipped
+// AggregateFunctionAvg uses, dim3.hpp:1466-1473): SUM, MIN, MAX.
+template <int OP>
+struct AggValue {
+    long long* a;
+    explicit AggValue(int n) { a = (long long*)malloc((size_t)(n > 0 ? n : 1) * sizeof(long long)); }
+    ~AggValue() { free(a); }
+    inline void insert(const int key, const long long v1, const long long v2) { a[key] = v1 + v2; }
+    inline void update(const int key, const long long v1, const long long v2) {
+        const long long v = v1 + v2;
+        if (OP == 0) a[key] += v;
+        else if (OP == 1) a[key] = v < a[key] ? v : a[key];
+        else a[key] = v > a[key] ? v : a[key];
+    }
+    inline long long get(const int key) const { return a[key]; }
 };
 
 typedef DIM3<int, int, int, myHasher, myHasher, myHasher, myHasher, uint32_t> RefDIM3;
@@ -60,6 +79,13 @@ static bool guard_empty(const int32_t* R, uint32_t nR, const int32_t* S, uint32_
     if (nR == 0 || nS == 0) return true;
     if ((nR < 1000 || nS < 1000) && !has_paths_small(R, nR, S, nS)) return true;
     return false;
+}
+
+template <typename Pol>
+static void run_agg(myvector<TUPLE<int, int, long long>>& vr, myvector<TUPLE<int, int, long long>>& vs,
+                    myvector<TUPLE<int, int, long long>>& result) {
+    DIM3JoinAggregate<int, int, int, long long, Pol, myHasher, myHasher, myHasher, myHasher, uint32_t>::doJoinAggregate(
+        vr, vs, result);
 }
 
 extern "C" {
@@ -112,6 +138,39 @@ int ref_join_aggregate_count(const int32_t* R, uint32_t nR, const int32_t* S, ui
     free(vs.data);
     *out_pairs = p;
     *out_counts = c;
+    *nout = n;
+    return 0;
+}
+
+// Value aggregate agg (1 SUM, 2 MIN, 3 MAX, 4 AVG through the reference's own
+// AggregateFunctionAvg<long long>) of R(x, y, vR) join S(z, y, vS).
+int ref_join_aggregate(const int32_t* R, const int32_t* vR, uint32_t nR, const int32_t* S, const int32_t* vS,
+                       uint32_t nS, int agg, int32_t** out_pairs, int64_t** out_vals, uint64_t* nout) {
+    *out_pairs = NULL;
+    *out_vals = NULL;
+    *nout = 0;
+    if (agg < 1 || agg > 4) return 1;
+    if (guard_empty(R, nR, S, nS)) return 0;
+    myvector<TUPLE<int, int, long long>> vr(nR), vs(nS), result;
+    for (uint32_t i = 0; i < nR; i++) vr.push_back(TUPLE<int, int, long long>(R[2 * i], R[2 * i + 1], vR[i]));
+    for (uint32_t i = 0; i < nS; i++) vs.push_back(TUPLE<int, int, long long>(S[2 * i], S[2 * i + 1], vS[i]));
+    if (agg == 1) run_agg<AggValue<0>>(vr, vs, result);
+    else if (agg == 2) run_agg<AggValue<1>>(vr, vs, result);
+    else if (agg == 3) run_agg<AggValue<2>>(vr, vs, result);
+    else run_agg<AggregateFunctionAvg<long long>>(vr, vs, result);
+    uint32_t n = result.cur_num;
+    int32_t* p = (int32_t*)malloc((size_t)(n ? n : 1) * 2 * sizeof(int32_t));
+    int64_t* c = (int64_t*)malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+    for (uint32_t i = 0; i < n; i++) {
+        p[2 * i] = result.data[i].x;
+        p[2 * i + 1] = result.data[i].y;
+        c[i] = result.data[i].v;
+    }
+    free(result.data);
+    free(vr.data);
+    free(vs.data);
+    *out_pairs = p;
+    *out_vals = c;
     *nout = n;
     return 0;
 }

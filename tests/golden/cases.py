@@ -57,6 +57,22 @@ def small_cases():
     return cases
 
 
+def value_cases():
+    """Inputs of the value aggregates: some small cases with int32 values per
+    tuple (full int32 range, small ranges, negative sums)."""
+    cs = small_cases()
+    rng = np.random.default_rng(20240602)
+    out = {}
+    for name, lo, hi, nr in [("zipf", -(1 << 31), (1 << 31) - 1, 800), ("swap", -1000, 1000, 600),
+                             ("dups_neg", -50, 10, 3000), ("cross", -(1 << 31), (1 << 31) - 1, 150)]:
+        R, S = cs[name]
+        R = R[:nr]
+        vR = rng.integers(lo, hi, R.shape[0], endpoint=True).astype(np.int32)
+        vS = rng.integers(lo, hi, S.shape[0], endpoint=True).astype(np.int32)
+        out[name] = (R, vR, S, vS)
+    return out
+
+
 # medium cases: count + fingerprint of the reference output
 MEDIUM = {
     "kat": {"gen": "gen_rand_data", "nR": 10_000_000, "nS": 10_000_000, "mod": 1_000_000, "seed": 492,

@@ -4,6 +4,9 @@ sources).  Run in the build container:  python tests/golden/make_golden.py
 
 small_*.npz   inputs R, S and the reference's sorted join-project pairs and
               sorted COUNT rows (DIM3 serial and DIM3JoinAggregate + COUNT).
+values_*.npz  inputs R, vR, S, vS and the reference's sorted rows of the
+              SUM / MIN / MAX / AVG value aggregates (DIM3JoinAggregate with
+              value policies and its AggregateFunctionAvg<long long>).
 medium.json   inputs regenerated from a named seeded generator (with a
               sha256 of the inputs to detect generator drift), and the count
               and fingerprint of the reference's output.
@@ -40,6 +43,20 @@ def small():
         np.savez_compressed(os.path.join(HERE, "small_%s.npz" % name), R=R, S=S, project=p, agg_pairs=cp,
                             agg_counts=cc, swapped=np.array([R.shape[0] < S.shape[0]]))
         print(name, R.shape[0], S.shape[0], "->", p.shape[0], "pairs,", cp.shape[0], "groups", flush=True)
+
+
+def values():
+    for name, (R, vR, S, vS) in cases.value_cases().items():
+        rec = {}
+        pairs = None
+        for agg in ["sum", "min", "max", "avg"]:
+            p, v = oracle.ref_join_aggregate(R, vR, S, vS, agg)
+            p, v = oracle.sort_pairs(p, v)
+            assert pairs is None or np.array_equal(pairs, p)
+            pairs = p
+            rec[agg] = v
+        np.savez_compressed(os.path.join(HERE, "values_%s.npz" % name), R=R, vR=vR, S=S, vS=vS, pairs=pairs, **rec)
+        print(name, R.shape[0], S.shape[0], "->", pairs.shape[0], "groups", flush=True)
 
 
 def medium():
@@ -96,9 +113,11 @@ def large():
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("reference checker missing: make -C oracle ref")
-    which = sys.argv[1:] or ["small", "medium"]
+    which = sys.argv[1:] or ["small", "values", "medium"]
     if "small" in which:
         small()
+    if "values" in which:
+        values()
     if "medium" in which:
         medium()
     if "large" in which:

@@ -249,3 +249,85 @@ def test_oracle_estimate_and_generator_match_reference():
     # |S| <= 10000: the sample keeps every key and the estimate is exact
     R3, S3 = oracle.gen_rand_data(20_000, 8_000, 3_000, 500, 3_000, 5)
     assert oracle.estimate(R3, S3) == oracle.ref_estimate(R3, S3) == oracle.outj(R3, S3)
+
+
+def brute_values(R, vR, S, vS, agg):
+    byy = defaultdict(list)
+    for (z, y), v in zip(S.tolist(), vS.tolist()):
+        byy[y].append((z, v))
+    acc = defaultdict(list)
+    for (x, y), v in zip(R.tolist(), vR.tolist()):
+        for z, w in byy.get(y, ()):
+            acc[(x, z)].append(v + w)
+    f = {"sum": sum, "min": min, "max": max,
+         "avg": lambda l: int(sum(l) / len(l)) if abs(sum(l)) < 2 ** 52 else None}
+    return {k: f[agg](l) for k, l in acc.items()}
+
+
+def rand_vals(rng, n, lo=-(1 << 31), hi=(1 << 31) - 1):
+    return rng.integers(lo, hi, n, endpoint=True).astype(np.int32)
+
+
+@pytest.mark.parametrize("agg", ["sum", "min", "max", "avg"])
+@pytest.mark.parametrize("seed", range(3))
+def test_oracle_values_match_brute_force(agg, seed):
+    rng = np.random.default_rng(200 + seed)
+    R = rand_rel(rng, int(rng.integers(1, 400)), 30, 20, -2, -1)
+    S = rand_rel(rng, int(rng.integers(1, 400)), 30, 20, -2, -1)
+    vR, vS = rand_vals(rng, R.shape[0], -10 ** 6, 10 ** 6), rand_vals(rng, S.shape[0])
+    p, v, fp, sw = oracle.join_aggregate(R, vR, S, vS, agg, exclude_equal=bool(seed == 2))
+    big, vb, small, vs = (R, vR, S, vS) if R.shape[0] >= S.shape[0] else (S, vS, R, vR)
+    assert sw == (R.shape[0] < S.shape[0])
+    want = brute_values(big, vb, small, vs, agg)
+    if seed == 2:
+        want = {k: w for k, w in want.items() if k[0] != k[1]}
+    got = {(int(a), int(b)): int(k) for (a, b), k in zip(p.tolist(), v.tolist())}
+    assert got == want
+    assert fp.count == len(want) and fp.csum == sum(want.values()) % (1 << 64)
+
+
+def test_oracle_values_closed_forms():
+    rng = np.random.default_rng(7)
+    R = rand_rel(rng, 3000, 200, 60)
+    S = rand_rel(rng, 2000, 200, 60)
+    p, c, _, _ = oracle.join_aggregate_count(R, S)
+    cnt = dict(zip(map(tuple, p.tolist()), c.tolist()))
+    # constant values: SUM = 5 * count, AVG = MIN = MAX = 5
+    five, zero = np.full(R.shape[0], 5, np.int32), np.zeros(S.shape[0], np.int32)
+    for agg in ["sum", "min", "max", "avg"]:
+        q, v, _, _ = oracle.join_aggregate(R, five, S, zero, agg)
+        got = dict(zip(map(tuple, q.tolist()), v.tolist()))
+        assert got.keys() == cnt.keys()
+        assert all(got[k] == (5 * cnt[k] if agg == "sum" else 5) for k in got)
+    # MIN <= AVG <= MAX, and AVG truncates toward zero (negative sums)
+    vR, vS = rand_vals(rng, R.shape[0], -100, 50), rand_vals(rng, S.shape[0], -100, 50)
+    out = {a: dict(zip(map(tuple, q.tolist()), v.tolist()))
+           for a in ["sum", "min", "max", "avg"] for q, v, _, _ in [oracle.join_aggregate(R, vR, S, vS, a)]}
+    for k, n in cnt.items():
+        s = out["sum"][k]
+        assert out["min"][k] <= out["avg"][k] <= out["max"][k]
+        assert out["avg"][k] == int(s / n) and (s >= 0 or out["avg"][k] * n >= s)
+
+
+@needs_ref
+@pytest.mark.parametrize("agg", ["sum", "min", "max", "avg"])
+def test_oracle_values_match_compiled_reference(agg):
+    rng = np.random.default_rng(300)
+    for trial in range(3):
+        R = rand_rel(rng, int(rng.integers(2000, 20000)), 2000, 800)
+        S = rand_rel(rng, int(rng.integers(2000, 20000)), 2000, 800)
+        vR, vS = rand_vals(rng, R.shape[0]), rand_vals(rng, S.shape[0])
+        p, v, _, _ = oracle.join_aggregate(R, vR, S, vS, agg)
+        rp, rv = oracle.ref_join_aggregate(R, vR, S, vS, agg)
+        a, b = oracle.sort_pairs(p, v), oracle.sort_pairs(rp, rv)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "values_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_oracle_values_match_reference_golden(path):
+    d = np.load(path)
+    for agg in ["sum", "min", "max", "avg"]:
+        p, v, _, _ = oracle.join_aggregate(d["R"], d["vR"], d["S"], d["vS"], agg)
+        gp, gv = oracle.sort_pairs(p, v)
+        assert np.array_equal(gp, d["pairs"]) and np.array_equal(gv, d[agg]), agg
