@@ -170,6 +170,28 @@ int dim3_join_aggregate_count_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_
                                   uint64_t n_s, const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap,
                                   void* stream, dim3_result* res);
 
+/* ---- value aggregates: SELECT x, z, AGG(R.v + S.v) GROUP BY x, z ----
+ * Replaces DIM3JoinAggregate<int,int,int,_Tv,_Agg,...>::doJoinAggregate
+ * (dim3.hpp:1313-1373) with a value policy over the path value
+ * R.v + S.v, the combination of the shipped AggregateFunctionAvg
+ * (dim3.hpp:1453-1480): R_v[i] / S_v[i] is the int32 value of tuple i.
+ * Results are int64 and exact (independent of the path order): SUM, MIN,
+ * MAX of R.v + S.v, and AVG = SUM / path count truncated toward zero (the
+ * reference's get() on integral values).  Rows are dim3_count_row records
+ * whose `count` field holds the aggregate.  The fingerprint treats the
+ * aggregate as the row's count (sum2 += k2(x) v g(z), csum += v).  The swap
+ * rule moves the values with their relation.  No dense part (the
+ * reference's join-aggregate has none): opts->dense_mode is ignored. */
+#define DIM3_AGG_SUM 1
+#define DIM3_AGG_MIN 2
+#define DIM3_AGG_MAX 3
+#define DIM3_AGG_AVG 4
+int dim3_join_aggregate(dim3_ctx* ctx, const int32_t* R_xy, const int32_t* R_v, uint64_t n_r, const int32_t* S_zy,
+                        const int32_t* S_v, uint64_t n_s, int agg, const dim3_opts* opts, dim3_result* res);
+int dim3_join_aggregate_dev(dim3_ctx* ctx, const int32_t* d_R, const int32_t* d_Rv, uint64_t n_r, const int32_t* d_S,
+                            const int32_t* d_Sv, uint64_t n_s, int agg, const dim3_opts* opts, dim3_count_row* d_rows,
+                            uint64_t cap, void* stream, dim3_result* res);
+
 /* ---- streamed output: rows handed to the caller in chunks ----
  * For outputs larger than any one buffer (the chunked STREAM output mode).
  * The query runs in chunks of consecutive x rows, each producing at most

@@ -1,4 +1,4 @@
-"""Python binding of libdim3 (MI355X DIM3 join-project / join-aggregate COUNT).
+"""Python binding of libdim3 (MI355X DIM3 join-project / join-aggregate).
 
 Host-side plumbing over the C-ABI declared in include/dim3.h, used by the
 tests, the benchmark and the server front end.  It mirrors the reference's
@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "libdim3.so")
 OUT_MATERIALIZE, OUT_FINGERPRINT = 0, 1
 OPT_NO_SWAP, OPT_EXCLUDE_EQUAL, OPT_HASH_ONLY = 1, 2, 4
 DENSE_AUTO, DENSE_OFF, DENSE_BITOR, DENSE_MFMA = -1, 0, 1, 2
+AGG = {"sum": 1, "min": 2, "max": 3, "avg": 4}  # DIM3_AGG_*
 
 
 class Opts(ctypes.Structure):
@@ -108,7 +109,7 @@ EXPORTS = [
     "dim3_set_tuning", "dim3_ctx_release", "dim3_ctx_set_cu_share", "dim3_server_opts_default", "dim3_server_create", "dim3_server_destroy",
     "dim3_server_put_relation", "dim3_server_put_relation_dev", "dim3_server_drop_relation", "dim3_submit",
     "dim3_poll", "dim3_wait", "dim3_server_get_stats", "dim3_join_project_stream",
-    "dim3_join_aggregate_count_stream", "dim3_version",
+    "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
 ]
 
 _lib = None
@@ -165,6 +166,11 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(Opts), u64, ROWS_CB, vp, ctypes.POINTER(Result)]
         f.restype = i32
+    L.dim3_join_aggregate.argtypes = [vp, vp, vp, u64, vp, vp, u64, i32, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
+    L.dim3_join_aggregate.restype = i32
+    L.dim3_join_aggregate_dev.argtypes = [vp, vp, vp, u64, vp, vp, u64, i32, ctypes.POINTER(Opts), vp, u64, vp,
+                                          ctypes.POINTER(Result)]
+    L.dim3_join_aggregate_dev.restype = i32
     L.dim3_result_free.argtypes = [ctypes.POINTER(Result)]
     L.dim3_result_free.restype = None
     L.dim3_count_join_paths.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
@@ -240,6 +246,18 @@ def _pairs(a):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _values(v, n):
+    v = np.ascontiguousarray(v, dtype=np.int32).reshape(-1)
+    if v.shape[0] != n:
+        raise ValueError("one value per tuple: %d values for %d tuples" % (v.shape[0], n))
+    return v
+
+
+def _agg(agg):
+    """DIM3_AGG_* of a name ("sum", "min", "max", "avg") or a code."""
+    return AGG[agg] if isinstance(agg, str) else int(agg)
 
 
 class QueryResult:
@@ -348,6 +366,43 @@ class Context:
         out = QueryResult(res, rows=rows)
         self._L.dim3_result_free(ctypes.byref(res))
         return out
+
+    def join_aggregate(self, R, vR, S, vS, agg, **kw):
+        """DIM3JoinAggregate with a value policy: agg ("sum", "min", "max",
+        "avg") of vR + vS over the join paths of each (x, z); vR / vS hold one
+        int32 value per tuple.  Rows have fields x, z, count (the int64
+        aggregate)."""
+        opts = make_opts(**kw)
+        R, S = _pairs(R), _pairs(S)
+        vR = _values(vR, R.shape[0])
+        vS = _values(vS, S.shape[0])
+        res = Result()
+        rc = self._L.dim3_join_aggregate(self._h, _ptr(R), _ptr(vR), R.shape[0], _ptr(S), _ptr(vS), S.shape[0],
+                                         _agg(agg), ctypes.byref(opts), ctypes.byref(res))
+        self._check(rc, "dim3_join_aggregate")
+        rows = None
+        if opts.output_mode == OUT_MATERIALIZE:
+            n = int(res.n)
+            if n:
+                buf = (ctypes.c_char * (16 * n)).from_address(res.rows)
+                rows = np.frombuffer(buf, dtype=COUNT_ROW_DTYPE).copy()
+            else:
+                rows = np.zeros((0,), dtype=COUNT_ROW_DTYPE)
+        out = QueryResult(res, rows=rows)
+        self._L.dim3_result_free(ctypes.byref(res))
+        return out
+
+    def join_aggregate_dev(self, d_R, d_vR, n_r, d_S, d_vS, n_s, agg, d_out, cap, stream=None, **kw):
+        """Device-resident value aggregate (addresses as ints); rows are
+        16-byte (x, z, int64) records.  Returns (status, QueryResult)."""
+        opts = make_opts(**kw)
+        res = Result()
+        vp = ctypes.c_void_p
+        rc = self._L.dim3_join_aggregate_dev(self._h, vp(d_R), vp(d_vR), n_r, vp(d_S), vp(d_vS), n_s, _agg(agg),
+                                             ctypes.byref(opts), vp(d_out), cap, vp(stream or 0), ctypes.byref(res))
+        if rc not in (OK, ERR_CAPACITY):
+            self._check(rc, "dim3_join_aggregate_dev")
+        return rc, QueryResult(res)
 
     def _stream(self, fn, name, R, S, on_rows, chunk_rows, width, dtype, kw):
         opts = make_opts(**kw)

@@ -59,6 +59,10 @@ enum SlotId {
     WS_KEY_S,
     WS_ZID_S,
     WS_COL_S,
+    WS_IN_VR,
+    WS_IN_VS,
+    WS_VAL_R,
+    WS_VAL_S,
     WS_OUT,
     WS_ROW_BOUND,
     WS_XW,
@@ -111,6 +115,9 @@ struct QueryPlan {
     bool empty = true;
     bool count_mode = false;  // expansion with counters (false also for a COUNT fingerprint, see agg_fp)
     bool agg_fp = false;      // COUNT fingerprint: groups from the project expansion, sum2 / csum from path sums
+    int agg = 0;              // value aggregate DIM3_AGG_* (0: none) over the tuple values vR, vS
+    const int32_t* vR = nullptr;
+    const int32_t* vS = nullptr;
     uint32_t flags = 0;
     uint32_t nx = 0, ny = 0, nz = 0;
     uint32_t nnzR = 0, nnzS = 0;
@@ -136,6 +143,13 @@ struct QueryPlan {
     double p1_gain = -1e300;   // saving of the row-and-column-complete choice (phase 1)
     bool core = false;         // rows outside the core keep heavy entries (split S rows)
     uint32_t thx = 1, thz = 1;
+};
+
+// The tuple values of a value-aggregate query (agg 0: none).
+struct ValueSpec {
+    int agg = 0;
+    const int32_t* vR = nullptr;
+    const int32_t* vS = nullptr;
 };
 
 // One query's device workspace and plan.  A context holds one bank per
@@ -289,10 +303,13 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
     auto& P = c->bank().plan;
     const bool cnt = P.count_mode;
     const uint32_t per_word = cnt ? 1u : 32u;  // z IDs per LDS word
+    // a value tile holds an int64 accumulator and a u32 counter per z ID
+    const uint64_t wpu = P.agg ? 3u : 1u;
     P.touched_cap = cnt ? 4096u : 2048u;
     const uint64_t nz_words = ((uint64_t)P.nz + per_word - 1) / per_word;
     // static shared + slack, and the fingerprint table of fingerprint mode
-    const uint64_t fp_words = o.output_mode == DIM3_OUT_FINGERPRINT ? 2u * dim3d::kFpTableU64 : 0u;
+    // (the value kernel computes its z factors instead)
+    const uint64_t fp_words = o.output_mode == DIM3_OUT_FINGERPRINT && !P.agg ? 2u * dim3d::kFpTableU64 : 0u;
     const uint64_t lds_words = c->max_lds / 4u - 256u;
     struct Cand {
         int block;
@@ -304,9 +321,10 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
     auto make = [&](int wgs, int block, uint64_t forced_units) {
         Cand k{block, 1, 0, 0, false, wgs};
         const uint64_t budget = lds_words / (uint64_t)wgs - fp_words;
-        const uint64_t min_stage = std::max<uint64_t>(1024u, 3u * (uint64_t)block + 1u);
-        if (budget <= P.touched_cap + min_stage + 1) return k;
-        uint64_t max_words = budget - P.touched_cap - min_stage;
+        const uint64_t min_stage =
+            P.agg ? 4u * (uint64_t)block + 1u : std::max<uint64_t>(1024u, 3u * (uint64_t)block + 1u);
+        if (budget <= P.touched_cap + min_stage + wpu) return k;
+        uint64_t max_words = (budget - P.touched_cap - min_stage) / wpu;
         uint64_t tw;
         if (forced_units) {
             tw = (forced_units + per_word - 1) / per_word;
@@ -319,7 +337,7 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
         k.tile_words = (uint32_t)tw;
         k.ntiles = (uint32_t)(((uint64_t)P.nz + tw * per_word - 1) / (tw * per_word));
         if (k.ntiles == 0) k.ntiles = 1;
-        k.stage = (uint32_t)(budget - P.touched_cap - tw);
+        k.stage = (uint32_t)(budget - P.touched_cap - tw * wpu);
         k.ok = true;
         return k;
     };
@@ -570,10 +588,14 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
 // previous one copied, so the host synchronises between stages; a batch runs
 // each stage over all its instances before one synchronisation.
 int prepare_start(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
-                  const dim3_opts& o, hipStream_t st) {
+                  const dim3_opts& o, hipStream_t st, const ValueSpec& vals = ValueSpec()) {
     Bank& B = c->bank();
     auto& P = B.plan;
     P = QueryPlan();
+    P.agg = vals.agg;
+    P.vR = vals.vR;
+    P.vS = vals.vS;
+    if (P.agg) count_mode = true;  // per-z counters; never the COUNT fingerprint shortcut
     if (!B.hsc) {
         HIPCHK(c, dim3h::host_malloc((void**)&B.hsc, sizeof(Scalars)));
         HIPCHK(c, dim3h::host_malloc((void**)&B.hhist, 6 * dim3h::kDenseBins * sizeof(unsigned long long)));
@@ -582,13 +604,14 @@ int prepare_start(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, con
     // rows (n, sum) and its sum2 / csum are sums over the join paths, taken
     // from the key degrees.  With the u != w filter the paths are counted
     // per group instead.
-    P.agg_fp = count_mode && o.output_mode == DIM3_OUT_FINGERPRINT && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL);
+    P.agg_fp = count_mode && !P.agg && o.output_mode == DIM3_OUT_FINGERPRINT && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL);
     if (P.agg_fp) count_mode = false;
     P.count_mode = count_mode;
     P.flags = o.flags;
     if (!(o.flags & DIM3_OPT_NO_SWAP) && nR < nS) {
         std::swap(dR, dS);
         std::swap(nR, nS);
+        std::swap(P.vR, P.vS);
         P.swapped = true;
     }
     P.R = dR;
@@ -837,7 +860,12 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     dim3h::finish_rowptr(rowR, P.nx, &sc->tot_x, st);
     if (!P.z_ident) dim3h::compact_ids(P.mz, degZ, exZ, id2z, nullptr, st);
     HIPCHK(c, hipMemcpyAsync(curR, rowR, (size_t)P.nx * 4, hipMemcpyDeviceToDevice, st));
-    dim3h::scatter_r(slotR, slotRx, nR, exY, exX, hid, hx, curR, colR, st);
+    int32_t *valR = nullptr, *valS = nullptr;
+    if (P.agg) {
+        TRY(ws(c, WS_VAL_R, P.nnzR, &valR));
+        TRY(ws(c, WS_VAL_S, P.nnzS, &valS));
+    }
+    dim3h::scatter_r(slotR, slotRx, nR, exY, exX, hid, hx, curR, colR, P.vR, valR, st);
     dim3h::fill_u32(cntS, nkeyS + 1, 0, st);
     uint32_t *cntSD = nullptr, *curSD = nullptr;
     if (P.core) {
@@ -852,7 +880,7 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
     if (P.core) dim3h::split_cursors(rowS, cntSD, (uint32_t)nkeyS, curS, curSD, st);
     else HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
-    dim3h::scatter_s(keyS, zidS, nS, curS, curSD, colS, st);
+    dim3h::scatter_s(keyS, zidS, nS, curS, curSD, colS, P.vS, valS, st);
     if (o.flags & DIM3_OPT_EXCLUDE_EQUAL)
         dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
     LAUNCHCHK(c, "csr");
@@ -896,8 +924,8 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
 
 // The three stages of one query, synchronising after each.
 int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
-            const dim3_opts& o, hipStream_t st, int force_wgs = 0) {
-    TRY(prepare_start(c, count_mode, dR, nR, dS, nS, o, st));
+            const dim3_opts& o, hipStream_t st, int force_wgs = 0, const ValueSpec& vals = ValueSpec()) {
+    TRY(prepare_start(c, count_mode, dR, nR, dS, nS, o, st, vals));
     auto& P = c->bank().plan;
     if (P.empty) return DIM3_OK;
     HIPCHK(c, hipStreamSynchronize(st));
@@ -940,6 +968,10 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
     if (P.core) a.splitS = (const uint32_t*)W[WS_CNT_SD].p;
+    if (P.agg) {
+        a.valR = (const int32_t*)W[WS_VAL_R].p;
+        a.valS = (const int32_t*)W[WS_VAL_S].p;
+    }
     if (P.dense != DIM3_DENSE_OFF) {
         a.hx = (const int32_t*)W[WS_HX].p;
         if (P.count_mode) {
@@ -983,8 +1015,9 @@ int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st) {
     hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
     hipMemsetAsync(&sc->task, 0, sizeof(uint32_t), st);
     HIPCHK(c, hipEventRecord(c->ev[4], st));
-    int e = P.count_mode ? dim3h::expand_count(a, P.block, c->grid_cus(), st)
-                         : dim3h::expand_project(a, P.block, c->grid_cus(), st);
+    int e = P.agg          ? dim3h::expand_value(a, P.agg, P.block, c->grid_cus(), st)
+            : P.count_mode ? dim3h::expand_count(a, P.block, c->grid_cus(), st)
+                           : dim3h::expand_project(a, P.block, c->grid_cus(), st);
     if (e != 0) return fail(c, DIM3_ERR_DEVICE, "expansion launch failed: %s", hipGetErrorString((hipError_t)e));
     HIPCHK(c, hipEventRecord(c->ev[5], st));
     HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -1137,6 +1170,107 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
             res->rows = (dim3_count_row*)h;
         else
             res->pairs = (int32_t*)h;
+    }
+    return DIM3_OK;
+}
+
+// Value aggregates (DIM3_AGG_*): the COUNT pipeline with the tuple values
+// carried into the CSRs and the value expansion; never a dense part.
+int check_value_query(dim3_ctx* c, int agg, uint64_t nR, const int32_t* vR, uint64_t nS, const int32_t* vS) {
+    if (agg < DIM3_AGG_SUM || agg > DIM3_AGG_AVG) return fail(c, DIM3_ERR_INVALID, "bad aggregate %d", agg);
+    if ((nR && !vR) || (nS && !vS)) return fail(c, DIM3_ERR_INVALID, "null value pointer");
+    return DIM3_OK;
+}
+
+int run_value_dev(dim3_ctx* c, int agg, const int32_t* d_R, const int32_t* d_vR, uint64_t nR, const int32_t* d_S,
+                  const int32_t* d_vS, uint64_t nS, const dim3_opts* opts, void* d_out, uint64_t cap, void* stream,
+                  dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if ((nR && !d_R) || (nS && !d_S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    TRY(check_value_query(c, agg, nR, d_vR, nS, d_vS));
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    o.dense_mode = DIM3_DENSE_OFF;
+    const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
+    if (!fp && cap && !d_out) return fail(c, DIM3_ERR_INVALID, "null output buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    c->qstream = st;
+    ValueSpec vals;
+    vals.agg = agg;
+    vals.vR = d_vR;
+    vals.vS = d_vS;
+    TRY(prepare(c, true, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st, 0, vals));
+    TRY(expand(c, fp, d_out, cap, st));
+    fill_result(c, res, *c->h_sc, true);
+    if (!fp && res->n > cap)
+        return fail(c, DIM3_ERR_CAPACITY, "output needs %llu rows, buffer holds %llu", (unsigned long long)res->n,
+                    (unsigned long long)cap);
+    return DIM3_OK;
+}
+
+int run_value_host(dim3_ctx* c, int agg, const int32_t* R, const int32_t* vR, uint64_t nR, const int32_t* S,
+                   const int32_t* vS, uint64_t nS, const dim3_opts* opts, dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if ((nR && !R) || (nS && !S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    TRY(check_value_query(c, agg, nR, vR, nS, vS));
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    o.dense_mode = DIM3_DENSE_OFF;
+    const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    c->qstream = st;
+    int2 *dR, *dS;
+    int32_t *dvR, *dvS;
+    TRY(ws(c, WS_IN_R, std::max<uint64_t>(nR, 1), &dR));
+    TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
+    TRY(ws(c, WS_IN_VR, std::max<uint64_t>(nR, 1), &dvR));
+    TRY(ws(c, WS_IN_VS, std::max<uint64_t>(nS, 1), &dvS));
+    if (nR) HIPCHK(c, hipMemcpyAsync(dR, R, nR * 8, hipMemcpyHostToDevice, st));
+    if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
+    if (nR) HIPCHK(c, hipMemcpyAsync(dvR, vR, nR * 4, hipMemcpyHostToDevice, st));
+    if (nS) HIPCHK(c, hipMemcpyAsync(dvS, vS, nS * 4, hipMemcpyHostToDevice, st));
+    ValueSpec vals;
+    vals.agg = agg;
+    vals.vR = dvR;
+    vals.vS = dvS;
+    TRY(prepare(c, true, dR, nR, dS, nS, o, st, 0, vals));
+    const size_t row_bytes = sizeof(dim3_count_row);
+    const auto& P = c->bank().plan;
+    void* dout = nullptr;
+    uint64_t cap = 0;
+    if (!fp && !P.empty) {
+        // groups <= min(OUT_J, nx * nz): size for that bound when it is
+        // modest, otherwise guess and re-run the expansion at the exact size
+        const uint64_t bound = std::min<uint64_t>(P.out_j, (uint64_t)P.nx * P.nz);
+        const uint64_t room = c->budget > c->held ? (c->budget - c->held) / row_bytes / 2 : 0;
+        cap = std::max<uint64_t>(std::min<uint64_t>(bound, std::max<uint64_t>(room, 1u << 20)), 1);
+        TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
+        cap = c->bank().ws[WS_OUT].bytes / row_bytes;
+    }
+    TRY(expand(c, fp, dout, cap, st));
+    if (!fp && !P.empty && c->h_sc->out_n > cap) {
+        cap = c->h_sc->out_n;
+        TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
+        TRY(expand(c, fp, dout, cap, st));
+    }
+    fill_result(c, res, *c->h_sc, true);
+    if (!fp && res->n) {
+        void* h = malloc(res->n * row_bytes);
+        if (!h) return fail(c, DIM3_ERR_OOM, "host malloc of %llu rows", (unsigned long long)res->n);
+        hipError_t e = hipMemcpy(h, dout, res->n * row_bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            free(h);
+            return fail(c, DIM3_ERR_DEVICE, "D2H copy: %s", hipGetErrorString(e));
+        }
+        res->rows = (dim3_count_row*)h;
     }
     return DIM3_OK;
 }
@@ -1675,6 +1809,17 @@ int dim3_join_aggregate_count_stream(dim3_ctx* c, const int32_t* R_xy, uint64_t 
                                      uint64_t n_s, const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb,
                                      void* user, dim3_result* res) {
     return run_stream(c, true, R_xy, n_r, S_zy, n_s, opts, chunk_rows, cb, user, res);
+}
+
+int dim3_join_aggregate(dim3_ctx* c, const int32_t* R_xy, const int32_t* R_v, uint64_t n_r, const int32_t* S_zy,
+                        const int32_t* S_v, uint64_t n_s, int agg, const dim3_opts* opts, dim3_result* res) {
+    return run_value_host(c, agg, R_xy, R_v, n_r, S_zy, S_v, n_s, opts, res);
+}
+
+int dim3_join_aggregate_dev(dim3_ctx* c, const int32_t* d_R, const int32_t* d_Rv, uint64_t n_r, const int32_t* d_S,
+                            const int32_t* d_Sv, uint64_t n_s, int agg, const dim3_opts* opts, dim3_count_row* d_rows,
+                            uint64_t cap, void* stream, dim3_result* res) {
+    return run_value_dev(c, agg, d_R, d_Rv, n_r, d_S, d_Sv, n_s, opts, d_rows, cap, stream, res);
 }
 
 void dim3_result_free(dim3_result* res) {

@@ -412,9 +412,10 @@ void compact_ids(const KeyMap& m, const uint32_t* deg, const uint64_t* ex, int32
 void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_t st);
 // CSR of R by x: col = y id, with bit 31 set for the heavy keys of product
 // rows (hid[y] >= 0 and hx[x] >= 0; hid nullable)
+// vin / vout (nullable): the tuple values, written beside the columns
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y,
                const uint64_t* ex_x, const int32_t* hid, const int32_t* hx, uint32_t* cursor, int32_t* col,
-               hipStream_t st);
+               const int32_t* vin, int32_t* vout, hipStream_t st);
 // tiled CSR of S by (y, z-tile); ex_z == nullptr means z identity (zid = z - zlo)
 // hid (nullable): heavy index per y ID; a heavy key's tuple with a product
 // column (zflag) is left out, or, with cntD (dense core: rows outside the
@@ -426,8 +427,9 @@ void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uin
                    hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st);
+// vin / vout (nullable): the tuple values, written beside the columns
 void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, uint32_t* cursorD,
-               int32_t* col, hipStream_t st);
+               int32_t* col, const int32_t* vin, int32_t* vout, hipStream_t st);
 void x2z_self(const int32_t* id2x, uint32_t nx, const KeyMap& mz, const uint32_t* degZ, const uint64_t* ex_z,
               int32_t* x2z, hipStream_t st);
 
@@ -468,6 +470,9 @@ struct ExpandArgs {
     // a materialised product row
     const uint32_t* bkeys;    // K x dwords key bitmaps over z IDs (nullptr: seed from dbits / cint)
     const int32_t* hid;       // heavy index of each y ID
+    // value aggregates: the tuple value of each R / S CSR entry
+    const int32_t* valR;
+    const int32_t* valS;
 };
 
 // returns 0 or a hipError_t
@@ -516,6 +521,10 @@ void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t*
 void row_bounds(const ExpandArgs& a, uint32_t nz, uint32_t* bound, hipStream_t st);
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
+// value aggregates (k_value.hip): agg 1 SUM, 2 MIN, 3 MAX, 4 AVG of vR + vS
+int expand_value(const ExpandArgs& a, int agg, int block, int cu_count, hipStream_t st);
+// dynamic LDS of one value-aggregate workgroup (tile_size groups of 12 bytes)
+size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block);
 // dynamic LDS of one expansion workgroup (fp: plus the fingerprint table)
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp);
 

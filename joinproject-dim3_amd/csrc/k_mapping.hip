@@ -378,7 +378,7 @@ __global__ void k_finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* to
 __global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
                             const uint64_t* __restrict__ ex_y, const uint64_t* __restrict__ ex_x,
                             const int32_t* __restrict__ hid, const int32_t* __restrict__ hx, uint32_t* cursor,
-                            int32_t* __restrict__ col) {
+                            int32_t* __restrict__ col, const int32_t* __restrict__ vin, int32_t* __restrict__ vout) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
     const int32_t sx = slotRx[i];
@@ -389,6 +389,7 @@ __global__ void k_scatter_r(const uint32_t* __restrict__ slotR, const int32_t* _
     // a dense-core row's heavy entry: its product-column paths come from the product
     const uint32_t flag = (hid != nullptr && hid[yid] >= 0 && hx[xid] >= 0) ? 0x80000000u : 0u;
     col[pos] = (int32_t)(yid | flag);
+    if (vout != nullptr) vout[pos] = vin[i];
 }
 
 // z ID of a surviving S tuple: the range slot itself (z identity) or the
@@ -440,7 +441,8 @@ __global__ void k_split_cursors(const uint32_t* __restrict__ rowptr, const uint3
 }
 
 __global__ void k_scatter_s(const uint32_t* __restrict__ keyS, const int32_t* __restrict__ zidS, uint64_t nS,
-                            uint32_t* cursor, uint32_t* cursorD, int32_t* __restrict__ col) {
+                            uint32_t* cursor, uint32_t* cursorD, int32_t* __restrict__ col,
+                            const int32_t* __restrict__ vin, int32_t* __restrict__ vout) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const uint32_t raw = keyS[i];
@@ -454,6 +456,7 @@ __global__ void k_scatter_s(const uint32_t* __restrict__ keyS, const int32_t* __
     }
     if (!valid) return;
     col[pos] = zidS[i];
+    if (vout != nullptr) vout[pos] = vin[i];
 }
 
 // z ID of each x value, or -1 (self-pair filter of a self-join)
@@ -589,10 +592,11 @@ void finish_rowptr(uint32_t* rowptr, uint32_t n, const uint64_t* tot, hipStream_
     hipLaunchKernelGGL(k_finish_rowptr, dim3(1), dim3(64), 0, st, rowptr, n, tot);
 }
 void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const uint64_t* ex_y, const uint64_t* ex_x,
-               const int32_t* hid, const int32_t* hx, uint32_t* cursor, int32_t* col, hipStream_t st) {
+               const int32_t* hid, const int32_t* hx, uint32_t* cursor, int32_t* col, const int32_t* vin,
+               int32_t* vout, hipStream_t st) {
     if (nR)
         hipLaunchKernelGGL(k_scatter_r, dim3(grid_for(nR)), dim3(kThreads), 0, st, slotR, slotRx, nR, ex_y, ex_x,
-                           hid, hx, cursor, col);
+                           hid, hx, cursor, col, vin, vout);
 }
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
              const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
@@ -610,10 +614,10 @@ void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* par
     scan_exclusive(ValU32{cnt}, OutU32{out_excl}, n, partials, tot, st);
 }
 void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, uint32_t* cursorD,
-               int32_t* col, hipStream_t st) {
+               int32_t* col, const int32_t* vin, int32_t* vout, hipStream_t st) {
     if (nS)
         hipLaunchKernelGGL(k_scatter_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, keyS, zidS, nS, cursor, cursorD,
-                           col);
+                           col, vin, vout);
 }
 void x2z_self(const int32_t* id2x, uint32_t nx, const KeyMap& mz, const uint32_t* degZ, const uint64_t* ex_z,
               int32_t* x2z, hipStream_t st) {
