@@ -64,6 +64,8 @@ def lib():
         L.oracle_join_aggregate_count.restype = i32
         L.oracle_join_aggregate.argtypes = [vp, vp, u64, vp, vp, u64, i32, i32, i32, vp, vp, vp, vp, vp]
         L.oracle_join_aggregate.restype = i32
+        L.oracle_join_aggregate_f32.argtypes = [vp, vp, u64, vp, vp, u64, i32, i32, i32, vp, vp, vp, vp, vp]
+        L.oracle_join_aggregate_f32.restype = i32
         L.oracle_outj.argtypes = [vp, u64, vp, u64]
         L.oracle_outj.restype = u64
         L.oracle_estimate.argtypes = [vp, u64, vp, u64]
@@ -100,6 +102,8 @@ def ref():
         L.ref_join_aggregate_count.restype = ctypes.c_int
         L.ref_join_aggregate.argtypes = [vp, vp, u32, vp, vp, u32, ctypes.c_int, vp, vp, vp]
         L.ref_join_aggregate.restype = ctypes.c_int
+        L.ref_join_aggregate_f32.argtypes = [vp, vp, u32, vp, vp, u32, ctypes.c_int, vp, vp, vp]
+        L.ref_join_aggregate_f32.restype = ctypes.c_int
         L.ref_estimate.argtypes = [vp, u32, vp, u32]
         L.ref_estimate.restype = ctypes.c_longlong
         L.ref_gen_rand_data.argtypes = [vp, u32, vp, u32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -212,6 +216,38 @@ def join_aggregate(R, vR, S, vS, agg, exclude_equal=False, fingerprint_only=Fals
             fp, bool(sw.value))
 
 
+def _fvals(v, n):
+    v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+    if v.shape[0] != n:
+        raise ValueError("one value per tuple")
+    return v
+
+
+def join_aggregate_f32(R, vR, S, vS, agg, exclude_equal=False, fingerprint_only=False):
+    """join_aggregate on float32 values: float path value vR + vS, SUM / AVG
+    accumulated in double and rounded once, MIN / MAX by totalOrder.
+    Returns (pairs[n,2] int32, values[n] float32, Fingerprint, swapped); the
+    fingerprint counts the result's bit pattern."""
+    L = lib()
+    R, S = _pairs(R), _pairs(S)
+    vR, vS = _fvals(vR, R.shape[0]), _fvals(vS, S.shape[0])
+    out = ctypes.c_void_p()
+    val = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    fp = Fingerprint()
+    sw = ctypes.c_int()
+    rc = L.oracle_join_aggregate_f32(_p(R), _p(vR), R.shape[0], _p(S), _p(vS), S.shape[0], AGG[agg],
+                                     int(exclude_equal), int(fingerprint_only), ctypes.byref(out),
+                                     ctypes.byref(val), ctypes.byref(n), ctypes.byref(fp), ctypes.byref(sw))
+    if rc != 0:
+        raise MemoryError("oracle_join_aggregate_f32 failed rc=%d" % rc)
+    if fingerprint_only:
+        return None, None, fp, bool(sw.value)
+    bits = _take_counts(val, n.value, L.oracle_free)
+    return (_take_pairs(L, out, n.value, L.oracle_free), bits.astype(np.uint32).view(np.float32), fp,
+            bool(sw.value))
+
+
 def outj(R, S):
     L = lib()
     R, S = _pairs(R), _pairs(S)
@@ -289,6 +325,28 @@ def ref_join_aggregate(R, vR, S, vS, agg):
     if rc != 0:
         raise ValueError("ref_join_aggregate rc=%d" % rc)
     return _take_pairs(L, out, n.value, L.ref_free), _take_counts(val, n.value, L.ref_free)
+
+
+def ref_join_aggregate_f32(R, vR, S, vS, agg):
+    L = ref()
+    R, S = _pairs(R), _pairs(S)
+    vR, vS = _fvals(vR, R.shape[0]), _fvals(vS, S.shape[0])
+    out = ctypes.c_void_p()
+    val = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    rc = L.ref_join_aggregate_f32(_p(R), _p(vR), R.shape[0], _p(S), _p(vS), S.shape[0], AGG[agg],
+                                  ctypes.byref(out), ctypes.byref(val), ctypes.byref(n))
+    if rc != 0:
+        raise ValueError("ref_join_aggregate_f32 rc=%d" % rc)
+    nn = n.value
+    if nn == 0 or not val.value:
+        if val.value:
+            L.ref_free(val)
+        vals = np.zeros(0, np.float32)
+    else:
+        vals = np.frombuffer((ctypes.c_float * nn).from_address(val.value), dtype=np.float32).copy()
+        L.ref_free(val)
+    return _take_pairs(L, out, nn, L.ref_free), vals
 
 
 def ref_estimate(R, S):

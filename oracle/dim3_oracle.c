@@ -24,6 +24,8 @@
  *   oracle_join_aggregate   SpmmAggregate with a SUM / MIN / MAX / AVG policy
  *                           over the path value vR + vS   dim3.hpp:1267-1311,
  *                           (the value AggregateFunctionAvg combines) 1453-1480
+ *   oracle_join_aggregate_f32  the same on float values (float path value,
+ *                           double accumulation, see run_value_f32)
  *   oracle_estimate         EstimateJoinCardinality       dim3.hpp:193-224, 262-269
  *   oracle_outj             exact sum_b degR(b)*degS(b)   (closed form of the above)
  *   oracle_gen_rand_data    gen_rand_data (minstd_rand0)  dim3.hpp:1556-1568
@@ -460,11 +462,91 @@ static int run_value(const orc_plan* p, int agg, int exclude_equal, orc_sink* k)
     return k->oom ? ORC_OOM : ORC_OK;
 }
 
+/* Float values (the reference's TUPLE<int,int,float> workload,
+ * join_aggregate.cpp:21-23): the path value is the float sum vR + vS (one
+ * float add, as the policy's val1 + val2 on float); SUM and AVG accumulate
+ * the path values in double in first-touch order (exact, hence independent
+ * of the order, whenever every partial sum is a double, e.g. for
+ * integer-valued data below 2^53) and round once: SUM = (float)sum,
+ * AVG = (float)(sum / paths).  MIN / MAX order the path values by IEEE
+ * totalOrder (-0 < +0; NaN inputs unsupported).  The result float's bit
+ * pattern is the row value (and the fingerprint's count). */
+static inline float orc_as_float(int32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static inline uint32_t orc_float_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+static inline int32_t orc_fkey(float f) {  /* order-preserving int of a float */
+    int32_t b = (int32_t)orc_float_bits(f);
+    return b >= 0 ? b : b ^ 0x7FFFFFFF;
+}
+
+static int run_value_f32(const orc_plan* p, int agg, int exclude_equal, orc_sink* k) {
+    if (p->nx == 0) return ORC_OK;
+    int32_t* spaw = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    int64_t* cnt = (int64_t*)malloc((size_t)p->nz * sizeof(int64_t));
+    double* acc = (double*)malloc((size_t)p->nz * sizeof(double));
+    int32_t* key = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    int32_t* list = (int32_t*)malloc((size_t)p->nz * sizeof(int32_t));
+    if (!spaw || !cnt || !acc || !key || !list) {
+        free(spaw); free(cnt); free(acc); free(key); free(list);
+        return ORC_OOM;
+    }
+    for (int32_t i = 0; i < p->nz; i++) spaw[i] = -1;
+    for (int32_t x = 0; x < p->nx; x++) {
+        int32_t xv = p->id2x[x];
+        int32_t nl = 0;
+        for (uint64_t i = p->rjr[x]; i < p->rjr[x + 1]; i++) {
+            int32_t y = p->ric[i];
+            for (uint64_t j = p->sjr[y]; j < p->sjr[y + 1]; j++) {
+                int32_t z = p->sic[j];
+                const float v = orc_as_float(p->rval[i]) + orc_as_float(p->sval[j]);
+                const int32_t kv = orc_fkey(v);
+                if (spaw[z] != x) { spaw[z] = x; cnt[z] = 1; acc[z] = v; key[z] = kv; list[nl++] = z; continue; }
+                cnt[z] += 1;
+                acc[z] += v;
+                if (agg == 2 ? kv < key[z] : kv > key[z]) key[z] = kv;
+            }
+        }
+        for (int32_t i = 0; i < nl; i++) {
+            const int32_t z = list[i];
+            int32_t zv = p->id2z[z];
+            if (exclude_equal && zv == xv) continue;
+            uint32_t bits;
+            if (agg == 1) bits = orc_float_bits((float)acc[z]);
+            else if (agg == 4) bits = orc_float_bits((float)(acc[z] / (double)cnt[z]));
+            else bits = (uint32_t)(key[z] >= 0 ? key[z] : key[z] ^ 0x7FFFFFFF);
+            sink_push(k, xv, zv, (int64_t)bits);
+        }
+    }
+    free(spaw); free(cnt); free(acc); free(key); free(list);
+    return k->oom ? ORC_OOM : ORC_OK;
+}
+
 /* The value aggregate (agg 1-4, see run_value) of R(x, y, vR) join S(z, y, vS);
  * the swap rule moves the values with their relation. */
+static int join_aggregate_values(const int32_t* R_xy, const int32_t* vR, uint64_t nR, const int32_t* S_zy,
+                                 const int32_t* vS, uint64_t nS, int agg, int f32, int exclude_equal,
+                                 int fingerprint_only, int32_t** out_pairs, int64_t** out_vals, uint64_t* n_out,
+                                 orc_fp* fp, int* swapped);
+
 int oracle_join_aggregate(const int32_t* R_xy, const int32_t* vR, uint64_t nR, const int32_t* S_zy,
                           const int32_t* vS, uint64_t nS, int agg, int exclude_equal, int fingerprint_only,
                           int32_t** out_pairs, int64_t** out_vals, uint64_t* n_out, orc_fp* fp, int* swapped) {
+    return join_aggregate_values(R_xy, vR, nR, S_zy, vS, nS, agg, 0, exclude_equal, fingerprint_only, out_pairs,
+                                 out_vals, n_out, fp, swapped);
+}
+
+/* As oracle_join_aggregate on float values (bit patterns in vR / vS); each
+ * row value is the result float's bit pattern. */
+int oracle_join_aggregate_f32(const int32_t* R_xy, const int32_t* vR, uint64_t nR, const int32_t* S_zy,
+                              const int32_t* vS, uint64_t nS, int agg, int exclude_equal, int fingerprint_only,
+                              int32_t** out_pairs, int64_t** out_vals, uint64_t* n_out, orc_fp* fp, int* swapped) {
+    return join_aggregate_values(R_xy, vR, nR, S_zy, vS, nS, agg, 1, exclude_equal, fingerprint_only, out_pairs,
+                                 out_vals, n_out, fp, swapped);
+}
+
+static int join_aggregate_values(const int32_t* R_xy, const int32_t* vR, uint64_t nR, const int32_t* S_zy,
+                                 const int32_t* vS, uint64_t nS, int agg, int f32, int exclude_equal,
+                                 int fingerprint_only, int32_t** out_pairs, int64_t** out_vals, uint64_t* n_out,
+                                 orc_fp* fp, int* swapped) {
     if (agg < 1 || agg > 4) return ORC_INVALID;
     const int32_t* R = R_xy; const int32_t* S = S_zy;
     int sw = 0;
@@ -481,7 +563,7 @@ int oracle_join_aggregate(const int32_t* R_xy, const int32_t* vR, uint64_t nR, c
     memset(&k, 0, sizeof(k));
     k.fingerprint_only = fingerprint_only;
     k.with_counts = 1;
-    rc = p.nx ? run_value(&p, agg, exclude_equal, &k) : ORC_OK;
+    rc = !p.nx ? ORC_OK : f32 ? run_value_f32(&p, agg, exclude_equal, &k) : run_value(&p, agg, exclude_equal, &k);
     plan_free(&p);
     if (fp) *fp = k.fp;
     if (n_out) *n_out = k.fp.count;

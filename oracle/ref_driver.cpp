@@ -38,19 +38,19 @@ struct AggCount {
 
 // Value policies over the path value v1 + v2 (the combination the shipped
 // AggregateFunctionAvg uses, dim3.hpp:1466-1473): SUM, MIN, MAX.
-template <int OP>
+template <int OP, typename T = long long>
 struct AggValue {
-    long long* a;
-    explicit AggValue(int n) { a = (long long*)malloc((size_t)(n > 0 ? n : 1) * sizeof(long long)); }
+    T* a;
+    explicit AggValue(int n) { a = (T*)malloc((size_t)(n > 0 ? n : 1) * sizeof(T)); }
     ~AggValue() { free(a); }
-    inline void insert(const int key, const long long v1, const long long v2) { a[key] = v1 + v2; }
-    inline void update(const int key, const long long v1, const long long v2) {
-        const long long v = v1 + v2;
+    inline void insert(const int key, const T v1, const T v2) { a[key] = v1 + v2; }
+    inline void update(const int key, const T v1, const T v2) {
+        const T v = v1 + v2;
         if (OP == 0) a[key] += v;
         else if (OP == 1) a[key] = v < a[key] ? v : a[key];
         else a[key] = v > a[key] ? v : a[key];
     }
-    inline long long get(const int key) const { return a[key]; }
+    inline T get(const int key) const { return a[key]; }
 };
 
 typedef DIM3<int, int, int, myHasher, myHasher, myHasher, myHasher, uint32_t> RefDIM3;
@@ -81,10 +81,10 @@ static bool guard_empty(const int32_t* R, uint32_t nR, const int32_t* S, uint32_
     return false;
 }
 
-template <typename Pol>
-static void run_agg(myvector<TUPLE<int, int, long long>>& vr, myvector<TUPLE<int, int, long long>>& vs,
-                    myvector<TUPLE<int, int, long long>>& result) {
-    DIM3JoinAggregate<int, int, int, long long, Pol, myHasher, myHasher, myHasher, myHasher, uint32_t>::doJoinAggregate(
+template <typename Pol, typename T = long long>
+static void run_agg(myvector<TUPLE<int, int, T>>& vr, myvector<TUPLE<int, int, T>>& vs,
+                    myvector<TUPLE<int, int, T>>& result) {
+    DIM3JoinAggregate<int, int, int, T, Pol, myHasher, myHasher, myHasher, myHasher, uint32_t>::doJoinAggregate(
         vr, vs, result);
 }
 
@@ -161,6 +161,40 @@ int ref_join_aggregate(const int32_t* R, const int32_t* vR, uint32_t nR, const i
     uint32_t n = result.cur_num;
     int32_t* p = (int32_t*)malloc((size_t)(n ? n : 1) * 2 * sizeof(int32_t));
     int64_t* c = (int64_t*)malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+    for (uint32_t i = 0; i < n; i++) {
+        p[2 * i] = result.data[i].x;
+        p[2 * i + 1] = result.data[i].y;
+        c[i] = result.data[i].v;
+    }
+    free(result.data);
+    free(vr.data);
+    free(vs.data);
+    *out_pairs = p;
+    *out_vals = c;
+    *nout = n;
+    return 0;
+}
+
+// Float values: the reference's own AggregateFunctionAvg<float> (the
+// join_aggregate.cpp workload) for AVG, float SUM / MIN / MAX policies else;
+// float accumulation in first-touch order.
+int ref_join_aggregate_f32(const int32_t* R, const float* vR, uint32_t nR, const int32_t* S, const float* vS,
+                           uint32_t nS, int agg, int32_t** out_pairs, float** out_vals, uint64_t* nout) {
+    *out_pairs = NULL;
+    *out_vals = NULL;
+    *nout = 0;
+    if (agg < 1 || agg > 4) return 1;
+    if (guard_empty(R, nR, S, nS)) return 0;
+    myvector<TUPLE<int, int, float>> vr(nR), vs(nS), result;
+    for (uint32_t i = 0; i < nR; i++) vr.push_back(TUPLE<int, int, float>(R[2 * i], R[2 * i + 1], vR[i]));
+    for (uint32_t i = 0; i < nS; i++) vs.push_back(TUPLE<int, int, float>(S[2 * i], S[2 * i + 1], vS[i]));
+    if (agg == 1) run_agg<AggValue<0, float>, float>(vr, vs, result);
+    else if (agg == 2) run_agg<AggValue<1, float>, float>(vr, vs, result);
+    else if (agg == 3) run_agg<AggValue<2, float>, float>(vr, vs, result);
+    else run_agg<AggregateFunctionAvg<float>, float>(vr, vs, result);
+    uint32_t n = result.cur_num;
+    int32_t* p = (int32_t*)malloc((size_t)(n ? n : 1) * 2 * sizeof(int32_t));
+    float* c = (float*)malloc((size_t)(n ? n : 1) * sizeof(float));
     for (uint32_t i = 0; i < n; i++) {
         p[2 * i] = result.data[i].x;
         p[2 * i + 1] = result.data[i].y;

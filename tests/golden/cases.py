@@ -73,6 +73,22 @@ def value_cases():
     return out
 
 
+def value_cases_f32():
+    """Float-valued inputs whose float sums are exact (small multiples of
+    1/4), so the reference's first-touch float accumulation and a double
+    accumulation agree bit for bit."""
+    cs = small_cases()
+    rng = np.random.default_rng(20240603)
+    out = {}
+    for name, nr in [("zipf", 800), ("dups_neg", 3000)]:
+        R, S = cs[name]
+        R = R[:nr]
+        vR = (rng.integers(-4000, 4000, R.shape[0]) / 4).astype(np.float32)
+        vS = (rng.integers(-4000, 4000, S.shape[0]) / 4).astype(np.float32)
+        out[name] = (R, vR, S, vS)
+    return out
+
+
 # medium cases: count + fingerprint of the reference output
 MEDIUM = {
     "kat": {"gen": "gen_rand_data", "nR": 10_000_000, "nS": 10_000_000, "mod": 1_000_000, "seed": 492,

@@ -7,6 +7,8 @@ small_*.npz   inputs R, S and the reference's sorted join-project pairs and
 values_*.npz  inputs R, vR, S, vS and the reference's sorted rows of the
               SUM / MIN / MAX / AVG value aggregates (DIM3JoinAggregate with
               value policies and its AggregateFunctionAvg<long long>).
+values_f32_*.npz  the same on float values with exact float sums (its
+              AggregateFunctionAvg<float> and float SUM / MIN / MAX).
 medium.json   inputs regenerated from a named seeded generator (with a
               sha256 of the inputs to detect generator drift), and the count
               and fingerprint of the reference's output.
@@ -57,6 +59,22 @@ def values():
             rec[agg] = v
         np.savez_compressed(os.path.join(HERE, "values_%s.npz" % name), R=R, vR=vR, S=S, vS=vS, pairs=pairs, **rec)
         print(name, R.shape[0], S.shape[0], "->", pairs.shape[0], "groups", flush=True)
+
+
+def values_f32():
+    for name, (R, vR, S, vS) in cases.value_cases_f32().items():
+        rec = {}
+        pairs = None
+        for agg in ["sum", "min", "max", "avg"]:
+            p, v = oracle.ref_join_aggregate_f32(R, vR, S, vS, agg)
+            bits = v.view(np.int32).astype(np.int64)
+            p, bits = oracle.sort_pairs(p, bits)
+            assert pairs is None or np.array_equal(pairs, p)
+            pairs = p
+            rec[agg] = bits.astype(np.int32).view(np.float32)
+        np.savez_compressed(os.path.join(HERE, "values_f32_%s.npz" % name), R=R, vR=vR, S=S, vS=vS, pairs=pairs,
+                            **rec)
+        print("f32", name, R.shape[0], S.shape[0], "->", pairs.shape[0], "groups", flush=True)
 
 
 def medium():
@@ -118,6 +136,7 @@ if __name__ == "__main__":
         small()
     if "values" in which:
         values()
+        values_f32()
     if "medium" in which:
         medium()
     if "large" in which:

@@ -41,7 +41,8 @@ def check(ctx, R, vR, S, vS, agg, **kw):
     return g
 
 
-@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "values_*.npz"))),
+@pytest.mark.parametrize("path", sorted(set(glob.glob(os.path.join(GOLDEN, "values_*.npz")))
+                                       - set(glob.glob(os.path.join(GOLDEN, "values_f32_*.npz")))),
                          ids=lambda p: os.path.basename(p))
 def test_values_match_reference_golden(ctx, path):
     d = np.load(path)

@@ -323,7 +323,8 @@ def test_oracle_values_match_compiled_reference(agg):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "values_*.npz"))),
+@pytest.mark.parametrize("path", sorted(set(glob.glob(os.path.join(GOLDEN, "values_*.npz")))
+                                       - set(glob.glob(os.path.join(GOLDEN, "values_f32_*.npz")))),
                          ids=lambda p: os.path.basename(p))
 def test_oracle_values_match_reference_golden(path):
     d = np.load(path)
@@ -331,3 +332,64 @@ def test_oracle_values_match_reference_golden(path):
         p, v, _, _ = oracle.join_aggregate(d["R"], d["vR"], d["S"], d["vS"], agg)
         gp, gv = oracle.sort_pairs(p, v)
         assert np.array_equal(gp, d["pairs"]) and np.array_equal(gv, d[agg]), agg
+
+
+def brute_values_f32(R, vR, S, vS, agg):
+    byy = defaultdict(list)
+    for (z, y), v in zip(S.tolist(), vS.tolist()):
+        byy[y].append((z, np.float32(v)))
+    acc = defaultdict(list)
+    for (x, y), v in zip(R.tolist(), vR.tolist()):
+        for z, w in byy.get(y, ()):
+            acc[(x, z)].append(np.float32(v) + w)  # float add of the path value
+    out = {}
+    for k, l in acc.items():
+        s = sum(float(t) for t in l)  # exact here: the values are multiples of 1/8 below 2^20
+        out[k] = {"sum": np.float32(s), "avg": np.float32(s / len(l)), "min": min(l), "max": max(l)}[agg]
+    return out
+
+
+@pytest.mark.parametrize("agg", ["sum", "min", "max", "avg"])
+def test_oracle_values_f32_match_brute_force(agg):
+    rng = np.random.default_rng(400)
+    R = rand_rel(rng, 300, 30, 20, -2, -1)
+    S = rand_rel(rng, 250, 30, 20, -2, -1)
+    vR = (rng.integers(-8000, 8000, R.shape[0]) / 8).astype(np.float32)
+    vS = (rng.integers(-8000, 8000, S.shape[0]) / 8).astype(np.float32)
+    p, v, fp, _ = oracle.join_aggregate_f32(R, vR, S, vS, agg)
+    want = brute_values_f32(R, vR, S, vS, agg)
+    got = {(int(a), int(b)): np.float32(k) for (a, b), k in zip(p.tolist(), v.tolist())}
+    assert got.keys() == want.keys()
+    assert all(got[k].tobytes() == want[k].tobytes() for k in got)
+    # the fingerprint counts the result's bit pattern
+    assert fp.csum == int(v.view(np.uint32).astype(np.uint64).sum()) % (1 << 64)
+
+
+@needs_ref
+def test_oracle_values_f32_against_reference_workload():
+    """The join_aggregate.cpp workload shape (values = tuple index) with
+    values above 2^24, where the reference's float accumulation rounds: equal
+    groups and AVG within its rounding (paths * 2^-23 relative)."""
+    R, S = oracle.gen_rand_data(200_000, 200_000, 20_000, 20_000, 20_000, 492)
+    vR = (np.arange(R.shape[0]) + (1 << 24)).astype(np.float32)
+    vS = (np.arange(S.shape[0]) + (1 << 24)).astype(np.float32)
+    p, v, _, _ = oracle.join_aggregate_f32(R, vR, S, vS, "avg")
+    _, c, _, _ = oracle.join_aggregate_count(R, S)
+    rp, rv = oracle.ref_join_aggregate_f32(R, vR, S, vS, "avg")
+    o = np.lexsort((p[:, 1], p[:, 0]))
+    ro = np.lexsort((rp[:, 1], rp[:, 0]))
+    assert np.array_equal(p[o], rp[ro])
+    a, b = v[o].astype(np.float64), rv[ro].astype(np.float64)
+    cnt = oracle.sort_pairs(*oracle.join_aggregate_count(R, S)[:2])[1].astype(np.float64)
+    assert np.all(np.abs(a - b) <= cnt * 2.0 ** -23 * np.abs(b))
+    assert (a != b).any()  # the reference does round here
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "values_f32_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_oracle_values_f32_match_reference_golden(path):
+    d = np.load(path)
+    for agg in ["sum", "min", "max", "avg"]:
+        p, v, _, _ = oracle.join_aggregate_f32(d["R"], d["vR"], d["S"], d["vS"], agg)
+        gp, gb = oracle.sort_pairs(p, v.view(np.int32).astype(np.int64))
+        assert np.array_equal(gp, d["pairs"]) and gb.astype(np.int32).tobytes() == d[agg].tobytes(), agg
