@@ -1,5 +1,6 @@
 /*
- * libdim3 — MI355X (gfx950) DIM3 join-project and join-aggregate (COUNT).
+ * libdim3 — MI355X (gfx950) DIM3 join-project and join-aggregate (COUNT, and
+ * SUM / MIN / MAX / AVG of tuple values).
  *
  * C-ABI drop-in for the reference's query entry points.  Plain pointers and
  * sizes; relations are arrays of interleaved int32 pairs laid out exactly
@@ -113,6 +114,14 @@ typedef struct dim3_count_row {
     int64_t count;
 } dim3_count_row;
 
+/* Row record of a float value aggregate; same layout as the reference's
+ * TUPLE<int,int,float> {x, y, v} (12 bytes). */
+typedef struct dim3_value_row_f32 {
+    int32_t x;
+    int32_t z;
+    float v;
+} dim3_value_row_f32;
+
 typedef struct dim3_result {
     uint64_t n;                 /* rows */
     int32_t* pairs;             /* project: 2*n int32 (x,z) pairs, malloc'd by the library */
@@ -121,6 +130,7 @@ typedef struct dim3_result {
     uint32_t swapped;           /* 1: rows are (S.first, R.first) */
     uint32_t reserved;
     dim3_stats stats;
+    dim3_value_row_f32* rows_f32; /* float value aggregate: n rows, malloc'd by the library */
 } dim3_result;
 
 typedef struct dim3_ctx dim3_ctx;
@@ -191,6 +201,22 @@ int dim3_join_aggregate(dim3_ctx* ctx, const int32_t* R_xy, const int32_t* R_v, 
 int dim3_join_aggregate_dev(dim3_ctx* ctx, const int32_t* d_R, const int32_t* d_Rv, uint64_t n_r, const int32_t* d_S,
                             const int32_t* d_Sv, uint64_t n_s, int agg, const dim3_opts* opts, dim3_count_row* d_rows,
                             uint64_t cap, void* stream, dim3_result* res);
+/* Float values (the reference's TUPLE<int,int,float> workload,
+ * join_aggregate.cpp:21-23 with AggregateFunctionAvg<float>): the path
+ * value is the float sum R.v + S.v; SUM and AVG accumulate the path values
+ * in double and round once (SUM = (float)sum, AVG = (float)(sum / paths));
+ * MIN / MAX order by IEEE totalOrder (-0 < +0; NaN values unsupported).
+ * The double sums are exact, so the result does not depend on the path
+ * order, whenever every partial sum is a double (integer-valued data below
+ * 2^53, as the reference's generator makes); the reference instead adds in
+ * float in first-touch order, so its AVG can differ by its own rounding
+ * (relative paths * 2^-24).  Rows: res->rows_f32 / d_rows.  The fingerprint
+ * counts each result float's bit pattern. */
+int dim3_join_aggregate_f32(dim3_ctx* ctx, const int32_t* R_xy, const float* R_v, uint64_t n_r, const int32_t* S_zy,
+                            const float* S_v, uint64_t n_s, int agg, const dim3_opts* opts, dim3_result* res);
+int dim3_join_aggregate_f32_dev(dim3_ctx* ctx, const int32_t* d_R, const float* d_Rv, uint64_t n_r,
+                                const int32_t* d_S, const float* d_Sv, uint64_t n_s, int agg, const dim3_opts* opts,
+                                dim3_value_row_f32* d_rows, uint64_t cap, void* stream, dim3_result* res);
 
 /* ---- streamed output: rows handed to the caller in chunks ----
  * For outputs larger than any one buffer (the chunked STREAM output mode).

@@ -60,7 +60,7 @@ class CountRow(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint64), ("pairs", ctypes.c_void_p), ("rows", ctypes.c_void_p),
                 ("fp", Fingerprint), ("swapped", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
-                ("stats", Stats)]
+                ("stats", Stats), ("rows_f32", ctypes.c_void_p)]
 
 
 class Instance(ctypes.Structure):
@@ -99,6 +99,7 @@ class ServerStats(ctypes.Structure):
 
 
 COUNT_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("count", "<i8")])
+F32_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("v", "<f4")])  # dim3_value_row_f32
 
 # every symbol include/dim3.h declares (checked by the ABI test)
 EXPORTS = [
@@ -110,6 +111,7 @@ EXPORTS = [
     "dim3_server_put_relation", "dim3_server_put_relation_dev", "dim3_server_drop_relation", "dim3_submit",
     "dim3_poll", "dim3_wait", "dim3_server_get_stats", "dim3_join_project_stream",
     "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
+    "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev",
 ]
 
 _lib = None
@@ -171,6 +173,10 @@ def lib():
     L.dim3_join_aggregate_dev.argtypes = [vp, vp, vp, u64, vp, vp, u64, i32, ctypes.POINTER(Opts), vp, u64, vp,
                                           ctypes.POINTER(Result)]
     L.dim3_join_aggregate_dev.restype = i32
+    L.dim3_join_aggregate_f32.argtypes = L.dim3_join_aggregate.argtypes
+    L.dim3_join_aggregate_f32.restype = i32
+    L.dim3_join_aggregate_f32_dev.argtypes = L.dim3_join_aggregate_dev.argtypes
+    L.dim3_join_aggregate_f32_dev.restype = i32
     L.dim3_result_free.argtypes = [ctypes.POINTER(Result)]
     L.dim3_result_free.restype = None
     L.dim3_count_join_paths.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
@@ -255,6 +261,13 @@ def _values(v, n):
     return v
 
 
+def _fvalues(v, n):
+    v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+    if v.shape[0] != n:
+        raise ValueError("one value per tuple: %d values for %d tuples" % (v.shape[0], n))
+    return v
+
+
 def _agg(agg):
     """DIM3_AGG_* of a name ("sum", "min", "max", "avg") or a code."""
     return AGG[agg] if isinstance(agg, str) else int(agg)
@@ -273,7 +286,8 @@ class QueryResult:
 
     @property
     def counts(self):
-        return None if self.rows is None else self.rows["count"]
+        """The row values: counts, int64 aggregates or float32 aggregates."""
+        return None if self.rows is None else self.rows[self.rows.dtype.names[2]]
 
     @property
     def xz(self):
@@ -391,6 +405,43 @@ class Context:
         out = QueryResult(res, rows=rows)
         self._L.dim3_result_free(ctypes.byref(res))
         return out
+
+    def join_aggregate_f32(self, R, vR, S, vS, agg, **kw):
+        """Value aggregate on float32 values (the reference's
+        AggregateFunctionAvg<float> workload): rows with fields x, z, v
+        (float32)."""
+        opts = make_opts(**kw)
+        R, S = _pairs(R), _pairs(S)
+        vR = _fvalues(vR, R.shape[0])
+        vS = _fvalues(vS, S.shape[0])
+        res = Result()
+        rc = self._L.dim3_join_aggregate_f32(self._h, _ptr(R), _ptr(vR), R.shape[0], _ptr(S), _ptr(vS), S.shape[0],
+                                             _agg(agg), ctypes.byref(opts), ctypes.byref(res))
+        self._check(rc, "dim3_join_aggregate_f32")
+        rows = None
+        if opts.output_mode == OUT_MATERIALIZE:
+            n = int(res.n)
+            if n:
+                buf = (ctypes.c_char * (12 * n)).from_address(res.rows_f32)
+                rows = np.frombuffer(buf, dtype=F32_ROW_DTYPE).copy()
+            else:
+                rows = np.zeros((0,), dtype=F32_ROW_DTYPE)
+        out = QueryResult(res, rows=rows)
+        self._L.dim3_result_free(ctypes.byref(res))
+        return out
+
+    def join_aggregate_f32_dev(self, d_R, d_vR, n_r, d_S, d_vS, n_s, agg, d_out, cap, stream=None, **kw):
+        """Device-resident float value aggregate; rows are 12-byte
+        (x, z, float) records."""
+        opts = make_opts(**kw)
+        res = Result()
+        vp = ctypes.c_void_p
+        rc = self._L.dim3_join_aggregate_f32_dev(self._h, vp(d_R), vp(d_vR), n_r, vp(d_S), vp(d_vS), n_s, _agg(agg),
+                                                 ctypes.byref(opts), vp(d_out), cap, vp(stream or 0),
+                                                 ctypes.byref(res))
+        if rc not in (OK, ERR_CAPACITY):
+            self._check(rc, "dim3_join_aggregate_f32_dev")
+        return rc, QueryResult(res)
 
     def join_aggregate_dev(self, d_R, d_vR, n_r, d_S, d_vS, n_s, agg, d_out, cap, stream=None, **kw):
         """Device-resident value aggregate (addresses as ints); rows are

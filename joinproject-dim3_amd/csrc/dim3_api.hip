@@ -116,6 +116,7 @@ struct QueryPlan {
     bool count_mode = false;  // expansion with counters (false also for a COUNT fingerprint, see agg_fp)
     bool agg_fp = false;      // COUNT fingerprint: groups from the project expansion, sum2 / csum from path sums
     int agg = 0;              // value aggregate DIM3_AGG_* (0: none) over the tuple values vR, vS
+    bool vf32 = false;        // float values
     const int32_t* vR = nullptr;
     const int32_t* vS = nullptr;
     uint32_t flags = 0;
@@ -148,6 +149,7 @@ struct QueryPlan {
 // The tuple values of a value-aggregate query (agg 0: none).
 struct ValueSpec {
     int agg = 0;
+    bool f32 = false;  // float values (bit patterns), float results
     const int32_t* vR = nullptr;
     const int32_t* vS = nullptr;
 };
@@ -593,6 +595,7 @@ int prepare_start(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, con
     auto& P = B.plan;
     P = QueryPlan();
     P.agg = vals.agg;
+    P.vf32 = vals.f32;
     P.vR = vals.vR;
     P.vS = vals.vS;
     if (P.agg) count_mode = true;  // per-z counters; never the COUNT fingerprint shortcut
@@ -1015,7 +1018,7 @@ int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st) {
     hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
     hipMemsetAsync(&sc->task, 0, sizeof(uint32_t), st);
     HIPCHK(c, hipEventRecord(c->ev[4], st));
-    int e = P.agg          ? dim3h::expand_value(a, P.agg, P.block, c->grid_cus(), st)
+    int e = P.agg          ? dim3h::expand_value(a, P.agg, P.vf32, P.block, c->grid_cus(), st)
             : P.count_mode ? dim3h::expand_count(a, P.block, c->grid_cus(), st)
                            : dim3h::expand_project(a, P.block, c->grid_cus(), st);
     if (e != 0) return fail(c, DIM3_ERR_DEVICE, "expansion launch failed: %s", hipGetErrorString((hipError_t)e));
@@ -1182,9 +1185,9 @@ int check_value_query(dim3_ctx* c, int agg, uint64_t nR, const int32_t* vR, uint
     return DIM3_OK;
 }
 
-int run_value_dev(dim3_ctx* c, int agg, const int32_t* d_R, const int32_t* d_vR, uint64_t nR, const int32_t* d_S,
-                  const int32_t* d_vS, uint64_t nS, const dim3_opts* opts, void* d_out, uint64_t cap, void* stream,
-                  dim3_result* res) {
+int run_value_dev(dim3_ctx* c, int agg, bool f32, const int32_t* d_R, const int32_t* d_vR, uint64_t nR,
+                  const int32_t* d_S, const int32_t* d_vS, uint64_t nS, const dim3_opts* opts, void* d_out,
+                  uint64_t cap, void* stream, dim3_result* res) {
     if (!c || !res) return DIM3_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     c->err.clear();
@@ -1201,6 +1204,7 @@ int run_value_dev(dim3_ctx* c, int agg, const int32_t* d_R, const int32_t* d_vR,
     c->qstream = st;
     ValueSpec vals;
     vals.agg = agg;
+    vals.f32 = f32;
     vals.vR = d_vR;
     vals.vS = d_vS;
     TRY(prepare(c, true, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st, 0, vals));
@@ -1212,8 +1216,8 @@ int run_value_dev(dim3_ctx* c, int agg, const int32_t* d_R, const int32_t* d_vR,
     return DIM3_OK;
 }
 
-int run_value_host(dim3_ctx* c, int agg, const int32_t* R, const int32_t* vR, uint64_t nR, const int32_t* S,
-                   const int32_t* vS, uint64_t nS, const dim3_opts* opts, dim3_result* res) {
+int run_value_host(dim3_ctx* c, int agg, bool f32, const int32_t* R, const int32_t* vR, uint64_t nR,
+                   const int32_t* S, const int32_t* vS, uint64_t nS, const dim3_opts* opts, dim3_result* res) {
     if (!c || !res) return DIM3_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     c->err.clear();
@@ -1239,10 +1243,11 @@ int run_value_host(dim3_ctx* c, int agg, const int32_t* R, const int32_t* vR, ui
     if (nS) HIPCHK(c, hipMemcpyAsync(dvS, vS, nS * 4, hipMemcpyHostToDevice, st));
     ValueSpec vals;
     vals.agg = agg;
+    vals.f32 = f32;
     vals.vR = dvR;
     vals.vS = dvS;
     TRY(prepare(c, true, dR, nR, dS, nS, o, st, 0, vals));
-    const size_t row_bytes = sizeof(dim3_count_row);
+    const size_t row_bytes = f32 ? sizeof(dim3_value_row_f32) : sizeof(dim3_count_row);
     const auto& P = c->bank().plan;
     void* dout = nullptr;
     uint64_t cap = 0;
@@ -1270,7 +1275,8 @@ int run_value_host(dim3_ctx* c, int agg, const int32_t* R, const int32_t* vR, ui
             free(h);
             return fail(c, DIM3_ERR_DEVICE, "D2H copy: %s", hipGetErrorString(e));
         }
-        res->rows = (dim3_count_row*)h;
+        if (f32) res->rows_f32 = (dim3_value_row_f32*)h;
+        else res->rows = (dim3_count_row*)h;
     }
     return DIM3_OK;
 }
@@ -1813,21 +1819,35 @@ int dim3_join_aggregate_count_stream(dim3_ctx* c, const int32_t* R_xy, uint64_t 
 
 int dim3_join_aggregate(dim3_ctx* c, const int32_t* R_xy, const int32_t* R_v, uint64_t n_r, const int32_t* S_zy,
                         const int32_t* S_v, uint64_t n_s, int agg, const dim3_opts* opts, dim3_result* res) {
-    return run_value_host(c, agg, R_xy, R_v, n_r, S_zy, S_v, n_s, opts, res);
+    return run_value_host(c, agg, false, R_xy, R_v, n_r, S_zy, S_v, n_s, opts, res);
+}
+
+int dim3_join_aggregate_f32(dim3_ctx* c, const int32_t* R_xy, const float* R_v, uint64_t n_r, const int32_t* S_zy,
+                            const float* S_v, uint64_t n_s, int agg, const dim3_opts* opts, dim3_result* res) {
+    return run_value_host(c, agg, true, R_xy, (const int32_t*)R_v, n_r, S_zy, (const int32_t*)S_v, n_s, opts, res);
+}
+
+int dim3_join_aggregate_f32_dev(dim3_ctx* c, const int32_t* d_R, const float* d_Rv, uint64_t n_r, const int32_t* d_S,
+                                const float* d_Sv, uint64_t n_s, int agg, const dim3_opts* opts,
+                                dim3_value_row_f32* d_rows, uint64_t cap, void* stream, dim3_result* res) {
+    return run_value_dev(c, agg, true, d_R, (const int32_t*)d_Rv, n_r, d_S, (const int32_t*)d_Sv, n_s, opts, d_rows,
+                         cap, stream, res);
 }
 
 int dim3_join_aggregate_dev(dim3_ctx* c, const int32_t* d_R, const int32_t* d_Rv, uint64_t n_r, const int32_t* d_S,
                             const int32_t* d_Sv, uint64_t n_s, int agg, const dim3_opts* opts, dim3_count_row* d_rows,
                             uint64_t cap, void* stream, dim3_result* res) {
-    return run_value_dev(c, agg, d_R, d_Rv, n_r, d_S, d_Sv, n_s, opts, d_rows, cap, stream, res);
+    return run_value_dev(c, agg, false, d_R, d_Rv, n_r, d_S, d_Sv, n_s, opts, d_rows, cap, stream, res);
 }
 
 void dim3_result_free(dim3_result* res) {
     if (!res) return;
     free(res->pairs);
     free(res->rows);
+    free(res->rows_f32);
     res->pairs = nullptr;
     res->rows = nullptr;
+    res->rows_f32 = nullptr;
     res->n = 0;
 }
 

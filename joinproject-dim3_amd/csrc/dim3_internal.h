@@ -522,7 +522,8 @@ void row_bounds(const ExpandArgs& a, uint32_t nz, uint32_t* bound, hipStream_t s
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 // value aggregates (k_value.hip): agg 1 SUM, 2 MIN, 3 MAX, 4 AVG of vR + vS
-int expand_value(const ExpandArgs& a, int agg, int block, int cu_count, hipStream_t st);
+// (f32: the values are float bit patterns and the rows dim3_value_row_f32)
+int expand_value(const ExpandArgs& a, int agg, bool f32, int block, int cu_count, hipStream_t st);
 // dynamic LDS of one value-aggregate workgroup (tile_size groups of 12 bytes)
 size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block);
 // dynamic LDS of one expansion workgroup (fp: plus the fingerprint table)

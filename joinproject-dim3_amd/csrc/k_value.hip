@@ -1,5 +1,6 @@
 // Value aggregates of the join: SUM, MIN, MAX or AVG of vR + vS over the join
-// paths of each (x, z) group, with int32 tuple values and int64 results.
+// paths of each (x, z) group, with int32 tuple values and int64 results, or
+// float tuple values and float results.
 //
 // Replaces SpmmAggregate (dim3.hpp:1267-1311) run with a value policy.  The
 // reference's policy concept is insert(z, v1, v2) on a group's first path,
@@ -13,10 +14,18 @@
 // accumulator (one 64-bit ds_add / ds_min / ds_max per path) and a u32 path
 // counter (one ds_add per path; its first touch lists the group for
 // emission).  Accumulators start at the operation's identity, so the order
-// of the paths does not matter and the results are exact; AVG divides with
-// truncation toward zero, as the reference's get() does on integral values.
+// of the paths does not matter and the integer results are exact; AVG
+// divides with truncation toward zero, as the reference's get() does on
+// integral values.
+// Float values (F32): the path value is one float add vR + vS, as the
+// policy's val1 + val2 on float; SUM and AVG accumulate it in double (LDS
+// f64 atomics) and round once at the end, SUM = (float)sum and
+// AVG = (float)(sum / paths); the double sums are exact, hence independent
+// of the path order, whenever every partial sum is a double (integer-valued
+// data below 2^53, e.g. the reference's generated values).  MIN / MAX keep
+// an order-preserving integer key of the float (IEEE totalOrder, -0 < +0).
 // Per join path: two 4-byte column reads (z and vS) and two LDS atomics.
-// Per output row: one 16-byte store (x, z, int64 value).
+// Per output row: one 16-byte store (x, z, int64 value), 12 bytes for F32.
 #include "dim3_internal.h"
 
 namespace {
@@ -40,8 +49,34 @@ __device__ __forceinline__ void agg_apply(int64_t* acc, int64_t v) {
     else atomicAdd(reinterpret_cast<unsigned long long*>(acc), (unsigned long long)v);
 }
 
-template <int AGG>
+// order-preserving int32 key of a float and back (an involution on the
+// negative keys)
+__device__ __forceinline__ int32_t float_key(float f) {
+    const int32_t b = __float_as_int(f);
+    return b >= 0 ? b : b ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ uint32_t key_bits(int32_t k) { return (uint32_t)(k >= 0 ? k : k ^ 0x7FFFFFFF); }
+
+// one join path with tuple values vr, vs (bit patterns for F32)
+template <int AGG, bool F32>
+__device__ __forceinline__ void agg_path(int64_t* acc, int32_t vr, int32_t vs) {
+    if (F32) {
+        const float v = __int_as_float(vr) + __int_as_float(vs);
+        if (AGG == DIM3_AGG_SUM || AGG == DIM3_AGG_AVG) atomicAdd(reinterpret_cast<double*>(acc), (double)v);
+        else agg_apply<AGG>(acc, (int64_t)float_key(v));
+    } else {
+        agg_apply<AGG>(acc, (int64_t)vr + (int64_t)vs);
+    }
+}
+
+// the group's result: the int64 value, or the result float's bit pattern
+template <int AGG, bool F32>
 __device__ __forceinline__ int64_t agg_get(int64_t acc, uint32_t paths) {
+    if (F32) {
+        if (AGG == DIM3_AGG_SUM) return (int64_t)__float_as_uint((float)__longlong_as_double(acc));
+        if (AGG == DIM3_AGG_AVG) return (int64_t)__float_as_uint((float)(__longlong_as_double(acc) / (double)paths));
+        return (int64_t)key_bits((int32_t)acc);
+    }
     return AGG == DIM3_AGG_AVG ? acc / (int64_t)paths : acc;
 }
 
@@ -59,7 +94,7 @@ struct VLds {
     uint64_t* s_gbase;
 };
 
-template <int BLOCK, int AGG, bool ZID>
+template <int BLOCK, int AGG, bool ZID, bool F32>
 __device__ __forceinline__ void value_task(const ExpandArgs& a, uint32_t x, uint32_t t, const VLds& L,
                                            uint64_t& fsum, uint64_t& fsum2, uint64_t& fcs, uint64_t& npaths) {
     constexpr uint32_t NW = BLOCK / 64;
@@ -116,7 +151,7 @@ __device__ __forceinline__ void value_task(const ExpandArgs& a, uint32_t x, uint
             const uint32_t off = (k - s_pre[seg]) * kPiece;
             const uint32_t base = s_sb[seg] + off;  // piece k holds >= 1 path: base is in the S row
             const uint32_t len = min(s_len[seg] - off, kPiece);
-            const int64_t vr64 = s_vr[seg];
+            const int32_t vseg = s_vr[seg];  // the R value of the piece's segment
             uint32_t zl[kUnroll];
             int32_t vs[kUnroll];
 #pragma unroll
@@ -134,7 +169,7 @@ __device__ __forceinline__ void value_task(const ExpandArgs& a, uint32_t x, uint
                 uint32_t old = 1u;
                 if (zl[u] != kNone) {
                     old = atomicAdd(&L.cnt[zl[u]], 1u);
-                    agg_apply<AGG>(&L.acc[zl[u]], vr64 + (int64_t)vs[u]);
+                    agg_path<AGG, F32>(&L.acc[zl[u]], vseg, vs[u]);
                 }
                 fm[u] = __ballot(old == 0u);
                 ftot += (uint32_t)__popcll(fm[u]);
@@ -188,13 +223,14 @@ __device__ __forceinline__ void value_task(const ExpandArgs& a, uint32_t x, uint
     const int64_t ident = agg_identity<AGG>();
 
     if (a.fp_mode) {
-        // fingerprint row (x, z, v): sum += k1(x) g(z), sum2 += k2(x) v g(z), csum += v
+        // fingerprint row (x, z, v): sum += k1(x) g(z), sum2 += k2(x) v g(z),
+        // csum += v (v: the int64 value, or the result float's bit pattern)
         uint64_t S1 = 0, S2 = 0;
         for (uint32_t i = wlo + lane; i < whi; i += 64u) {
             const uint32_t w = listed ? L.touched[i] : i;
             const uint32_t n = L.cnt[w];
             if (n) {
-                const int64_t v = agg_get<AGG>(L.acc[w], n);
+                const int64_t v = agg_get<AGG, F32>(L.acc[w], n);
                 const uint64_t g = zfactor(ZID ? (int32_t)(zoff + w) : id2z[w]);
                 S1 += g;
                 S2 += (uint64_t)v * g;
@@ -216,6 +252,7 @@ __device__ __forceinline__ void value_task(const ExpandArgs& a, uint32_t x, uint
     __syncthreads();
     uint64_t pos0 = *L.s_gbase + woff;
     dim3_count_row* out = reinterpret_cast<dim3_count_row*>(a.out);
+    dim3_value_row_f32* outf = reinterpret_cast<dim3_value_row_f32*>(a.out);
     for (uint32_t i0 = wlo; i0 < whi; i0 += 64u) {
         const uint32_t i = i0 + lane;
         uint32_t n = 0, w = 0;
@@ -227,11 +264,21 @@ __device__ __forceinline__ void value_task(const ExpandArgs& a, uint32_t x, uint
         const uint64_t pos = pos0 + mbcnt64(m);
         pos0 += (uint64_t)__popcll(m);
         if (n != 0u && pos < a.cap) {
-            dim3_count_row row;
-            row.x = xv;
-            row.z = ZID ? (int32_t)(zoff + w) : id2z[w];
-            row.count = agg_get<AGG>(L.acc[w], n);
-            out[pos] = row;
+            const int32_t zv = ZID ? (int32_t)(zoff + w) : id2z[w];
+            const int64_t v = agg_get<AGG, F32>(L.acc[w], n);
+            if (F32) {
+                dim3_value_row_f32 row;
+                row.x = xv;
+                row.z = zv;
+                row.v = __uint_as_float((uint32_t)v);
+                outf[pos] = row;
+            } else {
+                dim3_count_row row;
+                row.x = xv;
+                row.z = zv;
+                row.count = v;
+                out[pos] = row;
+            }
         }
         if (i < whi) {
             L.cnt[w] = 0;
@@ -241,7 +288,7 @@ __device__ __forceinline__ void value_task(const ExpandArgs& a, uint32_t x, uint
     __syncthreads();
 }
 
-template <int BLOCK, int AGG, bool ZID>
+template <int BLOCK, int AGG, bool ZID, bool F32>
 __global__ __launch_bounds__(BLOCK) void k_value(ExpandArgs a) {
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr uint32_t NW = BLOCK / 64;
@@ -271,7 +318,7 @@ __global__ __launch_bounds__(BLOCK) void k_value(ExpandArgs a) {
         const uint32_t task = a.task_lo + __builtin_amdgcn_readfirstlane(s_task);
         if (task >= a.task_hi) break;
         const uint32_t x = task / a.ntiles;
-        value_task<BLOCK, AGG, ZID>(a, x, task - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
+        value_task<BLOCK, AGG, ZID, F32>(a, x, task - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
     }
 
     const uint64_t np = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
@@ -288,9 +335,9 @@ __global__ __launch_bounds__(BLOCK) void k_value(ExpandArgs a) {
     }
 }
 
-template <int BLOCK, int AGG, bool ZID>
+template <int BLOCK, int AGG, bool ZID, bool F32>
 int launch_one(const ExpandArgs& a, size_t lds, int cu_count, hipStream_t st) {
-    const auto kern = k_value<BLOCK, AGG, ZID>;
+    const auto kern = k_value<BLOCK, AGG, ZID, F32>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
     int occ = 0;
@@ -305,20 +352,31 @@ int launch_one(const ExpandArgs& a, size_t lds, int cu_count, hipStream_t st) {
     return (int)hipGetLastError();
 }
 
-template <int AGG, bool ZID>
+template <int AGG, bool ZID, bool F32>
 int launch_block(const ExpandArgs& a, int block, size_t lds, int cu_count, hipStream_t st) {
     switch (block) {
-        case 256: return launch_one<256, AGG, ZID>(a, lds, cu_count, st);
-        case 512: return launch_one<512, AGG, ZID>(a, lds, cu_count, st);
-        case 1024: return launch_one<1024, AGG, ZID>(a, lds, cu_count, st);
+        case 256: return launch_one<256, AGG, ZID, F32>(a, lds, cu_count, st);
+        case 512: return launch_one<512, AGG, ZID, F32>(a, lds, cu_count, st);
+        case 1024: return launch_one<1024, AGG, ZID, F32>(a, lds, cu_count, st);
         default: return (int)hipErrorInvalidValue;
     }
 }
 
-template <int AGG>
+template <int AGG, bool F32>
 int launch_agg(const ExpandArgs& a, int block, size_t lds, int cu_count, hipStream_t st) {
-    if (a.id2z == nullptr) return launch_block<AGG, true>(a, block, lds, cu_count, st);
-    return launch_block<AGG, false>(a, block, lds, cu_count, st);
+    if (a.id2z == nullptr) return launch_block<AGG, true, F32>(a, block, lds, cu_count, st);
+    return launch_block<AGG, false, F32>(a, block, lds, cu_count, st);
+}
+
+template <bool F32>
+int launch_value(const ExpandArgs& a, int agg, int block, size_t lds, int cu_count, hipStream_t st) {
+    switch (agg) {
+        case DIM3_AGG_SUM: return launch_agg<DIM3_AGG_SUM, F32>(a, block, lds, cu_count, st);
+        case DIM3_AGG_MIN: return launch_agg<DIM3_AGG_MIN, F32>(a, block, lds, cu_count, st);
+        case DIM3_AGG_MAX: return launch_agg<DIM3_AGG_MAX, F32>(a, block, lds, cu_count, st);
+        case DIM3_AGG_AVG: return launch_agg<DIM3_AGG_AVG, F32>(a, block, lds, cu_count, st);
+        default: return (int)hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -329,16 +387,11 @@ size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block)
     return (size_t)tile_size * 12u + ((size_t)touched_cap + 4u * block + 1u) * 4u;
 }
 
-int expand_value(const ExpandArgs& a, int agg, int block, int cu_count, hipStream_t st) {
+int expand_value(const ExpandArgs& a, int agg, bool f32, int block, int cu_count, hipStream_t st) {
     if (a.valR == nullptr || a.valS == nullptr || a.tile_size == 0) return (int)hipErrorInvalidValue;
     const size_t lds = value_lds_bytes(a.tile_size, a.touched_cap, (uint32_t)block);
-    switch (agg) {
-        case DIM3_AGG_SUM: return launch_agg<DIM3_AGG_SUM>(a, block, lds, cu_count, st);
-        case DIM3_AGG_MIN: return launch_agg<DIM3_AGG_MIN>(a, block, lds, cu_count, st);
-        case DIM3_AGG_MAX: return launch_agg<DIM3_AGG_MAX>(a, block, lds, cu_count, st);
-        case DIM3_AGG_AVG: return launch_agg<DIM3_AGG_AVG>(a, block, lds, cu_count, st);
-        default: return (int)hipErrorInvalidValue;
-    }
+    return f32 ? launch_value<true>(a, agg, block, lds, cu_count, st)
+               : launch_value<false>(a, agg, block, lds, cu_count, st);
 }
 
 }  // namespace dim3h

@@ -142,3 +142,74 @@ def test_values_dev_capacity(ctx, dim3, torch):
     # a bad aggregate code is an error, not a launch
     with pytest.raises(dim3.Dim3Error):
         ctx.join_aggregate(R, vR, S, vS, 9)
+
+
+# ---- float values ----
+
+def check_f32(ctx, R, vR, S, vS, agg, **kw):
+    ex = kw.get("exclude_equal", False)
+    wp, wv, wfp, sw = oracle.join_aggregate_f32(R, vR, S, vS, agg, exclude_equal=ex)
+    want = oracle.sort_pairs(wp, wv.view(np.int32).astype(np.int64))
+    g = ctx.join_aggregate_f32(R, vR, S, vS, agg, **kw)
+    got = oracle.sort_pairs(g.xz, g.counts.view(np.int32).astype(np.int64))
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), agg  # bit for bit
+    assert g.swapped == sw and g.n == wfp.count, agg
+    f = ctx.join_aggregate_f32(R, vR, S, vS, agg, output_mode=1, **kw)
+    assert f.fp.as_tuple() == wfp.as_tuple(), agg
+    return g
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "values_f32_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_values_f32_match_reference_golden(ctx, path):
+    d = np.load(path)
+    say("golden f32", os.path.basename(path), d["R"].shape, d["S"].shape)
+    for agg in AGGS:
+        g = ctx.join_aggregate_f32(d["R"], d["vR"], d["S"], d["vS"], agg)
+        gp, gb = oracle.sort_pairs(g.xz, g.counts.view(np.int32).astype(np.int64))
+        assert np.array_equal(gp, d["pairs"]) and gb.astype(np.int32).tobytes() == d[agg].tobytes(), agg
+
+
+@pytest.mark.parametrize("agg", AGGS)
+def test_values_f32_zipf(ctx, agg):
+    spec = cases.MEDIUM["aggregate_scaled"]
+    R, S = cases.make_medium(spec)
+    R = R[R[:, 0] % 8 == 0]
+    rng = np.random.default_rng(14)
+    # multiples of 1/8 below 2^20: float path values and double sums are exact
+    vR = (rng.integers(-(1 << 22), 1 << 22, R.shape[0]) / 8).astype(np.float32)
+    vS = (rng.integers(-(1 << 22), 1 << 22, S.shape[0]) / 8).astype(np.float32)
+    say("f32 zipf", agg, R.shape, S.shape)
+    check_f32(ctx, R, vR, S, vS, agg)
+    check_f32(ctx, R, vR, S, vS, agg, tile_bits=1024)
+    # the reference generator's values (tuple index) above 2^24: the float
+    # path value rounds (one IEEE add on both sides), the double sums stay exact
+    vR = (np.arange(R.shape[0]) + (1 << 24)).astype(np.float32)
+    vS = (np.arange(S.shape[0]) * 3 + (1 << 24)).astype(np.float32)
+    check_f32(ctx, R, vR, S, vS, agg)
+    E = rel(rng, 20_000, 3000, 3000)
+    E = np.concatenate([E, E[:, ::-1]])
+    vE = rng.integers(-1000, 1000, E.shape[0]).astype(np.float32)
+    check_f32(ctx, E, vE, E, vE, agg, exclude_equal=True)
+
+
+def test_values_f32_dev(ctx, dim3, torch):
+    rng = np.random.default_rng(15)
+    R, S = rel(rng, 20_000, 2000, 500), rel(rng, 15_000, 2000, 500)
+    vR = rng.integers(0, 10 ** 6, R.shape[0]).astype(np.float32)
+    vS = rng.integers(0, 10 ** 6, S.shape[0]).astype(np.float32)
+    wp, wv, _, _ = oracle.join_aggregate_f32(R, vR, S, vS, "avg")
+    n = wp.shape[0]
+    dR, dS = torch.from_numpy(R).cuda(), torch.from_numpy(S).cuda()
+    dvR, dvS = torch.from_numpy(vR).cuda(), torch.from_numpy(vS).cuda()
+    args = (dR.data_ptr(), dvR.data_ptr(), R.shape[0], dS.data_ptr(), dvS.data_ptr(), S.shape[0], "avg")
+    rc, r0 = ctx.join_aggregate_f32_dev(*args, 0, 0)
+    assert rc == dim3.ERR_CAPACITY and r0.n == n
+    out = torch.zeros((n, 3), dtype=torch.int32, device="cuda")
+    say("f32 dev materialise", n)
+    rc, r = ctx.join_aggregate_f32_dev(*args, out.data_ptr(), n)
+    assert rc == dim3.OK and r.n == n
+    rows = np.ascontiguousarray(out.cpu().numpy()).view(dim3.F32_ROW_DTYPE).reshape(-1)
+    gp, gb = oracle.sort_pairs(np.stack([rows["x"], rows["z"]], 1), rows["v"].view(np.int32).astype(np.int64))
+    want = oracle.sort_pairs(wp, wv.view(np.int32).astype(np.int64))
+    assert np.array_equal(gp, want[0]) and np.array_equal(gb, want[1])
