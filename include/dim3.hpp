@@ -10,7 +10,7 @@
 //   myvector<T,_CountType> (same layout: data, cur_num, max_num)         Include/myvector.hpp:9-86
 //   TUPLE<_Tx,_Ty,_Tv> (same layout: x, y, v)                            dim3.hpp:20-33
 //   myHasher (placeholder: hashing happens on the device)                Include/structures.hpp:92
-//   gen_rand_data (minstd_rand0, same draws)                             dim3.hpp:1556-1591
+//   gen_rand_data (minstd_rand0, same draws; TUPLE value = tuple index)   dim3.hpp:1556-1591
 //
 // Conventions kept: inputs by value (shallow), result cleared then filled
 // with malloc-compatible storage the caller frees, |R| < |S| swaps the roles
@@ -22,8 +22,12 @@
 // would wrap (Include/myvector.hpp:33-46).
 //
 // Supported instantiations: 4-byte integer _Tx/_Ty/_Tz (int, as every
-// reference driver uses); join-aggregate with the COUNT policy
-// AggregateFunctionCount<_Tv>.  Device: DIM3_DEVICE env var, default 0.
+// reference driver uses); join-aggregate with the policies
+// AggregateFunctionAvg<_Tv> (the reference's, dim3.hpp:1453-1480),
+// AggregateFunctionSum / Min / Max<_Tv> (value policies over v1 + v2) and
+// AggregateFunctionCount<_Tv>, for _Tv float (float results, double
+// accumulation) or an integer type whose values fit int32 (int64 results).
+// Device: DIM3_DEVICE env var, default 0.
 #pragma once
 
 #include <stdint.h>
@@ -103,6 +107,30 @@ struct AggregateFunctionCount {
     typedef _Tv value_type;
 };
 
+// Value policies over the path value v1 + v2: the reference's AVG
+// (dim3.hpp:1453-1480, sum / paths) and SUM / MIN / MAX.  Tags only: the
+// aggregation runs on the device.
+template <typename _Tv>
+struct AggregateFunctionAvg {
+    typedef _Tv value_type;
+    static const int code = DIM3_AGG_AVG;
+};
+template <typename _Tv>
+struct AggregateFunctionSum {
+    typedef _Tv value_type;
+    static const int code = DIM3_AGG_SUM;
+};
+template <typename _Tv>
+struct AggregateFunctionMin {
+    typedef _Tv value_type;
+    static const int code = DIM3_AGG_MIN;
+};
+template <typename _Tv>
+struct AggregateFunctionMax {
+    typedef _Tv value_type;
+    static const int code = DIM3_AGG_MAX;
+};
+
 namespace dim3_shim {
 
 inline dim3_ctx* context() {
@@ -176,6 +204,48 @@ struct DIM3_parallel : public DIM3<_Tx, _Ty, _Tz, _Hx, _Hy, _Hz, _Hxz, _Tcounter
     }
 };
 
+namespace dim3_shim {
+
+template <typename A, typename _Tv>
+constexpr bool is_value_policy() {
+    return std::is_same<A, AggregateFunctionAvg<_Tv>>::value || std::is_same<A, AggregateFunctionSum<_Tv>>::value ||
+           std::is_same<A, AggregateFunctionMin<_Tv>>::value || std::is_same<A, AggregateFunctionMax<_Tv>>::value;
+}
+
+template <typename A, typename _Tv>
+struct policy_code {
+    static const int value = A::code;
+};
+template <typename _Tv>
+struct policy_code<AggregateFunctionCount<_Tv>, _Tv> {
+    static const int value = 0;
+};
+
+// Packs (x|z, y) pairs and, for value policies, the int32 / float values.
+template <typename _Ta, typename _Ty, typename _Tv, typename _Tcounter, typename V>
+int32_t* pack(const myvector<TUPLE<_Ta, _Ty, _Tv>, _Tcounter>& T, V* vals) {
+    int32_t* p = (int32_t*)malloc(sizeof(int32_t) * 2 * (T.size() ? T.size() : 1));
+    if (!p) {
+        printf("Fail to malloc the packed relations in doJoinAggregate.\n");
+        exit(-1);
+    }
+    for (_Tcounter i = 0; i < T.size(); i++) {
+        p[2 * i] = (int32_t)T.data[i].x;
+        p[2 * i + 1] = (int32_t)T.data[i].y;
+        if (vals) {
+            const _Tv v = T.data[i].v;
+            if (std::is_integral<_Tv>::value && (v < (_Tv)INT32_MIN || v > (_Tv)INT32_MAX)) {
+                printf("DIM3JoinAggregate: tuple value %lld does not fit int32.\n", (long long)v);
+                exit(-1);
+            }
+            vals[i] = (V)v;
+        }
+    }
+    return p;
+}
+
+}  // namespace dim3_shim
+
 template <typename _Tx, typename _Ty, typename _Tz, typename _Tv, typename _Aggregate_type,
           typename _Hx = myHasher, typename _Hy = myHasher, typename _Hz = myHasher, typename _Hxz = myHasher,
           typename _Tcounter = uint32_t>
@@ -183,39 +253,58 @@ struct DIM3JoinAggregate {
     static void doJoinAggregate(myvector<TUPLE<_Tx, _Ty, _Tv>, _Tcounter>& _R,
                                 myvector<TUPLE<_Tz, _Ty, _Tv>, _Tcounter>& _S,
                                 myvector<TUPLE<_Tx, _Tz, _Tv>, _Tcounter>& result) {
-        static_assert(std::is_same<_Aggregate_type, AggregateFunctionCount<_Tv>>::value,
-                      "libdim3 implements the COUNT aggregate (AggregateFunctionCount<_Tv>)");
+        static_assert(std::is_same<_Aggregate_type, AggregateFunctionCount<_Tv>>::value ||
+                          dim3_shim::is_value_policy<_Aggregate_type, _Tv>(),
+                      "libdim3 implements AggregateFunctionAvg / Sum / Min / Max / Count<_Tv>");
         static_assert(dim3_shim::is_i32<_Tx>() && dim3_shim::is_i32<_Ty>() && dim3_shim::is_i32<_Tz>(),
                       "libdim3 joins 4-byte integer attributes");
+        static_assert(std::is_same<_Tv, float>::value || std::is_integral<_Tv>::value,
+                      "tuple values are float or integers");
+        const int agg = dim3_shim::policy_code<_Aggregate_type, _Tv>::value;
+        const bool f32 = std::is_same<_Tv, float>::value;
         result.clear();
-        int32_t* r = (int32_t*)malloc(sizeof(int32_t) * 2 * (_R.size() ? _R.size() : 1));
-        int32_t* s = (int32_t*)malloc(sizeof(int32_t) * 2 * (_S.size() ? _S.size() : 1));
-        if (!r || !s) {
-            printf("Fail to malloc the packed relations in doJoinAggregate.\n");
+        const uint64_t nr = _R.size(), ns = _S.size();
+        int32_t *vri = nullptr, *vsi = nullptr;
+        float *vrf = nullptr, *vsf = nullptr;
+        if (agg && f32) {
+            vrf = (float*)malloc(sizeof(float) * (nr ? nr : 1));
+            vsf = (float*)malloc(sizeof(float) * (ns ? ns : 1));
+        } else if (agg) {
+            vri = (int32_t*)malloc(sizeof(int32_t) * (nr ? nr : 1));
+            vsi = (int32_t*)malloc(sizeof(int32_t) * (ns ? ns : 1));
+        }
+        if (agg && !(f32 ? (vrf && vsf) : (vri && vsi))) {
+            printf("Fail to malloc the packed values in doJoinAggregate.\n");
             exit(-1);
         }
-        for (_Tcounter i = 0; i < _R.size(); i++) {
-            r[2 * i] = (int32_t)_R.data[i].x;
-            r[2 * i + 1] = (int32_t)_R.data[i].y;
-        }
-        for (_Tcounter i = 0; i < _S.size(); i++) {
-            s[2 * i] = (int32_t)_S.data[i].x;
-            s[2 * i + 1] = (int32_t)_S.data[i].y;
-        }
+        int32_t* r = f32 ? dim3_shim::pack(_R, vrf) : dim3_shim::pack(_R, vri);
+        int32_t* s = f32 ? dim3_shim::pack(_S, vsf) : dim3_shim::pack(_S, vsi);
         dim3_opts o;
         dim3_opts_default(&o);
         dim3_result res;
-        int rc = dim3_join_aggregate_count(dim3_shim::context(), r, _R.size(), s, _S.size(), &o, &res);
+        int rc;
+        if (!agg) rc = dim3_join_aggregate_count(dim3_shim::context(), r, nr, s, ns, &o, &res);
+        else if (f32) rc = dim3_join_aggregate_f32(dim3_shim::context(), r, vrf, nr, s, vsf, ns, agg, &o, &res);
+        else rc = dim3_join_aggregate(dim3_shim::context(), r, vri, nr, s, vsi, ns, agg, &o, &res);
         free(r);
         free(s);
+        free(vri);
+        free(vsi);
+        free(vrf);
+        free(vsf);
         if (rc != DIM3_OK) dim3_shim::die("DIM3JoinAggregate doJoinAggregate", rc);
         if (res.n > (uint64_t)(_Tcounter)~(_Tcounter)0) {
             printf("DIM3JoinAggregate: %llu rows do not fit the result counter type.\n", (unsigned long long)res.n);
             exit(-1);
         }
         result.reserve((_Tcounter)(res.n ? res.n : 1));
-        for (uint64_t i = 0; i < res.n; i++)
-            result.data[i] = TUPLE<_Tx, _Tz, _Tv>((_Tx)res.rows[i].x, (_Tz)res.rows[i].z, (_Tv)res.rows[i].count);
+        for (uint64_t i = 0; i < res.n; i++) {
+            if (agg && f32)
+                result.data[i] = TUPLE<_Tx, _Tz, _Tv>((_Tx)res.rows_f32[i].x, (_Tz)res.rows_f32[i].z,
+                                                      (_Tv)res.rows_f32[i].v);
+            else
+                result.data[i] = TUPLE<_Tx, _Tz, _Tv>((_Tx)res.rows[i].x, (_Tz)res.rows[i].z, (_Tv)res.rows[i].count);
+        }
         result.cur_num = (_Tcounter)res.n;
         dim3_result_free(&res);
     }
@@ -240,20 +329,32 @@ inline void gen_rand_data(myvector<pair<int, int>>& R, myvector<pair<int, int>>&
     }
 }
 
-inline void gen_rand_data(myvector<TUPLE<int, int, long long>>& R, myvector<TUPLE<int, int, long long>>& S,
-                          int nR = 1e7, int nS = 1e7, int MODX = 1e6, int MODY = 1e6, int MODZ = 1e6,
-                          int seed = 492) {
+// TUPLE overloads (dim3.hpp:1570-1591): the same draws, value = tuple index.
+template <typename _Tv>
+inline void gen_rand_tuples(myvector<TUPLE<int, int, _Tv>>& R, myvector<TUPLE<int, int, _Tv>>& S, int nR, int nS,
+                            int MODX, int MODY, int MODZ, int seed) {
     R.clear();
     S.clear();
     std::minstd_rand0 e(seed);
     for (int i = 0; i < nR; i++) {
         int x = (int)(e() % MODX);
         int y = (int)(e() % MODY);
-        R.push_back(TUPLE<int, int, long long>(x, y, i));
+        R.push_back(TUPLE<int, int, _Tv>(x, y, (_Tv)i));
     }
     for (int i = 0; i < nS; i++) {
         int z = (int)(e() % MODZ);
         int y = (int)(e() % MODY);
-        S.push_back(TUPLE<int, int, long long>(z, y, i));
+        S.push_back(TUPLE<int, int, _Tv>(z, y, (_Tv)i));
     }
+}
+
+inline void gen_rand_data(myvector<TUPLE<int, int, float>>& R, myvector<TUPLE<int, int, float>>& S, int nR = 1e7,
+                          int nS = 1e7, int MODX = 1e6, int MODY = 1e6, int MODZ = 1e6, int seed = 492) {
+    gen_rand_tuples(R, S, nR, nS, MODX, MODY, MODZ, seed);
+}
+
+inline void gen_rand_data(myvector<TUPLE<int, int, long long>>& R, myvector<TUPLE<int, int, long long>>& S,
+                          int nR = 1e7, int nS = 1e7, int MODX = 1e6, int MODY = 1e6, int MODZ = 1e6,
+                          int seed = 492) {
+    gen_rand_tuples(R, S, nR, nS, MODX, MODY, MODZ, seed);
 }

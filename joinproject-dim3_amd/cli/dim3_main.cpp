@@ -1,7 +1,10 @@
 // Counterpart of the reference drivers main.cpp, main_parallel.cpp and
 // join_aggregate.cpp (the same data and the same two output lines), built
 // against the drop-in header include/dim3.hpp:
-//   dim3_main [project|parallel|aggregate]
+//   dim3_main [project|parallel|aggregate|count]
+// aggregate is join_aggregate.cpp's query, AVG(R.v + S.v) over
+// TUPLE<int,int,float> (join_aggregate.cpp:6-23); count is the COUNT
+// aggregate over TUPLE<int,int,long long>.
 #include <chrono>
 #include <cstring>
 
@@ -12,6 +15,15 @@ int main(int argc, char* argv[]) {
     std::chrono::high_resolution_clock::time_point t0, t1;
     unsigned n = 0;
     if (!strcmp(mode, "aggregate")) {
+        myvector<TUPLE<int, int, float>> R, S, result;
+        gen_rand_data(R, S);
+        result.clear();
+        t0 = std::chrono::high_resolution_clock::now();
+        DIM3JoinAggregate<int, int, int, float, AggregateFunctionAvg<float>, myHasher, myHasher, myHasher, myHasher,
+                          uint32_t>::doJoinAggregate(R, S, result);
+        t1 = std::chrono::high_resolution_clock::now();
+        n = result.size();
+    } else if (!strcmp(mode, "count")) {
         myvector<TUPLE<int, int, long long>> R, S, result;
         gen_rand_data(R, S);
         result.clear();

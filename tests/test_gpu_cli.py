@@ -1,0 +1,29 @@
+"""The reference's drivers rebuilt against the drop-in header
+(joinproject-dim3_amd/cli/dim3_main.cpp over include/dim3.hpp): main.cpp,
+main_parallel.cpp, join_aggregate.cpp (AVG of R.v + S.v over
+TUPLE<int,int,float>) and the COUNT aggregate print the reference's output
+contract on gen_rand_data's defaults, |Result|=98931829 (main.cpp:23-24,
+join_aggregate.cpp:27-28)."""
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "joinproject-dim3_amd", "dim3_main")
+
+
+@pytest.mark.parametrize("mode", ["project", "parallel", "aggregate", "count"])
+def test_reference_driver_output_contract(mode):
+    if not os.path.exists(EXE):
+        pytest.fail("dim3_main is not built (make -C joinproject-dim3_amd)")
+    print("dim3_main", mode, flush=True)
+    sys.stdout.flush()
+    out = subprocess.run([EXE, mode], capture_output=True, text=True, timeout=240, check=True).stdout
+    print(out, flush=True)
+    lines = out.strip().splitlines()
+    assert re.fullmatch(r"It takes [0-9.]+ seconds\.", lines[-2])
+    assert lines[-1] == "|Result|=98931829"
