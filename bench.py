@@ -255,6 +255,97 @@ def aggregate_cpu_baseline(R, S, target_s, slices):
                       % (done, slices, slices, total, total_t)}
 
 
+def value_alg_bytes(st, n_out, row_bytes, lookup=False):
+    """Algorithmic HBM bytes of one value-aggregate launch: the z column and
+    the value of every join path (8 bytes), per R entry its y, its value and
+    its S row bounds (16 bytes), the row bounds and value of every x (12
+    bytes), and the output rows (row_bytes each, plus a 4-byte z value read
+    when z is hash-mapped)."""
+    return 8 * st.paths_sparse + 16 * st.n_r + 12 * st.nx + (row_bytes + (4 if lookup else 0)) * n_out
+
+
+def value_bench(args, d, torch):
+    """join_aggregate.cpp's query: SELECT R.x, S.z, AVG(R.v + S.v) GROUP BY x, z
+    over gen_rand_data's TUPLE<int,int,float> relations (1e7 tuples per side,
+    attributes mod 1e6, seed 492, value = tuple index); 98,931,829 groups
+    materialised as 12-byte (x, z, float) rows in HBM."""
+    from dim3_amd import workloads
+
+    t0 = time.perf_counter()
+    R, vR, S, vS = workloads.gen_rand_tuples()
+    log("aggregate_avg: %d + %d tuples in %.1fs" % (R.shape[0], S.shape[0], time.perf_counter() - t0))
+    ctx = d.Context(0)
+    dev = torch.device("cuda:0")
+    dR, dS = torch.from_numpy(R).to(dev), torch.from_numpy(S).to(dev)
+    dvR, dvS = torch.from_numpy(vR).to(dev), torch.from_numpy(vS).to(dev)
+    torch.cuda.synchronize()
+    args_q = (dR.data_ptr(), dvR.data_ptr(), R.shape[0], dS.data_ptr(), dvS.data_ptr(), S.shape[0], "avg")
+    kw = dict(block=args.block, tile_bits=args.tile_bits, hash_only=args.hash_only)
+    rc, r0 = ctx.join_aggregate_f32_dev(*args_q, 0, 0, **kw)
+    n_out = r0.n
+    out = torch.empty((max(n_out, 1), 3), dtype=torch.int32, device=dev)
+    log("output rows: %d" % n_out)
+
+    def step():
+        rc, res = ctx.join_aggregate_f32_dev(*args_q, out.data_ptr(), n_out, **kw)
+        if rc != d.OK:
+            raise RuntimeError("query failed: %s" % ctx.last_error())
+        return res
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ress = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    res = ress[-1]
+    st = res.stats
+    exp_ms = float(np.mean([r.stats.ms_expand for r in ress]))
+    b_kernel = value_alg_bytes(st, n_out, 12, z_lookup(S))
+    achieved = b_kernel / (exp_ms * 1e-3) / 1e9
+    log("stats: " + json.dumps({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.as_dict().items()}))
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+
+            if oracle.ref_available():
+                t0 = time.perf_counter()
+                rp, rv = oracle.ref_join_aggregate_f32(R, vR, S, vS, "avg")
+                dt = time.perf_counter() - t0
+                cpu = {"value": rp.shape[0] / dt, "unit": "pairs/s", "cores": 1, "kind": "reference",
+                       "sample": "the whole query: reference DIM3JoinAggregate with its AggregateFunctionAvg<float> "
+                                 "(oracle/_ref, one thread), %d groups in %.2f s" % (rp.shape[0], dt)}
+                del rp, rv
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline failed: %r" % (e,))
+    line = {
+        "metric": METRIC,
+        "value": n_out * args.steps / elapsed,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 values (f64 accumulation), int32 attributes",
+        "data": "synthetic (the reference's gen_rand_data draws, resident in HBM before timing)",
+        "config": {"workload": "join-aggregate AVG(R.v + S.v) of join_aggregate.cpp, gen_rand_data defaults",
+                   "tuples_per_side": int(R.shape[0]), "attr_mod": 1_000_000, "seed": 492,
+                   "output": "materialized (x, z, float) rows in HBM", "groups": int(n_out),
+                   "join_paths": int(st.out_j), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_value", "kernel_ms": exp_ms,
+                     "alg_bytes": b_kernel},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def config_bench(args, d, torch):
     """The two single-query Zipf workloads.
     project_small: join-project of R(a,b), S(c,b), 1M distinct tuples each,
@@ -517,10 +608,12 @@ def main():
     ap.add_argument("--edges", type=int, default=4_000_000)
     ap.add_argument("--max-degree", type=int, default=50_000)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--workload", choices=["two_hop", "project_small", "aggregate", "planted_block", "batch", "serve"],
+    ap.add_argument("--workload", choices=["two_hop", "project_small", "aggregate", "aggregate_avg", "planted_block",
+                                           "batch", "serve"],
                     default="two_hop",
                     help="two_hop: the Chung-Lu self-join (default); project_small: 1M x 1M Zipf(1.0) "
                          "join-project; aggregate: 10M x 10M Zipf(1.2) COUNT join-aggregate (fingerprint); "
+                         "aggregate_avg: join_aggregate.cpp's AVG(R.v + S.v) on gen_rand_data's float tuples; "
                          "planted_block: R and S are each a "
                          "--block-keys square block of (attr, key) cells at --block-density (the dense block "
                          "of the served stream's large requests), for the dense-part comparison; batch: "
@@ -573,6 +666,8 @@ def main():
 
     if args.workload == "batch":
         return batch_bench(args, d, torch)
+    if args.workload == "aggregate_avg":
+        return value_bench(args, d, torch)
     if args.workload == "serve":
         return serve_bench(args, d, torch)
     if args.workload in ("project_small", "aggregate"):

@@ -65,6 +65,16 @@ enum SlotId {
     WS_VAL_S,
     WS_OUT,
     WS_ROW_BOUND,
+    WS_VT_FW,
+    WS_VT_PW,
+    WS_VT_FH,
+    WS_VT_PH,
+    WS_VT_WAVE,
+    WS_VT_HASH,
+    WS_VT_BIG,
+    WS_ROWPTR_Y,
+    WS_VT_ROWN,
+    WS_VT_ROWOFF,
     WS_XW,
     WS_ZW,
     WS_DHIST,
@@ -995,6 +1005,57 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
 
 int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st);
 
+// Task lists of a value expansion: rows with at most 256 join paths are
+// expanded one per wavefront (k_value_rows); rows with at most
+// min(H / 2, touched_cap) paths go through an LDS hash table of H slots
+// (the largest power of two whose 16-byte slots fit the tile's 12-byte
+// groups), one workgroup task per row; the others are tiled.
+int value_task_lists(dim3_ctx* c, dim3h::ExpandArgs& a, hipStream_t st) {
+    auto& P = c->bank().plan;
+    if (P.nx == 0) return DIM3_OK;
+    uint32_t bits = 0;
+    while (4ull * (2ull << bits) <= 3ull * P.tile_size) bits++;
+    // a table under 64 slots: no hash rows
+    const uint32_t limit = bits < 6 ? 0u : std::min<uint32_t>((1u << bits) / 2u, P.touched_cap);
+    uint32_t *bound, *fw, *pw, *fh, *ph, *wx, *hx, *bx;
+    TRY(ws(c, WS_ROW_BOUND, P.nx, &bound));
+    TRY(ws(c, WS_VT_FW, P.nx, &fw));
+    TRY(ws(c, WS_VT_PW, P.nx, &pw));
+    TRY(ws(c, WS_VT_FH, P.nx, &fh));
+    TRY(ws(c, WS_VT_PH, P.nx, &ph));
+    TRY(ws(c, WS_VT_WAVE, P.nx, &wx));
+    TRY(ws(c, WS_VT_HASH, P.nx, &hx));
+    TRY(ws(c, WS_VT_BIG, P.nx, &bx));
+    uint64_t* part;
+    TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed((uint64_t)P.nx + 1), &part));
+    Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
+    dim3h::value_tasks(a, P.nz, limit, bound, fw, pw, fh, ph, part, &sc->n_wave, &sc->n_hash, wx, hx, bx, st);
+    // untiled rows read S by y alone: a compact row index (the tiled one is
+    // ntiles times larger and its y boundaries lie ntiles words apart)
+    if (P.ntiles > 1) {
+        uint32_t* ry;
+        TRY(ws(c, WS_ROWPTR_Y, (uint64_t)P.ny + 1, &ry));
+        dim3h::rowptr_y(a.rowptrS, P.ny, P.ntiles, ry, st);
+        a.rowptrY = ry;
+    } else {
+        a.rowptrY = a.rowptrS;
+    }
+    uint32_t* rn;
+    uint64_t* ro;
+    TRY(ws(c, WS_VT_ROWN, P.nx, &rn));
+    TRY(ws(c, WS_VT_ROWOFF, P.nx, &ro));
+    a.row_n = rn;
+    a.row_off = ro;
+    a.scan_part = part;
+    a.wave_x = wx;
+    a.hash_x = hx;
+    a.big_x = bx;
+    a.n_wave = &sc->n_wave;
+    a.n_hash = &sc->n_hash;
+    a.hash_bits = bits;
+    return DIM3_OK;
+}
+
 // Expansion of the prepared plan into `out` (cap rows) or the fingerprint.
 int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
     auto& P = c->bank().plan;
@@ -1006,7 +1067,9 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
         HIPCHK(c, hipStreamSynchronize(st));
         return DIM3_OK;
     }
-    return expand_args(c, make_args(c, fp_mode, out, cap), st);
+    dim3h::ExpandArgs a = make_args(c, fp_mode, out, cap);
+    if (P.agg) TRY(value_task_lists(c, a, st));
+    return expand_args(c, a, st);
 }
 
 // One expansion launch with prepared arguments (all rows, or one chunk of
@@ -1016,7 +1079,7 @@ int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st) {
     Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
     // reset the counters the expansion owns (a re-run after a capacity miss)
     hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
-    hipMemsetAsync(&sc->task, 0, sizeof(uint32_t), st);
+    hipMemsetAsync(&sc->task, 0, 2 * sizeof(uint32_t), st);  // task, task2
     HIPCHK(c, hipEventRecord(c->ev[4], st));
     int e = P.agg          ? dim3h::expand_value(a, P.agg, P.vf32, P.block, c->grid_cus(), st)
             : P.count_mode ? dim3h::expand_count(a, P.block, c->grid_cus(), st)

@@ -339,12 +339,14 @@ struct Scalars {
     uint64_t fp_csum;
     uint64_t paths;      // sparse join paths expanded
     uint32_t task;       // expansion work counter
-    uint32_t pad0;
+    uint32_t task2;      // second work counter (value expansion: wave rows)
     uint64_t heavy_n;    // heavy keys selected
     uint64_t nxh, nzh;
     uint64_t s_surv;     // S tuples whose y joined
     int32_t mm[8];       // min/max of R.x, R.y, S.z, S.y
     uint64_t agg_sum2;   // COUNT fingerprint from the join paths: sum of k2(x) g(z) over paths
+    uint64_t n_hash;     // value aggregates: rows through the LDS hash table
+    uint64_t n_wave;     // value aggregates: rows expanded one per wavefront
     uint64_t spare[4];
 };
 
@@ -427,6 +429,9 @@ void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uin
                    hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st);
+// the same with 64-bit offsets
+void scan_u32_u64(const uint32_t* cnt, uint32_t n, uint64_t* out_excl, uint64_t* partials, uint64_t* tot,
+                  hipStream_t st);
 // vin / vout (nullable): the tuple values, written beside the columns
 void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, uint32_t* cursorD,
                int32_t* col, const int32_t* vin, int32_t* vout, hipStream_t st);
@@ -473,6 +478,21 @@ struct ExpandArgs {
     // value aggregates: the tuple value of each R / S CSR entry
     const int32_t* valR;
     const int32_t* valS;
+    // value task lists (hash_x == nullptr: every row tiled): *n_wave rows
+    // wave_x[] one per wavefront, *n_hash rows hash_x[] through the
+    // 2^hash_bits-slot LDS hash table, the others big_x[] in z tiles
+    const uint32_t* rowptrY;  // S rows by y over all tiles (ny + 1 entries)
+    const uint32_t* wave_x;
+    const uint32_t* hash_x;
+    const uint32_t* big_x;
+    const uint64_t* n_wave;
+    const uint64_t* n_hash;
+    uint32_t hash_bits;
+    // materialised wave rows: group count and output offset per wave row,
+    // and the scan's partials (nx entries each)
+    uint32_t* row_n;
+    uint64_t* row_off;
+    uint64_t* scan_part;
 };
 
 // returns 0 or a hipError_t
@@ -526,6 +546,14 @@ int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_value(const ExpandArgs& a, int agg, bool f32, int block, int cu_count, hipStream_t st);
 // dynamic LDS of one value-aggregate workgroup (tile_size groups of 12 bytes)
 size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block);
+// rowptrY[y] = rowptrS[y * ntiles] for y <= ny
+void rowptr_y(const uint32_t* rowptrS, uint32_t ny, uint32_t ntiles, uint32_t* rowptrY, hipStream_t st);
+// the value expansion's task lists (see ExpandArgs): rows with at most 256
+// join paths are wave rows, then rows with at most `limit` hash rows;
+// bound / fw / pw / fh / ph: nx words each
+void value_tasks(const ExpandArgs& a, uint32_t nz, uint32_t limit, uint32_t* bound, uint32_t* fw, uint32_t* pw,
+                 uint32_t* fh, uint32_t* ph, uint64_t* partials, uint64_t* n_wave, uint64_t* n_hash,
+                 uint32_t* wave_x, uint32_t* hash_x, uint32_t* big_x, hipStream_t st);
 // dynamic LDS of one expansion workgroup (fp: plus the fingerprint table)
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp);
 

@@ -613,6 +613,10 @@ void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* par
               hipStream_t st) {
     scan_exclusive(ValU32{cnt}, OutU32{out_excl}, n, partials, tot, st);
 }
+void scan_u32_u64(const uint32_t* cnt, uint32_t n, uint64_t* out_excl, uint64_t* partials, uint64_t* tot,
+                  hipStream_t st) {
+    scan_exclusive(ValU32{cnt}, OutU64{out_excl}, n, partials, tot, st);
+}
 void scatter_s(const uint32_t* keyS, const int32_t* zidS, uint64_t nS, uint32_t* cursor, uint32_t* cursorD,
                int32_t* col, const int32_t* vin, int32_t* vout, hipStream_t st) {
     if (nS)

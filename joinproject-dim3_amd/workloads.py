@@ -12,6 +12,9 @@ permutation so that input order carries no structure.
   aggregate_large   R, S of 10M distinct tuples, a,c ~ U[0,2e6), b ~ Zipf(1.2) on [0,1e6)
   batch_instances   256 independent (R, S) with log-uniform sizes in [1e4, 2e6]
   served_stream     Poisson arrival stream of mixed project / COUNT requests
+  gen_rand_tuples   the reference's gen_rand_data (dim3.hpp:1556-1591): the
+                    std::minstd_rand0 draws from seed 492, (x, y) then (z, y),
+                    tuple value = tuple index (join_aggregate.cpp's data)
 """
 import math
 
@@ -218,3 +221,31 @@ def served_stream(n_requests=200, rate_per_s=50.0, large_frac=0.05, agg_frac=0.2
             p.update({"large": True, "n": large_n or 50_000_000})
         reqs.append({"id": i, "t": float(t[i]), "kind": kind, "params": p})
     return reqs
+
+
+def minstd_draws(seed, n):
+    """The first n outputs of std::minstd_rand0 seeded with `seed`:
+    x_k = seed * 16807^k mod (2^31 - 1), k = 1..n, from a table of powers
+    (16807^(B i) and 16807^j for j < B), all products below 2^62."""
+    m, a, B = 2147483647, 16807, 1 << 16
+    k = np.arange(1, n + 1, dtype=np.int64)
+    lo = np.empty(B, dtype=np.int64)
+    lo[0] = 1
+    for j in range(1, B):
+        lo[j] = lo[j - 1] * a % m
+    aB = lo[B - 1] * a % m
+    hi = np.empty(n // B + 2, dtype=np.int64)
+    hi[0] = 1
+    for i in range(1, hi.shape[0]):
+        hi[i] = hi[i - 1] * aB % m
+    p = hi[k // B] * lo[k % B] % m
+    return p * (seed % m) % m
+
+
+def gen_rand_tuples(nR=10_000_000, nS=10_000_000, modx=1_000_000, mody=1_000_000, modz=1_000_000, seed=492):
+    """R (x, y) and S (z, y) int32 pairs of gen_rand_data and their float32
+    tuple values (the tuple index, as the TUPLE<int,int,float> overload)."""
+    d = minstd_draws(seed, 2 * (nR + nS))
+    R = np.stack([d[0:2 * nR:2] % modx, d[1:2 * nR:2] % mody], 1).astype(np.int32)
+    S = np.stack([d[2 * nR::2] % modz, d[2 * nR + 1::2] % mody], 1).astype(np.int32)
+    return R, np.arange(nR, dtype=np.float32), S, np.arange(nS, dtype=np.float32)

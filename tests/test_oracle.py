@@ -393,3 +393,17 @@ def test_oracle_values_f32_match_reference_golden(path):
         p, v, _, _ = oracle.join_aggregate_f32(d["R"], d["vR"], d["S"], d["vS"], agg)
         gp, gb = oracle.sort_pairs(p, v.view(np.int32).astype(np.int64))
         assert np.array_equal(gp, d["pairs"]) and gb.astype(np.int32).tobytes() == d[agg].tobytes(), agg
+
+
+def test_workload_generator_matches_oracle_generator():
+    """The benchmark's numpy minstd_rand0 generator (workloads.gen_rand_tuples)
+    draws what the oracle's gen_rand_data draws."""
+    import __graft_entry__ as GE
+
+    GE.load_package()
+    from dim3_amd import workloads
+
+    R, vR, S, vS = workloads.gen_rand_tuples(300_000, 200_000, 50_000, 20_000, 50_000, 77)
+    R2, S2 = oracle.gen_rand_data(300_000, 200_000, 50_000, 20_000, 50_000, 77)
+    assert np.array_equal(R, R2) and np.array_equal(S, S2)
+    assert np.array_equal(vR, np.arange(300_000, dtype=np.float32)) and vS.shape == (200_000,)
