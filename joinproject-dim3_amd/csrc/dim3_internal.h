@@ -338,8 +338,6 @@ struct Scalars {
     uint64_t fp_sum2;
     uint64_t fp_csum;
     uint64_t paths;      // sparse join paths expanded
-    uint32_t task;       // expansion work counter
-    uint32_t task2;      // second work counter (value expansion: wave rows)
     uint64_t heavy_n;    // heavy keys selected
     uint64_t nxh, nzh;
     uint64_t s_surv;     // S tuples whose y joined
@@ -348,6 +346,11 @@ struct Scalars {
     uint64_t n_hash;     // value aggregates: rows through the LDS hash table
     uint64_t n_wave;     // value aggregates: rows expanded one per wavefront
     uint64_t spare[4];
+    // work counters on a cache line of their own: atomics on one line
+    // serialise, and every task fetch would queue behind the row reservations
+    // on out_n
+    alignas(128) uint32_t task;  // expansion work counter
+    uint32_t task2;              // second work counter (value expansion: wave rows)
 };
 
 size_t scan_partials_needed(uint64_t n);
