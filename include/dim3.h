@@ -264,6 +264,16 @@ int dim3_batch_join_aggregate_count_dev(dim3_ctx* ctx, const dim3_instance* inst
 int dim3_count_join_paths(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy, uint64_t n_s,
                           uint64_t* out_j);
 
+/* EstimateJoinCardinality<...>::estimate (dim3.hpp:190-270) on the device:
+ * the reference's sampled join size, the same number (identity hash
+ * myHasher: keys with their low b bits zero, b the halvings that bring |S|
+ * to at most 10000; the sampled path count << b; with no sampled path, the
+ * paths between the first 1000 tuples of each side scaled by
+ * |R|/1e3 * |S|/1e3, prefixes clamped to the input sizes).  The swap rule
+ * applies first (the reference asserts |R| >= |S|).  Host inputs. */
+int dim3_estimate_join_cardinality(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy,
+                                   uint64_t n_s, int64_t* estimate);
+
 /* The dense part's product on its own (the boolean product that
  * Hybrid_solution's DENSE part evaluates one (x, dense z) pair at a time,
  * dim3.hpp:894-950): C = A * Bt^T over int8 on the MFMA cores with int32

@@ -111,7 +111,7 @@ EXPORTS = [
     "dim3_server_put_relation", "dim3_server_put_relation_dev", "dim3_server_drop_relation", "dim3_submit",
     "dim3_poll", "dim3_wait", "dim3_server_get_stats", "dim3_join_project_stream",
     "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
-    "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev",
+    "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev", "dim3_estimate_join_cardinality",
 ]
 
 _lib = None
@@ -179,6 +179,8 @@ def lib():
     L.dim3_join_aggregate_f32_dev.restype = i32
     L.dim3_result_free.argtypes = [ctypes.POINTER(Result)]
     L.dim3_result_free.restype = None
+    L.dim3_estimate_join_cardinality.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_int64)]
+    L.dim3_estimate_join_cardinality.restype = i32
     L.dim3_count_join_paths.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
     L.dim3_count_join_paths.restype = i32
     for f in (L.dim3_batch_join_project_dev, L.dim3_batch_join_aggregate_count_dev):
@@ -555,6 +557,15 @@ class Context:
         for k, v in d.items():
             setattr(t, k, v)
         self._check(self._L.dim3_set_tuning(self._h, ctypes.byref(t)), "dim3_set_tuning")
+
+    def estimate_join_cardinality(self, R, S):
+        """EstimateJoinCardinality::estimate: the reference's sampled join size."""
+        R, S = _pairs(R), _pairs(S)
+        out = ctypes.c_int64()
+        rc = self._L.dim3_estimate_join_cardinality(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0],
+                                                    ctypes.byref(out))
+        self._check(rc, "dim3_estimate_join_cardinality")
+        return int(out.value)
 
     def count_join_paths(self, R, S):
         R, S = _pairs(R), _pairs(S)

@@ -1976,6 +1976,58 @@ int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const i
     return DIM3_OK;
 }
 
+int dim3_estimate_join_cardinality(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,
+                                   int64_t* estimate) {
+    if (!c || !estimate) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    *estimate = 0;
+    if (nR < nS) {  // the swap rule; the reference asserts |R| >= |S| (dim3.hpp:194)
+        std::swap(R_xy, S_zy);
+        std::swap(nR, nS);
+    }
+    if (nS == 0) return DIM3_OK;
+    if (!R_xy || !S_zy) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    if (nR >= (1ull << 31)) return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^31-1 tuples");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    c->qstream = st;
+    int2 *dR, *dS;
+    TRY(ws(c, WS_IN_R, nR, &dR));
+    TRY(ws(c, WS_IN_S, nS, &dS));
+    HIPCHK(c, hipMemcpyAsync(dR, R_xy, nR * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(dS, S_zy, nS * 8, hipMemcpyHostToDevice, st));
+    // sample bits: halve |S| until it is at most 10000 (dim3.hpp:262-269)
+    uint32_t bits = 0;
+    for (uint64_t n = nS; n > 10000; n /= 2) bits++;
+    const uint32_t smask = bits >= 32 ? 0xFFFFFFFFu : (1u << bits) - 1u;
+    const uint32_t cap = pow2_at_least(2 * nS);  // every S tuple sampled, at most
+    uint32_t *keys, *cnt;
+    Scalars* sc;
+    TRY(ws(c, WS_KEYS_Y, (uint64_t)cap + 1, &keys));
+    TRY(ws(c, WS_DEGS_Y, (uint64_t)cap + 1, &cnt));
+    TRY(ws(c, WS_SCALARS, 1, &sc));
+    HIPCHK(c, hipMemsetAsync(sc, 0, sizeof(Scalars), st));
+    dim3h::KeyMap m{keys, cap - 1, 0, cap + 1, 0, 0};
+    dim3h::map_init(m, st);
+    dim3h::fill_u32(cnt, (uint64_t)cap + 1, 0, st);
+    dim3h::estimate_sample((const int2*)dR, nR, (const int2*)dS, nS, smask, keys, cap - 1, cnt,
+                           (unsigned long long*)&sc->spare[0], st);
+    dim3h::estimate_prefix((const int2*)dR, (uint32_t)std::min<uint64_t>(nR, 1000), (const int2*)dS,
+                           (uint32_t)std::min<uint64_t>(nS, 1000), (unsigned long long*)&sc->spare[1], st);
+    LAUNCHCHK(c, "estimate");
+    HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const int64_t sampled = (int64_t)c->h_sc->spare[0];
+    if (sampled != 0) {
+        *estimate = sampled << bits;
+    } else {  // dim3.hpp:215-221
+        const double r2 = (double)(int64_t)c->h_sc->spare[1];
+        *estimate = (int64_t)(r2 * ((double)nR / 1e3) * ((double)nS / 1e3));
+    }
+    return DIM3_OK;
+}
+
 int dim3_dense_product(dim3_ctx* c, const int8_t* A, uint64_t M, const int8_t* Bt, uint64_t N, uint64_t K,
                        uint32_t* bits, int32_t* counts) {
     if (!c) return DIM3_ERR_INVALID;

@@ -549,6 +549,13 @@ int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_value(const ExpandArgs& a, int agg, bool f32, int block, int cu_count, hipStream_t st);
 // dynamic LDS of one value-aggregate workgroup (tile_size groups of 12 bytes)
 size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block);
+// EstimateJoinCardinality (k_estimate.hip): *out += the join paths of the
+// tuples whose key has no bit of smask set (keys / cnt: a hash table of
+// hmask + 2 slots, keys initialised empty, cnt zero); estimate_prefix:
+// *out = the join paths between R[0, a) and S[0, b), a, b <= 1000
+void estimate_sample(const int2* R, uint64_t nR, const int2* S, uint64_t nS, uint32_t smask, uint32_t* keys,
+                     uint32_t hmask, uint32_t* cnt, unsigned long long* out, hipStream_t st);
+void estimate_prefix(const int2* R, uint32_t a, const int2* S, uint32_t b, unsigned long long* out, hipStream_t st);
 // rowptrY[y] = rowptrS[y * ntiles] for y <= ny
 void rowptr_y(const uint32_t* rowptrS, uint32_t ny, uint32_t ntiles, uint32_t* rowptrY, hipStream_t st);
 // the value expansion's task lists (see ExpandArgs): rows with at most 256
