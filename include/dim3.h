@@ -218,6 +218,23 @@ int dim3_join_aggregate_f32_dev(dim3_ctx* ctx, const int32_t* d_R, const float* 
                                 const int32_t* d_S, const float* d_Sv, uint64_t n_s, int agg, const dim3_opts* opts,
                                 dim3_value_row_f32* d_rows, uint64_t cap, void* stream, dim3_result* res);
 
+/* Value aggregates straight from the reference's TUPLE arrays (the payload
+ * of myvector<TUPLE<int,int,_Tv>>, dim3.hpp:20-33): tuple i of R is the
+ * int32 first and the int32 key at R_tuples + i * stride, and its value at
+ * byte value_offset, of value_type DIM3_VALUE_I32, DIM3_VALUE_I64 (must fit
+ * int32, else DIM3_ERR_INVALID) or DIM3_VALUE_F32.  The tuples are copied to
+ * the device as they are and split there.  agg: DIM3_AGG_* on the values,
+ * or 0 for COUNT (values ignored).  Rows: res->rows (COUNT, integer
+ * values) or res->rows_f32 (float values).  Stride and offset are multiples
+ * of 4, the offset at least 8.  TUPLE<int,int,float>: stride 12, offset 8;
+ * TUPLE<int,int,long long>: 16, 8. */
+#define DIM3_VALUE_I32 0
+#define DIM3_VALUE_I64 1
+#define DIM3_VALUE_F32 2
+int dim3_join_aggregate_tuples(dim3_ctx* ctx, const void* R_tuples, uint64_t n_r, const void* S_tuples, uint64_t n_s,
+                               uint32_t stride, uint32_t value_offset, uint32_t value_type, int agg,
+                               const dim3_opts* opts, dim3_result* res);
+
 /* ---- streamed output: rows handed to the caller in chunks ----
  * For outputs larger than any one buffer (the chunked STREAM output mode).
  * The query runs in chunks of consecutive x rows, each producing at most

@@ -221,29 +221,6 @@ struct policy_code<AggregateFunctionCount<_Tv>, _Tv> {
     static const int value = 0;
 };
 
-// Packs (x|z, y) pairs and, for value policies, the int32 / float values.
-template <typename _Ta, typename _Ty, typename _Tv, typename _Tcounter, typename V>
-int32_t* pack(const myvector<TUPLE<_Ta, _Ty, _Tv>, _Tcounter>& T, V* vals) {
-    int32_t* p = (int32_t*)malloc(sizeof(int32_t) * 2 * (T.size() ? T.size() : 1));
-    if (!p) {
-        printf("Fail to malloc the packed relations in doJoinAggregate.\n");
-        exit(-1);
-    }
-    for (_Tcounter i = 0; i < T.size(); i++) {
-        p[2 * i] = (int32_t)T.data[i].x;
-        p[2 * i + 1] = (int32_t)T.data[i].y;
-        if (vals) {
-            const _Tv v = T.data[i].v;
-            if (std::is_integral<_Tv>::value && (v < (_Tv)INT32_MIN || v > (_Tv)INT32_MAX)) {
-                printf("DIM3JoinAggregate: tuple value %lld does not fit int32.\n", (long long)v);
-                exit(-1);
-            }
-            vals[i] = (V)v;
-        }
-    }
-    return p;
-}
-
 }  // namespace dim3_shim
 
 template <typename _Tx, typename _Ty, typename _Tz, typename _Tv, typename _Aggregate_type,
@@ -260,52 +237,51 @@ struct DIM3JoinAggregate {
                       "libdim3 joins 4-byte integer attributes");
         static_assert(std::is_same<_Tv, float>::value || std::is_integral<_Tv>::value,
                       "tuple values are float or integers");
+        static_assert(std::is_same<_Tv, float>::value || sizeof(_Tv) == 4 || sizeof(_Tv) == 8,
+                      "integer tuple values of 4 or 8 bytes");
+        typedef TUPLE<_Tx, _Ty, _Tv> InT;
+        typedef TUPLE<_Tx, _Tz, _Tv> OutT;
         const int agg = dim3_shim::policy_code<_Aggregate_type, _Tv>::value;
         const bool f32 = std::is_same<_Tv, float>::value;
+        // the tuples go to the device as they are (the value's offset from a
+        // probe object: TUPLE has no default constructor)
+        const InT probe((_Tx)0, (_Ty)0, (_Tv)0);
+        const uint32_t voff = (uint32_t)((const char*)&probe.v - (const char*)&probe);
+        const uint32_t vtype = f32 ? DIM3_VALUE_F32 : sizeof(_Tv) == 8 ? DIM3_VALUE_I64 : DIM3_VALUE_I32;
         result.clear();
-        const uint64_t nr = _R.size(), ns = _S.size();
-        int32_t *vri = nullptr, *vsi = nullptr;
-        float *vrf = nullptr, *vsf = nullptr;
-        if (agg && f32) {
-            vrf = (float*)malloc(sizeof(float) * (nr ? nr : 1));
-            vsf = (float*)malloc(sizeof(float) * (ns ? ns : 1));
-        } else if (agg) {
-            vri = (int32_t*)malloc(sizeof(int32_t) * (nr ? nr : 1));
-            vsi = (int32_t*)malloc(sizeof(int32_t) * (ns ? ns : 1));
-        }
-        if (agg && !(f32 ? (vrf && vsf) : (vri && vsi))) {
-            printf("Fail to malloc the packed values in doJoinAggregate.\n");
-            exit(-1);
-        }
-        int32_t* r = f32 ? dim3_shim::pack(_R, vrf) : dim3_shim::pack(_R, vri);
-        int32_t* s = f32 ? dim3_shim::pack(_S, vsf) : dim3_shim::pack(_S, vsi);
         dim3_opts o;
         dim3_opts_default(&o);
         dim3_result res;
-        int rc;
-        if (!agg) rc = dim3_join_aggregate_count(dim3_shim::context(), r, nr, s, ns, &o, &res);
-        else if (f32) rc = dim3_join_aggregate_f32(dim3_shim::context(), r, vrf, nr, s, vsf, ns, agg, &o, &res);
-        else rc = dim3_join_aggregate(dim3_shim::context(), r, vri, nr, s, vsi, ns, agg, &o, &res);
-        free(r);
-        free(s);
-        free(vri);
-        free(vsi);
-        free(vrf);
-        free(vsf);
+        int rc = dim3_join_aggregate_tuples(dim3_shim::context(), _R.data, _R.size(), _S.data, _S.size(),
+                                            (uint32_t)sizeof(InT), voff, vtype, agg, &o, &res);
         if (rc != DIM3_OK) dim3_shim::die("DIM3JoinAggregate doJoinAggregate", rc);
         if (res.n > (uint64_t)(_Tcounter)~(_Tcounter)0) {
+            dim3_result_free(&res);
             printf("DIM3JoinAggregate: %llu rows do not fit the result counter type.\n", (unsigned long long)res.n);
             exit(-1);
         }
-        result.reserve((_Tcounter)(res.n ? res.n : 1));
-        for (uint64_t i = 0; i < res.n; i++) {
-            if (agg && f32)
-                result.data[i] = TUPLE<_Tx, _Tz, _Tv>((_Tx)res.rows_f32[i].x, (_Tz)res.rows_f32[i].z,
-                                                      (_Tv)res.rows_f32[i].v);
-            else
-                result.data[i] = TUPLE<_Tx, _Tz, _Tv>((_Tx)res.rows[i].x, (_Tz)res.rows[i].z, (_Tv)res.rows[i].count);
+        const OutT oprobe((_Tx)0, (_Tz)0, (_Tv)0);
+        const uint32_t ooff = (uint32_t)((const char*)&oprobe.v - (const char*)&oprobe);
+        // rows whose layout is the library's row record are handed over as they are
+        const bool same_f32 = agg && f32 && sizeof(OutT) == sizeof(dim3_value_row_f32) && ooff == 8;
+        const bool same_i64 = !f32 && std::is_integral<_Tv>::value && sizeof(_Tv) == 8 &&
+                              sizeof(OutT) == sizeof(dim3_count_row) && ooff == 8;
+        if (res.n && (same_f32 || same_i64)) {
+            free(result.data);
+            result.data = same_f32 ? (OutT*)(void*)res.rows_f32 : (OutT*)(void*)res.rows;
+            res.rows_f32 = nullptr;
+            res.rows = nullptr;
+            result.cur_num = result.max_num = (_Tcounter)res.n;
+        } else if (res.n) {
+            result.reserve((_Tcounter)res.n);
+            for (uint64_t i = 0; i < res.n; i++) {
+                if (agg && f32)
+                    result.data[i] = OutT((_Tx)res.rows_f32[i].x, (_Tz)res.rows_f32[i].z, (_Tv)res.rows_f32[i].v);
+                else
+                    result.data[i] = OutT((_Tx)res.rows[i].x, (_Tz)res.rows[i].z, (_Tv)res.rows[i].count);
+            }
+            result.cur_num = (_Tcounter)res.n;
         }
-        result.cur_num = (_Tcounter)res.n;
         dim3_result_free(&res);
     }
 };

@@ -100,6 +100,11 @@ class ServerStats(ctypes.Structure):
 
 COUNT_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("count", "<i8")])
 F32_ROW_DTYPE = np.dtype([("x", "<i4"), ("z", "<i4"), ("v", "<f4")])  # dim3_value_row_f32
+# the reference's TUPLE<int,int,float> (12 bytes) and TUPLE<int,int,long long> (16 bytes)
+TUPLE_F32 = np.dtype({"names": ["x", "y", "v"], "formats": ["<i4", "<i4", "<f4"], "offsets": [0, 4, 8],
+                      "itemsize": 12})
+TUPLE_I64 = np.dtype({"names": ["x", "y", "v"], "formats": ["<i4", "<i4", "<i8"], "offsets": [0, 4, 8],
+                      "itemsize": 16})
 
 # every symbol include/dim3.h declares (checked by the ABI test)
 EXPORTS = [
@@ -112,6 +117,7 @@ EXPORTS = [
     "dim3_poll", "dim3_wait", "dim3_server_get_stats", "dim3_join_project_stream",
     "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
     "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev", "dim3_estimate_join_cardinality",
+    "dim3_join_aggregate_tuples",
 ]
 
 _lib = None
@@ -177,6 +183,9 @@ def lib():
     L.dim3_join_aggregate_f32.restype = i32
     L.dim3_join_aggregate_f32_dev.argtypes = L.dim3_join_aggregate_dev.argtypes
     L.dim3_join_aggregate_f32_dev.restype = i32
+    L.dim3_join_aggregate_tuples.argtypes = [vp, vp, u64, vp, u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                             i32, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
+    L.dim3_join_aggregate_tuples.restype = i32
     L.dim3_result_free.argtypes = [ctypes.POINTER(Result)]
     L.dim3_result_free.restype = None
     L.dim3_estimate_join_cardinality.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_int64)]
@@ -444,6 +453,34 @@ class Context:
         if rc not in (OK, ERR_CAPACITY):
             self._check(rc, "dim3_join_aggregate_f32_dev")
         return rc, QueryResult(res)
+
+    def join_aggregate_tuples(self, R, S, agg, **kw):
+        """Value aggregate (agg "sum" / "min" / "max" / "avg", or "count")
+        over TUPLE arrays as the reference lays them out: structured arrays
+        with int32 fields x (or z), y and a value field v (int32, int64 or
+        float32) at any 4-byte-aligned offset >= 8, e.g. TUPLE_F32 /
+        TUPLE_I64.  Rows: COUNT_ROW_DTYPE, or F32_ROW_DTYPE for float values."""
+        opts = make_opts(**kw)
+        if R.dtype != S.dtype:
+            raise ValueError("R and S must share the tuple layout")
+        vdt, voff = R.dtype.fields["v"][0], R.dtype.fields["v"][1]
+        vtype = {np.dtype("<i4"): 0, np.dtype("<i8"): 1, np.dtype("<f4"): 2}[vdt]
+        code = 0 if agg == "count" else _agg(agg)
+        R, S = np.ascontiguousarray(R), np.ascontiguousarray(S)
+        res = Result()
+        rc = self._L.dim3_join_aggregate_tuples(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0], R.dtype.itemsize,
+                                                voff, vtype, code, ctypes.byref(opts), ctypes.byref(res))
+        self._check(rc, "dim3_join_aggregate_tuples")
+        f32 = vtype == 2 and code != 0
+        rows = None
+        if opts.output_mode == OUT_MATERIALIZE:
+            n = int(res.n)
+            dt, width, ptr = (F32_ROW_DTYPE, 12, res.rows_f32) if f32 else (COUNT_ROW_DTYPE, 16, res.rows)
+            rows = (np.frombuffer((ctypes.c_char * (width * n)).from_address(ptr), dtype=dt).copy() if n
+                    else np.zeros((0,), dtype=dt))
+        out = QueryResult(res, rows=rows)
+        self._L.dim3_result_free(ctypes.byref(res))
+        return out
 
     def join_aggregate_dev(self, d_R, d_vR, n_r, d_S, d_vS, n_s, agg, d_out, cap, stream=None, **kw):
         """Device-resident value aggregate (addresses as ints); rows are

@@ -1279,31 +1279,12 @@ int run_value_dev(dim3_ctx* c, int agg, bool f32, const int32_t* d_R, const int3
     return DIM3_OK;
 }
 
-int run_value_host(dim3_ctx* c, int agg, bool f32, const int32_t* R, const int32_t* vR, uint64_t nR,
-                   const int32_t* S, const int32_t* vS, uint64_t nS, const dim3_opts* opts, dim3_result* res) {
-    if (!c || !res) return DIM3_ERR_INVALID;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    c->err.clear();
-    memset(res, 0, sizeof(*res));
-    if ((nR && !R) || (nS && !S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
-    TRY(check_value_query(c, agg, nR, vR, nS, vS));
-    dim3_opts o;
-    TRY(check_opts(c, opts, &o));
-    o.dense_mode = DIM3_DENSE_OFF;
+// A value query (agg 0: COUNT) on device-resident inputs, rows copied to a
+// malloc'd host buffer (res->rows, or res->rows_f32 for float values).
+int value_query_to_host(dim3_ctx* c, int agg, bool f32, const int2* dR, const int32_t* dvR, uint64_t nR,
+                        const int2* dS, const int32_t* dvS, uint64_t nS, dim3_opts o, hipStream_t st,
+                        dim3_result* res) {
     const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
-    HIPCHK(c, hipSetDevice(c->device));
-    hipStream_t st = c->stream;
-    c->qstream = st;
-    int2 *dR, *dS;
-    int32_t *dvR, *dvS;
-    TRY(ws(c, WS_IN_R, std::max<uint64_t>(nR, 1), &dR));
-    TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
-    TRY(ws(c, WS_IN_VR, std::max<uint64_t>(nR, 1), &dvR));
-    TRY(ws(c, WS_IN_VS, std::max<uint64_t>(nS, 1), &dvS));
-    if (nR) HIPCHK(c, hipMemcpyAsync(dR, R, nR * 8, hipMemcpyHostToDevice, st));
-    if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
-    if (nR) HIPCHK(c, hipMemcpyAsync(dvR, vR, nR * 4, hipMemcpyHostToDevice, st));
-    if (nS) HIPCHK(c, hipMemcpyAsync(dvS, vS, nS * 4, hipMemcpyHostToDevice, st));
     ValueSpec vals;
     vals.agg = agg;
     vals.f32 = f32;
@@ -1324,6 +1305,11 @@ int run_value_host(dim3_ctx* c, int agg, bool f32, const int32_t* R, const int32
         cap = c->bank().ws[WS_OUT].bytes / row_bytes;
     }
     TRY(expand(c, fp, dout, cap, st));
+    if (dense_overflowed(c, *c->h_sc)) {  // COUNT: a multiplicity above 127, redo without the int8 product
+        o.dense_mode = DIM3_DENSE_OFF;
+        TRY(prepare(c, true, dR, nR, dS, nS, o, st, 0, vals));
+        TRY(expand(c, fp, dout, cap, st));
+    }
     if (!fp && !P.empty && c->h_sc->out_n > cap) {
         cap = c->h_sc->out_n;
         TRY(ws_get(c, WS_OUT, cap * row_bytes, &dout));
@@ -1342,6 +1328,85 @@ int run_value_host(dim3_ctx* c, int agg, bool f32, const int32_t* R, const int32
         else res->rows = (dim3_count_row*)h;
     }
     return DIM3_OK;
+}
+
+int run_value_host(dim3_ctx* c, int agg, bool f32, const int32_t* R, const int32_t* vR, uint64_t nR,
+                   const int32_t* S, const int32_t* vS, uint64_t nS, const dim3_opts* opts, dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if ((nR && !R) || (nS && !S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    TRY(check_value_query(c, agg, nR, vR, nS, vS));
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    o.dense_mode = DIM3_DENSE_OFF;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    c->qstream = st;
+    int2 *dR, *dS;
+    int32_t *dvR, *dvS;
+    TRY(ws(c, WS_IN_R, std::max<uint64_t>(nR, 1), &dR));
+    TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
+    TRY(ws(c, WS_IN_VR, std::max<uint64_t>(nR, 1), &dvR));
+    TRY(ws(c, WS_IN_VS, std::max<uint64_t>(nS, 1), &dvS));
+    if (nR) HIPCHK(c, hipMemcpyAsync(dR, R, nR * 8, hipMemcpyHostToDevice, st));
+    if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
+    if (nR) HIPCHK(c, hipMemcpyAsync(dvR, vR, nR * 4, hipMemcpyHostToDevice, st));
+    if (nS) HIPCHK(c, hipMemcpyAsync(dvS, vS, nS * 4, hipMemcpyHostToDevice, st));
+    return value_query_to_host(c, agg, f32, dR, dvR, nR, dS, dvS, nS, o, st, res);
+}
+
+// Value aggregate (agg 0: COUNT) over the reference's TUPLE arrays: the
+// tuples are copied as they are and split into (first, key) pairs and int32
+// values (float bit patterns) on the device.
+int run_tuples(dim3_ctx* c, int agg, const void* R, uint64_t nR, const void* S, uint64_t nS, uint32_t stride,
+               uint32_t voff, uint32_t vtype, const dim3_opts* opts, dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if ((nR && !R) || (nS && !S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    if (agg < 0 || agg > DIM3_AGG_AVG) return fail(c, DIM3_ERR_INVALID, "bad aggregate %d", agg);
+    const uint32_t vbytes = vtype == DIM3_VALUE_I64 ? 8u : 4u;
+    if (vtype > DIM3_VALUE_F32 || stride % 4u || voff % 4u || voff < 8u || voff + vbytes > stride)
+        return fail(c, DIM3_ERR_INVALID, "bad tuple layout (stride %u, value at %u, type %u)", stride, voff, vtype);
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    if (agg) o.dense_mode = DIM3_DENSE_OFF;
+    if (!agg && o.dense_mode == DIM3_DENSE_BITOR)
+        return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    c->qstream = st;
+    int2 *dR, *dS;
+    int32_t *dvR, *dvS;
+    uint8_t* raw;
+    Scalars* sc;
+    TRY(ws(c, WS_IN_R, std::max<uint64_t>(nR, 1), &dR));
+    TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
+    TRY(ws(c, WS_IN_VR, std::max<uint64_t>(nR, 1), &dvR));
+    TRY(ws(c, WS_IN_VS, std::max<uint64_t>(nS, 1), &dvS));
+    TRY(ws(c, WS_SCALARS, 1, &sc));
+    HIPCHK(c, hipMemsetAsync(sc, 0, sizeof(Scalars), st));
+    uint32_t* bad = (uint32_t*)&sc->spare[2];
+    const uint64_t nmax = std::max<uint64_t>(std::max(nR, nS), 1);
+    TRY(ws(c, WS_OUT, nmax * stride, &raw));  // staging; the output reuses the slot afterwards
+    if (nR) {
+        HIPCHK(c, hipMemcpyAsync(raw, R, nR * stride, hipMemcpyHostToDevice, st));
+        dim3h::split_tuples(raw, nR, stride, voff, vtype, dR, dvR, bad, st);
+    }
+    if (nS) {
+        HIPCHK(c, hipMemcpyAsync(raw, S, nS * stride, hipMemcpyHostToDevice, st));
+        dim3h::split_tuples(raw, nS, stride, voff, vtype, dS, dvS, bad, st);
+    }
+    LAUNCHCHK(c, "split tuples");
+    uint32_t hbad = 0;
+    HIPCHK(c, hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (hbad && agg) return fail(c, DIM3_ERR_INVALID, "a 64-bit tuple value does not fit int32");
+    return value_query_to_host(c, agg, agg && vtype == DIM3_VALUE_F32, dR, agg ? dvR : nullptr, nR, dS,
+                               agg ? dvS : nullptr, nS, o, st, res);
 }
 
 // Streamed output: the plan's rows are expanded in chunks of consecutive x
@@ -1883,6 +1948,12 @@ int dim3_join_aggregate_count_stream(dim3_ctx* c, const int32_t* R_xy, uint64_t 
 int dim3_join_aggregate(dim3_ctx* c, const int32_t* R_xy, const int32_t* R_v, uint64_t n_r, const int32_t* S_zy,
                         const int32_t* S_v, uint64_t n_s, int agg, const dim3_opts* opts, dim3_result* res) {
     return run_value_host(c, agg, false, R_xy, R_v, n_r, S_zy, S_v, n_s, opts, res);
+}
+
+int dim3_join_aggregate_tuples(dim3_ctx* c, const void* R_tuples, uint64_t n_r, const void* S_tuples, uint64_t n_s,
+                               uint32_t stride, uint32_t value_offset, uint32_t value_type, int agg,
+                               const dim3_opts* opts, dim3_result* res) {
+    return run_tuples(c, agg, R_tuples, n_r, S_tuples, n_s, stride, value_offset, value_type, opts, res);
 }
 
 int dim3_join_aggregate_f32(dim3_ctx* c, const int32_t* R_xy, const float* R_v, uint64_t n_r, const int32_t* S_zy,

@@ -432,6 +432,9 @@ void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uin
                    hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st);
+// TUPLE arrays (dim3_join_aggregate_tuples) into pairs and int32 values
+void split_tuples(const uint8_t* raw, uint64_t n, uint32_t stride, uint32_t voff, uint32_t vtype, int2* pairs,
+                  int32_t* vals, uint32_t* bad, hipStream_t st);
 // the same with 64-bit offsets
 void scan_u32_u64(const uint32_t* cnt, uint32_t n, uint64_t* out_excl, uint64_t* partials, uint64_t* tot,
                   hipStream_t st);

@@ -459,6 +459,23 @@ __global__ void k_scatter_s(const uint32_t* __restrict__ keyS, const int32_t* __
     if (vout != nullptr) vout[pos] = vin[i];
 }
 
+// The reference's TUPLE arrays (stride bytes per tuple: int32 first, int32
+// key, the value at voff) split into pairs and int32 values; a 64-bit value
+// outside int32 sets *bad.
+__global__ void k_split_tuples(const uint8_t* __restrict__ raw, uint64_t n, uint32_t stride, uint32_t voff,
+                               uint32_t vtype, int2* __restrict__ pairs, int32_t* __restrict__ vals, uint32_t* bad) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* t = reinterpret_cast<const uint32_t*>(raw + i * stride);  // stride, voff: multiples of 4
+    pairs[i] = make_int2((int32_t)t[0], (int32_t)t[1]);
+    const uint32_t lo = t[voff / 4u];
+    if (vtype == DIM3_VALUE_I64) {
+        const uint32_t hi = t[voff / 4u + 1u];
+        if (hi != ((int32_t)lo < 0 ? 0xFFFFFFFFu : 0u)) atomicOr(bad, 1u);
+    }
+    vals[i] = (int32_t)lo;
+}
+
 // z ID of each x value, or -1 (self-pair filter of a self-join)
 __global__ void k_x2z_self(const int32_t* __restrict__ id2x, uint32_t nx, dim3h::KeyMap mz,
                            const uint32_t* __restrict__ degZ, const uint64_t* __restrict__ ex_z,
@@ -612,6 +629,12 @@ void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uin
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
               hipStream_t st) {
     scan_exclusive(ValU32{cnt}, OutU32{out_excl}, n, partials, tot, st);
+}
+void split_tuples(const uint8_t* raw, uint64_t n, uint32_t stride, uint32_t voff, uint32_t vtype, int2* pairs,
+                  int32_t* vals, uint32_t* bad, hipStream_t st) {
+    if (n)
+        hipLaunchKernelGGL(k_split_tuples, dim3(grid_for(n)), dim3(kThreads), 0, st, raw, n, stride, voff, vtype, pairs,
+                           vals, bad);
 }
 void scan_u32_u64(const uint32_t* cnt, uint32_t n, uint64_t* out_excl, uint64_t* partials, uint64_t* tot,
                   hipStream_t st) {

@@ -213,3 +213,40 @@ def test_values_f32_dev(ctx, dim3, torch):
     gp, gb = oracle.sort_pairs(np.stack([rows["x"], rows["z"]], 1), rows["v"].view(np.int32).astype(np.int64))
     want = oracle.sort_pairs(wp, wv.view(np.int32).astype(np.int64))
     assert np.array_equal(gp, want[0]) and np.array_equal(gb, want[1])
+
+
+# ---- TUPLE-array ingestion ----
+
+def tuples(dt, pairs, v):
+    t = np.zeros(pairs.shape[0], dtype=dt)
+    t["x"], t["y"], t["v"] = pairs[:, 0], pairs[:, 1], v
+    return t
+
+
+def test_values_from_tuple_arrays(ctx, dim3):
+    rng = np.random.default_rng(16)
+    R, S = rel(rng, 30_000, 3000, 800), rel(rng, 20_000, 3000, 800)
+    # TUPLE<int,int,float>, AVG (join_aggregate.cpp's layout)
+    vR = rng.integers(0, 10 ** 6, R.shape[0]).astype(np.float32)
+    vS = rng.integers(0, 10 ** 6, S.shape[0]).astype(np.float32)
+    g = ctx.join_aggregate_tuples(tuples(dim3.TUPLE_F32, R, vR), tuples(dim3.TUPLE_F32, S, vS), "avg")
+    wp, wv, _, _ = oracle.join_aggregate_f32(R, vR, S, vS, "avg")
+    got = oracle.sort_pairs(g.xz, g.counts.view(np.int32).astype(np.int64))
+    want = oracle.sort_pairs(wp, wv.view(np.int32).astype(np.int64))
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    # TUPLE<int,int,long long> (16 bytes, padded), SUM and COUNT
+    iR = rng.integers(-(1 << 31), 1 << 31, R.shape[0]).astype(np.int64)
+    iS = rng.integers(-(1 << 31), 1 << 31, S.shape[0]).astype(np.int64)
+    tR, tS = tuples(dim3.TUPLE_I64, R, iR), tuples(dim3.TUPLE_I64, S, iS)
+    g = ctx.join_aggregate_tuples(tR, tS, "sum")
+    wp, wv, _, _ = oracle.join_aggregate(R, iR.astype(np.int32), S, iS.astype(np.int32), "sum")
+    got, want = oracle.sort_pairs(g.xz, g.counts), oracle.sort_pairs(wp, wv)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    g = ctx.join_aggregate_tuples(tR, tS, "count")
+    wp, wc, _, _ = oracle.join_aggregate_count(R, S)
+    got, want = oracle.sort_pairs(g.xz, g.counts), oracle.sort_pairs(wp, wc)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    # a 64-bit value outside int32 is an error, not a wrong answer
+    tR["v"][7] = 1 << 40
+    with pytest.raises(dim3.Dim3Error):
+        ctx.join_aggregate_tuples(tR, tS, "max")
