@@ -1010,13 +1010,13 @@ int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st);
 // min(H / 2, touched_cap) paths go through an LDS hash table of H slots
 // (the largest power of two whose 16-byte slots fit the tile's 12-byte
 // groups), one workgroup task per row; the others are tiled.
-int value_task_lists(dim3_ctx* c, dim3h::ExpandArgs& a, hipStream_t st) {
+int value_task_lists(dim3_ctx* c, dim3h::ExpandArgs& a, hipStream_t st, bool hash_rows) {
     auto& P = c->bank().plan;
     if (P.nx == 0) return DIM3_OK;
     uint32_t bits = 0;
     while (4ull * (2ull << bits) <= 3ull * P.tile_size) bits++;
     // a table under 64 slots: no hash rows
-    const uint32_t limit = bits < 6 ? 0u : std::min<uint32_t>((1u << bits) / 2u, P.touched_cap);
+    const uint32_t limit = bits < 6 || !hash_rows ? 0u : std::min<uint32_t>((1u << bits) / 2u, P.touched_cap);
     uint32_t *bound, *fw, *pw, *fh, *ph, *wx, *hx, *bx;
     TRY(ws(c, WS_ROW_BOUND, P.nx, &bound));
     TRY(ws(c, WS_VT_FW, P.nx, &fw));
@@ -1068,7 +1068,11 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
         return DIM3_OK;
     }
     dim3h::ExpandArgs a = make_args(c, fp_mode, out, cap);
-    if (P.agg) TRY(value_task_lists(c, a, st));
+    if (P.agg) TRY(value_task_lists(c, a, st, true));
+    // an all-sparse COUNT or join-project: rows of at most 256 paths one per
+    // wavefront, the others as (x, z-tile) tasks (a million short rows in z
+    // tiles would be a million near-empty tasks per tile)
+    else if (P.dense == DIM3_DENSE_OFF && !P.core) TRY(value_task_lists(c, a, st, false));
     return expand_args(c, a, st);
 }
 
@@ -1081,6 +1085,10 @@ int expand_args(dim3_ctx* c, const dim3h::ExpandArgs& a, hipStream_t st) {
     hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
     hipMemsetAsync(&sc->task, 0, 2 * sizeof(uint32_t), st);  // task, task2
     HIPCHK(c, hipEventRecord(c->ev[4], st));
+    if (!P.agg && a.wave_x != nullptr) {
+        const int e = dim3h::expand_short_rows(a, P.count_mode, c->grid_cus(), st);
+        if (e != 0) return fail(c, DIM3_ERR_DEVICE, "short-row launch failed: %s", hipGetErrorString((hipError_t)e));
+    }
     int e = P.agg          ? dim3h::expand_value(a, P.agg, P.vf32, P.block, c->grid_cus(), st)
             : P.count_mode ? dim3h::expand_count(a, P.block, c->grid_cus(), st)
                            : dim3h::expand_project(a, P.block, c->grid_cus(), st);

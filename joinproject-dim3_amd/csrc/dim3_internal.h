@@ -550,6 +550,9 @@ int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 // value aggregates (k_value.hip): agg 1 SUM, 2 MIN, 3 MAX, 4 AVG of vR + vS
 // (f32: the values are float bit patterns and the rows dim3_value_row_f32)
 int expand_value(const ExpandArgs& a, int agg, bool f32, int block, int cu_count, hipStream_t st);
+// the short rows (wave_x) of an all-sparse COUNT / join-project query, one
+// per wavefront; k_expand then takes the other rows (big_x)
+int expand_short_rows(const ExpandArgs& a, bool count, int cu_count, hipStream_t st);
 // dynamic LDS of one value-aggregate workgroup (tile_size groups of 12 bytes)
 size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block);
 // EstimateJoinCardinality (k_estimate.hip): *out += the join paths of the

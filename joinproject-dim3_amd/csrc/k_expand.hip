@@ -642,6 +642,11 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     if (a.fp_mode) fp_table_init<BLOCK>(reinterpret_cast<uint64_t*>(smem));
     for (uint32_t w = tid; w < a.tile_words; w += BLOCK) L.tile[w] = 0;
     uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
+    // with a row list the short rows went one per wavefront (k_value_rows):
+    // the tasks are (big_x[r], t) for the other rows
+    const bool lists = a.big_x != nullptr;
+    const uint32_t lo = lists ? 0u : a.task_lo;
+    const uint32_t hi = lists ? (a.nx - (uint32_t)*a.n_wave) * a.ntiles : a.task_hi;
 
     for (;;) {
         if (tid == 0) {
@@ -650,10 +655,11 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         }
         __syncthreads();
         // task and x are wavefront-uniform: scalar loads
-        const uint32_t task = a.task_lo + __builtin_amdgcn_readfirstlane(s_task);
-        if (task >= a.task_hi) break;
-        const uint32_t x = task / a.ntiles;
-        row_task<BLOCK, COUNT, ZID>(a, x, task - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
+        const uint32_t task = lo + __builtin_amdgcn_readfirstlane(s_task);
+        if (task >= hi) break;
+        const uint32_t r = task / a.ntiles;
+        const uint32_t x = lists ? a.big_x[r] : r;
+        row_task<BLOCK, COUNT, ZID>(a, x, task - r * a.ntiles, L, fsum, fsum2, fcs, npaths);
     }
 
     flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);

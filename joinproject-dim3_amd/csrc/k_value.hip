@@ -46,6 +46,12 @@ constexpr uint32_t kUnroll = 4;
 constexpr uint32_t kPiece = 64 * kUnroll;  // join paths per wavefront work item
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
+// Besides DIM3_AGG_SUM .. DIM3_AGG_AVG, the wave-row kernel serves the short
+// rows of all-sparse COUNT and join-project queries: no values, a group's
+// result is its path counter (COUNT) or nothing (project: the distinct z).
+constexpr int kAggCount = 5;
+constexpr int kAggProject = 6;
+
 template <int AGG>
 __device__ __forceinline__ int64_t agg_identity() {
     return AGG == DIM3_AGG_MIN ? INT64_MAX : AGG == DIM3_AGG_MAX ? INT64_MIN : 0;
@@ -69,6 +75,7 @@ __device__ __forceinline__ uint32_t key_bits(int32_t k) { return (uint32_t)(k >=
 // one join path with tuple values vr, vs (bit patterns for F32)
 template <int AGG, bool F32>
 __device__ __forceinline__ void agg_path(int64_t* acc, int32_t vr, int32_t vs) {
+    if (AGG >= kAggCount) return;  // the path counter is the whole state
     if (F32) {
         const float v = __int_as_float(vr) + __int_as_float(vs);
         if (AGG == DIM3_AGG_SUM || AGG == DIM3_AGG_AVG) atomicAdd(reinterpret_cast<double*>(acc), (double)v);
@@ -81,6 +88,7 @@ __device__ __forceinline__ void agg_path(int64_t* acc, int32_t vr, int32_t vs) {
 // the group's result: the int64 value, or the result float's bit pattern
 template <int AGG, bool F32>
 __device__ __forceinline__ int64_t agg_get(int64_t acc, uint32_t paths) {
+    if (AGG >= kAggCount) return (int64_t)paths;
     if (F32) {
         if (AGG == DIM3_AGG_SUM) return (int64_t)__float_as_uint((float)__longlong_as_double(acc));
         if (AGG == DIM3_AGG_AVG) return (int64_t)__float_as_uint((float)(__longlong_as_double(acc) / (double)paths));
@@ -491,7 +499,7 @@ __global__ __launch_bounds__(256) void k_value_rows(ExpandArgs a) {
             const bool in_row = i < re;
             const uint32_t ic = in_row ? i : re - 1u;
             const uint32_t y = (uint32_t)a.colR[ic] & 0x7FFFFFFFu;
-            const int32_t vr = a.valR[ic];
+            const int32_t vr = AGG >= kAggCount ? 0 : a.valR[ic];
             const uint32_t sb = a.rowptrY[y], se = a.rowptrY[y + 1];  // the y's S row over all tiles
             const uint32_t len = in_row ? se - sb : 0u;
             if (PASS != 1) npaths += len;
@@ -511,7 +519,7 @@ __global__ __launch_bounds__(256) void k_value_rows(ExpandArgs a) {
                     if (W.pre[seg + step] <= p) seg += step;
                 const uint32_t j = ok ? W.sb[seg] + (p - W.pre[seg]) : W.sb[0];
                 const uint32_t z = (uint32_t)a.colS[j];
-                const int32_t vs = a.valS[j];
+                const int32_t vs = AGG >= kAggCount ? 0 : a.valS[j];
                 uint32_t old = 1u, slot = 0;
                 if (ok) {
                     slot = hash_slot(W.keys, z, kWaveBits);
@@ -553,15 +561,18 @@ __global__ __launch_bounds__(256) void k_value_rows(ExpandArgs a) {
                     const uint32_t zid = W.keys[w];
                     const uint64_t g = zfactor(ZID ? (int32_t)(zlo + zid) : a.id2z[zid]);
                     S1 += g;
-                    S2 += (uint64_t)v * g;
-                    fcs += (uint64_t)v;
+                    if (AGG != kAggProject) {
+                        S2 += (uint64_t)v * g;
+                        fcs += (uint64_t)v;
+                    }
                 }
                 W.cnt[w] = 0;
                 W.acc[w] = ident;
                 W.keys[w] = kNone;
             }
+            // a project row (x, z) adds k2(x) g(z) to sum2
             fsum += xfactor1(xv) * S1;
-            fsum2 += xfactor2(xv) * S2;
+            fsum2 += xfactor2(xv) * (AGG == kAggProject ? S1 : S2);
             if (lane == 0) nrows += T;
         } else {
             uint64_t base = a.row_off[ti];
@@ -581,7 +592,9 @@ __global__ __launch_bounds__(256) void k_value_rows(ExpandArgs a) {
                     const uint32_t zid = W.keys[w];
                     const int32_t zv = ZID ? (int32_t)(zlo + zid) : a.id2z[zid];
                     const int64_t v = agg_get<AGG, F32>(W.acc[w], n);
-                    if (F32) {
+                    if (AGG == kAggProject) {
+                        reinterpret_cast<int2*>(a.out)[pos] = make_int2(xv, zv);
+                    } else if (F32) {
                         dim3_value_row_f32 row;
                         row.x = xv;
                         row.z = zv;
@@ -751,6 +764,15 @@ void value_tasks(const ExpandArgs& a, uint32_t nz, uint32_t limit, uint32_t* bou
 
 size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block) {
     return (size_t)tile_size * 12u + ((size_t)touched_cap + 4u * block + 1u) * 4u;
+}
+
+int expand_short_rows(const ExpandArgs& a, bool count, int cu_count, hipStream_t st) {
+    const bool zid = a.id2z == nullptr;
+    if (count)
+        return zid ? launch_wave_rows<kAggCount, true, false>(a, cu_count, st)
+                   : launch_wave_rows<kAggCount, false, false>(a, cu_count, st);
+    return zid ? launch_wave_rows<kAggProject, true, false>(a, cu_count, st)
+               : launch_wave_rows<kAggProject, false, false>(a, cu_count, st);
 }
 
 int expand_value(const ExpandArgs& a, int agg, bool f32, int block, int cu_count, hipStream_t st) {
