@@ -61,8 +61,9 @@ def query_alg_bytes(st, n_out, row_bytes=8):
 
 def measured_traffic(args):
     """HBM bytes per expansion launch measured by rocprofv3 FETCH_SIZE +
-    WRITE_SIZE (calibrated against a plain copy), recorded in
-    profiles/expand_two_hop_graph.traffic.json for the default workload."""
+    WRITE_SIZE (calibrated against a plain copy) of the expansion's
+    dispatches, recorded in profiles/expand_two_hop_graph.traffic.json for
+    the default workload."""
     default = (args.vertices, args.edges, args.max_degree, args.seed, args.mode, args.dense, args.block,
                args.tile_bits, args.hash_only) == (1_000_000, 4_000_000, 50_000, 7, "materialize", -1, 0, 0, False)
     path = os.path.join(ROOT, "profiles", "expand_two_hop_graph.traffic.json")
@@ -774,7 +775,8 @@ def main():
         "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                     "kernel": "k_expand", "kernel_ms": exp_ms, "alg_bytes": b_kernel},
+                     "kernel": "expansion: k_expand + k_value_rows (rows of <= 256 paths)", "kernel_ms": exp_ms,
+                     "alg_bytes": b_kernel},
         "cpu_baseline": cpu,
         "dense": dense,
     }

@@ -542,9 +542,9 @@ void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t*
                    const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
                    hipStream_t st);
 
-// bound[x] >= output rows of row x: its S paths (plus the product columns
-// of a product row), at most nz
-void row_bounds(const ExpandArgs& a, uint32_t nz, uint32_t* bound, hipStream_t st);
+// bound[x] = min(cap, the S paths of row x plus the product columns of a
+// product row) >= min(cap, output rows of row x); cap <= nz
+void row_bounds(const ExpandArgs& a, uint32_t cap, uint32_t* bound, hipStream_t st);
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 int expand_count(const ExpandArgs& a, int block, int cu_count, hipStream_t st);
 // value aggregates (k_value.hip): agg 1 SUM, 2 MIN, 3 MAX, 4 AVG of vR + vS

@@ -726,16 +726,17 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
 }
 
 // Upper bound of each row's output: its S paths over all tiles (a product
-// row also its product columns), capped at nz.
-__global__ __launch_bounds__(256) void k_row_bound(ExpandArgs a, uint32_t nz, uint32_t* __restrict__ bound) {
+// row also its product columns), capped at `cap` (<= nz; the loop stops
+// there, so a hub row costs no more than a short one).
+__global__ __launch_bounds__(256) void k_row_bound(ExpandArgs a, uint32_t cap, uint32_t* __restrict__ bound) {
     const uint32_t x = blockIdx.x * 256u + threadIdx.x;
     if (x >= a.nx) return;
     uint64_t b = (a.hx != nullptr && a.hx[x] >= 0) ? a.Nh : 0u;
-    for (uint32_t i = a.rowptrR[x], e = a.rowptrR[x + 1]; i < e && b < nz; i++) {
+    for (uint32_t i = a.rowptrR[x], e = a.rowptrR[x + 1]; i < e && b < cap; i++) {
         const uint32_t y = (uint32_t)a.colR[i] & 0x7FFFFFFFu;
         b += a.rowptrS[(y + 1u) * a.ntiles] - a.rowptrS[y * a.ntiles];
     }
-    bound[x] = (uint32_t)min<uint64_t>(b, nz);
+    bound[x] = (uint32_t)min<uint64_t>(b, cap);
 }
 
 // Zero the expansion-owned scalars of every instance of a batch (a kernel, so
@@ -805,8 +806,8 @@ size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stag
     return ((size_t)tile_words + touched_cap + area) * 4u + (fp ? (size_t)kFpTableU64 * 8u : 0u);
 }
 
-void row_bounds(const ExpandArgs& a, uint32_t nz, uint32_t* bound, hipStream_t st) {
-    if (a.nx) hipLaunchKernelGGL(k_row_bound, dim3((a.nx + 255u) / 256u), dim3(256), 0, st, a, nz, bound);
+void row_bounds(const ExpandArgs& a, uint32_t cap, uint32_t* bound, hipStream_t st) {
+    if (a.nx) hipLaunchKernelGGL(k_row_bound, dim3((a.nx + 255u) / 256u), dim3(256), 0, st, a, cap, bound);
 }
 
 int expand_project(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {

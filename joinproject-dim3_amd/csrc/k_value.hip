@@ -753,7 +753,8 @@ void rowptr_y(const uint32_t* rowptrS, uint32_t ny, uint32_t ntiles, uint32_t* r
 void value_tasks(const ExpandArgs& a, uint32_t nz, uint32_t limit, uint32_t* bound, uint32_t* fw, uint32_t* pw,
                  uint32_t* fh, uint32_t* ph, uint64_t* partials, uint64_t* n_wave, uint64_t* n_hash,
                  uint32_t* wave_x, uint32_t* hash_x, uint32_t* big_x, hipStream_t st) {
-    row_bounds(a, nz, bound, st);
+    // the classification needs the bounds up to the larger threshold only
+    row_bounds(a, std::min<uint32_t>(nz, std::max<uint32_t>(limit, kWaveRow) + 1u), bound, st);
     const unsigned g = (a.nx + 255u) / 256u;
     if (a.nx) hipLaunchKernelGGL(k_value_split, dim3(g), dim3(256), 0, st, bound, a.nx, limit, fw, fh);
     scan_u32(fw, a.nx, pw, partials, n_wave, st);
