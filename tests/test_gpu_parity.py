@@ -254,3 +254,34 @@ def test_repeated_queries_reuse_workspace(ctx):
     for _ in range(3):
         assert ctx.join_project(R, S, output_mode=1).fp.as_tuple() == a
     assert ctx.device_bytes() == held
+
+
+def test_low_outj_and_short_rows(ctx, dim3):
+    """Join sizes near n (the reference's f1 picks its classical hash join
+    here, dim3.hpp:41-50, 273-391) and rows of a few paths each, which go
+    one per wavefront: rows and counts equal the oracle's, in materialised
+    and fingerprint modes, with and without the u != w filter."""
+    from dim3_amd import workloads
+
+    R, _, S, _ = workloads.gen_rand_tuples(1_000_000, 800_000, 100_000, 1_000_000, 100_000, 11)
+    want, wfp, sw = oracle.join_project(R, S)
+    g = ctx.join_project(R, S)
+    assert g.swapped == sw and np.array_equal(oracle.sort_pairs(g.pairs), oracle.sort_pairs(want))
+    assert ctx.join_project(R, S, output_mode=1).fp.as_tuple() == wfp.as_tuple()
+    wp, wc, cfp, _ = oracle.join_aggregate_count(R, S)
+    a = ctx.join_aggregate_count(R, S)
+    gp, gc = oracle.sort_pairs(a.xz, a.counts)
+    ep, ec = oracle.sort_pairs(wp, wc)
+    assert np.array_equal(gp, ep) and np.array_equal(gc, ec)
+    assert ctx.join_aggregate_count(R, S, output_mode=1).fp.as_tuple() == cfp.as_tuple()
+    # a self-join of short rows with the filter
+    E = np.concatenate([R[:200_000], R[:200_000, ::-1]]).astype(np.int32)
+    for count in (False, True):
+        if count:
+            wp, wc, cfp, _ = oracle.join_aggregate_count(E, E, exclude_equal=True)
+            f = ctx.join_aggregate_count(E, E, exclude_equal=True, output_mode=1)
+            assert f.fp.as_tuple() == cfp.as_tuple()
+        else:
+            want, wfp, _ = oracle.join_project(E, E, exclude_equal=True)
+            g = ctx.join_project(E, E, exclude_equal=True)
+            assert np.array_equal(oracle.sort_pairs(g.pairs), oracle.sort_pairs(want))
