@@ -656,8 +656,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: the join-project of one query is sharded by output row "
-              "across GPUs only in design (see DESIGN.md); pass --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: sharding a query by output row across GPUs is not built "
+              "(a query runs on one device); pass --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
