@@ -1029,9 +1029,9 @@ int value_task_lists(dim3_ctx* c, dim3h::ExpandArgs& a, hipStream_t st, bool has
     uint64_t* part;
     TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed((uint64_t)P.nx + 1), &part));
     Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
-    dim3h::value_tasks(a, P.nz, limit, bound, fw, pw, fh, ph, part, &sc->n_wave, &sc->n_hash, wx, hx, bx, st);
-    // untiled rows read S by y alone: a compact row index (the tiled one is
-    // ntiles times larger and its y boundaries lie ntiles words apart)
+    // untiled rows (and the row bounds) read S by y alone: a compact row
+    // index (the tiled one is ntiles times larger and its y boundaries lie
+    // ntiles words apart)
     if (P.ntiles > 1) {
         uint32_t* ry;
         TRY(ws(c, WS_ROWPTR_Y, (uint64_t)P.ny + 1, &ry));
@@ -1040,6 +1040,7 @@ int value_task_lists(dim3_ctx* c, dim3h::ExpandArgs& a, hipStream_t st, bool has
     } else {
         a.rowptrY = a.rowptrS;
     }
+    dim3h::value_tasks(a, P.nz, limit, bound, fw, pw, fh, ph, part, &sc->n_wave, &sc->n_hash, wx, hx, bx, st);
     uint32_t* rn;
     uint64_t* ro;
     TRY(ws(c, WS_VT_ROWN, P.nx, &rn));

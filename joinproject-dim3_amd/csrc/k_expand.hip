@@ -734,7 +734,8 @@ __global__ __launch_bounds__(256) void k_row_bound(ExpandArgs a, uint32_t cap, u
     uint64_t b = (a.hx != nullptr && a.hx[x] >= 0) ? a.Nh : 0u;
     for (uint32_t i = a.rowptrR[x], e = a.rowptrR[x + 1]; i < e && b < cap; i++) {
         const uint32_t y = (uint32_t)a.colR[i] & 0x7FFFFFFFu;
-        b += a.rowptrS[(y + 1u) * a.ntiles] - a.rowptrS[y * a.ntiles];
+        b += a.rowptrY != nullptr ? a.rowptrY[y + 1u] - a.rowptrY[y]
+                                  : a.rowptrS[(y + 1u) * a.ntiles] - a.rowptrS[y * a.ntiles];
     }
     bound[x] = (uint32_t)min<uint64_t>(b, cap);
 }
