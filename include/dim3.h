@@ -328,10 +328,11 @@ typedef struct dim3_server_opts {
 
 #define DIM3_REQ_PROJECT 0
 #define DIM3_REQ_COUNT 1
+#define DIM3_REQ_VALUE 2 /* value aggregate `agg` over the relations' values (dim3_server_put_values) */
 
 typedef struct dim3_request {
     uint32_t kind;     /* DIM3_REQ_* */
-    uint32_t reserved;
+    uint32_t agg;      /* DIM3_REQ_VALUE: DIM3_AGG_*; otherwise 0 */
     uint64_t r, s;     /* relation handles: R (x, y) and S (z, y) */
     dim3_opts opts;
 } dim3_request;
@@ -350,6 +351,11 @@ int dim3_server_destroy(dim3_server* srv);
 /* Copies n (x|z, y) pairs (host or device memory) into the server. */
 int dim3_server_put_relation(dim3_server* srv, const int32_t* pairs, uint64_t n, uint64_t* handle);
 int dim3_server_put_relation_dev(dim3_server* srv, const int32_t* d_pairs, uint64_t n, uint64_t* handle);
+/* Attaches one value per tuple (host or device memory; DIM3_VALUE_I32 or
+ * DIM3_VALUE_F32) to a relation, once, before any request uses it; value
+ * requests (DIM3_REQ_VALUE, not batched) need values of one type on both
+ * relations; their rows are dim3_count_row (int) or dim3_value_row_f32. */
+int dim3_server_put_values(dim3_server* srv, uint64_t handle, const void* values, uint32_t value_type);
 /* Frees a relation once no pending request uses it. */
 int dim3_server_drop_relation(dim3_server* srv, uint64_t handle);
 int dim3_submit(dim3_server* srv, const dim3_request* req, uint64_t* ticket);

@@ -84,7 +84,7 @@ class ServerOpts(ctypes.Structure):
 
 
 class Request(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("r", ctypes.c_uint64),
+    _fields_ = [("kind", ctypes.c_uint32), ("agg", ctypes.c_uint32), ("r", ctypes.c_uint64),
                 ("s", ctypes.c_uint64), ("opts", Opts)]
 
 
@@ -117,7 +117,7 @@ EXPORTS = [
     "dim3_poll", "dim3_wait", "dim3_server_get_stats", "dim3_join_project_stream",
     "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
     "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev", "dim3_estimate_join_cardinality",
-    "dim3_join_aggregate_tuples",
+    "dim3_join_aggregate_tuples", "dim3_server_put_values",
 ]
 
 _lib = None
@@ -215,6 +215,8 @@ def lib():
     for f in (L.dim3_server_put_relation, L.dim3_server_put_relation_dev):
         f.argtypes = [vp, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
         f.restype = i32
+    L.dim3_server_put_values.argtypes = [vp, u64, vp, ctypes.c_uint32]
+    L.dim3_server_put_values.restype = i32
     L.dim3_server_drop_relation.argtypes = [vp, u64]
     L.dim3_server_drop_relation.restype = i32
     L.dim3_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
@@ -682,15 +684,27 @@ class Server:
             raise Dim3Error(rc, "dim3_server_put_relation_dev")
         return int(h.value)
 
+    def put_values(self, handle, values):
+        """Attach one int32 or float32 value per tuple to a relation (once,
+        before any request uses it)."""
+        v = np.ascontiguousarray(values).reshape(-1)
+        vtype = {np.dtype("<i4"): 0, np.dtype("<f4"): 2}[v.dtype]
+        rc = self._L.dim3_server_put_values(self._h, handle, _ptr(v), vtype)
+        if rc != OK:
+            raise Dim3Error(rc, "dim3_server_put_values")
+
     def drop(self, handle):
         rc = self._L.dim3_server_drop_relation(self._h, handle)
         if rc != OK:
             raise Dim3Error(rc, "dim3_server_drop_relation")
 
-    def submit(self, r, s, count=False, **kw):
-        """Returns (status, ticket); status OK, ERR_BUSY or ERR_ADMISSION."""
+    def submit(self, r, s, count=False, agg=None, **kw):
+        """Returns (status, ticket); status OK, ERR_BUSY or ERR_ADMISSION.
+        agg ("sum", "min", "max", "avg"): a value aggregate over the
+        relations' values (put_values)."""
         req = Request()
-        req.kind = 1 if count else 0
+        req.kind = 2 if agg is not None else 1 if count else 0
+        req.agg = _agg(agg) if agg is not None else 0
         req.r, req.s = r, s
         req.opts = make_opts(**kw)
         t = ctypes.c_uint64()
@@ -719,6 +733,8 @@ class Server:
             pairs = np.frombuffer((ctypes.c_int32 * (2 * n)).from_address(res.pairs), dtype=np.int32).reshape(n, 2).copy()
         if rc == OK and res.rows:
             rows = np.frombuffer((CountRow * n).from_address(res.rows), dtype=COUNT_ROW_DTYPE).copy()
+        if rc == OK and res.rows_f32:
+            rows = np.frombuffer((ctypes.c_char * (12 * n)).from_address(res.rows_f32), dtype=F32_ROW_DTYPE).copy()
         out = QueryResult(res, pairs=pairs, rows=rows)
         self._L.dim3_result_free(ctypes.byref(res))
         return rc, out, q.value, t.value

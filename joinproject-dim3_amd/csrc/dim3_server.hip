@@ -38,9 +38,17 @@ struct Rel {
     int32_t* d = nullptr;
     uint64_t n = 0;
     uint64_t bytes = 0;
+    int32_t* dv = nullptr;  // tuple values (value aggregates), DIM3_VALUE_I32 or DIM3_VALUE_F32
+    uint32_t vtype = 0;
+    uint64_t vbytes = 0;
     uint32_t users = 0;
     bool dropped = false;
 };
+
+void free_rel(Rel& r) {
+    dim3h::dev_free(r.d);
+    if (r.dv) dim3h::dev_free(r.dv);
+}
 
 struct Job {
     dim3_request req{};
@@ -58,7 +66,8 @@ struct Job {
 uint64_t footprint(uint64_t nr, uint64_t ns) { return 96ull * (nr + ns) + (16ull << 20); }
 
 bool same_batch_class(const dim3_request& a, const dim3_request& b) {
-    return a.kind == b.kind && a.opts.output_mode == DIM3_OUT_FINGERPRINT &&
+    // value aggregates run one request at a time
+    return a.kind == b.kind && a.kind != DIM3_REQ_VALUE && a.opts.output_mode == DIM3_OUT_FINGERPRINT &&
            b.opts.output_mode == DIM3_OUT_FINGERPRINT && a.opts.flags == b.opts.flags &&
            a.opts.dense_mode == b.opts.dense_mode && a.opts.heavy_k == b.opts.heavy_k &&
            a.opts.tile_bits == b.opts.tile_bits && a.opts.block == b.opts.block;
@@ -158,14 +167,23 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
     Job* j = js[0];
     const dim3_opts& o = j->req.opts;
     const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
-    const size_t row = count ? sizeof(dim3_count_row) : 2 * sizeof(int32_t);
+    const bool value = j->req.kind == DIM3_REQ_VALUE;
+    const bool f32 = value && R[0]->vtype == DIM3_VALUE_F32;
+    const size_t row = f32 ? sizeof(dim3_value_row_f32)
+                       : (count || value) ? sizeof(dim3_count_row) : 2 * sizeof(int32_t);
     uint64_t cap = fp ? 0 : *out_bytes / row;
     dim3_result res;
     for (int attempt = 0; attempt < 2; attempt++) {
-        int rc = count ? dim3_join_aggregate_count_dev(c, R[0]->d, R[0]->n, S[0]->d, S[0]->n, &o,
-                                                       (dim3_count_row*)*out_buf, cap, st, &res)
-                       : dim3_join_project_dev(c, R[0]->d, R[0]->n, S[0]->d, S[0]->n, &o, (int32_t*)*out_buf, cap,
-                                               st, &res);
+        const int agg = (int)j->req.agg;
+        int rc = f32     ? dim3_join_aggregate_f32_dev(c, R[0]->d, (const float*)R[0]->dv, R[0]->n, S[0]->d,
+                                                       (const float*)S[0]->dv, S[0]->n, agg, &o,
+                                                       (dim3_value_row_f32*)*out_buf, cap, st, &res)
+                 : value ? dim3_join_aggregate_dev(c, R[0]->d, R[0]->dv, R[0]->n, S[0]->d, S[0]->dv, S[0]->n, agg, &o,
+                                                   (dim3_count_row*)*out_buf, cap, st, &res)
+                 : count ? dim3_join_aggregate_count_dev(c, R[0]->d, R[0]->n, S[0]->d, S[0]->n, &o,
+                                                         (dim3_count_row*)*out_buf, cap, st, &res)
+                         : dim3_join_project_dev(c, R[0]->d, R[0]->n, S[0]->d, S[0]->n, &o, (int32_t*)*out_buf, cap,
+                                                 st, &res);
         if (rc == DIM3_ERR_CAPACITY && attempt == 0) {  // grow the output buffer to the size reported
             if (*out_buf) dim3h::dev_free(*out_buf);
             *out_buf = nullptr;
@@ -192,7 +210,8 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
             j->res.n = 0;
             return;
         }
-        if (count) j->res.rows = (dim3_count_row*)h;
+        if (f32) j->res.rows_f32 = (dim3_value_row_f32*)h;
+        else if (count || value) j->res.rows = (dim3_count_row*)h;
         else j->res.pairs = (int32_t*)h;
     }
 }
@@ -320,8 +339,8 @@ void dim3_server::worker(uint32_t w) {
                 for (uint64_t hdl : {j->req.r, j->req.s}) {
                     Rel& r = rels.at(hdl);
                     if (--r.users == 0 && r.dropped) {
-                        dim3h::dev_free(r.d);
-                        resident -= r.bytes;
+                        free_rel(r);
+                        resident -= r.bytes + r.vbytes;
                         rels.erase(hdl);
                     }
                 }
@@ -422,7 +441,7 @@ int dim3_server_destroy(dim3_server* s) {
     s->cv_work.notify_all();
     for (auto& t : s->threads) t.join();
     hipSetDevice(s->device);
-    for (auto& kv : s->rels) dim3h::dev_free(kv.second.d);
+    for (auto& kv : s->rels) free_rel(kv.second);
     for (auto& kv : s->jobs) dim3_result_free(&kv.second.res);
     for (auto* c : s->ctxs) dim3_ctx_destroy(c);
     for (auto x : s->full_stream) hipStreamDestroy(x);
@@ -473,6 +492,43 @@ int dim3_server_put_relation_dev(dim3_server* s, const int32_t* d_pairs, uint64_
     return put_relation(s, d_pairs, n, handle, hipMemcpyDeviceToDevice);
 }
 
+int dim3_server_put_values(dim3_server* s, uint64_t handle, const void* values, uint32_t value_type) {
+    if (!s || (value_type != DIM3_VALUE_I32 && value_type != DIM3_VALUE_F32)) return DIM3_ERR_INVALID;
+    uint64_t n, bytes;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->rels.find(handle);
+        if (it == s->rels.end() || it->second.dropped) return DIM3_ERR_NOT_FOUND;
+        if (it->second.dv || it->second.users) return DIM3_ERR_INVALID;  // once, before any request uses it
+        n = it->second.n;
+        bytes = std::max<uint64_t>(n, 1) * 4;
+        if (n && !values) return DIM3_ERR_INVALID;
+        if (s->resident + bytes > s->budget) return DIM3_ERR_ADMISSION;
+        s->resident += bytes;
+    }
+    hipSetDevice(s->device);
+    int32_t* d = nullptr;
+    int rc = DIM3_OK;
+    if (dim3h::dev_malloc((void**)&d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = DIM3_ERR_OOM;
+    } else if (n && hipMemcpy(d, values, n * 4, hipMemcpyDefault) != hipSuccess) {
+        dim3h::dev_free(d);
+        rc = DIM3_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rc != DIM3_OK) {
+        s->resident -= bytes;
+        return rc;
+    }
+    Rel& r = s->rels.at(handle);
+    r.dv = d;
+    r.vtype = value_type;
+    r.vbytes = bytes;
+    s->note_peak();
+    return DIM3_OK;
+}
+
 int dim3_server_drop_relation(dim3_server* s, uint64_t handle) {
     if (!s) return DIM3_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -483,8 +539,8 @@ int dim3_server_drop_relation(dim3_server* s, uint64_t handle) {
         return DIM3_OK;
     }
     hipSetDevice(s->device);
-    dim3h::dev_free(it->second.d);
-    s->resident -= it->second.bytes;
+    free_rel(it->second);
+    s->resident -= it->second.bytes + it->second.vbytes;
     s->rels.erase(it);
     return DIM3_OK;
 }
@@ -492,12 +548,17 @@ int dim3_server_drop_relation(dim3_server* s, uint64_t handle) {
 int dim3_submit(dim3_server* s, const dim3_request* req, uint64_t* ticket) {
     if (!s || !req || !ticket) return DIM3_ERR_INVALID;
     *ticket = 0;
-    if (req->kind > DIM3_REQ_COUNT) return DIM3_ERR_INVALID;
+    if (req->kind > DIM3_REQ_VALUE) return DIM3_ERR_INVALID;
     if (req->kind == DIM3_REQ_COUNT && req->opts.dense_mode == DIM3_DENSE_BITOR) return DIM3_ERR_UNSUPPORTED;
+    if (req->kind == DIM3_REQ_VALUE && (req->agg < DIM3_AGG_SUM || req->agg > DIM3_AGG_AVG)) return DIM3_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
     auto ir = s->rels.find(req->r), is = s->rels.find(req->s);
     if (ir == s->rels.end() || is == s->rels.end() || ir->second.dropped || is->second.dropped)
         return DIM3_ERR_NOT_FOUND;
+    // a value aggregate needs values of one type on both relations
+    if (req->kind == DIM3_REQ_VALUE &&
+        (!ir->second.dv || !is->second.dv || ir->second.vtype != is->second.vtype))
+        return DIM3_ERR_INVALID;
     if (s->pending.size() >= s->queue_cap) return DIM3_ERR_BUSY;
     const uint64_t est = footprint(ir->second.n, is->second.n);
     s->st.submitted++;

@@ -100,3 +100,49 @@ def test_large_lane_partition(ctx, dim3):
             alone = ctx.join_aggregate_count(R, S, output_mode=1) if count else ctx.join_project(R, S, output_mode=1)
             assert res.fp.as_tuple() == alone.fp.as_tuple(), i
         assert srv.stats()["completed"] == len(reqs)
+
+
+def test_server_value_requests(dim3):
+    """Value aggregates through the server (int and float values, materialised
+    and fingerprint) equal the oracle's; mixed with project requests."""
+    import oracle
+
+    rng = np.random.default_rng(77)
+    R = np.stack([rng.integers(0, 2000, 30_000), rng.integers(0, 500, 30_000)], 1).astype(np.int32)
+    S = np.stack([rng.integers(0, 2000, 20_000), rng.integers(0, 500, 20_000)], 1).astype(np.int32)
+    vi_r, vi_s = rng.integers(-10 ** 6, 10 ** 6, 30_000).astype(np.int32), rng.integers(-10 ** 6, 10 ** 6, 20_000).astype(np.int32)
+    vf_r, vf_s = (rng.integers(-4000, 4000, 30_000) / 4).astype(np.float32), (rng.integers(-4000, 4000, 20_000) / 4).astype(np.float32)
+    with dim3.Server(0, workers=3) as srv:
+        ri, si, rf, sf = srv.put(R), srv.put(S), srv.put(R), srv.put(S)
+        srv.put_values(ri, vi_r)
+        srv.put_values(si, vi_s)
+        srv.put_values(rf, vf_r)
+        srv.put_values(sf, vf_s)
+        with pytest.raises(dim3.Dim3Error):
+            srv.put_values(ri, vi_r)  # once per relation
+        tickets = []
+        for agg in ("sum", "min", "max", "avg"):
+            tickets.append(("i", agg, 0, srv.submit(ri, si, agg=agg)[1]))
+            tickets.append(("f", agg, 0, srv.submit(rf, sf, agg=agg)[1]))
+            tickets.append(("f", agg, 1, srv.submit(rf, sf, agg=agg, output_mode=1)[1]))
+        tp = srv.submit(ri, si)[1]
+        # values of two different types: refused
+        with pytest.raises(dim3.Dim3Error):
+            srv.submit(ri, sf, agg="sum")
+        for kind, agg, fpm, t in tickets:
+            rc, res, _, _ = srv.wait(t)
+            assert rc == dim3.OK, (kind, agg)
+            if kind == "i":
+                wp, wv, wfp, _ = oracle.join_aggregate(R, vi_r, S, vi_s, agg)
+                got, want = oracle.sort_pairs(res.xz, res.counts), oracle.sort_pairs(wp, wv)
+            else:
+                wp, wv, wfp, _ = oracle.join_aggregate_f32(R, vf_r, S, vf_s, agg)
+                if fpm:
+                    assert res.fp.as_tuple() == wfp.as_tuple(), agg
+                    continue
+                got = oracle.sort_pairs(res.xz, res.counts.view(np.int32).astype(np.int64))
+                want = oracle.sort_pairs(wp, wv.view(np.int32).astype(np.int64))
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), (kind, agg)
+        rc, res, _, _ = srv.wait(tp)
+        want, _, _ = oracle.join_project(R, S)
+        assert rc == dim3.OK and np.array_equal(oracle.sort_pairs(res.pairs), oracle.sort_pairs(want))
