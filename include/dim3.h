@@ -254,6 +254,16 @@ int dim3_join_aggregate_count_stream(dim3_ctx* ctx, const int32_t* R_xy, uint64_
                                      uint64_t n_s, const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb,
                                      void* user, dim3_result* res);
 
+/* Streamed value aggregates (as dim3_join_aggregate / _f32, rows in chunks
+ * as above: n dim3_count_row, or n dim3_value_row_f32 for float values). */
+int dim3_join_aggregate_stream(dim3_ctx* ctx, const int32_t* R_xy, const int32_t* R_v, uint64_t n_r,
+                               const int32_t* S_zy, const int32_t* S_v, uint64_t n_s, int agg, const dim3_opts* opts,
+                               uint64_t chunk_rows, dim3_rows_cb cb, void* user, dim3_result* res);
+int dim3_join_aggregate_f32_stream(dim3_ctx* ctx, const int32_t* R_xy, const float* R_v, uint64_t n_r,
+                                   const int32_t* S_zy, const float* S_v, uint64_t n_s, int agg,
+                                   const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb, void* user,
+                                   dim3_result* res);
+
 /* ---- batches: many independent instances in one persistent launch ----
  * No counterpart in the reference (it runs one query per call); the batch
  * returns, per instance, exactly what dim3_join_project_dev /

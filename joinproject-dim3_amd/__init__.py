@@ -117,7 +117,8 @@ EXPORTS = [
     "dim3_poll", "dim3_wait", "dim3_server_get_stats", "dim3_join_project_stream",
     "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
     "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev", "dim3_estimate_join_cardinality",
-    "dim3_join_aggregate_tuples", "dim3_server_put_values",
+    "dim3_join_aggregate_tuples", "dim3_server_put_values", "dim3_join_aggregate_stream",
+    "dim3_join_aggregate_f32_stream",
 ]
 
 _lib = None
@@ -186,6 +187,11 @@ def lib():
     L.dim3_join_aggregate_tuples.argtypes = [vp, vp, u64, vp, u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                              i32, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
     L.dim3_join_aggregate_tuples.restype = i32
+    for name in ("dim3_join_aggregate_stream", "dim3_join_aggregate_f32_stream"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp, vp, u64, vp, vp, u64, i32, ctypes.POINTER(Opts), u64, ROWS_CB, vp,
+                      ctypes.POINTER(Result)]
+        f.restype = i32
     L.dim3_result_free.argtypes = [ctypes.POINTER(Result)]
     L.dim3_result_free.restype = None
     L.dim3_estimate_join_cardinality.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_int64)]
@@ -517,6 +523,35 @@ class Context:
             raise err[0]
         if rc != ERR_CANCELLED:
             self._check(rc, name)
+        return rc, QueryResult(res)
+
+    def join_aggregate_stream(self, R, vR, S, vS, agg, on_rows, chunk_rows=0, f32=False, **kw):
+        """Streamed value aggregate: on_rows(rows) gets each chunk as a
+        COUNT_ROW_DTYPE view (int32 values) or an F32_ROW_DTYPE view (f32)."""
+        opts = make_opts(**kw)
+        R, S = _pairs(R), _pairs(S)
+        vR = _fvalues(vR, R.shape[0]) if f32 else _values(vR, R.shape[0])
+        vS = _fvalues(vS, S.shape[0]) if f32 else _values(vS, S.shape[0])
+        res = Result()
+        err = []
+        dtype, width = (F32_ROW_DTYPE, 12) if f32 else (COUNT_ROW_DTYPE, 16)
+
+        def cb(_user, rows, n):
+            try:
+                buf = (ctypes.c_char * (width * n)).from_address(rows)
+                return 1 if on_rows(np.frombuffer(buf, dtype=dtype)) is False else 0
+            except BaseException as e:  # stop the query, re-raised below
+                err.append(e)
+                return 1
+
+        c_cb = ROWS_CB(cb)
+        fn = self._L.dim3_join_aggregate_f32_stream if f32 else self._L.dim3_join_aggregate_stream
+        rc = fn(self._h, _ptr(R), _ptr(vR), R.shape[0], _ptr(S), _ptr(vS), S.shape[0], _agg(agg), ctypes.byref(opts),
+                chunk_rows, c_cb, None, ctypes.byref(res))
+        if err:
+            raise err[0]
+        if rc != ERR_CANCELLED:
+            self._check(rc, "dim3_join_aggregate_stream")
         return rc, QueryResult(res)
 
     def join_project_stream(self, R, S, on_rows, chunk_rows=0, **kw):

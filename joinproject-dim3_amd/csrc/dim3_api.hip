@@ -971,6 +971,8 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.ntiles = P.ntiles;
     a.task_lo = 0;
     a.task_hi = P.nx * P.ntiles;
+    a.row_lo = 0;
+    a.row_hi = P.nx;
     a.tile_size = P.tile_size;
     a.tile_words = P.tile_words;
     a.touched_cap = P.touched_cap;
@@ -1423,16 +1425,19 @@ int run_tuples(dim3_ctx* c, int agg, const void* R, uint64_t nR, const void* S, 
 // max(chunk_rows, largest row bound) rows; each chunk is copied to a pinned
 // host buffer and handed to the callback before the next one runs.
 int run_stream(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const int32_t* S, uint64_t nS,
-               const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb, void* user, dim3_result* res) {
+               const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb, void* user, dim3_result* res, int agg = 0,
+               bool f32 = false, const int32_t* vR = nullptr, const int32_t* vS = nullptr) {
     if (!c || !res) return DIM3_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     c->err.clear();
     memset(res, 0, sizeof(*res));
     if (!cb) return fail(c, DIM3_ERR_INVALID, "null row callback");
     if ((nR && !R) || (nS && !S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    if (agg) TRY(check_value_query(c, agg, nR, vR, nS, vS));
     dim3_opts o;
     TRY(check_opts(c, opts, &o));
     o.output_mode = DIM3_OUT_MATERIALIZE;
+    if (agg) o.dense_mode = DIM3_DENSE_OFF;
     if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
         return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1443,7 +1448,19 @@ int run_stream(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, cons
     TRY(ws(c, WS_IN_S, std::max<uint64_t>(nS, 1), &dS));
     if (nR) HIPCHK(c, hipMemcpyAsync(dR, R, nR * 8, hipMemcpyHostToDevice, st));
     if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
-    TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
+    ValueSpec vals;
+    if (agg) {
+        int32_t *dvR, *dvS;
+        TRY(ws(c, WS_IN_VR, std::max<uint64_t>(nR, 1), &dvR));
+        TRY(ws(c, WS_IN_VS, std::max<uint64_t>(nS, 1), &dvS));
+        if (nR) HIPCHK(c, hipMemcpyAsync(dvR, vR, nR * 4, hipMemcpyHostToDevice, st));
+        if (nS) HIPCHK(c, hipMemcpyAsync(dvS, vS, nS * 4, hipMemcpyHostToDevice, st));
+        vals.agg = agg;
+        vals.f32 = f32;
+        vals.vR = dvR;
+        vals.vS = dvS;
+    }
+    TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st, 0, vals));
     {
         // a multiplicity above 127 in the int8 product: plan again all-sparse
         Scalars* sc = (Scalars*)c->bank().ws[WS_SCALARS].p;
@@ -1451,11 +1468,11 @@ int run_stream(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, cons
         HIPCHK(c, hipStreamSynchronize(st));
         if (dense_overflowed(c, *c->h_sc)) {
             o.dense_mode = DIM3_DENSE_OFF;
-            TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
+            TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st, 0, vals));
         }
     }
     const auto& P = c->bank().plan;
-    const size_t row_bytes = P.count_mode ? sizeof(dim3_count_row) : 8;
+    const size_t row_bytes = P.agg && P.vf32 ? sizeof(dim3_value_row_f32) : P.count_mode ? sizeof(dim3_count_row) : 8;
     uint64_t total = 0, paths = 0;
     double ms_expand = 0;
     if (!P.empty) {
@@ -1484,6 +1501,12 @@ int run_stream(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, cons
             a = make_args(c, false, dout, cap);
             a.task_lo = x0 * P.ntiles;
             a.task_hi = x1 * P.ntiles;
+            a.row_lo = x0;
+            a.row_hi = x1;
+            // the chunk's rows by size, as a whole query's (value aggregates;
+            // the short rows of an all-sparse plan)
+            if (P.agg) TRY(value_task_lists(c, a, st, true));
+            else if (P.dense == DIM3_DENSE_OFF && !P.core) TRY(value_task_lists(c, a, st, false));
             TRY(expand_args(c, a, st));
             const uint64_t n = c->h_sc->out_n;
             if (n > cap)
@@ -1957,6 +1980,21 @@ int dim3_join_aggregate_count_stream(dim3_ctx* c, const int32_t* R_xy, uint64_t 
 int dim3_join_aggregate(dim3_ctx* c, const int32_t* R_xy, const int32_t* R_v, uint64_t n_r, const int32_t* S_zy,
                         const int32_t* S_v, uint64_t n_s, int agg, const dim3_opts* opts, dim3_result* res) {
     return run_value_host(c, agg, false, R_xy, R_v, n_r, S_zy, S_v, n_s, opts, res);
+}
+
+int dim3_join_aggregate_stream(dim3_ctx* c, const int32_t* R_xy, const int32_t* R_v, uint64_t n_r, const int32_t* S_zy,
+                               const int32_t* S_v, uint64_t n_s, int agg, const dim3_opts* opts, uint64_t chunk_rows,
+                               dim3_rows_cb cb, void* user, dim3_result* res) {
+    return run_stream(c, true, R_xy, n_r, S_zy, n_s, opts, chunk_rows, cb, user, res, agg ? agg : -1, false, R_v,
+                      S_v);
+}
+
+int dim3_join_aggregate_f32_stream(dim3_ctx* c, const int32_t* R_xy, const float* R_v, uint64_t n_r,
+                                   const int32_t* S_zy, const float* S_v, uint64_t n_s, int agg,
+                                   const dim3_opts* opts, uint64_t chunk_rows, dim3_rows_cb cb, void* user,
+                                   dim3_result* res) {
+    return run_stream(c, true, R_xy, n_r, S_zy, n_s, opts, chunk_rows, cb, user, res, agg ? agg : -1, true,
+                      (const int32_t*)R_v, (const int32_t*)S_v);
 }
 
 int dim3_join_aggregate_tuples(dim3_ctx* c, const void* R_tuples, uint64_t n_r, const void* S_tuples, uint64_t n_s,

@@ -494,6 +494,7 @@ struct ExpandArgs {
     const uint64_t* n_wave;
     const uint64_t* n_hash;
     uint32_t hash_bits;
+    uint32_t row_lo, row_hi;  // the lists cover rows [row_lo, row_hi) (a streamed chunk, or all)
     // materialised wave rows: group count and output offset per wave row,
     // and the scan's partials (nx entries each)
     uint32_t* row_n;

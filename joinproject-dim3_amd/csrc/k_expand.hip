@@ -646,7 +646,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     // the tasks are (big_x[r], t) for the other rows
     const bool lists = a.big_x != nullptr;
     const uint32_t lo = lists ? 0u : a.task_lo;
-    const uint32_t hi = lists ? (a.nx - (uint32_t)*a.n_wave) * a.ntiles : a.task_hi;
+    const uint32_t hi = lists ? ((a.row_hi - a.row_lo) - (uint32_t)*a.n_wave) * a.ntiles : a.task_hi;
 
     for (;;) {
         if (tid == 0) {

@@ -386,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_value(ExpandArgs a) {
     const bool lists = a.hash_x != nullptr;
     const uint32_t nh = lists ? (uint32_t)*a.n_hash : 0u;
     const uint32_t nw = lists ? (uint32_t)*a.n_wave : 0u;  // rows of k_value_rows
-    const uint32_t total = nh + (a.nx - nh - nw) * a.ntiles;
+    const uint32_t total = lists ? nh + ((a.row_hi - a.row_lo) - nh - nw) * a.ntiles : a.nx * a.ntiles;
     bool hash = nh > 0;
     VLds L = value_layout<BLOCK, AGG>(a, smem, hash, sh32, s_wtot, &s_nt, &s_gbase);
     uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
@@ -643,24 +643,27 @@ __global__ void k_rowptr_y(const uint32_t* __restrict__ rowptrS, uint32_t ny, ui
 
 // Task lists: rows with at most kWaveRow paths are wave rows, rows with at
 // most `limit` hash rows, the others tiled.
-__global__ void k_value_split(const uint32_t* __restrict__ bound, uint32_t nx, uint32_t limit,
-                              uint32_t* __restrict__ fw, uint32_t* __restrict__ fh) {
+__global__ void k_value_split(const uint32_t* __restrict__ bound, uint32_t nx, uint32_t lo, uint32_t hi,
+                              uint32_t limit, uint32_t* __restrict__ fw, uint32_t* __restrict__ fh) {
     const uint32_t x = blockIdx.x * 256u + threadIdx.x;
     if (x >= nx) return;
+    const bool in = x >= lo && x < hi;
     const uint32_t b = bound[x];
-    fw[x] = b <= kWaveRow ? 1u : 0u;
-    fh[x] = (b > kWaveRow && b <= limit) ? 1u : 0u;
+    fw[x] = in && b <= kWaveRow ? 1u : 0u;
+    fh[x] = in && b > kWaveRow && b <= limit ? 1u : 0u;
 }
 
+// rows outside [lo, hi) are in no list; the tiled rows of the range fill
+// big_x in order (x - lo rows of the range precede x)
 __global__ void k_value_lists(const uint32_t* __restrict__ fw, const uint32_t* __restrict__ pw,
-                              const uint32_t* __restrict__ fh, const uint32_t* __restrict__ ph, uint32_t nx,
-                              uint32_t* __restrict__ wave_x, uint32_t* __restrict__ hash_x,
+                              const uint32_t* __restrict__ fh, const uint32_t* __restrict__ ph, uint32_t lo,
+                              uint32_t hi, uint32_t* __restrict__ wave_x, uint32_t* __restrict__ hash_x,
                               uint32_t* __restrict__ big_x) {
-    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
-    if (x >= nx) return;
+    const uint32_t x = lo + blockIdx.x * 256u + threadIdx.x;
+    if (x >= hi) return;
     if (fw[x]) wave_x[pw[x]] = x;
     else if (fh[x]) hash_x[ph[x]] = x;
-    else big_x[x - pw[x] - ph[x]] = x;
+    else big_x[(x - lo) - pw[x] - ph[x]] = x;
 }
 
 template <int AGG, bool ZID, bool F32, int PASS>
@@ -756,11 +759,13 @@ void value_tasks(const ExpandArgs& a, uint32_t nz, uint32_t limit, uint32_t* bou
     // the classification needs the bounds up to the larger threshold only
     row_bounds(a, std::min<uint32_t>(nz, std::max<uint32_t>(limit, kWaveRow) + 1u), bound, st);
     const unsigned g = (a.nx + 255u) / 256u;
-    if (a.nx) hipLaunchKernelGGL(k_value_split, dim3(g), dim3(256), 0, st, bound, a.nx, limit, fw, fh);
+    if (a.nx)
+        hipLaunchKernelGGL(k_value_split, dim3(g), dim3(256), 0, st, bound, a.nx, a.row_lo, a.row_hi, limit, fw, fh);
     scan_u32(fw, a.nx, pw, partials, n_wave, st);
     scan_u32(fh, a.nx, ph, partials, n_hash, st);
-    if (a.nx)
-        hipLaunchKernelGGL(k_value_lists, dim3(g), dim3(256), 0, st, fw, pw, fh, ph, a.nx, wave_x, hash_x, big_x);
+    if (a.row_hi > a.row_lo)
+        hipLaunchKernelGGL(k_value_lists, dim3((a.row_hi - a.row_lo + 255u) / 256u), dim3(256), 0, st, fw, pw, fh, ph,
+                           a.row_lo, a.row_hi, wave_x, hash_x, big_x);
 }
 
 size_t value_lds_bytes(uint32_t tile_size, uint32_t touched_cap, uint32_t block) {

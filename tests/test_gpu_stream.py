@@ -97,3 +97,35 @@ def test_stream_cancel_and_empty(ctx, dim3):
     assert rc == dim3.OK and res.n == 0 and not calls
     rc, res = ctx.join_aggregate_count_stream(R, (S + [0, 10_000]).astype(np.int32), lambda rows: calls.append(1))
     assert rc == dim3.OK and res.n == 0 and not calls
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_stream_value_aggregates(ctx, dim3, f32):
+    """Value aggregates streamed in chunks equal the oracle's rows; chunks of
+    short rows (one per wavefront) and of tiled long rows."""
+    rng = np.random.default_rng(12)
+    spec = cases.MEDIUM["aggregate_scaled"]
+    R, S = cases.make_medium(spec)
+    R = R[R[:, 0] % 8 == 0]
+    if f32:
+        vR = (rng.integers(-4000, 4000, R.shape[0]) / 4).astype(np.float32)
+        vS = (rng.integers(-4000, 4000, S.shape[0]) / 4).astype(np.float32)
+    else:
+        vR = rng.integers(-10 ** 6, 10 ** 6, R.shape[0]).astype(np.int32)
+        vS = rng.integers(-10 ** 6, 10 ** 6, S.shape[0]).astype(np.int32)
+    for agg in ("sum", "avg"):
+        chunks = []
+        rc, res = ctx.join_aggregate_stream(R, vR, S, vS, agg, lambda rows: chunks.append(rows.copy()),
+                                            chunk_rows=40_000, f32=f32)
+        assert rc == dim3.OK and len(chunks) >= 3
+        rows = np.concatenate(chunks)
+        if f32:
+            wp, wv, _, _ = oracle.join_aggregate_f32(R, vR, S, vS, agg)
+            got = oracle.sort_pairs(np.stack([rows["x"], rows["z"]], 1), rows["v"].view(np.int32).astype(np.int64))
+            want = oracle.sort_pairs(wp, wv.view(np.int32).astype(np.int64))
+        else:
+            wp, wv, _, _ = oracle.join_aggregate(R, vR, S, vS, agg)
+            got = oracle.sort_pairs(np.stack([rows["x"], rows["z"]], 1), rows["count"])
+            want = oracle.sort_pairs(wp, wv)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), agg
+        assert res.n == rows.shape[0]
