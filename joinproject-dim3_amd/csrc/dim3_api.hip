@@ -345,6 +345,7 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
             tw = (nz_words + nt - 1) / nt;
         }
         if (tw == 0) tw = 1;
+        if (!cnt && ((tw + 3) & ~3ull) <= max_words) tw = (tw + 3) & ~3ull;  // 16-byte tile rows
         if (tw > max_words) return k;
         k.tile_words = (uint32_t)tw;
         k.ntiles = (uint32_t)(((uint64_t)P.nz + tw * per_word - 1) / (tw * per_word));
@@ -514,7 +515,7 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     // columns are its consecutive z IDs and the seeding copies words
     P.all_z = !P.count_mode && (uint64_t)P.Nh * 10 >= (uint64_t)P.nz * 9;
     if (P.all_z) P.Nh = P.nz;
-    P.dwords = (P.Nh + 31u) / 32u;
+    P.dwords = ((P.Nh + 31u) / 32u + 3u) & ~3u;  // rows of 16-byte multiples
 }
 
 // Dense core of the keys with bin >= P.bc, from the heavy-degree histogram
@@ -588,7 +589,7 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
     P.core = P.thx > 1;
     P.all_z = !count && (uint64_t)P.Nh * 10 >= (uint64_t)P.nz * 9;
     if (P.all_z) P.Nh = P.nz;
-    P.dwords = (P.Nh + 31u) / 32u;
+    P.dwords = ((P.Nh + 31u) / 32u + 3u) & ~3u;  // rows of 16-byte multiples
     return true;
 }
 

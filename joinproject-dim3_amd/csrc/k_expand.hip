@@ -170,6 +170,36 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 const uint32_t pos = block_exclusive_scan<BLOCK, uint32_t>(h >= 0 ? 1u : 0u, nh, sh32);
                 if (h >= 0) s_hk[pos] = (uint32_t)h;
                 __syncthreads();
+                if ((W & 3u) == 0u && (a.dwords & 3u) == 0u) {
+                    // 16-byte loads: tile rows and bitmap rows start on
+                    // 16-byte boundaries (W and dwords multiples of 4 words)
+                    const uint32_t wn4 = wn >> 2;
+                    uint4* tile4 = reinterpret_cast<uint4*>(tile);
+                    for (uint32_t q0 = tid; q0 < wn4; q0 += 2u * BLOCK) {
+                        const uint32_t q1 = q0 + BLOCK;
+                        const bool ok1 = q1 < wn4;
+                        uint4 c0 = make_uint4(0u, 0u, 0u, 0u), c1 = make_uint4(0u, 0u, 0u, 0u);
+                        for (uint32_t j = 0; j < nh; j++) {
+                            const uint4* src = reinterpret_cast<const uint4*>(a.bkeys + (size_t)s_hk[j] * a.dwords + wb);
+                            const uint4 v0 = src[q0];
+                            const uint4 v1 = src[ok1 ? q1 : q0];  // clamped, unconditional
+                            c0.x |= v0.x; c0.y |= v0.y; c0.z |= v0.z; c0.w |= v0.w;
+                            c1.x |= v1.x; c1.y |= v1.y; c1.z |= v1.z; c1.w |= v1.w;
+                        }
+                        if ((c0.x | c0.y | c0.z | c0.w) != 0u) {
+                            uint4 t4 = tile4[q0];
+                            t4.x |= c0.x; t4.y |= c0.y; t4.z |= c0.z; t4.w |= c0.w;
+                            tile4[q0] = t4;
+                        }
+                        if (ok1 && (c1.x | c1.y | c1.z | c1.w) != 0u) {
+                            uint4 t4 = tile4[q1];
+                            t4.x |= c1.x; t4.y |= c1.y; t4.z |= c1.z; t4.w |= c1.w;
+                            tile4[q1] = t4;
+                        }
+                    }
+                    __syncthreads();
+                    continue;
+                }
                 for (uint32_t q0 = tid; q0 < wn; q0 += 4u * BLOCK) {
                     uint32_t acc[4] = {0u, 0u, 0u, 0u};
                     for (uint32_t j = 0; j < nh; j++) {
