@@ -168,8 +168,16 @@ __device__ __forceinline__ T wave_reduce_xor(T v) {
 // lane's key share one atomic; the remaining lanes add on their own.  Every
 // lane still running must call these (valid = false lanes take part without
 // counting).
-__device__ __forceinline__ void wave_count_add(uint32_t* ctr, uint32_t key, bool valid) {
+// hot: the lane's key is the side's most frequent key (all hot lanes share
+// one counter); they share one atomic first, then the rounds above.
+__device__ __forceinline__ void wave_count_add(uint32_t* ctr, uint32_t key, bool valid, bool hot = false) {
     uint64_t pending = __ballot(valid);
+    const uint64_t hm = __ballot(valid && hot);
+    if (hm != 0ull) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)hm) - 1u;
+        if (lane_id() == leader) atomicAdd(&ctr[key], (uint32_t)__popcll(hm));
+        pending &= ~hm;
+    }
 #pragma unroll
     for (int r = 0; r < 2; r++) {
         if (pending == 0) return;
@@ -224,8 +232,17 @@ __device__ __forceinline__ void wave_max_update(uint32_t* arr, uint32_t key, uin
 
 // As wave_count_add for 64-bit sums (mod 2^64): the lanes sharing the first
 // pending lane's key (two rounds) send one atomicAdd of their total.
-__device__ __forceinline__ void wave_add_u64(uint64_t* arr, uint32_t key, uint64_t v, bool valid) {
+__device__ __forceinline__ void wave_add_u64(uint64_t* arr, uint32_t key, uint64_t v, bool valid, bool hot = false) {
     uint64_t pending = __ballot(valid);
+    const uint64_t hm = __ballot(valid && hot);
+    if (hm != 0ull) {  // the hot key's lanes first (see wave_count_add)
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)hm) - 1u;
+        uint64_t m = (valid && hot) ? v : 0ull;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) m += (uint64_t)__shfl_xor((unsigned long long)m, d, 64);
+        if (lane_id() == leader) atomicAdd((unsigned long long*)&arr[key], (unsigned long long)m);
+        pending &= ~hm;
+    }
 #pragma unroll
     for (int r = 0; r < 2; r++) {
         if (pending == 0) return;
@@ -345,6 +362,7 @@ struct Scalars {
     uint64_t agg_sum2;   // COUNT fingerprint from the join paths: sum of k2(x) g(z) over paths
     uint64_t n_hash;     // value aggregates: rows through the LDS hash table
     uint64_t n_wave;     // value aggregates: rows expanded one per wavefront
+    int32_t hot[4];      // the most frequent join key of R and of S in a sample, each with a found flag
     uint64_t spare[4];
     // work counters on a cache line of their own: atomics on one line
     // serialise, and every task fetch would queue behind the row reservations
@@ -392,9 +410,12 @@ void map_init(const KeyMap& m, hipStream_t st);
 void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm, hipStream_t st);
 
 // y mapping: insert R.y, count degR, record slots; probe S.y, count degS and survivors
-void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st);
+// hot (nullable): {R's most frequent key, valid, S's, valid} (hot_keys)
+void hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st);
+void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
+             const int32_t* hot = nullptr);
 void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, uint32_t* degS, int32_t* slotS,
-             uint64_t* surv, hipStream_t st);
+             uint64_t* surv, hipStream_t st, const int32_t* hot = nullptr);
 // scan of the y slots: ex[s] = (#joined y before s) << 32; out_j accumulated
 void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* ex, uint64_t* partials,
             uint64_t* tot, uint64_t* out_j, hipStream_t st);
@@ -402,7 +423,7 @@ void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint
 // *out += sum over y slots of KR(y) GS(y): the path-sum term of the COUNT
 // fingerprint (kr, gs: nslots u64 scratch each)
 void path_sums(const int2* R, uint64_t nR, const uint32_t* slotR, const int2* S, uint64_t nS, const int32_t* slotS,
-               uint32_t nslots, uint64_t* kr, uint64_t* gs, uint64_t* out, hipStream_t st);
+               uint32_t nslots, uint64_t* kr, uint64_t* gs, uint64_t* out, hipStream_t st, const int32_t* hot = nullptr);
 // x / z mapping over the tuples whose y joined
 void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
              uint32_t* degX, int32_t* slotRx, hipStream_t st);

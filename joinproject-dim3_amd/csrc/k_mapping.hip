@@ -129,31 +129,84 @@ __global__ __launch_bounds__(kThreads) void k_minmax(const int2* __restrict__ V,
 }
 
 // mapping_y insert side (dim3.hpp:822-828): every R.y gets a slot; degR counts it.
+// hot (nullable): {the side's most frequent key, valid}; its lanes share one
+// atomic per wavefront (a Zipf key can hold a fifth of the tuples)
+__device__ __forceinline__ bool is_hot(const int32_t* hot, int32_t y) {
+    return hot != nullptr && hot[1] != 0 && y == hot[0];
+}
+
 __global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, dim3h::KeyMap m, uint32_t* degR,
-                          uint32_t* __restrict__ slotR) {
+                          uint32_t* __restrict__ slotR, const int32_t* __restrict__ hot) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nR) return;
-    const uint32_t s = slot_insert(m, R[i].y);
-    wave_count_add(degR, s, true);
+    const int32_t y = R[i].y;
+    const uint32_t s = slot_insert(m, y);
+    wave_count_add(degR, s, true, is_hot(hot, y));
     slotR[i] = s;
+}
+
+// The mode of 4096 evenly spaced keys of T (one workgroup): out = {key, found}.
+__global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, uint64_t n, int32_t* out) {
+    constexpr uint32_t kSample = 4096, kSlots = 8192;
+    __shared__ uint32_t keys[kSlots];
+    __shared__ uint32_t cnt[kSlots];
+    __shared__ uint64_t best[1024 / 64];
+    for (uint32_t k = threadIdx.x; k < kSlots; k += 1024) {
+        keys[k] = 0xFFFFFFFFu;
+        cnt[k] = 0;
+    }
+    __syncthreads();
+    const uint64_t m = n < kSample ? n : kSample;
+    const uint64_t step = m ? n / m : 0;  // index k * step < m * step <= n
+    for (uint32_t k = threadIdx.x; k < m; k += 1024) {
+        const uint32_t y = (uint32_t)T[(uint64_t)k * step].y;
+        // open addressing; the key 0xFFFFFFFF (-1) is not a candidate
+        if (y == 0xFFFFFFFFu) continue;
+        uint32_t s = (y * 0x9E3779B1u) >> 19;  // 13 bits: kSlots
+        for (;;) {
+            const uint32_t old = atomicCAS(&keys[s], 0xFFFFFFFFu, y);
+            if (old == 0xFFFFFFFFu || old == y) break;
+            s = (s + 1u) & (kSlots - 1u);
+        }
+        atomicAdd(&cnt[s], 1u);
+    }
+    __syncthreads();
+    // largest count (ties: the largest slot), packed as count << 32 | slot
+    uint64_t b = 0;
+    for (uint32_t k = threadIdx.x; k < kSlots; k += 1024)
+        if (cnt[k]) b = max(b, ((uint64_t)cnt[k] << 32) | k);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) b = max(b, (uint64_t)__shfl_xor((unsigned long long)b, d, 64));
+    if (lane_id() == 0) best[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t r = 0;
+        for (uint32_t w = 0; w < 1024 / 64; w++) r = max(r, best[w]);
+        out[0] = r ? (int32_t)keys[(uint32_t)r] : 0;
+        out[1] = r ? 1 : 0;
+    }
 }
 
 // mapping_y probe side (dim3.hpp:830-836): S tuples whose y is not in R are
 // dropped (slot -1); degS counts the survivors, *surv their number.
 __global__ __launch_bounds__(kThreads) void k_probe_y(const int2* __restrict__ S, uint64_t nS, dim3h::KeyMap m,
                                                       const uint32_t* __restrict__ degR, uint32_t* degS,
-                                                      int32_t* __restrict__ slotS, uint64_t* surv) {
+                                                      int32_t* __restrict__ slotS, uint64_t* surv,
+                                                      const int32_t* __restrict__ hot) {
     __shared__ uint64_t sh[kThreads / 64 + 1];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t ok = 0;
     int32_t s = -1;
+    bool h = false;
     if (i < nS) {
-        s = slot_find(m, S[i].y);
+        const int32_t y = S[i].y;
+        s = slot_find(m, y);
         if (s >= 0 && m.range && degR[s] == 0) s = -1;
         slotS[i] = s;
         ok = s >= 0 ? 1 : 0;
+        h = is_hot(hot, y);
     }
-    wave_count_add(degS, (uint32_t)s, s >= 0);
+    wave_count_add(degS, (uint32_t)s, s >= 0, h);
     ok = block_reduce_sum<kThreads>(ok, sh);
     if (threadIdx.x == 0 && ok) atomicAdd((unsigned long long*)surv, (unsigned long long)ok);
 }
@@ -504,19 +557,21 @@ __global__ void k_grid_rel(int2* __restrict__ out, uint32_t X, uint32_t Y) {
 // KR(y) = sum of k2 over R(y) and GS(y) = sum of g over S(y) (mod 2^64;
 // tuples counted with multiplicity, as the COUNT is).
 __global__ __launch_bounds__(kThreads) void k_path_kr(const int2* __restrict__ R, uint64_t nR,
-                                                      const uint32_t* __restrict__ slotR, uint64_t* kr) {
+                                                      const uint32_t* __restrict__ slotR, uint64_t* kr,
+                                                      const int32_t* __restrict__ hot) {
     const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     const bool ok = i < nR;
     const uint32_t s = ok ? slotR[i] : 0u;
     const uint64_t v = ok ? dim3d::xfactor2(R[i].x) : 0ull;
-    wave_add_u64(kr, s, v, ok);
+    wave_add_u64(kr, s, v, ok, ok && is_hot(hot, R[i].y));
 }
 __global__ __launch_bounds__(kThreads) void k_path_gs(const int2* __restrict__ S, uint64_t nS,
-                                                      const int32_t* __restrict__ slotS, uint64_t* gs) {
+                                                      const int32_t* __restrict__ slotS, uint64_t* gs,
+                                                      const int32_t* __restrict__ hot) {
     const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     const int32_t s = i < nS ? slotS[i] : -1;
     const uint64_t g = s >= 0 ? dim3d::zfactor(S[i].x) : 0ull;
-    wave_add_u64(gs, s >= 0 ? (uint32_t)s : 0u, g, s >= 0);
+    wave_add_u64(gs, s >= 0 ? (uint32_t)s : 0u, g, s >= 0, s >= 0 && is_hot(hot, S[i].y));
 }
 __global__ __launch_bounds__(kThreads) void k_path_dot(const uint64_t* __restrict__ kr,
                                                        const uint64_t* __restrict__ gs, uint32_t n, uint64_t* out) {
@@ -555,15 +610,21 @@ void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm,
     if (nR) hipLaunchKernelGGL(k_minmax, dim3(gr), dim3(kThreads), 0, st, R, nR, mm);
     if (nS) hipLaunchKernelGGL(k_minmax, dim3(gs), dim3(kThreads), 0, st, S, nS, mm + 4);
 }
-void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st) {
+void hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st) {
+    hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, R, nR, hot);
+    hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, S, nS, hot + 2);
+}
+void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
+             const int32_t* hot) {
     if (count_lds<0>(R, nR, m, nullptr, nullptr, nullptr, degR, (int32_t*)slotR, nullptr, st)) return;
-    if (nR) hipLaunchKernelGGL(k_build_y, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, m, degR, slotR);
+    if (nR) hipLaunchKernelGGL(k_build_y, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, m, degR, slotR, hot);
 }
 void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, uint32_t* degS, int32_t* slotS,
-             uint64_t* surv, hipStream_t st) {
+             uint64_t* surv, hipStream_t st, const int32_t* hot) {
     if (count_lds<1>(S, nS, m, degR, nullptr, nullptr, degS, slotS, surv, st)) return;
     if (nS)
-        hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, m, degR, degS, slotS, surv);
+        hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, m, degR, degS, slotS, surv,
+                           hot != nullptr ? hot + 2 : nullptr);
 }
 void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* ex, uint64_t* partials,
             uint64_t* tot, uint64_t* out_j, hipStream_t st) {
@@ -576,11 +637,13 @@ void outj_only(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint
     hipLaunchKernelGGL(k_outj, dim3(g), dim3(kThreads), 0, st, degR, degS, nslots, out_j);
 }
 void path_sums(const int2* R, uint64_t nR, const uint32_t* slotR, const int2* S, uint64_t nS, const int32_t* slotS,
-               uint32_t nslots, uint64_t* kr, uint64_t* gs, uint64_t* out, hipStream_t st) {
+               uint32_t nslots, uint64_t* kr, uint64_t* gs, uint64_t* out, hipStream_t st, const int32_t* hot) {
     fill_u64(kr, nslots, 0, st);
     fill_u64(gs, nslots, 0, st);
-    if (nR) hipLaunchKernelGGL(k_path_kr, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, kr);
-    if (nS) hipLaunchKernelGGL(k_path_gs, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, gs);
+    if (nR) hipLaunchKernelGGL(k_path_kr, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, kr, hot);
+    if (nS)
+        hipLaunchKernelGGL(k_path_gs, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, gs,
+                           hot != nullptr ? hot + 2 : nullptr);
     unsigned g = grid_for(nslots);
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(k_path_dot, dim3(g), dim3(kThreads), 0, st, kr, gs, nslots, out);
