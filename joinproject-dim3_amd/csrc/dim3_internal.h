@@ -168,15 +168,19 @@ __device__ __forceinline__ T wave_reduce_xor(T v) {
 // lane's key share one atomic; the remaining lanes add on their own.  Every
 // lane still running must call these (valid = false lanes take part without
 // counting).
-// hot: the lane's key is the side's most frequent key (all hot lanes share
-// one counter); they share one atomic first, then the rounds above.
-__device__ __forceinline__ void wave_count_add(uint32_t* ctr, uint32_t key, bool valid, bool hot = false) {
+// hot: the lane's index among the side's most frequent keys (hot_index), or
+// -1; the lanes of each hot key present share one atomic first, then the
+// rounds above take the rest.
+__device__ __forceinline__ void wave_count_add(uint32_t* ctr, uint32_t key, bool valid, int hot = -1) {
     uint64_t pending = __ballot(valid);
-    const uint64_t hm = __ballot(valid && hot);
-    if (hm != 0ull) {
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)hm) - 1u;
-        if (lane_id() == leader) atomicAdd(&ctr[key], (uint32_t)__popcll(hm));
-        pending &= ~hm;
+    uint64_t hp = __ballot(valid && hot >= 0);
+    pending &= ~hp;
+    while (hp != 0ull) {  // at most kHotKeys rounds
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)hp) - 1u;
+        const int lh = __builtin_amdgcn_readlane(hot, (int)leader);
+        const uint64_t same = __ballot(hot == lh) & hp;
+        if (lane_id() == leader) atomicAdd(&ctr[key], (uint32_t)__popcll(same));
+        hp &= ~same;
     }
 #pragma unroll
     for (int r = 0; r < 2; r++) {
@@ -232,16 +236,19 @@ __device__ __forceinline__ void wave_max_update(uint32_t* arr, uint32_t key, uin
 
 // As wave_count_add for 64-bit sums (mod 2^64): the lanes sharing the first
 // pending lane's key (two rounds) send one atomicAdd of their total.
-__device__ __forceinline__ void wave_add_u64(uint64_t* arr, uint32_t key, uint64_t v, bool valid, bool hot = false) {
+__device__ __forceinline__ void wave_add_u64(uint64_t* arr, uint32_t key, uint64_t v, bool valid, int hot = -1) {
     uint64_t pending = __ballot(valid);
-    const uint64_t hm = __ballot(valid && hot);
-    if (hm != 0ull) {  // the hot key's lanes first (see wave_count_add)
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)hm) - 1u;
-        uint64_t m = (valid && hot) ? v : 0ull;
+    uint64_t hp = __ballot(valid && hot >= 0);
+    pending &= ~hp;
+    while (hp != 0ull) {  // the hot keys' lanes first (see wave_count_add)
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)hp) - 1u;
+        const int lh = __builtin_amdgcn_readlane(hot, (int)leader);
+        const uint64_t same = __ballot(hot == lh) & hp;
+        uint64_t m = ((same >> lane_id()) & 1ull) ? v : 0ull;
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) m += (uint64_t)__shfl_xor((unsigned long long)m, d, 64);
         if (lane_id() == leader) atomicAdd((unsigned long long*)&arr[key], (unsigned long long)m);
-        pending &= ~hm;
+        hp &= ~same;
     }
 #pragma unroll
     for (int r = 0; r < 2; r++) {
@@ -344,6 +351,8 @@ inline hipError_t host_free(void* p) {
 
 
 // Device scalar block, zeroed at the start of every query.
+constexpr int kHotKeys = 15, kHotSlots = kHotKeys + 1;
+
 struct Scalars {
     uint64_t tot_y;      // scan total of the y table: (ny << 32) | 0
     uint64_t tot_x;      // (nx << 32) | nnzR
@@ -362,7 +371,9 @@ struct Scalars {
     uint64_t agg_sum2;   // COUNT fingerprint from the join paths: sum of k2(x) g(z) over paths
     uint64_t n_hash;     // value aggregates: rows through the LDS hash table
     uint64_t n_wave;     // value aggregates: rows expanded one per wavefront
-    int32_t hot[4];      // the most frequent join key of R and of S in a sample, each with a found flag
+    // the most frequent join keys of R (hot[0..kHotSlots)) and of S (the
+    // rest) in a sample: a count, then the keys
+    int32_t hot[2 * kHotSlots];
     uint64_t spare[4];
     // work counters on a cache line of their own: atomics on one line
     // serialise, and every task fetch would queue behind the row reservations
@@ -411,6 +422,9 @@ void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm,
 
 // y mapping: insert R.y, count degR, record slots; probe S.y, count degS and survivors
 // hot (nullable): {R's most frequent key, valid, S's, valid} (hot_keys)
+// The kHotKeys most frequent join keys of each side among 4096 evenly spaced
+// tuples (those seen at least 8 times): hot[0] = count, hot[1..] = keys; S's
+// list starts at hot + kHotSlots.
 void hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st);
 void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
              const int32_t* hot = nullptr);

@@ -129,10 +129,16 @@ __global__ __launch_bounds__(kThreads) void k_minmax(const int2* __restrict__ V,
 }
 
 // mapping_y insert side (dim3.hpp:822-828): every R.y gets a slot; degR counts it.
-// hot (nullable): {the side's most frequent key, valid}; its lanes share one
-// atomic per wavefront (a Zipf key can hold a fifth of the tuples)
-__device__ __forceinline__ bool is_hot(const int32_t* hot, int32_t y) {
-    return hot != nullptr && hot[1] != 0 && y == hot[0];
+// hot (nullable): {count, keys...}, the side's most frequent keys; each key's
+// lanes share one atomic per wavefront (a Zipf key can hold a fifth of the
+// tuples).  Returns y's index in the list, or -1.
+__device__ __forceinline__ int hot_index(const int32_t* hot, int32_t y) {
+    if (hot == nullptr) return -1;
+    const int n = hot[0];
+    int h = -1;
+    for (int k = 0; k < n; k++)
+        if (y == hot[1 + k]) h = k;
+    return h;
 }
 
 __global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, dim3h::KeyMap m, uint32_t* degR,
@@ -141,13 +147,15 @@ __global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, dim3h::KeyMap
     if (i >= nR) return;
     const int32_t y = R[i].y;
     const uint32_t s = slot_insert(m, y);
-    wave_count_add(degR, s, true, is_hot(hot, y));
+    wave_count_add(degR, s, true, hot_index(hot, y));
     slotR[i] = s;
 }
 
-// The mode of 4096 evenly spaced keys of T (one workgroup): out = {key, found}.
+// The most frequent keys of 4096 evenly spaced tuples of T (one workgroup):
+// out = {count, keys...}, up to kHotKeys keys seen at least kMinSeen times,
+// most frequent first.
 __global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, uint64_t n, int32_t* out) {
-    constexpr uint32_t kSample = 4096, kSlots = 8192;
+    constexpr uint32_t kSample = 4096, kSlots = 8192, kMinSeen = 8;
     __shared__ uint32_t keys[kSlots];
     __shared__ uint32_t cnt[kSlots];
     __shared__ uint64_t best[1024 / 64];
@@ -171,20 +179,28 @@ __global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, ui
         atomicAdd(&cnt[s], 1u);
     }
     __syncthreads();
-    // largest count (ties: the largest slot), packed as count << 32 | slot
-    uint64_t b = 0;
-    for (uint32_t k = threadIdx.x; k < kSlots; k += 1024)
-        if (cnt[k]) b = max(b, ((uint64_t)cnt[k] << 32) | k);
+    int found = 0;
+    for (int r = 0; r < dim3h::kHotKeys; r++) {
+        // largest count (ties: the largest slot), packed as count << 32 | slot
+        uint64_t b = 0;
+        for (uint32_t k = threadIdx.x; k < kSlots; k += 1024)
+            if (cnt[k] >= kMinSeen) b = max(b, ((uint64_t)cnt[k] << 32) | k);
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) b = max(b, (uint64_t)__shfl_xor((unsigned long long)b, d, 64));
-    if (lane_id() == 0) best[threadIdx.x >> 6] = b;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t r = 0;
-        for (uint32_t w = 0; w < 1024 / 64; w++) r = max(r, best[w]);
-        out[0] = r ? (int32_t)keys[(uint32_t)r] : 0;
-        out[1] = r ? 1 : 0;
+        for (int d = 32; d > 0; d >>= 1) b = max(b, (uint64_t)__shfl_xor((unsigned long long)b, d, 64));
+        if (lane_id() == 0) best[threadIdx.x >> 6] = b;
+        __syncthreads();
+        uint64_t w = 0;
+        for (uint32_t v = 0; v < 1024 / 64; v++) w = max(w, best[v]);
+        __syncthreads();  // best is rewritten next round; cnt[slot] is cleared below
+        if (w == 0) break;  // uniform: every thread read the same best[]
+        if (threadIdx.x == 0) {
+            out[1 + found] = (int32_t)keys[(uint32_t)w];
+            cnt[(uint32_t)w] = 0;
+        }
+        found++;
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = found;
 }
 
 // mapping_y probe side (dim3.hpp:830-836): S tuples whose y is not in R are
@@ -197,14 +213,14 @@ __global__ __launch_bounds__(kThreads) void k_probe_y(const int2* __restrict__ S
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t ok = 0;
     int32_t s = -1;
-    bool h = false;
+    int h = -1;
     if (i < nS) {
         const int32_t y = S[i].y;
         s = slot_find(m, y);
         if (s >= 0 && m.range && degR[s] == 0) s = -1;
         slotS[i] = s;
         ok = s >= 0 ? 1 : 0;
-        h = is_hot(hot, y);
+        h = hot_index(hot, y);
     }
     wave_count_add(degS, (uint32_t)s, s >= 0, h);
     ok = block_reduce_sum<kThreads>(ok, sh);
@@ -563,7 +579,7 @@ __global__ __launch_bounds__(kThreads) void k_path_kr(const int2* __restrict__ R
     const bool ok = i < nR;
     const uint32_t s = ok ? slotR[i] : 0u;
     const uint64_t v = ok ? dim3d::xfactor2(R[i].x) : 0ull;
-    wave_add_u64(kr, s, v, ok, ok && is_hot(hot, R[i].y));
+    wave_add_u64(kr, s, v, ok, ok ? hot_index(hot, R[i].y) : -1);
 }
 __global__ __launch_bounds__(kThreads) void k_path_gs(const int2* __restrict__ S, uint64_t nS,
                                                       const int32_t* __restrict__ slotS, uint64_t* gs,
@@ -571,7 +587,7 @@ __global__ __launch_bounds__(kThreads) void k_path_gs(const int2* __restrict__ S
     const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     const int32_t s = i < nS ? slotS[i] : -1;
     const uint64_t g = s >= 0 ? dim3d::zfactor(S[i].x) : 0ull;
-    wave_add_u64(gs, s >= 0 ? (uint32_t)s : 0u, g, s >= 0, s >= 0 && is_hot(hot, S[i].y));
+    wave_add_u64(gs, s >= 0 ? (uint32_t)s : 0u, g, s >= 0, s >= 0 ? hot_index(hot, S[i].y) : -1);
 }
 __global__ __launch_bounds__(kThreads) void k_path_dot(const uint64_t* __restrict__ kr,
                                                        const uint64_t* __restrict__ gs, uint32_t n, uint64_t* out) {
@@ -612,7 +628,7 @@ void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm,
 }
 void hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st) {
     hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, R, nR, hot);
-    hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, S, nS, hot + 2);
+    hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, S, nS, hot + kHotSlots);
 }
 void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
              const int32_t* hot) {
@@ -624,7 +640,7 @@ void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, 
     if (count_lds<1>(S, nS, m, degR, nullptr, nullptr, degS, slotS, surv, st)) return;
     if (nS)
         hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, m, degR, degS, slotS, surv,
-                           hot != nullptr ? hot + 2 : nullptr);
+                           hot != nullptr ? hot + kHotSlots : nullptr);
 }
 void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* ex, uint64_t* partials,
             uint64_t* tot, uint64_t* out_j, hipStream_t st) {
@@ -643,7 +659,7 @@ void path_sums(const int2* R, uint64_t nR, const uint32_t* slotR, const int2* S,
     if (nR) hipLaunchKernelGGL(k_path_kr, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, kr, hot);
     if (nS)
         hipLaunchKernelGGL(k_path_gs, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, gs,
-                           hot != nullptr ? hot + 2 : nullptr);
+                           hot != nullptr ? hot + kHotSlots : nullptr);
     unsigned g = grid_for(nslots);
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(k_path_dot, dim3(g), dim3(kThreads), 0, st, kr, gs, nslots, out);
