@@ -300,6 +300,11 @@ int dim3_count_join_paths(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, cons
  * applies first (the reference asserts |R| >= |S|).  Host inputs. */
 int dim3_estimate_join_cardinality(dim3_ctx* ctx, const int32_t* R_xy, uint64_t n_r, const int32_t* S_zy,
                                    uint64_t n_s, int64_t* estimate);
+/* The same over device-resident relations, on `stream` (NULL: the
+ * context's stream); returns once the estimate is on the host.  The server
+ * sizes a materialising request's output with it at submit. */
+int dim3_estimate_join_cardinality_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_r, const int32_t* d_S,
+                                       uint64_t n_s, void* stream, int64_t* estimate);
 
 /* The dense part's product on its own (the boolean product that
  * Hybrid_solution's DENSE part evaluates one (x, dense z) pair at a time,

@@ -118,7 +118,7 @@ EXPORTS = [
     "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
     "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev", "dim3_estimate_join_cardinality",
     "dim3_join_aggregate_tuples", "dim3_server_put_values", "dim3_join_aggregate_stream",
-    "dim3_join_aggregate_f32_stream",
+    "dim3_join_aggregate_f32_stream", "dim3_estimate_join_cardinality_dev",
 ]
 
 _lib = None
@@ -196,6 +196,8 @@ def lib():
     L.dim3_result_free.restype = None
     L.dim3_estimate_join_cardinality.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_int64)]
     L.dim3_estimate_join_cardinality.restype = i32
+    L.dim3_estimate_join_cardinality_dev.argtypes = [vp, vp, u64, vp, u64, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.dim3_estimate_join_cardinality_dev.restype = i32
     L.dim3_count_join_paths.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
     L.dim3_count_join_paths.restype = i32
     for f in (L.dim3_batch_join_project_dev, L.dim3_batch_join_aggregate_count_dev):
@@ -639,6 +641,14 @@ class Context:
         rc = self._L.dim3_estimate_join_cardinality(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0],
                                                     ctypes.byref(out))
         self._check(rc, "dim3_estimate_join_cardinality")
+        return int(out.value)
+
+    def estimate_join_cardinality_dev(self, d_R, n_r, d_S, n_s, stream=None):
+        """The same over device addresses (ints) of int32 pair arrays."""
+        out = ctypes.c_int64()
+        rc = self._L.dim3_estimate_join_cardinality_dev(self._h, ctypes.c_void_p(d_R), n_r, ctypes.c_void_p(d_S),
+                                                        n_s, ctypes.c_void_p(stream or 0), ctypes.byref(out))
+        self._check(rc, "dim3_estimate_join_cardinality_dev")
         return int(out.value)
 
     def count_join_paths(self, R, S):

@@ -2098,27 +2098,9 @@ int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const i
     return DIM3_OK;
 }
 
-int dim3_estimate_join_cardinality(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,
-                                   int64_t* estimate) {
-    if (!c || !estimate) return DIM3_ERR_INVALID;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    c->err.clear();
-    *estimate = 0;
-    if (nR < nS) {  // the swap rule; the reference asserts |R| >= |S| (dim3.hpp:194)
-        std::swap(R_xy, S_zy);
-        std::swap(nR, nS);
-    }
-    if (nS == 0) return DIM3_OK;
-    if (!R_xy || !S_zy) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
-    if (nR >= (1ull << 31)) return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^31-1 tuples");
-    HIPCHK(c, hipSetDevice(c->device));
-    hipStream_t st = c->stream;
-    c->qstream = st;
-    int2 *dR, *dS;
-    TRY(ws(c, WS_IN_R, nR, &dR));
-    TRY(ws(c, WS_IN_S, nS, &dS));
-    HIPCHK(c, hipMemcpyAsync(dR, R_xy, nR * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(c, hipMemcpyAsync(dS, S_zy, nS * 8, hipMemcpyHostToDevice, st));
+// The estimate over device relations (|R| >= |S| > 0), on stream st.
+static int estimate_on(dim3_ctx* c, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS, hipStream_t st,
+                       int64_t* estimate) {
     // sample bits: halve |S| until it is at most 10000 (dim3.hpp:262-269)
     uint32_t bits = 0;
     for (uint64_t n = nS; n > 10000; n /= 2) bits++;
@@ -2148,6 +2130,52 @@ int dim3_estimate_join_cardinality(dim3_ctx* c, const int32_t* R_xy, uint64_t nR
         *estimate = (int64_t)(r2 * ((double)nR / 1e3) * ((double)nS / 1e3));
     }
     return DIM3_OK;
+}
+
+// Argument checks and the swap rule; the reference asserts |R| >= |S|
+// (dim3.hpp:194).  Returns -1 when there is nothing to sample (estimate 0).
+static int estimate_args(dim3_ctx* c, const int32_t*& R_xy, uint64_t& nR, const int32_t*& S_zy, uint64_t& nS,
+                         int64_t* estimate) {
+    *estimate = 0;
+    if (nR < nS) {
+        std::swap(R_xy, S_zy);
+        std::swap(nR, nS);
+    }
+    if (nS == 0) return -1;
+    if (!R_xy || !S_zy) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    if (nR >= (1ull << 31)) return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^31-1 tuples");
+    return DIM3_OK;
+}
+
+int dim3_estimate_join_cardinality(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS,
+                                   int64_t* estimate) {
+    if (!c || !estimate) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    const int a = estimate_args(c, R_xy, nR, S_zy, nS, estimate);
+    if (a != DIM3_OK) return a < 0 ? DIM3_OK : a;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    c->qstream = st;
+    int2 *dR, *dS;
+    TRY(ws(c, WS_IN_R, nR, &dR));
+    TRY(ws(c, WS_IN_S, nS, &dS));
+    HIPCHK(c, hipMemcpyAsync(dR, R_xy, nR * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(dS, S_zy, nS * 8, hipMemcpyHostToDevice, st));
+    return estimate_on(c, dR, nR, dS, nS, st, estimate);
+}
+
+int dim3_estimate_join_cardinality_dev(dim3_ctx* c, const int32_t* d_R, uint64_t nR, const int32_t* d_S, uint64_t nS,
+                                       void* stream, int64_t* estimate) {
+    if (!c || !estimate) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    const int a = estimate_args(c, d_R, nR, d_S, nS, estimate);
+    if (a != DIM3_OK) return a < 0 ? DIM3_OK : a;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    c->qstream = st;
+    return estimate_on(c, (const int2*)d_R, nR, (const int2*)d_S, nS, st, estimate);
 }
 
 int dim3_dense_product(dim3_ctx* c, const int8_t* A, uint64_t M, const int8_t* Bt, uint64_t N, uint64_t K,

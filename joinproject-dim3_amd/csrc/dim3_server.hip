@@ -52,7 +52,8 @@ void free_rel(Rel& r) {
 
 struct Job {
     dim3_request req{};
-    uint64_t est = 0;
+    uint64_t est = 0;      // footprint: workspace, plus out_est
+    uint64_t out_est = 0;  // materialising requests: the estimated output bytes
     uint64_t tuples = 0;
     Clock::time_point t_submit, t_start, t_end;
     int status = DIM3_OK;
@@ -92,6 +93,9 @@ struct dim3_server {
     std::vector<char> release;    // per worker: asked to give its idle workspace back
     std::vector<dim3_ctx*> ctxs;
     std::vector<std::thread> threads;
+    // submit-time join size estimates (materialising requests), one at a time
+    std::mutex est_mu;
+    dim3_ctx* est_ctx = nullptr;
     // Compute-unit partition (2+ workers): lane 0's large requests run on a
     // stream masked to the first large_cus CUs; while one runs, the other
     // workers launch on streams masked to the remaining CUs, so neither kind
@@ -109,6 +113,17 @@ struct dim3_server {
             if (v != w) s += held[v];
         return s;
     }
+    // a request's hold on its relations ends; a dropped relation is freed by its last user (mu held)
+    void unpin(uint64_t hdl) {
+        Rel& r = rels.at(hdl);
+        if (--r.users == 0 && r.dropped) {
+            hipSetDevice(device);
+            free_rel(r);
+            resident -= r.bytes + r.vbytes;
+            rels.erase(hdl);
+        }
+    }
+    int estimate_paths(const int32_t* dR, uint64_t nR, const int32_t* dS, uint64_t nS, int64_t* out);
     void note_peak() {
         uint64_t s = resident;
         for (auto h : held) s += h;
@@ -171,6 +186,17 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
     const bool f32 = value && R[0]->vtype == DIM3_VALUE_F32;
     const size_t row = f32 ? sizeof(dim3_value_row_f32)
                        : (count || value) ? sizeof(dim3_count_row) : 2 * sizeof(int32_t);
+    if (!fp && j->out_est > *out_bytes) {  // the estimated output (admitted with the request)
+        if (*out_buf) dim3h::dev_free(*out_buf);
+        *out_buf = nullptr;
+        *out_bytes = 0;
+        if (dim3h::dev_malloc(out_buf, j->out_est) == hipSuccess) {
+            *out_bytes = j->out_est;
+        } else {
+            (void)hipGetLastError();
+            *out_buf = nullptr;
+        }
+    }
     uint64_t cap = fp ? 0 : *out_bytes / row;
     dim3_result res;
     for (int attempt = 0; attempt < 2; attempt++) {
@@ -246,7 +272,7 @@ void dim3_server::worker(uint32_t w) {
                         j.status = DIM3_ERR_ADMISSION;
                         j.done = true;
                         j.t_start = j.t_end = Clock::now();
-                        for (uint64_t hdl : {j.req.r, j.req.s}) rels.at(hdl).users--;
+                        for (uint64_t hdl : {j.req.r, j.req.s}) unpin(hdl);
                         st.rejected++;
                         jt = pending.erase(jt);
                         cv_done.notify_all();
@@ -336,14 +362,7 @@ void dim3_server::worker(uint32_t w) {
                 j->t_end = now;
                 j->done = true;
                 st.completed++;
-                for (uint64_t hdl : {j->req.r, j->req.s}) {
-                    Rel& r = rels.at(hdl);
-                    if (--r.users == 0 && r.dropped) {
-                        free_rel(r);
-                        resident -= r.bytes + r.vbytes;
-                        rels.erase(hdl);
-                    }
-                }
+                for (uint64_t hdl : {j->req.r, j->req.s}) unpin(hdl);
             }
             held[w] = h;
             busy[w] = 0;
@@ -444,6 +463,7 @@ int dim3_server_destroy(dim3_server* s) {
     for (auto& kv : s->rels) free_rel(kv.second);
     for (auto& kv : s->jobs) dim3_result_free(&kv.second.res);
     for (auto* c : s->ctxs) dim3_ctx_destroy(c);
+    if (s->est_ctx) dim3_ctx_destroy(s->est_ctx);
     for (auto x : s->full_stream) hipStreamDestroy(x);
     for (auto x : s->part_stream) hipStreamDestroy(x);
     delete s;
@@ -545,13 +565,26 @@ int dim3_server_drop_relation(dim3_server* s, uint64_t handle) {
     return DIM3_OK;
 }
 
+int dim3_server::estimate_paths(const int32_t* dR, uint64_t nR, const int32_t* dS, uint64_t nS, int64_t* out) {
+    std::lock_guard<std::mutex> lk(est_mu);
+    hipSetDevice(device);
+    if (!est_ctx) {
+        const int rc = dim3_ctx_create(device, budget, &est_ctx);
+        if (rc != DIM3_OK) return rc;
+    }
+    const int rc = dim3_estimate_join_cardinality_dev(est_ctx, dR, nR, dS, nS, nullptr, out);
+    // its hash table grows with |S|: give a large one back
+    if (dim3_ctx_device_bytes(est_ctx) > (64ull << 20)) dim3_ctx_release(est_ctx);
+    return rc;
+}
+
 int dim3_submit(dim3_server* s, const dim3_request* req, uint64_t* ticket) {
     if (!s || !req || !ticket) return DIM3_ERR_INVALID;
     *ticket = 0;
     if (req->kind > DIM3_REQ_VALUE) return DIM3_ERR_INVALID;
     if (req->kind == DIM3_REQ_COUNT && req->opts.dense_mode == DIM3_DENSE_BITOR) return DIM3_ERR_UNSUPPORTED;
     if (req->kind == DIM3_REQ_VALUE && (req->agg < DIM3_AGG_SUM || req->agg > DIM3_AGG_AVG)) return DIM3_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     auto ir = s->rels.find(req->r), is = s->rels.find(req->s);
     if (ir == s->rels.end() || is == s->rels.end() || ir->second.dropped || is->second.dropped)
         return DIM3_ERR_NOT_FOUND;
@@ -560,9 +593,43 @@ int dim3_submit(dim3_server* s, const dim3_request* req, uint64_t* ticket) {
         (!ir->second.dv || !is->second.dv || ir->second.vtype != is->second.vtype))
         return DIM3_ERR_INVALID;
     if (s->pending.size() >= s->queue_cap) return DIM3_ERR_BUSY;
-    const uint64_t est = footprint(ir->second.n, is->second.n);
+    const uint64_t nR = ir->second.n, nS = is->second.n;
+    uint64_t est = footprint(nR, nS), out_est = 0;
+    // the request holds its relations from here (the job takes these holds)
+    ir->second.users++;
+    is->second.users++;
+    if (req->opts.output_mode != DIM3_OUT_FINGERPRINT) {
+        // a materialising request also needs its output: at most one row per
+        // join path, the count EstimateJoinCardinality samples (run on the
+        // device, without the server lock)
+        const size_t row = req->kind == DIM3_REQ_PROJECT ? 2 * sizeof(int32_t)
+                           : (req->kind == DIM3_REQ_VALUE && ir->second.vtype == DIM3_VALUE_F32)
+                               ? sizeof(dim3_value_row_f32)
+                               : sizeof(dim3_count_row);
+        const int32_t *dR = ir->second.d, *dS = is->second.d;
+        lk.unlock();
+        int64_t paths = 0;
+        const int rc = s->estimate_paths(dR, nR, dS, nS, &paths);
+        lk.lock();
+        const bool gone = s->rels.at(req->r).dropped || s->rels.at(req->s).dropped;
+        if (rc != DIM3_OK || gone) {
+            s->unpin(req->r);
+            s->unpin(req->s);
+            return rc != DIM3_OK ? rc : DIM3_ERR_NOT_FOUND;
+        }
+        const double rows = std::min((double)std::max<int64_t>(paths, 0), (double)nR * (double)nS);
+        out_est = rows * row > 1e18 ? (uint64_t)1e18 : (uint64_t)(rows * row);
+        est += out_est;
+    }
+    if (s->pending.size() >= s->queue_cap) {  // filled while the estimate ran
+        s->unpin(req->r);
+        s->unpin(req->s);
+        return DIM3_ERR_BUSY;
+    }
     s->st.submitted++;
     if (s->resident + est > s->budget) {
+        s->unpin(req->r);
+        s->unpin(req->s);
         s->st.rejected++;
         return DIM3_ERR_ADMISSION;
     }
@@ -570,10 +637,9 @@ int dim3_submit(dim3_server* s, const dim3_request* req, uint64_t* ticket) {
     Job& j = s->jobs[t];
     j.req = *req;
     j.est = est;
-    j.tuples = ir->second.n + is->second.n;
+    j.out_est = out_est;
+    j.tuples = nR + nS;
     j.t_submit = Clock::now();
-    ir->second.users++;
-    is->second.users++;
     s->pending.push_back(t);
     *ticket = t;
     s->cv_work.notify_one();

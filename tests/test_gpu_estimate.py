@@ -55,3 +55,16 @@ def test_estimate_fallback_and_empty(ctx):
     # fewer than 1000 tuples: the prefixes are clamped
     assert ctx.estimate_join_cardinality(R[:300], (S[:200] + [0, 1]).astype(np.int32)) == 0
     assert ctx.estimate_join_cardinality(np.zeros((0, 2), np.int32), S) == 0
+
+
+def test_estimate_dev(ctx):
+    import torch
+    R, S = oracle.gen_rand_data(300_000, 200_000, 50_000, 20_000, 50_000, 78)
+    dR = torch.from_numpy(R).cuda()
+    dS = torch.from_numpy(S).cuda()
+    torch.cuda.synchronize()
+    say("device estimate 300k x 200k")
+    w = want(R, S)
+    assert ctx.estimate_join_cardinality_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0]) == w
+    assert ctx.estimate_join_cardinality_dev(dS.data_ptr(), S.shape[0], dR.data_ptr(), R.shape[0]) == w
+    assert ctx.estimate_join_cardinality_dev(dR.data_ptr(), R.shape[0], 0, 0) == 0

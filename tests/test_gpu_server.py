@@ -146,3 +146,53 @@ def test_server_value_requests(dim3):
         rc, res, _, _ = srv.wait(tp)
         want, _, _ = oracle.join_project(R, S)
         assert rc == dim3.OK and np.array_equal(oracle.sort_pairs(res.pairs), oracle.sort_pairs(want))
+
+
+def test_materialize_admission_by_estimate(dim3):
+    """A materialising request is admitted with its estimated output (one row
+    per sampled join path): one key shared by 4000 tuples on each side is
+    1.6e7 rows (128 MB), more than a 64 MiB budget holds next to the
+    workspace; the fingerprint of the same query fits."""
+    n = 4000
+    R = np.stack([np.arange(n), np.zeros(n)], 1).astype(np.int32)
+    S = np.stack([np.arange(n) + 7, np.zeros(n)], 1).astype(np.int32)
+    with dim3.Server(0, hbm_budget_bytes=64 << 20, workers=1) as srv:
+        hr, hs = srv.put(R), srv.put(S)
+        rc, t = srv.submit(hr, hs, output_mode=1)
+        assert rc == dim3.OK
+        rc, res, _, _ = srv.wait(t)
+        assert rc == dim3.OK and res.n == n * n
+        rc, _ = srv.submit(hr, hs, output_mode=0)
+        assert rc == dim3.ERR_ADMISSION
+        assert srv.stats()["rejected"] == 1
+    with dim3.Server(0, hbm_budget_bytes=512 << 20, workers=1) as srv:
+        hr, hs = srv.put(R), srv.put(S)
+        rc, t = srv.submit(hr, hs, output_mode=0)
+        assert rc == dim3.OK
+        rc, res, _, _ = srv.wait(t)
+        assert rc == dim3.OK and res.n == n * n
+        want, _, _ = oracle.join_project(R, S)
+        assert np.array_equal(oracle.sort_pairs(res.pairs), oracle.sort_pairs(want))
+    say("materialise admission ok")
+
+
+def test_materialize_under_estimate(dim3):
+    """The estimate can miss every path (here: no sampled key joins and the
+    1000-tuple prefixes do not meet); the worker then grows its output
+    buffer to the size the query reports and runs it again."""
+    n = 20_000
+    i = np.arange(n)
+    R = np.stack([i, 2 * i + 100_001], 1).astype(np.int32)
+    S = np.stack([i, 2 * i + 200_001], 1).astype(np.int32)
+    R[1000:1100, 1] = 7
+    S[1000:1100, 1] = 7
+    assert oracle.estimate(R, S) == 0
+    with dim3.Server(0, workers=1) as srv:
+        hr, hs = srv.put(R), srv.put(S)
+        rc, t = srv.submit(hr, hs, output_mode=0)
+        assert rc == dim3.OK
+        rc, res, _, _ = srv.wait(t)
+        assert rc == dim3.OK and res.n == 100 * 100
+        want, _, _ = oracle.join_project(R, S)
+        assert np.array_equal(oracle.sort_pairs(res.pairs), oracle.sort_pairs(want))
+    say("under-estimate ok")
