@@ -1242,7 +1242,7 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
     if (!fp && res->n) {
         void* h = malloc(res->n * row_bytes);
         if (!h) return fail(c, DIM3_ERR_OOM, "host malloc of %llu rows", (unsigned long long)res->n);
-        hipError_t e = hipMemcpy(h, dout, res->n * row_bytes, hipMemcpyDeviceToHost);
+        hipError_t e = dim3h::copy_wait(h, dout, res->n * row_bytes, hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) {
             free(h);
             return fail(c, DIM3_ERR_DEVICE, "D2H copy: %s", hipGetErrorString(e));
@@ -1334,7 +1334,7 @@ int value_query_to_host(dim3_ctx* c, int agg, bool f32, const int2* dR, const in
     if (!fp && res->n) {
         void* h = malloc(res->n * row_bytes);
         if (!h) return fail(c, DIM3_ERR_OOM, "host malloc of %llu rows", (unsigned long long)res->n);
-        hipError_t e = hipMemcpy(h, dout, res->n * row_bytes, hipMemcpyDeviceToHost);
+        hipError_t e = dim3h::copy_wait(h, dout, res->n * row_bytes, hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) {
             free(h);
             return fail(c, DIM3_ERR_DEVICE, "D2H copy: %s", hipGetErrorString(e));

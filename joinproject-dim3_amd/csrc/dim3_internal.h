@@ -340,6 +340,14 @@ inline hipError_t dev_free_async(void* p, hipStream_t st) {
     std::shared_lock<std::shared_mutex> lk(capture_mutex());
     return hipFreeAsync(p, st);
 }
+// A copy on stream st, waited for.  Never hipMemcpy: a legacy-stream copy
+// waits for every blocking stream, and fails while another thread captures a
+// graph on one (the server's CU-masked streams are blocking).
+inline hipError_t copy_wait(void* dst, const void* src, size_t n, hipMemcpyKind k, hipStream_t st) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, k, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
 inline hipError_t host_malloc(void** p, size_t n) {
     std::shared_lock<std::shared_mutex> lk(capture_mutex());
     return hipHostMalloc(p, n, hipHostMallocDefault);

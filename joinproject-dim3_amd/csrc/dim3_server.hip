@@ -100,6 +100,7 @@ struct dim3_server {
     // stream masked to the first large_cus CUs; while one runs, the other
     // workers launch on streams masked to the remaining CUs, so neither kind
     // shares a CU with the other.  Otherwise every worker uses all CUs.
+    hipStream_t io_stream = nullptr;       // relation uploads (client threads)
     std::vector<hipStream_t> full_stream;  // per worker, all CUs
     std::vector<hipStream_t> part_stream;  // per worker: lane 0 the large part, others the rest
     uint32_t cus = 0, large_cus = 0;
@@ -230,7 +231,7 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
     j->res = res;
     if (j->status == DIM3_OK && !fp && res.n) {
         void* h = malloc(res.n * row);
-        if (!h || hipMemcpy(h, *out_buf, res.n * row, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (!h || dim3h::copy_wait(h, *out_buf, res.n * row, hipMemcpyDeviceToHost, st) != hipSuccess) {
             free(h);
             j->status = DIM3_ERR_OOM;
             j->res.n = 0;
@@ -436,11 +437,13 @@ int dim3_server_create(const dim3_server_opts* opts, dim3_server** out) {
             s->full_stream.push_back(f);
             s->part_stream.push_back(p);
         }
+        ok = ok && hipStreamCreateWithFlags(&s->io_stream, hipStreamNonBlocking) == hipSuccess;
         if (!ok) {
             for (auto x : s->full_stream)
                 if (x) hipStreamDestroy(x);
             for (auto x : s->part_stream)
                 if (x) hipStreamDestroy(x);
+            if (s->io_stream) hipStreamDestroy(s->io_stream);
             for (auto* x : s->ctxs) dim3_ctx_destroy(x);
             delete s;
             return DIM3_ERR_DEVICE;
@@ -465,6 +468,7 @@ int dim3_server_destroy(dim3_server* s) {
     for (auto* c : s->ctxs) dim3_ctx_destroy(c);
     if (s->est_ctx) dim3_ctx_destroy(s->est_ctx);
     for (auto x : s->full_stream) hipStreamDestroy(x);
+    if (s->io_stream) hipStreamDestroy(s->io_stream);
     for (auto x : s->part_stream) hipStreamDestroy(x);
     delete s;
     return DIM3_OK;
@@ -487,7 +491,7 @@ static int put_relation(dim3_server* s, const int32_t* pairs, uint64_t n, uint64
         s->resident -= bytes;
         return DIM3_ERR_OOM;
     }
-    if (n && hipMemcpy(d, pairs, n * 8, kind) != hipSuccess) {
+    if (n && dim3h::copy_wait(d, pairs, n * 8, kind, s->io_stream) != hipSuccess) {
         dim3h::dev_free(d);
         std::lock_guard<std::mutex> lk(s->mu);
         s->resident -= bytes;
@@ -532,7 +536,7 @@ int dim3_server_put_values(dim3_server* s, uint64_t handle, const void* values, 
     if (dim3h::dev_malloc((void**)&d, bytes) != hipSuccess) {
         (void)hipGetLastError();
         rc = DIM3_ERR_OOM;
-    } else if (n && hipMemcpy(d, values, n * 4, hipMemcpyDefault) != hipSuccess) {
+    } else if (n && dim3h::copy_wait(d, values, n * 4, hipMemcpyDefault, s->io_stream) != hipSuccess) {
         dim3h::dev_free(d);
         rc = DIM3_ERR_DEVICE;
     }
