@@ -724,15 +724,15 @@ int prepare_map(dim3_ctx* c, const dim3_opts& o, hipStream_t st) {
     dim3h::fill_u32(degX, nx1, 0, st);
     // the most frequent key of each side: its tuples share one atomic per
     // wavefront in the key-counting kernels
-    dim3h::hot_keys(dR, nR, dS, nS, sc->hot, st);
-    dim3h::build_y(dR, nR, P.my, degR, slotR, st, sc->hot);
-    dim3h::probe_y(dS, nS, P.my, degR, degS, slotS, &sc->s_surv, st, sc->hot);
+    const int32_t* hot = dim3h::hot_keys(dR, nR, dS, nS, sc->hot, st) ? sc->hot : nullptr;
+    dim3h::build_y(dR, nR, P.my, degR, slotR, st, hot);
+    dim3h::probe_y(dS, nS, P.my, degR, degS, slotS, &sc->s_surv, st, hot);
     dim3h::scan_y(degR, degS, (uint32_t)ny1, exY, part, &sc->tot_y, &sc->out_j, st);
     if (P.agg_fp) {
         uint64_t *kr, *gs;
         TRY(ws(c, WS_AGG_KR, ny1, &kr));
         TRY(ws(c, WS_AGG_GS, ny1, &gs));
-        dim3h::path_sums(dR, nR, slotR, dS, nS, slotS, (uint32_t)ny1, kr, gs, &sc->agg_sum2, st, sc->hot);
+        dim3h::path_sums(dR, nR, slotR, dS, nS, slotS, (uint32_t)ny1, kr, gs, &sc->agg_sum2, st, hot);
     }
     dim3h::build_x(dR, nR, slotR, degS, P.mx, degX, slotRx, st);
     dim3h::scan_deg(degX, (uint32_t)nx1, exX, part, &sc->tot_x, st);

@@ -360,6 +360,7 @@ inline hipError_t host_free(void* p) {
 
 // Device scalar block, zeroed at the start of every query.
 constexpr int kHotKeys = 15, kHotSlots = kHotKeys + 1;
+constexpr uint64_t kHotMinTuples = 1ull << 20;  // smaller relations take no hot-key sample
 
 struct Scalars {
     uint64_t tot_y;      // scan total of the y table: (ny << 32) | 0
@@ -432,8 +433,9 @@ void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm,
 // hot (nullable): {R's most frequent key, valid, S's, valid} (hot_keys)
 // The kHotKeys most frequent join keys of each side among 4096 evenly spaced
 // tuples (those seen at least 8 times): hot[0] = count, hot[1..] = keys; S's
-// list starts at hot + kHotSlots.
-void hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st);
+// list starts at hot + kHotSlots.  Returns false, writing nothing, when
+// neither side has kHotMinTuples tuples.
+bool hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st);
 void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
              const int32_t* hot = nullptr);
 void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, uint32_t* degS, int32_t* slotS,

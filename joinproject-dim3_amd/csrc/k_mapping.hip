@@ -158,11 +158,13 @@ __global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, ui
     constexpr uint32_t kSample = 4096, kSlots = 8192, kMinSeen = 8;
     __shared__ uint32_t keys[kSlots];
     __shared__ uint32_t cnt[kSlots];
-    __shared__ uint64_t best[1024 / 64];
+    __shared__ uint64_t cand[kSample / kMinSeen];
+    __shared__ uint32_t ncand;
     for (uint32_t k = threadIdx.x; k < kSlots; k += 1024) {
         keys[k] = 0xFFFFFFFFu;
         cnt[k] = 0;
     }
+    if (threadIdx.x == 0) ncand = 0;
     __syncthreads();
     const uint64_t m = n < kSample ? n : kSample;
     const uint64_t step = m ? n / m : 0;  // index k * step < m * step <= n
@@ -179,28 +181,21 @@ __global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, ui
         atomicAdd(&cnt[s], 1u);
     }
     __syncthreads();
-    int found = 0;
-    for (int r = 0; r < dim3h::kHotKeys; r++) {
-        // largest count (ties: the largest slot), packed as count << 32 | slot
-        uint64_t b = 0;
-        for (uint32_t k = threadIdx.x; k < kSlots; k += 1024)
-            if (cnt[k] >= kMinSeen) b = max(b, ((uint64_t)cnt[k] << 32) | k);
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) b = max(b, (uint64_t)__shfl_xor((unsigned long long)b, d, 64));
-        if (lane_id() == 0) best[threadIdx.x >> 6] = b;
-        __syncthreads();
-        uint64_t w = 0;
-        for (uint32_t v = 0; v < 1024 / 64; v++) w = max(w, best[v]);
-        __syncthreads();  // best is rewritten next round; cnt[slot] is cleared below
-        if (w == 0) break;  // uniform: every thread read the same best[]
-        if (threadIdx.x == 0) {
-            out[1 + found] = (int32_t)keys[(uint32_t)w];
-            cnt[(uint32_t)w] = 0;
-        }
-        found++;
-        __syncthreads();
+    // candidates: slots seen at least kMinSeen times (at most
+    // kSample / kMinSeen), packed as count << 32 | slot (distinct)
+    for (uint32_t k = threadIdx.x; k < kSlots; k += 1024)
+        if (cnt[k] >= kMinSeen) cand[atomicAdd(&ncand, 1u)] = ((uint64_t)cnt[k] << 32) | k;
+    __syncthreads();
+    const uint32_t nc = ncand;
+    // a candidate's rank is the number of larger ones: the first kHotKeys go
+    // out, most frequent first
+    if (threadIdx.x < nc) {
+        const uint64_t me = cand[threadIdx.x];
+        uint32_t rank = 0;
+        for (uint32_t k = 0; k < nc; k++) rank += cand[k] > me ? 1u : 0u;
+        if (rank < (uint32_t)dim3h::kHotKeys) out[1 + rank] = (int32_t)keys[(uint32_t)me];
     }
-    if (threadIdx.x == 0) out[0] = found;
+    if (threadIdx.x == 0) out[0] = (int32_t)min(nc, (uint32_t)dim3h::kHotKeys);
 }
 
 // mapping_y probe side (dim3.hpp:830-836): S tuples whose y is not in R are
@@ -626,9 +621,13 @@ void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm,
     if (nR) hipLaunchKernelGGL(k_minmax, dim3(gr), dim3(kThreads), 0, st, R, nR, mm);
     if (nS) hipLaunchKernelGGL(k_minmax, dim3(gs), dim3(kThreads), 0, st, S, nS, mm + 4);
 }
-void hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st) {
+bool hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st) {
+    // below 2^20 tuples a side's hottest keys serialise for well under a
+    // millisecond: no sample
+    if (nR < kHotMinTuples && nS < kHotMinTuples) return false;
     hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, R, nR, hot);
     hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, S, nS, hot + kHotSlots);
+    return true;
 }
 void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
              const int32_t* hot) {
