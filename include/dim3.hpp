@@ -7,6 +7,7 @@
 //   DIM3<_Tx,_Ty,_Tz,_Hx,_Hy,_Hz,_Hxz,_Tcounter>::doJoinProject          dim3.hpp:982-1055
 //   DIM3_parallel<...>::doJoinProject_parallel                           dim3_parallel.hpp:137-209
 //   DIM3JoinAggregate<_Tx,_Ty,_Tz,_Tv,_Agg,...>::doJoinAggregate         dim3.hpp:1313-1373
+//   EstimateJoinCardinality<_Tx,_Ty,_Tz,_H,_Tcounter>::estimate          dim3.hpp:190-270
 //   myvector<T,_CountType> (same layout: data, cur_num, max_num)         Include/myvector.hpp:9-86
 //   TUPLE<_Tx,_Ty,_Tv> (same layout: x, y, v)                            dim3.hpp:20-33
 //   myHasher (placeholder: hashing happens on the device)                Include/structures.hpp:92
@@ -191,6 +192,24 @@ struct DIM3 {
                               const myvector<pair<_Tz, _Ty>, _Tcounter> _S,
                               myvector<pair<_Tx, _Tz>, _Tcounter>& result) {
         dim3_shim::join_project(_R, _S, result);
+    }
+};
+
+// The reference's sampled join size (keys whose low sample bits are zero
+// under the identity hash, the 1000-tuple-prefix fallback), computed on the
+// device: the same number.  Where the reference asserts |R| >= |S|, the
+// library swaps the sides.
+template <typename _Tx, typename _Ty, typename _Tz, typename _H = myHasher, typename _Tcounter = uint32_t>
+struct EstimateJoinCardinality {
+    static long long estimate(const myvector<pair<_Tx, _Ty>, _Tcounter> R,
+                              const myvector<pair<_Tz, _Ty>, _Tcounter> S) {
+        static_assert(dim3_shim::is_i32<_Tx>() && dim3_shim::is_i32<_Ty>() && dim3_shim::is_i32<_Tz>(),
+                      "libdim3 joins 4-byte integer attributes");
+        int64_t est = 0;
+        const int rc = dim3_estimate_join_cardinality(dim3_shim::context(), (const int32_t*)R.data, R.size(),
+                                                      (const int32_t*)S.data, S.size(), &est);
+        if (rc != DIM3_OK) dim3_shim::die("EstimateJoinCardinality::estimate", rc);
+        return (long long)est;
     }
 };
 

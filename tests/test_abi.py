@@ -102,6 +102,8 @@ int main() {
     myvector<pair<int, int>> R, S, o;
     gen_rand_data(R, S, 10, 10);
     DIM3<int, int, int, myHasher, myHasher, myHasher, myHasher, uint32_t>::doJoinProject(R, S, o);
+    long long est = EstimateJoinCardinality<int, int, int, myHasher, uint32_t>::estimate(R, S);
+    (void)est;
     return 0;
 }
 ''')
