@@ -613,13 +613,15 @@ int dim3_submit(dim3_server* s, const dim3_request* req, uint64_t* ticket) {
         const int32_t *dR = ir->second.d, *dS = is->second.d;
         lk.unlock();
         int64_t paths = 0;
-        const int rc = s->estimate_paths(dR, nR, dS, nS, &paths);
+        // advisory: when the estimate cannot run (no memory for its table),
+        // the request is admitted on its workspace alone and its worker
+        // sizes the output from the query itself
+        if (s->estimate_paths(dR, nR, dS, nS, &paths) != DIM3_OK) paths = 0;
         lk.lock();
-        const bool gone = s->rels.at(req->r).dropped || s->rels.at(req->s).dropped;
-        if (rc != DIM3_OK || gone) {
+        if (s->rels.at(req->r).dropped || s->rels.at(req->s).dropped) {
             s->unpin(req->r);
             s->unpin(req->s);
-            return rc != DIM3_OK ? rc : DIM3_ERR_NOT_FOUND;
+            return DIM3_ERR_NOT_FOUND;
         }
         const double rows = std::min((double)std::max<int64_t>(paths, 0), (double)nR * (double)nS);
         out_est = rows * row > 1e18 ? (uint64_t)1e18 : (uint64_t)(rows * row);
