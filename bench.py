@@ -33,6 +33,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """CPU cores this process may run on (the reference's OpenMP team is
+    fixed at THREAD_NUM = 8 threads whatever the host has)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count()
+
+
 def z_lookup(S):
     """True when the z values of S are hash-mapped (their span exceeds
     2 |S| + 64), so that each output row reads its z value from a table;
@@ -98,7 +107,7 @@ def cpu_baseline(E, target_s, slices, S=None, self_filter=True):
         log("cpu baseline slice %d/%d: %d pairs in %.2fs" % (r, slices, n, dt))
         if total_t >= target_s:
             break
-    return {"value": total_pairs / total_t, "unit": "pairs/s", "cores": threads, "kind": "reference",
+    return {"value": total_pairs / total_t, "unit": "pairs/s", "cores": threads, "host_cores": host_cores(), "kind": "reference",
             "sample": "reference DIM3_parallel (oracle/_ref, %d OpenMP threads) on %d of %d vertex slices "
                       "(R restricted to src %% %d == r, all of S), %d pairs in %.2f s%s"
                       % (threads, len(done), slices, slices, total_pairs, total_t,
@@ -250,7 +259,7 @@ def aggregate_cpu_baseline(R, S, target_s, slices):
         log("cpu baseline slice %d/%d: %d groups in %.2fs" % (r, slices, total, dt))
         if total_t >= target_s:
             break
-    return {"value": total / total_t, "unit": "pairs/s", "cores": 1, "kind": "reference",
+    return {"value": total / total_t, "unit": "pairs/s", "cores": 1, "host_cores": host_cores(), "kind": "reference",
             "sample": "reference DIM3JoinAggregate with a COUNT policy (oracle/_ref, one thread) on %d of %d z "
                       "slices (S restricted to c %% %d == r, all of R), %d groups in %.2f s"
                       % (done, slices, slices, total, total_t)}
@@ -315,7 +324,7 @@ def value_bench(args, d, torch):
                 t0 = time.perf_counter()
                 rp, rv = oracle.ref_join_aggregate_f32(R, vR, S, vS, "avg")
                 dt = time.perf_counter() - t0
-                cpu = {"value": rp.shape[0] / dt, "unit": "pairs/s", "cores": 1, "kind": "reference",
+                cpu = {"value": rp.shape[0] / dt, "unit": "pairs/s", "cores": 1, "host_cores": host_cores(), "kind": "reference",
                        "sample": "the whole query: reference DIM3JoinAggregate with its AggregateFunctionAvg<float> "
                                  "(oracle/_ref, one thread), %d groups in %.2f s" % (rp.shape[0], dt)}
                 del rp, rv
@@ -595,7 +604,7 @@ def batch_cpu_baseline(params, dev, target_s):
         del out
         if spent >= target_s:
             break
-    return {"value": pairs / spent, "unit": "pairs/s", "cores": threads, "kind": "reference",
+    return {"value": pairs / spent, "unit": "pairs/s", "cores": threads, "host_cores": host_cores(), "kind": "reference",
             "sample": "reference DIM3_parallel (oracle/_ref, %d OpenMP threads) on the %d smallest of %d "
                       "instances, %d pairs in %.2f s" % (threads, done, len(params), pairs, spent)}
 

@@ -147,6 +147,7 @@ struct QueryPlan {
     uint64_t z_span = 0;  // dense z IDs (range mapping): the 32-aligned span of z values
     bool all_z = false;  // every z ID is a product column
     bool fused = false;  // bit-or product read from the key bitmaps by the seeding (no product rows)
+    bool plain = false;  // project tiles in the plain bit layout (word seeding / word fingerprints), else swizzled
     // dense core: rows / columns with at least thx / thz heavy entries
     bool core_try = false;     // analyse the core of the keys with bin >= bc
     uint32_t bc = 0;
@@ -188,7 +189,7 @@ struct DenseModel {
     double seed_word_s = 1.0e-12;  // seeding, per 32-bit word copied
     double mfma_ops = 1.5e15;      // int8 ops/s reached by k_gemm_i8
     double hbm_bps = 5.0e12;       // streaming bytes/s
-    double l2_bps = 10.0e12;       // bit-or re-reads of key bitmaps
+    double l2_bps = 10.0e12;       // bit-or re-reads of key bitmaps (mostly L2 hits)
     double launch_s = 60e-6;       // fixed cost of the dense launches
 };
 
@@ -395,11 +396,22 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
 struct DenseShape {
     uint64_t K = 0, Mh = 0, Nh = 0, paths = 0, sumR = 0, sumS = 0;
     uint64_t nz = 0;  // all z IDs: product columns covering nearly all of them seed whole words
+    uint64_t tile_words = 0, ntiles = 0;  // project tile shape: a fused key bitmap holds ntiles * tile_words words
 };
 
-// seconds the dense part takes for one shape and mode, <0 if it cannot run
+// LDS bandwidth of a fused seed's tile updates (one read-or-write of every
+// seeded tile word, all compute units)
+constexpr double kLdsBps = 50e12;
+
+// Seconds the dense part takes for one shape and mode, <0 if it cannot run.
+// Join-project bit-or has two forms: fused (any_fused, or the columns are
+// nearly all z IDs) keeps one bitmap per heavy key over every z ID, in the
+// expansion tile's layout, and each row tile ORs its keys' words straight
+// into LDS (no product rows); the materialised form writes product rows over
+// the columns only and seeds them bit by bit.  *fused_out reports the form.
 double dense_cost(const DenseModel& m, const DenseShape& d, int mode, bool count, uint64_t mem_room,
-                  uint64_t* bytes_out) {
+                  uint64_t* bytes_out, bool any_fused = false, bool* fused_out = nullptr) {
+    if (fused_out) *fused_out = false;
     if (d.K == 0 || d.Mh == 0 || d.Nh == 0) return -1;
     const uint64_t Kp = (d.K + 63) / 64 * 64;
     const uint64_t dwords = (d.Nh + 31) / 32;
@@ -408,27 +420,45 @@ double dense_cost(const DenseModel& m, const DenseShape& d, int mode, bool count
     uint64_t bytes = 0;
     double t = 0;
     if (mode != DIM3_DENSE_MFMA && mode != DIM3_DENSE_BITOR) return -1;
+    const bool words = !count && d.Nh * 10 >= d.nz * 9;
     if (mode == DIM3_DENSE_MFMA) {
         if (d.Mh >= 65535ull * 128ull || d.Nh >= (1ull << 31)) return -1;
         if (count && Kp > 131072) return -1;  // int32 accumulator bound for multiplicities up to 127
         bytes = (d.Mh + d.Nh) * Kp + (count ? d.Mh * d.Nh * 4 : d.Mh * dwords * 4);
         t = 2.0 * cells * (double)Kp / m.mfma_ops + (double)((d.Mh + d.Nh) * Kp) / m.hbm_bps;
+        if (bytes > mem_room) return -1;
+        // product written once and read back by the seeding, plus one LDS op
+        // per set entry (per word when the columns are nearly all z IDs: word
+        // copies)
+        t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps;
+        t += (words ? cells / 32.0 * m.seed_word_s : set_bits * m.seed_s) + m.launch_s;
+        if (bytes_out) *bytes_out = bytes;
+        return t;
     }
-    const bool words = !count && d.Nh * 10 >= d.nz * 9;
-    // bit-or over (nearly) all z IDs is fused into the seeding: the product
-    // rows are never written, each row tile ORs its keys' bitmap words
-    const bool fused = mode == DIM3_DENSE_BITOR && words;
-    if (mode == DIM3_DENSE_BITOR) {
-        if (count) return -1;
-        bytes = d.K * dwords * 4 + (fused ? 0 : d.Mh * dwords * 4);
-        t = (double)d.sumR * (double)dwords * 4.0 / m.l2_bps + (double)(d.K * dwords * 4) / m.hbm_bps;
+    if (count) return -1;
+    double tf = -1, tm = -1;
+    uint64_t bf = 0, bm = 0;
+    if (words || any_fused) {
+        // fused: every row tile of a row with heavy keys ORs its keys' words
+        // (L2 / MALL reads) into its LDS tile
+        const uint64_t fw = std::max<uint64_t>(d.ntiles * d.tile_words, (d.nz + 31) / 32);
+        bf = d.K * fw * 4;
+        tf = (double)d.sumR * (double)fw * 4.0 / m.l2_bps + (double)bf / m.hbm_bps +
+             (double)d.Mh * (double)fw * 4.0 / kLdsBps + m.launch_s;
+        if (bf > mem_room) tf = -1;
     }
-    if (bytes > mem_room) return -1;
-    // product written once and read back by the seeding (unless fused), plus
-    // one LDS op per set entry (per word when the columns are nearly all z
-    // IDs: word copies)
-    if (!fused) t += 2.0 * (count ? cells * 4.0 : cells / 8.0) / m.hbm_bps;
-    t += (words ? cells / 32.0 * m.seed_word_s : set_bits * m.seed_s) + m.launch_s;
+    if (!words) {
+        // materialised product rows over the columns, seeded bit by bit
+        bm = d.K * dwords * 4 + d.Mh * dwords * 4;
+        tm = (double)d.sumR * (double)dwords * 4.0 / m.l2_bps + (double)(d.K * dwords * 4) / m.hbm_bps +
+             2.0 * cells / 8.0 / m.hbm_bps + set_bits * m.seed_s + m.launch_s;
+        if (bm > mem_room) tm = -1;
+    }
+    const bool fused = tf >= 0 && (tm < 0 || tf <= tm);
+    t = fused ? tf : tm;
+    bytes = fused ? bf : bm;
+    if (t < 0) return -1;
+    if (fused_out) *fused_out = fused;
     if (bytes_out) *bytes_out = bytes;
     return t;
 }
@@ -445,9 +475,12 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     // suffix sums from the heaviest bin down
     DenseShape acc;
     acc.nz = P.nz;
+    acc.tile_words = P.tile_words;
+    acc.ntiles = P.ntiles;
     int best_mode = DIM3_DENSE_OFF;
     uint32_t best_b = 0;
     double best_gain = -1e300;
+    bool best_fused = false;
     DenseShape best_shape;
     for (int b = (int)NB - 1; b >= 1; b--) {
         acc.K += h[b];
@@ -461,14 +494,17 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
         const int modes[2] = {DIM3_DENSE_MFMA, DIM3_DENSE_BITOR};
         int bm = DIM3_DENSE_OFF;
         double bg = -1e300;
+        bool bf = false;
         for (int mode : modes) {
             if (o.dense_mode > 0 && o.dense_mode != mode) continue;
-            const double t = dense_cost(m, acc, mode, count, room, nullptr);
+            bool fz = false;
+            const double t = dense_cost(m, acc, mode, count, room, nullptr, true, &fz);
             if (t < 0) continue;
             const double gain = (double)acc.paths * m.path_s - t;
             if (gain > bg) {
                 bg = gain;
                 bm = mode;
+                bf = fz;
             }
         }
         if (bm == DIM3_DENSE_OFF) continue;
@@ -478,6 +514,7 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
             best_mode = bm;
             best_b = (uint32_t)b;
             best_shape = acc;
+            best_fused = bf;
         }
     }
     // dense-core candidate: the keys whose R and S degrees are both >= 64, when
@@ -511,9 +548,10 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     P.Kp = (P.K + 63u) / 64u * 64u;
     P.Mh = (uint32_t)best_shape.Mh;
     P.Nh = (uint32_t)best_shape.Nh;
-    // columns covering nearly every z: take them all, so that each tile's
-    // columns are its consecutive z IDs and the seeding copies words
-    P.all_z = !P.count_mode && (uint64_t)P.Nh * 10 >= (uint64_t)P.nz * 9;
+    // columns covering nearly every z (or a fused bit-or, whose key bitmaps
+    // span every z ID): take them all, so that each tile's columns are its
+    // consecutive z IDs and the seeding copies words
+    P.all_z = !P.count_mode && ((uint64_t)P.Nh * 10 >= (uint64_t)P.nz * 9 || (best_mode == DIM3_DENSE_BITOR && best_fused));
     if (P.all_z) P.Nh = P.nz;
     P.dwords = ((P.Nh + 31u) / 32u + 3u) & ~3u;  // rows of 16-byte multiples
 }
@@ -591,6 +629,21 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
     if (P.all_z) P.Nh = P.nz;
     P.dwords = ((P.Nh + 31u) / 32u + 3u) & ~3u;  // rows of 16-byte multiples
     return true;
+}
+
+// Tile bit layout and fused key-bitmap width of a planned query.  Project
+// tiles are swizzled (hot z IDs spread over words and banks) unless whole
+// words pay more: a fingerprint over a dense z range (a plain word sums its
+// 32 z factors with one word factor and four byte-table lookups: the two-hop
+// graph's fingerprint query 64.8 -> 42.5 ms, all-sparse), or an MFMA product
+// whose columns are every z ID (word copies into the tile).  A fused bit-or
+// keeps one bitmap per heavy key in the tile layout, ntiles * tile_words
+// words, so that a row tile ORs words t * tile_words .. + tile_words of each
+// of its keys.
+void set_layout(QueryPlan& P, const dim3_opts& o) {
+    P.plain = (o.output_mode == DIM3_OUT_FINGERPRINT && P.z_ident) ||
+              (P.dense == DIM3_DENSE_MFMA && P.all_z);
+    if (P.fused) P.dwords = P.ntiles * P.tile_words;
 }
 
 // Mapping + CSR of one query.  Leaves the plan in c->bank().plan.
@@ -806,6 +859,7 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     plan_tiles(c, o, force_wgs);
     plan_dense(c, o);
     P.fused = P.dense == DIM3_DENSE_BITOR && P.all_z;
+    set_layout(P, o);
     if ((uint64_t)P.ny * P.ntiles >= (1ull << 31))
         return fail(c, DIM3_ERR_INVALID, "tiled S index does not fit 32 bits (ny=%u, tiles=%u)", P.ny, P.ntiles);
 
@@ -850,6 +904,7 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
             HIPCHK(c, hipStreamSynchronize(st));
             const bool core = plan_core(c, o);
             P.fused = P.dense == DIM3_DENSE_BITOR && P.all_z;
+            set_layout(P, o);
             if (!core && P.dense != DIM3_DENSE_OFF) {  // back to the phase-1 keys
                 HIPCHK(c, hipMemsetAsync(counters, 0, 8, st));
                 dim3h::dense_mark_keys(degR, degS, (uint32_t)ny1, exY, P.bmin, hid, h2y, counters, st);
@@ -925,8 +980,11 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
         } else {
             TRY(ws(c, WS_DB, (uint64_t)P.K * P.dwords, &B));
             dim3h::fill_u32(B, (uint64_t)P.K * P.dwords, 0, st);
+            // fused: the tile layout (row tile t ORs words t W .. t W + W - 1);
+            // product rows: plain bitmaps over the columns
             dim3h::dense_build_b(2, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY, hid, zflag, hzpos,
-                                 0, P.dwords, B, overflow, st);
+                                 0, P.dwords, B, overflow, st, P.fused ? P.tile_words : 0u,
+                                 P.fused && !P.plain ? P.tile_words / 32u : 0u);
             if (!P.fused) {  // fused: the seeding reads B itself
                 const int e = dim3h::bitor_rows(rowR, colR, hid, hx2x, P.Mh, B, P.dwords, dbits, st);
                 if (e != 0)
@@ -983,7 +1041,7 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.stage_cap = P.stage_cap;
     // plain bit layout only where the dense seed copies whole words (every z
     // ID a product column); elsewhere the swizzle keeps hub z off one word
-    a.swz = (P.dense == DIM3_DENSE_OFF || !P.all_z) ? 1u : 0u;
+    a.swz = P.plain ? 0u : 1u;
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
     if (P.core) a.splitS = (const uint32_t*)W[WS_CNT_SD].p;
@@ -998,6 +1056,8 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
         } else if (P.fused) {
             a.bkeys = (const uint32_t*)W[WS_DB].p;
             a.hid = (const int32_t*)W[WS_HID].p;
+            // an XCD's L2 is 4 MiB: larger key bitmaps are read tile by tile
+            a.tile_major = (uint64_t)P.K * P.dwords * 4u > (4ull << 20) ? 1u : 0u;
         } else {
             a.dbits = (const uint32_t*)W[WS_DBITS].p;
         }

@@ -526,6 +526,7 @@ struct ExpandArgs {
     // a materialised product row
     const uint32_t* bkeys;    // K x dwords key bitmaps over z IDs (nullptr: seed from dbits / cint)
     const int32_t* hid;       // heavy index of each y ID
+    uint32_t tile_major;      // fused seed: take the tasks tile by tile (key bitmaps larger than an L2)
     // value aggregates: the tuple value of each R / S CSR entry
     const int32_t* valR;
     const int32_t* valS;
@@ -582,11 +583,13 @@ void dense_columns(const uint32_t* zflag, const uint32_t* hzpos, uint32_t nz, ui
 void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid,
                    const uint32_t* hx2x, uint32_t Mh, uint32_t Kp, uint32_t* A, uint64_t* overflow, hipStream_t st);
 // from the heavy S tuples: kind 0: Bt [Nh][Kp] 0/1 bytes, 1: Bt byte
-// multiplicities, 2: bits of per-key z bitmaps [K][dwords]
+// multiplicities, 2: bits of per-key z bitmaps [K][dwords] (tile_words > 0:
+// in the layout of project tiles of tile_words words, swizzled with stride
+// swz_c, 0 = plain)
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
                    int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* zflag,
                    const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
-                   hipStream_t st);
+                   hipStream_t st, uint32_t tile_words = 0, uint32_t swz_c = 0);
 
 // bound[x] = min(cap, the S paths of row x plus the product columns of a
 // product row) >= min(cap, output rows of row x); cap <= nz

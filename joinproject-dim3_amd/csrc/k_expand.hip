@@ -122,7 +122,131 @@ struct Lds {
     uint32_t* s_wtot;
     uint32_t* s_nt;
     uint64_t* s_gbase;
+    uint32_t* s_cursor;  // rows of the task handed out so far (single-pass emission)
 };
+
+// Emission of a swept join-project task (rows too dense for the touched
+// list) in one pass, without a counting sweep: the block sum of the first-set
+// bits (nb) is the row count, reserved with one global atomic; then every
+// wavefront walks its slice of the tile 64 words at a time, clears them,
+// places the chunk's rows by a wavefront prefix of the word popcounts into
+// its own LDS stage region (the touched list and the area, free here) and
+// flushes full regions with coalesced 8-byte stores at offsets handed out by
+// an LDS cursor.  No block barrier between the reservation and the end.
+// Returns false, reserving nothing, for a row denser than a quarter of its
+// tile: its rows go through the block-staged emission instead (balanced
+// per-thread shares written block-wide, measured faster there: 1M x 1M Zipf
+// join-project, 69% dense rows).
+template <int BLOCK, bool ZID>
+__device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, int32_t xv, uint32_t zbase, uint32_t nb,
+                                           uint32_t self_item, uint32_t self_keep, uint32_t wlo, uint32_t whi) {
+    constexpr uint32_t NW = BLOCK / 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    uint32_t* tile = L.tile;
+    const uint32_t W = a.tile_words;
+    const uint32_t C = a.swz ? W / 32u : 0u;
+    const uint32_t wsum = wave_sum_u32(nb);
+    if (lane == 0) L.s_wtot[wid] = wsum;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t T = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < NW; w++) T += L.s_wtot[w];
+        if (self_item != 0xFFFFFFFFu) {  // the u != w filter: drop the row's own z
+            const uint32_t v = tile[self_item];
+            if (v & ~self_keep) {
+                tile[self_item] = v & self_keep;
+                T -= 1u;
+            }
+        }
+        const bool dense = (uint64_t)T * 4u >= (uint64_t)a.tile_size;
+        *L.s_gbase = dense ? ~0ull : T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
+        *L.s_cursor = 0u;
+    }
+    __syncthreads();
+    const uint64_t gbase = *L.s_gbase;
+    if (gbase == ~0ull) return false;
+    const uint32_t region = a.touched_cap + max(a.stage_cap, 3u * (uint32_t)BLOCK + 1u);
+    const uint32_t SW = region / NW;
+    uint32_t* stg = L.touched + wid * SW;
+    const uint32_t zoff = (uint32_t)a.zlo + zbase;
+    const int32_t* id2z = a.id2z + zbase;
+    int2* out = reinterpret_cast<int2*>(a.out);
+    uint32_t fill = 0;  // wavefront-uniform
+    auto flush = [&](uint32_t n) {
+        uint32_t off = 0;
+        if (lane == 0) off = atomicAdd(L.s_cursor, n);
+        off = (uint32_t)__builtin_amdgcn_readfirstlane((int)off);
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t g0 = gbase + off;
+        if (g0 + n <= a.cap) {
+            // four rows per lane per round: the four LDS reads are in flight
+            // together, then four coalesced stores
+            int2* o = out + g0;
+            uint32_t k = lane;
+            for (; k + 192u < n; k += 256u) {
+                const uint32_t e0 = stg[k], e1 = stg[k + 64u], e2 = stg[k + 128u], e3 = stg[k + 192u];
+                put_pair(o + k, xv, ZID ? (int32_t)(zoff + e0) : id2z[e0]);
+                put_pair(o + k + 64u, xv, ZID ? (int32_t)(zoff + e1) : id2z[e1]);
+                put_pair(o + k + 128u, xv, ZID ? (int32_t)(zoff + e2) : id2z[e2]);
+                put_pair(o + k + 192u, xv, ZID ? (int32_t)(zoff + e3) : id2z[e3]);
+            }
+            for (; k < n; k += 64u) {
+                const uint32_t e = stg[k];
+                put_pair(o + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+            }
+        } else {
+            for (uint32_t k = lane; k < n; k += 64u) {
+                const uint32_t e = stg[k];
+                if (g0 + k < a.cap) put_pair(out + g0 + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+    for (uint32_t i0 = wlo; i0 < whi; i0 += 64u) {
+        const uint32_t w = i0 + lane;
+        uint32_t v = 0;
+        if (w < whi) {
+            v = tile[w];
+            if (v) tile[w] = 0u;
+        }
+        const uint32_t cnt = (uint32_t)__popc(v);
+        const uint32_t incl = wave_scan_u32(cnt);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        if (tot == 0u) continue;
+        if (fill + tot > SW) {
+            flush(fill);
+            fill = 0;
+        }
+        if (tot > SW) {
+            // a chunk denser than the stage region: each lane writes its own rows
+            uint32_t off = 0;
+            if (lane == 0) off = atomicAdd(L.s_cursor, tot);
+            off = (uint32_t)__builtin_amdgcn_readfirstlane((int)off);
+            uint64_t pos = gbase + off + incl - cnt;
+            while (v) {
+                const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                v &= v - 1u;
+                const uint32_t e = swz_offset(w, b, W, C);
+                if (pos < a.cap) put_pair(out + pos, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                ++pos;
+            }
+            continue;
+        }
+        uint32_t p = fill + incl - cnt;
+        while (v) {
+            const uint32_t b = (uint32_t)__ffs(v) - 1u;
+            v &= v - 1u;
+            stg[p++] = swz_offset(w, b, W, C);
+        }
+        fill += tot;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (fill) flush(fill);
+    __syncthreads();
+    return true;
+}
 
 template <int BLOCK, bool COUNT, bool ZID>
 __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32_t t, const Lds& L, uint64_t& fsum,
@@ -148,14 +272,19 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const int32_t xv = a.id2x[x];
     const int32_t zs_self = a.x2z != nullptr ? a.x2z[x] : -1;
 
+    // Bits this thread set first (seed and paths; join-project): their block
+    // sum is the task's row count, so an untracked task needs no counting
+    // sweep before it emits.
+    uint32_t nb = 0;
     // ---- dense seed: a row with a heavy key starts from its product row ----
     const int32_t hxr = a.hx != nullptr ? a.hx[x] : -1;
     if (hxr >= 0) {
         const uint32_t h0 = a.hz_tile[t], h1 = a.hz_tile[t + 1];
         if (!COUNT && a.bkeys != nullptr) {
-            // fused bit-or: the OR of the row's heavy key bitmaps (plain
-            // layout, every z ID a column: tile word q is bitmap word t W + q).
-            // The heavy keys are listed BLOCK entries at a time.
+            // fused bit-or: the OR of the row's heavy key bitmaps, kept over
+            // every z ID in the tile's own bit layout (plain or swizzled), so
+            // tile word q is bitmap word t W + q.  The heavy keys are listed
+            // BLOCK entries at a time.
             uint32_t* s_hk = L.area;
             const uint32_t wb = t * W;
             const uint32_t wn = wb < a.dwords ? min(W, a.dwords - wb) : 0u;
@@ -170,31 +299,55 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 const uint32_t pos = block_exclusive_scan<BLOCK, uint32_t>(h >= 0 ? 1u : 0u, nh, sh32);
                 if (h >= 0) s_hk[pos] = (uint32_t)h;
                 __syncthreads();
+                if (nh == 0u) continue;  // block-uniform; nobody reads s_hk
                 if ((W & 3u) == 0u && (a.dwords & 3u) == 0u) {
                     // 16-byte loads: tile rows and bitmap rows start on
-                    // 16-byte boundaries (W and dwords multiples of 4 words)
+                    // 16-byte boundaries (W and dwords multiples of 4 words).
+                    // Four 16-byte positions per thread and two keys per
+                    // round: eight loads in flight (L2 / MALL latency, not
+                    // bandwidth, bounds the seed of a few keys).
                     const uint32_t wn4 = wn >> 2;
                     uint4* tile4 = reinterpret_cast<uint4*>(tile);
-                    for (uint32_t q0 = tid; q0 < wn4; q0 += 2u * BLOCK) {
-                        const uint32_t q1 = q0 + BLOCK;
-                        const bool ok1 = q1 < wn4;
-                        uint4 c0 = make_uint4(0u, 0u, 0u, 0u), c1 = make_uint4(0u, 0u, 0u, 0u);
-                        for (uint32_t j = 0; j < nh; j++) {
-                            const uint4* src = reinterpret_cast<const uint4*>(a.bkeys + (size_t)s_hk[j] * a.dwords + wb);
-                            const uint4 v0 = src[q0];
-                            const uint4 v1 = src[ok1 ? q1 : q0];  // clamped, unconditional
-                            c0.x |= v0.x; c0.y |= v0.y; c0.z |= v0.z; c0.w |= v0.w;
-                            c1.x |= v1.x; c1.y |= v1.y; c1.z |= v1.z; c1.w |= v1.w;
+                    const uint4* base4 = reinterpret_cast<const uint4*>(a.bkeys + wb);
+                    const size_t kstride4 = a.dwords >> 2;
+                    for (uint32_t q0 = tid; q0 < wn4; q0 += 4u * BLOCK) {
+                        uint32_t qq[4];
+                        bool okq[4];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++) {
+                            const uint32_t q = q0 + u * BLOCK;
+                            okq[u] = q < wn4;
+                            qq[u] = okq[u] ? q : q0;  // clamped, unconditional
                         }
-                        if ((c0.x | c0.y | c0.z | c0.w) != 0u) {
-                            uint4 t4 = tile4[q0];
-                            t4.x |= c0.x; t4.y |= c0.y; t4.z |= c0.z; t4.w |= c0.w;
-                            tile4[q0] = t4;
+                        uint4 acc[4];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++) acc[u] = make_uint4(0u, 0u, 0u, 0u);
+                        for (uint32_t j = 0; j < nh; j += 2u) {
+                            const uint4* s0 = base4 + (size_t)s_hk[j] * kstride4;
+                            const uint4* s1 = base4 + (size_t)s_hk[j + 1u < nh ? j + 1u : j] * kstride4;
+                            uint4 v[8];
+#pragma unroll
+                            for (uint32_t u = 0; u < 4; u++) {
+                                v[u] = s0[qq[u]];
+                                v[4 + u] = s1[qq[u]];
+                            }
+#pragma unroll
+                            for (uint32_t u = 0; u < 4; u++) {
+                                acc[u].x |= v[u].x | v[4 + u].x;
+                                acc[u].y |= v[u].y | v[4 + u].y;
+                                acc[u].z |= v[u].z | v[4 + u].z;
+                                acc[u].w |= v[u].w | v[4 + u].w;
+                            }
                         }
-                        if (ok1 && (c1.x | c1.y | c1.z | c1.w) != 0u) {
-                            uint4 t4 = tile4[q1];
-                            t4.x |= c1.x; t4.y |= c1.y; t4.z |= c1.z; t4.w |= c1.w;
-                            tile4[q1] = t4;
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++) {
+                            if (okq[u] && (acc[u].x | acc[u].y | acc[u].z | acc[u].w) != 0u) {
+                                uint4 t4 = tile4[qq[u]];
+                                nb += __popc(acc[u].x & ~t4.x) + __popc(acc[u].y & ~t4.y) +
+                                      __popc(acc[u].z & ~t4.z) + __popc(acc[u].w & ~t4.w);
+                                t4.x |= acc[u].x; t4.y |= acc[u].y; t4.z |= acc[u].z; t4.w |= acc[u].w;
+                                tile4[qq[u]] = t4;
+                            }
                         }
                     }
                     __syncthreads();
@@ -214,7 +367,11 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
 #pragma unroll
                     for (uint32_t u = 0; u < 4; u++) {
                         const uint32_t q = q0 + u * BLOCK;
-                        if (q < wn && acc[u] != 0u) tile[q] |= acc[u];
+                        if (q < wn && acc[u] != 0u) {
+                            const uint32_t old = tile[q];
+                            nb += __popc(acc[u] & ~old);
+                            tile[q] = old | acc[u];
+                        }
                     }
                 }
                 __syncthreads();
@@ -243,7 +400,8 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 const int64_t first = (int64_t)h0 - c0, last = (int64_t)h1 - 1 - c0;  // bit range in v
                 if (first > 0) v &= first >= 32 ? 0u : ~0u << first;
                 if (last < 31) v &= last < 0 ? 0u : (~0u >> (31 - last));
-                tile[q] = v;
+                tile[q] = v;  // one writer per word of a zeroed tile
+                nb += __popc(v);
             }
         } else if (h0 < h1) {
             const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
@@ -266,7 +424,10 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < 4; u++)
-                        if (zl[u] != 0xFFFFFFFFu) atomicOr(&tile[swz_word(zl[u], W, C)], 1u << (zl[u] & 31u));
+                        if (zl[u] != 0xFFFFFFFFu) {
+                            const uint32_t bit = 1u << (zl[u] & 31u);
+                            nb += (atomicOr(&tile[swz_word(zl[u], W, C)], bit) & bit) ? 0u : 1u;
+                        }
                 }
             }
         }
@@ -354,6 +515,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
 #pragma unroll
                 for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
                     const uint32_t old = COUNT ? atomicAdd(&tile[addr[u]], val[u]) : atomicOr(&tile[addr[u]], val[u]);
+                    if (!COUNT) nb += __popc(val[u] & ~old);
                     fm[u] = __ballot(val[u] != 0u && old == 0u);
                     ftot += (uint32_t)__popcll(fm[u]);
                 }
@@ -370,6 +532,10 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                         fbase += (uint32_t)__popcll(fm[u]);
                     }
                 }
+            } else if (!COUNT && !a.fp_mode) {
+                // untracked join-project rows: the returned words count the new bits
+#pragma unroll
+                for (uint32_t u = 0; u < kPairs * kUnroll; u++) nb += __popc(val[u] & ~atomicOr(&tile[addr[u]], val[u]));
             } else {
 #pragma unroll
                 for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
@@ -400,6 +566,8 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     }
     const uint32_t wper = (count + NW - 1) / NW;
     const uint32_t wlo = min(wid * wper, count), whi = min(wlo + wper, count);
+    if (!COUNT && !a.fp_mode && !listed && emit_swept<BLOCK, ZID>(a, L, xv, zbase, nb, self_item, self_keep, wlo, whi))
+        return;
     uint32_t c = 0;
     uint64_t nzmask = 0;  // bit g: the lane's word of step g holds rows
     {
@@ -446,7 +614,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 if (v) {
                     const uint64_t ps = fpt[32u + (v & 255u)] + fpt[288u + ((v >> 8) & 255u)] +
                                         fpt[544u + ((v >> 16) & 255u)] + fpt[800u + (v >> 24)];
-                    S1 += (uint64_t)zword_factor(wv0 + w) * ps;
+                    S1 += (uint64_t)zword_factor((wv0 + w) & 0x07FFFFFFu) * ps;  // (z >> 5) of a 32-bit z: 27 bits
                 }
             }
             S2 = S1;
@@ -662,11 +830,11 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     __shared__ uint64_t sh[NW + 1];
     __shared__ uint32_t sh32[2 * (NW + 1)];
     __shared__ uint32_t s_wtot[NW];
-    __shared__ uint32_t s_task, s_nt;
+    __shared__ uint32_t s_task, s_nt, s_cursor;
     __shared__ uint64_t s_gbase;
     uint32_t* dyn = smem + (a.fp_mode ? 2u * kFpTableU64 : 0u);
     const Lds L{reinterpret_cast<const uint64_t*>(smem), dyn, dyn + a.tile_words,
-                dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
+                dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase, &s_cursor};
 
     const uint32_t tid = threadIdx.x;
     if (a.fp_mode) fp_table_init<BLOCK>(reinterpret_cast<uint64_t*>(smem));
@@ -677,23 +845,28 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     const bool lists = a.big_x != nullptr;
     const uint32_t lo = lists ? 0u : a.task_lo;
     const uint32_t hi = lists ? ((a.row_hi - a.row_lo) - (uint32_t)*a.n_wave) * a.ntiles : a.task_hi;
-    // TM (the fused bit-or seed): tasks are taken tile-major, so the
-    // workgroups running at one time share one or two z tiles and the
-    // heavy-key bitmap words of those tiles stay in every XCD's L2 (10M x 10M
-    // Zipf COUNT fingerprint: 327 -> 319 ms).  Otherwise row-major: a row's
-    // tiles run side by side and share its y list.  A template argument:
-    // the branch alone cost the two-hop graph 1.3%.
+    // TM (a fused bit-or seed whose key bitmaps exceed an XCD's L2): tasks
+    // are taken tile-major, so the workgroups running at one time share one
+    // or two z tiles and the heavy-key bitmap words of those tiles stay in
+    // every XCD's L2 (10M x 10M Zipf COUNT fingerprint: 327 -> 319 ms).
+    // Otherwise row-major: a row's tiles run side by side and share its y
+    // list.  A template argument: the branch alone cost the two-hop graph 1.3%.
     const uint32_t nrows = (hi - lo) / a.ntiles;
 
+    // The next task is claimed while the current one runs (the claim's
+    // latency overlaps the task; its value reaches LDS after the task).
+    uint32_t next = 0;
+    if (tid == 0) next = atomicAdd(&a.sc->task, 1u);
     for (;;) {
         if (tid == 0) {
-            s_task = atomicAdd(&a.sc->task, 1u);
+            s_task = next;
             s_nt = 0;
         }
         __syncthreads();
         // task and x are wavefront-uniform: scalar loads
         uint32_t task = lo + __builtin_amdgcn_readfirstlane(s_task);
         if (task >= hi) break;
+        if (tid == 0) next = atomicAdd(&a.sc->task, 1u);
         if (TM) {  // the k-th claim is row k % nrows of tile k / nrows
             const uint32_t k = task - lo, t = k / nrows;
             task = lo + (k - t * nrows) * a.ntiles + t;
@@ -721,7 +894,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
     __shared__ uint64_t sh[NW + 1];
     __shared__ uint32_t sh32[2 * (NW + 1)];
     __shared__ uint32_t s_wtot[NW];
-    __shared__ uint32_t s_task, s_nt;
+    __shared__ uint32_t s_task, s_nt, s_cursor;
     __shared__ uint64_t s_gbase;
 
     const uint32_t tid = threadIdx.x;
@@ -733,14 +906,17 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
     const bool fp = fp_mode != 0u;
     uint32_t* dyn = smem + (fp ? 2u * kFpTableU64 : 0u);
     if (fp) fp_table_init<BLOCK>(reinterpret_cast<uint64_t*>(smem));
+    uint32_t next = 0;  // the next task, claimed while the current one runs
+    if (tid == 0) next = atomicAdd(counter, 1u);
     for (;;) {
         if (tid == 0) {
-            s_task = atomicAdd(counter, 1u);
+            s_task = next;
             s_nt = 0;
         }
         __syncthreads();
         const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
         if (task >= total) break;
+        if (tid == 0) next = atomicAdd(counter, 1u);
         // largest i with task_start[i] <= task (task_start[n_inst] = total > task)
         uint32_t lo = 0, hi = n_inst;
         while (hi - lo > 1) {
@@ -759,7 +935,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
         const uint32_t local = task - task_start[lo];
         const uint32_t x = local / a.ntiles;
         const Lds L{reinterpret_cast<const uint64_t*>(smem), dyn, dyn + a.tile_words,
-                    dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase};
+                    dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase, &s_cursor};
         if (a.id2z == nullptr) row_task<BLOCK, COUNT, true>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
         else row_task<BLOCK, COUNT, false>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
     }
@@ -835,8 +1011,8 @@ template <bool COUNT>
 int launch(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
     const size_t lds = dim3h::expand_lds_bytes(a.tile_words, a.touched_cap, a.stage_cap, (uint32_t)block,
                                                a.fp_mode != 0u);
-    // the fused bit-or seed (join-project, every z ID a column): tile-major tasks
-    if (!COUNT && a.bkeys != nullptr && a.id2z == nullptr && a.ntiles > 1u)
+    // a fused bit-or seed whose key bitmaps outgrow an L2: tile-major tasks
+    if (!COUNT && a.bkeys != nullptr && a.id2z == nullptr && a.ntiles > 1u && a.tile_major)
         return launch_z<false, true, true>(a, block, lds, cu_count, st);
     if (a.id2z == nullptr) return launch_z<COUNT, true>(a, block, lds, cu_count, st);
     return launch_z<COUNT, false>(a, block, lds, cu_count, st);
