@@ -21,6 +21,9 @@
  * Errors are status codes, never exit(): the reference printf+exit()s on
  * allocation failure (e.g. dim3.hpp:303-307).
  *
+ * Size limit: each relation holds at most 2^30 - 1 tuples (DIM3_ERR_INVALID
+ * beyond), so that every device key table fits 2^31 slots.
+ *
  * Thread safety: a context serialises its calls on its own HIP stream; use
  * one context per host thread, or the server API, which is thread safe.
  */
@@ -305,6 +308,14 @@ int dim3_estimate_join_cardinality(dim3_ctx* ctx, const int32_t* R_xy, uint64_t 
  * sizes a materialising request's output with it at submit. */
 int dim3_estimate_join_cardinality_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_r, const int32_t* d_S,
                                        uint64_t n_s, void* stream, int64_t* estimate);
+
+/* The same over the reference's TUPLE arrays (EstimateJoinCardinality's
+ * TUPLE overload, dim3.hpp:226-260): tuple i is the int32 first and the
+ * int32 key at R_tuples + i * stride (stride a multiple of 4, at least 8);
+ * values are ignored.  Host inputs, copied as they are and split on the
+ * device. */
+int dim3_estimate_join_cardinality_tuples(dim3_ctx* ctx, const void* R_tuples, uint64_t n_r, const void* S_tuples,
+                                          uint64_t n_s, uint32_t stride, int64_t* estimate);
 
 /* The dense part's product on its own (the boolean product that
  * Hybrid_solution's DENSE part evaluates one (x, dense z) pair at a time,

@@ -31,18 +31,32 @@
 // Device: DIM3_DEVICE env var, default 0.
 #pragma once
 
+#include <assert.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <chrono>
+#include <functional>
 #include <random>
+#include <string>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "dim3.h"
 
-using std::pair;
+// What the reference's header gives its callers besides its own names
+// (dim3.hpp:6-18): <chrono> and the rest of the standard library, the whole
+// std namespace (main.cpp:17 writes chrono::high_resolution_clock), LL and
+// FORCE_INLINE (Include/structures.hpp:7).
+using namespace std;
+typedef long long LL;
+#ifndef FORCE_INLINE
+#define FORCE_INLINE inline __attribute__((always_inline))
+#endif
 
 template <class T, typename _CountType = uint32_t>
 struct myvector {
@@ -198,9 +212,16 @@ struct DIM3 {
 // The reference's sampled join size (keys whose low sample bits are zero
 // under the identity hash, the 1000-tuple-prefix fallback), computed on the
 // device: the same number.  Where the reference asserts |R| >= |S|, the
-// library swaps the sides.
-template <typename _Tx, typename _Ty, typename _Tz, typename _H = myHasher, typename _Tcounter = uint32_t>
+// library swaps the sides.  _H defaults to std::hash<_Ty> as the reference's
+// (dim3.hpp:190); the sample is taken under the identity hash, which is what
+// std::hash of a 4-byte integer (libstdc++) and myHasher compute, so other
+// hashers are refused at compile time.
+template <typename _Tx, typename _Ty, typename _Tz, typename _H = std::hash<_Ty>, typename _Tcounter = uint32_t>
 struct EstimateJoinCardinality {
+    static_assert(std::is_same<_H, myHasher>::value ||
+                      (std::is_same<_H, std::hash<_Ty>>::value && dim3_shim::is_i32<_Ty>()),
+                  "libdim3 samples join keys under the identity hash (myHasher or std::hash of a 4-byte integer)");
+
     static long long estimate(const myvector<pair<_Tx, _Ty>, _Tcounter> R,
                               const myvector<pair<_Tz, _Ty>, _Tcounter> S) {
         static_assert(dim3_shim::is_i32<_Tx>() && dim3_shim::is_i32<_Ty>() && dim3_shim::is_i32<_Tz>(),
@@ -208,6 +229,20 @@ struct EstimateJoinCardinality {
         int64_t est = 0;
         const int rc = dim3_estimate_join_cardinality(dim3_shim::context(), (const int32_t*)R.data, R.size(),
                                                       (const int32_t*)S.data, S.size(), &est);
+        if (rc != DIM3_OK) dim3_shim::die("EstimateJoinCardinality::estimate", rc);
+        return (long long)est;
+    }
+
+    // the TUPLE overload (dim3.hpp:226-260): the key is TUPLE::y
+    template <typename _Tv>
+    static long long estimate(const myvector<TUPLE<_Tx, _Ty, _Tv>, _Tcounter> R,
+                              const myvector<TUPLE<_Tz, _Ty, _Tv>, _Tcounter> S) {
+        static_assert(dim3_shim::is_i32<_Tx>() && dim3_shim::is_i32<_Ty>() && dim3_shim::is_i32<_Tz>(),
+                      "libdim3 joins 4-byte integer attributes");
+        static_assert(sizeof(TUPLE<_Tx, _Ty, _Tv>) == sizeof(TUPLE<_Tz, _Ty, _Tv>), "tuple layouts");
+        int64_t est = 0;
+        const int rc = dim3_estimate_join_cardinality_tuples(dim3_shim::context(), R.data, R.size(), S.data, S.size(),
+                                                             (uint32_t)sizeof(TUPLE<_Tx, _Ty, _Tv>), &est);
         if (rc != DIM3_OK) dim3_shim::die("EstimateJoinCardinality::estimate", rc);
         return (long long)est;
     }

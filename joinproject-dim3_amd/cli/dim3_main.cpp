@@ -1,11 +1,12 @@
 // Counterpart of the reference drivers main.cpp, main_parallel.cpp and
 // join_aggregate.cpp (the same data and the same two output lines), built
 // against the drop-in header include/dim3.hpp:
-//   dim3_main [project|parallel|aggregate|count|estimate]
+//   dim3_main [project|parallel|aggregate|count|estimate|estimate_tuples]
 // aggregate is join_aggregate.cpp's query, AVG(R.v + S.v) over
 // TUPLE<int,int,float> (join_aggregate.cpp:6-23); count is the COUNT
 // aggregate over TUPLE<int,int,long long>; estimate prints
-// EstimateJoinCardinality::estimate of the same data (|Estimate|=...).
+// EstimateJoinCardinality::estimate of the same data (|Estimate|=...), and
+// estimate_tuples its TUPLE overload over the TUPLE<int,int,float> data.
 #include <chrono>
 #include <cstring>
 
@@ -20,6 +21,16 @@ int main(int argc, char* argv[]) {
         gen_rand_data(R, S);
         t0 = std::chrono::high_resolution_clock::now();
         const long long est = EstimateJoinCardinality<int, int, int, myHasher, uint32_t>::estimate(R, S);
+        t1 = std::chrono::high_resolution_clock::now();
+        printf("It takes %f seconds.\n", std::chrono::duration<double, std::ratio<1, 1>>(t1 - t0).count());
+        printf("|Estimate|=%lld\n", est);
+        return 0;
+    }
+    if (!strcmp(mode, "estimate_tuples")) {  // the TUPLE overload, the reference's default hasher
+        myvector<TUPLE<int, int, float>> R, S;
+        gen_rand_data(R, S);
+        t0 = std::chrono::high_resolution_clock::now();
+        const long long est = EstimateJoinCardinality<int, int, int>::estimate(R, S);
         t1 = std::chrono::high_resolution_clock::now();
         printf("It takes %f seconds.\n", std::chrono::duration<double, std::ratio<1, 1>>(t1 - t0).count());
         printf("|Estimate|=%lld\n", est);

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "dim3_internal.h"
+#include "dim3_sizes.h"
 
 using dim3h::Scalars;
 
@@ -106,11 +107,7 @@ struct Slot {
     size_t bytes = 0;
 };
 
-inline uint32_t pow2_at_least(uint64_t v) {
-    uint64_t c = 64;
-    while (c < v) c <<= 1;
-    return (uint32_t)c;
-}
+using dim3s::pow2_at_least;
 
 }  // namespace
 
@@ -685,8 +682,8 @@ int prepare_start(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, con
     P.S = dS;
     P.nR = nR;
     P.nS = nS;
-    if (nR >= (1ull << 31) || nS >= (1ull << 31))
-        return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^31-1 tuples (got %llu, %llu)",
+    if (nR > dim3s::kMaxTuples || nS > dim3s::kMaxTuples)
+        return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^30-1 tuples (got %llu, %llu)",
                     (unsigned long long)nR, (unsigned long long)nS);
 
     Scalars* sc;
@@ -717,9 +714,11 @@ int prepare_map(dim3_ctx* c, const dim3_opts& o, hipStream_t st) {
     auto span = [](int32_t lo, int32_t hi) { return (uint64_t)((int64_t)hi - (int64_t)lo + 1); };
     const uint64_t xr = span(mm[0], mm[1]), yr = span(mm[2], mm[3]);
     uint64_t zr = span(mm[4], mm[5]);
-    P.x_range = !hash_only && xr <= 2 * nR + 64;
-    P.y_range = !hash_only && yr <= 2 * nR + 64;
-    P.z_ident = !hash_only && zr <= 2 * nS + 64;
+    P.x_range = !hash_only && dim3s::range_fits(xr, nR);
+    P.y_range = !hash_only && dim3s::range_fits(yr, nR);
+    // z IDs are value - zlo (identity slots, no power of two); the 32-aligned
+    // start below adds at most 31 values
+    P.z_ident = !hash_only && zr <= 2 * nS + 64 && zr + 31 <= dim3s::kMaxSlots;
     // dense z IDs start at a multiple of 32 below min(z), so that a word of
     // a project tile is one 32-aligned word of z values (the fingerprint
     // sums such a word at once); the IDs below min(z) stay empty
@@ -738,11 +737,11 @@ int prepare_map(dim3_ctx* c, const dim3_opts& o, hipStream_t st) {
         m->mask = 0;
         m->span = range ? (uint32_t)span_v : 0u;
         if (range) {
-            const uint32_t p2 = pow2_at_least(span_v);
+            const uint32_t p2 = (uint32_t)pow2_at_least(span_v);  // <= 2^31 (range_fits)
             m->mask = identity ? 0u : p2 - 1u;
             m->nslots = identity ? (uint32_t)span_v : p2;
         } else {
-            const uint32_t cap = pow2_at_least(2 * n);
+            const uint32_t cap = (uint32_t)dim3s::key_table_cap(n);  // <= 2^31 for n <= kMaxTuples
             TRY(ws(c, keys_slot, (uint64_t)cap + 1, keys));
             m->keys = *keys;
             m->mask = cap - 1;
@@ -2133,7 +2132,9 @@ int dim3_count_join_paths(dim3_ctx* c, const int32_t* R_xy, uint64_t nR, const i
     TRY(ws(c, WS_IN_S, nS, &dS));
     HIPCHK(c, hipMemcpyAsync(dR, R_xy, nR * 8, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(dS, S_zy, nS * 8, hipMemcpyHostToDevice, st));
-    const uint32_t cap = pow2_at_least(2 * nR);
+    if (nR > dim3s::kMaxTuples || nS > dim3s::kMaxTuples)
+        return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^30-1 tuples");
+    const uint32_t cap = (uint32_t)dim3s::key_table_cap(nR);
     uint32_t *keys, *degR, *degS, *slotR;
     int32_t* slotS;
     Scalars* sc;
@@ -2165,7 +2166,9 @@ static int estimate_on(dim3_ctx* c, const int2* dR, uint64_t nR, const int2* dS,
     uint32_t bits = 0;
     for (uint64_t n = nS; n > 10000; n /= 2) bits++;
     const uint32_t smask = bits >= 32 ? 0xFFFFFFFFu : (1u << bits) - 1u;
-    const uint32_t cap = pow2_at_least(2 * nS);  // every S tuple sampled, at most
+    // the distinct sampled keys: at most |S|, and at most the int32 values
+    // whose low `bits` bits are zero (the key -1 has its own slot)
+    const uint32_t cap = (uint32_t)dim3s::estimate_table_cap(nS, bits);
     uint32_t *keys, *cnt;
     Scalars* sc;
     TRY(ws(c, WS_KEYS_Y, (uint64_t)cap + 1, &keys));
@@ -2203,7 +2206,7 @@ static int estimate_args(dim3_ctx* c, const int32_t*& R_xy, uint64_t& nR, const 
     }
     if (nS == 0) return -1;
     if (!R_xy || !S_zy) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
-    if (nR >= (1ull << 31)) return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^31-1 tuples");
+    if (nR > dim3s::kMaxTuples) return fail(c, DIM3_ERR_INVALID, "relations are limited to 2^30-1 tuples");
     return DIM3_OK;
 }
 
@@ -2236,6 +2239,37 @@ int dim3_estimate_join_cardinality_dev(dim3_ctx* c, const int32_t* d_R, uint64_t
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     c->qstream = st;
     return estimate_on(c, (const int2*)d_R, nR, (const int2*)d_S, nS, st, estimate);
+}
+
+int dim3_estimate_join_cardinality_tuples(dim3_ctx* c, const void* R_tuples, uint64_t nR, const void* S_tuples,
+                                          uint64_t nS, uint32_t stride, int64_t* estimate) {
+    if (!c || !estimate) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    if (stride < 8u || stride % 4u) return fail(c, DIM3_ERR_INVALID, "bad tuple stride %u", stride);
+    const int32_t* R = (const int32_t*)R_tuples;
+    const int32_t* S = (const int32_t*)S_tuples;
+    const int a = estimate_args(c, R, nR, S, nS, estimate);
+    if (a != DIM3_OK) return a < 0 ? DIM3_OK : a;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    c->qstream = st;
+    int2 *dR, *dS;
+    int32_t* dv;
+    uint8_t* raw;
+    Scalars* sc;
+    TRY(ws(c, WS_IN_R, nR, &dR));
+    TRY(ws(c, WS_IN_S, nS, &dS));
+    TRY(ws(c, WS_IN_VR, nR, &dv));  // the split's value output, unused
+    TRY(ws(c, WS_SCALARS, 1, &sc));
+    TRY(ws(c, WS_OUT, nR * stride, &raw));  // staging (nR >= nS after the swap)
+    // (first, key) of each tuple, split on the device as dim3_join_aggregate_tuples does
+    HIPCHK(c, hipMemcpyAsync(raw, R, nR * stride, hipMemcpyHostToDevice, st));
+    dim3h::split_tuples(raw, nR, stride, 0, DIM3_VALUE_I32, dR, dv, (uint32_t*)&sc->spare[2], st);
+    HIPCHK(c, hipMemcpyAsync(raw, S, nS * stride, hipMemcpyHostToDevice, st));
+    dim3h::split_tuples(raw, nS, stride, 0, DIM3_VALUE_I32, dS, dv, (uint32_t*)&sc->spare[2], st);
+    LAUNCHCHK(c, "split tuples");
+    return estimate_on(c, dR, nR, dS, nS, st, estimate);
 }
 
 int dim3_dense_product(dim3_ctx* c, const int8_t* A, uint64_t M, const int8_t* Bt, uint64_t N, uint64_t K,

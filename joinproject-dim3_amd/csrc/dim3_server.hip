@@ -215,7 +215,14 @@ void dim3_server::run(uint32_t w, std::vector<uint64_t>& batch, std::vector<Job*
             if (*out_buf) dim3h::dev_free(*out_buf);
             *out_buf = nullptr;
             *out_bytes = 0;
-            if (dim3h::dev_malloc(out_buf, res.n * row) != hipSuccess) {
+            uint64_t room = 0;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                const uint64_t used = resident + others_held(w) + dim3_ctx_device_bytes(c);
+                room = budget > used ? budget - used : 0;
+            }
+            // an output larger than the HBM budget leaves fails the request
+            if (res.n * row > room || dim3h::dev_malloc(out_buf, res.n * row) != hipSuccess) {
                 (void)hipGetLastError();
                 *out_buf = nullptr;
                 j->status = DIM3_ERR_OOM;
@@ -625,6 +632,13 @@ int dim3_submit(dim3_server* s, const dim3_request* req, uint64_t* ticket) {
         }
         const double rows = std::min((double)std::max<int64_t>(paths, 0), (double)nR * (double)nS);
         out_est = rows * row > 1e18 ? (uint64_t)1e18 : (uint64_t)(rows * row);
+        // The estimate counts join paths; the rows are distinct (x, z)
+        // groups, often far fewer (a dense block: ~1e3 paths per row).  It
+        // sizes a pre-allocation only: at most a quarter of the memory the
+        // relations leave, so it never refuses a request by itself; a larger
+        // output is grown to the exact size the query reports.
+        const uint64_t room = s->budget > s->resident ? s->budget - s->resident : 0;
+        out_est = std::min<uint64_t>(out_est, room / 4);
         est += out_est;
     }
     if (s->pending.size() >= s->queue_cap) {  // filled while the estimate ran

@@ -149,10 +149,12 @@ def test_server_value_requests(dim3):
 
 
 def test_materialize_admission_by_estimate(dim3):
-    """A materialising request is admitted with its estimated output (one row
-    per sampled join path): one key shared by 4000 tuples on each side is
-    1.6e7 rows (128 MB), more than a 64 MiB budget holds next to the
-    workspace; the fingerprint of the same query fits."""
+    """The join-size estimate only sizes a materialising request's output
+    pre-allocation (capped at a quarter of the free budget); it never refuses
+    a request.  One key shared by 4000 tuples on each side is 1.6e7 rows
+    (128 MB): under a 64 MiB budget the request is admitted and ends with
+    DIM3_ERR_OOM once the query reports its real size; the fingerprint of
+    the same query fits."""
     n = 4000
     R = np.stack([np.arange(n), np.zeros(n)], 1).astype(np.int32)
     S = np.stack([np.arange(n) + 7, np.zeros(n)], 1).astype(np.int32)
@@ -162,9 +164,11 @@ def test_materialize_admission_by_estimate(dim3):
         assert rc == dim3.OK
         rc, res, _, _ = srv.wait(t)
         assert rc == dim3.OK and res.n == n * n
-        rc, _ = srv.submit(hr, hs, output_mode=0)
-        assert rc == dim3.ERR_ADMISSION
-        assert srv.stats()["rejected"] == 1
+        rc, t = srv.submit(hr, hs, output_mode=0)
+        assert rc == dim3.OK
+        rc, res, _, _ = srv.wait(t)
+        assert rc == dim3.ERR_OOM
+        assert srv.stats()["rejected"] == 0
     with dim3.Server(0, hbm_budget_bytes=512 << 20, workers=1) as srv:
         hr, hs = srv.put(R), srv.put(S)
         rc, t = srv.submit(hr, hs, output_mode=0)
@@ -174,6 +178,35 @@ def test_materialize_admission_by_estimate(dim3):
         want, _, _ = oracle.join_project(R, S)
         assert np.array_equal(oracle.sort_pairs(res.pairs), oracle.sort_pairs(want))
     say("materialise admission ok")
+
+
+def test_materialize_paths_far_above_rows(dim3):
+    """1e7 join paths but 1e4 distinct rows (100 x 100 values joined through
+    the same 1000 keys): the estimate (one row per path, 80 MB) exceeds a
+    64 MiB budget, yet the request is admitted and its rows are exact."""
+    x = np.repeat(np.arange(100), 1000)
+    y = np.tile(np.arange(1000), 100)
+    R = np.stack([x, y], 1).astype(np.int32)
+    S = np.stack([x + 50, y], 1).astype(np.int32)
+    assert oracle.estimate(R, S) >= 8_000_000
+    with dim3.Server(0, hbm_budget_bytes=64 << 20, workers=1) as srv:
+        hr, hs = srv.put(R), srv.put(S)
+        for kind in ("project", "count"):
+            rc, t = srv.submit(hr, hs, count=kind == "count", output_mode=0)
+            assert rc == dim3.OK, kind
+            rc, res, _, _ = srv.wait(t)
+            assert rc == dim3.OK and res.n == 100 * 100, kind
+            if kind == "project":
+                want, _, _ = oracle.join_project(R, S)
+                assert np.array_equal(oracle.sort_pairs(res.pairs), oracle.sort_pairs(want))
+            else:
+                wp, wc, _, _ = oracle.join_aggregate_count(R, S)
+                gp, gc = oracle.sort_pairs(res.xz, res.counts)
+                ep, ec = oracle.sort_pairs(wp, wc)
+                assert np.array_equal(gp, ep) and np.array_equal(gc, ec)
+                assert int(gc.sum()) == 100 * 100 * 1000
+        assert srv.stats()["rejected"] == 0
+    say("paths far above rows ok")
 
 
 def test_materialize_under_estimate(dim3):
