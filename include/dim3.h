@@ -183,6 +183,20 @@ int dim3_join_aggregate_count_dev(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_
                                   uint64_t n_s, const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap,
                                   void* stream, dim3_result* res);
 
+/* dim3_join_project_dev without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as dim3_join_project_dev (the rows of the
+ * all-sparse plan, opts->dense_mode ignored), with every workspace buffer
+ * carved from a 48 MiB device array the library holds per device until the
+ * process ends (its address taken at dim3_ctx_create).  At most 2^17 tuples
+ * per relation (DIM3_ERR_INVALID beyond).  Calls that overlap on one device
+ * share that array and race: serialise them (one context per device, or
+ * one thread).  The caller passes inputs and the output buffer. */
+int dim3_join_project_dev_noalloc(dim3_ctx* ctx, const int32_t* d_R, uint64_t n_r, const int32_t* d_S, uint64_t n_s,
+                                  const dim3_opts* opts, int32_t* d_pairs, uint64_t cap, void* stream,
+                                  dim3_result* res);
+
 /* ---- value aggregates: SELECT x, z, AGG(R.v + S.v) GROUP BY x, z ----
  * Replaces DIM3JoinAggregate<int,int,int,_Tv,_Agg,...>::doJoinAggregate
  * (dim3.hpp:1313-1373) with a value policy over the path value

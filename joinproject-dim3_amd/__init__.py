@@ -167,7 +167,7 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
         f.restype = i32
-    for name in ("dim3_join_project_dev", "dim3_join_aggregate_count_dev"):
+    for name in ("dim3_join_project_dev", "dim3_join_aggregate_count_dev", "dim3_join_project_dev_noalloc"):
         f = getattr(L, name)
         f.argtypes = [vp, vp, u64, vp, u64, ctypes.POINTER(Opts), vp, u64, vp, ctypes.POINTER(Result)]
         f.restype = i32
@@ -580,6 +580,19 @@ class Context:
                                            ctypes.c_void_p(stream or 0), ctypes.byref(res))
         if rc not in (OK, ERR_CAPACITY):
             self._check(rc, "dim3_join_project_dev")
+        return rc, QueryResult(res)
+
+    def join_project_dev_noalloc(self, d_R, n_r, d_S, n_s, d_out, cap, stream=None, **kw):
+        """dim3_join_project_dev_noalloc: join_project_dev with every workspace
+        buffer in the library's per-device scratch array (no allocation; at
+        most 2^17 tuples per relation).  Returns (status, QueryResult)."""
+        opts = make_opts(**kw)
+        res = Result()
+        rc = self._L.dim3_join_project_dev_noalloc(self._h, ctypes.c_void_p(d_R), n_r, ctypes.c_void_p(d_S), n_s,
+                                                   ctypes.byref(opts), ctypes.c_void_p(d_out), cap,
+                                                   ctypes.c_void_p(stream or 0), ctypes.byref(res))
+        if rc not in (OK, ERR_CAPACITY, ERR_INVALID):
+            self._check(rc, "dim3_join_project_dev_noalloc")
         return rc, QueryResult(res)
 
     def join_aggregate_count_dev(self, d_R, n_r, d_S, n_s, d_out, cap, stream=None, **kw):

@@ -25,6 +25,39 @@
 
 using dim3h::Scalars;
 
+// Device workspace of the allocation-free entry point
+// (dim3_join_project_dev_noalloc): one array per device, held until the
+// process ends.  In host code `scratch` only names the symbol; its device
+// address comes from hipGetSymbolAddress, once per device (scratch_address).
+constexpr size_t kScratchBytes = 48ull << 20;
+constexpr uint64_t kNoAllocMaxTuples = 1ull << 17;  // per relation: the workspace then fits the array
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+// The device address of `scratch` on the current device (nullptr if HIP
+// cannot give it), looked up once per device.
+uint8_t* scratch_address(int device) {
+    static std::mutex mu;
+    static uint8_t* addr[64] = {};
+    static bool done[64] = {};
+    if (device < 0 || device >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!done[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+        }
+        addr[device] = (uint8_t*)p;
+        done[device] = true;
+    }
+    return addr[device];
+}
+
+}  // namespace
+
 namespace {
 
 enum SlotId {
@@ -215,8 +248,16 @@ struct dim3_ctx {
     std::recursive_mutex mu;
     DenseModel model;           // cost-model rates (defaults or measured)
     uint32_t model_source = 0;  // 0 defaults, 1 measured, 2 read from a cache file
+    // allocation-free queries (dim3_join_project_dev_noalloc): this device's
+    // copy of the library's scratch array and the bank whose slots are
+    // carved from it (pinned mirrors made at creation); arena != nullptr
+    // while such a query runs
+    uint8_t* scratch = nullptr;
+    std::unique_ptr<Bank> nab;
+    uint8_t* arena = nullptr;
+    size_t arena_used = 0, arena_size = 0;
 
-    Bank& bank() { return *banks[cur]; }
+    Bank& bank() { return arena ? *nab : *banks[cur]; }
     void use_bank(uint32_t b) {
         while (banks.size() <= b) banks.emplace_back(new Bank());
         cur = b;
@@ -262,6 +303,18 @@ int ws_get(dim3_ctx* c, SlotId id, size_t bytes, void** out) {
     bytes = (bytes + 255) & ~(size_t)255;
     Slot& s = c->bank().ws[id];
     if (s.bytes >= bytes) {
+        *out = s.p;
+        return DIM3_OK;
+    }
+    if (c->arena) {
+        // an allocation-free query: slots are carved from the scratch array
+        // (a slot that must grow within the call takes a new piece)
+        if (bytes > c->arena_size - c->arena_used)
+            return fail(c, DIM3_ERR_OOM, "workspace slot %d needs %zu bytes: the %zu-byte scratch array is full",
+                        (int)id, bytes, c->arena_size);
+        s.p = c->arena + c->arena_used;
+        s.bytes = bytes;
+        c->arena_used += bytes;
         *out = s.p;
         return DIM3_OK;
     }
@@ -1965,6 +2018,17 @@ int dim3_ctx_create(int device, uint64_t hbm_budget_bytes, dim3_ctx** out) {
     memset(c->h_sc, 0, sizeof(Scalars));
     for (auto& e : c->ev) hipEventCreate(&e);
     c->qstream = c->stream;
+    // the allocation-free entry point's bank (its pinned mirrors made here)
+    // and this device's scratch array (device is current)
+    c->nab.reset(new (std::nothrow) Bank());
+    if (!c->nab || dim3h::host_malloc((void**)&c->nab->hsc, sizeof(Scalars)) != hipSuccess ||
+        dim3h::host_malloc((void**)&c->nab->hhist, 6 * dim3h::kDenseBins * sizeof(unsigned long long)) !=
+            hipSuccess) {
+        dim3_ctx_destroy(c);
+        return DIM3_ERR_DEVICE;
+    }
+    c->scratch = scratch_address(device);
+    c->err.reserve(600);  // error texts (< 512 bytes) never allocate
     *out = c;
     return DIM3_OK;
 }
@@ -2104,6 +2168,48 @@ int dim3_join_aggregate_count_dev(dim3_ctx* c, const int32_t* d_R, uint64_t n_r,
                                   const dim3_opts* opts, dim3_count_row* d_rows, uint64_t cap, void* stream,
                                   dim3_result* res) {
     return run_dev(c, true, d_R, n_r, d_S, n_s, opts, d_rows, cap, stream, res);
+}
+
+// dim3_join_project_dev with every workspace buffer carved from the
+// device's scratch array: no device or host allocation.  The all-sparse
+// plan (the same rows); at most kNoAllocMaxTuples tuples per relation.
+int dim3_join_project_dev_noalloc(dim3_ctx* c, const int32_t* d_R, uint64_t n_r, const int32_t* d_S, uint64_t n_s,
+                                  const dim3_opts* opts, int32_t* d_pairs, uint64_t cap, void* stream,
+                                  dim3_result* res) {
+    if (!c || !res) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(res, 0, sizeof(*res));
+    if (!c->scratch) return fail(c, DIM3_ERR_DEVICE, "no scratch array on device %d", c->device);
+    if (n_r > kNoAllocMaxTuples || n_s > kNoAllocMaxTuples)
+        return fail(c, DIM3_ERR_INVALID, "the allocation-free join takes at most %llu tuples per relation (got %llu, %llu)",
+                    (unsigned long long)kNoAllocMaxTuples, (unsigned long long)n_r, (unsigned long long)n_s);
+    if ((n_r && !d_R) || (n_s && !d_S)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    o.dense_mode = DIM3_DENSE_OFF;
+    const bool fp = o.output_mode == DIM3_OUT_FINGERPRINT;
+    if (!fp && cap && !d_pairs) return fail(c, DIM3_ERR_INVALID, "null output buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    c->qstream = st;
+    struct Arena {  // the scratch array for this call only; every slot starts empty
+        dim3_ctx* c;
+        explicit Arena(dim3_ctx* cc) : c(cc) {
+            for (auto& w : c->nab->ws) w = Slot();
+            c->arena = c->scratch;
+            c->arena_used = 0;
+            c->arena_size = kScratchBytes;
+        }
+        ~Arena() { c->arena = nullptr; }
+    } arena(c);
+    TRY(prepare(c, false, (const int2*)d_R, n_r, (const int2*)d_S, n_s, o, st));
+    TRY(expand(c, fp, d_pairs, cap, st));
+    fill_result(c, res, *c->h_sc, true);
+    if (!fp && res->n > cap)
+        return fail(c, DIM3_ERR_CAPACITY, "output needs %llu rows, buffer holds %llu", (unsigned long long)res->n,
+                    (unsigned long long)cap);
+    return DIM3_OK;
 }
 
 int dim3_batch_join_project_dev(dim3_ctx* c, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
