@@ -81,6 +81,9 @@ def lib():
         L.oracle_zfactor.restype = u64
         L.oracle_free.argtypes = [vp]
         L.oracle_free.restype = None
+        L.oracle_project_fp_bitset.argtypes = [vp, u64, vp, u64, i32, u64, ctypes.c_uint32, ctypes.c_uint32, i32,
+                                               vp, vp]
+        L.oracle_project_fp_bitset.restype = i32
         _lib = L
     return _lib
 
@@ -161,6 +164,48 @@ def join_project(R, S, exclude_equal=False, fingerprint_only=False):
         raise MemoryError("oracle_join_project failed rc=%d" % rc)
     pairs = None if fingerprint_only else _take_pairs(L, out, n.value, L.oracle_free)
     return pairs, fp, bool(sw.value)
+
+
+_FP_JOB = None
+
+
+def _fp_slice(rem):
+    R, S, exclude_equal, heavy, xmod, with_paths = _FP_JOB
+    fp = Fingerprint()
+    sw = ctypes.c_int()
+    rc = lib().oracle_project_fp_bitset(_p(R), R.shape[0], _p(S), S.shape[0], int(exclude_equal), int(heavy),
+                                        int(xmod), int(rem), int(with_paths), ctypes.byref(fp), ctypes.byref(sw))
+    if rc != 0:
+        raise MemoryError("oracle_project_fp_bitset failed rc=%d" % rc)
+    return fp.as_tuple(), bool(sw.value)
+
+
+def fingerprint_bitset(R, S, exclude_equal=False, heavy=1024, slices=1, workers=1, count=False, only=None):
+    """Join-project fingerprint (count=True: the COUNT fingerprint) by the
+    bitset restatement, oracle_project_fp_bitset: keys with at least `heavy`
+    S tuples as z bitsets, rows split into `slices` x slices ((u32)x % slices)
+    run by `workers` forked processes and summed mod 2^64 (only=r: slice r
+    alone).  Returns (Fingerprint, swapped)."""
+    global _FP_JOB
+    R, S = _pairs(R), _pairs(S)
+    _FP_JOB = (R, S, exclude_equal, heavy, slices, count)
+    rems = [only] if only is not None else list(range(max(slices, 1)))
+    try:
+        if workers > 1 and len(rems) > 1:
+            import multiprocessing as mp
+
+            with mp.get_context("fork").Pool(min(workers, len(rems))) as pool:
+                parts = pool.map(_fp_slice, rems)
+        else:
+            parts = [_fp_slice(r) for r in rems]
+    finally:
+        _FP_JOB = None
+    acc = [0, 0, 0, 0]
+    for t, _ in parts:
+        acc = [(a + v) & 0xFFFFFFFFFFFFFFFF for a, v in zip(acc, t)]
+    fp = Fingerprint()
+    fp.count, fp.sum, fp.sum2, fp.csum = acc
+    return fp, parts[0][1]
 
 
 def join_aggregate_count(R, S, exclude_equal=False, fingerprint_only=False):

@@ -29,6 +29,9 @@
  *   oracle_estimate         EstimateJoinCardinality       dim3.hpp:193-224, 262-269
  *   oracle_outj             exact sum_b degR(b)*degS(b)   (closed form of the above)
  *   oracle_gen_rand_data    gen_rand_data (minstd_rand0)  dim3.hpp:1556-1568
+ *   oracle_project_fp_bitset join-project (and COUNT) fingerprints with heavy
+ *                           keys as z bitsets, by x slices (config scale):
+ *                           the dense/sparse hybrid of dim3.hpp:894-973 per key
  *
  * Deliberate differences, none of which is observable in the output set:
  *   - no radix partitioning (dim3.hpp:600-731 is a cache optimisation),
@@ -421,6 +424,147 @@ int oracle_join_project(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, u
     if (rc) { free(k.pairs); return rc; }
     if (out_pairs) *out_pairs = k.pairs; else free(k.pairs);
     return ORC_OK;
+}
+
+/* Join-project fingerprint with the heavy join keys as z bitsets, for
+ * config-scale checks (1e10-1e12 join paths).  It restates the reference's
+ * hybrid (dim3.hpp:894-973: a dense part of bitvectors, a sparse SPA part)
+ * per join key: a key whose S row holds at least `heavy` tuples keeps one
+ * bitset over the z IDs (numbered here in z value order, so a key whose z
+ * values cluster spans few words), OR-ed into the row's bitset word by word;
+ * the other keys set their paths' bits one by one.  The row's set bits are
+ * its distinct z.  Only rows whose x value has (u32)x % xmod == xrem are
+ * expanded (xmod <= 1: every row), so disjoint slices add up to the whole
+ * fingerprint.  with_paths: the COUNT fingerprint, whose count-weighted
+ * terms are sums over the join paths: sum2 = sum over paths of k2(x) g(z) =
+ * sum over the row's keys of k2(x) GS(y), GS(y) the sum of g over S[y], and
+ * csum = the row's paths (not with exclude_equal).  Swap rule as
+ * oracle_join_project. */
+int oracle_project_fp_bitset(const int32_t* R_xy, uint64_t nR, const int32_t* S_zy, uint64_t nS, int exclude_equal,
+                             uint64_t heavy, uint32_t xmod, uint32_t xrem, int with_paths, orc_fp* fp, int* swapped) {
+    const int32_t* R = R_xy; const int32_t* S = S_zy;
+    int sw = 0;
+    if (with_paths && exclude_equal) return ORC_INVALID;
+    if (nR < nS) { const int32_t* t = R; R = S; S = t; uint64_t u = nR; nR = nS; nS = u; sw = 1; }
+    if (swapped) *swapped = sw;
+    memset(fp, 0, sizeof(*fp));
+    orc_plan p;
+    int rc = build_plan(R, nR, S, nS, &p);
+    if (rc) return rc;
+    if (p.nx == 0) { plan_free(&p); return ORC_OK; }
+    const int32_t nz = p.nz, ny = p.ny;
+    const uint64_t nw = ((uint64_t)nz + 63) / 64;
+    /* z IDs renumbered by value: rank[id] */
+    int32_t* order = (int32_t*)malloc((size_t)nz * sizeof(int32_t));
+    int32_t* rank = (int32_t*)malloc((size_t)nz * sizeof(int32_t));
+    uint64_t* gz = (uint64_t*)malloc((size_t)nz * sizeof(uint64_t));
+    int32_t* zval = (int32_t*)malloc((size_t)nz * sizeof(int32_t));
+    int32_t* hid = (int32_t*)malloc((size_t)(ny ? ny : 1) * sizeof(int32_t));
+    uint64_t* gs = with_paths ? (uint64_t*)calloc((size_t)(ny ? ny : 1), sizeof(uint64_t)) : NULL;
+    uint64_t* row = (uint64_t*)calloc(nw, sizeof(uint64_t));
+    uint64_t* bits = NULL; uint32_t* hlo = NULL; uint32_t* hhi = NULL;
+    if (!order || !rank || !gz || !zval || !hid || (with_paths && !gs) || !row) { rc = ORC_OOM; goto out; }
+    {
+        /* sort z IDs by value (counting is not possible over int32: a
+         * radix sort of (value, id) pairs in two 16-bit passes) */
+        int32_t* tmp = (int32_t*)malloc((size_t)nz * sizeof(int32_t));
+        uint64_t* cnt = (uint64_t*)calloc(65537, sizeof(uint64_t));
+        if (!tmp || !cnt) { free(tmp); free(cnt); rc = ORC_OOM; goto out; }
+        for (int32_t i = 0; i < nz; i++) order[i] = i;
+        for (int pass = 0; pass < 2; pass++) {
+            const int shift = pass * 16;
+            memset(cnt, 0, 65537 * sizeof(uint64_t));
+            for (int32_t i = 0; i < nz; i++) {
+                uint32_t key = ((uint32_t)p.id2z[order[i]] ^ 0x80000000u) >> shift & 0xFFFFu;
+                cnt[key + 1]++;
+            }
+            for (int k = 0; k < 65536; k++) cnt[k + 1] += cnt[k];
+            for (int32_t i = 0; i < nz; i++) {
+                uint32_t key = ((uint32_t)p.id2z[order[i]] ^ 0x80000000u) >> shift & 0xFFFFu;
+                tmp[cnt[key]++] = order[i];
+            }
+            memcpy(order, tmp, (size_t)nz * sizeof(int32_t));
+        }
+        free(tmp); free(cnt);
+        for (int32_t r = 0; r < nz; r++) {
+            rank[order[r]] = r;
+            zval[r] = p.id2z[order[r]];
+            gz[r] = orc_zfactor(zval[r]);
+        }
+    }
+    /* heavy keys and their bitsets (over value-ordered z IDs) */
+    int32_t K = 0;
+    for (int32_t y = 0; y < ny; y++) hid[y] = (p.sjr[y + 1] - p.sjr[y] >= heavy && heavy > 0) ? K++ : -1;
+    if (K) {
+        bits = (uint64_t*)calloc((size_t)K * nw, sizeof(uint64_t));
+        hlo = (uint32_t*)malloc((size_t)K * sizeof(uint32_t));
+        hhi = (uint32_t*)malloc((size_t)K * sizeof(uint32_t));
+        if (!bits || !hlo || !hhi) { rc = ORC_OOM; goto out; }
+    }
+    for (int32_t y = 0; y < ny; y++) {
+        const int32_t h = hid[y];
+        uint64_t acc = 0;
+        uint32_t lo = (uint32_t)nw, hi = 0;
+        for (uint64_t j = p.sjr[y]; j < p.sjr[y + 1]; j++) {
+            const int32_t z = rank[p.sic[j]];
+            p.sic[j] = z;  /* the plan's S columns in value order from here on */
+            if (with_paths) acc += gz[z];
+            if (h >= 0) {
+                bits[(size_t)h * nw + (uint32_t)z / 64] |= 1ull << (z & 63);
+                if ((uint32_t)z / 64 < lo) lo = (uint32_t)z / 64;
+                if ((uint32_t)z / 64 > hi) hi = (uint32_t)z / 64;
+            }
+        }
+        if (with_paths) gs[y] = acc;
+        if (h >= 0) { hlo[h] = lo; hhi[h] = hi; }
+    }
+    for (int32_t x = 0; x < p.nx; x++) {
+        const int32_t xv = p.id2x[x];
+        if (xmod > 1 && (uint32_t)xv % xmod != xrem) continue;
+        uint32_t lo = (uint32_t)nw, hi = 0;
+        uint64_t s2 = 0, cs = 0;
+        for (uint64_t i = p.rjr[x]; i < p.rjr[x + 1]; i++) {
+            const int32_t y = p.ric[i];
+            if (with_paths) { s2 += gs[y]; cs += p.sjr[y + 1] - p.sjr[y]; }
+            const int32_t h = hid[y];
+            if (h >= 0) {
+                const uint64_t* b = bits + (size_t)h * nw;
+                for (uint32_t w = hlo[h]; w <= hhi[h]; w++) row[w] |= b[w];
+                if (hlo[h] < lo) lo = hlo[h];
+                if (hhi[h] > hi) hi = hhi[h];
+            } else {
+                for (uint64_t j = p.sjr[y]; j < p.sjr[y + 1]; j++) {
+                    const uint32_t z = (uint32_t)p.sic[j];
+                    row[z / 64] |= 1ull << (z & 63);
+                    if (z / 64 < lo) lo = z / 64;
+                    if (z / 64 > hi) hi = z / 64;
+                }
+            }
+        }
+        uint64_t s1 = 0, n = 0;
+        for (uint32_t w = lo; w <= hi && lo <= hi; w++) {
+            uint64_t v = row[w];
+            row[w] = 0;
+            while (v) {
+                const uint32_t z = w * 64 + (uint32_t)__builtin_ctzll(v);
+                v &= v - 1;
+                if (exclude_equal && zval[z] == xv) continue;
+                s1 += gz[z];
+                n++;
+            }
+        }
+        const uint64_t ux = (uint64_t)(uint32_t)xv;
+        const uint64_t k1 = orc_mix64(ux ^ 0x243F6A8885A308D3ULL) | 1u, k2 = orc_mix64(ux ^ 0x13198A2E03707344ULL) | 1u;
+        fp->count += n;
+        fp->sum += k1 * s1;
+        if (with_paths) { fp->sum2 += k2 * s2; fp->csum += cs; }
+        else fp->sum2 += k2 * s1;
+    }
+out:
+    free(order); free(rank); free(gz); free(zval); free(hid); free(gs); free(row);
+    free(bits); free(hlo); free(hhi);
+    plan_free(&p);
+    return rc;
 }
 
 /* Join-aggregate COUNT(*) GROUP BY x,z.  Same swap rule (dim3.hpp:1325-1326). */

@@ -117,6 +117,82 @@ LARGE = {
 }
 
 
+# config-scale cases checked through the bitset restatement
+# (oracle.fingerprint_bitset): the two-hop graph of the headline benchmark
+# with u != w (and the reference on two of its 256 vertex slices), and the
+# largest instance of the 256-instance batch (project and COUNT)
+GRAPH_FULL = {"gen": "two_hop_graph", "nv": 1_000_000, "ne": 4_000_000, "gamma": 2.1, "dmax": 50_000, "seed": 7,
+              "exclude_equal": True, "slice_mod": 256, "slices": [0, 1], "heavy": 4096}
+BATCH_LARGEST = {"gen": "batch_instance", "n_instances": 256, "seed": 1000, "rank": 0, "heavy": 4096}
+
+
+def graph_full():
+    wl = _wl()
+    s = GRAPH_FULL
+    return wl.two_hop_graph(s["nv"], s["ne"], s["gamma"], s["dmax"], s["seed"])
+
+
+def batch_largest(spec=BATCH_LARGEST):
+    """The instance of batch_instance_params(n_instances, seed) with the
+    rank-th largest size (numpy generator), and its parameters."""
+    wl = _wl()
+    params = wl.batch_instance_params(spec["n_instances"], seed=spec["seed"])
+    order = sorted(range(len(params)), key=lambda i: (-params[i]["n"], i))
+    p = params[order[spec["rank"]]]
+    R, S = wl.batch_instance(p)
+    return R, S, order[spec["rank"]], p
+
+
+# The served stream of the served-stream test: Poisson arrivals of
+# join-project (80%) and COUNT (20%) requests with the batch size law, and
+# the requests at positions `large_at` made large: a planted block_keys x
+# block_keys block at `density` (keys 0..block_keys-1, the hottest of the
+# background) added to a Zipf(1.0) background of large_bg tuples per side
+# (the specification's 5e7 reduced; the block is the specified one).
+SERVED = {"n_requests": 20, "seed": 4242, "rate_per_s": 40.0, "large_at": [4, 13], "large_bg": 1_000_000,
+          "block_keys": 20_000, "density": 0.3, "heavy": 1024}
+
+
+def served_large_relation(n_bg, block_keys, density, seed):
+    """Zipf(1.0) background (attribute domain 10%, key domain 1% of n_bg)
+    plus every (attr, key) cell of the block kept with probability
+    `density`; the background's tuples inside the block that the block also
+    holds are dropped (the union is distinct)."""
+    wl = _wl()
+    bg = wl.zipf_relation(n_bg, max(1, n_bg // 10), max(1, n_bg // 100), 1.0, seed + 17)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    parts, mask = [], np.zeros((block_keys, block_keys), dtype=bool)
+    for r0 in range(0, block_keys, 2048):
+        rows = min(2048, block_keys - r0)
+        m = rng.random((rows, block_keys), dtype=np.float32) < density
+        mask[r0:r0 + rows] = m
+        xi, yi = np.nonzero(m)
+        parts.append(np.stack([(xi + r0).astype(np.int32), yi.astype(np.int32)], 1))
+    inb = (bg[:, 0] < block_keys) & (bg[:, 1] < block_keys) & (bg[:, 0] >= 0) & (bg[:, 1] >= 0)
+    dup = np.zeros(bg.shape[0], dtype=bool)
+    dup[inb] = mask[bg[inb, 0], bg[inb, 1]]
+    return np.concatenate(parts + [bg[~dup]])
+
+
+def served_requests(spec=SERVED):
+    """[(request, R, S)] of the served stream, request = the served_stream
+    descriptor (id, arrival t at rate_per_s, kind, params)."""
+    wl = _wl()
+    reqs = wl.served_stream(spec["n_requests"], rate_per_s=spec["rate_per_s"], large_frac=0.0, seed=spec["seed"])
+    out = []
+    for q in reqs:
+        p = q["params"]
+        if q["id"] in spec["large_at"]:
+            p = dict(p, large=True, n=spec["large_bg"])
+            q = dict(q, params=p)
+            R = served_large_relation(spec["large_bg"], spec["block_keys"], spec["density"], p["seed_r"])
+            S = served_large_relation(spec["large_bg"], spec["block_keys"], spec["density"], p["seed_s"])
+        else:
+            R, S = wl.batch_instance(p)
+        out.append((q, R, S))
+    return out
+
+
 def make_large_full(spec):
     """The unsliced relations of a LARGE case."""
     wl = _wl()

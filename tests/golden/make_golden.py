@@ -128,6 +128,79 @@ def large():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def large_scale():
+    """Full-size headline graph and the largest batch instance, added to
+    large.json: fingerprints by oracle.fingerprint_bitset (pinned against
+    the SPA oracle and the compiled reference in tests/test_oracle.py), and
+    the compiled reference's own rows on two vertex slices of the graph
+    (src % 256 == r against all of S; the reference swaps, |R_r| < |S|, so
+    its (S.first, R.first) rows are turned back to (u, w); u != w applied
+    after it, as the reference has no such filter), each cross-checked by
+    the bitset oracle on the same x slice."""
+    path = os.path.join(HERE, "large.json")
+    with open(path) as f:
+        out = json.load(f)
+    workers = min(8, os.cpu_count() or 1)
+    spec = cases.GRAPH_FULL
+    E = cases.graph_full()
+    rec = {"spec": spec, "sha256": sha(E, E), "n_r": int(E.shape[0]), "n_s": int(E.shape[0])}
+    fp, _ = oracle.fingerprint_bitset(E, E, exclude_equal=True, heavy=spec["heavy"], slices=64, workers=workers)
+    rec["project"] = {"n": int(fp.count), "fp": [str(v) for v in fp.as_tuple()]}
+    print("graph_full", rec["project"], flush=True)
+    rec["slices"] = {}
+    for r in spec["slices"]:
+        Rr = E[(E[:, 0] % spec["slice_mod"]) == r]
+        pairs = oracle.ref_join_project(Rr, E, parallel=True)
+        if Rr.shape[0] < E.shape[0]:
+            pairs = pairs[:, ::-1]
+        pairs = np.ascontiguousarray(pairs[pairs[:, 0] != pairs[:, 1]])
+        assert np.all(pairs[:, 0] % spec["slice_mod"] == r)
+        fr = oracle.fingerprint(pairs).as_tuple()
+        sl = _slice_fp(E, spec, r)
+        assert sl == fr, (r, sl, fr)
+        rec["slices"][str(r)] = {"n": int(fr[0]), "fp": [str(v) for v in fr], "checker": "reference"}
+        print("graph slice", r, fr, flush=True)
+        del pairs
+    out["graph_full"] = rec
+    bspec = cases.BATCH_LARGEST
+    R, S, idx, p = cases.batch_largest(bspec)
+    brec = {"spec": bspec, "instance": idx, "params": p, "sha256": sha(R, S), "n_r": int(R.shape[0]),
+            "n_s": int(S.shape[0])}
+    fp, sw = oracle.fingerprint_bitset(R, S, heavy=bspec["heavy"], slices=64, workers=workers)
+    brec["project"] = {"n": int(fp.count), "fp": [str(v) for v in fp.as_tuple()], "swapped": sw}
+    fpc, _ = oracle.fingerprint_bitset(R, S, heavy=bspec["heavy"], slices=64, workers=workers, count=True)
+    brec["aggregate"] = {"n": int(fpc.count), "fp": [str(v) for v in fpc.as_tuple()]}
+    print("batch_largest", idx, p["n"], brec["project"], brec["aggregate"], flush=True)
+    out["batch_largest"] = brec
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+def _slice_fp(E, spec, r):
+    """The bitset oracle's fingerprint of the rows of x slice r alone."""
+    fp, _ = oracle.fingerprint_bitset(E, E, exclude_equal=True, heavy=spec["heavy"], slices=spec["slice_mod"], only=r)
+    return fp.as_tuple()
+
+
+def served():
+    """Per-request fingerprints of the served stream (served.json), by the
+    bitset oracle (COUNT requests: the COUNT fingerprint)."""
+    spec = cases.SERVED
+    workers = min(8, os.cpu_count() or 1)
+    out = {"spec": spec, "requests": []}
+    for q, R, S in cases.served_requests(spec):
+        count = q["kind"] == "aggregate"
+        big = q["params"].get("large", False)
+        fp, sw = oracle.fingerprint_bitset(R, S, heavy=spec["heavy"], slices=16 if big else 1,
+                                           workers=workers if big else 1, count=count)
+        rec = {"id": q["id"], "t": q["t"], "kind": q["kind"], "large": big, "n_r": int(R.shape[0]),
+               "n_s": int(S.shape[0]), "sha256": sha(R, S), "fp": [str(v) for v in fp.as_tuple()], "swapped": sw}
+        out["requests"].append(rec)
+        print(json.dumps(rec), flush=True)
+    with open(os.path.join(HERE, "served.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("reference checker missing: make -C oracle ref")
@@ -141,3 +214,7 @@ if __name__ == "__main__":
         medium()
     if "large" in which:
         large()
+    if "large_scale" in which:
+        large_scale()
+    if "served" in which:
+        served()

@@ -18,6 +18,7 @@ import sys
 import numpy as np
 import pytest
 
+import oracle
 from tests.golden import cases
 from tests.golden.make_golden import sha
 
@@ -187,3 +188,101 @@ def test_aggregate_full_slices(ctx, dim3, torch):
         acc = [(a + v) & M64 for a, v in zip(acc, part.fp.as_tuple())]
         say("slice", r, part.n)
     assert tuple(acc) == full.fp.as_tuple()
+
+
+def _row_terms(torch, x, z, pos):
+    """Per-row fingerprint terms k1(x) g(z) and k2(x) g(z) (int64, mod 2^64)."""
+    x = x.to(torch.int64) & 0xFFFFFFFF
+    z = z.to(torch.int64) & 0xFFFFFFFF
+    g = (_fmix32((z >> 5) ^ 0x243F6A88) | 1) * pos[z & 31]
+    k1 = _mix64(x ^ _s64(0x243F6A8885A308D3)) | 1
+    k2 = _mix64(x ^ _s64(0x13198A2E03707344)) | 1
+    return x, k1 * g, k2 * g
+
+
+def test_graph_full(ctx, dim3, torch):
+    """The headline workload at full size: the two-hop self-join of the
+    Chung-Lu graph (1M vertices, 4M undirected edges, u != w), 15.1e9 rows.
+    Fingerprint mode, and the output materialised in HBM and fingerprinted on
+    the device, against the oracle's fingerprint of the whole query; the rows
+    of two vertex slices (u % 256 = 0, 1) of that same output against the
+    compiled reference's rows for those slices; and the slices queried on
+    their own (R restricted to the slice, no swap)."""
+    rec = _large()["graph_full"]
+    spec = rec["spec"]
+    E = cases.graph_full()
+    assert sha(E, E) == rec["sha256"], "generator drift"
+    want = tuple(int(v) for v in rec["project"]["fp"])
+    dE = torch.from_numpy(E).cuda()
+    n = E.shape[0]
+    say("graph_full fingerprint mode")
+    rc, f = ctx.join_project_dev(dE.data_ptr(), n, dE.data_ptr(), n, 0, 0, output_mode=1, exclude_equal=True)
+    assert rc == dim3.OK and f.fp.as_tuple() == want, (f.fp.as_tuple(), want)
+    for r in spec["slices"]:
+        ws = tuple(int(v) for v in rec["slices"][str(r)]["fp"])
+        sl = dE[(dE[:, 0] % spec["slice_mod"]) == r].contiguous()
+        rc, g = ctx.join_project_dev(sl.data_ptr(), sl.shape[0], dE.data_ptr(), n, 0, 0, output_mode=1,
+                                     exclude_equal=True, no_swap=True)
+        say("slice query", r, g.n)
+        assert rc == dim3.OK and g.fp.as_tuple() == ws
+    say("graph_full materialise", want[0], "rows")
+    out = torch.empty((want[0], 2), dtype=torch.int32, device="cuda")
+    rc, m = ctx.join_project_dev(dE.data_ptr(), n, dE.data_ptr(), n, out.data_ptr(), want[0], exclude_equal=True)
+    assert rc == dim3.OK and m.n == want[0]
+    torch.cuda.synchronize()
+    pos = torch.tensor([_s64(_mix64_int((0x9E3779B97F4A7C15 * (b + 1)) & M64) | 1) for b in range(32)],
+                       dtype=torch.int64, device="cuda")
+    s1 = s2 = 0
+    sl = {r: [0, 0, 0] for r in spec["slices"]}
+    chunk = 1 << 27
+    for i in range(0, want[0], chunk):
+        x, t1, t2 = _row_terms(torch, out[i:i + chunk, 0], out[i:i + chunk, 1], pos)
+        s1 = (s1 + _sum_mod64(t1)) & M64
+        s2 = (s2 + _sum_mod64(t2)) & M64
+        for r in spec["slices"]:
+            keep = (x % spec["slice_mod"]) == r
+            a = sl[r]
+            a[0] += int(keep.sum().item())
+            a[1] = (a[1] + _sum_mod64(t1[keep])) & M64
+            a[2] = (a[2] + _sum_mod64(t2[keep])) & M64
+        del x, t1, t2
+    say("materialised fingerprint", s1, s2)
+    assert (want[0], s1, s2) == want[:3]
+    for r in spec["slices"]:
+        ws = tuple(int(v) for v in rec["slices"][str(r)]["fp"])
+        assert tuple(sl[r]) == ws[:3], (r, sl[r], ws)
+    del out
+
+
+def test_batch_largest_instance(ctx, dim3, torch):
+    """The largest instance of the 256-instance batch (1.99M tuples per side,
+    Zipf s = 1.47 over a 19,923-key domain, 3.97e10 distinct pairs) alone
+    (join-project and COUNT fingerprints) and inside a batch with smaller
+    instances of the same law, against the oracle."""
+    rec = _large()["batch_largest"]
+    R, S, idx, p = cases.batch_largest(rec["spec"])
+    assert idx == rec["instance"] and sha(R, S) == rec["sha256"], "generator drift"
+    want = tuple(int(v) for v in rec["project"]["fp"])
+    wagg = tuple(int(v) for v in rec["aggregate"]["fp"])
+    dR, dS = torch.from_numpy(R).cuda(), torch.from_numpy(S).cuda()
+    say("batch largest alone", R.shape, S.shape)
+    rc, g = ctx.join_project_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, output_mode=1)
+    assert rc == dim3.OK and g.fp.as_tuple() == want, g.stats.as_dict()
+    rc, c = ctx.join_aggregate_count_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, output_mode=1)
+    assert rc == dim3.OK and c.fp.as_tuple() == wagg
+    from dim3_amd import workloads
+
+    small = [workloads.batch_instance(q) for q in workloads.batch_instance_params(6, lo=20_000, hi=200_000, seed=77)]
+    dev = [(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()) for a, b in small]
+    inst = [(a.data_ptr(), a.shape[0], b.data_ptr(), b.shape[0]) for a, b in dev]
+    inst.insert(3, (dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0]))
+    say("batch with the largest instance")
+    for count in (False, True):
+        fn = ctx.batch_join_aggregate_count_dev if count else ctx.batch_join_project_dev
+        rc, res = fn(inst, output_mode=1)
+        assert rc == dim3.OK
+        assert res[3].fp.as_tuple() == (wagg if count else want)
+        for k, (a, b) in enumerate(small):
+            j = k if k < 3 else k + 1
+            fpk, _ = oracle.fingerprint_bitset(a, b, heavy=1024, count=count)
+            assert res[j].fp.as_tuple() == fpk.as_tuple(), (count, j)

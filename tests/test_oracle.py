@@ -407,3 +407,66 @@ def test_workload_generator_matches_oracle_generator():
     R2, S2 = oracle.gen_rand_data(300_000, 200_000, 50_000, 20_000, 50_000, 77)
     assert np.array_equal(R, R2) and np.array_equal(S, S2)
     assert np.array_equal(vR, np.arange(300_000, dtype=np.float32)) and vS.shape == (200_000,)
+
+
+def _bitset_cases():
+    rng = np.random.default_rng(11)
+    out = []
+    # uniform, Zipf keys (heavy keys), negative and wide values, duplicates
+    out.append(("uniform", rand(rng, 3000, 400, 300), rand(rng, 2500, 500, 300)))
+    zk = lambda n, dom: np.minimum((rng.pareto(0.8, n) * 3).astype(np.int64), dom - 1)
+    R = np.stack([rng.integers(-5000, 5000, 20000), zk(20000, 2000)], 1).astype(np.int32)
+    S = np.stack([rng.integers(-2**31, 2**31 - 1, 15000), zk(15000, 2000)], 1).astype(np.int32)
+    out.append(("zipf_wide", R, S))
+    S2 = np.concatenate([S, S[:3000]])  # duplicate tuples
+    out.append(("zipf_dups", R, S2))
+    out.append(("swap", S[:4000], R))
+    return out
+
+
+def rand(rng, n, xs, ys):
+    return np.stack([rng.integers(0, xs, n), rng.integers(0, ys, n)], 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("heavy", [0, 1, 8, 64, 1 << 40])
+def test_bitset_fingerprint_equals_spa(heavy):
+    """oracle_project_fp_bitset (heavy keys as z bitsets, x slices) equals the
+    SPA restatement on every case, for every heavy threshold (0 and 2^40: no
+    bitset; 1: every key a bitset), summed over 1 or 7 x slices."""
+    for name, R, S in _bitset_cases():
+        _, want, sw = oracle.join_project(R, S, fingerprint_only=True)
+        for slices in (1, 7):
+            got, gsw = oracle.fingerprint_bitset(R, S, heavy=heavy, slices=slices)
+            assert got.as_tuple() == want.as_tuple() and gsw == sw, (name, heavy, slices)
+
+
+def test_bitset_fingerprint_self_join_and_count():
+    """The u != w filter on a symmetric graph, and the COUNT fingerprint
+    (groups from the bitsets, count-weighted terms from path sums) against
+    the SPA COUNT with per-group counters."""
+    rng = np.random.default_rng(5)
+    E = rand(rng, 6000, 900, 900)
+    E = np.unique(np.concatenate([E, E[:, ::-1]]), axis=0)
+    _, want, _ = oracle.join_project(E, E, exclude_equal=True, fingerprint_only=True)
+    for heavy in (0, 4, 16):
+        got, _ = oracle.fingerprint_bitset(E, E, exclude_equal=True, heavy=heavy, slices=3, workers=3)
+        assert got.as_tuple() == want.as_tuple()
+    for name, R, S in _bitset_cases():
+        _, _, want, _ = oracle.join_aggregate_count(R, S, fingerprint_only=True)
+        got, _ = oracle.fingerprint_bitset(R, S, heavy=16, slices=5, count=True)
+        assert got.as_tuple() == want.as_tuple(), name
+
+
+def test_bitset_fingerprint_against_reference():
+    """On a medium Zipf input the bitset fingerprint equals the fingerprint
+    of the compiled reference's own rows (DIM3_parallel)."""
+    if not oracle.ref_available():
+        pytest.skip("reference checker not built")
+    rng = np.random.default_rng(8)
+    zk = np.minimum((rng.pareto(1.0, 60000) * 5).astype(np.int64), 4999)
+    R = np.stack([rng.integers(0, 20000, 60000), zk], 1).astype(np.int32)
+    zk = np.minimum((rng.pareto(1.0, 50000) * 5).astype(np.int64), 4999)
+    S = np.stack([rng.integers(0, 20000, 50000), zk], 1).astype(np.int32)
+    pairs = oracle.ref_join_project(R, S, parallel=True)
+    got, _ = oracle.fingerprint_bitset(R, S, heavy=200, slices=4, workers=4)
+    assert got.as_tuple() == oracle.fingerprint(pairs).as_tuple()
