@@ -10,7 +10,11 @@ Prints ONE JSON line.  `value` is distinct output pairs per second.
 `roofline` is for the dominant kernel (the sparse expansion), timed with
 HIP events on the stream it runs on; `cpu_baseline` times the reference
 DIM3_parallel (OpenMP) on a bounded sample of the same query (vertex
-slices of R against all of S) on this host's cores.
+slices of R against all of S) on this host's cores; `serving` is the
+metric's latency half, measured after the timed region: a 64-request
+served stream (Poisson arrivals, 80% join-project / 20% COUNT, the batch
+size law, two large requests with a planted 20k x 20k block at 0.3) run
+through the request server and one request at a time, p50 / p99 of both.
 """
 import argparse
 import json
@@ -476,19 +480,25 @@ def percentile(v, q):
     return float(np.percentile(np.asarray(v, dtype=np.float64), q))
 
 
-def serve_bench(args, d, torch):
-    """Served stream: Poisson arrivals of join-project (80%) and COUNT (20%)
-    requests with the batch workload's size law plus 5% large requests
-    (Zipf background plus a planted 20k x 20k block at 0.3).  The same
-    stream is replayed against the server (workers, continuous batching)
-    and against one request at a time (one worker, no batching) at an
-    arrival rate of --load times the one-at-a-time capacity."""
+def served_stream_block(d, torch, n_requests, large_n, load, workers, large_at=None, seed=4242):
+    """Replay of a served stream: Poisson arrivals of join-project (80%) and
+    COUNT (20%) requests with the batch workload's size law plus large
+    requests (a Zipf background of large_n tuples per side plus a planted
+    20k x 20k block at 0.3; large_at: the positions made large, else 5% at
+    random), against the server (workers, continuous batching) and against
+    one request at a time (one worker, no batching), at an arrival rate of
+    `load` times the one-at-a-time capacity; every request's fingerprint
+    must agree between the two.  Returns (serving dict, stream dict)."""
     import threading
 
     from dim3_amd import workloads
 
     t0 = time.perf_counter()
-    reqs = workloads.served_stream(args.requests, rate_per_s=1.0, seed=4242, large_n=args.large_n)
+    reqs = workloads.served_stream(n_requests, rate_per_s=1.0, seed=seed, large_n=large_n,
+                                   large_frac=0.0 if large_at is not None else 0.05)
+    if large_at is not None:
+        for i in large_at:
+            reqs[i]["params"] = dict(reqs[i]["params"], large=True, n=large_n)
     rels = []
     for q in reqs:
         p = q["params"]
@@ -539,11 +549,11 @@ def serve_bench(args, d, torch):
     # service capacity of the plain path: the requests back to back, one at a time
     _, t_plain_all, _ = replay(1, 1, [0.0] * len(reqs))
     capacity = len(reqs) / t_plain_all
-    lam = args.load * capacity
+    lam = load * capacity
     gaps = np.array([reqs[0]["t"]] + [reqs[i]["t"] - reqs[i - 1]["t"] for i in range(1, len(reqs))])
     arrivals = list(np.cumsum(gaps / lam))  # unit-rate Poisson times rescaled to rate lam
     log("plain capacity %.1f req/s (%.2fs back to back); arrival rate %.1f req/s" % (capacity, t_plain_all, lam))
-    served, span_s, st_s = replay(args.workers, 32, arrivals)
+    served, span_s, st_s = replay(workers, 32, arrivals)
     plain, span_p, _ = replay(1, 1, arrivals)
     for i, ((a, _, _), (b, _, _)) in enumerate(zip(served, plain)):
         if a.fp.as_tuple() != b.fp.as_tuple():
@@ -551,31 +561,40 @@ def serve_bench(args, d, torch):
     pairs = sum(r.n for r, _, _ in served)
     lat_s = [1000 * t for _, _, t in served]
     lat_p = [1000 * t for _, _, t in plain]
+    serving = {"requests": len(reqs), "large_requests": n_large, "large_background_tuples": large_n,
+               "arrival_rate_per_s": lam, "load_vs_plain_capacity": load, "plain_capacity_req_per_s": capacity,
+               "p50_ms": percentile(lat_s, 50), "p99_ms": percentile(lat_s, 99),
+               "plain_p50_ms": percentile(lat_p, 50), "plain_p99_ms": percentile(lat_p, 99),
+               "served_pairs_per_s": pairs / span_s, "plain_pairs_per_s": pairs / span_p,
+               "served_req_per_s": len(reqs) / span_s, "plain_req_per_s": len(reqs) / span_p,
+               "workers": workers, "max_batch": 32, "batches": st_s["batches"],
+               "batched_requests": st_s["batched_requests"], "peak_bytes": st_s["peak_bytes"]}
+    stream = {"pairs": pairs, "span_s": span_s, "tuples": n_tuples, "large": n_large, "requests": len(reqs)}
+    return serving, stream
+
+
+def serve_bench(args, d, torch):
+    """The served stream on its own (see served_stream_block)."""
+    serving, stream = served_stream_block(d, torch, args.requests, args.large_n, args.load, args.workers)
     line = {
         "metric": METRIC,
-        "value": pairs / span_s,
+        "value": stream["pairs"] / stream["span_s"],
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": 1,
         "warmup": 1,
-        "ms_per_step": 1000 * span_s,
+        "ms_per_step": 1000 * stream["span_s"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (seeded Zipf / planted-block requests drawn on the device, resident in HBM)",
-        "config": {"workload": "served stream of join-project (80%) and COUNT (20%) requests", "requests": len(reqs),
-                   "large_requests": n_large, "large_background_tuples": args.large_n,
-                   "large_block": "20000 x 20000 keys at 0.3, added to the background", "tuples": n_tuples,
+        "config": {"workload": "served stream of join-project (80%) and COUNT (20%) requests",
+                   "requests": stream["requests"], "large_requests": stream["large"],
+                   "large_background_tuples": args.large_n,
+                   "large_block": "20000 x 20000 keys at 0.3, added to the background", "tuples": stream["tuples"],
                    "output": "fingerprint per request", "parallelism": "single GPU"},
-        "serving": {"arrival_rate_per_s": lam, "load_vs_plain_capacity": args.load,
-                    "plain_capacity_req_per_s": capacity,
-                    "p50_ms": percentile(lat_s, 50), "p99_ms": percentile(lat_s, 99),
-                    "plain_p50_ms": percentile(lat_p, 50), "plain_p99_ms": percentile(lat_p, 99),
-                    "served_req_per_s": len(reqs) / span_s, "plain_req_per_s": len(reqs) / span_p,
-                    "plain_pairs_per_s": pairs / span_p, "workers": args.workers, "max_batch": 32,
-                    "batches": st_s["batches"], "batched_requests": st_s["batched_requests"],
-                    "peak_bytes": st_s["peak_bytes"]},
+        "serving": serving,
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -647,6 +666,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-slices", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="two_hop: skip the served-stream block (served vs one-at-a-time p50/p99 latency)")
+    ap.add_argument("--serving-requests", type=int, default=64,
+                    help="two_hop: requests of the served-stream block (two of them large: a planted 20k x 20k "
+                         "block at 0.3 on a Zipf background of --serving-large-n tuples)")
+    ap.add_argument("--serving-large-n", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -747,6 +772,20 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %r" % (e,))
 
+    serving = None
+    if not planted and not args.no_serving:
+        # the metric's second half: served vs one-at-a-time latency on a
+        # compact stream (after the timed region; its own timing)
+        try:
+            del out
+        except NameError:
+            pass
+        torch.cuda.empty_cache()
+        n_req = max(args.serving_requests, 8)
+        serving, _ = served_stream_block(d, torch, n_req, args.serving_large_n, args.load, args.workers,
+                                         large_at=[n_req // 8, (5 * n_req) // 8])
+        log("serving: " + json.dumps(serving))
+
     output = "materialized int32 pairs in HBM" if args.mode == "materialize" else "fingerprint only"
     if planted:
         config = {"workload": "planted dense block join-project", "block_keys": args.block_keys,
@@ -788,6 +827,7 @@ def main():
                      "alg_bytes": b_kernel},
         "cpu_baseline": cpu,
         "dense": dense,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -254,6 +254,10 @@ struct dim3_ctx {
     // while such a query runs
     uint8_t* scratch = nullptr;
     std::unique_ptr<Bank> nab;
+    // the batch launch's instantiated graph, replayed while its shape and
+    // buffers are those of the call (captured again when they change)
+    hipGraphExec_t batch_exec = nullptr;
+    uint64_t batch_key[12] = {};
     uint8_t* arena = nullptr;
     size_t arena_used = 0, arena_size = 0;
 
@@ -591,7 +595,13 @@ void plan_dense(dim3_ctx* c, const dim3_opts& o) {
     }
     P.p1_gain = best_mode == DIM3_DENSE_OFF ? -1e300 : best_gain;
     if (best_mode == DIM3_DENSE_OFF) return;
-    if (o.dense_mode == DIM3_DENSE_AUTO && o.heavy_k == 0 && best_gain <= 0) return;
+    // The rates are good to about a factor of two (a fused seed's bitmap
+    // word has measured between 0.4 and 1 path's cost), so AUTO takes a
+    // dense part only when it promises to halve the sparse work it replaces:
+    // marginal ones (the two-hop graph's hubs, whose 3%-dense bitmaps cost
+    // what their columns cost) stay sparse.
+    if (o.dense_mode == DIM3_DENSE_AUTO && o.heavy_k == 0 && best_gain <= 0.5 * (double)best_shape.paths * m.path_s)
+        return;
     P.dense = best_mode;
     P.bmin = best_b;
     P.K = (uint32_t)best_shape.K;
@@ -1743,28 +1753,47 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     int e = dim3h::expand_batch_setup(kcount, block, lds, c->grid_cus(), std::max<uint64_t>(total, 1), &grid);
     if (e != 0) return fail(c, DIM3_ERR_DEVICE, "batch setup: %s", hipGetErrorString((hipError_t)e));
 
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    // no allocation anywhere in the process while the capture is open
-    std::unique_lock<std::shared_mutex> capture_lock(dim3h::capture_mutex());
-    HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    dim3h::expand_batch_launch(kcount, fp, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl, d_scout,
-                               st);
-    hipMemcpyAsync(c->h_batch, d_scout, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st);
-    HIPCHK(c, hipStreamEndCapture(st, &graph));
-    capture_lock.unlock();
-    hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ge != hipSuccess) {
+    // The launch is a HIP graph (scalar reset, expansion, scalar gather, copy
+    // to the host).  Its kernels read every instance's arguments from device
+    // tables, so the instantiated graph is replayed by later batches whose
+    // shape (kernel, workgroup, LDS, grid, instance count) and table and
+    // host buffers are the same; otherwise it is captured again.
+    const uint64_t key[12] = {(uint64_t)kcount, (uint64_t)fp, (uint64_t)block, (uint64_t)lds, (uint64_t)grid,
+                              (uint64_t)n, (uint64_t)(uintptr_t)d_args, (uint64_t)(uintptr_t)d_start,
+                              (uint64_t)(uintptr_t)d_scl, (uint64_t)(uintptr_t)d_scout,
+                              (uint64_t)(uintptr_t)c->h_batch, (uint64_t)(uintptr_t)st};
+    if (c->batch_exec && memcmp(key, c->batch_key, sizeof(key)) != 0) {
+        hipGraphExecDestroy(c->batch_exec);
+        c->batch_exec = nullptr;
+    }
+    if (!c->batch_exec) {
+        hipGraph_t graph = nullptr;
+        {
+            // no allocation anywhere in the process while the capture is open
+            std::unique_lock<std::shared_mutex> capture_lock(dim3h::capture_mutex());
+            HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            dim3h::expand_batch_launch(kcount, fp, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl,
+                                       d_scout, st);
+            hipMemcpyAsync(c->h_batch, d_scout, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st);
+            HIPCHK(c, hipStreamEndCapture(st, &graph));
+        }
+        const hipError_t ie = hipGraphInstantiate(&c->batch_exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
-        return fail(c, DIM3_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ge));
+        if (ie != hipSuccess) {
+            c->batch_exec = nullptr;
+            return fail(c, DIM3_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+        }
+        memcpy(c->batch_key, key, sizeof(key));
     }
     HIPCHK(c, hipEventRecord(c->ev[4], st));
-    ge = hipGraphLaunch(exec, st);
+    hipError_t ge = hipGraphLaunch(c->batch_exec, st);
     if (ge == hipSuccess) ge = hipEventRecord(c->ev[5], st);
     if (ge == hipSuccess) ge = hipStreamSynchronize(st);
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
-    if (ge != hipSuccess) return fail(c, DIM3_ERR_DEVICE, "batch launch: %s", hipGetErrorString(ge));
+    if (ge != hipSuccess) {
+        hipGraphExecDestroy(c->batch_exec);
+        c->batch_exec = nullptr;
+        return fail(c, DIM3_ERR_DEVICE, "batch launch: %s", hipGetErrorString(ge));
+    }
     float ms_expand = 0;
     hipEventElapsedTime(&ms_expand, c->ev[4], c->ev[5]);
 
@@ -2040,6 +2069,7 @@ int dim3_ctx_destroy(dim3_ctx* c) {
     for (auto& b : c->banks)
         for (auto& s : b->ws)
             if (s.p) dim3h::dev_free(s.p);
+    if (c->batch_exec) hipGraphExecDestroy(c->batch_exec);
     if (c->h_batch) dim3h::host_free(c->h_batch);
     if (c->h_stream) dim3h::host_free(c->h_stream);
     for (auto& e : c->ev)
