@@ -31,6 +31,9 @@ using dim3h::Scalars;
 // address comes from hipGetSymbolAddress, once per device (scratch_address).
 constexpr size_t kScratchBytes = 48ull << 20;
 constexpr uint64_t kNoAllocMaxTuples = 1ull << 17;  // per relation: the workspace then fits the array
+// project tiles: blocked bit layout, blocks of 2^kTileBlockLog words (32,
+// 64 and 128 measured alike on the two-hop graph)
+constexpr uint32_t kTileBlockLog = 5;
 static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 
 namespace {
@@ -400,7 +403,15 @@ void plan_tiles(dim3_ctx* c, const dim3_opts& o, int force_wgs = 0) {
             tw = (nz_words + nt - 1) / nt;
         }
         if (tw == 0) tw = 1;
-        if (!cnt && ((tw + 3) & ~3ull) <= max_words) tw = (tw + 3) & ~3ull;  // 16-byte tile rows
+        // project tiles: multiples of the block width (the blocked bit
+        // layout) where they fit and the caller did not fix a width that is
+        // not one, else 16-byte tile rows
+        if (!cnt) {
+            const uint64_t bw = 1ull << kTileBlockLog;
+            const bool blk = ((tw + bw - 1) & ~(bw - 1)) <= max_words && (!forced_units || forced_units % (32u * bw) == 0);
+            const uint64_t r = blk ? bw - 1 : 3u;
+            if (((tw + r) & ~r) <= max_words) tw = (tw + r) & ~r;
+        }
         if (tw > max_words) return k;
         k.tile_words = (uint32_t)tw;
         k.ntiles = (uint32_t)(((uint64_t)P.nz + tw * per_word - 1) / (tw * per_word));
@@ -692,8 +703,9 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
 }
 
 // Tile bit layout and fused key-bitmap width of a planned query.  Project
-// tiles are swizzled (hot z IDs spread over words and banks) unless whole
-// words pay more: a fingerprint over a dense z range (a plain word sums its
+// Project tiles take the blocked bit layout (hot z IDs spread over words and
+// banks) unless the tile width is not a multiple of 32 words or whole words
+// pay more: a fingerprint over a dense z range (a plain word sums its
 // 32 z factors with one word factor and four byte-table lookups: the two-hop
 // graph's fingerprint query 64.8 -> 42.5 ms, all-sparse), or an MFMA product
 // whose columns are every z ID (word copies into the tile).  A fused bit-or
@@ -702,7 +714,7 @@ bool plan_core(dim3_ctx* c, const dim3_opts& o) {
 // of its keys.
 void set_layout(QueryPlan& P, const dim3_opts& o) {
     P.plain = (o.output_mode == DIM3_OUT_FINGERPRINT && P.z_ident) ||
-              (P.dense == DIM3_DENSE_MFMA && P.all_z);
+              (P.dense == DIM3_DENSE_MFMA && P.all_z) || P.tile_words % (1u << kTileBlockLog) != 0;
     if (P.fused) P.dwords = P.ntiles * P.tile_words;
 }
 
@@ -1046,7 +1058,7 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
             // product rows: plain bitmaps over the columns
             dim3h::dense_build_b(2, dS, slotS, slotSz, P.z_ident ? nullptr : exZ, P.zlo, nS, exY, hid, zflag, hzpos,
                                  0, P.dwords, B, overflow, st, P.fused ? P.tile_words : 0u,
-                                 P.fused && !P.plain ? P.tile_words / 32u : 0u);
+                                 P.fused && !P.plain ? kTileBlockLog : 0u);
             if (!P.fused) {  // fused: the seeding reads B itself
                 const int e = dim3h::bitor_rows(rowR, colR, hid, hx2x, P.Mh, B, P.dwords, dbits, st);
                 if (e != 0)
@@ -1103,7 +1115,7 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     a.stage_cap = P.stage_cap;
     // plain bit layout only where the dense seed copies whole words (every z
     // ID a product column); elsewhere the swizzle keeps hub z off one word
-    a.swz = P.plain ? 0u : 1u;
+    a.swz = P.plain ? 0u : kTileBlockLog;
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
     if (P.core) a.splitS = (const uint32_t*)W[WS_CNT_SD].p;

@@ -313,6 +313,26 @@ __device__ __forceinline__ T block_reduce_sum(T v, T* sh) {
     return r;
 }
 
+// ---------------------------------------------------------------------------
+// bit layout of project tiles (LDS bitmaps over the local z offsets zl of a
+// z tile)
+// ---------------------------------------------------------------------------
+// Plain (m = 0, sh = 0): offset zl is bit zl & 31 of word zl >> 5.
+// Blocked by B = 2^sh words (m = B - 1, sh >= 5; tile words a multiple of
+// B): every run of B words holds 32 B consecutive offsets, offset j of a run
+// being bit j >> sh of the run's word j & m, so B neighbouring z IDs (the
+// hubs of a graph numbered by degree) fall in B different words and LDS
+// banks.  One formula serves all: word = ((zl >> 5) & ~m) | (zl & m),
+// bit = (zl >> sh) & 31, and back zl = tl_base(word) + (bit << sh).  A tile
+// base that is a multiple of 32 B (blocked) or 32 (plain) offsets commutes
+// with both maps, so the maps can take the z ID itself and subtract the
+// base's word afterwards.
+__host__ __device__ __forceinline__ uint32_t tl_word(uint32_t zl, uint32_t m) { return ((zl >> 5) & ~m) | (zl & m); }
+__host__ __device__ __forceinline__ uint32_t tl_bit(uint32_t zl, uint32_t sh) { return 1u << ((zl >> sh) & 31u); }
+__host__ __device__ __forceinline__ uint32_t tl_base(uint32_t word, uint32_t m) {
+    return ((word & ~m) << 5) | (word & m);
+}
+
 }  // namespace dim3d
 
 // ---------------------------------------------------------------------------
@@ -505,7 +525,7 @@ struct ExpandArgs {
     uint32_t touched_cap;
     uint32_t stage_cap;
     uint32_t fp_mode;
-    uint32_t swz;  // project tile bit layout: 1 swizzled, 0 plain (dense seeding copies words)
+    uint32_t swz;  // project tile bit layout: log2 of the block width in words (tile_words a multiple of it), 0 plain
     // the (x, z-tile) tasks of this launch: x * ntiles + t in [task_lo, task_hi)
     // (all rows, or one chunk of rows of a streamed query)
     uint32_t task_lo, task_hi;
@@ -584,12 +604,12 @@ void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, con
                    const uint32_t* hx2x, uint32_t Mh, uint32_t Kp, uint32_t* A, uint64_t* overflow, hipStream_t st);
 // from the heavy S tuples: kind 0: Bt [Nh][Kp] 0/1 bytes, 1: Bt byte
 // multiplicities, 2: bits of per-key z bitmaps [K][dwords] (tile_words > 0:
-// in the layout of project tiles of tile_words words, swizzled with stride
-// swz_c, 0 = plain)
+// in the layout of project tiles of tile_words words, blocked by 2^blk
+// words when blk > 0, else plain)
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
                    int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* zflag,
                    const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
-                   hipStream_t st, uint32_t tile_words = 0, uint32_t swz_c = 0);
+                   hipStream_t st, uint32_t tile_words = 0, uint32_t blk = 0);
 
 // bound[x] = min(cap, the S paths of row x plus the product columns of a
 // product row) >= min(cap, output rows of row x); cap <= nz

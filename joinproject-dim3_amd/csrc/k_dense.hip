@@ -591,15 +591,15 @@ __device__ __forceinline__ uint32_t z_of(const int2* S, uint64_t i, const int32_
 // Bt[hz][h] for each S tuple with a heavy key (MODE 0: 0/1, 1: multiplicity),
 // or bit hz of the z bitmap of key h (MODE 2, bit-or).  MODE 2 with tw > 0:
 // the bitmap in the layout of project tiles of tw words (tile t = hz / 32 tw
-// holds words t tw .. t tw + tw - 1; local offset zl = 32 q + r is bit r of
-// word q + r swz_c mod tw, plain when swz_c = 0), so a fused seed ORs whole
-// words into its tile.
+// holds words t tw .. t tw + tw - 1, local offset zl in the layout blocked
+// by 2^blk words when blk > 0, else plain), so a fused seed ORs whole words
+// into its tile.
 template <int MODE>
 __global__ void k_build_b(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
                           const int32_t* __restrict__ slotSz, const uint64_t* __restrict__ ex_z, int32_t zlo,
                           uint64_t nS, const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid,
                           const uint32_t* __restrict__ zflag, const uint32_t* __restrict__ hzpos, uint32_t Kp,
-                          uint32_t dwords, uint32_t* B, uint64_t* overflow, uint32_t tw, uint32_t swz_c) {
+                          uint32_t dwords, uint32_t* B, uint64_t* overflow, uint32_t tw, uint32_t blk) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nS) return;
     const int32_t sy = slotS[i];
@@ -610,14 +610,13 @@ __global__ void k_build_b(const int2* __restrict__ S, const int32_t* __restrict_
     if (!zflag[zid]) return;  // a light column: its paths stay sparse
     const uint32_t hz = hzpos[zid];
     if (MODE == 2) {
-        uint32_t word = hz >> 5;
-        if (tw != 0u && swz_c != 0u) {
+        uint32_t word = hz >> 5, bit = 1u << (hz & 31u);
+        if (tw != 0u && blk != 0u) {
             const uint32_t t = hz / (32u * tw), zl = hz - t * 32u * tw;
-            uint32_t q = (zl >> 5) + (zl & 31u) * swz_c;  // < 2 tw: swz_c <= tw / 32
-            if (q >= tw) q -= tw;
-            word = t * tw + q;
+            word = t * tw + tl_word(zl, (1u << blk) - 1u);
+            bit = tl_bit(zl, blk);
         }
-        atomicOr(&B[(uint64_t)h * dwords + word], 1u << (hz & 31u));
+        atomicOr(&B[(uint64_t)h * dwords + word], bit);
     } else {
         put_byte<MODE == 1>(B, (uint64_t)hz * Kp + (uint32_t)h, overflow);
     }
@@ -736,12 +735,12 @@ void dense_build_a(bool count, const uint32_t* rowptrR, const int32_t* colR, con
 void dense_build_b(int kind, const int2* S, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_z,
                    int32_t zlo, uint64_t nS, const uint64_t* ex_y, const int32_t* hid, const uint32_t* zflag,
                    const uint32_t* hzpos, uint32_t Kp, uint32_t dwords, uint32_t* B, uint64_t* overflow,
-                   hipStream_t st, uint32_t tile_words, uint32_t swz_c) {
+                   hipStream_t st, uint32_t tile_words, uint32_t blk) {
     if (!nS) return;
     const unsigned g = grid1(nS);
     if (kind == 2)
         hipLaunchKernelGGL(k_build_b<2>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, zflag,
-                           hzpos, Kp, dwords, B, overflow, tile_words, swz_c);
+                           hzpos, Kp, dwords, B, overflow, tile_words, blk);
     else if (kind == 1)
         hipLaunchKernelGGL(k_build_b<1>, dim3(g), dim3(256), 0, st, S, slotS, slotSz, ex_z, zlo, nS, ex_y, hid, zflag,
                            hzpos, Kp, dwords, B, overflow, 0u, 0u);

@@ -35,6 +35,9 @@ using dim3h::ExpandArgs;
 constexpr uint32_t kUnroll = 4;
 constexpr uint32_t kPiece = 64 * kUnroll;  // join paths per wavefront work item
 constexpr uint32_t kPairs = 2;             // pieces in flight per wavefront
+// word 3 of a raw (untyped, unswizzled) buffer descriptor on CDNA: 32-bit
+// data format, so loads past num_records return 0
+constexpr int kRawBufferWord3 = 0x00020000;
 
 __device__ __forceinline__ uint32_t list_at(const uint32_t* touched, bool listed, uint32_t i) {
     return listed ? touched[i] : i;
@@ -77,25 +80,12 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
 // One output row (plain store: non-temporal stores measured slower here).
 __device__ __forceinline__ void put_pair(int2* p, int32_t x, int32_t z) { *p = make_int2(x, z); }
 
-// Project tiles store local z offset zl = 32 q + r in bit r of word
-// (q + r * C) mod W (W words, C = W / 32): the 32 offsets that share a word
-// in the plain layout land in 32 different words and LDS banks, so hot z
-// values with neighbouring IDs (hubs of a graph numbered by degree) do not
-// serialise on one word.
-__device__ __forceinline__ uint32_t swz_word(uint32_t zl, uint32_t W, uint32_t C) {
-    const uint32_t q = (zl >> 5) + __umul24(zl & 31u, C);  // C < 2^24: the full-rate 24-bit multiply
-    return q >= W ? q - W : q;
-}
-
-__device__ __forceinline__ uint32_t swz_offset(uint32_t word, uint32_t r, uint32_t W, uint32_t C) {
-    int32_t q = (int32_t)word - (int32_t)__umul24(r, C);
-    if (q < 0) q += (int32_t)W;
-    return (uint32_t)q * 32u + r;
-}
-
+// Project tiles use the blocked bit layout (tl_word / tl_bit / tl_base) when
+// ExpandArgs::swz is set, else the plain one; COUNT tiles are one counter per
+// local z offset.
 template <bool COUNT>
-__device__ __forceinline__ uint32_t list_item(uint32_t zl, uint32_t W, uint32_t C) {
-    return COUNT ? zl : swz_word(zl, W, C);
+__device__ __forceinline__ uint32_t list_item(uint32_t zl, uint32_t m) {
+    return COUNT ? zl : tl_word(zl, m);
 }
 
 template <bool COUNT>
@@ -126,28 +116,40 @@ struct Lds {
 };
 
 // Emission of a swept join-project task (rows too dense for the touched
-// list) in one pass, without a counting sweep: the block sum of the first-set
-// bits (nb) is the row count, reserved with one global atomic; then every
-// wavefront walks its slice of the tile 64 words at a time, clears them,
-// places the chunk's rows by a wavefront prefix of the word popcounts into
-// its own LDS stage region (the touched list and the area, free here) and
-// flushes full regions with coalesced 8-byte stores at offsets handed out by
-// an LDS cursor.  No block barrier between the reservation and the end.
-// Returns false, reserving nothing, for a row denser than a quarter of its
-// tile: its rows go through the block-staged emission instead (balanced
-// per-thread shares written block-wide, measured faster there: 1M x 1M Zipf
-// join-project, 69% dense rows).
+// list) in one pass over the tile after a popcount of it: the row count is
+// reserved with one global atomic; then every wavefront walks its share of
+// the tile 64 words at a time, clears them, places the chunk's rows by a
+// wavefront prefix of the word popcounts into its own LDS stage region (the
+// touched list and the area, free here) and flushes full regions with
+// coalesced 8-byte stores at offsets handed out by an LDS cursor.  The stage
+// holds z values (dense z range) or local z offsets.  Returns false,
+// reserving nothing, for a row denser than a quarter of its tile: its rows go
+// through the block-staged emission instead (balanced per-thread shares
+// written block-wide, measured faster there: 1M x 1M Zipf join-project, 69%
+// dense rows).
 template <int BLOCK, bool ZID>
-__device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, int32_t xv, uint32_t zbase, uint32_t nb,
-                                           uint32_t self_item, uint32_t self_keep, uint32_t wlo, uint32_t whi) {
+__device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, int32_t xv, uint32_t zbase,
+                                           uint32_t self_item, uint32_t self_keep) {
     constexpr uint32_t NW = BLOCK / 64;
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     uint32_t* tile = L.tile;
     const uint32_t W = a.tile_words;
-    const uint32_t C = a.swz ? W / 32u : 0u;
-    const uint32_t wsum = wave_sum_u32(nb);
-    if (lane == 0) L.s_wtot[wid] = wsum;
+    const uint32_t lsh = a.swz, lm = lsh ? (1u << lsh) - 1u : 0u;
+    {
+        // the row count: popcount of the tile, 16 bytes per lane and step
+        // (the tile starts 16-byte aligned)
+        uint32_t c = 0;
+        const uint32_t W4 = W >> 2;
+        const uint4* t4 = reinterpret_cast<const uint4*>(tile);
+        for (uint32_t i = tid; i < W4; i += BLOCK) {
+            const uint4 q = t4[i];
+            c += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+        }
+        for (uint32_t i = 4u * W4 + tid; i < W; i += BLOCK) c += __popc(tile[i]);
+        const uint32_t wsum = wave_sum_u32(c);
+        if (lane == 0) L.s_wtot[wid] = wsum;
+    }
     __syncthreads();
     if (tid == 0) {
         uint32_t T = 0;
@@ -170,7 +172,7 @@ __device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, in
     const uint32_t region = a.touched_cap + max(a.stage_cap, 3u * (uint32_t)BLOCK + 1u);
     const uint32_t SW = region / NW;
     uint32_t* stg = L.touched + wid * SW;
-    const uint32_t zoff = (uint32_t)a.zlo + zbase;
+    const uint32_t zoff = ZID ? (uint32_t)a.zlo + zbase : 0u;  // stage entry = zoff + local offset
     const int32_t* id2z = a.id2z + zbase;
     int2* out = reinterpret_cast<int2*>(a.out);
     uint32_t fill = 0;  // wavefront-uniform
@@ -187,27 +189,29 @@ __device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, in
             uint32_t k = lane;
             for (; k + 192u < n; k += 256u) {
                 const uint32_t e0 = stg[k], e1 = stg[k + 64u], e2 = stg[k + 128u], e3 = stg[k + 192u];
-                put_pair(o + k, xv, ZID ? (int32_t)(zoff + e0) : id2z[e0]);
-                put_pair(o + k + 64u, xv, ZID ? (int32_t)(zoff + e1) : id2z[e1]);
-                put_pair(o + k + 128u, xv, ZID ? (int32_t)(zoff + e2) : id2z[e2]);
-                put_pair(o + k + 192u, xv, ZID ? (int32_t)(zoff + e3) : id2z[e3]);
+                put_pair(o + k, xv, ZID ? (int32_t)e0 : id2z[e0]);
+                put_pair(o + k + 64u, xv, ZID ? (int32_t)e1 : id2z[e1]);
+                put_pair(o + k + 128u, xv, ZID ? (int32_t)e2 : id2z[e2]);
+                put_pair(o + k + 192u, xv, ZID ? (int32_t)e3 : id2z[e3]);
             }
             for (; k < n; k += 64u) {
                 const uint32_t e = stg[k];
-                put_pair(o + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                put_pair(o + k, xv, ZID ? (int32_t)e : id2z[e]);
             }
         } else {
             for (uint32_t k = lane; k < n; k += 64u) {
                 const uint32_t e = stg[k];
-                if (g0 + k < a.cap) put_pair(out + g0 + k, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
+                if (g0 + k < a.cap) put_pair(out + g0 + k, xv, ZID ? (int32_t)e : id2z[e]);
             }
         }
         __builtin_amdgcn_wave_barrier();
     };
-    for (uint32_t i0 = wlo; i0 < whi; i0 += 64u) {
+    // 64-word chunks dealt to the wavefronts round robin: a dense z range
+    // (the hubs) spreads over all of them instead of loading one
+    for (uint32_t i0 = wid * 64u; i0 < W; i0 += 64u * NW) {
         const uint32_t w = i0 + lane;
         uint32_t v = 0;
-        if (w < whi) {
+        if (w < W) {
             v = tile[w];
             if (v) tile[w] = 0u;
         }
@@ -219,18 +223,27 @@ __device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, in
             flush(fill);
             fill = 0;
         }
+        const uint32_t zw = zoff + tl_base(w, lm);  // entry of the word's bit 0
         if (tot > SW) {
-            // a chunk denser than the stage region: each lane writes its own rows
-            uint32_t off = 0;
-            if (lane == 0) off = atomicAdd(L.s_cursor, tot);
-            off = (uint32_t)__builtin_amdgcn_readfirstlane((int)off);
-            uint64_t pos = gbase + off + incl - cnt;
-            while (v) {
-                const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                v &= v - 1u;
-                const uint32_t e = swz_offset(w, b, W, C);
-                if (pos < a.cap) put_pair(out + pos, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
-                ++pos;
+            // A chunk denser than the stage region (the hub z IDs of a graph
+            // numbered by degree fill whole words): rounds of at most SW / 64
+            // rows per lane, each staged and flushed with coalesced stores.
+            // The stage is empty here (flushed above).
+            const uint32_t m = SW / 64u;
+            for (;;) {
+                uint32_t cr = (uint32_t)__popc(v);
+                cr = cr < m ? cr : m;
+                const uint32_t ir = wave_scan_u32(cr);
+                const uint32_t tr = (uint32_t)__builtin_amdgcn_readlane((int)ir, 63);
+                if (tr == 0u) break;
+                uint32_t p = ir - cr;
+                for (uint32_t j = 0; j < cr; j++) {
+                    const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                    v &= v - 1u;
+                    stg[p++] = zw + (b << lsh);
+                }
+                __builtin_amdgcn_wave_barrier();
+                flush(tr);
             }
             continue;
         }
@@ -238,7 +251,7 @@ __device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, in
         while (v) {
             const uint32_t b = (uint32_t)__ffs(v) - 1u;
             v &= v - 1u;
-            stg[p++] = swz_offset(w, b, W, C);
+            stg[p++] = zw + (b << lsh);
         }
         fill += tot;
         __builtin_amdgcn_wave_barrier();
@@ -263,19 +276,18 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t W = a.tile_words;
-    const uint32_t C = (COUNT || !a.swz) ? 0u : W / 32u;  // bit swizzle stride of project tiles (0: plain)
+    const uint32_t lsh = COUNT ? 0u : a.swz, lm = lsh ? (1u << lsh) - 1u : 0u;  // tile bit layout
     const uint32_t dummy = min(lane, W - 1u);             // an empty path slot's own word (it adds 0)
     // rows with more paths than this sweep their tile instead of listing first touches
     const uint64_t track_limit = min(4ull * a.touched_cap, 4ull * a.tile_words);
     const uint32_t zbase = t * a.tile_size;
+    // word of the tile base: a path's z ID maps straight to a word of tb
+    const uint32_t wbase = COUNT ? zbase : zbase >> 5;
+    uint32_t* const tb = tile - wbase;
     const uint32_t rb = a.rowptrR[x], re = a.rowptrR[x + 1];
     const int32_t xv = a.id2x[x];
     const int32_t zs_self = a.x2z != nullptr ? a.x2z[x] : -1;
 
-    // Bits this thread set first (seed and paths; join-project): their block
-    // sum is the task's row count, so an untracked task needs no counting
-    // sweep before it emits.
-    uint32_t nb = 0;
     // ---- dense seed: a row with a heavy key starts from its product row ----
     const int32_t hxr = a.hx != nullptr ? a.hx[x] : -1;
     if (hxr >= 0) {
@@ -343,8 +355,6 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                         for (uint32_t u = 0; u < 4; u++) {
                             if (okq[u] && (acc[u].x | acc[u].y | acc[u].z | acc[u].w) != 0u) {
                                 uint4 t4 = tile4[qq[u]];
-                                nb += __popc(acc[u].x & ~t4.x) + __popc(acc[u].y & ~t4.y) +
-                                      __popc(acc[u].z & ~t4.z) + __popc(acc[u].w & ~t4.w);
                                 t4.x |= acc[u].x; t4.y |= acc[u].y; t4.z |= acc[u].z; t4.w |= acc[u].w;
                                 tile4[qq[u]] = t4;
                             }
@@ -368,9 +378,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                     for (uint32_t u = 0; u < 4; u++) {
                         const uint32_t q = q0 + u * BLOCK;
                         if (q < wn && acc[u] != 0u) {
-                            const uint32_t old = tile[q];
-                            nb += __popc(acc[u] & ~old);
-                            tile[q] = old | acc[u];
+                            tile[q] |= acc[u];
                         }
                     }
                 }
@@ -383,7 +391,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 const int32_t v = crow[h];
                 if (v != 0) tile[a.hz2zid[h] - zbase] = (uint32_t)v;
             }
-        } else if (h0 < h1 && C == 0u && a.hz2zid[h1 - 1u] - a.hz2zid[h0] == h1 - 1u - h0) {
+        } else if (h0 < h1 && lm == 0u && a.hz2zid[h1 - 1u] - a.hz2zid[h0] == h1 - 1u - h0) {
             // the tile's product columns are consecutive z IDs: shifted word
             // copies (plain bit layout), tile bit p <- column p - off
             const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
@@ -401,7 +409,6 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 if (first > 0) v &= first >= 32 ? 0u : ~0u << first;
                 if (last < 31) v &= last < 0 ? 0u : (~0u >> (31 - last));
                 tile[q] = v;  // one writer per word of a zeroed tile
-                nb += __popc(v);
             }
         } else if (h0 < h1) {
             const uint32_t* brow = a.dbits + (size_t)hxr * a.dwords;
@@ -424,10 +431,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < 4; u++)
-                        if (zl[u] != 0xFFFFFFFFu) {
-                            const uint32_t bit = 1u << (zl[u] & 31u);
-                            nb += (atomicOr(&tile[swz_word(zl[u], W, C)], bit) & bit) ? 0u : 1u;
-                        }
+                        if (zl[u] != 0xFFFFFFFFu) atomicOr(&tile[tl_word(zl[u], lm)], tl_bit(zl[u], lsh));
                 }
             }
         }
@@ -468,7 +472,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
         // searching the whole table.
         uint32_t seg = 0;
         for (uint32_t k0 = wid; k0 < NP; k0 += kPairs * NW) {
-            uint32_t zl[kPairs * kUnroll];
+            uint32_t col[kPairs * kUnroll], lens[kPairs];
 #pragma unroll
             for (uint32_t h = 0; h < kPairs; h++) {
                 const uint32_t k = k0 + h * NW;
@@ -487,25 +491,42 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                     base = s_sb[seg] + off;
                     len = min(s_len[seg] - off, kPiece);
                 }
-                // unconditional loads from a clamped, in-bounds index keep all of
-                // them in flight (a branch per load makes the compiler wait on each)
+                // The piece's columns through a buffer descriptor over exactly
+                // its len words (wavefront-uniform base and size, no per-lane
+                // address arithmetic): a lane past the end reads 0 without a
+                // memory access and is masked below.  All loads are
+                // unconditional, so they stay in flight together.
+                base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+                lens[h] = (uint32_t)__builtin_amdgcn_readfirstlane((int)len);
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void*)(a.colS + base), (short)0, (int)(lens[h] * 4u), kRawBufferWord3);
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; u++) {
-                    const uint32_t idx = lane + 64u * u;
-                    const bool ok = idx < len;
-                    const uint32_t v = (uint32_t)a.colS[ok ? base + idx : base];
-                    zl[h * kUnroll + u] = ok ? v - zbase : 0xFFFFFFFFu;
-                }
+                for (uint32_t u = 0; u < kUnroll; u++)
+                    col[h * kUnroll + u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((lane + 64u * u) * 4u), 0, 0);
             }
-            // Every slot applies one LDS atomic (an empty slot ORs / adds 0
-            // into a word of its own), so the atomics issue back to back with
-            // no branch and, tracked, one wait covers all their returns.
+            // Slot -> (LDS word, bit / increment).  The tile base is a
+            // multiple of 1024 (blocked) or 32 (plain) z IDs, so the layout
+            // maps the z ID itself, to a word of tb (the tile shifted back by
+            // the base's words).
             uint32_t addr[kPairs * kUnroll], val[kPairs * kUnroll];
 #pragma unroll
             for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                const bool ok = zl[u] != 0xFFFFFFFFu;
-                addr[u] = ok ? list_item<COUNT>(zl[u], W, C) : dummy;
-                val[u] = ok ? (COUNT ? 1u : 1u << (zl[u] & 31u)) : 0u;
+                const uint32_t v = col[u];
+                addr[u] = COUNT ? v : tl_word(v, lm);
+                val[u] = COUNT ? 1u : tl_bit(v, lsh);
+            }
+            bool full = true;
+#pragma unroll
+            for (uint32_t h = 0; h < kPairs; h++) full = full && lens[h] == kPiece;
+            if (!full || track) {
+                // an empty slot ORs / adds 0 into a word of its own, so the
+                // atomics issue back to back with no branch
+#pragma unroll
+                for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                    const bool ok = lane + 64u * (u % kUnroll) < lens[u / kUnroll];
+                    addr[u] = ok ? addr[u] : dummy + wbase;
+                    val[u] = ok ? val[u] : 0u;
+                }
             }
             if (track) {
                 // first touches (the word / counter was zero) of all slots go
@@ -514,8 +535,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 uint32_t ftot = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                    const uint32_t old = COUNT ? atomicAdd(&tile[addr[u]], val[u]) : atomicOr(&tile[addr[u]], val[u]);
-                    if (!COUNT) nb += __popc(val[u] & ~old);
+                    const uint32_t old = COUNT ? atomicAdd(&tb[addr[u]], val[u]) : atomicOr(&tb[addr[u]], val[u]);
                     fm[u] = __ballot(val[u] != 0u && old == 0u);
                     ftot += (uint32_t)__popcll(fm[u]);
                 }
@@ -527,20 +547,18 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                     for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
                         if ((fm[u] >> lane) & 1ull) {
                             const uint32_t p = fbase + mbcnt64(fm[u]);
-                            if (p < a.touched_cap) touched[p] = addr[u];
+                            if (p < a.touched_cap) touched[p] = addr[u] - wbase;
                         }
                         fbase += (uint32_t)__popcll(fm[u]);
                     }
                 }
-            } else if (!COUNT && !a.fp_mode) {
-                // untracked join-project rows: the returned words count the new bits
-#pragma unroll
-                for (uint32_t u = 0; u < kPairs * kUnroll; u++) nb += __popc(val[u] & ~atomicOr(&tile[addr[u]], val[u]));
             } else {
+                // untracked: no returns, so the wavefront never waits on its
+                // atomics (the emission counts the swept tile itself)
 #pragma unroll
                 for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                    if (COUNT) atomicAdd(&tile[addr[u]], val[u]);
-                    else atomicOr(&tile[addr[u]], val[u]);
+                    if (COUNT) atomicAdd(&tb[addr[u]], val[u]);
+                    else atomicOr(&tb[addr[u]], val[u]);
                 }
             }
         }
@@ -548,10 +566,11 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     }
 
     // ---- emission ----
-    // The list (touched words, or the whole tile) is cut into one
-    // contiguous slice per wavefront; lane l of a wavefront owns the slice
-    // words wlo + l + 64 g, so a wavefront reads 64 consecutive words per
-    // step (no bank conflicts).
+    // The list (touched words, or the whole tile) is dealt to the wavefronts
+    // in chunks of 64 entries round robin: lane l of wavefront w owns entries
+    // 64 (w + NW g) + l, so a wavefront reads 64 consecutive entries per step
+    // (no bank conflicts) and a dense z range (the hubs of a graph numbered
+    // by degree) spreads over all wavefronts instead of loading one.
     const uint32_t nt = *L.s_nt;
     const bool listed = track && nt <= a.touched_cap;
     const uint32_t count = listed ? nt : a.tile_words;
@@ -560,19 +579,18 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
         const int32_t zs = zs_self;
         const uint32_t zl = (uint32_t)zs - zbase;
         if (zs >= 0 && zl < a.tile_size) {
-            self_item = list_item<COUNT>(zl, W, C);
-            self_keep = COUNT ? 0u : ~(1u << (zl & 31u));
+            self_item = list_item<COUNT>(zl, lm);
+            self_keep = COUNT ? 0u : ~tl_bit(zl, lsh);
         }
     }
-    const uint32_t wper = (count + NW - 1) / NW;
-    const uint32_t wlo = min(wid * wper, count), whi = min(wlo + wper, count);
-    if (!COUNT && !a.fp_mode && !listed && emit_swept<BLOCK, ZID>(a, L, xv, zbase, nb, self_item, self_keep, wlo, whi))
+    const uint32_t cstep = 64u * NW, cfirst = wid * 64u + lane;
+    if (!COUNT && !a.fp_mode && !listed && emit_swept<BLOCK, ZID>(a, L, xv, zbase, self_item, self_keep))
         return;
     uint32_t c = 0;
     uint64_t nzmask = 0;  // bit g: the lane's word of step g holds rows
     {
         uint32_t g = 0;
-        for (uint32_t i = wlo + lane; i < whi; i += 64u, g++) {
+        for (uint32_t i = cfirst; i < count; i += cstep, g++) {
             const uint32_t w = list_at(touched, listed, i);
             uint32_t v = tile[w];
             if (w == self_item) {
@@ -605,9 +623,9 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
         // takes its own z factor.
         const uint64_t* fpt = L.fpt;
         uint64_t S1 = 0, S2 = 0;
-        if (!COUNT && ZID && C == 0u) {
+        if (!COUNT && ZID && lm == 0u) {
             const uint32_t wv0 = zoff >> 5;
-            for (uint32_t i = wlo + lane; i < whi; i += 64u) {
+            for (uint32_t i = cfirst; i < count; i += cstep) {
                 const uint32_t w = list_at(touched, listed, i);
                 const uint32_t v = tile[w];
                 tile[w] = 0;
@@ -619,7 +637,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
             }
             S2 = S1;
         } else {
-            for (uint32_t i = wlo + lane; i < whi; i += 64u) {
+            for (uint32_t i = cfirst; i < count; i += cstep) {
                 const uint32_t w = list_at(touched, listed, i);
                 uint32_t v = tile[w];
                 tile[w] = 0;
@@ -635,7 +653,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                     while (v) {
                         const uint32_t b = (uint32_t)__ffs(v) - 1u;
                         v &= v - 1u;
-                        const uint32_t e = swz_offset(w, b, W, C);
+                        const uint32_t e = tl_base(w, lm) + (b << lsh);
                         const uint32_t u = ZID ? zoff + e : (uint32_t)id2z[e];
                         S1 += (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
                     }
@@ -661,10 +679,10 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
         if (tid == 0) *L.s_gbase = T ? atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T) : 0ull;
         __syncthreads();
         uint64_t pos0 = *L.s_gbase + woff;
-        for (uint32_t i0 = wlo; i0 < whi; i0 += 64u) {
+        for (uint32_t i0 = wid * 64u; i0 < count; i0 += cstep) {
             const uint32_t i = i0 + lane;
             uint32_t v = 0, w = 0;
-            if (i < whi) {
+            if (i < count) {
                 w = list_at(touched, listed, i);
                 v = tile[w];
             }
@@ -694,12 +712,12 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 while (v) {
                     const uint32_t b = (uint32_t)__ffs(v) - 1u;
                     v &= v - 1u;
-                    const uint32_t e = swz_offset(w, b, W, C);
+                    const uint32_t e = tl_base(w, lm) + (b << lsh);
                     if (pos < a.cap) put_pair(out + pos, xv, ZID ? (int32_t)(zoff + e) : id2z[e]);
                     ++pos;
                 }
             }
-            if (i < whi) tile[w] = 0;
+            if (i < count) tile[w] = 0;
         }
         __syncthreads();
         return;
@@ -714,10 +732,10 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     if (tid == 0 && T) my_base = atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
     const uint32_t P = (T + (a.stage_cap - BLOCK) - 1) / (a.stage_cap - BLOCK);
     const uint32_t cq = c / P, cr = c - cq * P;
-    const bool use_mask = whi - wlo <= 64u * 64u;
+    const bool use_mask = count <= wid * 64u + 64u * cstep;  // at most 64 steps per lane
     uint32_t g = 0, cw = 0, cv = 0;
-    if (!use_mask && wlo + lane < whi) {
-        cw = list_at(touched, listed, wlo + lane);
+    if (!use_mask && cfirst < count) {
+        cw = list_at(touched, listed, cfirst);
         cv = tile[cw];
     }
     uint64_t pass_base = 0;
@@ -731,13 +749,13 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 if (cv == 0u) {
                     const uint32_t gg = (uint32_t)__ffsll((unsigned long long)nzmask) - 1u;
                     nzmask &= nzmask - 1ull;
-                    cw = list_at(touched, listed, wlo + lane + 64u * gg);
+                    cw = list_at(touched, listed, cfirst + cstep * gg);
                     cv = tile[cw];
                 }
             } else {
                 while (cv == 0u) {
                     ++g;
-                    cw = list_at(touched, listed, wlo + lane + 64u * g);
+                    cw = list_at(touched, listed, cfirst + cstep * g);
                     cv = tile[cw];
                 }
             }
@@ -747,7 +765,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
             } else {
                 const uint32_t b = (uint32_t)__ffs(cv) - 1u;
                 cv &= cv - 1u;
-                stage[op + j] = swz_offset(cw, b, W, C);
+                stage[op + j] = tl_base(cw, lm) + (b << lsh);
             }
         }
         if (tid == 0 && p == 0) *L.s_gbase = my_base;
@@ -782,7 +800,7 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
         pass_base += nq;
         __syncthreads();
     }
-    for (uint32_t i = wlo + lane; i < whi; i += 64u) tile[list_at(touched, listed, i)] = 0;
+    for (uint32_t i = cfirst; i < count; i += cstep) tile[list_at(touched, listed, i)] = 0;
     __syncthreads();
 }
 

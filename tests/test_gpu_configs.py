@@ -182,6 +182,7 @@ def test_aggregate_full_slices(ctx, dim3, torch):
     mod = 16
     for r in range(mod):
         sl = dS[(dS[:, 0] % mod) == r].contiguous()
+        torch.cuda.synchronize()  # torch's gather runs on its own stream, the query on the context's
         rc, part = ctx.join_aggregate_count_dev(dR.data_ptr(), R.shape[0], sl.data_ptr(), sl.shape[0], 0, 0,
                                                 output_mode=1)
         assert rc == dim3.OK and not part.swapped
@@ -221,6 +222,7 @@ def test_graph_full(ctx, dim3, torch):
     for r in spec["slices"]:
         ws = tuple(int(v) for v in rec["slices"][str(r)]["fp"])
         sl = dE[(dE[:, 0] % spec["slice_mod"]) == r].contiguous()
+        torch.cuda.synchronize()  # torch's gather runs on its own stream, the query on the context's
         rc, g = ctx.join_project_dev(sl.data_ptr(), sl.shape[0], dE.data_ptr(), n, 0, 0, output_mode=1,
                                      exclude_equal=True, no_swap=True)
         say("slice query", r, g.n)

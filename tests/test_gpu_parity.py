@@ -159,6 +159,23 @@ def test_wide_rows_and_overflowing_touch_list(ctx):
     check_count(ctx, R2, S2)
 
 
+@pytest.mark.parametrize("tile_bits", [0, 50000])
+def test_dense_z_cluster_in_a_swept_row(ctx, tile_bits):
+    """Every row reaches all 2048 z IDs of a hub cluster (32 or 64 fully set
+    tile words, more rows per 64 words than a wavefront's stage holds) plus a
+    sparse tail, in rows with too many paths for the first-touch list: the
+    swept emission's dense-chunk rounds.  tile_bits 0: blocked bit layout,
+    50000 (not a multiple of 1024): plain layout over two tiles."""
+    rng = np.random.default_rng(77)
+    ys = np.arange(40)
+    S = np.concatenate([np.stack([rng.permutation(2048), np.full(2048, y)], 1) for y in ys] +
+                       [np.stack([rng.integers(2048, 100000, 300), np.full(300, y)], 1) for y in ys]).astype(np.int32)
+    R = np.stack([np.repeat(np.arange(50), 40), np.tile(ys, 50)], 1).astype(np.int32)
+    got = check_project(ctx, R, S, no_swap=True, tile_bits=tile_bits)
+    say("  rows", got.n, "tiles", got.stats.n_tiles, "tile bits", got.stats.tile_bits)
+    assert got.n > 50 * 2048
+
+
 def test_duplicates_count_multiplicity(ctx):
     rng = np.random.default_rng(4)
     R = rand_rel(rng, 30000, 50, 40)
