@@ -466,99 +466,102 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
         s_pre[tid] = pre;
         if (c0 == rb) track = hxr < 0 && (uint64_t)NP * kPiece <= track_limit;
         __syncthreads();
-        const uint32_t nseg = min((uint32_t)BLOCK, re - c0);
-        // Pieces come in increasing order, so each wavefront gallops
-        // forward from the segment of its previous piece instead of
-        // searching the whole table.
-        uint32_t seg = 0;
-        for (uint32_t k0 = wid; k0 < NP; k0 += kPairs * NW) {
-            uint32_t col[kPairs * kUnroll], lens[kPairs];
+        // A wavefront's j-th piece is wid + NW j.  Each wavefront resolves
+        // the segments of its next 64 pieces at once: lane l binary-searches
+        // the piece prefix table for piece j0 + l (the last segment whose
+        // prefix is <= it; entries past the chunk's last y hold NP), so the
+        // piece loop takes bases and lengths by readlane and never waits on
+        // LDS between pieces.
+        const uint32_t npw = NP > wid ? (NP - wid + NW - 1) / NW : 0u;
+        for (uint32_t j0 = 0; j0 < npw; j0 += 64u) {
+            uint32_t pb = 0, pl = 0;
+            const uint32_t kl = wid + NW * (j0 + lane);
+            if (kl < NP) {
+                uint32_t seg = 0;
 #pragma unroll
-            for (uint32_t h = 0; h < kPairs; h++) {
-                const uint32_t k = k0 + h * NW;
-                uint32_t base = 0, len = 0;
-                if (k < NP) {
-                    uint32_t step = 1;
-                    while (seg + step < nseg && s_pre[seg + step] <= k) {
-                        seg += step;
-                        step <<= 1;
-                    }
-                    while (step > 1) {
-                        step >>= 1;
-                        if (seg + step < nseg && s_pre[seg + step] <= k) seg += step;
-                    }
-                    const uint32_t off = (k - s_pre[seg]) * kPiece;
-                    base = s_sb[seg] + off;
-                    len = min(s_len[seg] - off, kPiece);
+                for (uint32_t step = BLOCK / 2; step > 0; step >>= 1)
+                    if (s_pre[seg + step] <= kl) seg += step;
+                const uint32_t off = (kl - s_pre[seg]) * kPiece;
+                pb = s_sb[seg] + off;
+                pl = min(s_len[seg] - off, kPiece);
+            }
+            const uint32_t nbat = min(npw - j0, 64u);
+            for (uint32_t j = 0; j < nbat; j += kPairs) {
+                uint32_t col[kPairs * kUnroll], lens[kPairs];
+#pragma unroll
+                for (uint32_t h = 0; h < kPairs; h++) {
+                    const uint32_t jj = j + h;
+                    uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)pb, (int)(jj & 63u));
+                    uint32_t len = jj < nbat ? (uint32_t)__builtin_amdgcn_readlane((int)pl, (int)(jj & 63u)) : 0u;
+                    // The piece's columns through a buffer descriptor over exactly
+                    // its len words (wavefront-uniform base and size, no per-lane
+                    // address arithmetic): a lane past the end reads 0 without a
+                    // memory access and is masked below.  All loads are
+                    // unconditional, so they stay in flight together.
+                    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+                    lens[h] = (uint32_t)__builtin_amdgcn_readfirstlane((int)len);
+                    const __amdgpu_buffer_rsrc_t rs =
+                        __builtin_amdgcn_make_buffer_rsrc((void*)(a.colS + base), (short)0, (int)(lens[h] * 4u), kRawBufferWord3);
+#pragma unroll
+                    for (uint32_t u = 0; u < kUnroll; u++)
+                        col[h * kUnroll + u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((lane + 64u * u) * 4u), 0, 0);
                 }
-                // The piece's columns through a buffer descriptor over exactly
-                // its len words (wavefront-uniform base and size, no per-lane
-                // address arithmetic): a lane past the end reads 0 without a
-                // memory access and is masked below.  All loads are
-                // unconditional, so they stay in flight together.
-                base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-                lens[h] = (uint32_t)__builtin_amdgcn_readfirstlane((int)len);
-                const __amdgpu_buffer_rsrc_t rs =
-                    __builtin_amdgcn_make_buffer_rsrc((void*)(a.colS + base), (short)0, (int)(lens[h] * 4u), kRawBufferWord3);
-#pragma unroll
-                for (uint32_t u = 0; u < kUnroll; u++)
-                    col[h * kUnroll + u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((lane + 64u * u) * 4u), 0, 0);
-            }
-            // Slot -> (LDS word, bit / increment).  The tile base is a
-            // multiple of 1024 (blocked) or 32 (plain) z IDs, so the layout
-            // maps the z ID itself, to a word of tb (the tile shifted back by
-            // the base's words).
-            uint32_t addr[kPairs * kUnroll], val[kPairs * kUnroll];
-#pragma unroll
-            for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                const uint32_t v = col[u];
-                addr[u] = COUNT ? v : tl_word(v, lm);
-                val[u] = COUNT ? 1u : tl_bit(v, lsh);
-            }
-            bool full = true;
-#pragma unroll
-            for (uint32_t h = 0; h < kPairs; h++) full = full && lens[h] == kPiece;
-            if (!full || track) {
-                // an empty slot ORs / adds 0 into a word of its own, so the
-                // atomics issue back to back with no branch
+                // Slot -> (LDS word, bit / increment).  The tile base is a
+                // multiple of 1024 (blocked) or 32 (plain) z IDs, so the layout
+                // maps the z ID itself, to a word of tb (the tile shifted back by
+                // the base's words).
+                uint32_t addr[kPairs * kUnroll], val[kPairs * kUnroll];
 #pragma unroll
                 for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                    const bool ok = lane + 64u * (u % kUnroll) < lens[u / kUnroll];
-                    addr[u] = ok ? addr[u] : dummy + wbase;
-                    val[u] = ok ? val[u] : 0u;
+                    const uint32_t v = col[u];
+                    addr[u] = COUNT ? v : tl_word(v, lm);
+                    val[u] = COUNT ? 1u : tl_bit(v, lsh);
                 }
-            }
-            if (track) {
-                // first touches (the word / counter was zero) of all slots go
-                // to the list with one reservation per wavefront
-                uint64_t fm[kPairs * kUnroll];
-                uint32_t ftot = 0;
+                bool full = true;
 #pragma unroll
-                for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                    const uint32_t old = COUNT ? atomicAdd(&tb[addr[u]], val[u]) : atomicOr(&tb[addr[u]], val[u]);
-                    fm[u] = __ballot(val[u] != 0u && old == 0u);
-                    ftot += (uint32_t)__popcll(fm[u]);
-                }
-                if (ftot) {
-                    uint32_t fbase = 0;
-                    if (lane == 0u) fbase = atomicAdd(L.s_nt, ftot);
-                    fbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)fbase);
+                for (uint32_t h = 0; h < kPairs; h++) full = full && lens[h] == kPiece;
+                if (!full || track) {
+                    // an empty slot ORs / adds 0 into a word of its own, so the
+                    // atomics issue back to back with no branch
 #pragma unroll
                     for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                        if ((fm[u] >> lane) & 1ull) {
-                            const uint32_t p = fbase + mbcnt64(fm[u]);
-                            if (p < a.touched_cap) touched[p] = addr[u] - wbase;
-                        }
-                        fbase += (uint32_t)__popcll(fm[u]);
+                        const bool ok = lane + 64u * (u % kUnroll) < lens[u / kUnroll];
+                        addr[u] = ok ? addr[u] : dummy + wbase;
+                        val[u] = ok ? val[u] : 0u;
                     }
                 }
-            } else {
-                // untracked: no returns, so the wavefront never waits on its
-                // atomics (the emission counts the swept tile itself)
+                if (track) {
+                    // first touches (the word / counter was zero) of all slots go
+                    // to the list with one reservation per wavefront
+                    uint64_t fm[kPairs * kUnroll];
+                    uint32_t ftot = 0;
 #pragma unroll
-                for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
-                    if (COUNT) atomicAdd(&tb[addr[u]], val[u]);
-                    else atomicOr(&tb[addr[u]], val[u]);
+                    for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                        const uint32_t old = COUNT ? atomicAdd(&tb[addr[u]], val[u]) : atomicOr(&tb[addr[u]], val[u]);
+                        fm[u] = __ballot(val[u] != 0u && old == 0u);
+                        ftot += (uint32_t)__popcll(fm[u]);
+                    }
+                    if (ftot) {
+                        uint32_t fbase = 0;
+                        if (lane == 0u) fbase = atomicAdd(L.s_nt, ftot);
+                        fbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)fbase);
+#pragma unroll
+                        for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                            if ((fm[u] >> lane) & 1ull) {
+                                const uint32_t p = fbase + mbcnt64(fm[u]);
+                                if (p < a.touched_cap) touched[p] = addr[u] - wbase;
+                            }
+                            fbase += (uint32_t)__popcll(fm[u]);
+                        }
+                    }
+                } else {
+                    // untracked: no returns, so the wavefront never waits on its
+                    // atomics (the emission counts the swept tile itself)
+#pragma unroll
+                    for (uint32_t u = 0; u < kPairs * kUnroll; u++) {
+                        if (COUNT) atomicAdd(&tb[addr[u]], val[u]);
+                        else atomicOr(&tb[addr[u]], val[u]);
+                    }
                 }
             }
         }
