@@ -252,6 +252,40 @@ int dim3_join_aggregate_tuples(dim3_ctx* ctx, const void* R_tuples, uint64_t n_r
                                uint32_t stride, uint32_t value_offset, uint32_t value_type, int agg,
                                const dim3_opts* opts, dim3_result* res);
 
+/* ---- join paths grouped by (x, z): caller-written aggregate policies ----
+ * The reference's join-aggregate takes its aggregate as a policy class
+ * (SpmmAggregate<..., _Aggregate_type>, dim3.hpp:1263-1311, with the policy
+ * concept of AggregateFunctionAvg, dim3.hpp:1453-1480: ctor(nz),
+ * insert(z, v1, v2) on a group's first path, update(z, v1, v2) on the others,
+ * get(z) at the row's end).  The library runs its own policies (COUNT and
+ * the value aggregates above) on the device; for any other policy this call
+ * does the join on the device and returns every join path's values grouped
+ * by (x, z), for the caller's policy code to fold (dim3.hpp's
+ * DIM3JoinAggregate does).  Tuples as dim3_join_aggregate_tuples (int32
+ * attribute at offset 0, int32 key at 4), the value value_size bytes (4 or
+ * 8) at value_offset, both multiples of value_size, value_offset >= 8.
+ * Fills g: n_groups groups sorted by (x, z) (x-major, signed order), group i
+ * with x[i], z[i], a dense z key zkey[i] in [0, nz) (the rank of z among the
+ * distinct S attribute values; nz = their count) and its paths
+ * [path_start[i], path_start[i + 1]); the paths' R values v1 and S values v2
+ * (value_size bytes each), within a group in (R tuple index, S tuple index)
+ * order, as the reference visits them.  opts: exclude_equal drops paths with
+ * x == z; at most 2^31 - 1 join paths.  Free with dim3_grouped_free. */
+typedef struct {
+    uint64_t n_groups, n_paths;
+    uint32_t nz, value_size;
+    int32_t* x;
+    int32_t* z;
+    uint32_t* zkey;
+    uint64_t* path_start; /* n_groups + 1 entries */
+    void* v1;
+    void* v2;
+} dim3_grouped;
+int dim3_join_paths_grouped(dim3_ctx* ctx, const void* R_tuples, uint64_t n_r, const void* S_tuples, uint64_t n_s,
+                            uint32_t stride, uint32_t value_offset, uint32_t value_size, const dim3_opts* opts,
+                            dim3_grouped* g);
+void dim3_grouped_free(dim3_grouped* g);
+
 /* ---- streamed output: rows handed to the caller in chunks ----
  * For outputs larger than any one buffer (the chunked STREAM output mode).
  * The query runs in chunks of consecutive x rows, each producing at most

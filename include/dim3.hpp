@@ -27,7 +27,13 @@
 // AggregateFunctionAvg<_Tv> (the reference's, dim3.hpp:1453-1480),
 // AggregateFunctionSum / Min / Max<_Tv> (value policies over v1 + v2) and
 // AggregateFunctionCount<_Tv>, for _Tv float (float results, double
-// accumulation) or an integer type whose values fit int32 (int64 results).
+// accumulation) or an integer type whose values fit int32 (int64 results),
+// aggregated on the device; and any caller-written _Aggregate_type with the
+// reference's policy concept (ctor(int nz), insert(int z, v1, v2),
+// update(int z, v1, v2), get(int z)) and a trivially copyable _Tv of 4 or 8
+// bytes: the device joins and groups the paths by (x, z)
+// (dim3_join_paths_grouped), the policy folds them on the host in the
+// reference's call pattern.
 // Device: DIM3_DEVICE env var, default 0.
 #pragma once
 
@@ -277,16 +283,88 @@ struct policy_code<AggregateFunctionCount<_Tv>, _Tv> {
 
 }  // namespace dim3_shim
 
+namespace dim3_shim {
+
+template <typename A, typename _Tv>
+constexpr bool is_library_policy() {
+    return std::is_same<A, AggregateFunctionCount<_Tv>>::value || is_value_policy<A, _Tv>();
+}
+
+// A policy the library does not implement is the caller's host code (the
+// reference's concept, dim3.hpp:1453-1480: ctor(nz), insert / update / get
+// with a dense z key): the device joins and groups the join paths by (x, z)
+// (dim3_join_paths_grouped) and the policy folds them as SpmmAggregate does
+// (dim3.hpp:1275-1307): per x row, insert on a group's first path, update on
+// the others, get for each of the row's groups at its end.
+template <typename A, typename _Tx, typename _Ty, typename _Tz, typename _Tv, typename _Tcounter>
+void join_aggregate_policy(myvector<TUPLE<_Tx, _Ty, _Tv>, _Tcounter>& R, myvector<TUPLE<_Tz, _Ty, _Tv>, _Tcounter>& S,
+                           myvector<TUPLE<_Tx, _Tz, _Tv>, _Tcounter>& result) {
+    static_assert(is_i32<_Tx>() && is_i32<_Ty>() && is_i32<_Tz>(), "libdim3 joins 4-byte integer attributes");
+    static_assert(sizeof(_Tv) == 4 || sizeof(_Tv) == 8, "tuple values of 4 or 8 bytes");
+    static_assert(std::is_trivially_copyable<_Tv>::value, "trivially copyable tuple values");
+    typedef TUPLE<_Tx, _Ty, _Tv> InT;
+    typedef TUPLE<_Tx, _Tz, _Tv> OutT;
+    const InT probe((_Tx)0, (_Ty)0, _Tv());
+    const uint32_t voff = (uint32_t)((const char*)&probe.v - (const char*)&probe);
+    dim3_opts o;
+    dim3_opts_default(&o);
+    dim3_grouped g;
+    const int rc = dim3_join_paths_grouped(context(), R.data, R.size(), S.data, S.size(), (uint32_t)sizeof(InT), voff,
+                                           (uint32_t)sizeof(_Tv), &o, &g);
+    if (rc != DIM3_OK) die("DIM3JoinAggregate doJoinAggregate", rc);
+    if (g.n_groups > (uint64_t)(_Tcounter)~(_Tcounter)0) {
+        printf("DIM3JoinAggregate: %llu rows do not fit the result counter type.\n", (unsigned long long)g.n_groups);
+        exit(-1);
+    }
+    result.clear();
+    if (g.n_groups) result.reserve((_Tcounter)g.n_groups);
+    A aggregate_function((int)g.nz);
+    const char* v1 = (const char*)g.v1;
+    const char* v2 = (const char*)g.v2;
+    auto value = [](const char* base, uint64_t i) {
+        _Tv v;
+        memcpy(&v, base + i * sizeof(_Tv), sizeof(_Tv));
+        return v;
+    };
+    uint64_t row0 = 0;
+    for (uint64_t i = 0; i < g.n_groups; i++) {
+        const int key = (int)g.zkey[i];
+        uint64_t p = g.path_start[i];
+        const uint64_t pe = g.path_start[i + 1];
+        aggregate_function.insert(key, value(v1, p), value(v2, p));
+        for (++p; p < pe; ++p) aggregate_function.update(key, value(v1, p), value(v2, p));
+        if (i + 1 == g.n_groups || g.x[i + 1] != g.x[i]) {
+            for (uint64_t k = row0; k <= i; k++)
+                result.push_back(OutT((_Tx)g.x[k], (_Tz)g.z[k], aggregate_function.get((int)g.zkey[k])));
+            row0 = i + 1;
+        }
+    }
+    dim3_grouped_free(&g);
+}
+
+}  // namespace dim3_shim
+
 template <typename _Tx, typename _Ty, typename _Tz, typename _Tv, typename _Aggregate_type,
           typename _Hx = myHasher, typename _Hy = myHasher, typename _Hz = myHasher, typename _Hxz = myHasher,
           typename _Tcounter = uint32_t>
 struct DIM3JoinAggregate {
+    // The library's policies (COUNT, SUM, MIN, MAX, AVG) aggregate on the
+    // device; any other _Aggregate_type is folded on the host over the
+    // device's grouped join paths.
     static void doJoinAggregate(myvector<TUPLE<_Tx, _Ty, _Tv>, _Tcounter>& _R,
                                 myvector<TUPLE<_Tz, _Ty, _Tv>, _Tcounter>& _S,
                                 myvector<TUPLE<_Tx, _Tz, _Tv>, _Tcounter>& result) {
-        static_assert(std::is_same<_Aggregate_type, AggregateFunctionCount<_Tv>>::value ||
-                          dim3_shim::is_value_policy<_Aggregate_type, _Tv>(),
-                      "libdim3 implements AggregateFunctionAvg / Sum / Min / Max / Count<_Tv>");
+        run(_R, _S, result, std::integral_constant<bool, dim3_shim::is_library_policy<_Aggregate_type, _Tv>()>());
+    }
+
+private:
+    static void run(myvector<TUPLE<_Tx, _Ty, _Tv>, _Tcounter>& _R, myvector<TUPLE<_Tz, _Ty, _Tv>, _Tcounter>& _S,
+                    myvector<TUPLE<_Tx, _Tz, _Tv>, _Tcounter>& result, std::false_type) {
+        dim3_shim::join_aggregate_policy<_Aggregate_type>(_R, _S, result);
+    }
+
+    static void run(myvector<TUPLE<_Tx, _Ty, _Tv>, _Tcounter>& _R, myvector<TUPLE<_Tz, _Ty, _Tv>, _Tcounter>& _S,
+                    myvector<TUPLE<_Tx, _Tz, _Tv>, _Tcounter>& result, std::true_type) {
         static_assert(dim3_shim::is_i32<_Tx>() && dim3_shim::is_i32<_Ty>() && dim3_shim::is_i32<_Tz>(),
                       "libdim3 joins 4-byte integer attributes");
         static_assert(std::is_same<_Tv, float>::value || std::is_integral<_Tv>::value,

@@ -63,6 +63,13 @@ class Result(ctypes.Structure):
                 ("stats", Stats), ("rows_f32", ctypes.c_void_p)]
 
 
+class Grouped(ctypes.Structure):
+    _fields_ = [("n_groups", ctypes.c_uint64), ("n_paths", ctypes.c_uint64), ("nz", ctypes.c_uint32),
+                ("value_size", ctypes.c_uint32), ("x", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("zkey", ctypes.c_void_p), ("path_start", ctypes.c_void_p), ("v1", ctypes.c_void_p),
+                ("v2", ctypes.c_void_p)]
+
+
 class Instance(ctypes.Structure):
     _fields_ = [("R_xy", ctypes.c_void_p), ("n_r", ctypes.c_uint64), ("S_zy", ctypes.c_void_p),
                 ("n_s", ctypes.c_uint64), ("out", ctypes.c_void_p), ("cap", ctypes.c_uint64)]
@@ -118,7 +125,8 @@ EXPORTS = [
     "dim3_join_aggregate_count_stream", "dim3_version", "dim3_join_aggregate", "dim3_join_aggregate_dev",
     "dim3_join_aggregate_f32", "dim3_join_aggregate_f32_dev", "dim3_estimate_join_cardinality",
     "dim3_join_aggregate_tuples", "dim3_server_put_values", "dim3_join_aggregate_stream",
-    "dim3_join_aggregate_f32_stream", "dim3_estimate_join_cardinality_dev",
+    "dim3_join_aggregate_f32_stream", "dim3_estimate_join_cardinality_dev", "dim3_join_paths_grouped",
+    "dim3_grouped_free",
 ]
 
 _lib = None
@@ -187,6 +195,11 @@ def lib():
     L.dim3_join_aggregate_tuples.argtypes = [vp, vp, u64, vp, u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                              i32, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
     L.dim3_join_aggregate_tuples.restype = i32
+    L.dim3_join_paths_grouped.argtypes = [vp, vp, u64, vp, u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.POINTER(Opts), ctypes.POINTER(Grouped)]
+    L.dim3_join_paths_grouped.restype = i32
+    L.dim3_grouped_free.argtypes = [ctypes.POINTER(Grouped)]
+    L.dim3_grouped_free.restype = None
     for name in ("dim3_join_aggregate_stream", "dim3_join_aggregate_f32_stream"):
         f = getattr(L, name)
         f.argtypes = [vp, vp, vp, u64, vp, vp, u64, i32, ctypes.POINTER(Opts), u64, ROWS_CB, vp,
@@ -490,6 +503,33 @@ class Context:
                     else np.zeros((0,), dtype=dt))
         out = QueryResult(res, rows=rows)
         self._L.dim3_result_free(ctypes.byref(res))
+        return out
+
+    def join_paths_grouped(self, R, S, **kw):
+        """dim3_join_paths_grouped over TUPLE arrays (structured, int32 x / z
+        and y, a 4- or 8-byte value field v): the join paths grouped by
+        (x, z), for a caller-written aggregate policy.  Returns a dict of
+        numpy arrays: x, z, zkey (n_groups), path_start (n_groups + 1), v1, v2
+        (n_paths, the value dtype), and nz."""
+        opts = make_opts(**kw)
+        if R.dtype != S.dtype:
+            raise ValueError("R and S must share the tuple layout")
+        vdt, voff = R.dtype.fields["v"][0], R.dtype.fields["v"][1]
+        R, S = np.ascontiguousarray(R), np.ascontiguousarray(S)
+        g = Grouped()
+        rc = self._L.dim3_join_paths_grouped(self._h, _ptr(R), R.shape[0], _ptr(S), S.shape[0], R.dtype.itemsize, voff,
+                                             vdt.itemsize, ctypes.byref(opts), ctypes.byref(g))
+        self._check(rc, "dim3_join_paths_grouped")
+        G, P = int(g.n_groups), int(g.n_paths)
+
+        def take(ptr, n, dt):
+            dt = np.dtype(dt)
+            return (np.frombuffer((ctypes.c_char * (dt.itemsize * n)).from_address(ptr), dtype=dt).copy() if n
+                    else np.zeros((0,), dtype=dt))
+        out = {"x": take(g.x, G, np.int32), "z": take(g.z, G, np.int32), "zkey": take(g.zkey, G, np.uint32),
+               "path_start": take(g.path_start, G + 1, np.uint64), "v1": take(g.v1, P, vdt), "v2": take(g.v2, P, vdt),
+               "nz": int(g.nz)}
+        self._L.dim3_grouped_free(ctypes.byref(g))
         return out
 
     def join_aggregate_dev(self, d_R, d_vR, n_r, d_S, d_vS, n_s, agg, d_out, cap, stream=None, **kw):

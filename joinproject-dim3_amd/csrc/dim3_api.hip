@@ -2190,6 +2190,44 @@ int dim3_join_aggregate_dev(dim3_ctx* c, const int32_t* d_R, const int32_t* d_Rv
     return run_value_dev(c, agg, false, d_R, d_Rv, n_r, d_S, d_Sv, n_s, opts, d_rows, cap, stream, res);
 }
 
+int dim3_join_paths_grouped(dim3_ctx* c, const void* R_tuples, uint64_t n_r, const void* S_tuples, uint64_t n_s,
+                            uint32_t stride, uint32_t value_offset, uint32_t value_size, const dim3_opts* opts,
+                            dim3_grouped* g) {
+    if (!c || !g) return DIM3_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->err.clear();
+    memset(g, 0, sizeof(*g));
+    if ((n_r && !R_tuples) || (n_s && !S_tuples)) return fail(c, DIM3_ERR_INVALID, "null relation pointer");
+    if ((value_size != 4 && value_size != 8) || stride % value_size || value_offset % value_size || value_offset < 8 ||
+        value_offset + value_size > stride)
+        return fail(c, DIM3_ERR_INVALID, "bad tuple layout (stride %u, value of %u bytes at %u)", stride, value_size,
+                    value_offset);
+    if (n_r > dim3s::kMaxTuples || n_s > dim3s::kMaxTuples)
+        return fail(c, DIM3_ERR_INVALID, "relations take at most %llu tuples", (unsigned long long)dim3s::kMaxTuples);
+    dim3_opts o;
+    TRY(check_opts(c, opts, &o));
+    HIPCHK(c, hipSetDevice(c->device));
+    std::string e;
+    const int rc = dim3h::grouped_paths(R_tuples, n_r, S_tuples, n_s, stride, value_offset, value_size,
+                                        (o.flags & DIM3_OPT_EXCLUDE_EQUAL) != 0, c->stream, g, &e);
+    if (rc != DIM3_OK) {
+        dim3_grouped_free(g);
+        return fail(c, rc, "%s", e.c_str());
+    }
+    return DIM3_OK;
+}
+
+void dim3_grouped_free(dim3_grouped* g) {
+    if (!g) return;
+    free(g->x);
+    free(g->z);
+    free(g->zkey);
+    free(g->path_start);
+    free(g->v1);
+    free(g->v2);
+    memset(g, 0, sizeof(*g));
+}
+
 void dim3_result_free(dim3_result* res) {
     if (!res) return;
     free(res->pairs);

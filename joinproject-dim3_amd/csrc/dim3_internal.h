@@ -16,6 +16,7 @@
 
 #include <mutex>
 #include <shared_mutex>
+#include <string>
 
 #include "../../include/dim3.h"
 
@@ -641,6 +642,11 @@ void value_tasks(const ExpandArgs& a, uint32_t nz, uint32_t limit, uint32_t* bou
                  uint32_t* wave_x, uint32_t* hash_x, uint32_t* big_x, hipStream_t st);
 // dynamic LDS of one expansion workgroup (fp: plus the fingerprint table)
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp);
+// Join paths grouped by (x, z) into host arrays (dim3_join_paths_grouped);
+// host tuple arrays in, temporaries allocated and freed on st.  Returns a
+// DIM3_* status, with *err set on failure.
+int grouped_paths(const void* R, uint64_t nR, const void* S, uint64_t nS, uint32_t stride, uint32_t voff, uint32_t vsize,
+                  bool exclude_equal, hipStream_t st, dim3_grouped* out, std::string* err);
 
 // Batched expansion: one persistent launch over the tasks of n_inst
 // instances (args[i], tasks task_start[i] .. task_start[i + 1]).
