@@ -280,6 +280,10 @@ std::shared_mutex& dim3h::capture_mutex() {
     return mu;
 }
 
+// EstimateJoinCardinality over device relations (defined below)
+static int estimate_on(dim3_ctx* c, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS, hipStream_t st,
+                       int64_t* estimate);
+
 namespace {
 
 int fail(dim3_ctx* c, int code, const char* fmt, ...) {
@@ -1297,6 +1301,84 @@ int check_opts(dim3_ctx* c, const dim3_opts* opts, dim3_opts* o) {
     return DIM3_OK;
 }
 
+// ---------------------------------------------------------------------------
+// f1 (dim3.hpp:41-50): the classical path for low OUT_J
+// ---------------------------------------------------------------------------
+// The reference's f1 takes its classical radix hash join (dim3.hpp:273-391)
+// when the estimated OUT_J is under about 2.2 n.  Here the classical path is
+// a sort of one 64-bit (x, z) key per join path (dim3h::classical_join), a
+// few launches and two host synchronisations; the general plan maps x, y and
+// z to dense IDs and builds two CSRs first, work that only pays when the
+// rows are long.  Measured (uniform keys, materialised, ms general /
+// classical): 1e7 tuples per side at 0.5, 2, 4, 8 paths per tuple 11.6 / 7.9,
+// 14.7 / 10.1, 14.6 / 13.3, 14.7 / 19.2; 1e6 per side 1.55 / 2.3 and
+// 1.8 / 3.1 at 0.5 and 2; 1e5 per side 0.53 / 0.47 and 0.56 / 0.74.  So
+// AUTO plans of at least kClassicalMinTuples tuples take it when
+// EstimateJoinCardinality's estimate is at most kClassicalPathsPerTuple
+// paths per input tuple (the estimate, one small launch and a
+// synchronisation, runs only for those sizes).
+constexpr double kClassicalPathsPerTuple = 4.0;
+constexpr uint64_t kClassicalMinTuples = 8000000;
+
+// after the swap rule: nR >= nS
+int want_classical(dim3_ctx* c, const dim3_opts& o, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
+                   hipStream_t st, bool* yes) {
+    *yes = false;
+    if (o.flags & DIM3_OPT_NO_CLASSICAL) return DIM3_OK;
+    if (nR == 0 || nS == 0) return DIM3_OK;  // empty: the general plan's early exit
+    if (o.flags & DIM3_OPT_CLASSICAL) {
+        *yes = true;
+        return DIM3_OK;
+    }
+    if (o.dense_mode != DIM3_DENSE_AUTO || o.tile_bits || o.block || o.heavy_k || o.dense_core >= 0) return DIM3_OK;
+    if (nR + nS < kClassicalMinTuples) return DIM3_OK;
+    int64_t est = 0;
+    TRY(estimate_on(c, dR, nR, dS, nS, st, &est));
+    *yes = (double)est <= kClassicalPathsPerTuple * (double)(nR + nS) && est < (1ll << 30);
+    return DIM3_OK;
+}
+
+// The classical path of one project / COUNT query over device relations;
+// out_for(rows, &buf, &cap) names the output buffer (materialise).
+int run_classical(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
+                  const dim3_opts& o, bool fp, const std::function<int(uint64_t, void**, uint64_t*)>& out_for,
+                  hipStream_t st, dim3_result* res) {
+    const bool swapped = !(o.flags & DIM3_OPT_NO_SWAP) && nR < nS;
+    if (swapped) {
+        std::swap(dR, dS);
+        std::swap(nR, nS);
+    }
+    Scalars* sc;
+    TRY(ws(c, WS_SCALARS, 1, &sc));
+    HIPCHK(c, hipMemsetAsync(sc, 0, sizeof(Scalars), st));
+    HIPCHK(c, hipEventRecord(c->ev[0], st));
+    std::string e;
+    const int rc = dim3h::classical_join(dR, nR, dS, nS, count_mode, (o.flags & DIM3_OPT_EXCLUDE_EQUAL) != 0, fp,
+                                         out_for, sc, st, &e);
+    if (rc != DIM3_OK) return fail(c, rc, "%s", e.c_str());
+    HIPCHK(c, hipEventRecord(c->ev[5], st));
+    HIPCHK(c, dim3h::copy_wait(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    const Scalars& s = *c->h_sc;
+    res->n = s.out_n;
+    res->swapped = swapped ? 1u : 0u;
+    res->fp.count = s.out_n;
+    res->fp.sum = s.fp_sum;
+    res->fp.sum2 = s.fp_sum2;
+    res->fp.csum = s.fp_csum;
+    dim3_stats& st_ = res->stats;
+    memset(&st_, 0, sizeof(st_));
+    st_.n_r = nR;
+    st_.n_s = nS;
+    st_.out_j = s.paths;
+    st_.paths_sparse = s.paths;
+    st_.dense_mode = DIM3_DENSE_OFF;
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
+    st_.ms_expand = ms;
+    st_.ms_total = ms;
+    return DIM3_OK;
+}
+
 int run_dev(dim3_ctx* c, bool count_mode, const int32_t* d_R, uint64_t nR, const int32_t* d_S, uint64_t nS,
             const dim3_opts* opts, void* d_out, uint64_t cap, void* stream, dim3_result* res) {
     if (!c || !res) return DIM3_ERR_INVALID;
@@ -1313,6 +1395,24 @@ int run_dev(dim3_ctx* c, bool count_mode, const int32_t* d_R, uint64_t nR, const
     c->qstream = st;
     if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
         return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
+    {
+        const bool sw = !(o.flags & DIM3_OPT_NO_SWAP) && nR < nS;
+        bool classical = false;
+        TRY(want_classical(c, o, (const int2*)(sw ? d_S : d_R), sw ? nS : nR, (const int2*)(sw ? d_R : d_S),
+                           sw ? nR : nS, st, &classical));
+        if (classical) {
+            auto out_for = [&](uint64_t, void** buf, uint64_t* bcap) {
+                *buf = d_out;
+                *bcap = cap;
+                return DIM3_OK;
+            };
+            TRY(run_classical(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, fp, out_for, st, res));
+            if (!fp && res->n > cap)
+                return fail(c, DIM3_ERR_CAPACITY, "output needs %llu rows, buffer holds %llu",
+                            (unsigned long long)res->n, (unsigned long long)cap);
+            return DIM3_OK;
+        }
+    }
     TRY(prepare(c, count_mode, (const int2*)d_R, nR, (const int2*)d_S, nS, o, st));
     TRY(expand(c, fp, d_out, cap, st));
     if (dense_overflowed(c, *c->h_sc)) {  // a multiplicity above 127: redo without the int8 product
@@ -1347,8 +1447,37 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
     if (nS) HIPCHK(c, hipMemcpyAsync(dS, S, nS * 8, hipMemcpyHostToDevice, st));
     if (count_mode && o.dense_mode == DIM3_DENSE_BITOR)
         return fail(c, DIM3_ERR_UNSUPPORTED, "DIM3_DENSE_BITOR is join-project only");
-    TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
     const size_t row_bytes = count_mode ? sizeof(dim3_count_row) : 8;
+    {
+        const bool sw = !(o.flags & DIM3_OPT_NO_SWAP) && nR < nS;
+        bool classical = false;
+        TRY(want_classical(c, o, sw ? dS : dR, sw ? nS : nR, sw ? dR : dS, sw ? nR : nS, st, &classical));
+        if (classical) {
+            void* dout = nullptr;
+            auto out_for = [&](uint64_t rows, void** buf, uint64_t* bcap) {
+                const int rc = ws_get(c, WS_OUT, std::max<uint64_t>(rows, 1) * row_bytes, &dout);
+                *buf = dout;
+                *bcap = rows;
+                return rc;
+            };
+            TRY(run_classical(c, count_mode, dR, nR, dS, nS, o, fp, out_for, st, res));
+            if (!fp && res->n) {
+                void* h = malloc(res->n * row_bytes);
+                if (!h) return fail(c, DIM3_ERR_OOM, "host malloc of %llu rows", (unsigned long long)res->n);
+                hipError_t e = dim3h::copy_wait(h, dout, res->n * row_bytes, hipMemcpyDeviceToHost, st);
+                if (e != hipSuccess) {
+                    free(h);
+                    return fail(c, DIM3_ERR_DEVICE, "D2H copy: %s", hipGetErrorString(e));
+                }
+                if (count_mode)
+                    res->rows = (dim3_count_row*)h;
+                else
+                    res->pairs = (int32_t*)h;
+            }
+            return DIM3_OK;
+        }
+    }
+    TRY(prepare(c, count_mode, dR, nR, dS, nS, o, st));
     void* dout = nullptr;
     uint64_t cap = 0;
     if (!fp && !c->bank().plan.empty) {

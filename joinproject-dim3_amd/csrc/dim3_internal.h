@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <mutex>
 #include <shared_mutex>
 #include <string>
@@ -647,6 +648,13 @@ size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stag
 // DIM3_* status, with *err set on failure.
 int grouped_paths(const void* R, uint64_t nR, const void* S, uint64_t nS, uint32_t stride, uint32_t voff, uint32_t vsize,
                   bool exclude_equal, hipStream_t st, dim3_grouped* out, std::string* err);
+// The classical join-project / COUNT (low OUT_J, f1): sort-based, one call;
+// get_out(rows, &buf, &cap) supplies the output once the row count is known
+// (not called in fingerprint mode).  Fills sc->out_n, sc->paths and the
+// fingerprint sums (sc zeroed by the caller).
+int classical_join(const int2* R, uint64_t nR, const int2* S, uint64_t nS, bool count_mode, bool exclude_equal,
+                   bool fp, const std::function<int(uint64_t, void**, uint64_t*)>& get_out, Scalars* sc,
+                   hipStream_t st, std::string* err);
 
 // Batched expansion: one persistent launch over the tasks of n_inst
 // instances (args[i], tasks task_start[i] .. task_start[i + 1]).

@@ -23,6 +23,8 @@
 // 2 value gathers.  Paths with x == z are dropped under exclude_equal.
 #include <hipcub/hipcub.hpp>
 
+#include <functional>
+
 #include "dim3_internal.h"
 
 namespace {
@@ -128,6 +130,80 @@ __global__ void k_gather_values(const uint32_t* __restrict__ pid_sorted, uint64_
 __global__ void k_u32_to_u64(const uint32_t* __restrict__ in, uint64_t n, uint64_t* __restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (i < n) out[i] = in[i];
+}
+
+// ---- classical join-project / COUNT (low OUT_J) ----
+// sortable join key and attribute of each S tuple
+__global__ void k_split_s(const int2* __restrict__ S, uint64_t n, uint32_t* __restrict__ key, uint32_t* __restrict__ attr) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const int2 t = S[i];
+    key[i] = flip(t.y);
+    attr[i] = flip(t.x);
+}
+
+__global__ void k_r_ranges2(const int2* __restrict__ R, uint64_t nR, const uint32_t* __restrict__ skey_sorted,
+                            uint64_t nS, uint32_t* __restrict__ start, uint64_t* __restrict__ cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= nR) return;
+    const uint32_t k = flip(R[i].y);
+    const uint64_t lo = bound(skey_sorted, nS, k, false), hi = bound(skey_sorted, nS, k, true);
+    start[i] = (uint32_t)lo;
+    cnt[i] = hi - lo;
+}
+
+// one (x, z) key per join path (all ones: dropped by exclude_equal)
+__global__ void k_row_keys(const uint64_t* __restrict__ off, uint64_t nR, uint64_t P, const int2* __restrict__ R,
+                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ sattr_sorted,
+                           uint32_t exclude_equal, uint64_t* __restrict__ key, unsigned long long* __restrict__ dropped) {
+    const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (p >= P) return;
+    const uint32_t i = r_of_path(off, nR, p);
+    const uint32_t x = flip(R[i].x), z = sattr_sorted[start[i] + (uint32_t)(p - off[i])];
+    const bool drop = exclude_equal && x == z;
+    key[p] = drop ? ~0ull : ((uint64_t)x << 32) | z;
+    const uint64_t m = __ballot(drop);
+    if ((threadIdx.x & 63u) == 0u && m) atomicAdd(dropped, (unsigned long long)__popcll(m));
+}
+
+// rows (materialised up to cap) and the fingerprint of the distinct keys
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_classical_rows(const uint64_t* __restrict__ gkey,
+                                                        const uint32_t* __restrict__ gcnt, uint64_t G, void* out,
+                                                        uint64_t cap, uint32_t fp, dim3h::Scalars* sc) {
+    __shared__ uint64_t sh[5];
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    uint64_t s1 = 0, s2 = 0, cs = 0;
+    if (g < G) {
+        const uint64_t k = gkey[g];
+        const int32_t x = unflip((uint32_t)(k >> 32)), z = unflip((uint32_t)k);
+        const uint32_t c = gcnt[g];
+        if (fp) {
+            const uint64_t gz = dim3d::zfactor(z);
+            s1 = dim3d::xfactor1(x) * gz;
+            s2 = dim3d::xfactor2(x) * (COUNT ? (uint64_t)c : 1ull) * gz;
+            cs = COUNT ? c : 0u;
+        } else if (g < cap) {
+            if (COUNT) {
+                dim3_count_row r;
+                r.x = x;
+                r.z = z;
+                r.count = (int64_t)c;
+                reinterpret_cast<dim3_count_row*>(out)[g] = r;
+            } else {
+                reinterpret_cast<int2*>(out)[g] = make_int2(x, z);
+            }
+        }
+    }
+    if (!fp) return;
+    s1 = dim3d::block_reduce_sum<256, uint64_t>(s1, sh);
+    s2 = dim3d::block_reduce_sum<256, uint64_t>(s2, sh);
+    cs = dim3d::block_reduce_sum<256, uint64_t>(cs, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd((unsigned long long*)&sc->fp_sum, (unsigned long long)s1);
+        atomicAdd((unsigned long long*)&sc->fp_sum2, (unsigned long long)s2);
+        if (COUNT) atomicAdd((unsigned long long*)&sc->fp_csum, (unsigned long long)cs);
+    }
 }
 
 // Device buffers of one call, freed in order on the stream.
@@ -322,6 +398,92 @@ int grouped_paths(const void* R, uint64_t nR, const void* S, uint64_t nS, uint32
         PCHK(hipMemcpyAsync(out->v1, v1, Q * vsize, hipMemcpyDeviceToHost, st));
         PCHK(hipMemcpyAsync(out->v2, v2, Q * vsize, hipMemcpyDeviceToHost, st));
     }
+    PCHK(hipStreamSynchronize(st));
+    return DIM3_OK;
+}
+
+// Classical join-project / COUNT of device relations (after the swap rule):
+// the counterpart of the reference's Radix_hash_join_and_project
+// (dim3.hpp:273-391), chosen by f1 for low OUT_J.  S sorted by join key
+// with its attribute, each R tuple's S range by binary search, one (x, z)
+// key per join path, a radix sort of the keys and a run-length encoding: the
+// runs are the rows, their lengths the COUNTs.  Rows in (x, z) order.
+// Fills sc->out_n (rows), sc->paths (join paths) and, fingerprint mode, the
+// fingerprint sums; otherwise get_out(rows) names the output buffer and its
+// capacity once the row count is known, and min(rows, capacity) rows are
+// written.
+int classical_join(const int2* R, uint64_t nR, const int2* S, uint64_t nS, bool count_mode, bool exclude_equal,
+                   bool fp, const std::function<int(uint64_t, void**, uint64_t*)>& get_out, Scalars* sc,
+                   hipStream_t st, std::string* err) {
+    Pool pool(st);
+    uint32_t *skey, *sattr, *skey_s, *sattr_s, *start;
+    uint64_t *cnt, *off;
+    PCHK(pool.get(&skey, nS));
+    PCHK(pool.get(&sattr, nS));
+    PCHK(pool.get(&skey_s, nS));
+    PCHK(pool.get(&sattr_s, nS));
+    PCHK(pool.get(&start, nR));
+    PCHK(pool.get(&cnt, nR + 1));
+    PCHK(pool.get(&off, nR + 1));
+    k_split_s<<<blocks_for(nS), 256, 0, st>>>(S, nS, skey, sattr);
+    size_t tb = 0, tb2 = 0;
+    PCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, skey, skey_s, sattr, sattr_s, nS, 0, 32, st));
+    PCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt, off, nR + 1, st));
+    uint8_t* tmp;
+    PCHK(pool.get(&tmp, std::max(tb, tb2)));
+    PCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey_s, sattr, sattr_s, nS, 0, 32, st));
+    k_r_ranges2<<<blocks_for(nR), 256, 0, st>>>(R, nR, skey_s, nS, start, cnt);
+    PCHK(hipMemsetAsync(cnt + nR, 0, sizeof(uint64_t), st));
+    PCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, cnt, off, nR + 1, st));
+    uint64_t P = 0;
+    PCHK(copy_wait(&P, off + nR, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    if (P >= (1ull << 31)) {
+        *err = "the classical join takes fewer than 2^31 join paths (the query has " + std::to_string(P) + ")";
+        return DIM3_ERR_UNSUPPORTED;
+    }
+    uint64_t *key, *key_s, *gkey;
+    uint32_t *gcnt, *nruns;
+    unsigned long long* dropped;
+    PCHK(pool.get(&key, P));
+    PCHK(pool.get(&key_s, P));
+    PCHK(pool.get(&gkey, P));
+    PCHK(pool.get(&gcnt, P));
+    PCHK(pool.get(&nruns, 1));
+    PCHK(pool.get(&dropped, 1));
+    PCHK(hipMemsetAsync(dropped, 0, sizeof(unsigned long long), st));
+    PCHK(hipMemsetAsync(nruns, 0, sizeof(uint32_t), st));
+    if (P) k_row_keys<<<blocks_for(P), 256, 0, st>>>(off, nR, P, R, start, sattr_s, exclude_equal ? 1u : 0u, key, dropped);
+    size_t tb3 = 0, tb4 = 0;
+    PCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb3, key, key_s, P, 0, 64, st));
+    PCHK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb4, key_s, gkey, gcnt, nruns, (int)P, st));
+    uint8_t* tmp3;
+    PCHK(pool.get(&tmp3, std::max(tb3, tb4)));
+    if (P) {
+        PCHK(hipcub::DeviceRadixSort::SortKeys(tmp3, tb3, key, key_s, P, 0, 64, st));
+        PCHK(hipcub::DeviceRunLengthEncode::Encode(tmp3, tb4, key_s, gkey, gcnt, nruns, (int)P, st));
+    }
+    uint32_t hr[2] = {0, 0};
+    unsigned long long hd = 0;
+    PCHK(hipMemcpyAsync(&hr[0], nruns, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PCHK(copy_wait(&hd, dropped, sizeof(hd), hipMemcpyDeviceToHost, st));
+    const uint64_t G = hr[0] - (hd ? 1u : 0u);  // the dropped paths' key sorts last
+    void* dout = nullptr;
+    uint64_t dcap = 0;
+    if (!fp) {
+        const int rc = get_out(G, &dout, &dcap);
+        if (rc != DIM3_OK) {
+            *err = "no output buffer for " + std::to_string(G) + " rows";
+            return rc;
+        }
+    }
+    if (G && count_mode)
+        k_classical_rows<true><<<blocks_for(G), 256, 0, st>>>(gkey, gcnt, G, dout, dcap, fp ? 1u : 0u, sc);
+    else if (G)
+        k_classical_rows<false><<<blocks_for(G), 256, 0, st>>>(gkey, gcnt, G, dout, dcap, fp ? 1u : 0u, sc);
+    PCHK(hipGetLastError());
+    const unsigned long long tot[2] = {(unsigned long long)G, (unsigned long long)P};
+    PCHK(hipMemcpyAsync(&sc->out_n, &tot[0], sizeof(unsigned long long), hipMemcpyHostToDevice, st));
+    PCHK(hipMemcpyAsync(&sc->paths, &tot[1], sizeof(unsigned long long), hipMemcpyHostToDevice, st));
     PCHK(hipStreamSynchronize(st));
     return DIM3_OK;
 }

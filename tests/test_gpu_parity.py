@@ -302,3 +302,60 @@ def test_low_outj_and_short_rows(ctx, dim3):
             want, wfp, _ = oracle.join_project(E, E, exclude_equal=True)
             g = ctx.join_project(E, E, exclude_equal=True)
             assert np.array_equal(oracle.sort_pairs(g.pairs), oracle.sort_pairs(want))
+
+
+# ---- the classical path (f1, dim3.hpp:41-50 / 273-391) ----
+
+def classical_cases():
+    rng = np.random.default_rng(2024)
+    yield "uniform_low_outj", rand_rel(rng, 20000, 20000, 20000), rand_rel(rng, 15000, 20000, 20000)
+    yield "duplicates", np.repeat(rand_rel(rng, 3000, 500, 800), 3, 0), rand_rel(rng, 4000, 500, 800)
+    yield "negative_and_extremes", np.array([[-5, 1], [2**31 - 1, 1], [-2**31, 2], [0, 2], [7, -1]], np.int32), \
+        np.array([[2**31 - 1, 1], [-2**31, 1], [3, 2], [-1, -1], [9, 5]], np.int32)
+    yield "skewed_key", np.stack([np.arange(5000), np.zeros(5000)], 1).astype(np.int32), \
+        np.stack([np.arange(3), np.zeros(3)], 1).astype(np.int32)
+    E = rand_rel(rng, 6000, 3000, 3000)
+    yield "self_join", E, E
+
+
+@pytest.mark.parametrize("case", [c[0] for c in classical_cases()])
+def test_classical_path_vs_oracle(ctx, case):
+    R, S = next((r, s) for n, r, s in classical_cases() if n == case)
+    for ex in (False, True):
+        got = check_project(ctx, R, S, classical=True, exclude_equal=ex)
+        assert got.stats.n_tiles == 0  # the classical path ran
+        check_count(ctx, R, S, classical=True, exclude_equal=ex)
+        # fingerprints equal the general plan's
+        a = ctx.join_project(R, S, classical=True, exclude_equal=ex, output_mode=1)
+        b = ctx.join_project(R, S, classical=False, exclude_equal=ex, output_mode=1)
+        assert a.fp.as_tuple() == b.fp.as_tuple() and b.stats.n_tiles > 0
+        a = ctx.join_aggregate_count(R, S, classical=True, exclude_equal=ex, output_mode=1)
+        b = ctx.join_aggregate_count(R, S, classical=False, exclude_equal=ex, output_mode=1)
+        assert a.fp.as_tuple() == b.fp.as_tuple()
+    check_project(ctx, R, S, classical=True, no_swap=True)
+
+
+def test_classical_path_choice_and_device_api(ctx, dim3, torch):
+    rng = np.random.default_rng(9)
+    big = rand_rel(rng, 5_000_000, 5_000_000, 5_000_000)
+    big2 = big[::-1].copy()
+    auto = ctx.join_project(big, big2, output_mode=1)
+    say("low OUT_J auto (1e7 tuples): tiles", auto.stats.n_tiles, "paths", auto.stats.out_j)
+    assert auto.stats.n_tiles == 0  # f1 picks the classical path
+    gen = ctx.join_project(big, big2, output_mode=1, classical=False)
+    assert gen.stats.n_tiles > 0 and gen.fp.as_tuple() == auto.fp.as_tuple()
+    dense = rand_rel(rng, 4_100_000, 4_100_000, 40_000)  # about 100 paths per tuple
+    assert ctx.join_project(dense, dense, output_mode=1).stats.n_tiles > 0
+    R, S = rand_rel(rng, 200000, 200000, 200000), rand_rel(rng, 150000, 200000, 200000)
+    small = ctx.join_project(R, S, output_mode=1)
+    assert small.stats.n_tiles > 0  # small: the general plan
+    # device API: capacity shortfall reports the row count, then fills
+    dR, dS = torch.from_numpy(R).cuda(), torch.from_numpy(S).cuda()
+    rc, r = ctx.join_project_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, classical=True)
+    assert rc == dim3.ERR_CAPACITY and r.n == small.n
+    out = torch.empty((r.n, 2), dtype=torch.int32, device="cuda")
+    rc, r2 = ctx.join_project_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], out.data_ptr(), r.n,
+                                  classical=True)
+    assert rc == dim3.OK and r2.n == r.n
+    want, _, _ = oracle.join_project(R, S)
+    assert np.array_equal(oracle.sort_pairs(out.cpu().numpy()), oracle.sort_pairs(want))
