@@ -56,12 +56,13 @@ extern "C" {
 #define DIM3_OPT_NO_SWAP 1u       /* never swap R and S */
 #define DIM3_OPT_EXCLUDE_EQUAL 2u /* drop rows with x == z (self-join u != w filter) */
 #define DIM3_OPT_HASH_ONLY 4u     /* hash every attribute (no dense-range mapping) */
-/* The classical path (f1, dim3.hpp:41-50 / 273-391): project and COUNT
- * queries of at least 8e6 input tuples with an AUTO plan whose estimated
- * OUT_J is at most 4 join paths per input tuple skip the dense mapping and
- * CSRs and sort one (x, z) key per join path instead (stats.n_tiles = 0
- * reports it; rows in (x, z) order).  These flags force it (at most
- * 2^31 - 1 join paths) or forbid it. */
+/* The classical path (f1, dim3.hpp:41-50 / 273-391): join-project and
+ * materialised COUNT queries of at least 8e6 input tuples with an AUTO plan
+ * whose estimated OUT_J is at most 4 join paths per input tuple and 3 per x
+ * row (rows bounded by R.x's value range) skip the dense mapping and CSRs
+ * and sort one (x, z) key per join path instead (stats.n_tiles = 0 reports
+ * it; rows in (x, z) order).  These flags force it (project and COUNT, at
+ * most 2^31 - 1 join paths) or forbid it. */
 #define DIM3_OPT_CLASSICAL 8u
 #define DIM3_OPT_NO_CLASSICAL 16u
 

@@ -1309,20 +1309,26 @@ int check_opts(dim3_ctx* c, const dim3_opts* opts, dim3_opts* o) {
 // a sort of one 64-bit (x, z) key per join path (dim3h::classical_join), a
 // few launches and two host synchronisations; the general plan maps x, y and
 // z to dense IDs and builds two CSRs first, work that only pays when the
-// rows are long.  Measured (uniform keys, materialised, ms general /
-// classical): 1e7 tuples per side at 0.5, 2, 4, 8 paths per tuple 11.6 / 7.9,
-// 14.7 / 10.1, 14.6 / 13.3, 14.7 / 19.2; 1e6 per side 1.55 / 2.3 and
-// 1.8 / 3.1 at 0.5 and 2; 1e5 per side 0.53 / 0.47 and 0.56 / 0.74.  So
-// AUTO plans of at least kClassicalMinTuples tuples take it when
-// EstimateJoinCardinality's estimate is at most kClassicalPathsPerTuple
-// paths per input tuple (the estimate, one small launch and a
-// synchronisation, runs only for those sizes).
+// rows are long, and whose short rows (one per wavefront) cost per row.
+// Measured (materialised join-project, ms general / classical): uniform x, y,
+// z over 1e7 values, 1e7 tuples per side (about one path per x row) at 0.5,
+// 2, 4, 8 paths per tuple 11.6 / 7.9, 14.7 / 10.1, 14.6 / 13.3, 14.7 / 19.2;
+// the reference generator's data with a 1e7 key domain (x mod 1e6: ten paths
+// per x row, OUT_J 1e7) 5.6 / 7.6; 1e6 tuples per side 1.55 / 2.3; 1e5
+// 0.53 / 0.47.  COUNT fingerprints tie (8.9 / 8.7 at one path per row) and
+// stay on the general plan.  So AUTO plans of at least kClassicalMinTuples
+// tuples take it when EstimateJoinCardinality's estimate is at most
+// kClassicalPathsPerTuple paths per input tuple and kClassicalPathsPerRow
+// per x row, the rows bounded by nR and by R.x's value range (the estimate
+// and a min / max pass, two small launches and synchronisations, run only
+// for those sizes).
 constexpr double kClassicalPathsPerTuple = 4.0;
+constexpr double kClassicalPathsPerRow = 3.0;
 constexpr uint64_t kClassicalMinTuples = 8000000;
 
 // after the swap rule: nR >= nS
-int want_classical(dim3_ctx* c, const dim3_opts& o, const int2* dR, uint64_t nR, const int2* dS, uint64_t nS,
-                   hipStream_t st, bool* yes) {
+int want_classical(dim3_ctx* c, const dim3_opts& o, bool count_mode, const int2* dR, uint64_t nR, const int2* dS,
+                   uint64_t nS, hipStream_t st, bool* yes) {
     *yes = false;
     if (o.flags & DIM3_OPT_NO_CLASSICAL) return DIM3_OK;
     if (nR == 0 || nS == 0) return DIM3_OK;  // empty: the general plan's early exit
@@ -1332,9 +1338,17 @@ int want_classical(dim3_ctx* c, const dim3_opts& o, const int2* dR, uint64_t nR,
     }
     if (o.dense_mode != DIM3_DENSE_AUTO || o.tile_bits || o.block || o.heavy_k || o.dense_core >= 0) return DIM3_OK;
     if (nR + nS < kClassicalMinTuples) return DIM3_OK;
+    if (count_mode && o.output_mode == DIM3_OUT_FINGERPRINT) return DIM3_OK;
     int64_t est = 0;
     TRY(estimate_on(c, dR, nR, dS, nS, st, &est));
-    *yes = (double)est <= kClassicalPathsPerTuple * (double)(nR + nS) && est < (1ll << 30);
+    if ((double)est > kClassicalPathsPerTuple * (double)(nR + nS) || est >= (1ll << 30)) return DIM3_OK;
+    Scalars* sc;
+    TRY(ws(c, WS_SCALARS, 1, &sc));
+    dim3h::minmax(dR, nR, dS, 0, sc->mm, st);
+    int32_t mm[2] = {0, 0};
+    HIPCHK(c, dim3h::copy_wait(mm, sc->mm, sizeof(mm), hipMemcpyDeviceToHost, st));
+    const double rows = std::min<double>((double)nR, (double)mm[1] - (double)mm[0] + 1.0);
+    *yes = (double)est <= kClassicalPathsPerRow * rows;
     return DIM3_OK;
 }
 
@@ -1398,8 +1412,8 @@ int run_dev(dim3_ctx* c, bool count_mode, const int32_t* d_R, uint64_t nR, const
     {
         const bool sw = !(o.flags & DIM3_OPT_NO_SWAP) && nR < nS;
         bool classical = false;
-        TRY(want_classical(c, o, (const int2*)(sw ? d_S : d_R), sw ? nS : nR, (const int2*)(sw ? d_R : d_S),
-                           sw ? nR : nS, st, &classical));
+        TRY(want_classical(c, o, count_mode, (const int2*)(sw ? d_S : d_R), sw ? nS : nR,
+                           (const int2*)(sw ? d_R : d_S), sw ? nR : nS, st, &classical));
         if (classical) {
             auto out_for = [&](uint64_t, void** buf, uint64_t* bcap) {
                 *buf = d_out;
@@ -1451,7 +1465,8 @@ int run_host(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, const 
     {
         const bool sw = !(o.flags & DIM3_OPT_NO_SWAP) && nR < nS;
         bool classical = false;
-        TRY(want_classical(c, o, sw ? dS : dR, sw ? nS : nR, sw ? dR : dS, sw ? nR : nS, st, &classical));
+        TRY(want_classical(c, o, count_mode, sw ? dS : dR, sw ? nS : nR, sw ? dR : dS, sw ? nR : nS, st,
+                           &classical));
         if (classical) {
             void* dout = nullptr;
             auto out_for = [&](uint64_t rows, void** buf, uint64_t* bcap) {
