@@ -65,6 +65,17 @@ extern "C" {
  * most 2^31 - 1 join paths) or forbid it. */
 #define DIM3_OPT_CLASSICAL 8u
 #define DIM3_OPT_NO_CLASSICAL 16u
+/* A COUNT fingerprint (without DIM3_OPT_EXCLUDE_EQUAL) is by default a sum
+ * over the groups, each group's count formed on the device as the
+ * reference's SpmmAggregate forms it (dim3.hpp:1267-1311).  When a few heavy
+ * join keys make the (x, z) box dense, the heavy keys' counts are an int8
+ * 0/1 matrix product on the MFMA cores whose int32 counts are folded into
+ * the fingerprint in registers (stats.dense_mode = DIM3_DENSE_MFMA; forced
+ * by dense_mode = DIM3_DENSE_MFMA, at most 64 heavy keys, fewer with
+ * heavy_k).  This flag takes the join-project fingerprint of the groups and
+ * sum2 / csum as sums over the join paths instead (the same numbers; no
+ * group's count is formed). */
+#define DIM3_OPT_PATH_SUMS 32u
 
 /* ---- dense (heavy-key) strategy ---- */
 #define DIM3_DENSE_AUTO (-1)
@@ -96,7 +107,8 @@ typedef struct dim3_opts {
  * Every term factors into an x part and a z part, so the device sums the z
  * parts of a row and multiplies once per row; the z parts of one 32-aligned
  * word of z values share a(w), so a bitmap word sums with four byte-table
- * lookups.  A COUNT's sum2 and csum are sums over join paths. */
+ * lookups.  A COUNT's sum2 and csum equal sums over join paths, which
+ * DIM3_OPT_PATH_SUMS uses instead of the groups' counts. */
 typedef struct dim3_fingerprint {
     uint64_t count;
     uint64_t sum;
@@ -115,7 +127,9 @@ typedef struct dim3_stats {
     uint32_t tile_bits;       /* z-tile width actually used */
     uint32_t n_tiles;
     uint32_t block;
-    double ms_map, ms_csr, ms_dense, ms_expand, ms_total; /* device time per phase */
+    double ms_map, ms_csr, ms_dense, ms_expand, ms_total; /* device time per phase (a COUNT fingerprint
+                                                             with MFMA group counts: ms_dense is its box
+                                                             kernel, inside ms_expand) */
 } dim3_stats;
 
 /* Row record of join-aggregate COUNT; same layout as the reference's

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "libdim3.so")
 (OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_ADMISSION, ERR_BUSY, ERR_NOT_FOUND, ERR_UNSUPPORTED,
  ERR_CANCELLED) = range(10)
 OUT_MATERIALIZE, OUT_FINGERPRINT = 0, 1
-OPT_NO_SWAP, OPT_EXCLUDE_EQUAL, OPT_HASH_ONLY, OPT_CLASSICAL, OPT_NO_CLASSICAL = 1, 2, 4, 8, 16
+OPT_NO_SWAP, OPT_EXCLUDE_EQUAL, OPT_HASH_ONLY, OPT_CLASSICAL, OPT_NO_CLASSICAL, OPT_PATH_SUMS = 1, 2, 4, 8, 16, 32
 DENSE_AUTO, DENSE_OFF, DENSE_BITOR, DENSE_MFMA = -1, 0, 1, 2
 AGG = {"sum": 1, "min": 2, "max": 3, "avg": 4}  # DIM3_AGG_*
 
@@ -265,15 +265,17 @@ def status_string(s):
 
 
 def make_opts(output_mode=OUT_MATERIALIZE, no_swap=False, exclude_equal=False, dense_mode=DENSE_AUTO, heavy_k=0,
-              tile_bits=0, block=0, hash_only=False, dense_core=-1, classical=None):
+              tile_bits=0, block=0, hash_only=False, dense_core=-1, classical=None, path_sums=False):
     """classical: None lets f1 choose (AUTO plans), True forces the classical
-    path, False forbids it."""
+    path, False forbids it.  path_sums: a COUNT fingerprint from join-path
+    sums instead of the groups' counts."""
     o = Opts()
     lib().dim3_opts_default(ctypes.byref(o))
     o.output_mode = output_mode
     o.flags = ((OPT_NO_SWAP if no_swap else 0) | (OPT_EXCLUDE_EQUAL if exclude_equal else 0)
                | (OPT_HASH_ONLY if hash_only else 0)
-               | (OPT_CLASSICAL if classical is True else OPT_NO_CLASSICAL if classical is False else 0))
+               | (OPT_CLASSICAL if classical is True else OPT_NO_CLASSICAL if classical is False else 0)
+               | (OPT_PATH_SUMS if path_sums else 0))
     o.dense_mode = dense_mode
     o.heavy_k = heavy_k
     o.tile_bits = tile_bits

@@ -131,6 +131,19 @@ enum SlotId {
     WS_DB,
     WS_DBITS,
     WS_CINT,
+    WS_GC_HM,
+    WS_GC_ZM,
+    WS_GC_XFLAG,
+    WS_GC_ZFLAG,
+    WS_GC_XPOS,
+    WS_GC_ZPOS,
+    WS_GC_RLIST,
+    WS_GC_CLIST,
+    WS_GC_A8,
+    WS_GC_B8,
+    WS_GC_XK,
+    WS_GC_GZ,
+    WS_GC_G,
     WS_BATCH_ARGS,
     WS_BATCH_START,
     WS_BATCH_SCL,
@@ -188,6 +201,12 @@ struct QueryPlan {
     double p1_gain = -1e300;   // saving of the row-and-column-complete choice (phase 1)
     bool core = false;         // rows outside the core keep heavy entries (split S rows)
     uint32_t thx = 1, thz = 1;
+    // COUNT fingerprint with every group's count: heavy keys (bin >= gc_bmin,
+    // gc_K <= 64) as an MFMA product over the box of gc_Mh rows and gc_Nh
+    // columns (gc_Mp rows padded), light keys by (x, z-tile) wavefront tasks
+    bool gc = false;
+    bool gc_dup = false;  // a heavy tuple occurs twice (multiplicity > 1): not this path
+    uint32_t gc_bmin = 0, gc_K = 0, gc_Mh = 0, gc_Nh = 0, gc_Mp = 0;
 };
 
 // The tuple values of a value-aggregate query (agg 0: none).
@@ -722,6 +741,132 @@ void set_layout(QueryPlan& P, const dim3_opts& o) {
     if (P.fused) P.dwords = P.ntiles * P.tile_words;
 }
 
+// ---------------------------------------------------------------------------
+// planner: COUNT fingerprint with every group's count (k_group_count.hip).
+// The heavy keys are the top heaviness bins holding at most 64 keys (or
+// heavy_k); their paths become the MFMA box of the rows and columns that
+// reach one (Mh x Nh cells, from the heaviness histogram), the other paths
+// light (x, z-tile) tasks.  Taken when the model prices it under half of the
+// all-sparse COUNT (or forced by dense_mode = DIM3_DENSE_MFMA).  Rates on one
+// MI355X: the box at ~6e-14 s per cell (VALU-bound: the count bytes packed
+// and flagged in registers), light paths ~4e-12 s (a hash insert each),
+// light tasks ~1e-9 s, the all-sparse COUNT ~2.2e-12 s per join path.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kGcMaxKeys = 64;
+constexpr double kGcCellS = 6e-14, kGcLightPathS = 4e-12, kGcTaskS = 1e-9, kCountPathS = 2.2e-12;
+constexpr uint32_t kGcLightPerTask = 96;  // mean light paths per (x, z-tile) task the tiles aim at
+constexpr uint32_t kGcCols = 16384;       // box columns per dense task
+
+// internal: prepare() planning again after the masks found a duplicate heavy tuple
+constexpr uint32_t kOptNoGc = 1u << 31;
+
+void plan_gc(dim3_ctx* c, const dim3_opts& o, int force_wgs) {
+    auto& P = c->bank().plan;
+    P.gc = false;
+    P.gc_dup = false;
+    if ((o.flags & kOptNoGc) || !P.count_mode || P.agg || o.output_mode != DIM3_OUT_FINGERPRINT || (P.flags & DIM3_OPT_EXCLUDE_EQUAL)) return;
+    if (force_wgs > 0 || o.dense_mode == DIM3_DENSE_OFF || o.dense_mode == DIM3_DENSE_BITOR) return;
+    const unsigned long long* h = c->bank().hhist;
+    const uint32_t NB = dim3h::kDenseBins;
+    const uint64_t kmax = o.heavy_k > 0 ? std::min<uint64_t>((uint64_t)o.heavy_k, kGcMaxKeys) : kGcMaxKeys;
+    uint64_t K = 0, paths = 0, Mh = 0, Nh = 0;
+    uint32_t bmin = 0;
+    for (int b = (int)NB - 1; b >= 1; b--) {
+        if (K + h[b] > kmax) break;
+        K += h[b];
+        paths += h[NB + b];
+        Mh += h[4 * NB + b];
+        Nh += h[5 * NB + b];
+        if (h[b]) bmin = (uint32_t)b;
+    }
+    if (K == 0 || Mh == 0 || Nh == 0) return;
+    const uint64_t light = P.out_j > paths ? P.out_j - paths : 0;
+    uint64_t nt = (light / std::max<uint64_t>(P.nx, 1) + kGcLightPerTask - 1) / kGcLightPerTask;
+    if (o.tile_bits > 0) nt = ((uint64_t)P.nz + (uint64_t)o.tile_bits - 1) / (uint64_t)o.tile_bits;  // caller's z-tile width
+    nt = std::max<uint64_t>(1, std::min<uint64_t>(nt, 1024));
+    nt = std::min<uint64_t>(nt, P.nz);
+    while (nt > 1 && ((uint64_t)P.ny * nt >= (1ull << 31) || (uint64_t)P.nx * nt >= (1ull << 31))) nt >>= 1;
+    const uint64_t rows = Mh;
+    const uint64_t Mp = (rows + dim3h::gc_rows_per_block() - 1) / dim3h::gc_rows_per_block() *
+                        dim3h::gc_rows_per_block();
+    const uint64_t bytes = Mp * (64 + 16) + (Nh + 64) * (64 + 8 + 16) + ((uint64_t)P.nx + P.nz) * 12 + 4 * (uint64_t)P.nx +
+                           4 * (uint64_t)P.nz;
+    const uint64_t room = c->budget > c->held ? (c->budget - c->held) / 3 : 0;
+    if (bytes > room) return;
+    const double t_gc = (double)Mh * (double)Nh * kGcCellS + (double)light * kGcLightPathS +
+                        (double)P.nx * (double)nt * kGcTaskS + 1e-4;
+    const double t_sparse = (double)P.out_j * kCountPathS;
+    if (o.dense_mode != DIM3_DENSE_MFMA && t_gc >= 0.5 * t_sparse) return;
+    P.gc = true;
+    P.gc_bmin = bmin;
+    P.gc_K = (uint32_t)K;
+    P.dense = DIM3_DENSE_OFF;
+    P.core_try = false;
+    P.core = false;
+    P.fused = false;
+    P.ntiles = (uint32_t)nt;
+    P.tile_size = (uint32_t)((P.nz + nt - 1) / nt);
+}
+
+// The GC plan's masks, box lists and operands (after the CSRs); one host
+// synchronisation reads the box size.
+int gc_prepare(dim3_ctx* c, const int32_t* slotS, const int32_t* slotSz, const uint32_t* degR, const uint32_t* degS,
+               const uint64_t* exY, const uint64_t* exZ, uint64_t ny1, hipStream_t st) {
+    Bank& B = c->bank();
+    auto& P = B.plan;
+    Slot* W = B.ws;
+    Scalars* sc = (Scalars*)W[WS_SCALARS].p;
+    int32_t* hid;
+    uint32_t *h2y, *xflag, *zflag, *xpos, *zpos, *rlist, *clist;
+    uint64_t *hm, *zm, *part;
+    TRY(ws(c, WS_HID, P.ny, &hid));
+    TRY(ws(c, WS_H2Y, P.ny, &h2y));
+    TRY(ws(c, WS_GC_HM, P.nx, &hm));
+    TRY(ws(c, WS_GC_ZM, P.nz, &zm));
+    TRY(ws(c, WS_GC_XFLAG, P.nx, &xflag));
+    TRY(ws(c, WS_GC_ZFLAG, P.nz, &zflag));
+    TRY(ws(c, WS_GC_XPOS, P.nx, &xpos));
+    TRY(ws(c, WS_GC_ZPOS, P.nz, &zpos));
+    TRY(ws(c, WS_GC_RLIST, P.nx, &rlist));
+    TRY(ws(c, WS_GC_CLIST, P.nz, &clist));
+    TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed(std::max<uint64_t>(P.nx, P.nz) + 1), &part));
+    uint32_t* counters = (uint32_t*)&sc->spare[0];
+    HIPCHK(c, hipMemsetAsync(counters, 0, 8, st));
+    HIPCHK(c, hipMemsetAsync(&sc->spare[3], 0, 8, st));
+    dim3h::dense_mark_keys(degR, degS, (uint32_t)ny1, exY, P.gc_bmin, hid, h2y, counters, st);
+    dim3h::gc_masks((const uint32_t*)W[WS_ROWPTR_R].p, (int32_t*)W[WS_COL_R].p, P.nx, P.S, P.nS, slotS,
+                    P.z_ident ? nullptr : slotSz, exY, P.z_ident ? nullptr : exZ, P.zlo, hid, P.nz, hm, xflag, zm,
+                    zflag, &sc->spare[3], st);
+    dim3h::scan_u32(xflag, P.nx, xpos, part, &sc->nxh, st);
+    dim3h::gc_list(xflag, xpos, P.nx, rlist, st);
+    dim3h::scan_u32(zflag, P.nz, zpos, part, &sc->nzh, st);
+    dim3h::gc_list(zflag, zpos, P.nz, clist, st);
+    LAUNCHCHK(c, "group-count masks");
+    HIPCHK(c, dim3h::copy_wait(B.hsc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    P.gc_K = (uint32_t)B.hsc->spare[0];  // the keys marked (the bins' count)
+    P.gc_Mh = (uint32_t)B.hsc->nxh;
+    P.gc_Nh = (uint32_t)B.hsc->nzh;
+    const uint32_t rb = dim3h::gc_rows_per_block();
+    P.gc_Mp = (uint32_t)(((uint64_t)P.gc_Mh + rb - 1) / rb * rb);
+    if (P.gc_K > kGcMaxKeys) return fail(c, DIM3_ERR_DEVICE, "group count: %u heavy keys marked", P.gc_K);
+    if (B.hsc->spare[3] != 0) {  // a heavy tuple twice: prepare() plans again without this path
+        P.gc_dup = true;
+        return DIM3_OK;
+    }
+    int8_t *A8, *B8;
+    uint64_t *xk, *gz, *G;
+    const uint64_t Np = dim3h::gc_cols_padded(P.gc_Nh);
+    TRY(ws(c, WS_GC_A8, std::max<uint64_t>((uint64_t)P.gc_Mp * 64, 64), &A8));
+    TRY(ws(c, WS_GC_XK, std::max<uint64_t>(2ull * P.gc_Mp, 2), &xk));
+    TRY(ws(c, WS_GC_B8, std::max<uint64_t>(Np * 64, 64), &B8));
+    TRY(ws(c, WS_GC_GZ, std::max<uint64_t>(Np, 1), &gz));
+    TRY(ws(c, WS_GC_G, std::max<uint64_t>(Np * 2, 2), &G));  // 16 bytes per column
+    dim3h::gc_build(rlist, P.gc_Mh, P.gc_Mp, hm, (const int32_t*)W[WS_ID2X].p, clist, P.gc_Nh, zm,
+                    P.z_ident ? nullptr : (const int32_t*)W[WS_ID2Z].p, P.zlo, A8, xk, B8, gz, G, st);
+    LAUNCHCHK(c, "group-count operands");
+    return DIM3_OK;
+}
+
 // Mapping + CSR of one query.  Leaves the plan in c->bank().plan.
 // A query is prepared in three stages, each ending with a copy of the
 // device scalars to the bank's host mirror: start (swap rule, min/max of
@@ -743,11 +888,12 @@ int prepare_start(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, con
         HIPCHK(c, dim3h::host_malloc((void**)&B.hsc, sizeof(Scalars)));
         HIPCHK(c, dim3h::host_malloc((void**)&B.hhist, 6 * dim3h::kDenseBins * sizeof(unsigned long long)));
     }
-    // A COUNT fingerprint needs no counters: its groups are the join-project
-    // rows (n, sum) and its sum2 / csum are sums over the join paths, taken
-    // from the key degrees.  With the u != w filter the paths are counted
-    // per group instead.
-    P.agg_fp = count_mode && !P.agg && o.output_mode == DIM3_OUT_FINGERPRINT && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL);
+    // With DIM3_OPT_PATH_SUMS a COUNT fingerprint forms no counter: its
+    // groups are the join-project rows (n, sum) and its sum2 / csum are sums
+    // over the join paths, taken from the key degrees.  Otherwise (and always
+    // with the u != w filter) every group's count is formed.
+    P.agg_fp = count_mode && !P.agg && o.output_mode == DIM3_OUT_FINGERPRINT && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL) &&
+               (o.flags & DIM3_OPT_PATH_SUMS);
     if (P.agg_fp) count_mode = false;
     P.count_mode = count_mode;
     P.flags = o.flags;
@@ -936,6 +1082,7 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     P.empty = false;
     plan_tiles(c, o, force_wgs);
     plan_dense(c, o);
+    plan_gc(c, o, force_wgs);
     P.fused = P.dense == DIM3_DENSE_BITOR && P.all_z;
     set_layout(P, o);
     if ((uint64_t)P.ny * P.ntiles >= (1ull << 31))
@@ -1035,6 +1182,7 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
         dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
     LAUNCHCHK(c, "csr");
     HIPCHK(c, hipEventRecord(c->ev[3], st));
+    if (P.gc) TRY(gc_prepare(c, slotS, slotSz, degR, degS, exY, exZ, ny1, st));
     if (P.dense != DIM3_DENSE_OFF) {
         // the product of the heavy keys: Dbits (project) or Cint (COUNT)
         uint64_t* overflow = &sc->spare[1];
@@ -1087,7 +1235,13 @@ int prepare(dim3_ctx* c, bool count_mode, const int2* dR, uint64_t nR, const int
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     P.ms_map = ms;
-    return prepare_plan(c, o, st, force_wgs);
+    TRY(prepare_plan(c, o, st, force_wgs));
+    if (P.gc && P.gc_dup) {
+        dim3_opts o2 = o;
+        o2.flags |= kOptNoGc;
+        TRY(prepare_plan(c, o2, st, force_wgs));
+    }
+    return DIM3_OK;
 }
 
 // Expansion arguments of the current bank's prepared plan.
@@ -1212,6 +1366,29 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
         HIPCHK(c, hipStreamSynchronize(st));
         return DIM3_OK;
     }
+    if (P.gc) {
+        // the dense part's terms over the box, then the light groups'
+        const auto& W = c->bank().ws;
+        hipMemsetAsync(&sc->out_n, 0, sizeof(uint64_t) * 5, st);
+        hipMemsetAsync(&sc->task, 0, 2 * sizeof(uint32_t), st);
+        HIPCHK(c, hipEventRecord(c->ev[4], st));
+        int e = dim3h::gc_dense((const int8_t*)W[WS_GC_A8].p, (const uint64_t*)W[WS_GC_XK].p, P.gc_Mp,
+                                (const int8_t*)W[WS_GC_B8].p, W[WS_GC_G].p, dim3h::gc_cols_padded(P.gc_Nh), kGcCols,
+                                sc, st);
+        if (e != 0) return fail(c, DIM3_ERR_DEVICE, "group-count dense launch: %s", hipGetErrorString((hipError_t)e));
+        HIPCHK(c, hipEventRecord(c->ev[7], st));
+        e = dim3h::gc_light((const uint32_t*)W[WS_ROWPTR_R].p, (const int32_t*)W[WS_COL_R].p,
+                            (const uint32_t*)W[WS_ROWPTR_S].p, (const int32_t*)W[WS_COL_S].p,
+                            (const uint64_t*)W[WS_GC_HM].p,
+                            (const uint64_t*)W[WS_GC_ZM].p, (const int32_t*)W[WS_ID2X].p,
+                            P.z_ident ? nullptr : (const int32_t*)W[WS_ID2Z].p, P.zlo, P.nx, P.ntiles, P.tile_size,
+                            P.nz, c->grid_cus(), sc, st);
+        if (e != 0) return fail(c, DIM3_ERR_DEVICE, "group-count light launch: %s", hipGetErrorString((hipError_t)e));
+        HIPCHK(c, hipEventRecord(c->ev[5], st));
+        HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        return DIM3_OK;
+    }
     dim3h::ExpandArgs a = make_args(c, fp_mode, out, cap);
     if (P.agg) TRY(value_task_lists(c, a, st, true));
     // an all-sparse COUNT or join-project: rows of at most 256 paths one per
@@ -1269,10 +1446,10 @@ void fill_result(dim3_ctx* c, dim3_result* res, const Scalars& s, bool timing) {
     st.nz = P.nz;
     st.out_j = P.out_j;
     st.paths_sparse = P.empty ? 0 : s.paths;
-    st.dense_mode = (uint32_t)P.dense;
-    st.heavy_keys = P.dense ? P.K : 0;
-    st.nx_heavy = P.dense ? P.Mh : 0;
-    st.nz_heavy = P.dense ? P.Nh : 0;
+    st.dense_mode = P.gc ? (uint32_t)DIM3_DENSE_MFMA : (uint32_t)P.dense;
+    st.heavy_keys = P.gc ? P.gc_K : P.dense ? P.K : 0;
+    st.nx_heavy = P.gc ? P.gc_Mh : P.dense ? P.Mh : 0;
+    st.nz_heavy = P.gc ? P.gc_Nh : P.dense ? P.Nh : 0;
     st.tile_bits = P.empty ? 0 : P.tile_size;
     st.n_tiles = P.ntiles;
     st.block = (uint32_t)P.block;
@@ -1283,6 +1460,10 @@ void fill_result(dim3_ctx* c, dim3_result* res, const Scalars& s, bool timing) {
         st.ms_csr = ms;
         hipEventElapsedTime(&ms, c->ev[3], c->ev[6]);
         st.ms_dense = ms;
+        if (P.gc) {  // the MFMA box kernel (part of the expansion)
+            hipEventElapsedTime(&ms, c->ev[4], c->ev[7]);
+            st.ms_dense = ms;
+        }
         hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
         st.ms_expand = ms;
         hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
@@ -1903,9 +2084,10 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     HIPCHK(c, hipStreamSynchronize(st));  // host vectors are pageable
     if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block, fp);
     unsigned grid = 1;
-    // the expansion kernel of every instance (a COUNT fingerprint without
-    // the u != w filter expands as a join-project)
-    const bool kcount = count_mode && !(fp && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL));
+    // the expansion kernel of every instance (a COUNT fingerprint from path
+    // sums expands as a join-project)
+    const bool kcount =
+        count_mode && !(fp && (o.flags & DIM3_OPT_PATH_SUMS) && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL));
     int e = dim3h::expand_batch_setup(kcount, block, lds, c->grid_cus(), std::max<uint64_t>(total, 1), &grid);
     if (e != 0) return fail(c, DIM3_ERR_DEVICE, "batch setup: %s", hipGetErrorString((hipError_t)e));
 

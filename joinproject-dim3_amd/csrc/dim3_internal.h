@@ -656,6 +656,35 @@ int classical_join(const int2* R, uint64_t nR, const int2* S, uint64_t nS, bool 
                    bool fp, const std::function<int(uint64_t, void**, uint64_t*)>& get_out, Scalars* sc,
                    hipStream_t st, std::string* err);
 
+// COUNT fingerprint with every group's count formed (k_group_count.hip):
+// heavy keys hid[y] in [0, 64) as bit masks hm[x] / zm[z] (xflag / zflag:
+// mask != 0; the R CSR's heavy entries get bit 31), the box rows / columns
+// as lists, their operand bytes A8 [Mp][64] / B8 [Np][64] with the x factors
+// xk [Mp][2], the z factors gz [Np] and the reduction operand G (Np / 64
+// steps of 64 lanes x 16 bytes), where Mp is Mh rounded up to
+// gc_rows_per_block() and Np is gc_cols_padded(Nh) (zero rows and columns
+// past the box); then the dense part's terms (gc_dense, box columns per task
+// a multiple of 64) and the light part's terms with their corrections
+// (gc_light, over the tiled S CSR).  gc_masks sets *dup (zeroed by the
+// caller) when a heavy tuple occurs twice, which a mask cannot count: the
+// plan is then dropped.  Both add to sc's out_n and fingerprint sums;
+// gc_light also counts the light paths in sc->paths and uses sc->task.
+uint32_t gc_rows_per_block();
+uint32_t gc_cols_padded(uint32_t Nh);
+void gc_masks(const uint32_t* rowptrR, int32_t* colR, uint32_t nx, const int2* S, uint64_t nS, const int32_t* slotS,
+              const int32_t* slotSz, const uint64_t* ex_y, const uint64_t* ex_z, int32_t zlo, const int32_t* hid,
+              uint32_t nz, uint64_t* hm, uint32_t* xflag, uint64_t* zm, uint32_t* zflag, uint64_t* dup,
+              hipStream_t st);
+void gc_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list, hipStream_t st);
+void gc_build(const uint32_t* rlist, uint32_t Mh, uint32_t Mp, const uint64_t* hm, const int32_t* id2x,
+              const uint32_t* clist, uint32_t Nh, const uint64_t* zm, const int32_t* id2z, int32_t zlo, int8_t* A8,
+              uint64_t* xk, int8_t* B8, uint64_t* gz, void* G, hipStream_t st);
+int gc_dense(const int8_t* A8, const uint64_t* xk, uint32_t Mp, const int8_t* B8, const void* G, uint32_t Np,
+             uint32_t cols, Scalars* sc, hipStream_t st);
+int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowptrS, const int32_t* colS,
+             const uint64_t* hm, const uint64_t* zm, const int32_t* id2x, const int32_t* id2z, int32_t zlo, uint32_t nx,
+             uint32_t ntiles, uint32_t tile_size, uint32_t nz, int cu_count, Scalars* sc, hipStream_t st);
+
 // Batched expansion: one persistent launch over the tasks of n_inst
 // instances (args[i], tasks task_start[i] .. task_start[i + 1]).
 // setup: sets the LDS attribute and returns the grid (no stream work, so it

@@ -1,0 +1,588 @@
+// COUNT fingerprint with every group's count formed: the heavy keys' counts
+// as an int8 0/1 product on the MFMA cores, the light keys' counts in
+// per-wavefront LDS tables.
+//
+// Replaces SpmmAggregate with a COUNT policy (dim3.hpp:1267-1311) for a
+// query whose heavy join keys make the (x, z) box dense (the 10M x 10M
+// Zipf(1.2) COUNT has 2.4e12 groups in a 2e6 x 2e6 box).  The reference
+// counts each group in its SPA and hands (x, z, count) to the policy; here
+// the count of group (x, z) is
+//   c(x, z) = c_d(x, z) + c_l(x, z),
+//   c_d = popcount(hm(x) & zm(z))   (hm / zm: bit h set when x / z has a
+//                                    tuple with heavy key h, K <= 64 keys)
+//   c_l = join paths of (x, z) through light keys.
+// The query's fingerprint (count, sum, sum2, csum) is a sum of per-group
+// terms (1, k1(x) g(z), k2(x) c g(z), c) over the groups with c > 0, so it
+// splits exactly into
+//   dense part (k_gc_dense): the terms of c_d over every cell of the box of
+//     rows with hm != 0 and columns with zm != 0, where c_d is
+//     A[x][h] * B[z][h] summed over h on v_mfma_i32_16x16x64_i8 (A, B the
+//     0/1 bytes of hm, zm: exact, c_d <= 64);
+//   light part (k_gc_light): per light group with c_l > 0, its count c_l
+//     formed in LDS, c_d = popcount(hm(x) & zm(z)) looked up, and the terms
+//     (c_d == 0 ? 1 : 0, c_d == 0 ? k1 g : 0, k2 c_l g, c_l): a group the
+//     dense part already counted adds its light paths only.
+// The per-cell fingerprint of the dense part is itself a matrix product:
+// with g(z) = sum_b 2^(7 b) g_b(z) in 7-bit chunks (b < 10),
+//   sum_{x,z} k2(x) c(x,z) g(z) = sum_x k2(x) sum_b 2^(7 b) E2[b][x],
+//   E2[b][x] = sum_z g_b(z) c(x, z),
+// and E2 is a second int8 MFMA whose k dimension is z: its B operand is the
+// count tiles of the first product (computed transposed, C^T[z][x], so the
+// lane holding z = 4 q .. 4 q + 3 of a tile holds the operand bytes of
+// k = 16 q .. 16 q + 15 once the four z tiles of a step are packed to bytes
+// in registers) and its A operand the chunk bytes of the step's 64 z
+// factors, laid out per lane in advance.  E2 accumulates in the MFMA over
+// the whole z sweep of a task; k2(x) multiplies once per row at the end.
+// Row b = 10 of the chunk operand is all ones, so E2[10][x] = sum_z c
+// (csum).  E1 does the same over the bytes -128 [c > 0] (sum and, by its
+// ones row, count, each -128 times its exact value until the end).  Per
+// 4096 cells a wavefront issues 16 product and 8 reduction MFMAs and about
+// 90 VALU instructions; no count leaves the registers.
+#include "dim3_internal.h"
+
+namespace {
+
+using namespace dim3d;
+using dim3h::Scalars;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t kGcRowsW = 64;                 // box rows (x) per wavefront: 4 groups of 16
+constexpr uint32_t kGcRowsB = 4 * kGcRowsW;       // box rows per workgroup (4 wavefronts)
+constexpr uint32_t kGcColsStep = 64;              // box columns (z) per step: the reduction's k
+constexpr uint32_t kGcK = 64;                     // heavy keys (bytes per operand row)
+
+__device__ __forceinline__ uint32_t key_chunk(uint64_t k, uint32_t b) {
+    return b < 9u ? (uint32_t)(k >> (7u * b)) & 127u : b == 9u ? (uint32_t)(k >> 63) : b == 10u ? 1u : 0u;
+}
+
+// hm[x] = OR of 1 << hid[y] over the heavy keys y of row x (R CSR by x),
+// and bit 31 set on the row's heavy entries (the light part skips them).
+// A heavy (x, y) tuple seen twice sets *dup: a bit mask cannot count it.
+__global__ void k_gc_row_masks(const uint32_t* __restrict__ rowptrR, int32_t* __restrict__ colR, uint32_t nx,
+                               const int32_t* __restrict__ hid, uint64_t* __restrict__ hm,
+                               uint32_t* __restrict__ flag, unsigned long long* dup) {
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
+    if (x >= nx) return;
+    uint64_t m = 0;
+    bool twice = false;
+    for (uint32_t i = rowptrR[x], e = rowptrR[x + 1]; i < e; i++) {
+        const uint32_t y = (uint32_t)colR[i] & 0x7FFFFFFFu;
+        const int32_t h = hid[y];
+        if (h >= 0) {
+            twice = twice || ((m >> h) & 1ull);
+            m |= 1ull << h;
+            colR[i] = (int32_t)(y | 0x80000000u);
+        }
+    }
+    hm[x] = m;
+    flag[x] = m != 0ull ? 1u : 0u;
+    if (twice) atomicOr(dup, 1ull);
+}
+
+// zm[z] |= 1 << hid[y] for every S tuple (z, y) with a heavy key (a
+// duplicate sets *dup)
+__global__ void k_gc_col_masks(const int2* __restrict__ S, uint64_t nS, const int32_t* __restrict__ slotS,
+                               const int32_t* __restrict__ slotSz, const uint64_t* __restrict__ ex_y,
+                               const uint64_t* __restrict__ ex_z, int32_t zlo, const int32_t* __restrict__ hid,
+                               uint64_t* __restrict__ zm, unsigned long long* dup) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= nS) return;
+    const int32_t s = slotS[i];
+    if (s < 0) return;
+    const int32_t h = hid[(uint32_t)(ex_y[s] >> 32)];
+    if (h < 0) return;
+    const uint32_t zid = ex_z != nullptr ? (uint32_t)(ex_z[slotSz[i]] >> 32) : (uint32_t)S[i].x - (uint32_t)zlo;
+    const uint64_t old = atomicOr((unsigned long long*)&zm[zid], 1ull << h);
+    if ((old >> h) & 1ull) atomicOr(dup, 1ull);
+}
+
+__global__ void k_gc_col_flags(const uint64_t* __restrict__ zm, uint32_t nz, uint32_t* __restrict__ flag) {
+    const uint32_t z = blockIdx.x * 256u + threadIdx.x;
+    if (z < nz) flag[z] = zm[z] != 0ull ? 1u : 0u;
+}
+
+// compact list of the flagged IDs (pos: exclusive scan of flag)
+__global__ void k_gc_list(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, uint32_t n,
+                          uint32_t* __restrict__ list) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n && flag[i]) list[pos[i]] = i;
+}
+
+// 64 bits -> 64 bytes of 0 / 1 (16-byte stores)
+__device__ __forceinline__ void bits_to_bytes(uint64_t m, int8_t* dst) {
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+        uint32_t w[4];
+#pragma unroll
+        for (uint32_t d = 0; d < 4; d++) {
+            const uint32_t nib = (uint32_t)(m >> (16u * q + 4u * d)) & 15u;
+            // bit i of the nibble -> byte i
+            w[d] = ((nib * 0x00204081u) & 0x01010101u);
+        }
+        *reinterpret_cast<uint4*>(dst + 16u * q) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// Box rows r < Mp: A8[r] = bytes of hm(list[r]), xk[r] = (k1, k2) of its
+// x value; padding rows (r >= Mh) zero.
+__global__ void k_gc_build_rows(const uint32_t* __restrict__ rlist, uint32_t Mh, uint32_t Mp,
+                                const uint64_t* __restrict__ hm, const int32_t* __restrict__ id2x,
+                                int8_t* __restrict__ A8, uint64_t* __restrict__ xk) {
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= Mp) return;
+    uint64_t m = 0, k1 = 0, k2 = 0;
+    if (r < Mh) {
+        const uint32_t x = rlist[r];
+        m = hm[x];
+        const int32_t xv = id2x[x];
+        k1 = xfactor1(xv);
+        k2 = xfactor2(xv);
+    }
+    bits_to_bytes(m, A8 + (size_t)r * kGcK);
+    xk[2u * r] = k1;
+    xk[2u * r + 1u] = k2;
+}
+
+// Box columns n < Np (Np: Nh rounded up to 64): B8[n] = bytes of
+// zm(list[n]), gz[n] = g(z value); padding columns zero.
+__global__ void k_gc_build_cols(const uint32_t* __restrict__ clist, uint32_t Nh, uint32_t Np,
+                                const uint64_t* __restrict__ zm, const int32_t* __restrict__ id2z, int32_t zlo,
+                                int8_t* __restrict__ B8, uint64_t* __restrict__ gz) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= Np) return;
+    uint64_t m = 0, g = 0;
+    if (n < Nh) {
+        const uint32_t z = clist[n];
+        m = zm[z];
+        g = zfactor(id2z != nullptr ? id2z[z] : (int32_t)((uint32_t)zlo + z));
+    }
+    bits_to_bytes(m, B8 + (size_t)n * kGcK);
+    gz[n] = g;
+}
+
+// The reduction's A operand per 64-column step s, in lane order: lane l of
+// step s holds 16 bytes, byte j = chunk (l % 16) of g(z) for the step's
+// column z = 64 s + 16 (j / 4) + 4 (l / 16) + (j % 4) (the k index of byte j
+// of a packed count tile, see k_gc_dense); chunk 10 is 1, chunks 11-15 0.
+__global__ void k_gc_build_gchunks(const uint64_t* __restrict__ gz, uint32_t nsteps, uint4* __restrict__ G) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // step * 64 + lane
+    if (i >= nsteps * 64u) return;
+    const uint32_t s = i >> 6, l = i & 63u, b = l & 15u, q = l >> 4;
+    uint32_t w[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < 4; e++) v |= key_chunk(gz[64u * s + 16u * u + 4u * q + e], b) << (8u * e);
+        w[u] = v;
+    }
+    G[i] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// The dense part's fingerprint terms.  One workgroup = 4 wavefronts = one
+// task: box rows [256 xb, 256 xb + 256) (64 per wavefront, their operand
+// bytes held in 16 VGPRs) times box columns [cols * zc, cols * zc + cols),
+// 64 columns per step; the next step's column operands are loaded while the
+// current step computes.  Per step and 16-row group: four product MFMAs give
+// the transposed count tiles C^T[z][x] (z = 16 u + 4 q + e in lane (x, q)),
+// packed to bytes (count <= 64) and flagged (count > 0, as the int8 -128)
+// in registers, then two reduction MFMAs add E2 += G * counts,
+// E1 += G * flags.  Tasks are
+// numbered column-chunk major and dealt to the 8 XCDs in contiguous runs (a
+// block goes to XCD blockIdx % 8), so an XCD's workgroups share a column
+// chunk's operands in its L2.  Padding: A8 / xk cover Mp = 256 n_xb rows,
+// B8 / G cover Np columns (a multiple of 64), zero (G: chunk 10) past the box.
+__global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ A8, const uint64_t* __restrict__ xk,
+                                                      uint32_t n_xb, const int8_t* __restrict__ B8,
+                                                      const uint4* __restrict__ G, uint32_t Np, uint32_t cols,
+                                                      uint32_t n_tasks, uint32_t per_xcd, Scalars* sc) {
+    __shared__ uint64_t red[4][4];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const uint32_t q = lane >> 4, li = lane & 15u;
+    const uint32_t task = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    if (task >= n_tasks) return;  // workgroup-uniform, before any barrier
+    const uint32_t zc = task / n_xb, xb = task - zc * n_xb;
+    const uint32_t x0 = xb * kGcRowsB + wid * kGcRowsW;
+
+    // row operands: group gx, lane (li, q) = row x0 + 16 gx + li, keys 16 q .. 16 q + 15
+    v4i xa[4];
+#pragma unroll
+    for (uint32_t gx = 0; gx < 4; gx++)
+        xa[gx] = *reinterpret_cast<const v4i*>(A8 + (size_t)(x0 + 16u * gx + li) * kGcK + 16u * q);
+    v4i e1[4], e2[4];
+#pragma unroll
+    for (uint32_t gx = 0; gx < 4; gx++) {
+        e1[gx] = v4i{0, 0, 0, 0};
+        e2[gx] = v4i{0, 0, 0, 0};
+    }
+    const uint32_t z0 = zc * cols;
+    const uint32_t z1 = min(z0 + cols, Np);
+    // column operands of a step: tile u, lane (li, q) = column zs + 16 u + li, keys 16 q ..
+    auto load_step = [&](uint32_t zs, v4i* zf, v4i& gf) {
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++)
+            zf[u] = *reinterpret_cast<const v4i*>(B8 + (size_t)(zs + 16u * u + li) * kGcK + 16u * q);
+        const uint4 g4 = G[(size_t)(zs / kGcColsStep) * 64u + lane];
+        gf = v4i{(int)g4.x, (int)g4.y, (int)g4.z, (int)g4.w};
+    };
+    auto compute = [&](const v4i* zf, const v4i& gf) {
+#pragma unroll
+        for (uint32_t gx = 0; gx < 4; gx++) {
+            v4i c[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++)
+                c[u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(zf[u], xa[gx], v4i{0, 0, 0, 0}, 0, 0, 0);
+            v4i pk, fl;
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+                // counts <= 64: one byte each (v_perm_b32: selector bytes 0-3
+                // pick the second operand's bytes, 4-7 the first's, 0x0C zero)
+                const uint32_t lo = __builtin_amdgcn_perm((uint32_t)c[u][1], (uint32_t)c[u][0], 0x0C0C0400u);
+                const uint32_t hi = __builtin_amdgcn_perm((uint32_t)c[u][3], (uint32_t)c[u][2], 0x0C0C0400u);
+                const uint32_t p = __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+                pk[u] = (int)p;
+                // byte 0x80 (int8 -128) where the count is > 0 (no carry
+                // between bytes: count + 127 <= 191); E1 is -128 times the
+                // flag sums
+                fl[u] = (int)((p + 0x7F7F7F7Fu) & 0x80808080u);
+            }
+            e2[gx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gf, pk, e2[gx], 0, 0, 0);
+            e1[gx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gf, fl, e1[gx], 0, 0, 0);
+        }
+    };
+    // two operand buffers in turn: the next step's loads are issued before
+    // the current step's MFMAs (the address clamped to the last step, so the
+    // loads are unconditional; a scheduling barrier keeps the compiler from
+    // sinking them to their use)
+    const uint32_t zlast = z1 - kGcColsStep;
+    v4i zfa[4], gfa, zfb[4], gfb;
+    load_step(z0, zfa, gfa);
+    for (uint32_t zs = z0;; zs += 2u * kGcColsStep) {
+        load_step(min(zs + kGcColsStep, zlast), zfb, gfb);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(zfa, gfa);
+        if (zs + kGcColsStep >= z1) break;
+        load_step(min(zs + 2u * kGcColsStep, zlast), zfa, gfa);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(zfb, gfb);
+        if (zs + 2u * kGcColsStep >= z1) break;
+    }
+    // lane (li, q) of group gx: row x0 + 16 gx + li, chunk rows b = 4 q + e
+    uint64_t S1 = 0, S2 = 0, CNT = 0, CS = 0;
+#pragma unroll
+    for (uint32_t gx = 0; gx < 4; gx++) {
+        const uint32_t x = x0 + 16u * gx + li;
+        const uint64_t k1 = xk[2u * x], k2 = xk[2u * x + 1u];
+#pragma unroll
+        for (uint32_t e = 0; e < 4; e++) {
+            const uint32_t b = 4u * q + e;
+            // e1 = -128 x (a sum below 2^22): exact division
+            const uint64_t v1 = (uint64_t)(uint32_t)((-e1[gx][e]) >> 7), v2 = (uint64_t)(uint32_t)e2[gx][e];
+            if (b < 10u) {
+                S1 += k1 * (v1 << (7u * b));
+                S2 += k2 * (v2 << (7u * b));
+            } else if (b == 10u) {
+                CNT += v1;
+                CS += v2;
+            }
+        }
+    }
+    S1 = wave_reduce_sum(S1);
+    S2 = wave_reduce_sum(S2);
+    CNT = wave_reduce_sum(CNT);
+    CS = wave_reduce_sum(CS);
+    if (lane == 0) {
+        red[wid][0] = S1;
+        red[wid][1] = S2;
+        red[wid][2] = CNT;
+        red[wid][3] = CS;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const uint64_t v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        unsigned long long* dst = (unsigned long long*)(threadIdx.x == 0   ? &sc->fp_sum
+                                                        : threadIdx.x == 1 ? &sc->fp_sum2
+                                                        : threadIdx.x == 2 ? &sc->out_n
+                                                                           : &sc->fp_csum);
+        if (v) atomicAdd(dst, (unsigned long long)v);
+    }
+}
+
+// ---- light part: one (x, z-tile) task per wavefront ----
+// The task's light join paths (its R row's entries without the heavy flag,
+// each key's S segment in tile t of the tiled S CSR) are counted per z ID in
+// a private LDS hash table of kLSlots slots.  A task of at most kLCap paths
+// runs in one pass.  A longer one runs in passes over z ranges whose width
+// is guessed from its path count (kLCap / 2 paths per pass if spread
+// evenly); a pass whose groups pass kLCap stops, clears its slots and is
+// retried at half the width, so a pass never lists more than kLCap + 64
+// groups (the table stays under two thirds full) and a width of kLCap IDs
+// always fits.  A row of at most 64 R entries keeps its segment table in
+// LDS from the first sweep to the last pass.
+constexpr uint32_t kLBits = 9;
+constexpr uint32_t kLSlots = 1u << kLBits;
+constexpr uint32_t kLCap = kLSlots / 2;
+constexpr uint32_t kLChunk = 16;  // tasks per claim of the work counter
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+struct LightLds {
+    uint32_t keys[kLSlots];
+    uint32_t cnt[kLSlots];
+    uint32_t touched[kLCap + 64];
+    uint32_t sb[64];
+    uint32_t pre[64];
+};
+
+__device__ __forceinline__ void lds_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t light_slot(uint32_t* keys, uint32_t z) {
+    uint32_t s = (z * 0x9E3779B1u) >> (32u - kLBits);
+    for (;;) {
+        uint32_t k = keys[s];
+        if (k == z) return s;
+        if (k == kNone) {
+            k = atomicCAS(&keys[s], kNone, z);
+            if (k == kNone || k == z) return s;
+        }
+        s = (s + 1u) & (kLSlots - 1u);
+    }
+}
+
+// Segment (S start, path count) of R entry i of tile t: empty for a heavy entry.
+__device__ __forceinline__ void light_segment(const int32_t* colR, const uint32_t* rowptrS, uint32_t ntiles,
+                                              uint32_t t, uint32_t i, uint32_t& sb, uint32_t& len) {
+    sb = 0;
+    len = 0;
+    const int32_t craw = colR[i];
+    if (craw >= 0) {
+        const uint32_t k = (uint32_t)craw * ntiles + t;
+        sb = rowptrS[k];
+        len = rowptrS[k + 1] - sb;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ rowptrR, const int32_t* __restrict__ colR,
+                                                  const uint32_t* __restrict__ rowptrS, const int32_t* __restrict__ colS,
+                                                  const uint64_t* __restrict__ hm, const uint64_t* __restrict__ zm,
+                                                  const int32_t* __restrict__ id2x, const int32_t* __restrict__ id2z,
+                                                  int32_t zlo, uint32_t nx, uint32_t ntiles, uint32_t tile_size,
+                                                  uint32_t nz, Scalars* sc) {
+    __shared__ LightLds wl[4];
+    __shared__ uint64_t sh[5];
+    const uint32_t lane = threadIdx.x & 63u;
+    LightLds& W = wl[threadIdx.x >> 6];
+    for (uint32_t s = lane; s < kLSlots; s += 64u) {
+        W.keys[s] = kNone;
+        W.cnt[s] = 0;
+    }
+    lds_wave_sync();
+    const uint32_t ntask = nx * ntiles;
+    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0, nrows = 0;
+    uint32_t task = 0, task_end = 0;
+    for (;;) {
+        if (task == task_end) {
+            uint32_t t0 = 0;
+            if (lane == 0) t0 = atomicAdd(&sc->task, kLChunk);
+            task = (uint32_t)__builtin_amdgcn_readfirstlane((int)t0);
+            if (task >= ntask) break;
+            task_end = min(task + kLChunk, ntask);
+        }
+        const uint32_t ti = task++;
+        const uint32_t x = ti / ntiles, t = ti - x * ntiles;
+        const uint32_t rb = rowptrR[x], re = rowptrR[x + 1];
+        const bool short_row = re - rb <= 64u;  // wave-uniform
+        // the task's light paths; a short row's segment table stays in LDS
+        uint32_t P = 0;
+        if (short_row) {
+            uint32_t sb = 0, len = 0;
+            if (rb + lane < re) light_segment(colR, rowptrS, ntiles, t, rb + lane, sb, len);
+            const uint32_t inc = wave_scan_u32(len);
+            P = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+            W.sb[lane] = sb;
+            W.pre[lane] = inc - len;
+            lds_wave_sync();
+        } else {
+            for (uint32_t c0 = rb; c0 < re; c0 += 64u) {
+                uint32_t sb = 0, len = 0;
+                if (c0 + lane < re) light_segment(colR, rowptrS, ntiles, t, c0 + lane, sb, len);
+                P += wave_sum_u32(len);
+            }
+        }
+        if (P == 0) continue;  // wave-uniform
+        npaths += lane == 0 ? P : 0u;
+        const uint64_t hmx = hm[x];
+        const int32_t xv = id2x[x];
+        const uint64_t k1 = xfactor1(xv), k2 = xfactor2(xv);
+        const uint32_t zt0 = t * tile_size, zt1 = min(zt0 + tile_size, nz);
+        uint32_t width = zt1 - zt0;
+        if (P > kLCap) width = (uint32_t)max<uint64_t>(1, (uint64_t)(zt1 - zt0) * (kLCap / 2) / P);
+        uint32_t zcur = zt0;
+        while (zcur < zt1) {
+            const uint32_t zhi = zt1 - zcur <= width ? zt1 : zcur + width;
+            uint32_t nt = 0;  // listed slots (wave-uniform)
+            bool over = false;
+            for (uint32_t c0 = rb; c0 < re && !over; c0 += 64u) {
+                uint32_t PC = P;
+                if (!short_row) {  // this chunk's segment table
+                    uint32_t sb = 0, len = 0;
+                    if (c0 + lane < re) light_segment(colR, rowptrS, ntiles, t, c0 + lane, sb, len);
+                    const uint32_t inc = wave_scan_u32(len);
+                    PC = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+                    W.sb[lane] = sb;
+                    W.pre[lane] = inc - len;
+                    lds_wave_sync();
+                }
+                for (uint32_t p0 = 0; p0 < PC; p0 += 64u) {
+                    if (nt > kLCap) {  // too many groups for this width: retry narrower
+                        over = true;
+                        break;
+                    }
+                    const uint32_t p = p0 + lane;
+                    const bool in = p < PC;
+                    // the last segment whose path prefix is <= p holds path p
+                    uint32_t seg = 0;
+#pragma unroll
+                    for (uint32_t step = 32; step > 0; step >>= 1)
+                        if (W.pre[seg + step] <= p) seg += step;
+                    const uint32_t z = in ? (uint32_t)colS[W.sb[seg] + (p - W.pre[seg])] : 0u;
+                    const bool ok = in && z >= zcur && z < zhi;
+                    uint32_t old = 1u, slot = 0;
+                    if (ok) {
+                        slot = light_slot(W.keys, z);
+                        old = atomicAdd(&W.cnt[slot], 1u);
+                    }
+                    const uint64_t m = __ballot(old == 0u);
+                    if (old == 0u) W.touched[nt + mbcnt64(m)] = slot;
+                    nt += (uint32_t)__popcll(m);
+                }
+                if (!short_row) lds_wave_sync();  // the table is rewritten by the next 64 R entries
+            }
+            if (over) {
+                for (uint32_t j = lane; j < nt; j += 64u) {
+                    const uint32_t s = W.touched[j];
+                    W.cnt[s] = 0;
+                    W.keys[s] = kNone;
+                }
+                width = max(1u, width >> 1);
+                lds_wave_sync();
+                continue;
+            }
+            // emission: the light groups and their dense counts
+            uint64_t S1 = 0, S2 = 0;
+            uint32_t T = 0;
+            for (uint32_t j = lane; j < nt; j += 64u) {
+                const uint32_t s = W.touched[j];
+                const uint32_t cl = W.cnt[s];
+                const uint32_t zid = W.keys[s];
+                const uint64_t g = zfactor(id2z != nullptr ? id2z[zid] : (int32_t)((uint32_t)zlo + zid));
+                if ((hmx & zm[zid]) == 0ull) {  // a group of light paths only
+                    S1 += g;
+                    T += 1u;
+                }
+                S2 += (uint64_t)cl * g;
+                fcs += cl;
+                W.cnt[s] = 0;
+                W.keys[s] = kNone;
+            }
+            fsum += k1 * S1;
+            fsum2 += k2 * S2;
+            nrows += T;
+            lds_wave_sync();
+            zcur = zhi;
+        }
+    }
+    const uint64_t np = block_reduce_sum<256, uint64_t>(npaths, sh);
+    const uint64_t s1 = block_reduce_sum<256, uint64_t>(fsum, sh);
+    const uint64_t s2 = block_reduce_sum<256, uint64_t>(fsum2, sh);
+    const uint64_t s3 = block_reduce_sum<256, uint64_t>(fcs, sh);
+    const uint64_t nr = block_reduce_sum<256, uint64_t>(nrows, sh);
+    if (threadIdx.x == 0) {
+        if (np) atomicAdd((unsigned long long*)&sc->paths, (unsigned long long)np);
+        if (s1) atomicAdd((unsigned long long*)&sc->fp_sum, (unsigned long long)s1);
+        if (s2) atomicAdd((unsigned long long*)&sc->fp_sum2, (unsigned long long)s2);
+        if (s3) atomicAdd((unsigned long long*)&sc->fp_csum, (unsigned long long)s3);
+        if (nr) atomicAdd((unsigned long long*)&sc->out_n, (unsigned long long)nr);
+    }
+}
+
+inline unsigned blocks256(uint64_t n) { return (unsigned)(n == 0 ? 1 : (n + 255) / 256); }
+
+}  // namespace
+
+namespace dim3h {
+
+uint32_t gc_rows_per_block() { return kGcRowsB; }
+
+void gc_masks(const uint32_t* rowptrR, int32_t* colR, uint32_t nx, const int2* S, uint64_t nS, const int32_t* slotS,
+              const int32_t* slotSz, const uint64_t* ex_y, const uint64_t* ex_z, int32_t zlo, const int32_t* hid,
+              uint32_t nz, uint64_t* hm, uint32_t* xflag, uint64_t* zm, uint32_t* zflag, uint64_t* dup,
+              hipStream_t st) {
+    fill_u64(zm, nz, 0, st);
+    if (nx)
+        hipLaunchKernelGGL(k_gc_row_masks, dim3(blocks256(nx)), dim3(256), 0, st, rowptrR, colR, nx, hid, hm, xflag,
+                           (unsigned long long*)dup);
+    if (nS)
+        hipLaunchKernelGGL(k_gc_col_masks, dim3(blocks256(nS)), dim3(256), 0, st, S, nS, slotS, slotSz, ex_y, ex_z, zlo,
+                           hid, zm, (unsigned long long*)dup);
+    if (nz) hipLaunchKernelGGL(k_gc_col_flags, dim3(blocks256(nz)), dim3(256), 0, st, zm, nz, zflag);
+}
+
+void gc_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_gc_list, dim3(blocks256(n)), dim3(256), 0, st, flag, pos, n, list);
+}
+
+uint32_t gc_cols_padded(uint32_t Nh) { return (Nh + kGcColsStep - 1) / kGcColsStep * kGcColsStep; }
+
+void gc_build(const uint32_t* rlist, uint32_t Mh, uint32_t Mp, const uint64_t* hm, const int32_t* id2x,
+              const uint32_t* clist, uint32_t Nh, const uint64_t* zm, const int32_t* id2z, int32_t zlo, int8_t* A8,
+              uint64_t* xk, int8_t* B8, uint64_t* gz, void* G, hipStream_t st) {
+    const uint32_t Np = gc_cols_padded(Nh);
+    if (Mp) hipLaunchKernelGGL(k_gc_build_rows, dim3(blocks256(Mp)), dim3(256), 0, st, rlist, Mh, Mp, hm, id2x, A8, xk);
+    if (Np) {
+        hipLaunchKernelGGL(k_gc_build_cols, dim3(blocks256(Np)), dim3(256), 0, st, clist, Nh, Np, zm, id2z, zlo, B8,
+                           gz);
+        const uint32_t nsteps = Np / kGcColsStep;
+        hipLaunchKernelGGL(k_gc_build_gchunks, dim3(blocks256(64ull * nsteps)), dim3(256), 0, st, gz, nsteps,
+                           (uint4*)G);
+    }
+}
+
+int gc_dense(const int8_t* A8, const uint64_t* xk, uint32_t Mp, const int8_t* B8, const void* G, uint32_t Np,
+             uint32_t cols, Scalars* sc, hipStream_t st) {
+    if (Mp == 0 || Np == 0) return 0;
+    if (Mp % kGcRowsB != 0 || Np % kGcColsStep != 0 || cols == 0 || cols % kGcColsStep != 0)
+        return (int)hipErrorInvalidValue;
+    const uint32_t n_xb = Mp / kGcRowsB;
+    const uint64_t n_zc = ((uint64_t)Np + cols - 1) / cols;
+    const uint64_t n_tasks = (uint64_t)n_xb * n_zc;
+    if (n_tasks >= (1ull << 31)) return (int)hipErrorInvalidValue;
+    const uint32_t per = (uint32_t)((n_tasks + 7) / 8);
+    hipLaunchKernelGGL(k_gc_dense, dim3(8u * per), dim3(256), 0, st, A8, xk, n_xb, B8, (const uint4*)G, Np, cols,
+                       (uint32_t)n_tasks, per, sc);
+    return (int)hipGetLastError();
+}
+
+int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowptrS, const int32_t* colS,
+             const uint64_t* hm, const uint64_t* zm, const int32_t* id2x, const int32_t* id2z, int32_t zlo, uint32_t nx,
+             uint32_t ntiles, uint32_t tile_size, uint32_t nz, int cu_count, Scalars* sc, hipStream_t st) {
+    if (nx == 0) return 0;
+    if ((uint64_t)nx * ntiles >= (1ull << 32) - kLChunk) return (int)hipErrorInvalidValue;
+    int occ = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_gc_light, 256, 0);
+    if (e != hipSuccess) return (int)e;
+    if (occ < 1) occ = 1;
+    uint64_t grid = (uint64_t)cu_count * (uint64_t)occ;
+    const uint64_t waves = ((uint64_t)nx * ntiles + kLChunk - 1) / kLChunk;
+    if (grid * 4 > waves) grid = (waves + 3) / 4;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(k_gc_light, dim3((unsigned)grid), dim3(256), 0, st, rowptrR, colR, rowptrS, colS, hm, zm, id2x,
+                       id2z, zlo, nx, ntiles, tile_size, nz, sc);
+    return (int)hipGetLastError();
+}
+
+}  // namespace dim3h

@@ -12,7 +12,8 @@ values_f32_*.npz  the same on float values with exact float sums (its
 medium.json   inputs regenerated from a named seeded generator (with a
               sha256 of the inputs to detect generator drift), and the count
               and fingerprint of the reference's output.
-large.json    the same for the config-scale cases (python make_golden.py large).
+large.json    the same for the config-scale cases (python make_golden.py large,
+              large_scale, aggregate_full).
 """
 import hashlib
 import json
@@ -182,6 +183,31 @@ def _slice_fp(E, spec, r):
     return fp.as_tuple()
 
 
+def aggregate_full():
+    """The whole 10M x 10M Zipf(1.2) COUNT (the aggregate_zslice case's
+    unsliced relations), added to large.json as "aggregate_full": its COUNT
+    fingerprint by oracle.fingerprint_bitset (count=True; pinned against the
+    SPA oracle's per-group counts in tests/test_oracle.py), 256 x slices over
+    forked workers."""
+    path = os.path.join(HERE, "large.json")
+    with open(path) as f:
+        out = json.load(f)
+    workers = min(8, os.cpu_count() or 1)
+    spec = dict(cases.LARGE["aggregate_zslice"])
+    spec.pop("mod")
+    spec.pop("r")
+    spec["checker"] = "oracle_bitset"
+    spec["heavy"] = 4096
+    R, S = cases.make_large_full(cases.LARGE["aggregate_zslice"])
+    rec = {"spec": spec, "sha256": sha(R, S), "n_r": int(R.shape[0]), "n_s": int(S.shape[0])}
+    fp, sw = oracle.fingerprint_bitset(R, S, heavy=spec["heavy"], slices=256, workers=workers, count=True)
+    rec["aggregate"] = {"n": int(fp.count), "fp": [str(v) for v in fp.as_tuple()], "swapped": sw}
+    print("aggregate_full", rec["aggregate"], flush=True)
+    out["aggregate_full"] = rec
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 def served():
     """Per-request fingerprints of the served stream (served.json), by the
     bitset oracle (COUNT requests: the COUNT fingerprint)."""
@@ -218,3 +244,5 @@ if __name__ == "__main__":
         large_scale()
     if "served" in which:
         served()
+    if "aggregate_full" in which:
+        aggregate_full()
