@@ -144,6 +144,11 @@ enum SlotId {
     WS_GC_XK,
     WS_GC_GZ,
     WS_GC_G,
+    WS_GC_ZINFO,
+    WS_GC_CNTL,
+    WS_GC_ROWL,
+    WS_GC_CURL,
+    WS_GC_COLL,
     WS_BATCH_ARGS,
     WS_BATCH_START,
     WS_BATCH_SCL,
@@ -837,6 +842,25 @@ int gc_prepare(dim3_ctx* c, const int32_t* slotS, const int32_t* slotSz, const u
     dim3h::gc_masks((const uint32_t*)W[WS_ROWPTR_R].p, (int32_t*)W[WS_COL_R].p, P.nx, P.S, P.nS, slotS,
                     P.z_ident ? nullptr : slotSz, exY, P.z_ident ? nullptr : exZ, P.zlo, hid, P.nz, hm, xflag, zm,
                     zflag, &sc->spare[3], st);
+    // the light S tuples by (z tile, key)
+    const uint64_t nkl = (uint64_t)P.ntiles * P.ny;
+    uint32_t *cntL, *rowL, *curL, *keyL, *zidL;
+    int32_t* colL;
+    TRY(ws(c, WS_GC_CNTL, nkl + 1, &cntL));
+    TRY(ws(c, WS_GC_ROWL, nkl + 1, &rowL));
+    TRY(ws(c, WS_GC_CURL, nkl, &curL));
+    TRY(ws(c, WS_KEY_S, P.nS, &keyL));
+    TRY(ws(c, WS_ZID_S, P.nS, &zidL));
+    TRY(ws(c, WS_GC_COLL, std::max<uint64_t>(P.nS, 1), &colL));
+    TRY(ws(c, WS_PARTIALS, dim3h::scan_partials_needed(std::max<uint64_t>(nkl, std::max<uint64_t>(P.nx, P.nz)) + 1),
+           &part));
+    dim3h::fill_u32(cntL, nkl + 1, 0, st);
+    dim3h::gc_light_csr_count(P.S, P.nS, slotS, P.z_ident ? nullptr : slotSz, exY, P.z_ident ? nullptr : exZ, P.zlo,
+                              hid, P.ny, P.tile_size, cntL, keyL, zidL, st);
+    dim3h::scan_u32(cntL, (uint32_t)nkl, rowL, part, &sc->tot_s, st);
+    dim3h::finish_rowptr(rowL, (uint32_t)nkl, &sc->tot_s, st);
+    HIPCHK(c, hipMemcpyAsync(curL, rowL, (size_t)nkl * 4, hipMemcpyDeviceToDevice, st));
+    dim3h::gc_light_csr_scatter(keyL, zidL, P.nS, curL, colL, st);
     dim3h::scan_u32(xflag, P.nx, xpos, part, &sc->nxh, st);
     dim3h::gc_list(xflag, xpos, P.nx, rlist, st);
     dim3h::scan_u32(zflag, P.nz, zpos, part, &sc->nzh, st);
@@ -863,6 +887,9 @@ int gc_prepare(dim3_ctx* c, const int32_t* slotS, const int32_t* slotSz, const u
     TRY(ws(c, WS_GC_G, std::max<uint64_t>(Np * 2, 2), &G));  // 16 bytes per column
     dim3h::gc_build(rlist, P.gc_Mh, P.gc_Mp, hm, (const int32_t*)W[WS_ID2X].p, clist, P.gc_Nh, zm,
                     P.z_ident ? nullptr : (const int32_t*)W[WS_ID2Z].p, P.zlo, A8, xk, B8, gz, G, st);
+    uint64_t* zinfo;
+    TRY(ws(c, WS_GC_ZINFO, 2ull * P.nz, &zinfo));
+    dim3h::gc_zinfo(zm, P.nz, P.z_ident ? nullptr : (const int32_t*)W[WS_ID2Z].p, P.zlo, zinfo, st);
     LAUNCHCHK(c, "group-count operands");
     return DIM3_OK;
 }
@@ -1091,7 +1118,8 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     HIPCHK(c, hipEventRecord(c->ev[2], st));
     int32_t *id2x, *id2z = nullptr, *colR, *zidS, *colS, *x2z;
     uint32_t *rowR, *curR, *cntS, *rowS, *curS, *keyS;
-    const uint64_t nkeyS = (uint64_t)P.ny * P.ntiles;
+    // (the group-count plan builds its own light S CSR, tile-major)
+    const uint64_t nkeyS = P.gc ? 1 : (uint64_t)P.ny * P.ntiles;
     TRY(ws(c, WS_ID2X, P.nx, &id2x));
     if (!P.z_ident) TRY(ws(c, WS_ID2Z, (uint64_t)P.ntiles * P.tile_size, &id2z));
     TRY(ws(c, WS_ROWPTR_R, (uint64_t)P.nx + 1, &rowR));
@@ -1170,14 +1198,15 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
         TRY(ws(c, WS_CUR_SD, nkeyS, &curSD));
         dim3h::fill_u32(cntSD, nkeyS + 1, 0, st);
     }
-    dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, hid, zflag,
-                   cntS, cntSD, keyS, zidS, st);
-
-    dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
-    dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
-    if (P.core) dim3h::split_cursors(rowS, cntSD, (uint32_t)nkeyS, curS, curSD, st);
-    else HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
-    dim3h::scatter_s(keyS, zidS, nS, curS, curSD, colS, P.vS, valS, st);
+    if (!P.gc) {
+        dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, hid,
+                       zflag, cntS, cntSD, keyS, zidS, st);
+        dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
+        dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
+        if (P.core) dim3h::split_cursors(rowS, cntSD, (uint32_t)nkeyS, curS, curSD, st);
+        else HIPCHK(c, hipMemcpyAsync(curS, rowS, (size_t)nkeyS * 4, hipMemcpyDeviceToDevice, st));
+        dim3h::scatter_s(keyS, zidS, nS, curS, curSD, colS, P.vS, valS, st);
+    }
     if (o.flags & DIM3_OPT_EXCLUDE_EQUAL)
         dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
     LAUNCHCHK(c, "csr");
@@ -1378,11 +1407,9 @@ int expand(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap, hipStream_t st) {
         if (e != 0) return fail(c, DIM3_ERR_DEVICE, "group-count dense launch: %s", hipGetErrorString((hipError_t)e));
         HIPCHK(c, hipEventRecord(c->ev[7], st));
         e = dim3h::gc_light((const uint32_t*)W[WS_ROWPTR_R].p, (const int32_t*)W[WS_COL_R].p,
-                            (const uint32_t*)W[WS_ROWPTR_S].p, (const int32_t*)W[WS_COL_S].p,
-                            (const uint64_t*)W[WS_GC_HM].p,
-                            (const uint64_t*)W[WS_GC_ZM].p, (const int32_t*)W[WS_ID2X].p,
-                            P.z_ident ? nullptr : (const int32_t*)W[WS_ID2Z].p, P.zlo, P.nx, P.ntiles, P.tile_size,
-                            P.nz, c->grid_cus(), sc, st);
+                            (const uint32_t*)W[WS_GC_ROWL].p, (const int32_t*)W[WS_GC_COLL].p, P.ny,
+                            (const uint64_t*)W[WS_GC_HM].p, W[WS_GC_ZINFO].p, (const int32_t*)W[WS_ID2X].p, P.nx,
+                            P.ntiles, P.tile_size, P.nz, c->grid_cus(), sc, st);
         if (e != 0) return fail(c, DIM3_ERR_DEVICE, "group-count light launch: %s", hipGetErrorString((hipError_t)e));
         HIPCHK(c, hipEventRecord(c->ev[5], st));
         HIPCHK(c, hipMemcpyAsync(c->h_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));

@@ -665,7 +665,7 @@ int classical_join(const int2* R, uint64_t nR, const int2* S, uint64_t nS, bool 
 // gc_rows_per_block() and Np is gc_cols_padded(Nh) (zero rows and columns
 // past the box); then the dense part's terms (gc_dense, box columns per task
 // a multiple of 64) and the light part's terms with their corrections
-// (gc_light, over the tiled S CSR).  gc_masks sets *dup (zeroed by the
+// (gc_light, over the light S CSR by (z tile, key)).  gc_masks sets *dup (zeroed by the
 // caller) when a heavy tuple occurs twice, which a mask cannot count: the
 // plan is then dropped.  Both add to sc's out_n and fingerprint sums;
 // gc_light also counts the light paths in sc->paths and uses sc->task.
@@ -681,9 +681,19 @@ void gc_build(const uint32_t* rlist, uint32_t Mh, uint32_t Mp, const uint64_t* h
               uint64_t* xk, int8_t* B8, uint64_t* gz, void* G, hipStream_t st);
 int gc_dense(const int8_t* A8, const uint64_t* xk, uint32_t Mp, const int8_t* B8, const void* G, uint32_t Np,
              uint32_t cols, Scalars* sc, hipStream_t st);
-int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowptrS, const int32_t* colS,
-             const uint64_t* hm, const uint64_t* zm, const int32_t* id2x, const int32_t* id2z, int32_t zlo, uint32_t nx,
-             uint32_t ntiles, uint32_t tile_size, uint32_t nz, int cu_count, Scalars* sc, hipStream_t st);
+// zinfo [nz] x 16 bytes: (zm, g) of every z ID, read by gc_light
+void gc_zinfo(const uint64_t* zm, uint32_t nz, const int32_t* id2z, int32_t zlo, void* zinfo, hipStream_t st);
+// the light S tuples (joined, not heavy) by (z tile, key): keys t * ny + y
+// counted into cnt (zeroed by the caller), then, with cursor = the
+// exclusive scan of cnt, their z IDs placed into colL
+void gc_light_csr_count(const int2* S, uint64_t nS, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_y,
+                        const uint64_t* ex_z, int32_t zlo, const int32_t* hid, uint32_t ny, uint32_t tile_size,
+                        uint32_t* cnt, uint32_t* keyL, uint32_t* zidL, hipStream_t st);
+void gc_light_csr_scatter(const uint32_t* keyL, const uint32_t* zidL, uint64_t nS, uint32_t* cursor, int32_t* colL,
+                          hipStream_t st);
+int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowL, const int32_t* colL, uint32_t ny,
+             const uint64_t* hm, const void* zinfo, const int32_t* id2x, uint32_t nx, uint32_t ntiles,
+             uint32_t tile_size, uint32_t nz, int cu_count, Scalars* sc, hipStream_t st);
 
 // Batched expansion: one persistent launch over the tasks of n_inst
 // instances (args[i], tasks task_start[i] .. task_start[i + 1]).

@@ -51,6 +51,9 @@ constexpr uint32_t kGcRowsW = 64;                 // box rows (x) per wavefront:
 constexpr uint32_t kGcRowsB = 4 * kGcRowsW;       // box rows per workgroup (4 wavefronts)
 constexpr uint32_t kGcColsStep = 64;              // box columns (z) per step: the reduction's k
 constexpr uint32_t kGcK = 64;                     // heavy keys (bytes per operand row)
+// word 3 of a raw (untyped, unswizzled) buffer descriptor on CDNA: 32-bit
+// data format, loads past num_records return 0
+constexpr int kRawBufferWord3 = 0x00020000;
 
 __device__ __forceinline__ uint32_t key_chunk(uint64_t k, uint32_t b) {
     return b < 9u ? (uint32_t)(k >> (7u * b)) & 127u : b == 9u ? (uint32_t)(k >> 63) : b == 10u ? 1u : 0u;
@@ -100,6 +103,51 @@ __global__ void k_gc_col_masks(const int2* __restrict__ S, uint64_t nS, const in
 __global__ void k_gc_col_flags(const uint64_t* __restrict__ zm, uint32_t nz, uint32_t* __restrict__ flag) {
     const uint32_t z = blockIdx.x * 256u + threadIdx.x;
     if (z < nz) flag[z] = zm[z] != 0ull ? 1u : 0u;
+}
+
+// The light S tuples (joined key, not heavy) by (z tile, key): key
+// t * ny + y of tuple i in keyL[i] (~0u for the others), its z ID in zidL[i],
+// counted in cnt; then placed by cursors (a row is a set: no order kept).
+// Tile-major rows keep one tile's rows, columns and z info together, for the
+// light tasks that run tile by tile.
+__global__ void k_gc_light_count(const int2* __restrict__ S, uint64_t nS, const int32_t* __restrict__ slotS,
+                                 const int32_t* __restrict__ slotSz, const uint64_t* __restrict__ ex_y,
+                                 const uint64_t* __restrict__ ex_z, int32_t zlo, const int32_t* __restrict__ hid,
+                                 uint32_t ny, uint32_t tile_size, uint32_t* __restrict__ cnt,
+                                 uint32_t* __restrict__ keyL, uint32_t* __restrict__ zidL) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= nS) return;
+    uint32_t key = 0xFFFFFFFFu;
+    const int32_t s = slotS[i];
+    if (s >= 0) {
+        const uint32_t y = (uint32_t)(ex_y[s] >> 32);
+        if (hid[y] < 0) {
+            const uint32_t zid = ex_z != nullptr ? (uint32_t)(ex_z[slotSz[i]] >> 32) : (uint32_t)S[i].x - (uint32_t)zlo;
+            key = (zid / tile_size) * ny + y;
+            zidL[i] = zid;
+            atomicAdd(&cnt[key], 1u);
+        }
+    }
+    keyL[i] = key;
+}
+
+__global__ void k_gc_light_scatter(const uint32_t* __restrict__ keyL, const uint32_t* __restrict__ zidL, uint64_t nS,
+                                   uint32_t* __restrict__ cursor, int32_t* __restrict__ colL) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= nS) return;
+    const uint32_t key = keyL[i];
+    if (key == 0xFFFFFFFFu) return;
+    colL[atomicAdd(&cursor[key], 1u)] = (int32_t)zidL[i];
+}
+
+// zinfo[z] = (zm[z], g(z value)) for every z ID: one 16-byte read per light group
+__global__ void k_gc_zinfo(const uint64_t* __restrict__ zm, uint32_t nz, const int32_t* __restrict__ id2z, int32_t zlo,
+                           uint4* __restrict__ zinfo) {
+    const uint32_t z = blockIdx.x * 256u + threadIdx.x;
+    if (z >= nz) return;
+    const uint64_t m = zm[z];
+    const uint64_t g = zfactor(id2z != nullptr ? id2z[z] : (int32_t)((uint32_t)zlo + z));
+    zinfo[z] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), (uint32_t)g, (uint32_t)(g >> 32));
 }
 
 // compact list of the flagged IDs (pos: exclusive scan of flag)
@@ -218,13 +266,21 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
     }
     const uint32_t z0 = zc * cols;
     const uint32_t z1 = min(z0 + cols, Np);
-    // column operands of a step: tile u, lane (li, q) = column zs + 16 u + li, keys 16 q ..
+    // column operands of a step: tile u, lane (li, q) = column zs + 16 u + li,
+    // keys 16 q ..  Buffer loads over the chunk's columns: a scalar step
+    // offset plus a lane offset fixed for the task, no per-load address math.
+    const __amdgpu_buffer_rsrc_t rb8 = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(B8 + (size_t)z0 * kGcK), (short)0, (int)((z1 - z0) * kGcK), kRawBufferWord3);
+    const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(G + (size_t)(z0 / kGcColsStep) * 64u), (short)0, (int)((z1 - z0) / kGcColsStep * 1024u),
+        kRawBufferWord3);
+    const int vb = (int)(li * kGcK + 16u * q), vg = (int)(lane * 16u);
     auto load_step = [&](uint32_t zs, v4i* zf, v4i& gf) {
+        const int so = (int)((zs - z0) * kGcK);
 #pragma unroll
         for (uint32_t u = 0; u < 4; u++)
-            zf[u] = *reinterpret_cast<const v4i*>(B8 + (size_t)(zs + 16u * u + li) * kGcK + 16u * q);
-        const uint4 g4 = G[(size_t)(zs / kGcColsStep) * 64u + lane];
-        gf = v4i{(int)g4.x, (int)g4.y, (int)g4.z, (int)g4.w};
+            zf[u] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rb8, vb + (int)(u * 16u * kGcK), so, 0));
+        gf = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rg, vg, (int)((zs - z0) / kGcColsStep * 1024u), 0));
     };
     auto compute = [&](const v4i* zf, const v4i& gf) {
 #pragma unroll
@@ -251,22 +307,25 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
             e1[gx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gf, fl, e1[gx], 0, 0, 0);
         }
     };
-    // two operand buffers in turn: the next step's loads are issued before
-    // the current step's MFMAs (the address clamped to the last step, so the
-    // loads are unconditional; a scheduling barrier keeps the compiler from
-    // sinking them to their use)
+    // two operand buffers in turn: a buffer is refilled (two steps ahead)
+    // right after its step's MFMAs, so each load has the other buffer's step
+    // to arrive (the address clamped to the last step, so the loads are
+    // unconditional; scheduling barriers keep the compiler from moving them)
     const uint32_t zlast = z1 - kGcColsStep;
     v4i zfa[4], gfa, zfb[4], gfb;
     load_step(z0, zfa, gfa);
+    load_step(min(z0 + kGcColsStep, zlast), zfb, gfb);
     for (uint32_t zs = z0;; zs += 2u * kGcColsStep) {
-        load_step(min(zs + kGcColsStep, zlast), zfb, gfb);
         __builtin_amdgcn_sched_barrier(0);
         compute(zfa, gfa);
         if (zs + kGcColsStep >= z1) break;
+        __builtin_amdgcn_sched_barrier(0);
         load_step(min(zs + 2u * kGcColsStep, zlast), zfa, gfa);
         __builtin_amdgcn_sched_barrier(0);
         compute(zfb, gfb);
         if (zs + 2u * kGcColsStep >= z1) break;
+        __builtin_amdgcn_sched_barrier(0);
+        load_step(min(zs + 3u * kGcColsStep, zlast), zfb, gfb);
     }
     // lane (li, q) of group gx: row x0 + 16 gx + li, chunk rows b = 4 q + e
     uint64_t S1 = 0, S2 = 0, CNT = 0, CS = 0;
@@ -311,7 +370,7 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
 
 // ---- light part: one (x, z-tile) task per wavefront ----
 // The task's light join paths (its R row's entries without the heavy flag,
-// each key's S segment in tile t of the tiled S CSR) are counted per z ID in
+// each key's row in tile t of the light S CSR) are counted per z ID in
 // a private LDS hash table of kLSlots slots.  A task of at most kLCap paths
 // runs in one pass.  A longer one runs in passes over z ranges whose width
 // is guessed from its path count (kLCap / 2 paths per pass if spread
@@ -353,25 +412,26 @@ __device__ __forceinline__ uint32_t light_slot(uint32_t* keys, uint32_t z) {
     }
 }
 
-// Segment (S start, path count) of R entry i of tile t: empty for a heavy entry.
-__device__ __forceinline__ void light_segment(const int32_t* colR, const uint32_t* rowptrS, uint32_t ntiles,
-                                              uint32_t t, uint32_t i, uint32_t& sb, uint32_t& len) {
+// Segment (light S start, path count) of R entry i in tile t: empty for a
+// heavy entry (bit 31; a heavy key has no light row either).
+__device__ __forceinline__ void light_segment(const int32_t* colR, const uint32_t* rowL, uint32_t ny, uint32_t t,
+                                              uint32_t i, uint32_t& sb, uint32_t& len) {
     sb = 0;
     len = 0;
     const int32_t craw = colR[i];
     if (craw >= 0) {
-        const uint32_t k = (uint32_t)craw * ntiles + t;
-        sb = rowptrS[k];
-        len = rowptrS[k + 1] - sb;
+        const uint64_t k = (uint64_t)t * ny + (uint32_t)craw;
+        sb = rowL[k];
+        len = rowL[k + 1] - sb;
     }
 }
 
 __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ rowptrR, const int32_t* __restrict__ colR,
-                                                  const uint32_t* __restrict__ rowptrS, const int32_t* __restrict__ colS,
-                                                  const uint64_t* __restrict__ hm, const uint64_t* __restrict__ zm,
-                                                  const int32_t* __restrict__ id2x, const int32_t* __restrict__ id2z,
-                                                  int32_t zlo, uint32_t nx, uint32_t ntiles, uint32_t tile_size,
-                                                  uint32_t nz, Scalars* sc) {
+                                                  const uint32_t* __restrict__ rowL, const int32_t* __restrict__ colL,
+                                                  uint32_t ny, const uint64_t* __restrict__ hm,
+                                                  const uint4* __restrict__ zinfo, const int32_t* __restrict__ id2x,
+                                                  uint32_t nx, uint32_t ntiles, uint32_t tile_size, uint32_t nz,
+                                                  Scalars* sc) {
     __shared__ LightLds wl[4];
     __shared__ uint64_t sh[5];
     const uint32_t lane = threadIdx.x & 63u;
@@ -392,15 +452,17 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
             if (task >= ntask) break;
             task_end = min(task + kLChunk, ntask);
         }
+        // tile-major: the waves running at one time share a tile's light rows
+        // and z info in L2
         const uint32_t ti = task++;
-        const uint32_t x = ti / ntiles, t = ti - x * ntiles;
+        const uint32_t t = ti / nx, x = ti - t * nx;
         const uint32_t rb = rowptrR[x], re = rowptrR[x + 1];
         const bool short_row = re - rb <= 64u;  // wave-uniform
         // the task's light paths; a short row's segment table stays in LDS
         uint32_t P = 0;
         if (short_row) {
             uint32_t sb = 0, len = 0;
-            if (rb + lane < re) light_segment(colR, rowptrS, ntiles, t, rb + lane, sb, len);
+            if (rb + lane < re) light_segment(colR, rowL, ny, t, rb + lane, sb, len);
             const uint32_t inc = wave_scan_u32(len);
             P = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
             W.sb[lane] = sb;
@@ -409,7 +471,7 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
         } else {
             for (uint32_t c0 = rb; c0 < re; c0 += 64u) {
                 uint32_t sb = 0, len = 0;
-                if (c0 + lane < re) light_segment(colR, rowptrS, ntiles, t, c0 + lane, sb, len);
+                if (c0 + lane < re) light_segment(colR, rowL, ny, t, c0 + lane, sb, len);
                 P += wave_sum_u32(len);
             }
         }
@@ -430,7 +492,7 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
                 uint32_t PC = P;
                 if (!short_row) {  // this chunk's segment table
                     uint32_t sb = 0, len = 0;
-                    if (c0 + lane < re) light_segment(colR, rowptrS, ntiles, t, c0 + lane, sb, len);
+                    if (c0 + lane < re) light_segment(colR, rowL, ny, t, c0 + lane, sb, len);
                     const uint32_t inc = wave_scan_u32(len);
                     PC = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
                     W.sb[lane] = sb;
@@ -449,7 +511,7 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
 #pragma unroll
                     for (uint32_t step = 32; step > 0; step >>= 1)
                         if (W.pre[seg + step] <= p) seg += step;
-                    const uint32_t z = in ? (uint32_t)colS[W.sb[seg] + (p - W.pre[seg])] : 0u;
+                    const uint32_t z = in ? (uint32_t)colL[W.sb[seg] + (p - W.pre[seg])] : 0u;
                     const bool ok = in && z >= zcur && z < zhi;
                     uint32_t old = 1u, slot = 0;
                     if (ok) {
@@ -478,9 +540,9 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
             for (uint32_t j = lane; j < nt; j += 64u) {
                 const uint32_t s = W.touched[j];
                 const uint32_t cl = W.cnt[s];
-                const uint32_t zid = W.keys[s];
-                const uint64_t g = zfactor(id2z != nullptr ? id2z[zid] : (int32_t)((uint32_t)zlo + zid));
-                if ((hmx & zm[zid]) == 0ull) {  // a group of light paths only
+                const uint4 zi = zinfo[W.keys[s]];
+                const uint64_t zmz = ((uint64_t)zi.y << 32) | zi.x, g = ((uint64_t)zi.w << 32) | zi.z;
+                if ((hmx & zmz) == 0ull) {  // a group of light paths only
                     S1 += g;
                     T += 1u;
                 }
@@ -532,6 +594,10 @@ void gc_masks(const uint32_t* rowptrR, int32_t* colR, uint32_t nx, const int2* S
     if (nz) hipLaunchKernelGGL(k_gc_col_flags, dim3(blocks256(nz)), dim3(256), 0, st, zm, nz, zflag);
 }
 
+void gc_zinfo(const uint64_t* zm, uint32_t nz, const int32_t* id2z, int32_t zlo, void* zinfo, hipStream_t st) {
+    if (nz) hipLaunchKernelGGL(k_gc_zinfo, dim3(blocks256(nz)), dim3(256), 0, st, zm, nz, id2z, zlo, (uint4*)zinfo);
+}
+
 void gc_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_gc_list, dim3(blocks256(n)), dim3(256), 0, st, flag, pos, n, list);
 }
@@ -567,9 +633,22 @@ int gc_dense(const int8_t* A8, const uint64_t* xk, uint32_t Mp, const int8_t* B8
     return (int)hipGetLastError();
 }
 
-int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowptrS, const int32_t* colS,
-             const uint64_t* hm, const uint64_t* zm, const int32_t* id2x, const int32_t* id2z, int32_t zlo, uint32_t nx,
-             uint32_t ntiles, uint32_t tile_size, uint32_t nz, int cu_count, Scalars* sc, hipStream_t st) {
+void gc_light_csr_count(const int2* S, uint64_t nS, const int32_t* slotS, const int32_t* slotSz, const uint64_t* ex_y,
+                        const uint64_t* ex_z, int32_t zlo, const int32_t* hid, uint32_t ny, uint32_t tile_size,
+                        uint32_t* cnt, uint32_t* keyL, uint32_t* zidL, hipStream_t st) {
+    if (nS)
+        hipLaunchKernelGGL(k_gc_light_count, dim3(blocks256(nS)), dim3(256), 0, st, S, nS, slotS, slotSz, ex_y, ex_z,
+                           zlo, hid, ny, tile_size, cnt, keyL, zidL);
+}
+
+void gc_light_csr_scatter(const uint32_t* keyL, const uint32_t* zidL, uint64_t nS, uint32_t* cursor, int32_t* colL,
+                          hipStream_t st) {
+    if (nS) hipLaunchKernelGGL(k_gc_light_scatter, dim3(blocks256(nS)), dim3(256), 0, st, keyL, zidL, nS, cursor, colL);
+}
+
+int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowL, const int32_t* colL, uint32_t ny,
+             const uint64_t* hm, const void* zinfo, const int32_t* id2x, uint32_t nx, uint32_t ntiles,
+             uint32_t tile_size, uint32_t nz, int cu_count, Scalars* sc, hipStream_t st) {
     if (nx == 0) return 0;
     if ((uint64_t)nx * ntiles >= (1ull << 32) - kLChunk) return (int)hipErrorInvalidValue;
     int occ = 0;
@@ -580,8 +659,8 @@ int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowpt
     const uint64_t waves = ((uint64_t)nx * ntiles + kLChunk - 1) / kLChunk;
     if (grid * 4 > waves) grid = (waves + 3) / 4;
     if (grid == 0) grid = 1;
-    hipLaunchKernelGGL(k_gc_light, dim3((unsigned)grid), dim3(256), 0, st, rowptrR, colR, rowptrS, colS, hm, zm, id2x,
-                       id2z, zlo, nx, ntiles, tile_size, nz, sc);
+    hipLaunchKernelGGL(k_gc_light, dim3((unsigned)grid), dim3(256), 0, st, rowptrR, colR, rowL, colL, ny, hm,
+                       (const uint4*)zinfo, id2x, nx, ntiles, tile_size, nz, sc);
     return (int)hipGetLastError();
 }
 

@@ -166,6 +166,28 @@ def test_aggregate_zslice(ctx, dim3, torch):
     assert device_fingerprint(torch, rows[:, 0:2], counts) == want
 
 
+def test_aggregate_full_group_counts(ctx, dim3, torch):
+    """The whole 10M x 10M COUNT with every group's count formed (MFMA box
+    over the heavy keys, light (x, z-tile) tasks) against the bitset oracle's
+    fingerprint of all 2.36e12 groups, and against the path-sum shortcut."""
+    rec = _large()["aggregate_full"]
+    want = tuple(int(v) for v in rec["aggregate"]["fp"])
+    R, S = cases.make_large_full(rec["spec"])
+    assert sha(R, S) == rec["sha256"], "generator drift"
+    dR = torch.from_numpy(R).cuda()
+    dS = torch.from_numpy(S).cuda()
+    torch.cuda.synchronize()
+    say("aggregate full, group counts", R.shape, S.shape)
+    rc, g = ctx.join_aggregate_count_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, output_mode=1)
+    assert rc == dim3.OK
+    say("stats", g.stats.as_dict())
+    assert g.stats.dense_mode == dim3.DENSE_MFMA and g.stats.paths_sparse > 0
+    assert g.fp.as_tuple() == want
+    rc, p = ctx.join_aggregate_count_dev(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, output_mode=1,
+                                         path_sums=True)
+    assert rc == dim3.OK and p.fp.as_tuple() == want
+
+
 def test_aggregate_full_slices(ctx, dim3, torch):
     rec, R, S, _, _ = _inputs("aggregate_zslice")
     dR = torch.from_numpy(R).cuda()
