@@ -30,6 +30,7 @@ import numpy as np  # noqa: E402
 import __graft_entry__ as GE  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
 METRIC = "distinct output pairs/s + %HBM roofline; served p50/p99 latency vs plain GPU"
 
 
@@ -425,10 +426,39 @@ def config_bench(args, d, torch):
                 cpu = cpu_baseline(R, args.cpu_seconds, 64, S=S, self_filter=False)
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %r" % (e,))
+    group_counts = None
     if agg:
         config = {"workload": "join-aggregate COUNT, Zipf(1.2) keys", "tuples_per_side": int(R.shape[0]),
                   "a_c_domain": 2_000_000, "b_domain": 1_000_000, "zipf_s": 1.2, "output": "fingerprint",
-                  "groups": int(n_out), "join_paths": int(st.out_j), "parallelism": "single GPU"}
+                  "groups": int(n_out), "join_paths": int(st.out_j), "parallelism": "single GPU",
+                  "counts": "every group's count formed" if st.dense_mode == 2 else "counters per group"}
+        # the same query from join-path sums (no group's count formed), for comparison
+        kwp = dict(kw, path_sums=True)
+        fn(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, **kwp)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for _ in range(args.steps):
+            rc, rp = fn(dR.data_ptr(), R.shape[0], dS.data_ptr(), S.shape[0], 0, 0, **kwp)
+        torch.cuda.synchronize()
+        tp = (time.perf_counter() - tp) / args.steps
+        if rp.fp.as_tuple() != res.fp.as_tuple():
+            raise RuntimeError("group counts and path sums disagree")
+        group_counts = {"path_sums_ms_per_query": 1000.0 * tp, "light_paths": int(st.paths_sparse)}
+        if st.dense_mode == 2:
+            # the MFMA box: executed int8 ops (16 product + 8 reduction MFMAs
+            # of 16x16x64 per 64 x 64 cells: 192 ops per cell, rows padded to
+            # 256 and columns to 64)
+            mp = (int(st.nx_heavy) + 255) // 256 * 256
+            np_ = (int(st.nz_heavy) + 63) // 64 * 64
+            box_ms = float(np.mean([r.stats.ms_dense for r in ress]))
+            ops = 192.0 * mp * np_
+            tops = ops / (box_ms * 1e-3) / 1e12
+            group_counts.update({"box_cells": int(st.nx_heavy) * int(st.nz_heavy), "box_kernel_ms": box_ms,
+                                 "box_int8_ops": ops, "light_kernel_ms": exp_ms - box_ms})
+            roof = {"bound": "mfma", "achieved": tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)",
+                    "frac": tops / INT8_PEAK_TOPS, "traffic": None,
+                    "kernel": "k_gc_dense: heavy keys' counts (int8 0/1 product) and their fingerprint "
+                              "reduction on the MFMA cores", "kernel_ms": box_ms, "int8_ops": ops}
     else:
         config = {"workload": "join-project, Zipf(1.0) keys", "tuples_per_side": int(R.shape[0]),
                   "a_c_domain": 100_000, "b_domain": 10_000, "zipf_s": 1.0,
@@ -448,11 +478,13 @@ def config_bench(args, d, torch):
         "dtype": "int32",
         "data": "synthetic (seeded Zipf relations generated on the host, resident in HBM before timing)",
         "config": config,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": roof if agg and group_counts and "box_cells" in group_counts else
+                    {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_expand", "kernel_ms": exp_ms,
                      "alg_bytes": b_kernel,
-                     "limiter": ("vector ALU: the z factors of every group's fingerprint (~2.4e12 groups), "
-                                 "not HBM") if mode == "fingerprint" else "HBM: output rows"},
+                     "limiter": ("vector ALU: the z factors of every group's fingerprint, not HBM")
+                     if mode == "fingerprint" else "HBM: output rows"},
+        "group_counts": group_counts,
         "cpu_baseline": cpu,
         "dense": {"mode": {0: "off", 1: "bitor", 2: "mfma"}[int(st.dense_mode)], "heavy_keys": int(st.heavy_keys),
                   "rows": int(st.nx_heavy), "cols": int(st.nz_heavy),
