@@ -802,6 +802,10 @@ void plan_gc(dim3_ctx* c, const dim3_opts& o, int force_wgs) {
                         (double)P.nx * (double)nt * kGcTaskS + 1e-4;
     const double t_sparse = (double)P.out_j * kCountPathS;
     if (o.dense_mode != DIM3_DENSE_MFMA && t_gc >= 0.5 * t_sparse) return;
+    // forced MFMA without a key count: the box only when its keys carry most
+    // paths (a block of many equally heavy keys, which 64 keys cannot hold,
+    // goes to the materialised MFMA product instead)
+    if (o.dense_mode == DIM3_DENSE_MFMA && o.heavy_k == 0 && 2 * paths < P.out_j) return;
     P.gc = true;
     P.gc_bmin = bmin;
     P.gc_K = (uint32_t)K;
