@@ -283,8 +283,12 @@ struct dim3_ctx {
     std::unique_ptr<Bank> nab;
     // the batch launch's instantiated graph, replayed while its shape and
     // buffers are those of the call (captured again when they change)
-    hipGraphExec_t batch_exec = nullptr;
-    uint64_t batch_key[12] = {};
+    hipGraphExec_t batch_exec[2] = {nullptr, nullptr};  // chunk slots of a pipelined batch
+    uint64_t batch_key[2][12] = {};
+    // a pipelined batch's first chunk runs on this stream, masked to pipe_cus
+    // compute units (created on first use)
+    hipStream_t pipe_stream = nullptr;
+    int pipe_cus = 0;
     uint8_t* arena = nullptr;
     size_t arena_used = 0, arena_size = 0;
 
@@ -2031,6 +2035,30 @@ int run_stream(dim3_ctx* c, bool count_mode, const int32_t* R, uint64_t nR, cons
 // bank, then all of them expand in one persistent launch fed by one device
 // work counter.  The launch (scalar reset, expansion, scalar gather and the
 // copy to the host) is captured as a HIP graph and replayed once.
+// A batch of at least kPipeMin instances is planned in two chunks, largest
+// instances first: chunk A holds the largest instances up to half of the
+// batch's tuples, chunk B the rest.  Chunk A's expansion (most of the
+// batch's join paths) runs on a stream masked to kPipeShare of the compute
+// units while the host plans chunk B on the call's stream, whose small
+// kernels take the other units; chunk B's expansion then runs on the call's
+// stream beside chunk A's tail.  (Only on the context's own stream: a batch
+// on a caller's stream, a server worker's CU-masked one say, is one chunk.)
+constexpr uint32_t kPipeMin = 16;
+constexpr double kPipeShare = 0.875;
+
+int pipe_stream(dim3_ctx* c, hipStream_t* out) {
+    if (!c->pipe_stream) {
+        const uint32_t cus = (uint32_t)std::max(2, c->cu_count);
+        const uint32_t use = std::min<uint32_t>(cus - 1, std::max<uint32_t>(1, (uint32_t)(kPipeShare * cus + 0.5)));
+        std::vector<uint32_t> mask((cus + 31) / 32, 0);
+        for (uint32_t i = 0; i < use; i++) mask[i / 32] |= 1u << (i % 32);
+        HIPCHK(c, hipExtStreamCreateWithCUMask(&c->pipe_stream, (uint32_t)mask.size(), mask.data()));
+        c->pipe_cus = (int)use;
+    }
+    *out = c->pipe_stream;
+    return DIM3_OK;
+}
+
 int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
               void* stream, dim3_result* results) {
     if (!c || (n && (!inst || !results))) return DIM3_ERR_INVALID;
@@ -2062,43 +2090,58 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
     ob.block = block;
     const int wgs = block == 256 ? 4 : block == 512 ? 2 : 1;
     std::vector<dim3h::ExpandArgs> args(n);
-    std::vector<uint32_t> start(n + 1, 0);
     std::vector<Scalars*> scl(n);
-    size_t lds = 0;
-    uint64_t total = 0;
-    // each planning stage over every instance, then one synchronisation
-    for (uint32_t i = 0; i < n; i++) {
-        c->use_bank(i + 1);
-        TRY(prepare_start(c, count_mode, (const int2*)inst[i].R_xy, inst[i].n_r, (const int2*)inst[i].S_zy,
-                          inst[i].n_s, ob, st));
-    }
-    HIPCHK(c, hipStreamSynchronize(st));
-    for (uint32_t i = 0; i < n; i++) {
-        c->use_bank(i + 1);
-        TRY(prepare_map(c, ob, st));
-    }
-    HIPCHK(c, hipStreamSynchronize(st));
-    for (uint32_t i = 0; i < n; i++) {
-        c->use_bank(i + 1);
-        TRY(prepare_plan(c, ob, st, wgs));
-        const auto& P = c->bank().plan;
-        args[i] = make_args(c, fp, inst[i].out, inst[i].cap);
-        scl[i] = args[i].sc;
-        start[i] = (uint32_t)total;
-        if (!P.empty) {
-            total += (uint64_t)P.nx * P.ntiles;
-            lds = std::max(lds, dim3h::expand_lds_bytes(P.tile_words, P.touched_cap, P.stage_cap, (uint32_t)block, fp));
+    // the instances largest first, and the chunk split
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return inst[a].n_r + inst[a].n_s > inst[b].n_r + inst[b].n_s;
+    });
+    uint64_t tuples = 0, acc = 0;
+    for (uint32_t i = 0; i < n; i++) tuples += inst[i].n_r + inst[i].n_s;
+    uint32_t na = n;
+    if (n >= kPipeMin && stream == nullptr) {  // a caller's stream keeps one chunk (its CU mask is the caller's)
+        na = 0;
+        while (na < n && 2 * acc < tuples) {
+            acc += inst[order[na]].n_r + inst[order[na]].n_s;
+            na++;
         }
-        if (total >= (1ull << 32)) return fail(c, DIM3_ERR_INVALID, "batch holds more than 2^32 row tasks");
+        na = std::max<uint32_t>(1, std::min<uint32_t>(na, n - 1));
     }
-    start[n] = (uint32_t)total;
-    c->use_bank(0);
+    // each planning stage over every instance of a chunk, then one synchronisation
+    auto plan_chunk = [&](uint32_t first, uint32_t last) -> int {
+        for (uint32_t k = first; k < last; k++) {
+            const uint32_t i = order[k];
+            c->use_bank(i + 1);
+            TRY(prepare_start(c, count_mode, (const int2*)inst[i].R_xy, inst[i].n_r, (const int2*)inst[i].S_zy,
+                              inst[i].n_s, ob, st));
+        }
+        HIPCHK(c, hipStreamSynchronize(st));
+        for (uint32_t k = first; k < last; k++) {
+            c->use_bank(order[k] + 1);
+            TRY(prepare_map(c, ob, st));
+        }
+        HIPCHK(c, hipStreamSynchronize(st));
+        for (uint32_t k = first; k < last; k++) {
+            const uint32_t i = order[k];
+            c->use_bank(i + 1);
+            TRY(prepare_plan(c, ob, st, wgs));
+            args[k] = make_args(c, fp, inst[i].out, inst[i].cap);
+            scl[k] = args[k].sc;
+        }
+        c->use_bank(0);
+        return DIM3_OK;
+    };
+    // the expansion kernel of every instance (a COUNT fingerprint from path
+    // sums expands as a join-project)
+    const bool kcount =
+        count_mode && !(fp && (o.flags & DIM3_OPT_PATH_SUMS) && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL));
     dim3h::ExpandArgs* d_args;
     uint32_t* d_start;
     Scalars** d_scl;
     Scalars* d_scout;
     TRY(ws(c, WS_BATCH_ARGS, n, &d_args));
-    TRY(ws(c, WS_BATCH_START, (uint64_t)n + 2, &d_start));  // + the work counter
+    TRY(ws(c, WS_BATCH_START, (uint64_t)n + 4, &d_start));  // per chunk: its starts and a work counter
     TRY(ws(c, WS_BATCH_SCL, n, &d_scl));
     TRY(ws(c, WS_BATCH_SCOUT, n, &d_scout));
     if (c->h_batch_cap < n) {
@@ -2109,67 +2152,115 @@ int run_batch(dim3_ctx* c, bool count_mode, const dim3_instance* inst, uint32_t 
         HIPCHK(c, dim3h::host_malloc((void**)&c->h_batch, sizeof(Scalars) * cap));
         c->h_batch_cap = cap;
     }
-    HIPCHK(c, hipMemcpyAsync(d_args, args.data(), sizeof(dim3h::ExpandArgs) * n, hipMemcpyHostToDevice, st));
-    HIPCHK(c, hipMemcpyAsync(d_start, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
-    HIPCHK(c, hipMemcpyAsync(d_scl, scl.data(), sizeof(Scalars*) * n, hipMemcpyHostToDevice, st));
-    HIPCHK(c, hipStreamSynchronize(st));  // host vectors are pageable
-    if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block, fp);
-    unsigned grid = 1;
-    // the expansion kernel of every instance (a COUNT fingerprint from path
-    // sums expands as a join-project)
-    const bool kcount =
-        count_mode && !(fp && (o.flags & DIM3_OPT_PATH_SUMS) && !(o.flags & DIM3_OPT_EXCLUDE_EQUAL));
-    int e = dim3h::expand_batch_setup(kcount, block, lds, c->grid_cus(), std::max<uint64_t>(total, 1), &grid);
-    if (e != 0) return fail(c, DIM3_ERR_DEVICE, "batch setup: %s", hipGetErrorString((hipError_t)e));
+    // Launch of one chunk (entries [first, last) of order) on stream sx with
+    // cus compute units: a HIP graph (scalar reset, expansion, scalar
+    // gather, copy to the host).  Its kernels read every instance's
+    // arguments from device tables, so the instantiated graph (one per
+    // chunk slot) is replayed by later batches whose shape (kernel,
+    // workgroup, LDS, grid, instance count) and table and host buffers are
+    // the same; otherwise it is captured again.
+    auto launch_chunk = [&](uint32_t first, uint32_t last, hipStream_t sx, int cus, int slot) -> int {
+        const uint32_t m = last - first;
+        std::vector<uint32_t> start(m + 1, 0);
+        size_t lds = 0;
+        uint64_t total = 0;
+        for (uint32_t k = first; k < last; k++) {
+            const auto& P = c->banks[order[k] + 1]->plan;
+            start[k - first] = (uint32_t)total;
+            if (!P.empty) {
+                total += (uint64_t)P.nx * P.ntiles;
+                lds = std::max(lds, dim3h::expand_lds_bytes(P.tile_words, P.touched_cap, P.stage_cap, (uint32_t)block, fp));
+            }
+            if (total >= (1ull << 32)) return fail(c, DIM3_ERR_INVALID, "batch holds more than 2^32 row tasks");
+        }
+        start[m] = (uint32_t)total;
+        dim3h::ExpandArgs* da = d_args + first;
+        uint32_t* ds = d_start + first + 2u * (uint32_t)slot;  // m + 1 starts, then the counter
+        Scalars** dl = d_scl + first;
+        Scalars* dout = d_scout + first;
+        Scalars* hout = c->h_batch + first;
+        HIPCHK(c, hipMemcpyAsync(da, args.data() + first, sizeof(dim3h::ExpandArgs) * m, hipMemcpyHostToDevice, sx));
+        HIPCHK(c, hipMemcpyAsync(ds, start.data(), sizeof(uint32_t) * (m + 1), hipMemcpyHostToDevice, sx));
+        HIPCHK(c, hipMemcpyAsync(dl, scl.data() + first, sizeof(Scalars*) * m, hipMemcpyHostToDevice, sx));
+        HIPCHK(c, hipStreamSynchronize(sx));  // host vectors are pageable
+        if (lds == 0) lds = dim3h::expand_lds_bytes(1, 64, 3u * block + 1u, (uint32_t)block, fp);
+        unsigned grid = 1;
+        int e = dim3h::expand_batch_setup(kcount, block, lds, cus, std::max<uint64_t>(total, 1), &grid);
+        if (e != 0) return fail(c, DIM3_ERR_DEVICE, "batch setup: %s", hipGetErrorString((hipError_t)e));
+        const uint64_t key[12] = {(uint64_t)kcount, (uint64_t)fp, (uint64_t)block, (uint64_t)lds, (uint64_t)grid,
+                                  (uint64_t)m, (uint64_t)(uintptr_t)da, (uint64_t)(uintptr_t)ds,
+                                  (uint64_t)(uintptr_t)dl, (uint64_t)(uintptr_t)dout, (uint64_t)(uintptr_t)hout,
+                                  (uint64_t)(uintptr_t)sx};
+        hipGraphExec_t& ex = c->batch_exec[slot];
+        if (ex && memcmp(key, c->batch_key[slot], sizeof(key)) != 0) {
+            hipGraphExecDestroy(ex);
+            ex = nullptr;
+        }
+        if (!ex) {
+            hipGraph_t graph = nullptr;
+            {
+                // no allocation anywhere in the process while the capture is open
+                std::unique_lock<std::shared_mutex> capture_lock(dim3h::capture_mutex());
+                HIPCHK(c, hipStreamBeginCapture(sx, hipStreamCaptureModeThreadLocal));
+                dim3h::expand_batch_launch(kcount, fp, block, lds, grid, da, ds, m, ds + m + 1, dl, dout, sx);
+                hipMemcpyAsync(hout, dout, sizeof(Scalars) * m, hipMemcpyDeviceToHost, sx);
+                HIPCHK(c, hipStreamEndCapture(sx, &graph));
+            }
+            const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                ex = nullptr;
+                return fail(c, DIM3_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+            }
+            memcpy(c->batch_key[slot], key, sizeof(key));
+        }
+        const hipError_t ge = hipGraphLaunch(ex, sx);
+        if (ge != hipSuccess) {
+            hipGraphExecDestroy(ex);
+            ex = nullptr;
+            return fail(c, DIM3_ERR_DEVICE, "batch launch: %s", hipGetErrorString(ge));
+        }
+        return DIM3_OK;
+    };
 
-    // The launch is a HIP graph (scalar reset, expansion, scalar gather, copy
-    // to the host).  Its kernels read every instance's arguments from device
-    // tables, so the instantiated graph is replayed by later batches whose
-    // shape (kernel, workgroup, LDS, grid, instance count) and table and
-    // host buffers are the same; otherwise it is captured again.
-    const uint64_t key[12] = {(uint64_t)kcount, (uint64_t)fp, (uint64_t)block, (uint64_t)lds, (uint64_t)grid,
-                              (uint64_t)n, (uint64_t)(uintptr_t)d_args, (uint64_t)(uintptr_t)d_start,
-                              (uint64_t)(uintptr_t)d_scl, (uint64_t)(uintptr_t)d_scout,
-                              (uint64_t)(uintptr_t)c->h_batch, (uint64_t)(uintptr_t)st};
-    if (c->batch_exec && memcmp(key, c->batch_key, sizeof(key)) != 0) {
-        hipGraphExecDestroy(c->batch_exec);
-        c->batch_exec = nullptr;
-    }
-    if (!c->batch_exec) {
-        hipGraph_t graph = nullptr;
-        {
-            // no allocation anywhere in the process while the capture is open
-            std::unique_lock<std::shared_mutex> capture_lock(dim3h::capture_mutex());
-            HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            dim3h::expand_batch_launch(kcount, fp, block, lds, grid, d_args, d_start, n, d_start + n + 1, d_scl,
-                                       d_scout, st);
-            hipMemcpyAsync(c->h_batch, d_scout, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st);
-            HIPCHK(c, hipStreamEndCapture(st, &graph));
-        }
-        const hipError_t ie = hipGraphInstantiate(&c->batch_exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
-        if (ie != hipSuccess) {
-            c->batch_exec = nullptr;
-            return fail(c, DIM3_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
-        }
-        memcpy(c->batch_key, key, sizeof(key));
-    }
-    HIPCHK(c, hipEventRecord(c->ev[4], st));
-    hipError_t ge = hipGraphLaunch(c->batch_exec, st);
-    if (ge == hipSuccess) ge = hipEventRecord(c->ev[5], st);
-    if (ge == hipSuccess) ge = hipStreamSynchronize(st);
-    if (ge != hipSuccess) {
-        hipGraphExecDestroy(c->batch_exec);
-        c->batch_exec = nullptr;
-        return fail(c, DIM3_ERR_DEVICE, "batch launch: %s", hipGetErrorString(ge));
-    }
     float ms_expand = 0;
-    hipEventElapsedTime(&ms_expand, c->ev[4], c->ev[5]);
+    if (na == n) {
+        TRY(plan_chunk(0, n));
+        HIPCHK(c, hipEventRecord(c->ev[4], st));
+        TRY(launch_chunk(0, n, st, c->grid_cus(), 0));
+        HIPCHK(c, hipEventRecord(c->ev[5], st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        hipEventElapsedTime(&ms_expand, c->ev[4], c->ev[5]);
+    } else {
+        hipStream_t sp;
+        TRY(pipe_stream(c, &sp));
+        TRY(plan_chunk(0, na));
+        HIPCHK(c, hipEventRecord(c->ev[6], st));  // chunk A's CSRs are built on the call's stream
+        HIPCHK(c, hipStreamWaitEvent(sp, c->ev[6], 0));
+        HIPCHK(c, hipEventRecord(c->ev[4], sp));
+        TRY(launch_chunk(0, na, sp, std::max(1, (int)(c->pipe_cus * c->cu_share)), 0));
+        HIPCHK(c, hipEventRecord(c->ev[7], sp));
+        int rc = plan_chunk(na, n);  // beside chunk A's expansion
+        if (rc == DIM3_OK) rc = launch_chunk(na, n, st, c->grid_cus(), 1);
+        if (rc == DIM3_OK) {
+            hipError_t e = hipEventRecord(c->ev[5], st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = fail(c, DIM3_ERR_DEVICE, "batch chunk B: %s", hipGetErrorString(e));
+        }
+        const hipError_t ea = hipStreamSynchronize(sp);  // chunk A always finishes before we return
+        TRY(rc);
+        if (ea != hipSuccess) return fail(c, DIM3_ERR_DEVICE, "batch chunk A: %s", hipGetErrorString(ea));
+        float ma = 0, mb = 0;
+        hipEventElapsedTime(&ma, c->ev[4], c->ev[7]);
+        hipEventElapsedTime(&mb, c->ev[4], c->ev[5]);
+        ms_expand = std::max(ma, mb);
+    }
 
     int rc = DIM3_OK;
-    for (uint32_t i = 0; i < n; i++) {
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t i = order[k];
         c->use_bank(i + 1);
-        Scalars s = c->h_batch[i];
+        Scalars s = c->h_batch[k];
         if (dense_overflowed(c, s)) {  // a multiplicity above 127: this instance again, all-sparse
             dim3_opts oi = ob;
             oi.dense_mode = DIM3_DENSE_OFF;
@@ -2438,7 +2529,12 @@ int dim3_ctx_destroy(dim3_ctx* c) {
     for (auto& b : c->banks)
         for (auto& s : b->ws)
             if (s.p) dim3h::dev_free(s.p);
-    if (c->batch_exec) hipGraphExecDestroy(c->batch_exec);
+    for (auto& ex : c->batch_exec)
+        if (ex) hipGraphExecDestroy(ex);
+    if (c->pipe_stream) {
+        hipStreamSynchronize(c->pipe_stream);
+        hipStreamDestroy(c->pipe_stream);
+    }
     if (c->h_batch) dim3h::host_free(c->h_batch);
     if (c->h_stream) dim3h::host_free(c->h_stream);
     for (auto& e : c->ev)
