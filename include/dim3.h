@@ -354,7 +354,11 @@ typedef struct dim3_instance {
 /* results[i] is instance i's result (rows written to inst[i].out).  Returns
  * DIM3_ERR_CAPACITY when some instance's rows exceed its cap (its
  * results[i].n holds the rows needed).  The context keeps every instance's
- * workspace (one bank per instance) for reuse by the next batch. */
+ * workspace (one bank per instance) for reuse by the next batch.
+ * With stream == NULL and 16 or more instances, the larger half of the batch
+ * (by tuples) expands on a context-owned stream masked to 7/8 of the compute
+ * units while the rest is planned; the call returns after both halves end.
+ * On a caller's stream the whole batch runs on that stream. */
 int dim3_batch_join_project_dev(dim3_ctx* ctx, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
                                 void* stream, dim3_result* results);
 int dim3_batch_join_aggregate_count_dev(dim3_ctx* ctx, const dim3_instance* inst, uint32_t n, const dim3_opts* opts,
