@@ -470,10 +470,13 @@ int dim3_server_get_stats(dim3_server* srv, dim3_server_stats* st);
 
 /* ---- cost-model calibration (autotuner) ----
  * The planner prices the dense part against the sparse expansion with device
- * rates.  dim3_autotune measures them on the context's device with short
- * synthetic queries (complete bipartite joins with 1 and 32 paths per output
- * row, all-sparse, MFMA and bit-or) plus an int8 product and a streaming fill,
- * and installs them in the context.  It replaces the reference's offline
+ * rates.  dim3_autotune measures the context's device with short synthetic
+ * queries (rows reaching 32 keys of scattered z values against rows reaching
+ * one key with the same output, all-sparse and MFMA-seeded) plus an int8
+ * product and a streaming fill, and installs the rates in the context: the
+ * product and fill rates as measured, the path and seeding rates as the
+ * defaults (fitted on real workloads on MI355X) scaled by the device's speed
+ * on the synthetic queries relative to MI355X's.  It replaces the reference's offline
  * constants (server_parameters.hpp, produced by server_parameter_estimator.cpp).
  * cache_path (nullable): a text file; read when it holds a calibration of the
  * same device model, else written after measuring. */

@@ -2303,27 +2303,34 @@ float elapsed(dim3_ctx* c, int a, int b) {
     return ms;
 }
 
-// One fingerprint query of two complete bipartite relations X x Y and Z x Y
-// (every output row reached by Y paths); returns the plan's timings.
-int tune_query(dim3_ctx* c, uint32_t X, uint32_t Z, uint32_t Y, int dense, float* ms_dense, float* ms_expand,
-               QueryPlan* plan) {
+// One fingerprint query of R = X x Y (every row reaches every key) against
+// S = Y keys of D scattered z values each (affine maps of 0 .. D - 1 into
+// [0, 2^kTuneZBits), distinct per key): each row's output is the union of
+// its keys' z sets, spread over the z range as the z IDs of real data are
+// (consecutive z IDs would make the LDS updates of a path unrealistically
+// cheap).  Returns the plan's timings and the output rows.
+constexpr uint32_t kTuneZBits = 20;
+
+int tune_query(dim3_ctx* c, uint32_t X, uint32_t Y, uint32_t D, int dense, float* ms_dense, float* ms_expand,
+               QueryPlan* plan, uint64_t* rows) {
     hipStream_t st = c->stream;
     c->qstream = st;
     int2 *R, *S;
     TRY(ws(c, WS_IN_R, (uint64_t)X * Y, &R));
-    TRY(ws(c, WS_IN_S, (uint64_t)Z * Y, &S));
+    TRY(ws(c, WS_IN_S, (uint64_t)D * Y, &S));
     dim3h::grid_relation(R, X, Y, st);
-    dim3h::grid_relation(S, Z, Y, st);
+    dim3h::scattered_relation(S, D, Y, kTuneZBits, st);
     dim3_opts o;
     dim3_opts_default(&o);
     o.output_mode = DIM3_OUT_FINGERPRINT;
     o.flags = DIM3_OPT_NO_SWAP;
     o.dense_mode = dense;
-    TRY(prepare(c, false, R, (uint64_t)X * Y, S, (uint64_t)Z * Y, o, st));
+    TRY(prepare(c, false, R, (uint64_t)X * Y, S, (uint64_t)D * Y, o, st));
     TRY(expand(c, true, nullptr, 0, st));
     *ms_dense = elapsed(c, 3, 6);
     *ms_expand = elapsed(c, 4, 5);
     *plan = c->bank().plan;
+    *rows = c->h_sc->out_n;
     return DIM3_OK;
 }
 
@@ -2339,7 +2346,7 @@ int autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
             char name[256] = {0};
             dim3_tuning t;
             memset(&t, 0, sizeof(t));
-            const int n = fscanf(f, "dim3-tuning-v1 %255s %lf %lf %lf %lf %lf %lf", name, &t.path_ns, &t.seed_bit_ns,
+            const int n = fscanf(f, "dim3-tuning-v2 %255s %lf %lf %lf %lf %lf %lf", name, &t.path_ns, &t.seed_bit_ns,
                                  &t.seed_word_ns, &t.mfma_tops, &t.bitor_gbs, &t.hbm_gbs);
             fclose(f);
             if (n == 7 && dev == name && t.path_ns > 0 && t.mfma_tops > 0 && t.hbm_gbs > 0) {
@@ -2381,27 +2388,40 @@ int autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
         HIPCHK(c, hipStreamSynchronize(st));
         m.mfma_ops = 3.0 * 2.0 * M * M * (double)K / (elapsed(c, 0, 1) * 1e-3);
     }
-    // sparse paths: the same 16384 x 16384 output reached by 1 and by 32 paths
-    const uint32_t X = 16384, Z = 16384;
-    float d1, e1, d32, e32, dm, em, db, eb;
-    QueryPlan p1, p32, pm, pb;
-    TRY(tune_query(c, X, Z, 1, DIM3_DENSE_OFF, &d1, &e1, &p1));
-    TRY(tune_query(c, X, Z, 32, DIM3_DENSE_OFF, &d32, &e32, &p32));
+    // Sparse paths: 4096 rows, each reaching 32 keys of 32768 scattered z
+    // values (4.3e9 paths, about 2 / 3 of the 2^20 z values per row), then
+    // the same rows from one key whose z set has the same size: the emission
+    // is the same, so the difference is the paths' cost.  Then the 32-key
+    // query seeded from an MFMA product (word copies of the product rows).
+    // The model's path and seeding rates are fitted on real workloads on
+    // MI355X, whose rows mix key weights (partial pieces, per-task work):
+    // these heavy-row micro-queries run about three times faster per path
+    // than that, so they scale the fitted rates by this device's speed
+    // relative to the same micro-queries on MI355X (kRef*) rather than
+    // replacing them.  The bit-or rate keeps its fitted value.
+    constexpr double kRefPathS = 0.297e-12, kRefSeedWordS = 0.0739e-9;
+    auto scale = [](double measured, double ref) { return std::min(4.0, std::max(0.25, measured / ref)); };
+    const DenseModel fitted;
+    const uint32_t X = 4096, Y = 32, D = 32768;
+    float d32, e32, d1, e1, dm, em;
+    QueryPlan p32, p1, pm;
+    uint64_t r32, r1, rm;
+    TRY(tune_query(c, X, Y, D, DIM3_DENSE_OFF, &d32, &e32, &p32, &r32));
+    const uint32_t D1 = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(r32 / X, 1), 1u << kTuneZBits);
+    TRY(tune_query(c, X, 1, D1, DIM3_DENSE_OFF, &d1, &e1, &p1, &r1));
     const double dpaths = (double)p32.out_j - (double)p1.out_j;
-    if (e32 > e1 && dpaths > 0) m.path_s = (e32 - e1) * 1e-3 / dpaths;
-    m.seed_s = m.path_s;  // a product bit costs what a path costs: one z ID and one LDS op
-    // dense products of the 32-path query: MFMA, then bit-or
-    TRY(tune_query(c, X, Z, 32, DIM3_DENSE_MFMA, &dm, &em, &pm));
-    TRY(tune_query(c, X, Z, 32, DIM3_DENSE_BITOR, &db, &eb, &pb));
-    const double words = (double)X * ((Z + 31) / 32);
-    // seeding copies every word of the product; the emission is that of the 1-path query
-    if (pm.dense == DIM3_DENSE_MFMA && em > e1) m.seed_word_s = (em - e1) * 1e-3 / words;
-    if (pb.dense == DIM3_DENSE_BITOR) {
-        const double reads = (double)X * 32.0 * (double)pb.dwords * 4.0;  // each row ORs 32 key bitmaps
-        // fused into the seeding: the expansion's extra time less the word stores
-        const double t = pb.fused ? (eb - e1) * 1e-3 - words * m.seed_word_s : db * 1e-3;
-        if (t > 0) m.l2_bps = reads / t;
+    double path_s = 0;
+    if (e32 > e1 && dpaths > 0) {
+        path_s = (e32 - e1) * 1e-3 / dpaths;
+        m.path_s = fitted.path_s * scale(path_s, kRefPathS);
+        m.seed_s = fitted.seed_s * scale(path_s, kRefPathS);  // one z ID and one LDS op, as a path
     }
+    // the emission of the 32-key query's rows (the 1-key query less its paths)
+    const double emit_s = std::max(0.0, e1 * 1e-3 - (double)p1.out_j * path_s);
+    TRY(tune_query(c, X, Y, D, DIM3_DENSE_MFMA, &dm, &em, &pm, &rm));
+    const double words = (double)X * (double)(pm.dwords ? pm.dwords : (pm.nz + 31) / 32);
+    if (pm.dense == DIM3_DENSE_MFMA && em * 1e-3 > emit_s && rm == r32)
+        m.seed_word_s = fitted.seed_word_s * scale((em * 1e-3 - emit_s) / words, kRefSeedWordS);
     c->model = m;
     c->model_source = 1;
     if (out) model_to_tuning(m, 1, out);
@@ -2410,7 +2430,7 @@ int autotune(dim3_ctx* c, const char* cache_path, dim3_tuning* out) {
         if (f) {
             dim3_tuning t;
             model_to_tuning(m, 1, &t);
-            fprintf(f, "dim3-tuning-v1 %s %.6g %.6g %.6g %.6g %.6g %.6g\n", dev.c_str(), t.path_ns, t.seed_bit_ns,
+            fprintf(f, "dim3-tuning-v2 %s %.6g %.6g %.6g %.6g %.6g %.6g\n", dev.c_str(), t.path_ns, t.seed_bit_ns,
                     t.seed_word_ns, t.mfma_tops, t.bitor_gbs, t.hbm_gbs);
             fclose(f);
         }

@@ -446,6 +446,10 @@ struct KeyMap {
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 // complete bipartite relation (i / Y, i % Y), i < X * Y
 void grid_relation(int2* out, uint32_t X, uint32_t Y, hipStream_t st);
+// Y keys of D scattered z values each: tuple i = (z, y), y = i / D,
+// z = (a_y (i % D) + b_y) mod 2^zbits with a_y odd (distinct z per key),
+// D <= 2^zbits
+void scattered_relation(int2* out, uint32_t D, uint32_t Y, uint32_t zbits, hipStream_t st);
 void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
 void map_init(const KeyMap& m, hipStream_t st);
 // mm[0..3] = min/max of R.x, R.y; mm[4..7] = min/max of S.z, S.y

@@ -598,6 +598,20 @@ __global__ __launch_bounds__(kThreads) void k_path_dot(const uint64_t* __restric
 
 namespace dim3h {
 
+__global__ void k_scattered_rel(int2* __restrict__ out, uint32_t D, uint32_t Y, uint32_t zmask) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)D * Y) return;
+    const uint32_t y = (uint32_t)(i / D), j = (uint32_t)(i - (uint64_t)y * D);
+    const uint64_t h = mix64(0x9E3779B97F4A7C15ull * (y + 1u));
+    const uint32_t a = (uint32_t)h | 1u, b = (uint32_t)(h >> 32);
+    out[i] = make_int2((int32_t)((a * j + b) & zmask), (int32_t)y);
+}
+
+void scattered_relation(int2* out, uint32_t D, uint32_t Y, uint32_t zbits, hipStream_t st) {
+    const uint64_t n = (uint64_t)D * Y;
+    if (n) hipLaunchKernelGGL(k_scattered_rel, dim3(grid_for(n)), dim3(kThreads), 0, st, out, D, Y, (1u << zbits) - 1u);
+}
+
 void grid_relation(int2* out, uint32_t X, uint32_t Y, hipStream_t st) {
     const uint64_t n = (uint64_t)X * Y;
     if (n) hipLaunchKernelGGL(k_grid_rel, dim3(grid_for(n)), dim3(kThreads), 0, st, out, X, Y);
