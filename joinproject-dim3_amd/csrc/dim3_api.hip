@@ -186,6 +186,7 @@ struct QueryPlan {
     uint64_t out_j = 0;
     uint32_t ntiles = 1, tile_size = 0, tile_words = 0;
     bool x_range = false, y_range = false, z_ident = false;
+    bool hot = false;  // Scalars::hot holds the hot-value samples (large inputs)
     int32_t zlo = 0;
     dim3h::KeyMap mx{}, my{}, mz{};
     uint32_t touched_cap = 0, stage_cap = 0;
@@ -1037,6 +1038,7 @@ int prepare_map(dim3_ctx* c, const dim3_opts& o, hipStream_t st) {
     // the most frequent key of each side: its tuples share one atomic per
     // wavefront in the key-counting kernels
     const int32_t* hot = dim3h::hot_keys(dR, nR, dS, nS, sc->hot, st) ? sc->hot : nullptr;
+    P.hot = hot != nullptr;
     dim3h::build_y(dR, nR, P.my, degR, slotR, st, hot);
     dim3h::probe_y(dS, nS, P.my, degR, degS, slotS, &sc->s_surv, st, hot);
     dim3h::scan_y(degR, degS, (uint32_t)ny1, exY, part, &sc->tot_y, &sc->out_j, st);
@@ -1046,13 +1048,18 @@ int prepare_map(dim3_ctx* c, const dim3_opts& o, hipStream_t st) {
         TRY(ws(c, WS_AGG_GS, ny1, &gs));
         dim3h::path_sums(dR, nR, slotR, dS, nS, slotS, (uint32_t)ny1, kr, gs, &sc->agg_sum2, st, hot);
     }
-    dim3h::build_x(dR, nR, slotR, degS, P.mx, degX, slotRx, st);
+    dim3h::build_x(dR, nR, slotR, degS, P.mx, degX, slotRx, st, hot);
     dim3h::scan_deg(degX, (uint32_t)nx1, exX, part, &sc->tot_x, st);
     if (!P.z_ident) {
         dim3h::map_init(P.mz, st);
         dim3h::fill_u32(degZ, nz1, 0, st);
         dim3h::build_z(dS, nS, slotS, P.mz, degZ, slotSz, st);
         dim3h::scan_deg(degZ, (uint32_t)nz1, exZ, part, &sc->tot_z, st);
+    }
+    // the hot x and z values as slots, for the passes that count per slot
+    if (hot) {
+        dim3h::hot_slots(sc->hot + dim3h::kHotRX, P.mx, false, 0, 0, sc->hot_xs, st);
+        dim3h::hot_slots(sc->hot + dim3h::kHotSZ, P.mz, P.z_ident, P.zlo, P.z_span, sc->hot_zs, st);
     }
     // heaviness histogram of the dense split
     const bool want_dense = o.dense_mode != DIM3_DENSE_OFF;
@@ -1066,7 +1073,8 @@ int prepare_map(dim3_ctx* c, const dim3_opts& o, hipStream_t st) {
         dim3h::fill_u32(zw, nz1, 0, st);
         dim3h::fill_u64((uint64_t*)dhist, 6ull * dim3h::kDenseBins, 0, st);
         dim3h::dense_weights(dR, nR, slotR, slotRx, dS, nS, slotS, P.z_ident ? nullptr : slotSz, P.zlo, degR, degS, xw,
-                             (uint32_t)nx1, zw, (uint32_t)nz1, st);
+                             (uint32_t)nx1, zw, (uint32_t)nz1, st, hot ? sc->hot_xs : nullptr,
+                             hot ? sc->hot_zs : nullptr);
         dim3h::dense_hist(degR, degS, (uint32_t)ny1, xw, (uint32_t)nx1, zw, (uint32_t)nz1, dhist, st);
         HIPCHK(c, hipMemcpyAsync(B.hhist, dhist, 6ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
     }
@@ -1160,7 +1168,8 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
             dim3h::fill_u32(hdz, nz1, 0, st);
             dim3h::fill_u64((uint64_t*)hh, 4ull * dim3h::kDenseBins, 0, st);
             dim3h::dense_heavy_degrees(slotR, slotRx, nR, dS, slotS, P.z_ident ? nullptr : slotSz, P.zlo, nS, exY,
-                                       hid, hdx, (uint32_t)nx1, hdz, (uint32_t)nz1, hh, st);
+                                       hid, hdx, (uint32_t)nx1, hdz, (uint32_t)nz1, hh, st,
+                                       P.hot ? sc->hot_xs : nullptr, P.hot ? sc->hot_zs : nullptr);
             HIPCHK(c, hipMemcpyAsync(B.hhist, hh, 4ull * dim3h::kDenseBins * 8, hipMemcpyDeviceToHost, st));
             HIPCHK(c, hipStreamSynchronize(st));
             const bool core = plan_core(c, o);
@@ -1208,7 +1217,7 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     }
     if (!P.gc) {
         dim3h::count_s(dS, slotS, slotSz, nS, exY, P.z_ident ? nullptr : exZ, P.zlo, P.ntiles, P.tile_size, hid,
-                       zflag, cntS, cntSD, keyS, zidS, st);
+                       zflag, cntS, cntSD, keyS, zidS, st, P.hot ? sc->hot : nullptr);
         dim3h::scan_u32(cntS, (uint32_t)nkeyS, rowS, part, &sc->tot_s, st);
         dim3h::finish_rowptr(rowS, (uint32_t)nkeyS, &sc->tot_s, st);
         if (P.core) dim3h::split_cursors(rowS, cntSD, (uint32_t)nkeyS, curS, curSD, st);

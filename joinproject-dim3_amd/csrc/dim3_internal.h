@@ -165,6 +165,48 @@ __device__ __forceinline__ T wave_reduce_xor(T v) {
     return v;
 }
 
+// Index of v in a hot-key list {count, keys...} (the most frequent values
+// of an attribute in a sample, hot_keys), or -1; hot may be null.
+__device__ __forceinline__ int hot_index(const int32_t* hot, int32_t v) {
+    if (hot == nullptr) return -1;
+    const int n = hot[0];
+    int h = -1;
+    for (int k = 0; k < n; k++)
+        if (v == hot[1 + k]) h = k;
+    return h;
+}
+
+// Workgroup tallies of the hot keys.  Same-address global atomics
+// serialise, about 8 ns apart on MI355X, so a per-tuple pass over a
+// relation in which one key holds 50,000 tuples (a graph hub, a Zipf head)
+// takes ~0.4 ms on that key alone.  The counting passes therefore loop over
+// many tuples per workgroup (grid capped at kTallyBlocks) and count each hot
+// key's tuples in LDS, with the key's slot, adding each tally to its global
+// counter once per workgroup (tally_flush); the other keys keep the
+// wavefront-aggregated atomics below.
+constexpr unsigned kTallyBlocks = 2048;
+constexpr int kTallyKeys = 15;  // = the hot-key lists' length (dim3h::kHotKeys)
+
+template <int BLOCK>
+__device__ __forceinline__ void tally_init(uint32_t* tally, uint32_t* tslot, int n) {
+    for (int k = (int)threadIdx.x; k < n; k += BLOCK) {
+        tally[k] = 0;
+        if (tslot != nullptr) tslot[k] = 0;
+    }
+    __syncthreads();
+}
+
+// counter adds (MAX = false) or running maxima (MAX = true) of n tallies
+template <int BLOCK, bool MAX = false>
+__device__ __forceinline__ void tally_flush(uint32_t* ctr, const uint32_t* tally, const uint32_t* tslot, int n) {
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < n; k += BLOCK) {
+        if (tally[k] == 0u) continue;
+        if (MAX) atomicMax(&ctr[tslot[k]], tally[k]);
+        else atomicAdd(&ctr[tslot[k]], tally[k]);
+    }
+}
+
 // Wavefront-aggregated counter updates for skewed keys (Zipf join keys, hub
 // vertices): in two rounds, the lanes whose key equals the first pending
 // lane's key share one atomic; the remaining lanes add on their own.  Every
@@ -381,7 +423,9 @@ inline hipError_t host_free(void* p) {
 
 
 // Device scalar block, zeroed at the start of every query.
-constexpr int kHotKeys = 15, kHotSlots = kHotKeys + 1;
+constexpr int kHotKeys = dim3d::kTallyKeys, kHotSlots = kHotKeys + 1;
+// hot-key lists in Scalars::hot: R.y, S.y, R.x, S.x (S's x is its z)
+constexpr int kHotRY = 0, kHotSY = kHotSlots, kHotRX = 2 * kHotSlots, kHotSZ = 3 * kHotSlots;
 constexpr uint64_t kHotMinTuples = 1ull << 20;  // smaller relations take no hot-key sample
 
 struct Scalars {
@@ -402,9 +446,11 @@ struct Scalars {
     uint64_t agg_sum2;   // COUNT fingerprint from the join paths: sum of k2(x) g(z) over paths
     uint64_t n_hash;     // value aggregates: rows through the LDS hash table
     uint64_t n_wave;     // value aggregates: rows expanded one per wavefront
-    // the most frequent join keys of R (hot[0..kHotSlots)) and of S (the
-    // rest) in a sample: a count, then the keys
-    int32_t hot[2 * kHotSlots];
+    // the most frequent values in a sample of R.y, S.y, R.x and S.x (at
+    // kHotRY, kHotSY, kHotRX, kHotSZ): a count, then the values
+    int32_t hot[4 * kHotSlots];
+    // the hot R.x and S.x values' slots in the x and z maps (hot_slots)
+    int32_t hot_xs[kHotSlots], hot_zs[kHotSlots];
     uint64_t spare[4];
     // work counters on a cache line of their own: atomics on one line
     // serialise, and every task fetch would queue behind the row reservations
@@ -457,10 +503,10 @@ void minmax(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* mm,
 
 // y mapping: insert R.y, count degR, record slots; probe S.y, count degS and survivors
 // hot (nullable): {R's most frequent key, valid, S's, valid} (hot_keys)
-// The kHotKeys most frequent join keys of each side among 4096 evenly spaced
-// tuples (those seen at least 8 times): hot[0] = count, hot[1..] = keys; S's
-// list starts at hot + kHotSlots.  Returns false, writing nothing, when
-// neither side has kHotMinTuples tuples.
+// The kHotKeys most frequent values of R.y, S.y, R.x and S.x among 4096
+// evenly spaced tuples (those seen at least 8 times), as lists {count,
+// values...} at hot + kHotRY, kHotSY, kHotRX, kHotSZ.  Returns false, writing
+// nothing, when neither side has kHotMinTuples tuples.
 bool hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* hot, hipStream_t st);
 void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
              const int32_t* hot = nullptr);
@@ -476,7 +522,7 @@ void path_sums(const int2* R, uint64_t nR, const uint32_t* slotR, const int2* S,
                uint32_t nslots, uint64_t* kr, uint64_t* gs, uint64_t* out, hipStream_t st, const int32_t* hot = nullptr);
 // x / z mapping over the tuples whose y joined
 void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
-             uint32_t* degX, int32_t* slotRx, hipStream_t st);
+             uint32_t* degX, int32_t* slotRx, hipStream_t st, const int32_t* hot = nullptr);
 void build_z(const int2* S, uint64_t nS, const int32_t* slotS, const KeyMap& m, uint32_t* degZ, int32_t* slotSz,
              hipStream_t st);
 // scan of a degree array: ex[s] = (#occupied before s) << 32 | (sum deg before s)
@@ -498,7 +544,8 @@ void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const 
 // core still need it), counted in cntD and placed at the front of its row
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
              const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
-             const uint32_t* zflag, uint32_t* cnt, uint32_t* cntD, uint32_t* keyS, int32_t* zidS, hipStream_t st);
+             const uint32_t* zflag, uint32_t* cnt, uint32_t* cntD, uint32_t* keyS, int32_t* zidS, hipStream_t st,
+             const int32_t* hot = nullptr);
 void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uint32_t* curL, uint32_t* curD,
                    hipStream_t st);
 void scan_u32(const uint32_t* cnt, uint32_t n, uint32_t* out_excl, uint64_t* partials, uint64_t* tot,
@@ -583,7 +630,13 @@ int bitor_rows(const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid,
 // largest key heaviness over the tuples of each x slot / z slot (slotSz == nullptr: z identity)
 void dense_weights(const int2* R, uint64_t nR, const uint32_t* slotR, const int32_t* slotRx, const int2* S,
                    uint64_t nS, const int32_t* slotS, const int32_t* slotSz, int32_t zlo, const uint32_t* degR,
-                   const uint32_t* degS, uint32_t* xw, uint32_t nx1, uint32_t* zw, uint32_t nz1, hipStream_t st);
+                   const uint32_t* degS, uint32_t* xw, uint32_t nx1, uint32_t* zw, uint32_t nz1, hipStream_t st,
+                   const int32_t* hsx = nullptr, const int32_t* hsz = nullptr);
+// A hot-value list (hot_keys) as slots of map m (ident: value - zlo when in
+// [0, span), the z identity mapping), in the same {count, ...} format; kernels
+// that work in slot space tally those slots (hsx, hsz above and below).
+void hot_slots(const int32_t* hot, const KeyMap& m, bool ident, int32_t zlo, uint64_t span, int32_t* out,
+               hipStream_t st);
 // hist[6][kDenseBins]: keys, paths, sum degR, sum degS, x rows, z columns per heaviness bin
 void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint32_t* xw, uint32_t nx1,
                 const uint32_t* zw, uint32_t nz1, unsigned long long* hist, hipStream_t st);
@@ -595,7 +648,8 @@ void dense_mark_keys(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, c
 void dense_heavy_degrees(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const int2* S,
                          const int32_t* slotS, const int32_t* slotSz, int32_t zlo, uint64_t nS, const uint64_t* ex_y,
                          const int32_t* hid, uint32_t* hdx, uint32_t nx1, uint32_t* hdz, uint32_t nz1,
-                         unsigned long long* hist, hipStream_t st);
+                         unsigned long long* hist, hipStream_t st, const int32_t* hsx = nullptr,
+                         const int32_t* hsz = nullptr);
 // product row per x ID (+ inverse) and column flag per z ID: by heaviness
 // bin (hdx / hdz null) or by heavy degree >= thx / thz (the dense core);
 // counters[1] = rows

@@ -292,24 +292,45 @@ __device__ __forceinline__ uint32_t key_w(const uint32_t* degR, const uint32_t* 
     return min(degR[s], degS[s]);
 }
 
-__global__ void k_wmax_x(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
-                         const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS, uint32_t* xw) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nR) return;
-    const int32_t sx = slotRx[i];
-    const uint32_t w = sx >= 0 ? key_w(degR, degS, slotR[i]) : 0u;
-    wave_max_update(xw, (uint32_t)sx, w, sx >= 0);
+// hsx / hsz (nullable): the hot x / z values' slots, tallied per workgroup.
+__global__ __launch_bounds__(256) void k_wmax_x(const uint32_t* __restrict__ slotR,
+                                                const int32_t* __restrict__ slotRx, uint64_t nR,
+                                                const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS,
+                                                uint32_t* xw, const int32_t* __restrict__ hsx) {
+    __shared__ uint32_t tally[kTallyKeys], tslot[kTallyKeys];
+    tally_init<256>(tally, tslot, kTallyKeys);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < nR; i0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = i0 + threadIdx.x;
+        const int32_t sx = i < nR ? slotRx[i] : -1;
+        const uint32_t w = sx >= 0 ? key_w(degR, degS, slotR[i]) : 0u;
+        const int h = sx >= 0 ? hot_index(hsx, sx) : -1;
+        if (h >= 0) {
+            tslot[h] = (uint32_t)sx;
+            atomicMax(&tally[h], w);
+        }
+        wave_max_update(xw, (uint32_t)sx, w, sx >= 0 && h < 0);
+    }
+    tally_flush<256, true>(xw, tally, tslot, kTallyKeys);
 }
-
-__global__ void k_wmax_z(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
-                         const int32_t* __restrict__ slotSz, int32_t zlo, uint64_t nS,
-                         const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS, uint32_t* zw) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nS) return;
-    const int32_t sy = slotS[i];
-    const uint32_t zs = sy < 0 ? 0u : (slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i]);
-    const uint32_t w = sy >= 0 ? key_w(degR, degS, (uint32_t)sy) : 0u;
-    wave_max_update(zw, zs, w, sy >= 0);
+__global__ __launch_bounds__(256) void k_wmax_z(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
+                                                const int32_t* __restrict__ slotSz, int32_t zlo, uint64_t nS,
+                                                const uint32_t* __restrict__ degR, const uint32_t* __restrict__ degS,
+                                                uint32_t* zw, const int32_t* __restrict__ hsz) {
+    __shared__ uint32_t tally[kTallyKeys], tslot[kTallyKeys];
+    tally_init<256>(tally, tslot, kTallyKeys);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < nS; i0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = i0 + threadIdx.x;
+        const int32_t sy = i < nS ? slotS[i] : -1;
+        const uint32_t zs = sy < 0 ? 0u : (slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i]);
+        const uint32_t w = sy >= 0 ? key_w(degR, degS, (uint32_t)sy) : 0u;
+        const int h = sy >= 0 ? hot_index(hsz, (int32_t)zs) : -1;
+        if (h >= 0) {
+            tslot[h] = zs;
+            atomicMax(&tally[h], w);
+        }
+        wave_max_update(zw, zs, w, sy >= 0 && h < 0);
+    }
+    tally_flush<256, true>(zw, tally, tslot, kTallyKeys);
 }
 
 // hist layout: [6][kDenseBins] u64 = keys, paths, sum degR, sum degS, x rows, z columns
@@ -396,24 +417,44 @@ __global__ void k_mark_zflag(const uint32_t* __restrict__ degZ, const uint64_t* 
 }
 
 // Heavy entries per x slot and heavy tuples per z slot (dense-core selection).
-__global__ void k_hdeg_x(const uint32_t* __restrict__ slotR, const int32_t* __restrict__ slotRx, uint64_t nR,
-                         const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid, uint32_t* hdx) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nR) return;
-    const int32_t sx = slotRx[i];
-    const bool heavy = sx >= 0 && hid[(uint32_t)(ex_y[slotR[i]] >> 32)] >= 0;
-    wave_count_add(hdx, (uint32_t)sx, heavy);
+__global__ __launch_bounds__(256) void k_hdeg_x(const uint32_t* __restrict__ slotR,
+                                                const int32_t* __restrict__ slotRx, uint64_t nR,
+                                                const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid,
+                                                uint32_t* hdx, const int32_t* __restrict__ hsx) {
+    __shared__ uint32_t tally[kTallyKeys], tslot[kTallyKeys];
+    tally_init<256>(tally, tslot, kTallyKeys);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < nR; i0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = i0 + threadIdx.x;
+        const int32_t sx = i < nR ? slotRx[i] : -1;
+        const bool heavy = sx >= 0 && hid[(uint32_t)(ex_y[slotR[i]] >> 32)] >= 0;
+        const int h = heavy ? hot_index(hsx, sx) : -1;
+        if (h >= 0) {
+            tslot[h] = (uint32_t)sx;
+            atomicAdd(&tally[h], 1u);
+        }
+        wave_count_add(hdx, (uint32_t)sx, heavy && h < 0);
+    }
+    tally_flush<256>(hdx, tally, tslot, kTallyKeys);
 }
-
-__global__ void k_hdeg_z(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
-                         const int32_t* __restrict__ slotSz, int32_t zlo, uint64_t nS, const uint64_t* __restrict__ ex_y,
-                         const int32_t* __restrict__ hid, uint32_t* hdz) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nS) return;
-    const int32_t sy = slotS[i];
-    const bool heavy = sy >= 0 && hid[(uint32_t)(ex_y[sy] >> 32)] >= 0;
-    const uint32_t zs = heavy ? (slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i]) : 0u;
-    wave_count_add(hdz, zs, heavy);
+__global__ __launch_bounds__(256) void k_hdeg_z(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
+                                                const int32_t* __restrict__ slotSz, int32_t zlo, uint64_t nS,
+                                                const uint64_t* __restrict__ ex_y, const int32_t* __restrict__ hid,
+                                                uint32_t* hdz, const int32_t* __restrict__ hsz) {
+    __shared__ uint32_t tally[kTallyKeys], tslot[kTallyKeys];
+    tally_init<256>(tally, tslot, kTallyKeys);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < nS; i0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = i0 + threadIdx.x;
+        const int32_t sy = i < nS ? slotS[i] : -1;
+        const bool heavy = sy >= 0 && hid[(uint32_t)(ex_y[sy] >> 32)] >= 0;
+        const uint32_t zs = heavy ? (slotSz == nullptr ? (uint32_t)S[i].x - (uint32_t)zlo : (uint32_t)slotSz[i]) : 0u;
+        const int h = heavy ? hot_index(hsz, (int32_t)zs) : -1;
+        if (h >= 0) {
+            tslot[h] = zs;
+            atomicAdd(&tally[h], 1u);
+        }
+        wave_count_add(hdz, zs, heavy && h < 0);
+    }
+    tally_flush<256>(hdz, tally, tslot, kTallyKeys);
 }
 
 // The same counts (KIND 0: heavy entries per x slot, 1: heavy tuples per z
@@ -623,6 +664,8 @@ __global__ void k_build_b(const int2* __restrict__ S, const int32_t* __restrict_
 }
 
 inline unsigned grid1(uint64_t n) { return (unsigned)(n == 0 ? 1 : (n + 255) / 256); }
+// the counting passes: grid-stride over at most kTallyBlocks workgroups
+inline unsigned tally_grid(uint64_t n) { return std::min<unsigned>(grid1(n), kTallyBlocks); }
 
 }  // namespace
 
@@ -656,12 +699,14 @@ int bitor_rows(const uint32_t* rowptrR, const int32_t* colR, const int32_t* hid,
 
 void dense_weights(const int2* R, uint64_t nR, const uint32_t* slotR, const int32_t* slotRx, const int2* S,
                    uint64_t nS, const int32_t* slotS, const int32_t* slotSz, int32_t zlo, const uint32_t* degR,
-                   const uint32_t* degS, uint32_t* xw, uint32_t nx1, uint32_t* zw, uint32_t nz1, hipStream_t st) {
+                   const uint32_t* degS, uint32_t* xw, uint32_t nx1, uint32_t* zw, uint32_t nz1, hipStream_t st,
+                   const int32_t* hsx, const int32_t* hsz) {
     (void)R;
     if (nR && !slot_lds<2>(S, slotR, slotRx, slotS, slotSz, zlo, nR, nullptr, nullptr, degR, degS, nx1, xw, st))
-        hipLaunchKernelGGL(k_wmax_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, degR, degS, xw);
+        hipLaunchKernelGGL(k_wmax_x, dim3(tally_grid(nR)), dim3(256), 0, st, slotR, slotRx, nR, degR, degS, xw, hsx);
     if (nS && !slot_lds<3>(S, slotR, slotRx, slotS, slotSz, zlo, nS, nullptr, nullptr, degR, degS, nz1, zw, st))
-        hipLaunchKernelGGL(k_wmax_z, dim3(grid1(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, degR, degS, zw);
+        hipLaunchKernelGGL(k_wmax_z, dim3(tally_grid(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, degR, degS, zw,
+                           hsz);
 }
 
 void dense_hist(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, const uint32_t* xw, uint32_t nx1,
@@ -681,11 +726,12 @@ void dense_mark_keys(const uint32_t* degR, const uint32_t* degS, uint32_t ny1, c
 void dense_heavy_degrees(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const int2* S,
                          const int32_t* slotS, const int32_t* slotSz, int32_t zlo, uint64_t nS, const uint64_t* ex_y,
                          const int32_t* hid, uint32_t* hdx, uint32_t nx1, uint32_t* hdz, uint32_t nz1,
-                         unsigned long long* hist, hipStream_t st) {
+                         unsigned long long* hist, hipStream_t st, const int32_t* hsx, const int32_t* hsz) {
     if (nR && !slot_lds<0>(S, slotR, slotRx, slotS, slotSz, zlo, nR, ex_y, hid, nullptr, nullptr, nx1, hdx, st))
-        hipLaunchKernelGGL(k_hdeg_x, dim3(grid1(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, hid, hdx);
+        hipLaunchKernelGGL(k_hdeg_x, dim3(tally_grid(nR)), dim3(256), 0, st, slotR, slotRx, nR, ex_y, hid, hdx, hsx);
     if (nS && !slot_lds<1>(S, slotR, slotRx, slotS, slotSz, zlo, nS, ex_y, hid, nullptr, nullptr, nz1, hdz, st))
-        hipLaunchKernelGGL(k_hdeg_z, dim3(grid1(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, ex_y, hid, hdz);
+        hipLaunchKernelGGL(k_hdeg_z, dim3(tally_grid(nS)), dim3(256), 0, st, S, slotS, slotSz, zlo, nS, ex_y, hid, hdz,
+                           hsz);
     unsigned g = grid1(std::max(nx1, nz1));
     if (g > 1024) g = 1024;
     hipLaunchKernelGGL(k_hdeg_hist, dim3(g), dim3(256), 0, st, hdx, nx1, hdz, nz1, hist);

@@ -17,14 +17,43 @@ namespace {
 using namespace dim3d;
 
 constexpr uint32_t kThreads = 256;
+// grid-stride launches: each workgroup folds many tuples before touching
+// global counters
+constexpr uint32_t kMaxBlocks = 2048;
+// workgroup-private key counts: a heavy sampled key (the identity sample
+// takes every key with its low bits zero, key 0 included, often a hub) would
+// otherwise put one global atomic per tuple on a single counter
+constexpr uint32_t kLdsSlots = 1024;
+constexpr uint32_t kLdsProbes = 8;
 
 __global__ __launch_bounds__(kThreads) void k_est_count_s(const int2* __restrict__ S, uint64_t nS, uint32_t smask,
                                                           uint32_t* keys, uint32_t hmask, uint32_t* cnt) {
-    const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= nS) return;
-    const uint32_t y = (uint32_t)S[i].y;
-    if ((y & smask) != 0u) return;
-    atomicAdd(&cnt[ht_insert(keys, hmask, y)], 1u);
+    __shared__ uint32_t lk[kLdsSlots], lc[kLdsSlots];
+    for (uint32_t j = threadIdx.x; j < kLdsSlots; j += kThreads) {
+        lk[j] = DIM3_EMPTY_KEY;
+        lc[j] = 0;
+    }
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nS; i += (uint64_t)gridDim.x * kThreads) {
+        const uint32_t y = (uint32_t)S[i].y;
+        if ((y & smask) != 0u) continue;
+        bool done = false;
+        if (y != DIM3_EMPTY_KEY) {
+            uint32_t j = hash32(y) & (kLdsSlots - 1u);
+            for (uint32_t p = 0; p < kLdsProbes && !done; p++, j = (j + 1u) & (kLdsSlots - 1u)) {
+                uint32_t cur = lk[j];
+                if (cur == DIM3_EMPTY_KEY) cur = atomicCAS(&lk[j], DIM3_EMPTY_KEY, y);
+                if (cur == DIM3_EMPTY_KEY || cur == y) {
+                    atomicAdd(&lc[j], 1u);
+                    done = true;
+                }
+            }
+        }
+        if (!done) atomicAdd(&cnt[ht_insert(keys, hmask, y)], 1u);  // a full neighbourhood: straight to the table
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < kLdsSlots; j += kThreads)
+        if (lc[j] != 0u) atomicAdd(&cnt[ht_insert(keys, hmask, lk[j])], lc[j]);
 }
 
 __global__ __launch_bounds__(kThreads) void k_est_probe_r(const int2* __restrict__ R, uint64_t nR, uint32_t smask,
@@ -32,13 +61,12 @@ __global__ __launch_bounds__(kThreads) void k_est_probe_r(const int2* __restrict
                                                           const uint32_t* __restrict__ cnt,
                                                           unsigned long long* out) {
     __shared__ uint64_t sh[kThreads / 64 + 1];
-    const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     uint64_t v = 0;
-    if (i < nR) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nR; i += (uint64_t)gridDim.x * kThreads) {
         const uint32_t y = (uint32_t)R[i].y;
         if ((y & smask) == 0u) {
             const int32_t s = ht_find(keys, hmask, y);
-            if (s >= 0) v = cnt[s];
+            if (s >= 0) v += cnt[s];
         }
     }
     v = block_reduce_sum<kThreads, uint64_t>(v, sh);
@@ -68,12 +96,12 @@ namespace dim3h {
 
 void estimate_sample(const int2* R, uint64_t nR, const int2* S, uint64_t nS, uint32_t smask, uint32_t* keys,
                      uint32_t hmask, uint32_t* cnt, unsigned long long* out, hipStream_t st) {
+    auto blocks = [](uint64_t n) { return (unsigned)std::min<uint64_t>((n + kThreads - 1) / kThreads, kMaxBlocks); };
     if (nS)
-        hipLaunchKernelGGL(k_est_count_s, dim3((unsigned)((nS + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, S,
-                           nS, smask, keys, hmask, cnt);
+        hipLaunchKernelGGL(k_est_count_s, dim3(blocks(nS)), dim3(kThreads), 0, st, S, nS, smask, keys, hmask, cnt);
     if (nR)
-        hipLaunchKernelGGL(k_est_probe_r, dim3((unsigned)((nR + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, R,
-                           nR, smask, (const uint32_t*)keys, hmask, (const uint32_t*)cnt, out);
+        hipLaunchKernelGGL(k_est_probe_r, dim3(blocks(nR)), dim3(kThreads), 0, st, R, nR, smask,
+                           (const uint32_t*)keys, hmask, (const uint32_t*)cnt, out);
 }
 
 void estimate_prefix(const int2* R, uint32_t a, const int2* S, uint32_t b, unsigned long long* out, hipStream_t st) {

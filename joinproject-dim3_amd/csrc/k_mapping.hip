@@ -31,6 +31,12 @@ constexpr int kThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kThreads * kScanItems;  // 2048 elements per scan block
 
+// the counting passes: grid-stride over at most kTallyBlocks workgroups
+inline unsigned tally_grid(uint64_t n) {
+    const uint64_t b = (n + kThreads - 1) / kThreads;
+    return (unsigned)(b < 1 ? 1 : b < kTallyBlocks ? b : kTallyBlocks);
+}
+
 inline unsigned grid_for(uint64_t n, int threads = kThreads) {
     uint64_t g = (n + threads - 1) / threads;
     if (g == 0) g = 1;
@@ -77,6 +83,26 @@ __device__ __forceinline__ int32_t slot_find(const dim3h::KeyMap& m, int32_t v) 
         return (int32_t)(m.mask ? (off * kScatterMul) & m.mask : off);
     }
     return ht_find(m.keys, m.mask, (uint32_t)v);
+}
+
+// The slots of a hot-value list ({count, values...}, hot_keys) in a key map,
+// or, ident, value - zlo when it falls in [0, span): the same list format in
+// slot space (a value without a slot gets -1, which no valid slot equals).
+__global__ void k_hot_slots(const int32_t* __restrict__ hot, dim3h::KeyMap m, bool ident, int32_t zlo, uint64_t span,
+                            int32_t* out) {
+    const int t = (int)threadIdx.x;
+    const int n = hot[0];
+    if (t == 0) out[0] = n;
+    if (t >= n) return;
+    const int32_t v = hot[1 + t];
+    int32_t s = -1;
+    if (ident) {
+        const int64_t d = (int64_t)v - (int64_t)zlo;
+        s = d >= 0 && (uint64_t)d < span ? (int32_t)d : -1;
+    } else {
+        s = slot_find(m, v);
+    }
+    out[1 + t] = s;
 }
 
 __device__ __forceinline__ int32_t key_of(const dim3h::KeyMap& m, uint32_t s) {
@@ -129,31 +155,32 @@ __global__ __launch_bounds__(kThreads) void k_minmax(const int2* __restrict__ V,
 }
 
 // mapping_y insert side (dim3.hpp:822-828): every R.y gets a slot; degR counts it.
-// hot (nullable): {count, keys...}, the side's most frequent keys; each key's
-// lanes share one atomic per wavefront (a Zipf key can hold a fifth of the
-// tuples).  Returns y's index in the list, or -1.
-__device__ __forceinline__ int hot_index(const int32_t* hot, int32_t y) {
-    if (hot == nullptr) return -1;
-    const int n = hot[0];
-    int h = -1;
-    for (int k = 0; k < n; k++)
-        if (y == hot[1 + k]) h = k;
-    return h;
-}
-
-__global__ void k_build_y(const int2* __restrict__ R, uint64_t nR, dim3h::KeyMap m, uint32_t* degR,
-                          uint32_t* __restrict__ slotR, const int32_t* __restrict__ hot) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nR) return;
-    const int32_t y = R[i].y;
-    const uint32_t s = slot_insert(m, y);
-    wave_count_add(degR, s, true, hot_index(hot, y));
-    slotR[i] = s;
+// hot (nullable): R.y's hot-key list, tallied per workgroup.
+__global__ __launch_bounds__(kThreads) void k_build_y(const int2* __restrict__ R, uint64_t nR, dim3h::KeyMap m,
+                                                      uint32_t* degR, uint32_t* __restrict__ slotR,
+                                                      const int32_t* __restrict__ hot) {
+    __shared__ uint32_t tally[kTallyKeys], tslot[kTallyKeys];
+    tally_init<kThreads>(tally, tslot, kTallyKeys);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * kThreads; i0 < nR; i0 += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t i = i0 + threadIdx.x;
+        const bool ok = i < nR;
+        const int32_t y = ok ? R[i].y : 0;
+        const uint32_t s = ok ? slot_insert(m, y) : 0u;
+        const int h = ok ? hot_index(hot, y) : -1;
+        if (h >= 0) {
+            tslot[h] = s;  // every writer stores the key's one slot
+            atomicAdd(&tally[h], 1u);
+        }
+        wave_count_add(degR, s, ok && h < 0);
+        if (ok) slotR[i] = s;
+    }
+    tally_flush<kThreads>(degR, tally, tslot, kTallyKeys);
 }
 
 // The most frequent keys of 4096 evenly spaced tuples of T (one workgroup):
 // out = {count, keys...}, up to kHotKeys keys seen at least kMinSeen times,
 // most frequent first.
+template <bool X>  // sample T.x instead of T.y
 __global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, uint64_t n, int32_t* out) {
     constexpr uint32_t kSample = 4096, kSlots = 8192, kMinSeen = 8;
     __shared__ uint32_t keys[kSlots];
@@ -169,7 +196,8 @@ __global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, ui
     const uint64_t m = n < kSample ? n : kSample;
     const uint64_t step = m ? n / m : 0;  // index k * step < m * step <= n
     for (uint32_t k = threadIdx.x; k < m; k += 1024) {
-        const uint32_t y = (uint32_t)T[(uint64_t)k * step].y;
+        const int2 tk = T[(uint64_t)k * step];
+        const uint32_t y = (uint32_t)(X ? tk.x : tk.y);
         // open addressing; the key 0xFFFFFFFF (-1) is not a candidate
         if (y == 0xFFFFFFFFu) continue;
         uint32_t s = (y * 0x9E3779B1u) >> 19;  // 13 bits: kSlots
@@ -205,33 +233,58 @@ __global__ __launch_bounds__(kThreads) void k_probe_y(const int2* __restrict__ S
                                                       int32_t* __restrict__ slotS, uint64_t* surv,
                                                       const int32_t* __restrict__ hot) {
     __shared__ uint64_t sh[kThreads / 64 + 1];
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t tally[kTallyKeys], tslot[kTallyKeys];
+    tally_init<kThreads>(tally, tslot, kTallyKeys);
     uint64_t ok = 0;
-    int32_t s = -1;
-    int h = -1;
-    if (i < nS) {
-        const int32_t y = S[i].y;
-        s = slot_find(m, y);
-        if (s >= 0 && m.range && degR[s] == 0) s = -1;
-        slotS[i] = s;
-        ok = s >= 0 ? 1 : 0;
-        h = hot_index(hot, y);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * kThreads; i0 < nS; i0 += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t i = i0 + threadIdx.x;
+        int32_t s = -1;
+        int h = -1;
+        if (i < nS) {
+            const int32_t y = S[i].y;
+            s = slot_find(m, y);
+            if (s >= 0 && m.range && degR[s] == 0) s = -1;
+            slotS[i] = s;
+            ok += s >= 0 ? 1 : 0;
+            h = s >= 0 ? hot_index(hot, y) : -1;
+        }
+        if (h >= 0) {
+            tslot[h] = (uint32_t)s;
+            atomicAdd(&tally[h], 1u);
+        }
+        wave_count_add(degS, (uint32_t)s, s >= 0 && h < 0);
     }
-    wave_count_add(degS, (uint32_t)s, s >= 0, h);
+    tally_flush<kThreads>(degS, tally, tslot, kTallyKeys);
     ok = block_reduce_sum<kThreads>(ok, sh);
     if (threadIdx.x == 0 && ok) atomicAdd((unsigned long long*)surv, (unsigned long long)ok);
 }
 
 // mapping_xz over R.x (dim3.hpp:749-755), restricted to tuples whose y joined.
-__global__ void k_build_x(const int2* __restrict__ R, uint64_t nR, const uint32_t* __restrict__ slotR,
-                          const uint32_t* __restrict__ degS_y, dim3h::KeyMap m, uint32_t* degX,
-                          int32_t* __restrict__ slotRx) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nR) return;
-    int32_t out = -1;
-    if (degS_y[slotR[i]] > 0) out = (int32_t)slot_insert(m, R[i].x);
-    wave_count_add(degX, (uint32_t)out, out >= 0);
-    slotRx[i] = out;
+// hot (nullable): R.x's hot-key list, tallied per workgroup.
+__global__ __launch_bounds__(kThreads) void k_build_x(const int2* __restrict__ R, uint64_t nR,
+                                                      const uint32_t* __restrict__ slotR,
+                                                      const uint32_t* __restrict__ degS_y, dim3h::KeyMap m,
+                                                      uint32_t* degX, int32_t* __restrict__ slotRx,
+                                                      const int32_t* __restrict__ hot) {
+    __shared__ uint32_t tally[kTallyKeys], tslot[kTallyKeys];
+    tally_init<kThreads>(tally, tslot, kTallyKeys);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * kThreads; i0 < nR; i0 += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t i = i0 + threadIdx.x;
+        int32_t out = -1;
+        int h = -1;
+        if (i < nR && degS_y[slotR[i]] > 0) {
+            const int32_t x = R[i].x;
+            out = (int32_t)slot_insert(m, x);
+            h = hot_index(hot, x);
+        }
+        if (h >= 0) {
+            tslot[h] = (uint32_t)out;
+            atomicAdd(&tally[h], 1u);
+        }
+        wave_count_add(degX, (uint32_t)out, out >= 0 && h < 0);
+        if (i < nR) slotRx[i] = out;
+    }
+    tally_flush<kThreads>(degX, tally, tslot, kTallyKeys);
 }
 
 // mapping_xz over the surviving S.z.
@@ -466,32 +519,56 @@ __device__ __forceinline__ uint32_t z_id(const int2* S, uint64_t i, const int32_
 
 // divide_S (dim3.hpp:1186-1253): CSR of S keyed by (y, z-tile) so that the
 // expansion of one z tile reads only its own column range.
-__global__ void k_count_s(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
-                          const int32_t* __restrict__ slotSz, uint64_t nS, const uint64_t* __restrict__ ex_y,
-                          const uint64_t* __restrict__ ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size,
-                          const int32_t* __restrict__ hid, const uint32_t* __restrict__ zflag,
-                          uint32_t* cnt, uint32_t* cntD, uint32_t* __restrict__ keyS, int32_t* __restrict__ zidS) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nS) return;
-    const int32_t sy = slotS[i];
-    if (sy < 0) {  // no match
-        keyS[i] = 0xFFFFFFFFu;
-        return;
+// hot (nullable): S.y's hot-key list; with at most kTallyTiles tiles and no
+// dense core, each (hot key, tile) counter is tallied per workgroup.
+constexpr uint32_t kTallyTiles = 8;
+__global__ __launch_bounds__(kThreads) void k_count_s(const int2* __restrict__ S, const int32_t* __restrict__ slotS,
+                                                      const int32_t* __restrict__ slotSz, uint64_t nS,
+                                                      const uint64_t* __restrict__ ex_y,
+                                                      const uint64_t* __restrict__ ex_z, int32_t zlo, uint32_t ntiles,
+                                                      uint32_t tile_size, const int32_t* __restrict__ hid,
+                                                      const uint32_t* __restrict__ zflag, uint32_t* cnt,
+                                                      uint32_t* cntD, uint32_t* __restrict__ keyS,
+                                                      int32_t* __restrict__ zidS, const int32_t* __restrict__ hot) {
+    constexpr int kT = kTallyKeys * (int)kTallyTiles;
+    __shared__ uint32_t tally[kT], tslot[kT];
+    tally_init<kThreads>(tally, tslot, kT);
+    const int32_t* th = (ntiles <= kTallyTiles && cntD == nullptr) ? hot : nullptr;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * kThreads; i0 < nS; i0 += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t i = i0 + threadIdx.x;
+        bool counted = false, dcol = false;
+        uint32_t key = 0;
+        int h = -1;
+        if (i < nS) {
+            const int32_t sy = slotS[i];
+            uint32_t kraw = 0xFFFFFFFFu;  // no match
+            if (sy >= 0) {
+                const uint32_t yid = (uint32_t)(ex_y[sy] >> 32);
+                const uint32_t zid = z_id(S, i, slotSz, ex_z, zlo);
+                // a heavy key's tuple with a product column: only rows outside
+                // the dense core (cntD != nullptr) still need it, at the front
+                // of its CSR row
+                dcol = hid != nullptr && hid[yid] >= 0 && zflag[zid] != 0u;
+                if (!dcol || cntD != nullptr) {
+                    const uint32_t tile = ntiles == 1 ? 0u : zid / tile_size;
+                    key = ntiles == 1 ? yid : yid * ntiles + tile;
+                    counted = true;
+                    h = hot_index(th, S[i].y);
+                    if (h >= 0) h = h * (int)ntiles + (int)tile;
+                    kraw = key | (dcol ? 0x80000000u : 0u);
+                    zidS[i] = (int32_t)zid;
+                }
+            }
+            keyS[i] = kraw;
+        }
+        if (h >= 0) {
+            tslot[h] = key;
+            atomicAdd(&tally[h], 1u);
+        }
+        wave_count_add(cnt, key, counted && h < 0);
+        if (cntD != nullptr) wave_count_add(cntD, key, counted && dcol);
     }
-    const uint32_t yid = (uint32_t)(ex_y[sy] >> 32);
-    const uint32_t zid = z_id(S, i, slotSz, ex_z, zlo);
-    // a heavy key's tuple with a product column: only rows outside the dense
-    // core (cntD != nullptr) still need it, at the front of its CSR row
-    const bool dcol = hid != nullptr && hid[yid] >= 0 && zflag[zid] != 0u;
-    if (dcol && cntD == nullptr) {
-        keyS[i] = 0xFFFFFFFFu;
-        return;
-    }
-    const uint32_t key = ntiles == 1 ? yid : yid * ntiles + zid / tile_size;
-    wave_count_add(cnt, key, true);
-    if (cntD != nullptr) wave_count_add(cntD, key, dcol);
-    keyS[i] = key | (dcol ? 0x80000000u : 0u);
-    zidS[i] = (int32_t)zid;
+    tally_flush<kThreads>(cnt, tally, tslot, kT);
 }
 
 // cursors of the split S rows: product-column tuples from the row start,
@@ -639,21 +716,30 @@ bool hot_keys(const int2* R, uint64_t nR, const int2* S, uint64_t nS, int32_t* h
     // below 2^20 tuples a side's hottest keys serialise for well under a
     // millisecond: no sample
     if (nR < kHotMinTuples && nS < kHotMinTuples) return false;
-    hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, R, nR, hot);
-    hipLaunchKernelGGL(k_hot_key, dim3(1), dim3(1024), 0, st, S, nS, hot + kHotSlots);
+    hipLaunchKernelGGL(k_hot_key<false>, dim3(1), dim3(1024), 0, st, R, nR, hot + kHotRY);
+    hipLaunchKernelGGL(k_hot_key<false>, dim3(1), dim3(1024), 0, st, S, nS, hot + kHotSY);
+    hipLaunchKernelGGL(k_hot_key<true>, dim3(1), dim3(1024), 0, st, R, nR, hot + kHotRX);
+    hipLaunchKernelGGL(k_hot_key<true>, dim3(1), dim3(1024), 0, st, S, nS, hot + kHotSZ);
     return true;
 }
+void hot_slots(const int32_t* hot, const KeyMap& m, bool ident, int32_t zlo, uint64_t span, int32_t* out,
+               hipStream_t st) {
+    hipLaunchKernelGGL(k_hot_slots, dim3(1), dim3(64), 0, st, hot, m, ident, zlo, span, out);
+}
+
 void build_y(const int2* R, uint64_t nR, const KeyMap& m, uint32_t* degR, uint32_t* slotR, hipStream_t st,
              const int32_t* hot) {
     if (count_lds<0>(R, nR, m, nullptr, nullptr, nullptr, degR, (int32_t*)slotR, nullptr, st)) return;
-    if (nR) hipLaunchKernelGGL(k_build_y, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, m, degR, slotR, hot);
+    if (nR)
+        hipLaunchKernelGGL(k_build_y, dim3(tally_grid(nR)), dim3(kThreads), 0, st, R, nR, m, degR, slotR,
+                           hot != nullptr ? hot + kHotRY : nullptr);
 }
 void probe_y(const int2* S, uint64_t nS, const KeyMap& m, const uint32_t* degR, uint32_t* degS, int32_t* slotS,
              uint64_t* surv, hipStream_t st, const int32_t* hot) {
     if (count_lds<1>(S, nS, m, degR, nullptr, nullptr, degS, slotS, surv, st)) return;
     if (nS)
-        hipLaunchKernelGGL(k_probe_y, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, m, degR, degS, slotS, surv,
-                           hot != nullptr ? hot + kHotSlots : nullptr);
+        hipLaunchKernelGGL(k_probe_y, dim3(tally_grid(nS)), dim3(kThreads), 0, st, S, nS, m, degR, degS, slotS,
+                           surv, hot != nullptr ? hot + kHotSY : nullptr);
 }
 void scan_y(const uint32_t* degR, const uint32_t* degS, uint32_t nslots, uint64_t* ex, uint64_t* partials,
             uint64_t* tot, uint64_t* out_j, hipStream_t st) {
@@ -672,17 +758,17 @@ void path_sums(const int2* R, uint64_t nR, const uint32_t* slotR, const int2* S,
     if (nR) hipLaunchKernelGGL(k_path_kr, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, kr, hot);
     if (nS)
         hipLaunchKernelGGL(k_path_gs, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, nS, slotS, gs,
-                           hot != nullptr ? hot + kHotSlots : nullptr);
+                           hot != nullptr ? hot + kHotSY : nullptr);
     unsigned g = grid_for(nslots);
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(k_path_dot, dim3(g), dim3(kThreads), 0, st, kr, gs, nslots, out);
 }
 void build_x(const int2* R, uint64_t nR, const uint32_t* slotR, const uint32_t* degS_y, const KeyMap& m,
-             uint32_t* degX, int32_t* slotRx, hipStream_t st) {
+             uint32_t* degX, int32_t* slotRx, hipStream_t st, const int32_t* hot) {
     if (count_lds<2>(R, nR, m, degS_y, slotR, nullptr, degX, slotRx, nullptr, st)) return;
     if (nR)
-        hipLaunchKernelGGL(k_build_x, dim3(grid_for(nR)), dim3(kThreads), 0, st, R, nR, slotR, degS_y, m, degX,
-                           slotRx);
+        hipLaunchKernelGGL(k_build_x, dim3(tally_grid(nR)), dim3(kThreads), 0, st, R, nR, slotR, degS_y, m, degX,
+                           slotRx, hot != nullptr ? hot + kHotRX : nullptr);
 }
 void build_z(const int2* S, uint64_t nS, const int32_t* slotS, const KeyMap& m, uint32_t* degZ, int32_t* slotSz,
              hipStream_t st) {
@@ -709,10 +795,12 @@ void scatter_r(const uint32_t* slotR, const int32_t* slotRx, uint64_t nR, const 
 }
 void count_s(const int2* S, const int32_t* slotS, const int32_t* slotSz, uint64_t nS, const uint64_t* ex_y,
              const uint64_t* ex_z, int32_t zlo, uint32_t ntiles, uint32_t tile_size, const int32_t* hid,
-             const uint32_t* zflag, uint32_t* cnt, uint32_t* cntD, uint32_t* keyS, int32_t* zidS, hipStream_t st) {
+             const uint32_t* zflag, uint32_t* cnt, uint32_t* cntD, uint32_t* keyS, int32_t* zidS, hipStream_t st,
+             const int32_t* hot) {
     if (nS)
-        hipLaunchKernelGGL(k_count_s, dim3(grid_for(nS)), dim3(kThreads), 0, st, S, slotS, slotSz, nS, ex_y, ex_z,
-                           zlo, ntiles, tile_size, hid, zflag, cnt, cntD, keyS, zidS);
+        hipLaunchKernelGGL(k_count_s, dim3(tally_grid(nS)), dim3(kThreads), 0, st, S, slotS, slotSz, nS, ex_y, ex_z,
+                           zlo, ntiles, tile_size, hid, zflag, cnt, cntD, keyS, zidS,
+                           hot != nullptr ? hot + kHotSY : nullptr);
 }
 void split_cursors(const uint32_t* rowptr, const uint32_t* cntD, uint32_t n, uint32_t* curL, uint32_t* curD,
                    hipStream_t st) {
