@@ -35,6 +35,7 @@ using dim3h::ExpandArgs;
 constexpr uint32_t kUnroll = 4;
 constexpr uint32_t kPiece = 64 * kUnroll;  // join paths per wavefront work item
 constexpr uint32_t kPairs = 2;             // pieces in flight per wavefront
+constexpr uint32_t kSweepWords = 2;        // tile words per lane and step of a swept row's emission
 // word 3 of a raw (untyped, unswizzled) buffer descriptor on CDNA: 32-bit
 // data format, so loads past num_records return 0
 constexpr int kRawBufferWord3 = 0x00020000;
@@ -206,16 +207,67 @@ __device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, in
         }
         __builtin_amdgcn_wave_barrier();
     };
-    // 64-word chunks dealt to the wavefronts round robin: a dense z range
-    // (the hubs) spreads over all of them instead of loading one
-    for (uint32_t i0 = wid * 64u; i0 < W; i0 += 64u * NW) {
-        const uint32_t w = i0 + lane;
-        uint32_t v = 0;
-        if (w < W) {
-            v = tile[w];
-            if (v) tile[w] = 0u;
+    // Chunks of 64 kSweepWords words dealt to the wavefronts round robin (a
+    // dense z range, the hubs, spreads over all of them instead of loading
+    // one); lane l takes the chunk's words kSweepWords l and on, with one
+    // LDS read and clear, one popcount scan for all of them, and a bit loop
+    // per word.  The words of a lane lie in one 32-word block, so word q's
+    // entries start q (blocked layout) or 32 q (plain) after word 0's.
+    constexpr uint32_t Q = kSweepWords;
+    const uint32_t qstep = lm ? 1u : 32u;
+    // A chunk denser than the stage region (the hub z IDs of a graph
+    // numbered by degree fill whole words): rounds of at most SW / 64 rows
+    // per lane, each staged and flushed with coalesced stores.  The stage is
+    // empty when this runs (flushed before).
+    auto rounds = [&](uint32_t v, uint32_t zw) {
+        const uint32_t m = SW / 64u;
+        for (;;) {
+            uint32_t cr = (uint32_t)__popc(v);
+            cr = cr < m ? cr : m;
+            const uint32_t ir = wave_scan_u32(cr);
+            const uint32_t tr = (uint32_t)__builtin_amdgcn_readlane((int)ir, 63);
+            if (tr == 0u) break;
+            uint32_t p = ir - cr;
+            for (uint32_t j = 0; j < cr; j++) {
+                const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                v &= v - 1u;
+                stg[p++] = zw + (b << lsh);
+            }
+            __builtin_amdgcn_wave_barrier();
+            flush(tr);
         }
-        const uint32_t cnt = (uint32_t)__popc(v);
+    };
+    for (uint32_t i0 = wid * 64u * Q; i0 < W; i0 += 64u * Q * NW) {
+        const uint32_t w = i0 + Q * lane;
+        uint32_t v[Q];
+#pragma unroll
+        for (uint32_t q = 0; q < Q; q++) v[q] = 0u;
+        if (w + Q <= W) {
+            if (Q == 4u) {
+                uint4* t4 = reinterpret_cast<uint4*>(tile + w);
+                const uint4 t = *t4;
+                v[0] = t.x; v[Q > 1 ? 1 : 0] = t.y; v[Q > 2 ? 2 : 0] = t.z; v[Q > 3 ? 3 : 0] = t.w;
+                if (t.x | t.y | t.z | t.w) *t4 = make_uint4(0u, 0u, 0u, 0u);
+            } else if (Q == 2u) {
+                uint2* t2 = reinterpret_cast<uint2*>(tile + w);
+                const uint2 t = *t2;
+                v[0] = t.x; v[Q > 1 ? 1 : 0] = t.y;
+                if (t.x | t.y) *t2 = make_uint2(0u, 0u);
+            } else {
+                v[0] = tile[w];
+                if (v[0]) tile[w] = 0u;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t q = 0; q < Q; q++)
+                if (w + q < W) {
+                    v[q] = tile[w + q];
+                    if (v[q]) tile[w + q] = 0u;
+                }
+        }
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < Q; q++) cnt += (uint32_t)__popc(v[q]);
         const uint32_t incl = wave_scan_u32(cnt);
         const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         if (tot == 0u) continue;
@@ -223,35 +275,22 @@ __device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, in
             flush(fill);
             fill = 0;
         }
-        const uint32_t zw = zoff + tl_base(w, lm);  // entry of the word's bit 0
+        const uint32_t zw = zoff + tl_base(w, lm);  // entry of word 0's bit 0
         if (tot > SW) {
-            // A chunk denser than the stage region (the hub z IDs of a graph
-            // numbered by degree fill whole words): rounds of at most SW / 64
-            // rows per lane, each staged and flushed with coalesced stores.
-            // The stage is empty here (flushed above).
-            const uint32_t m = SW / 64u;
-            for (;;) {
-                uint32_t cr = (uint32_t)__popc(v);
-                cr = cr < m ? cr : m;
-                const uint32_t ir = wave_scan_u32(cr);
-                const uint32_t tr = (uint32_t)__builtin_amdgcn_readlane((int)ir, 63);
-                if (tr == 0u) break;
-                uint32_t p = ir - cr;
-                for (uint32_t j = 0; j < cr; j++) {
-                    const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                    v &= v - 1u;
-                    stg[p++] = zw + (b << lsh);
-                }
-                __builtin_amdgcn_wave_barrier();
-                flush(tr);
-            }
+#pragma unroll
+            for (uint32_t q = 0; q < Q; q++) rounds(v[q], zw + q * qstep);
             continue;
         }
         uint32_t p = fill + incl - cnt;
-        while (v) {
-            const uint32_t b = (uint32_t)__ffs(v) - 1u;
-            v &= v - 1u;
-            stg[p++] = zw + (b << lsh);
+#pragma unroll
+        for (uint32_t q = 0; q < Q; q++) {
+            uint32_t u = v[q];
+            const uint32_t zq = zw + q * qstep;
+            while (u) {
+                const uint32_t b = (uint32_t)__ffs(u) - 1u;
+                u &= u - 1u;
+                stg[p++] = zq + (b << lsh);
+            }
         }
         fill += tot;
         __builtin_amdgcn_wave_barrier();
