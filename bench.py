@@ -692,6 +692,7 @@ def main():
     ap.add_argument("--block-density", type=float, default=0.3)
     ap.add_argument("--mode", choices=["materialize", "fingerprint"], default="materialize")
     ap.add_argument("--dense", type=int, default=-1, help="dense mode: -1 auto, 0 off, 1 bit-or, 2 mfma")
+    ap.add_argument("--heavy-k", type=int, default=0, help="heavy join keys of the dense part (0: planner)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--hash-only", action="store_true", help="hash every attribute (no dense-range mapping)")
@@ -756,7 +757,8 @@ def main():
     dE = torch.from_numpy(E).to(dev)
     dE2 = dE if E2 is E else torch.from_numpy(E2).to(dev)
     torch.cuda.synchronize()
-    kw = dict(exclude_equal=not planted, dense_mode=args.dense, block=args.block, tile_bits=args.tile_bits,
+    kw = dict(exclude_equal=not planted, dense_mode=args.dense, heavy_k=args.heavy_k, block=args.block,
+              tile_bits=args.tile_bits,
               output_mode=1 if args.mode == "fingerprint" else 0, hash_only=args.hash_only)
 
     # size the output: a capacity-0 call reports the exact row count
