@@ -359,3 +359,44 @@ def test_classical_path_choice_and_device_api(ctx, dim3, torch):
     assert rc == dim3.OK and r2.n == r.n
     want, _, _ = oracle.join_project(R, S)
     assert np.array_equal(oracle.sort_pairs(out.cpu().numpy()), oracle.sort_pairs(want))
+
+
+def _zipf_values(rng, n, domain, s, lo=0):
+    """n values of [lo, lo + domain) whose ranks (over a shuffled domain) are
+    Zipf(s): a few values take percents of the tuples."""
+    p = np.arange(1, domain + 1, dtype=np.float64) ** -s
+    p /= p.sum()
+    return (rng.permutation(domain)[rng.choice(domain, size=n, p=p)] + lo).astype(np.int32)
+
+
+def _count_rows_no_swap(R, S):
+    """The oracle's COUNT rows of R x S in (R.x, S.z) orientation."""
+    if R.shape[0] < S.shape[0]:
+        wp, wc, _, _ = oracle.join_aggregate_count(S, R)
+        return oracle.sort_pairs(np.ascontiguousarray(wp[:, ::-1]), wc)
+    wp, wc, _, _ = oracle.join_aggregate_count(R, S)
+    return oracle.sort_pairs(wp, wc)
+
+
+@pytest.mark.parametrize("hash_only", [False, True])
+def test_hot_value_tallies_vs_oracle(ctx, hash_only):
+    """Relations of at least 2^20 tuples take samples of their most frequent
+    key and x / z values, whose tuples the y and x mapping, S count,
+    heaviness and heavy-degree passes count per workgroup in LDS.  A large
+    side with hub keys and hub x (as R) or hub z (as S) values joined with a
+    small side, in both orientations (no swap), with range and with hashed
+    slots, with and without a forced dense core: rows and COUNT rows against
+    the oracle."""
+    rng = np.random.default_rng(2026)
+    big, small = 1_300_000, 60_000
+    B = np.stack([_zipf_values(rng, big, 200_000, 1.1, lo=-50_000), _zipf_values(rng, big, 20_000, 1.0)], 1)
+    T = np.stack([_zipf_values(rng, small, 100_000, 1.1), rng.integers(0, 20_000, small).astype(np.int32)], 1)
+    for R, S in ((B, T), (T, B)):
+        for core in (-1, 1):
+            kw = dict(no_swap=True, hash_only=hash_only, dense_core=core, classical=False)
+            check_project(ctx, R, S, **kw)
+            got = ctx.join_aggregate_count(R, S, **kw)
+            gp, gc = oracle.sort_pairs(got.xz, got.counts)
+            ep, ec = _count_rows_no_swap(R, S)
+            say("count rows", gp.shape[0], ep.shape[0])
+            assert np.array_equal(gp, ep) and np.array_equal(gc, ec)
