@@ -41,8 +41,7 @@ __global__ __launch_bounds__(kThreads) void k_est_count_s(const int2* __restrict
         if (y != DIM3_EMPTY_KEY) {
             uint32_t j = hash32(y) & (kLdsSlots - 1u);
             for (uint32_t p = 0; p < kLdsProbes && !done; p++, j = (j + 1u) & (kLdsSlots - 1u)) {
-                uint32_t cur = lk[j];
-                if (cur == DIM3_EMPTY_KEY) cur = atomicCAS(&lk[j], DIM3_EMPTY_KEY, y);
+                const uint32_t cur = atomicCAS(&lk[j], DIM3_EMPTY_KEY, y);  // claims, or names the key there
                 if (cur == DIM3_EMPTY_KEY || cur == y) {
                     atomicAdd(&lc[j], 1u);
                     done = true;

@@ -399,15 +399,14 @@ __device__ __forceinline__ void lds_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// z's slot: one compare-and-swap per probe claims an empty slot or returns
+// the key already there (z itself ends the probe too), so the probe loop has
+// no inner branch.
 __device__ __forceinline__ uint32_t light_slot(uint32_t* keys, uint32_t z) {
     uint32_t s = (z * 0x9E3779B1u) >> (32u - kLBits);
     for (;;) {
-        uint32_t k = keys[s];
-        if (k == z) return s;
-        if (k == kNone) {
-            k = atomicCAS(&keys[s], kNone, z);
-            if (k == kNone || k == z) return s;
-        }
+        const uint32_t k = atomicCAS(&keys[s], kNone, z);
+        if (k == kNone || k == z) return s;
         s = (s + 1u) & (kLSlots - 1u);
     }
 }

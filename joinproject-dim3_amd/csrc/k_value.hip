@@ -116,17 +116,15 @@ struct VLds {
 
 // Slot of z ID z in the hash table (inserted if absent): multiplicative
 // hash, linear probing.  A row holds at most half as many groups as the
-// table has slots, so a free slot is always found.
+// table has slots, so a free slot is always found.  One compare-and-swap
+// per probe claims an empty slot or returns the key already there (z itself
+// ends the probe too), so the probe loop has no inner branch.
 __device__ __forceinline__ uint32_t hash_slot(uint32_t* keys, uint32_t z, uint32_t bits) {
     const uint32_t mask = (1u << bits) - 1u;
     uint32_t s = (z * 0x9E3779B1u) >> (32u - bits);
     for (;;) {
-        uint32_t k = keys[s];
-        if (k == z) return s;
-        if (k == kNone) {
-            k = atomicCAS(&keys[s], kNone, z);
-            if (k == kNone || k == z) return s;
-        }
+        const uint32_t k = atomicCAS(&keys[s], kNone, z);
+        if (k == kNone || k == z) return s;
         s = (s + 1u) & mask;
     }
 }
