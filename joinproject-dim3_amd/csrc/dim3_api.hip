@@ -149,6 +149,7 @@ enum SlotId {
     WS_GC_ROWL,
     WS_GC_CURL,
     WS_GC_COLL,
+    WS_FP_ZW,  // word factors of a fingerprint over a dense z range (plain layout)
     WS_BATCH_ARGS,
     WS_BATCH_START,
     WS_BATCH_SCL,
@@ -187,6 +188,7 @@ struct QueryPlan {
     uint32_t ntiles = 1, tile_size = 0, tile_words = 0;
     bool x_range = false, y_range = false, z_ident = false;
     bool hot = false;  // Scalars::hot holds the hot-value samples (large inputs)
+    bool fp_zw = false;  // WS_FP_ZW holds the word factors of the tiles (fingerprint, dense z, plain layout)
     int32_t zlo = 0;
     dim3h::KeyMap mx{}, my{}, mz{};
     uint32_t touched_cap = 0, stage_cap = 0;
@@ -1226,6 +1228,16 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     }
     if (o.flags & DIM3_OPT_EXCLUDE_EQUAL)
         dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
+    // a fingerprint over a dense z range in the plain layout takes each tile
+    // word's factor from a table (the allocation-free entry computes them)
+    P.fp_zw = o.output_mode == DIM3_OUT_FINGERPRINT && P.z_ident && P.plain && !P.count_mode && !P.agg &&
+              c->arena == nullptr;
+    if (P.fp_zw) {
+        uint32_t* zw;
+        const uint64_t nw = (uint64_t)P.ntiles * P.tile_words;
+        TRY(ws(c, WS_FP_ZW, nw, &zw));
+        dim3h::fp_word_factors(zw, (uint32_t)nw, P.zlo, st);
+    }
     LAUNCHCHK(c, "csr");
     HIPCHK(c, hipEventRecord(c->ev[3], st));
     if (P.gc) TRY(gc_prepare(c, slotS, slotSz, degR, degS, exY, exZ, ny1, st));
@@ -1320,6 +1332,7 @@ dim3h::ExpandArgs make_args(dim3_ctx* c, bool fp_mode, void* out, uint64_t cap) 
     // plain bit layout only where the dense seed copies whole words (every z
     // ID a product column); elsewhere the swizzle keeps hub z off one word
     a.swz = P.plain ? 0u : kTileBlockLog;
+    a.fp_zw = fp_mode && P.fp_zw ? (const uint32_t*)W[WS_FP_ZW].p : nullptr;
     a.out = out;
     a.cap = fp_mode ? 0 : cap;
     if (P.core) a.splitS = (const uint32_t*)W[WS_CNT_SD].p;

@@ -579,6 +579,10 @@ struct ExpandArgs {
     uint32_t stage_cap;
     uint32_t fp_mode;
     uint32_t swz;  // project tile bit layout: log2 of the block width in words (tile_words a multiple of it), 0 plain
+    // fingerprint of a dense z range in the plain layout (nullable): the word
+    // factor of word k of the tiles, zword_factor((zlo >> 5) + k), for
+    // ntiles * tile_words words (fp_word_factors)
+    const uint32_t* fp_zw;
     // the (x, z-tile) tasks of this launch: x * ntiles + t in [task_lo, task_hi)
     // (all rows, or one chunk of rows of a streamed query)
     uint32_t task_lo, task_hi;
@@ -700,6 +704,8 @@ void value_tasks(const ExpandArgs& a, uint32_t nz, uint32_t limit, uint32_t* bou
                  uint32_t* fh, uint32_t* ph, uint64_t* partials, uint64_t* n_wave, uint64_t* n_hash,
                  uint32_t* wave_x, uint32_t* hash_x, uint32_t* big_x, hipStream_t st);
 // dynamic LDS of one expansion workgroup (fp: plus the fingerprint table)
+// out[k] = zword_factor(((zlo >> 5) + k) mod 2^27), k < n (zlo a multiple of 32)
+void fp_word_factors(uint32_t* out, uint32_t n, int32_t zlo, hipStream_t st);
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp);
 // Join paths grouped by (x, z) into host arrays (dim3_join_paths_grouped);
 // host tuple arrays in, temporaries allocated and freed on st.  Returns a

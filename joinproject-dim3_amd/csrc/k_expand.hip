@@ -302,7 +302,7 @@ __device__ __forceinline__ bool emit_swept(const ExpandArgs& a, const Lds& L, in
 
 template <int BLOCK, bool COUNT, bool ZID>
 __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32_t t, const Lds& L, uint64_t& fsum,
-                                         uint64_t& fsum2, uint64_t& fcs, uint64_t& npaths) {
+                                         uint64_t& fsum2, uint64_t& fcs, uint64_t& npaths, uint64_t& frows) {
     constexpr uint32_t NW = BLOCK / 64;
     uint32_t* tile = L.tile;
     uint32_t* touched = L.touched;
@@ -628,6 +628,85 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const uint32_t cstep = 64u * NW, cfirst = wid * 64u + lane;
     if (!COUNT && !a.fp_mode && !listed && emit_swept<BLOCK, ZID>(a, L, xv, zbase, self_item, self_keep))
         return;
+
+    if (a.fp_mode) {
+        // Fingerprint, in one pass over the list (or tile) that also counts
+        // the rows (summed per thread across tasks): the z factors of the
+        // row's rows, then one multiply by each x factor.  A plain-layout
+        // word of a dense z range is one 32-aligned word of z values (zlo and
+        // the tile base are multiples of 32): its word factor (from the
+        // query's table when there is one) times the byte-table sums of its
+        // position factors.  Otherwise (swizzled layout, hashed z, counters)
+        // each row takes its own z factor.
+        const uint64_t* fpt = L.fpt;
+        const uint32_t zoff = (uint32_t)a.zlo + zbase;
+        const int32_t* id2z = a.id2z + zbase;
+        uint64_t S1 = 0, S2 = 0;
+        auto take = [&](uint32_t i) {  // word i of the list, filtered (u != w) and cleared
+            const uint32_t w = list_at(touched, listed, i);
+            uint32_t v = tile[w];
+            tile[w] = 0;
+            if (w == self_item) v &= self_keep;
+            frows += units_of<COUNT>(v);
+            return make_uint2(w, v);
+        };
+        if (!COUNT && ZID && lm == 0u) {
+            if (a.fp_zw != nullptr) {
+                const uint32_t* fzw = a.fp_zw + t * W;
+                for (uint32_t i = cfirst; i < count; i += cstep) {
+                    const uint2 wv = take(i);
+                    const uint32_t v = wv.y;
+                    if (v) {
+                        const uint64_t ps = fpt[32u + (v & 255u)] + fpt[288u + ((v >> 8) & 255u)] +
+                                            fpt[544u + ((v >> 16) & 255u)] + fpt[800u + (v >> 24)];
+                        S1 += (uint64_t)fzw[wv.x] * ps;
+                    }
+                }
+            } else {
+                const uint32_t wv0 = zoff >> 5;
+                for (uint32_t i = cfirst; i < count; i += cstep) {
+                    const uint2 wv = take(i);
+                    const uint32_t v = wv.y;
+                    if (v) {
+                        const uint64_t ps = fpt[32u + (v & 255u)] + fpt[288u + ((v >> 8) & 255u)] +
+                                            fpt[544u + ((v >> 16) & 255u)] + fpt[800u + (v >> 24)];
+                        S1 += (uint64_t)zword_factor((wv0 + wv.x) & 0x07FFFFFFu) * ps;  // (z >> 5) of a 32-bit z: 27 bits
+                    }
+                }
+            }
+            S2 = S1;
+        } else {
+            for (uint32_t i = cfirst; i < count; i += cstep) {
+                const uint2 wv = take(i);
+                const uint32_t w = wv.x;
+                uint32_t v = wv.y;
+                if (COUNT) {
+                    if (v) {
+                        const uint32_t u = ZID ? zoff + w : (uint32_t)id2z[w];
+                        const uint64_t g = (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
+                        S1 += g;
+                        S2 += (uint64_t)v * g;
+                        fcs += v;
+                    }
+                } else {
+                    while (v) {
+                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
+                        v &= v - 1u;
+                        const uint32_t e = tl_base(w, lm) + (b << lsh);
+                        const uint32_t u = ZID ? zoff + e : (uint32_t)id2z[e];
+                        S1 += (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
+                    }
+                }
+            }
+            if (!COUNT) S2 = S1;
+        }
+        if (S1 | S2) {
+            fsum += xfactor1(xv) * S1;
+            fsum2 += xfactor2(xv) * S2;
+        }
+        __syncthreads();
+        return;
+    }
     uint32_t c = 0;
     uint64_t nzmask = 0;  // bit g: the lane's word of step g holds rows
     {
@@ -656,61 +735,6 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
     const uint32_t zoff = (uint32_t)a.zlo + zbase;
     const int32_t* id2z = a.id2z + zbase;
 
-    if (a.fp_mode) {
-        // Fingerprint: sum the z factors of the row's rows, then one multiply
-        // by each x factor.  A plain-layout word of a dense z range is one
-        // 32-aligned word of z values (zlo and the tile base are multiples
-        // of 32): its word factor times the byte-table sums of its position
-        // factors.  Otherwise (swizzled layout, hashed z, counters) each row
-        // takes its own z factor.
-        const uint64_t* fpt = L.fpt;
-        uint64_t S1 = 0, S2 = 0;
-        if (!COUNT && ZID && lm == 0u) {
-            const uint32_t wv0 = zoff >> 5;
-            for (uint32_t i = cfirst; i < count; i += cstep) {
-                const uint32_t w = list_at(touched, listed, i);
-                const uint32_t v = tile[w];
-                tile[w] = 0;
-                if (v) {
-                    const uint64_t ps = fpt[32u + (v & 255u)] + fpt[288u + ((v >> 8) & 255u)] +
-                                        fpt[544u + ((v >> 16) & 255u)] + fpt[800u + (v >> 24)];
-                    S1 += (uint64_t)zword_factor((wv0 + w) & 0x07FFFFFFu) * ps;  // (z >> 5) of a 32-bit z: 27 bits
-                }
-            }
-            S2 = S1;
-        } else {
-            for (uint32_t i = cfirst; i < count; i += cstep) {
-                const uint32_t w = list_at(touched, listed, i);
-                uint32_t v = tile[w];
-                tile[w] = 0;
-                if (COUNT) {
-                    if (v) {
-                        const uint32_t u = ZID ? zoff + w : (uint32_t)id2z[w];
-                        const uint64_t g = (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
-                        S1 += g;
-                        S2 += (uint64_t)v * g;
-                        fcs += v;
-                    }
-                } else {
-                    while (v) {
-                        const uint32_t b = (uint32_t)__ffs(v) - 1u;
-                        v &= v - 1u;
-                        const uint32_t e = tl_base(w, lm) + (b << lsh);
-                        const uint32_t u = ZID ? zoff + e : (uint32_t)id2z[e];
-                        S1 += (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
-                    }
-                }
-            }
-            if (!COUNT) S2 = S1;
-        }
-        if (S1 | S2) {
-            fsum += xfactor1(xv) * S1;
-            fsum2 += xfactor2(xv) * S2;
-        }
-        if (tid == 0 && T) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)T);
-        __syncthreads();
-        return;
-    }
 
     if (COUNT || (uint64_t)T * 16u <= count) {
         // Direct (COUNT, or at most one row per 16 listed words): each lane
@@ -849,20 +873,22 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
 // Sums of the per-lane row statistics into the instance's scalars (block-uniform call).
 template <int BLOCK, bool COUNT>
 __device__ __forceinline__ void flush_stats(const ExpandArgs& a, uint64_t& fsum, uint64_t& fsum2, uint64_t& fcs,
-                                            uint64_t& npaths, uint64_t* sh) {
+                                            uint64_t& npaths, uint64_t& frows, uint64_t* sh) {
     const uint64_t np = block_reduce_sum<BLOCK, uint64_t>(npaths, sh);
     if (threadIdx.x == 0 && np) atomicAdd((unsigned long long*)&a.sc->paths, (unsigned long long)np);
     if (a.fp_mode) {
         const uint64_t s1 = block_reduce_sum<BLOCK, uint64_t>(fsum, sh);
         const uint64_t s2 = block_reduce_sum<BLOCK, uint64_t>(fsum2, sh);
         const uint64_t s3 = block_reduce_sum<BLOCK, uint64_t>(fcs, sh);
+        const uint64_t nr = block_reduce_sum<BLOCK, uint64_t>(frows, sh);
         if (threadIdx.x == 0) {
             atomicAdd((unsigned long long*)&a.sc->fp_sum, (unsigned long long)s1);
             atomicAdd((unsigned long long*)&a.sc->fp_sum2, (unsigned long long)s2);
             if (COUNT) atomicAdd((unsigned long long*)&a.sc->fp_csum, (unsigned long long)s3);
+            if (nr) atomicAdd((unsigned long long*)&a.sc->out_n, (unsigned long long)nr);
         }
     }
-    fsum = fsum2 = fcs = npaths = 0;
+    fsum = fsum2 = fcs = npaths = frows = 0;
 }
 
 // The fingerprint table in LDS: the 32 position factors, then for each byte
@@ -899,7 +925,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     const uint32_t tid = threadIdx.x;
     if (a.fp_mode) fp_table_init<BLOCK>(reinterpret_cast<uint64_t*>(smem));
     for (uint32_t w = tid; w < a.tile_words; w += BLOCK) L.tile[w] = 0;
-    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
+    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0, frows = 0;
     // with a row list the short rows went one per wavefront (k_value_rows):
     // the tasks are (big_x[r], t) for the other rows
     const bool lists = a.big_x != nullptr;
@@ -933,10 +959,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         }
         const uint32_t r = task / a.ntiles;
         const uint32_t x = lists ? a.big_x[r] : r;
-        row_task<BLOCK, COUNT, ZID>(a, x, task - r * a.ntiles, L, fsum, fsum2, fcs, npaths);
+        row_task<BLOCK, COUNT, ZID>(a, x, task - r * a.ntiles, L, fsum, fsum2, fcs, npaths, frows);
     }
 
-    flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);
+    flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, frows, sh);
 }
 
 // Many join instances in one persistent launch fed by one device work
@@ -959,7 +985,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
 
     const uint32_t tid = threadIdx.x;
     const uint32_t total = task_start[n_inst];
-    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0;
+    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0, frows = 0;
     int32_t cur = -1;
     ExpandArgs a;
     // every instance of a batch has the batch's output mode
@@ -985,7 +1011,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
             else hi = mid;
         }
         if ((int32_t)lo != cur) {
-            if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);
+            if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, frows, sh);
             cur = (int32_t)lo;
             a = args[lo];
             // the previous instance's touched list / staging area may overlap this tile
@@ -996,10 +1022,11 @@ __global__ __launch_bounds__(BLOCK) void k_expand_batch(const ExpandArgs* __rest
         const uint32_t x = local / a.ntiles;
         const Lds L{reinterpret_cast<const uint64_t*>(smem), dyn, dyn + a.tile_words,
                     dyn + a.tile_words + a.touched_cap, sh, sh32, s_wtot, &s_nt, &s_gbase, &s_cursor};
-        if (a.id2z == nullptr) row_task<BLOCK, COUNT, true>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
-        else row_task<BLOCK, COUNT, false>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths);
+        if (a.id2z == nullptr)
+            row_task<BLOCK, COUNT, true>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths, frows);
+        else row_task<BLOCK, COUNT, false>(a, x, local - x * a.ntiles, L, fsum, fsum2, fcs, npaths, frows);
     }
-    if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, sh);
+    if (cur >= 0) flush_stats<BLOCK, COUNT>(a, fsum, fsum2, fcs, npaths, frows, sh);
 }
 
 // Upper bound of each row's output: its S paths over all tiles (a product
@@ -1015,6 +1042,11 @@ __global__ __launch_bounds__(256) void k_row_bound(ExpandArgs a, uint32_t cap, u
                                   : a.rowptrS[(y + 1u) * a.ntiles] - a.rowptrS[y * a.ntiles];
     }
     bound[x] = (uint32_t)min<uint64_t>(b, cap);
+}
+
+__global__ void k_fp_word_factors(uint32_t* __restrict__ out, uint32_t n, uint32_t w0) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = zword_factor((w0 + k) & 0x07FFFFFFu);
 }
 
 // Zero the expansion-owned scalars of every instance of a batch (a kernel, so
@@ -1081,6 +1113,10 @@ int launch(const ExpandArgs& a, int block, int cu_count, hipStream_t st) {
 }  // namespace
 
 namespace dim3h {
+
+void fp_word_factors(uint32_t* out, uint32_t n, int32_t zlo, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fp_word_factors, dim3((n + 255u) / 256u), dim3(256), 0, st, out, n, (uint32_t)zlo >> 5);
+}
 
 size_t expand_lds_bytes(uint32_t tile_words, uint32_t touched_cap, uint32_t stage_cap, uint32_t block, bool fp) {
     const uint32_t area = stage_cap > 3u * block + 1u ? stage_cap : 3u * block + 1u;
