@@ -149,7 +149,7 @@ enum SlotId {
     WS_GC_ROWL,
     WS_GC_CURL,
     WS_GC_COLL,
-    WS_FP_ZW,  // word factors of a fingerprint over a dense z range (plain layout)
+    WS_FP_ZW,  // z word factors of a fingerprint over a dense z range
     WS_BATCH_ARGS,
     WS_BATCH_START,
     WS_BATCH_SCL,
@@ -188,7 +188,7 @@ struct QueryPlan {
     uint32_t ntiles = 1, tile_size = 0, tile_words = 0;
     bool x_range = false, y_range = false, z_ident = false;
     bool hot = false;  // Scalars::hot holds the hot-value samples (large inputs)
-    bool fp_zw = false;  // WS_FP_ZW holds the word factors of the tiles (fingerprint, dense z, plain layout)
+    bool fp_zw = false;  // WS_FP_ZW holds the factors of the tiles' z words (fingerprint, dense z)
     int32_t zlo = 0;
     dim3h::KeyMap mx{}, my{}, mz{};
     uint32_t touched_cap = 0, stage_cap = 0;
@@ -1228,13 +1228,12 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     }
     if (o.flags & DIM3_OPT_EXCLUDE_EQUAL)
         dim3h::x2z_self(id2x, P.nx, P.mz, degZ, P.z_ident ? nullptr : exZ, x2z, st);
-    // a fingerprint over a dense z range in the plain layout takes each tile
-    // word's factor from a table (the allocation-free entry computes them)
-    P.fp_zw = o.output_mode == DIM3_OUT_FINGERPRINT && P.z_ident && P.plain && !P.count_mode && !P.agg &&
-              c->arena == nullptr;
+    // a fingerprint over a dense z range takes each z word's factor from a
+    // table (the allocation-free entry computes them)
+    P.fp_zw = o.output_mode == DIM3_OUT_FINGERPRINT && P.z_ident && !P.agg && c->arena == nullptr;
     if (P.fp_zw) {
         uint32_t* zw;
-        const uint64_t nw = (uint64_t)P.ntiles * P.tile_words;
+        const uint64_t nw = ((uint64_t)P.ntiles * P.tile_size + 31u) / 32u;
         TRY(ws(c, WS_FP_ZW, nw, &zw));
         dim3h::fp_word_factors(zw, (uint32_t)nw, P.zlo, st);
     }

@@ -579,9 +579,9 @@ struct ExpandArgs {
     uint32_t stage_cap;
     uint32_t fp_mode;
     uint32_t swz;  // project tile bit layout: log2 of the block width in words (tile_words a multiple of it), 0 plain
-    // fingerprint of a dense z range in the plain layout (nullable): the word
-    // factor of word k of the tiles, zword_factor((zlo >> 5) + k), for
-    // ntiles * tile_words words (fp_word_factors)
+    // fingerprint of a dense z range (nullable): the factor of z word
+    // (zlo >> 5) + j, zword_factor((zlo >> 5) + j), for the words the tiles
+    // cover (fp_word_factors)
     const uint32_t* fp_zw;
     // the (x, z-tile) tasks of this launch: x * ntiles + t in [task_lo, task_hi)
     // (all rows, or one chunk of rows of a streamed query)

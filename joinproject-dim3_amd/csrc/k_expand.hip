@@ -650,9 +650,12 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
             frows += units_of<COUNT>(v);
             return make_uint2(w, v);
         };
+        // the table's word j is z word (zlo >> 5) + j: local z offset e is in
+        // word (zbase + e) >> 5 (a project tile's zbase is a multiple of 32)
+        const uint32_t* fzw = ZID ? a.fp_zw : nullptr;
         if (!COUNT && ZID && lm == 0u) {
-            if (a.fp_zw != nullptr) {
-                const uint32_t* fzw = a.fp_zw + t * W;
+            if (fzw != nullptr) {
+                fzw += zbase >> 5;
                 for (uint32_t i = cfirst; i < count; i += cstep) {
                     const uint2 wv = take(i);
                     const uint32_t v = wv.y;
@@ -683,7 +686,8 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                 if (COUNT) {
                     if (v) {
                         const uint32_t u = ZID ? zoff + w : (uint32_t)id2z[w];
-                        const uint64_t g = (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
+                        const uint32_t fw = fzw != nullptr ? fzw[(zbase + w) >> 5] : zword_factor(u >> 5);
+                        const uint64_t g = (uint64_t)fw * fpt[u & 31u];
                         S1 += g;
                         S2 += (uint64_t)v * g;
                         fcs += v;
@@ -694,7 +698,8 @@ __device__ __forceinline__ void row_task(const ExpandArgs& a, uint32_t x, uint32
                         v &= v - 1u;
                         const uint32_t e = tl_base(w, lm) + (b << lsh);
                         const uint32_t u = ZID ? zoff + e : (uint32_t)id2z[e];
-                        S1 += (uint64_t)zword_factor(u >> 5) * fpt[u & 31u];
+                        const uint32_t fw = fzw != nullptr ? fzw[(zbase + e) >> 5] : zword_factor(u >> 5);
+                        S1 += (uint64_t)fw * fpt[u & 31u];
                     }
                 }
             }
