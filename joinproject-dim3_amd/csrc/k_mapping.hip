@@ -177,10 +177,10 @@ __global__ __launch_bounds__(kThreads) void k_build_y(const int2* __restrict__ R
     tally_flush<kThreads>(degR, tally, tslot, kTallyKeys);
 }
 
-// The most frequent keys of 4096 evenly spaced tuples of T (one workgroup):
-// out = {count, keys...}, up to kHotKeys keys seen at least kMinSeen times,
-// most frequent first.
-template <bool X>  // sample T.x instead of T.y
+// The most frequent values of T.y (X: of T.x) among 4096 evenly spaced
+// tuples of T (one workgroup): out = {count, values...}, up to kHotKeys
+// values seen at least kMinSeen times, most frequent first.
+template <bool X>
 __global__ __launch_bounds__(1024) void k_hot_key(const int2* __restrict__ T, uint64_t n, int32_t* out) {
     constexpr uint32_t kSample = 4096, kSlots = 8192, kMinSeen = 8;
     __shared__ uint32_t keys[kSlots];
