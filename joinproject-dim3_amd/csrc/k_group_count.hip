@@ -382,7 +382,8 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
 constexpr uint32_t kLBits = 9;
 constexpr uint32_t kLSlots = 1u << kLBits;
 constexpr uint32_t kLCap = kLSlots / 2;
-constexpr uint32_t kLChunk = 16;  // tasks per claim of the work counter
+constexpr uint32_t kLChunk = 16;    // tasks per segment table (one lane each, and their row ends)
+constexpr uint32_t kLClaimMax = 128;  // tasks per claim of the work counter, at most
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 struct LightLds {
@@ -430,7 +431,7 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
                                                   uint32_t ny, const uint64_t* __restrict__ hm,
                                                   const uint4* __restrict__ zinfo, const int32_t* __restrict__ id2x,
                                                   uint32_t nx, uint32_t ntiles, uint32_t tile_size, uint32_t nz,
-                                                  Scalars* sc) {
+                                                  Scalars* sc, uint32_t claim) {
     __shared__ LightLds wl[4];
     __shared__ uint64_t sh[5];
     const uint32_t lane = threadIdx.x & 63u;
@@ -442,26 +443,77 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
     lds_wave_sync();
     const uint32_t ntask = nx * ntiles;
     uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0, nrows = 0;
-    uint32_t task = 0, task_end = 0;
+    // claims of `claim` tasks (a multiple of kLChunk): the work counter is
+    // one address, whose atomics are served one at a time
+    uint32_t task = 0, task_end = 0, claim_end = 0;
+    // A chunk's tasks in one tile have consecutive x, so their R entries are
+    // one range: its row bounds (lane i: rowptrR of the chunk's i-th x) and,
+    // for at most 128 entries, their light segments (entry e in lane e & 63 of
+    // register e >> 6) are read once per chunk, and a task takes its own by
+    // lane permutes instead of a chain of dependent loads.
+    uint32_t cfirst = 0, ct = 0, cx = 0, crp = 0, crb0 = 0, csb0 = 0, clen0 = 0, csb1 = 0, clen1 = 0;
+    bool ctab = false;  // wave-uniform
     for (;;) {
         if (task == task_end) {
-            uint32_t t0 = 0;
-            if (lane == 0) t0 = atomicAdd(&sc->task, kLChunk);
-            task = (uint32_t)__builtin_amdgcn_readfirstlane((int)t0);
-            if (task >= ntask) break;
-            task_end = min(task + kLChunk, ntask);
+            if (task == claim_end) {
+                uint32_t t0 = 0;
+                if (lane == 0) t0 = atomicAdd(&sc->task, claim);
+                task = (uint32_t)__builtin_amdgcn_readfirstlane((int)t0);
+                if (task >= ntask) break;
+                claim_end = min(task + claim, ntask);
+            }
+            task_end = min(task + kLChunk, claim_end);
+            const uint32_t tc = task / nx, xc = task - tc * nx, n = task_end - task;
+            ctab = (task_end - 1u) / nx == tc;  // then xc + n <= nx
+            if (ctab) {
+                crp = lane <= n ? rowptrR[xc + lane] : 0u;
+                crb0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)crp);
+                const uint32_t E = (uint32_t)__builtin_amdgcn_readlane((int)crp, (int)n) - crb0;
+                ctab = E <= 128u;
+                if (ctab) {
+                    csb0 = clen0 = csb1 = clen1 = 0;
+                    if (lane < E) light_segment(colR, rowL, ny, tc, crb0 + lane, csb0, clen0);
+                    if (lane + 64u < E) light_segment(colR, rowL, ny, tc, crb0 + 64u + lane, csb1, clen1);
+                    cfirst = task;
+                    ct = tc;
+                    cx = xc;
+                }
+            }
         }
         // tile-major: the waves running at one time share a tile's light rows
         // and z info in L2
         const uint32_t ti = task++;
-        const uint32_t t = ti / nx, x = ti - t * nx;
-        const uint32_t rb = rowptrR[x], re = rowptrR[x + 1];
+        uint32_t t, x, rb, re;
+        if (ctab) {
+            t = ct;
+            x = cx + (ti - cfirst);
+            rb = (uint32_t)__builtin_amdgcn_readlane((int)crp, (int)(ti - cfirst));
+            re = (uint32_t)__builtin_amdgcn_readlane((int)crp, (int)(ti - cfirst + 1u));
+        } else {
+            t = ti / nx;
+            x = ti - t * nx;
+            rb = rowptrR[x];
+            re = rowptrR[x + 1];
+        }
         const bool short_row = re - rb <= 64u;  // wave-uniform
         // the task's light paths; a short row's segment table stays in LDS
         uint32_t P = 0;
         if (short_row) {
             uint32_t sb = 0, len = 0;
-            if (rb + lane < re) light_segment(colR, rowL, ny, t, rb + lane, sb, len);
+            if (ctab) {
+                const uint32_t e = rb - crb0 + lane;  // < 128 for the row's lanes
+                const int src = (int)((e & 63u) << 2);
+                const uint32_t s0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)csb0);
+                const uint32_t s1 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)csb1);
+                const uint32_t l0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)clen0);
+                const uint32_t l1 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)clen1);
+                if (rb + lane < re) {
+                    sb = e < 64u ? s0 : s1;
+                    len = e < 64u ? l0 : l1;
+                }
+            } else if (rb + lane < re) {
+                light_segment(colR, rowL, ny, t, rb + lane, sb, len);
+            }
             const uint32_t inc = wave_scan_u32(len);
             P = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
             W.sb[lane] = sb;
@@ -498,6 +550,9 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
                     W.pre[lane] = inc - len;
                     lds_wave_sync();
                 }
+                // search steps: the highest power of two below the table's entries
+                const uint32_t nseg = min(64u, re - c0);
+                const uint32_t sbit = nseg > 1u ? 1u << (31 - __builtin_clz(nseg - 1u)) : 0u;
                 for (uint32_t p0 = 0; p0 < PC; p0 += 64u) {
                     if (nt > kLCap) {  // too many groups for this width: retry narrower
                         over = true;
@@ -506,9 +561,9 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
                     const uint32_t p = p0 + lane;
                     const bool in = p < PC;
                     // the last segment whose path prefix is <= p holds path p
+                    // (entries past the table's nseg have prefix PC)
                     uint32_t seg = 0;
-#pragma unroll
-                    for (uint32_t step = 32; step > 0; step >>= 1)
+                    for (uint32_t step = sbit; step > 0; step >>= 1)
                         if (W.pre[seg + step] <= p) seg += step;
                     const uint32_t z = in ? (uint32_t)colL[W.sb[seg] + (p - W.pre[seg])] : 0u;
                     const bool ok = in && z >= zcur && z < zhi;
@@ -649,17 +704,22 @@ int gc_light(const uint32_t* rowptrR, const int32_t* colR, const uint32_t* rowL,
              const uint64_t* hm, const void* zinfo, const int32_t* id2x, uint32_t nx, uint32_t ntiles,
              uint32_t tile_size, uint32_t nz, int cu_count, Scalars* sc, hipStream_t st) {
     if (nx == 0) return 0;
-    if ((uint64_t)nx * ntiles >= (1ull << 32) - kLChunk) return (int)hipErrorInvalidValue;
     int occ = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_gc_light, 256, 0);
     if (e != hipSuccess) return (int)e;
     if (occ < 1) occ = 1;
+    const uint64_t ntask = (uint64_t)nx * ntiles;
     uint64_t grid = (uint64_t)cu_count * (uint64_t)occ;
-    const uint64_t waves = ((uint64_t)nx * ntiles + kLChunk - 1) / kLChunk;
+    const uint64_t waves = (ntask + kLChunk - 1) / kLChunk;
     if (grid * 4 > waves) grid = (waves + 3) / 4;
     if (grid == 0) grid = 1;
+    // about 16 claims per wavefront at least
+    uint32_t claim = (uint32_t)std::min<uint64_t>(kLClaimMax, ntask / (grid * 4 * 16) / kLChunk * kLChunk);
+    claim = std::max(claim, kLChunk);
+    // the counter ends below ntask + one claim per wavefront
+    if (ntask + grid * 4 * claim >= (1ull << 32)) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gc_light, dim3((unsigned)grid), dim3(256), 0, st, rowptrR, colR, rowL, colL, ny, hm,
-                       (const uint4*)zinfo, id2x, nx, ntiles, tile_size, nz, sc);
+                       (const uint4*)zinfo, id2x, nx, ntiles, tile_size, nz, sc, claim);
     return (int)hipGetLastError();
 }
 
