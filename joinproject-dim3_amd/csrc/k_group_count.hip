@@ -373,9 +373,10 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
 // each key's row in tile t of the light S CSR) are counted per z ID in
 // a private LDS hash table of kLSlots slots.  A task of at most kLCap paths
 // runs in one pass.  A longer one runs in passes over z ranges whose width
-// is guessed from its path count (kLCap / 2 paths per pass if spread
-// evenly); a pass whose groups pass kLCap stops, clears its slots and is
-// retried at half the width, so a pass never lists more than kLCap + 64
+// is guessed from its path count (7 kLCap / 8 paths per pass if spread
+// evenly: every pass enumerates all the task's paths again, so fewer passes
+// beat the retries they cause); a pass whose groups pass kLCap stops, clears
+// its slots and is retried at half the width, so a pass never lists more than kLCap + 64
 // groups (the table stays under two thirds full) and a width of kLCap IDs
 // always fits.  A row of at most 64 R entries keeps its segment table in
 // LDS from the first sweep to the last pass.
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
         const uint64_t k1 = xfactor1(xv), k2 = xfactor2(xv);
         const uint32_t zt0 = t * tile_size, zt1 = min(zt0 + tile_size, nz);
         uint32_t width = zt1 - zt0;
-        if (P > kLCap) width = (uint32_t)max<uint64_t>(1, (uint64_t)(zt1 - zt0) * (kLCap / 2) / P);
+        if (P > kLCap) width = (uint32_t)max<uint64_t>(1, (uint64_t)(zt1 - zt0) * (kLCap * 7 / 8) / P);
         uint32_t zcur = zt0;
         while (zcur < zt1) {
             const uint32_t zhi = zt1 - zcur <= width ? zt1 : zcur + width;
