@@ -445,13 +445,13 @@ def config_bench(args, d, torch):
             raise RuntimeError("group counts and path sums disagree")
         group_counts = {"path_sums_ms_per_query": 1000.0 * tp, "light_paths": int(st.paths_sparse)}
         if st.dense_mode == 2:
-            # the MFMA box: executed int8 ops (16 product + 8 reduction MFMAs
-            # of 16x16x64 per 64 x 64 cells: 192 ops per cell, rows padded to
+            # the MFMA box: executed int8 ops (16 product + 4 reduction MFMAs
+            # of 16x16x64 per 64 x 64 cells: 160 ops per cell, rows padded to
             # 256 and columns to 64)
             mp = (int(st.nx_heavy) + 255) // 256 * 256
             np_ = (int(st.nz_heavy) + 63) // 64 * 64
             box_ms = float(np.mean([r.stats.ms_dense for r in ress]))
-            ops = 192.0 * mp * np_
+            ops = 160.0 * mp * np_
             tops = ops / (box_ms * 1e-3) / 1e12
             group_counts.update({"box_cells": int(st.nx_heavy) * int(st.nz_heavy), "box_kernel_ms": box_ms,
                                  "box_int8_ops": ops, "light_kernel_ms": exp_ms - box_ms})

@@ -1239,7 +1239,19 @@ int prepare_plan(dim3_ctx* c, const dim3_opts& o, hipStream_t st, int force_wgs)
     }
     LAUNCHCHK(c, "csr");
     HIPCHK(c, hipEventRecord(c->ev[3], st));
-    if (P.gc) TRY(gc_prepare(c, slotS, slotSz, degR, degS, exY, exZ, ny1, st));
+    if (P.gc) {
+        TRY(gc_prepare(c, slotS, slotSz, degR, degS, exY, exZ, ny1, st));
+        // sum2 and csum are sums over the join paths (sum over keys of the
+        // R-side k2 sum times the S-side g sum; csum = the path count), so
+        // the group kernels form only each group's presence terms
+        uint64_t *kr, *gs;
+        TRY(ws(c, WS_AGG_KR, ny1, &kr));
+        TRY(ws(c, WS_AGG_GS, ny1, &gs));
+        dim3h::fill_u64(&sc->agg_sum2, 1, 0, st);
+        dim3h::path_sums(P.R, nR, slotR, dS, nS, slotS, (uint32_t)ny1, kr, gs, &sc->agg_sum2, st,
+                         P.hot ? sc->hot : nullptr);
+        LAUNCHCHK(c, "group-count path sums");
+    }
     if (P.dense != DIM3_DENSE_OFF) {
         // the product of the heavy keys: Dbits (project) or Cint (COUNT)
         uint64_t* overflow = &sc->spare[1];
@@ -1491,8 +1503,9 @@ void fill_result(dim3_ctx* c, dim3_result* res, const Scalars& s, bool timing) {
     res->swapped = P.swapped ? 1u : 0u;
     res->fp.count = res->n;
     res->fp.sum = P.empty ? 0 : s.fp_sum;
-    res->fp.sum2 = P.empty ? 0 : P.agg_fp ? s.agg_sum2 : s.fp_sum2;
-    res->fp.csum = P.empty ? 0 : P.agg_fp ? P.out_j : s.fp_csum;
+    const bool path_sums = P.agg_fp || P.gc;  // sum2 and csum from the join paths
+    res->fp.sum2 = P.empty ? 0 : path_sums ? s.agg_sum2 : s.fp_sum2;
+    res->fp.csum = P.empty ? 0 : path_sums ? P.out_j : s.fp_csum;
     dim3_stats& st = res->stats;
     memset(&st, 0, sizeof(st));
     st.n_r = P.nR;

@@ -12,31 +12,34 @@
 //                                    tuple with heavy key h, K <= 64 keys)
 //   c_l = join paths of (x, z) through light keys.
 // The query's fingerprint (count, sum, sum2, csum) is a sum of per-group
-// terms (1, k1(x) g(z), k2(x) c g(z), c) over the groups with c > 0, so it
-// splits exactly into
-//   dense part (k_gc_dense): the terms of c_d over every cell of the box of
-//     rows with hm != 0 and columns with zm != 0, where c_d is
+// terms (1, k1(x) g(z), k2(x) c g(z), c) over the groups with c > 0.  The
+// last two are linear in c, so they are sums over the join paths: sum2 =
+// sum over keys y of (sum of k2(x) over y's R tuples) x (sum of g(z) over
+// y's S tuples) and csum = the path count, both formed in the planning
+// (path_sums).  The first two need each group's presence, and split
+// exactly into
+//   dense part (k_gc_dense): the terms of [c_d > 0] over every cell of the
+//     box of rows with hm != 0 and columns with zm != 0, where c_d is
 //     A[x][h] * B[z][h] summed over h on v_mfma_i32_16x16x64_i8 (A, B the
 //     0/1 bytes of hm, zm: exact, c_d <= 64);
-//   light part (k_gc_light): per light group with c_l > 0, its count c_l
-//     formed in LDS, c_d = popcount(hm(x) & zm(z)) looked up, and the terms
-//     (c_d == 0 ? 1 : 0, c_d == 0 ? k1 g : 0, k2 c_l g, c_l): a group the
-//     dense part already counted adds its light paths only.
-// The per-cell fingerprint of the dense part is itself a matrix product:
+//   light part (k_gc_light): per light group (c_l > 0) found in an LDS
+//     table, c_d = popcount(hm(x) & zm(z)) looked up, and the terms
+//     (1, k1 g) when c_d == 0: a group the dense part already counted adds
+//     nothing.
+// The per-cell presence sum of the dense part is itself a matrix product:
 // with g(z) = sum_b 2^(7 b) g_b(z) in 7-bit chunks (b < 10),
-//   sum_{x,z} k2(x) c(x,z) g(z) = sum_x k2(x) sum_b 2^(7 b) E2[b][x],
-//   E2[b][x] = sum_z g_b(z) c(x, z),
-// and E2 is a second int8 MFMA whose k dimension is z: its B operand is the
-// count tiles of the first product (computed transposed, C^T[z][x], so the
-// lane holding z = 4 q .. 4 q + 3 of a tile holds the operand bytes of
-// k = 16 q .. 16 q + 15 once the four z tiles of a step are packed to bytes
-// in registers) and its A operand the chunk bytes of the step's 64 z
-// factors, laid out per lane in advance.  E2 accumulates in the MFMA over
-// the whole z sweep of a task; k2(x) multiplies once per row at the end.
-// Row b = 10 of the chunk operand is all ones, so E2[10][x] = sum_z c
-// (csum).  E1 does the same over the bytes -128 [c > 0] (sum and, by its
-// ones row, count, each -128 times its exact value until the end).  Per
-// 4096 cells a wavefront issues 16 product and 8 reduction MFMAs and about
+//   sum_{x,z} k1(x) [c(x,z) > 0] g(z) = sum_x k1(x) sum_b 2^(7 b) E1[b][x],
+//   E1[b][x] = sum_z g_b(z) [c(x, z) > 0],
+// and E1 is a second int8 MFMA whose k dimension is z: its B operand is the
+// flag bytes (-128 where c > 0) of the count tiles of the first product
+// (computed transposed, C^T[z][x], so the lane holding z = 4 q .. 4 q + 3
+// of a tile holds the operand bytes of k = 16 q .. 16 q + 15 once the four
+// z tiles of a step are packed to bytes in registers) and its A operand the
+// chunk bytes of the step's 64 z factors, laid out per lane in advance.  E1
+// accumulates in the MFMA over the whole z sweep of a task (-128 times its
+// exact value); k1(x) multiplies once per row at the end.  Row b = 10 of
+// the chunk operand is all ones, so E1[10][x] counts the row's groups.  Per
+// 4096 cells a wavefront issues 16 product and 4 reduction MFMAs and about
 // 90 VALU instructions; no count leaves the registers.
 #include "dim3_internal.h"
 
@@ -235,8 +238,9 @@ __global__ void k_gc_build_gchunks(const uint64_t* __restrict__ gz, uint32_t nst
 // current step computes.  Per step and 16-row group: four product MFMAs give
 // the transposed count tiles C^T[z][x] (z = 16 u + 4 q + e in lane (x, q)),
 // packed to bytes (count <= 64) and flagged (count > 0, as the int8 -128)
-// in registers, then two reduction MFMAs add E2 += G * counts,
-// E1 += G * flags.  Tasks are
+// in registers, then a reduction MFMA adds E1 += G * flags (the count terms
+// sum2 and csum are sums over the join paths, formed from the key sums
+// instead).  Tasks are
 // numbered column-chunk major and dealt to the 8 XCDs in contiguous runs (a
 // block goes to XCD blockIdx % 8), so an XCD's workgroups share a column
 // chunk's operands in its L2.  Padding: A8 / xk cover Mp = 256 n_xb rows,
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
                                                       uint32_t n_xb, const int8_t* __restrict__ B8,
                                                       const uint4* __restrict__ G, uint32_t Np, uint32_t cols,
                                                       uint32_t n_tasks, uint32_t per_xcd, Scalars* sc) {
-    __shared__ uint64_t red[4][4];
+    __shared__ uint64_t red[4][2];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     const uint32_t q = lane >> 4, li = lane & 15u;
     const uint32_t task = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
@@ -258,12 +262,9 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
 #pragma unroll
     for (uint32_t gx = 0; gx < 4; gx++)
         xa[gx] = *reinterpret_cast<const v4i*>(A8 + (size_t)(x0 + 16u * gx + li) * kGcK + 16u * q);
-    v4i e1[4], e2[4];
+    v4i e1[4];
 #pragma unroll
-    for (uint32_t gx = 0; gx < 4; gx++) {
-        e1[gx] = v4i{0, 0, 0, 0};
-        e2[gx] = v4i{0, 0, 0, 0};
-    }
+    for (uint32_t gx = 0; gx < 4; gx++) e1[gx] = v4i{0, 0, 0, 0};
     const uint32_t z0 = zc * cols;
     const uint32_t z1 = min(z0 + cols, Np);
     // column operands of a step: tile u, lane (li, q) = column zs + 16 u + li,
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++)
                 c[u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(zf[u], xa[gx], v4i{0, 0, 0, 0}, 0, 0, 0);
-            v4i pk, fl;
+            v4i fl;
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
                 // counts <= 64: one byte each (v_perm_b32: selector bytes 0-3
@@ -297,13 +298,11 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
                 const uint32_t lo = __builtin_amdgcn_perm((uint32_t)c[u][1], (uint32_t)c[u][0], 0x0C0C0400u);
                 const uint32_t hi = __builtin_amdgcn_perm((uint32_t)c[u][3], (uint32_t)c[u][2], 0x0C0C0400u);
                 const uint32_t p = __builtin_amdgcn_perm(hi, lo, 0x05040100u);
-                pk[u] = (int)p;
                 // byte 0x80 (int8 -128) where the count is > 0 (no carry
                 // between bytes: count + 127 <= 191); E1 is -128 times the
                 // flag sums
                 fl[u] = (int)((p + 0x7F7F7F7Fu) & 0x80808080u);
             }
-            e2[gx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gf, pk, e2[gx], 0, 0, 0);
             e1[gx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gf, fl, e1[gx], 0, 0, 0);
         }
     };
@@ -328,42 +327,29 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
         load_step(min(zs + 3u * kGcColsStep, zlast), zfb, gfb);
     }
     // lane (li, q) of group gx: row x0 + 16 gx + li, chunk rows b = 4 q + e
-    uint64_t S1 = 0, S2 = 0, CNT = 0, CS = 0;
+    uint64_t S1 = 0, CNT = 0;
 #pragma unroll
     for (uint32_t gx = 0; gx < 4; gx++) {
-        const uint32_t x = x0 + 16u * gx + li;
-        const uint64_t k1 = xk[2u * x], k2 = xk[2u * x + 1u];
+        const uint64_t k1 = xk[2u * (x0 + 16u * gx + li)];
 #pragma unroll
         for (uint32_t e = 0; e < 4; e++) {
             const uint32_t b = 4u * q + e;
             // e1 = -128 x (a sum below 2^22): exact division
-            const uint64_t v1 = (uint64_t)(uint32_t)((-e1[gx][e]) >> 7), v2 = (uint64_t)(uint32_t)e2[gx][e];
-            if (b < 10u) {
-                S1 += k1 * (v1 << (7u * b));
-                S2 += k2 * (v2 << (7u * b));
-            } else if (b == 10u) {
-                CNT += v1;
-                CS += v2;
-            }
+            const uint64_t v1 = (uint64_t)(uint32_t)((-e1[gx][e]) >> 7);
+            if (b < 10u) S1 += k1 * (v1 << (7u * b));
+            else if (b == 10u) CNT += v1;
         }
     }
     S1 = wave_reduce_sum(S1);
-    S2 = wave_reduce_sum(S2);
     CNT = wave_reduce_sum(CNT);
-    CS = wave_reduce_sum(CS);
     if (lane == 0) {
         red[wid][0] = S1;
-        red[wid][1] = S2;
-        red[wid][2] = CNT;
-        red[wid][3] = CS;
+        red[wid][1] = CNT;
     }
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < 2) {
         const uint64_t v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-        unsigned long long* dst = (unsigned long long*)(threadIdx.x == 0   ? &sc->fp_sum
-                                                        : threadIdx.x == 1 ? &sc->fp_sum2
-                                                        : threadIdx.x == 2 ? &sc->out_n
-                                                                           : &sc->fp_csum);
+        unsigned long long* dst = (unsigned long long*)(threadIdx.x == 0 ? &sc->fp_sum : &sc->out_n);
         if (v) atomicAdd(dst, (unsigned long long)v);
     }
 }
@@ -389,7 +375,6 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 struct LightLds {
     uint32_t keys[kLSlots];
-    uint32_t cnt[kLSlots];
     uint32_t touched[kLCap + 64];
     uint32_t sb[64];
     uint32_t pre[64];
@@ -403,12 +388,15 @@ __device__ __forceinline__ void lds_wave_sync() {
 
 // z's slot: one compare-and-swap per probe claims an empty slot or returns
 // the key already there (z itself ends the probe too), so the probe loop has
-// no inner branch.
-__device__ __forceinline__ uint32_t light_slot(uint32_t* keys, uint32_t z) {
+// no inner branch; `fresh` tells the one lane that claimed it.
+__device__ __forceinline__ uint32_t light_slot(uint32_t* keys, uint32_t z, bool& fresh) {
     uint32_t s = (z * 0x9E3779B1u) >> (32u - kLBits);
     for (;;) {
         const uint32_t k = atomicCAS(&keys[s], kNone, z);
-        if (k == kNone || k == z) return s;
+        if (k == kNone || k == z) {
+            fresh = k == kNone;
+            return s;
+        }
         s = (s + 1u) & (kLSlots - 1u);
     }
 }
@@ -439,11 +427,10 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
     LightLds& W = wl[threadIdx.x >> 6];
     for (uint32_t s = lane; s < kLSlots; s += 64u) {
         W.keys[s] = kNone;
-        W.cnt[s] = 0;
     }
     lds_wave_sync();
     const uint32_t ntask = nx * ntiles;
-    uint64_t fsum = 0, fsum2 = 0, fcs = 0, npaths = 0, nrows = 0;
+    uint64_t fsum = 0, npaths = 0, nrows = 0;
     // claims of `claim` tasks (a multiple of kLChunk): the work counter is
     // one address, whose atomics are served one at a time
     uint32_t task = 0, task_end = 0, claim_end = 0;
@@ -531,7 +518,7 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
         npaths += lane == 0 ? P : 0u;
         const uint64_t hmx = hm[x];
         const int32_t xv = id2x[x];
-        const uint64_t k1 = xfactor1(xv), k2 = xfactor2(xv);
+        const uint64_t k1 = xfactor1(xv);
         const uint32_t zt0 = t * tile_size, zt1 = min(zt0 + tile_size, nz);
         uint32_t width = zt1 - zt0;
         if (P > kLCap) width = (uint32_t)max<uint64_t>(1, (uint64_t)(zt1 - zt0) * (kLCap * 7 / 8) / P);
@@ -568,46 +555,36 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
                         if (W.pre[seg + step] <= p) seg += step;
                     const uint32_t z = in ? (uint32_t)colL[W.sb[seg] + (p - W.pre[seg])] : 0u;
                     const bool ok = in && z >= zcur && z < zhi;
-                    uint32_t old = 1u, slot = 0;
-                    if (ok) {
-                        slot = light_slot(W.keys, z);
-                        old = atomicAdd(&W.cnt[slot], 1u);
-                    }
-                    const uint64_t m = __ballot(old == 0u);
-                    if (old == 0u) W.touched[nt + mbcnt64(m)] = slot;
+                    uint32_t slot = 0;
+                    bool fresh = false;
+                    if (ok) slot = light_slot(W.keys, z, fresh);
+                    const uint64_t m = __ballot(fresh);
+                    if (fresh) W.touched[nt + mbcnt64(m)] = slot;
                     nt += (uint32_t)__popcll(m);
                 }
                 if (!short_row) lds_wave_sync();  // the table is rewritten by the next 64 R entries
             }
             if (over) {
-                for (uint32_t j = lane; j < nt; j += 64u) {
-                    const uint32_t s = W.touched[j];
-                    W.cnt[s] = 0;
-                    W.keys[s] = kNone;
-                }
+                for (uint32_t j = lane; j < nt; j += 64u) W.keys[W.touched[j]] = kNone;
                 width = max(1u, width >> 1);
                 lds_wave_sync();
                 continue;
             }
-            // emission: the light groups and their dense counts
-            uint64_t S1 = 0, S2 = 0;
+            // emission: the groups of light paths only (the others are
+            // counted in the box)
+            uint64_t S1 = 0;
             uint32_t T = 0;
             for (uint32_t j = lane; j < nt; j += 64u) {
                 const uint32_t s = W.touched[j];
-                const uint32_t cl = W.cnt[s];
                 const uint4 zi = zinfo[W.keys[s]];
                 const uint64_t zmz = ((uint64_t)zi.y << 32) | zi.x, g = ((uint64_t)zi.w << 32) | zi.z;
-                if ((hmx & zmz) == 0ull) {  // a group of light paths only
+                if ((hmx & zmz) == 0ull) {
                     S1 += g;
                     T += 1u;
                 }
-                S2 += (uint64_t)cl * g;
-                fcs += cl;
-                W.cnt[s] = 0;
                 W.keys[s] = kNone;
             }
             fsum += k1 * S1;
-            fsum2 += k2 * S2;
             nrows += T;
             lds_wave_sync();
             zcur = zhi;
@@ -615,14 +592,10 @@ __global__ __launch_bounds__(256) void k_gc_light(const uint32_t* __restrict__ r
     }
     const uint64_t np = block_reduce_sum<256, uint64_t>(npaths, sh);
     const uint64_t s1 = block_reduce_sum<256, uint64_t>(fsum, sh);
-    const uint64_t s2 = block_reduce_sum<256, uint64_t>(fsum2, sh);
-    const uint64_t s3 = block_reduce_sum<256, uint64_t>(fcs, sh);
     const uint64_t nr = block_reduce_sum<256, uint64_t>(nrows, sh);
     if (threadIdx.x == 0) {
         if (np) atomicAdd((unsigned long long*)&sc->paths, (unsigned long long)np);
         if (s1) atomicAdd((unsigned long long*)&sc->fp_sum, (unsigned long long)s1);
-        if (s2) atomicAdd((unsigned long long*)&sc->fp_sum2, (unsigned long long)s2);
-        if (s3) atomicAdd((unsigned long long*)&sc->fp_csum, (unsigned long long)s3);
         if (nr) atomicAdd((unsigned long long*)&sc->out_n, (unsigned long long)nr);
     }
 }
