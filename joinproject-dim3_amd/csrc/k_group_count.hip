@@ -276,12 +276,23 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
         (void*)(G + (size_t)(z0 / kGcColsStep) * 64u), (short)0, (int)((z1 - z0) / kGcColsStep * 1024u),
         kRawBufferWord3);
     const int vb = (int)(li * kGcK + 16u * q), vg = (int)(lane * 16u);
-    auto load_step = [&](uint32_t zs, v4i* zf, v4i& gf) {
-        const int so = (int)((zs - z0) * kGcK);
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++)
-            zf[u] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rb8, vb + (int)(u * 16u * kGcK), so, 0));
-        gf = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rg, vg, (int)((zs - z0) / kGcColsStep * 1024u), 0));
+    // The step's column operands are read from L2 once per workgroup and
+    // shared through LDS (two buffers): thread (u = wid, lane) fetches tile
+    // u's bytes for lane position `lane`, wavefront 0 also the reduction
+    // operand; after one barrier per step every wavefront reads its lanes'
+    // 16-byte pieces.
+    __shared__ v4i sB[2][4][64];
+    __shared__ v4i sG[2][64];
+    const uint32_t nsteps = (z1 - z0) / kGcColsStep;
+    v4i nb, ng;
+    auto fetch = [&](uint32_t s) {
+        const int so = (int)(s * kGcColsStep * kGcK);
+        nb = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rb8, vb + (int)(wid * 16u * kGcK), so, 0));
+        if (wid == 0) ng = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rg, vg, (int)(s * 1024u), 0));
+    };
+    auto stage = [&](uint32_t k) {
+        sB[k][wid][lane] = nb;
+        if (wid == 0) sG[k][lane] = ng;
     };
     auto compute = [&](const v4i* zf, const v4i& gf) {
 #pragma unroll
@@ -306,25 +317,19 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
             e1[gx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gf, fl, e1[gx], 0, 0, 0);
         }
     };
-    // two operand buffers in turn: a buffer is refilled (two steps ahead)
-    // right after its step's MFMAs, so each load has the other buffer's step
-    // to arrive (the address clamped to the last step, so the loads are
-    // unconditional; scheduling barriers keep the compiler from moving them)
-    const uint32_t zlast = z1 - kGcColsStep;
-    v4i zfa[4], gfa, zfb[4], gfb;
-    load_step(z0, zfa, gfa);
-    load_step(min(z0 + kGcColsStep, zlast), zfb, gfb);
-    for (uint32_t zs = z0;; zs += 2u * kGcColsStep) {
-        __builtin_amdgcn_sched_barrier(0);
-        compute(zfa, gfa);
-        if (zs + kGcColsStep >= z1) break;
-        __builtin_amdgcn_sched_barrier(0);
-        load_step(min(zs + 2u * kGcColsStep, zlast), zfa, gfa);
-        __builtin_amdgcn_sched_barrier(0);
-        compute(zfb, gfb);
-        if (zs + 2u * kGcColsStep >= z1) break;
-        __builtin_amdgcn_sched_barrier(0);
-        load_step(min(zs + 3u * kGcColsStep, zlast), zfb, gfb);
+    fetch(0);
+    stage(0);
+    __syncthreads();
+    for (uint32_t s = 0; s < nsteps; s++) {
+        const uint32_t k = s & 1u;
+        if (s + 1u < nsteps) fetch(s + 1u);  // workgroup-uniform
+        v4i zf[4], gf;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) zf[u] = sB[k][u][lane];
+        gf = sG[k][lane];
+        compute(zf, gf);
+        if (s + 1u < nsteps) stage(k ^ 1u);  // buffer k ^ 1 was last read before the previous barrier
+        __syncthreads();
     }
     // lane (li, q) of group gx: row x0 + 16 gx + li, chunk rows b = 4 q + e
     uint64_t S1 = 0, CNT = 0;
