@@ -31,16 +31,17 @@
 //   sum_{x,z} k1(x) [c(x,z) > 0] g(z) = sum_x k1(x) sum_b 2^(7 b) E1[b][x],
 //   E1[b][x] = sum_z g_b(z) [c(x, z) > 0],
 // and E1 is a second int8 MFMA whose k dimension is z: its B operand is the
-// flag bytes (-128 where c > 0) of the count tiles of the first product
-// (computed transposed, C^T[z][x], so the lane holding z = 4 q .. 4 q + 3
-// of a tile holds the operand bytes of k = 16 q .. 16 q + 15 once the four
-// z tiles of a step are packed to bytes in registers) and its A operand the
-// chunk bytes of the step's 64 z factors, laid out per lane in advance.  E1
-// accumulates in the MFMA over the whole z sweep of a task (-128 times its
-// exact value); k1(x) multiplies once per row at the end.  Row b = 10 of
+// flag bytes (-1 where c > 0) of the count tiles of the first product
+// (computed transposed and negated, -C^T[z][x], so the lane holding z =
+// 4 q .. 4 q + 3 of a tile holds the operand bytes of k = 16 q .. 16 q + 15
+// once the sign bits of the four z tiles of a step are spread to bytes in
+// registers) and its A operand the chunk bytes of the step's 64 z factors,
+// laid out per lane in advance.  E1 accumulates in the MFMA over the whole
+// z sweep of a task (minus its value); k1(x) multiplies once per row at the
+// end.  Row b = 10 of
 // the chunk operand is all ones, so E1[10][x] counts the row's groups.  Per
-// 4096 cells a wavefront issues 16 product and 4 reduction MFMAs and about
-// 90 VALU instructions; no count leaves the registers.
+// 4096 cells a wavefront issues 16 product and 4 reduction MFMAs and 3 VALU
+// instructions per four counts; no count leaves the registers.
 #include "dim3_internal.h"
 
 namespace {
@@ -161,7 +162,8 @@ __global__ void k_gc_list(const uint32_t* __restrict__ flag, const uint32_t* __r
 }
 
 // 64 bits -> 64 bytes of 0 / 1 (16-byte stores)
-__device__ __forceinline__ void bits_to_bytes(uint64_t m, int8_t* dst) {
+// bit i of m -> byte i of dst: `one` (1, or 0xFF for the int8 -1) where set
+__device__ __forceinline__ void bits_to_bytes(uint64_t m, int8_t* dst, uint32_t one = 1u) {
 #pragma unroll
     for (uint32_t q = 0; q < 4; q++) {
         uint32_t w[4];
@@ -169,7 +171,7 @@ __device__ __forceinline__ void bits_to_bytes(uint64_t m, int8_t* dst) {
         for (uint32_t d = 0; d < 4; d++) {
             const uint32_t nib = (uint32_t)(m >> (16u * q + 4u * d)) & 15u;
             // bit i of the nibble -> byte i
-            w[d] = ((nib * 0x00204081u) & 0x01010101u);
+            w[d] = ((nib * 0x00204081u) & 0x01010101u) * one;
         }
         *reinterpret_cast<uint4*>(dst + 16u * q) = make_uint4(w[0], w[1], w[2], w[3]);
     }
@@ -196,7 +198,8 @@ __global__ void k_gc_build_rows(const uint32_t* __restrict__ rlist, uint32_t Mh,
 }
 
 // Box columns n < Np (Np: Nh rounded up to 64): B8[n] = bytes of
-// zm(list[n]), gz[n] = g(z value); padding columns zero.
+// zm(list[n]) as int8 -1 (so the product gives -c_d, whose sign bit flags
+// c_d > 0), gz[n] = g(z value); padding columns zero.
 __global__ void k_gc_build_cols(const uint32_t* __restrict__ clist, uint32_t Nh, uint32_t Np,
                                 const uint64_t* __restrict__ zm, const int32_t* __restrict__ id2z, int32_t zlo,
                                 int8_t* __restrict__ B8, uint64_t* __restrict__ gz) {
@@ -208,7 +211,7 @@ __global__ void k_gc_build_cols(const uint32_t* __restrict__ clist, uint32_t Nh,
         m = zm[z];
         g = zfactor(id2z != nullptr ? id2z[z] : (int32_t)((uint32_t)zlo + z));
     }
-    bits_to_bytes(m, B8 + (size_t)n * kGcK);
+    bits_to_bytes(m, B8 + (size_t)n * kGcK, 0xFFu);
     gz[n] = g;
 }
 
@@ -236,9 +239,10 @@ __global__ void k_gc_build_gchunks(const uint64_t* __restrict__ gz, uint32_t nst
 // bytes held in 16 VGPRs) times box columns [cols * zc, cols * zc + cols),
 // 64 columns per step; the next step's column operands are loaded while the
 // current step computes.  Per step and 16-row group: four product MFMAs give
-// the transposed count tiles C^T[z][x] (z = 16 u + 4 q + e in lane (x, q)),
-// packed to bytes (count <= 64) and flagged (count > 0, as the int8 -128)
-// in registers, then a reduction MFMA adds E1 += G * flags (the count terms
+// the negated transposed count tiles -C^T[z][x] (z = 16 u + 4 q + e in
+// lane (x, q); B8 holds -1 bytes), whose sign bits become flag bytes (-1
+// where count > 0) in registers, then a reduction MFMA adds E1 += G * flags
+// (the count terms
 // sum2 and csum are sums over the join paths, formed from the key sums
 // instead).  Tasks are
 // numbered column-chunk major and dealt to the 8 XCDs in contiguous runs (a
@@ -304,15 +308,12 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
             v4i fl;
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
-                // counts <= 64: one byte each (v_perm_b32: selector bytes 0-3
-                // pick the second operand's bytes, 4-7 the first's, 0x0C zero)
-                const uint32_t lo = __builtin_amdgcn_perm((uint32_t)c[u][1], (uint32_t)c[u][0], 0x0C0C0400u);
-                const uint32_t hi = __builtin_amdgcn_perm((uint32_t)c[u][3], (uint32_t)c[u][2], 0x0C0C0400u);
-                const uint32_t p = __builtin_amdgcn_perm(hi, lo, 0x05040100u);
-                // byte 0x80 (int8 -128) where the count is > 0 (no carry
-                // between bytes: count + 127 <= 191); E1 is -128 times the
-                // flag sums
-                fl[u] = (int)((p + 0x7F7F7F7Fu) & 0x80808080u);
+                // c = -c_d: flag byte 0xFF (int8 -1) where c_d > 0, from the
+                // sign bits (v_perm_b32 selector 9 replicates bit 31 of the
+                // second operand, 11 of the first, 0x0C gives zero)
+                const uint32_t lo = __builtin_amdgcn_perm((uint32_t)c[u][1], (uint32_t)c[u][0], 0x0C0C0B09u);
+                const uint32_t hi = __builtin_amdgcn_perm((uint32_t)c[u][3], (uint32_t)c[u][2], 0x0B090C0Cu);
+                fl[u] = (int)(lo | hi);
             }
             e1[gx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gf, fl, e1[gx], 0, 0, 0);
         }
@@ -339,8 +340,8 @@ __global__ __launch_bounds__(256, 2) void k_gc_dense(const int8_t* __restrict__ 
 #pragma unroll
         for (uint32_t e = 0; e < 4; e++) {
             const uint32_t b = 4u * q + e;
-            // e1 = -128 x (a sum below 2^22): exact division
-            const uint64_t v1 = (uint64_t)(uint32_t)((-e1[gx][e]) >> 7);
+            // e1 = minus a sum below 2^21
+            const uint64_t v1 = (uint64_t)(uint32_t)(-e1[gx][e]);
             if (b < 10u) S1 += k1 * (v1 << (7u * b));
             else if (b == 10u) CNT += v1;
         }
